@@ -1,0 +1,150 @@
+#!/usr/bin/env python3
+"""UTR throughput benchmark on one MI355X.
+
+Workload (the reference metric's configuration): 256x256 images at 0.6 A,
+360 orientation rows about y (1 degree steps), 200 nm defocus, 256^3 volume,
+300 keV, UTR mode 0 with Ewald sphere correction, Poisson noise at 36
+counts/pixel (NSR 1/6) on seeded synthetic projections of 3000 atoms
+(C:N:O:S = 62:17:19:2) in an 8 nm sphere.
+
+One step = one full reconstruction: the 2D stage of all 360 rows, gridding,
+normalisation and noise filter, Hermitian symmetrisation, 3D inverse FFT,
+mask and scaling; input images resident in HBM when the timed region starts,
+the volume left in HBM.  metric = rows * nx * ny * nz / seconds.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the reconstruction would shard over orientation rows; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import utr_public_amd as U
+    from utr_public_amd import synth
+
+    torch.cuda.set_device(0)
+    p, rows, at = synth.config("metric")
+    A = len(rows["angY"])
+    t0 = time.time()
+    imgs = synth.images_for(p, rows, at, seed=1)
+    log("synthetic inputs: %d x %d x %d in %.1f s" % (imgs.shape + (time.time() - t0,)))
+    d_img = torch.from_numpy(imgs).cuda()
+    rec = U.Reconstructor(p, rows, U.UTR_FP32)
+    nz, ny, nx = rec.shape
+    d_vol = torch.empty((nz, ny, nx), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+
+    def step():
+        rec.reset()
+        rec.add_images_device(d_img.data_ptr(), A)
+        rec.finalize_device(d_vol.data_ptr())
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    grid_ms, fft3_ms, fft2_ms = [], [], []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        t = rec.timings()
+        grid_ms.append(t["backprop_ms"])
+        fft3_ms.append(t["fft3d_ms"])
+        fft2_ms.append(t["fft2d_ctf_ms"])
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    ms_step = elapsed / args.steps * 1e3
+    work = float(A) * nx * ny * nz
+    value = work / (elapsed / args.steps)
+    vol32 = d_vol.cpu().numpy()
+
+    # dominant kernel: the gather gridding (k_gather).  Algorithmic bytes per
+    # launch: every spectrum point read once (8 B) + V written once (8 B).
+    g_ms = float(np.mean(grid_ms))
+    g_bytes = 8.0 * A * nx * ny + 8.0 * nx * ny * nz
+    achieved = g_bytes / (g_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_gather", "kernel_ms": round(g_ms, 4)}
+
+    extra = {"phase_ms": {"fft2d_ctf": round(float(np.mean(fft2_ms)), 4),
+                          "gridding_filters": round(g_ms, 4),
+                          "fft3d_mask_scale": round(float(np.mean(fft3_ms)), 4)}}
+
+    if not args.no_fp64:
+        rec64 = U.Reconstructor(p, rows, U.UTR_FP64)
+        vol64 = torch.empty((nz, ny, nx), dtype=torch.float32, device="cuda")
+        reps = 2
+        torch.cuda.synchronize()
+        rec64.add_images_device(d_img.data_ptr(), A)
+        rec64.finalize_device(vol64.data_ptr())
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        for _ in range(reps):
+            rec64.reset()
+            rec64.add_images_device(d_img.data_ptr(), A)
+            rec64.finalize_device(vol64.data_ptr())
+        torch.cuda.synchronize()
+        t64 = (time.perf_counter() - ts) / reps
+        v64 = vol64.cpu().numpy().astype(np.float64)
+        relL2 = float(np.linalg.norm(vol32 - v64) / np.linalg.norm(v64))
+        extra["fp64_gpu"] = {"value": work / t64, "ms_per_step": round(t64 * 1e3, 3),
+                             "speedup_fp32_over_fp64": round(t64 * 1e3 / ms_step, 2),
+                             "rel_l2_fp32_vs_fp64": relL2,
+                             "max_abs_fp32_vs_fp64": float(np.abs(vol32 - v64).max())}
+        rec64.close()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle
+        ncores = min(16, os.cpu_count() or 1)
+        ts = time.perf_counter()
+        oracle.reconstruct(p, rows, imgs, nthreads=ncores)
+        tc = time.perf_counter() - ts
+        cpu = {"value": work / tc, "unit": "angle*voxels/s", "cores": ncores, "kind": "port",
+               "sample": "full workload (360 rows, 256^3), FP64 CPU restatement, OpenMP over rows",
+               "seconds": round(tc, 2)}
+
+    out = {
+        "metric": "angle*voxels/s, UTR 256^3 x 360 angles, FP32 (rel-L2 vs FP64 reported)",
+        "value": value, "unit": "angle*voxels/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded projections of 3000 random atoms, Poisson noise)",
+        "config": {"workload": "UTR 256^3 x 360 rows about y, 256^2 px at 0.6 A, 300 keV, "
+                               "mode 0 + ESCC, NSR 1/6",
+                   "rows": A, "image": [ny, nx], "volume": [nz, ny, nx]},
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    out.update(extra)
+    rec.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,169 @@
+/*
+ * utr_hip.h -- C ABI of the MI355X (gfx950) UTR reconstruction library
+ * (libutr_hip.so).  Plain pointers and sizes only; no exceptions cross it.
+ *
+ * The reference (timg021/UTR_public) has no library API for this path: the
+ * seam it replaces is the FFTW plan wrappers of the absent XArrayLibrary and
+ * the OpenMP loop over orientations inside UTR.cpp:main.  Each entry point
+ * names the reference code it stands in for.
+ *
+ * Life cycle:  utr_create -> utr_add_images (any number of calls, any row
+ * order) -> utr_finalize -> utr_destroy.  Every function returns 0 on success
+ * and a negative UTR_E_* code on failure; utr_last_error() then holds the
+ * message (thread-local).  A context may be used by one host thread at a
+ * time.
+ */
+#ifndef UTR_HIP_H
+#define UTR_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UTR_ABI_VERSION 1
+
+/* error codes */
+#define UTR_OK 0
+#define UTR_E_INVALID (-1)   /* invalid parameter (the reference throws)   */
+#define UTR_E_DEVICE (-2)    /* HIP runtime / device failure               */
+#define UTR_E_NOMEM (-3)     /* device or host allocation failed           */
+#define UTR_E_STATE (-4)     /* call out of order (e.g. rows missing)      */
+#define UTR_E_UNSUPPORTED (-5)
+
+/* precision forms */
+#define UTR_FP32 0  /* batched FP32 path: gather gridding with compensated sums */
+#define UTR_FP64 1  /* plain FP64 path: per-row launches, scatter gridding      */
+
+/*
+ * Numeric parameters of the UTR parameter file (ReadmeUTR.txt:33-165, read at
+ * UTR.cpp:77-782), in the file's units.  Field order matches the parameter
+ * numbers.  For .xri files the caller sets modality = 1, zlo/zhi to the image
+ * x-extent and asigma = -1/(delta/beta) (UTR.cpp:308-310,446-451).
+ */
+typedef struct utr_params {
+    int nx, ny;            /* 8: image width/height in pixels (even)        */
+    int modality;          /* 1: 0 = TEM/Angstrom, 1 = X-ray/micron         */
+    double xst;            /* 9: pixel size (UL)                            */
+    double zlo, zhi;       /* 10: output defocus range (UL)                 */
+    double dzextra;        /* 10: defocus offset (UL)                       */
+    double dxc, dyc, dzc;  /* 11: centre-of-rotation shifts (UL)            */
+    double wl;             /* 12: wavelength (UL)                           */
+    double aobj_mrad;      /* 13: objective aperture half-angle (mrad)      */
+    double cs3_mm, cs5_mm; /* 14: spherical aberrations (mm)                */
+    int ctf_mode;          /* 15: 0 none, 1 TIE-Hom-2D, 2 TIE-Hom-3D, 3 CTF-2D, 4 CTF-3D */
+    int escc;              /* 16: Ewald sphere curvature correction 0/1     */
+    double asigma;         /* 17: beta/delta                                */
+    double nsr;            /* 18: average noise-to-signal ratio             */
+    double epsilon;        /* 19: Tikhonov regularisation                   */
+    int inv_laplace;       /* 22: inverse 3D Laplacian 0/1                  */
+    double norm_f1, norm_f2; /* 7: input -> input/f1 + f2                   */
+} utr_params;
+
+typedef struct utr_ctx utr_ctx;
+
+/* Phase timers in milliseconds, named as UTR.cpp:1554-1559 prints them. */
+typedef struct utr_timings {
+    double fft2d_ctf_ms;   /* "2D FFT and CTF correction"                   */
+    double backprop_ms;    /* "backpropagation" (gridding + 3D filters)     */
+    double fft3d_ms;       /* "3D FFT" (+ mask and scaling)                 */
+    double total_ms;       /* add_images + finalize, device time            */
+} utr_timings;
+
+/*
+ * Creates a reconstruction context on the current HIP device.
+ * Replaces the set-up at UTR.cpp:840-867 (Fftwd2fc plan, V3/Samp3
+ * allocation) together with the validation of UTR.cpp:55-784.
+ * rows: nrows orientation rows (UTR.cpp:122 ReadDefocusParamsFile output):
+ * angZ_deg, angY_deg (columns 1-2), angZ2_deg (column 3, may be NULL; non-zero
+ * values are UTR_E_UNSUPPORTED), defocus (column 4, UL), weights (subseries
+ * weights, UTR.cpp:1019; may be NULL).  precision: UTR_FP32 or UTR_FP64.
+ */
+int utr_create(const utr_params* params, int nrows, const double* angZ_deg,
+               const double* angY_deg, const double* angZ2_deg, const double* defocus,
+               const float* weights, int precision, utr_ctx** out);
+
+/*
+ * Supplies the flat-field corrected images of rows [first_row,
+ * first_row+count) as count*ny*nx floats (x fastest) in host memory.
+ * Replaces the per-row body of the OpenMP loop UTR.cpp:877-1040 up to and
+ * including the Fourier transform and 2D filter (Log0/Shuffle/ForwardFFT,
+ * UTR.cpp:951-1019).  Images must already be padded/trimmed to ny x nx.
+ */
+int utr_add_images(utr_ctx* ctx, int first_row, int count, const float* images);
+
+/* Same, with images already resident in device memory (HBM). */
+int utr_add_images_device(utr_ctx* ctx, int first_row, int count, const float* d_images);
+
+/*
+ * Runs the gridding onto the Ewald sphere (UTR.h:188-307, called at
+ * UTR.cpp:1027), the sampling normalisation and noise filter
+ * (UTR.cpp:1050-1172), the 3D CTF / TIE-Hom / inverse-Laplacian filters
+ * (UTR.cpp:1177-1201), the Hermitian symmetrisation (UTR.cpp:1203-1217), the
+ * inverse 3D FFT (Fftwd3frc, UTR.cpp:1219-1233), the sphere mask and final
+ * scaling (UTR.cpp:1235-1279).  Writes nz*ny*nx floats (x fastest, centred as
+ * K3out) to host memory.  All rows must have been added.  May be called again
+ * after further utr_reset/utr_add_images cycles.
+ */
+int utr_finalize(utr_ctx* ctx, float* vol);
+
+/* Same, writing to device memory. */
+int utr_finalize_device(utr_ctx* ctx, float* d_vol);
+
+/* Same, writing double precision to host memory (UTR_FP64 contexts only);
+ * for precision studies against an FP64 CPU computation. */
+int utr_finalize_f64(utr_ctx* ctx, double* vol);
+
+/* Keep the sampling matrix Samp3 of the next utr_finalize in device memory
+ * (parameter 32, UTR.cpp:706-712; off by default). */
+int utr_keep_sampling(utr_ctx* ctx, int on);
+
+/* Copies the sampling matrix Samp3 (nz*ny*nx, the weights the gridding
+ * accumulated) to host memory; valid after utr_finalize (FP32 contexts need
+ * utr_keep_sampling first).  Replaces the sampling-matrix output of
+ * UTR.cpp:1043-1048 (parameter 32). */
+int utr_get_sampling(utr_ctx* ctx, float* samp);
+
+/* Discards the images added so far so the context can be reused for another
+ * set of images with the same geometry and rows. */
+int utr_reset(utr_ctx* ctx);
+
+/* Volume dimensions derived from the parameters (nz from UTR.cpp:321). */
+int utr_dims(const utr_ctx* ctx, int* nx, int* ny, int* nz);
+
+/* Device-time phase timers of the last finalize. */
+int utr_get_timings(const utr_ctx* ctx, utr_timings* t);
+
+int utr_destroy(utr_ctx* ctx);
+
+/* Message of the last failure on this thread ("" if none). */
+const char* utr_last_error(void);
+
+/* Library / ABI version. */
+int utr_abi_version(void);
+
+/*
+ * Diagnostics (no counterpart in the reference): copy intermediate results
+ * to host memory so a CPU restatement can be compared stage by stage.
+ * utr_get_spectrum: the centred 2D spectrum K2four of one row (UTR.cpp:995/
+ * 1017-1019), ny*nx interleaved complex, FP32 contexts after the row was
+ * added.  utr_get_fourier_volume: V3 after normalisation, noise filter and 3D
+ * filters (UTR.cpp:1201), nz*ny*nx interleaved complex as float; valid after
+ * utr_finalize.
+ */
+int utr_get_spectrum(utr_ctx* ctx, int row, float* spec);
+int utr_get_fourier_volume(utr_ctx* ctx, float* vol_c);
+
+/*
+ * Grid-unit coefficients of the gridding coordinate map for one orientation
+ * (3x3 row-major, see UTR.h:243-256).  Exposed so a CPU restatement can check
+ * that both sides evaluate identical coordinates.
+ */
+int utr_angle_coeffs(const utr_params* params, double angZ_deg, double angY_deg, double* m9);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* UTR_HIP_H */

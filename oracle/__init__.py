@@ -1,0 +1,215 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes bindings of the FP64 CPU oracle.
+
+The oracle (``utr_oracle.c``) restates the UTR reconstruction path of
+MultisliceCppTest/PhaseRetrieval/UTR.cpp:840-1279 and UTR.h:15-307 in double
+precision.  Only ``tests/``, ``__graft_entry__.smoke()`` and the cpu_baseline
+leg of ``bench.py`` import this module; the product package never does.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle_utr.so")
+
+PARAM_FIELDS = [
+    ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("modality", ctypes.c_int),
+    ("xst", ctypes.c_double), ("zlo", ctypes.c_double), ("zhi", ctypes.c_double),
+    ("dzextra", ctypes.c_double), ("dxc", ctypes.c_double), ("dyc", ctypes.c_double),
+    ("dzc", ctypes.c_double), ("wl", ctypes.c_double), ("aobj_mrad", ctypes.c_double),
+    ("cs3_mm", ctypes.c_double), ("cs5_mm", ctypes.c_double), ("ctf_mode", ctypes.c_int),
+    ("escc", ctypes.c_int), ("asigma", ctypes.c_double), ("nsr", ctypes.c_double),
+    ("epsilon", ctypes.c_double), ("inv_laplace", ctypes.c_int),
+    ("norm_f1", ctypes.c_double), ("norm_f2", ctypes.c_double),
+]
+
+
+class OraParams(ctypes.Structure):
+    _fields_ = PARAM_FIELDS
+
+
+class OraGeom(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("nxd2", ctypes.c_int), ("nyd2", ctypes.c_int), ("nzd2", ctypes.c_int),
+        ("modality", ctypes.c_int), ("ctf_mode", ctypes.c_int), ("escc", ctypes.c_int),
+        ("inv_laplace", ctypes.c_int),
+        ("xst", ctypes.c_double), ("yst", ctypes.c_double), ("zst", ctypes.c_double),
+        ("xlo", ctypes.c_double), ("xhi", ctypes.c_double), ("ylo", ctypes.c_double),
+        ("yhi", ctypes.c_double), ("zlo", ctypes.c_double), ("zhi", ctypes.c_double),
+        ("fxst", ctypes.c_double), ("fyst", ctypes.c_double), ("fzst", ctypes.c_double),
+        ("fxlo", ctypes.c_double), ("fylo", ctypes.c_double), ("fzlo", ctypes.c_double),
+        ("dxc", ctypes.c_double), ("dyc", ctypes.c_double), ("dzc", ctypes.c_double),
+        ("rot_centre_shift", ctypes.c_int),
+        ("xc", ctypes.c_double), ("yc", ctypes.c_double), ("zc", ctypes.c_double),
+        ("ic", ctypes.c_long), ("jc", ctypes.c_long), ("kc", ctypes.c_long),
+        ("wl", ctypes.c_double), ("EE", ctypes.c_double), ("k2maxo", ctypes.c_double),
+        ("cs3", ctypes.c_double), ("cs5", ctypes.c_double),
+        ("asigma", ctypes.c_double), ("nsr", ctypes.c_double), ("epsilon", ctypes.c_double),
+        ("norm_f1", ctypes.c_double), ("norm_f2", ctypes.c_double),
+        ("zout_aver", ctypes.c_double), ("dzextra", ctypes.c_double),
+        ("rot_y_only", ctypes.c_int), ("rot_z_only", ctypes.c_int),
+        ("rot_z2_only", ctypes.c_int),
+        ("nR2", ctypes.c_long), ("mask_dtemp", ctypes.c_double), ("fnorm", ctypes.c_double),
+    ]
+
+
+def build():
+    """Compile liboracle_utr.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        fp = ctypes.POINTER(ctypes.c_float)
+        L.ora_last_error.restype = ctypes.c_char_p
+        L.ora_derive.argtypes = [ctypes.POINTER(OraParams), ctypes.c_int, dp, dp, dp, dp,
+                                 ctypes.POINTER(OraGeom)]
+        L.ora_fft.argtypes = [dp, ctypes.c_int, ctypes.c_int]
+        L.ora_angle_coeffs.argtypes = [ctypes.POINTER(OraGeom), ctypes.c_double,
+                                       ctypes.c_double, dp]
+        L.ora_spectrum.argtypes = [ctypes.POINTER(OraGeom), fp, ctypes.c_double,
+                                   ctypes.c_double, dp]
+        L.ora_grid.argtypes = [ctypes.POINTER(OraGeom), dp, ctypes.c_double, ctypes.c_double,
+                               dp, dp]
+        L.ora_normalize_filter.argtypes = [ctypes.POINTER(OraGeom), dp, dp]
+        L.ora_hermitian.argtypes = [ctypes.POINTER(OraGeom), dp]
+        L.ora_c2r.argtypes = [ctypes.POINTER(OraGeom), dp, dp]
+        L.ora_mask_scale.argtypes = [ctypes.POINTER(OraGeom), dp]
+        L.ora_reconstruct.argtypes = [ctypes.POINTER(OraParams), ctypes.c_int, dp, dp, dp, dp,
+                                      fp, fp, dp, dp, ctypes.c_int]
+        L.ora_last_timings.argtypes = [dp]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _fp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+def make_params(d):
+    p = OraParams()
+    for name, _ in PARAM_FIELDS:
+        setattr(p, name, d[name])
+    return p
+
+
+def _rows(rows):
+    angZ = np.ascontiguousarray(rows["angZ"], dtype=np.float64)
+    angY = np.ascontiguousarray(rows["angY"], dtype=np.float64)
+    angZ2 = np.ascontiguousarray(rows.get("angZ2", np.zeros_like(angZ)), dtype=np.float64)
+    defoc = np.ascontiguousarray(rows["defocus"], dtype=np.float64)
+    return angZ, angY, angZ2, defoc
+
+
+def derive(params, rows):
+    angZ, angY, angZ2, defoc = _rows(rows)
+    g = OraGeom()
+    p = make_params(params)
+    if lib().ora_derive(ctypes.byref(p), len(angZ), _dp(angZ), _dp(angY), _dp(angZ2),
+                        _dp(defoc), ctypes.byref(g)):
+        raise OracleError(lib().ora_last_error().decode())
+    return g
+
+
+def fft(x, sign=-1):
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.complex128)).copy()
+    if lib().ora_fft(_dp(a.view(np.float64)), a.size, sign):
+        raise OracleError(lib().ora_last_error().decode())
+    return a
+
+
+def angle_coeffs(g, angZ_deg, angY_deg):
+    m = np.zeros(9)
+    lib().ora_angle_coeffs(ctypes.byref(g), angZ_deg, angY_deg, _dp(m))
+    return m.reshape(3, 3)
+
+
+def spectrum(g, img, zout=0.0, weight=1.0):
+    img = np.ascontiguousarray(img, dtype=np.float32)
+    K = np.zeros((g.ny, g.nx), dtype=np.complex128)
+    lib().ora_spectrum(ctypes.byref(g), _fp(img), zout, weight, _dp(K.view(np.float64)))
+    return K
+
+
+def grid(g, K, angZ_deg, angY_deg, V=None, S=None):
+    K = np.ascontiguousarray(K, dtype=np.complex128)
+    if V is None:
+        V = np.zeros((g.nz, g.ny, g.nx), dtype=np.complex128)
+    if S is None:
+        S = np.zeros((g.nz, g.ny, g.nx), dtype=np.float64)
+    lib().ora_grid(ctypes.byref(g), _dp(K.view(np.float64)), angZ_deg, angY_deg,
+                   _dp(V.view(np.float64)), _dp(S))
+    return V, S
+
+
+def normalize_filter(g, V, S):
+    V = np.ascontiguousarray(V, dtype=np.complex128).copy()
+    S = np.ascontiguousarray(S, dtype=np.float64)
+    lib().ora_normalize_filter(ctypes.byref(g), _dp(V.view(np.float64)), _dp(S))
+    return V
+
+
+def hermitian(g, V):
+    V = np.ascontiguousarray(V, dtype=np.complex128).copy()
+    lib().ora_hermitian(ctypes.byref(g), _dp(V.view(np.float64)))
+    return V
+
+
+def c2r(g, V):
+    V = np.ascontiguousarray(V, dtype=np.complex128)
+    out = np.zeros((g.nz, g.ny, g.nx))
+    if lib().ora_c2r(ctypes.byref(g), _dp(V.view(np.float64)), _dp(out)):
+        raise OracleError(lib().ora_last_error().decode())
+    return out
+
+
+def mask_scale(g, vol):
+    vol = np.ascontiguousarray(vol, dtype=np.float64).copy()
+    lib().ora_mask_scale(ctypes.byref(g), _dp(vol))
+    return vol
+
+
+def reconstruct(params, rows, images, weights=None, nthreads=1, samp=False):
+    """Full FP64 reconstruction.  Returns the volume (nz, ny, nx) float64 and,
+    if samp is True, the sampling matrix."""
+    angZ, angY, angZ2, defoc = _rows(rows)
+    g = derive(params, rows)
+    images = np.ascontiguousarray(images, dtype=np.float32)
+    assert images.shape == (len(angZ), g.ny, g.nx), images.shape
+    vol = np.zeros((g.nz, g.ny, g.nx))
+    S = np.zeros((g.nz, g.ny, g.nx)) if samp else None
+    w = None
+    if weights is not None:
+        w = np.ascontiguousarray(weights, dtype=np.float32)
+    p = make_params(params)
+    rc = lib().ora_reconstruct(ctypes.byref(p), len(angZ), _dp(angZ), _dp(angY), _dp(angZ2),
+                               _dp(defoc), _fp(w) if w is not None else None, _fp(images),
+                               _dp(vol), _dp(S) if S is not None else None, nthreads)
+    if rc:
+        raise OracleError(lib().ora_last_error().decode())
+    return (vol, S) if samp else vol
+
+
+def last_timings():
+    t = np.zeros(4)
+    lib().ora_last_timings(_dp(t))
+    return {"fft2d_ctf_ms": t[0], "backprop_ms": t[1], "fft3d_ms": t[2], "total_ms": t[3]}

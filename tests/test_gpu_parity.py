@@ -1,0 +1,182 @@
+"""GPU parity: the HIP library (through its C ABI) against the FP64 CPU
+oracle on the same seeded inputs.
+
+Tolerances: the FP32 form must reach relative L2 <= 1e-5 against the FP64
+oracle (the bar the reference's metric names); the plain FP64 form must reach
+1e-10.  Results of the FP32 form are deterministic (no atomics), so two runs
+must agree bit for bit.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import utr_public_amd as U
+from utr_public_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+FP32_TOL = 1e-5
+FP64_TOL = 1e-10
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+
+
+def case(n=64, nrows=24, seed=1, escc=1, nsr=0.05, natoms=80, general=False, **kw):
+    xst = 0.4
+    p = U.default_params(nx=n, ny=n, xst=xst, zlo=-n * xst / 2, zhi=n * xst / 2, escc=escc,
+                         nsr=nsr, **kw)
+    rng = np.random.default_rng(seed)
+    if general:
+        angZ = rng.uniform(0, 360, nrows)
+        angY = np.degrees(np.arccos(2 * rng.uniform(0, 1, nrows) - 1))
+    else:
+        angZ = np.zeros(nrows)
+        angY = np.arange(nrows) * (180.0 / nrows)
+    rows = dict(angZ=angZ, angY=angY, defocus=np.full(nrows, 300.0))
+    xyz, Z = synth.atoms(seed, natoms, "cube", n * xst * 0.4, 1.4, ([6], [1.0]))
+    dose = (1.0 / nsr) ** 2 if nsr > 0 else 0.0
+    imgs = synth.project(xyz, Z, n, n, xst, angZ, angY, dose=dose, seed=seed)
+    return p, rows, imgs
+
+
+def run_both(p, rows, imgs):
+    print("case: %s rows=%d" % ({k: p[k] for k in ("nx", "ny", "ctf_mode", "escc", "nsr")},
+                                len(rows["angY"])), flush=True)
+    ref = oracle.reconstruct(p, rows, imgs)
+    v32 = U.reconstruct(p, rows, imgs, U.UTR_FP32)
+    v64 = U.reconstruct(p, rows, imgs, U.UTR_FP64, f64_out=True)
+    return ref, v32, v64
+
+
+def test_spectrum_matches_oracle():
+    p, rows, imgs = case(n=64, nrows=4)
+    g = oracle.derive(p, rows)
+    with U.Reconstructor(p, rows, U.UTR_FP32) as r:
+        r.add_images(imgs)
+        for row in range(4):
+            K = r.spectrum(row)
+            Ko = oracle.spectrum(g, imgs[row], zout=rows["defocus"][row])
+            assert rel(K, Ko) < 2e-6, row
+
+
+@pytest.mark.parametrize("escc", [0, 1])
+@pytest.mark.parametrize("nsr", [0.0, 0.05])
+def test_reconstruct_y_rotation(escc, nsr):
+    ref, v32, v64 = run_both(*case(n=64, nrows=24, escc=escc, nsr=nsr))
+    assert rel(v64, ref) < FP64_TOL
+    assert rel(v32, ref) < FP32_TOL
+
+
+def test_reconstruct_general_rotation():
+    ref, v32, v64 = run_both(*case(n=48, nrows=40, general=True, escc=1, nsr=0.05, seed=4))
+    assert rel(v64, ref) < FP64_TOL
+    assert rel(v32, ref) < FP32_TOL
+
+
+@pytest.mark.parametrize("mode,extra", [
+    (4, dict(asigma=0.07, epsilon=0.01, cs3_mm=1.0)),
+    (4, dict(asigma=0.07, epsilon=0.0, aobj_mrad=30.0)),
+    (2, dict(asigma=-0.05, modality=1)),
+    (0, dict(inv_laplace=1, epsilon=0.5)),
+    (1, dict(asigma=-0.05, modality=1)),
+    (3, dict(asigma=0.07, epsilon=0.01)),
+])
+def test_reconstruct_filter_modes(mode, extra):
+    ref, v32, v64 = run_both(*case(n=48, nrows=24, ctf_mode=mode, seed=5, **extra))
+    assert rel(v64, ref) < FP64_TOL
+    assert rel(v32, ref) < FP32_TOL
+
+
+def test_reconstruct_centre_shift_and_weights():
+    p, rows, imgs = case(n=48, nrows=20, seed=6, dxc=0.8, dyc=-0.4, dzc=0.6)
+    rows["weights"] = np.linspace(0.5, 1.5, 20).astype(np.float32)
+    ref = oracle.reconstruct(p, rows, imgs, weights=rows["weights"])
+    v32 = U.reconstruct(p, rows, imgs, U.UTR_FP32)
+    v64 = U.reconstruct(p, rows, imgs, U.UTR_FP64, f64_out=True)
+    assert rel(v64, ref) < FP64_TOL
+    assert rel(v32, ref) < FP32_TOL
+
+
+def test_reconstruct_rectangular_and_sizes():
+    # nx != ny != nz, radix-3 lengths (96 = 2^5 * 3)
+    xst = 0.5
+    p = U.default_params(nx=96, ny=64, xst=xst, zlo=-8.0, zhi=8.0, escc=1, nsr=0.03)
+    rows = dict(angZ=np.zeros(18), angY=np.arange(18) * 10.0, defocus=np.full(18, 200.0))
+    xyz, Z = synth.atoms(8, 60, "cube", 12.0, 1.4, ([6], [1.0]))
+    imgs = synth.project(xyz, Z, 96, 64, xst, rows["angZ"], rows["angY"], dose=1000.0, seed=8)
+    ref, v32, v64 = run_both(p, rows, imgs)
+    assert ref.shape == (32, 64, 96)
+    assert rel(v64, ref) < FP64_TOL
+    assert rel(v32, ref) < FP32_TOL
+
+
+def test_c1_config_128():
+    p, rows, at = synth.config("c1")
+    imgs = synth.images_for(p, rows, at, seed=1)
+    ref, v32, v64 = run_both(p, rows, imgs)
+    assert rel(v64, ref) < FP64_TOL
+    assert rel(v32, ref) < FP32_TOL
+    assert np.abs(v32 - ref).max() < 1e-4 * np.abs(ref).max()
+
+
+def test_sampling_matrix_matches_oracle():
+    p, rows, imgs = case(n=48, nrows=16, seed=9)
+    _, S = oracle.reconstruct(p, rows, imgs, samp=True)
+    with U.Reconstructor(p, rows, U.UTR_FP32) as r:
+        r.keep_sampling()
+        r.add_images(imgs)
+        r.finalize()
+        S32 = r.sampling()
+    assert rel(S32, S) < 1e-6
+    assert np.array_equal(S32 == 0, S == 0)
+
+
+def test_fp32_deterministic_and_reset_reuse():
+    p, rows, imgs = case(n=48, nrows=16, seed=10)
+    with U.Reconstructor(p, rows, U.UTR_FP32) as r:
+        r.add_images(imgs)
+        a = r.finalize()
+        r.reset()
+        # rows added out of order in two batches
+        r.add_images(imgs[8:], first_row=8)
+        r.add_images(imgs[:8], first_row=0)
+        b = r.finalize()
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_missing_rows_and_bad_ranges_are_errors():
+    p, rows, imgs = case(n=32, nrows=6, seed=11)
+    with U.Reconstructor(p, rows, U.UTR_FP32) as r:
+        r.add_images(imgs[:5])
+        with pytest.raises(U.UTRError, match="no image") as e:
+            r.finalize()
+        assert e.value.code == -4
+        with pytest.raises(U.UTRError, match="outside"):
+            r.add_images(imgs[:2], first_row=5)
+    with pytest.raises(U.UTRError, match="not supported"):
+        U.Reconstructor(U.default_params(nx=34, ny=34, zlo=-6.8, zhi=6.8), rows)
+    with pytest.raises(U.UTRError, match="Z''"):
+        U.Reconstructor(p, dict(rows, angZ2=np.ones(6)))
+
+
+def test_single_row():
+    p, rows, imgs = case(n=32, nrows=1, seed=12)
+    ref, v32, v64 = run_both(p, rows, imgs)
+    assert rel(v64, ref) < FP64_TOL
+    assert rel(v32, ref) < FP32_TOL
+
+
+def test_device_pointers_roundtrip():
+    import torch
+    p, rows, imgs = case(n=48, nrows=12, seed=13)
+    ref = oracle.reconstruct(p, rows, imgs)
+    d_img = torch.from_numpy(imgs).cuda()
+    d_out = torch.empty(ref.shape, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    with U.Reconstructor(p, rows, U.UTR_FP32) as r:
+        r.add_images_device(d_img.data_ptr(), 12)
+        r.finalize_device(d_out.data_ptr())
+    torch.cuda.synchronize()
+    assert rel(d_out.cpu().numpy(), ref) < FP32_TOL

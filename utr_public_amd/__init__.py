@@ -1,0 +1,235 @@
+"""utr_public_amd -- MI355X-native UTR (Unified Tomographic Reconstruction).
+
+Python bindings (ctypes) over the C ABI of ``_lib/libutr_hip.so``
+(``include/utr_hip.h``).  The library runs hand-written gfx950 HIP kernels;
+there is no CPU fallback: if the library is missing, ``load()`` raises.
+
+Reference: timg021/UTR_public, MultisliceCppTest/PhaseRetrieval/UTR.cpp.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "_lib", "libutr_hip.so")
+SYNTH_PATH = os.path.join(_PKG, "_lib", "libutr_synth.so")
+
+UTR_FP32 = 0
+UTR_FP64 = 1
+
+PARAM_FIELDS = [
+    ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("modality", ctypes.c_int),
+    ("xst", ctypes.c_double), ("zlo", ctypes.c_double), ("zhi", ctypes.c_double),
+    ("dzextra", ctypes.c_double), ("dxc", ctypes.c_double), ("dyc", ctypes.c_double),
+    ("dzc", ctypes.c_double), ("wl", ctypes.c_double), ("aobj_mrad", ctypes.c_double),
+    ("cs3_mm", ctypes.c_double), ("cs5_mm", ctypes.c_double), ("ctf_mode", ctypes.c_int),
+    ("escc", ctypes.c_int), ("asigma", ctypes.c_double), ("nsr", ctypes.c_double),
+    ("epsilon", ctypes.c_double), ("inv_laplace", ctypes.c_int),
+    ("norm_f1", ctypes.c_double), ("norm_f2", ctypes.c_double),
+]
+
+# Symbols include/utr_hip.h declares (checked by the CPU test suite).
+EXPORTS = [
+    "utr_create", "utr_add_images", "utr_add_images_device", "utr_finalize",
+    "utr_finalize_device", "utr_finalize_f64", "utr_keep_sampling", "utr_get_sampling",
+    "utr_reset", "utr_dims", "utr_get_timings", "utr_destroy", "utr_last_error",
+    "utr_abi_version", "utr_angle_coeffs", "utr_get_spectrum", "utr_get_fourier_volume",
+]
+
+
+class UtrParams(ctypes.Structure):
+    _fields_ = PARAM_FIELDS
+
+
+class UtrTimings(ctypes.Structure):
+    _fields_ = [("fft2d_ctf_ms", ctypes.c_double), ("backprop_ms", ctypes.c_double),
+                ("fft3d_ms", ctypes.c_double), ("total_ms", ctypes.c_double)]
+
+
+class UTRError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("UTR error %d: %s" % (code, msg))
+        self.code = code
+
+
+def default_params(**kw):
+    """Parameter set with the reference's neutral defaults (UTR.txt)."""
+    d = dict(nx=128, ny=128, modality=0, xst=0.4, zlo=-25.6, zhi=25.6, dzextra=0.0,
+             dxc=0.0, dyc=0.0, dzc=0.0, wl=0.02507934, aobj_mrad=0.0, cs3_mm=0.0,
+             cs5_mm=0.0, ctf_mode=0, escc=1, asigma=0.0, nsr=0.0, epsilon=1e-4,
+             inv_laplace=0, norm_f1=1.0, norm_f2=0.0)
+    d.update(kw)
+    return d
+
+
+def make_params(d):
+    p = UtrParams()
+    for name, _ in PARAM_FIELDS:
+        setattr(p, name, d[name])
+    return p
+
+
+_lib = None
+
+
+def load():
+    """Loads libutr_hip.so (built in-tree by __graft_entry__.build())."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libutr_hip.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.utr_create.argtypes = [ctypes.POINTER(UtrParams), ctypes.c_int, dp, dp, dp, dp, vp,
+                             ctypes.c_int, ctypes.POINTER(vp)]
+    L.utr_add_images.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
+    L.utr_add_images_device.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
+    L.utr_finalize.argtypes = [vp, vp]
+    L.utr_finalize_device.argtypes = [vp, vp]
+    L.utr_finalize_f64.argtypes = [vp, vp]
+    L.utr_keep_sampling.argtypes = [vp, ctypes.c_int]
+    L.utr_get_sampling.argtypes = [vp, vp]
+    L.utr_reset.argtypes = [vp]
+    L.utr_dims.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                           ctypes.POINTER(ctypes.c_int)]
+    L.utr_get_timings.argtypes = [vp, ctypes.POINTER(UtrTimings)]
+    L.utr_destroy.argtypes = [vp]
+    L.utr_last_error.restype = ctypes.c_char_p
+    L.utr_angle_coeffs.argtypes = [ctypes.POINTER(UtrParams), ctypes.c_double, ctypes.c_double, dp]
+    L.utr_get_spectrum.argtypes = [vp, ctypes.c_int, vp]
+    L.utr_get_fourier_volume.argtypes = [vp, vp]
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise UTRError(rc, load().utr_last_error().decode())
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def angle_coeffs(params, angZ_deg, angY_deg):
+    m = np.zeros(9)
+    p = make_params(params)
+    _check(load().utr_angle_coeffs(ctypes.byref(p), angZ_deg, angY_deg, _dp(m)))
+    return m.reshape(3, 3)
+
+
+class Reconstructor:
+    """One UTR reconstruction context on the current HIP device.
+
+    rows: dict with arrays angZ, angY (degrees), defocus (UL), optional angZ2
+    and weights -- the columns of the orientation file read at UTR.cpp:122.
+    """
+
+    def __init__(self, params, rows, precision=UTR_FP32):
+        L = load()
+        self._L = L
+        self.params = dict(params)
+        self.angZ = np.ascontiguousarray(rows["angZ"], dtype=np.float64)
+        self.angY = np.ascontiguousarray(rows["angY"], dtype=np.float64)
+        self.defocus = np.ascontiguousarray(rows["defocus"], dtype=np.float64)
+        self.angZ2 = None
+        if rows.get("angZ2") is not None:
+            self.angZ2 = np.ascontiguousarray(rows["angZ2"], dtype=np.float64)
+        self.weights = None
+        if rows.get("weights") is not None:
+            self.weights = np.ascontiguousarray(rows["weights"], dtype=np.float32)
+        self.nrows = len(self.angZ)
+        self.precision = precision
+        p = make_params(self.params)
+        ctx = ctypes.c_void_p()
+        _check(L.utr_create(ctypes.byref(p), self.nrows, _dp(self.angZ), _dp(self.angY),
+                            _dp(self.angZ2) if self.angZ2 is not None else None,
+                            _dp(self.defocus),
+                            self.weights.ctypes.data if self.weights is not None else None,
+                            precision, ctypes.byref(ctx)))
+        self._ctx = ctx
+        nx, ny, nz = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(L.utr_dims(ctx, ctypes.byref(nx), ctypes.byref(ny), ctypes.byref(nz)))
+        self.shape = (nz.value, ny.value, nx.value)
+
+    def add_images(self, images, first_row=0):
+        images = np.ascontiguousarray(images, dtype=np.float32)
+        n = images.shape[0]
+        assert images.shape[1:] == self.shape[1:], (images.shape, self.shape)
+        _check(self._L.utr_add_images(self._ctx, first_row, n, images.ctypes.data))
+
+    def add_images_device(self, ptr, count, first_row=0):
+        _check(self._L.utr_add_images_device(self._ctx, first_row, count, ctypes.c_void_p(ptr)))
+
+    def keep_sampling(self, on=True):
+        _check(self._L.utr_keep_sampling(self._ctx, 1 if on else 0))
+
+    def finalize(self, out=None):
+        if out is None:
+            out = np.empty(self.shape, dtype=np.float32)
+        assert out.dtype == np.float32 and out.shape == self.shape and out.flags.c_contiguous
+        _check(self._L.utr_finalize(self._ctx, out.ctypes.data))
+        return out
+
+    def finalize_f64(self):
+        out = np.empty(self.shape, dtype=np.float64)
+        _check(self._L.utr_finalize_f64(self._ctx, out.ctypes.data))
+        return out
+
+    def finalize_device(self, ptr):
+        _check(self._L.utr_finalize_device(self._ctx, ctypes.c_void_p(ptr)))
+
+    def sampling(self):
+        out = np.empty(self.shape, dtype=np.float32)
+        _check(self._L.utr_get_sampling(self._ctx, out.ctypes.data))
+        return out
+
+    def spectrum(self, row):
+        out = np.empty(self.shape[1:], dtype=np.complex64)
+        _check(self._L.utr_get_spectrum(self._ctx, row, out.ctypes.data))
+        return out
+
+    def fourier_volume(self):
+        out = np.empty(self.shape, dtype=np.complex64)
+        _check(self._L.utr_get_fourier_volume(self._ctx, out.ctypes.data))
+        return out
+
+    def reset(self):
+        _check(self._L.utr_reset(self._ctx))
+
+    def timings(self):
+        t = UtrTimings()
+        _check(self._L.utr_get_timings(self._ctx, ctypes.byref(t)))
+        return {k: getattr(t, k) for k, _ in t._fields_}
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            self._L.utr_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def reconstruct(params, rows, images, precision=UTR_FP32, f64_out=False):
+    """Convenience: one full reconstruction with host input and output."""
+    with Reconstructor(params, rows, precision) as r:
+        r.add_images(images)
+        if f64_out:
+            return r.finalize_f64()
+        return r.finalize()
+
+
+from . import synth  # noqa: E402,F401

@@ -1,0 +1,541 @@
+// utr_api.hip -- C ABI of libutr_hip.so: context, device buffers, launches.
+//
+// Replaces the FFTW plans and the OpenMP orientation loop of
+// MultisliceCppTest/PhaseRetrieval/UTR.cpp:840-1279 (see include/utr_hip.h).
+// Two forms share the FFT and epilogue kernels:
+//   UTR_FP32  rows are transformed in batches as they arrive; finalize runs
+//             the gather gridding over all rows (compensated FP32 sums,
+//             deterministic), the fused normalisation/filters and the 3D c2r.
+//   UTR_FP64  the plain form: one row at a time, scatter gridding with FP64
+//             atomics, then separate normalisation, symmetrisation and c2r
+//             passes (the structure of the reference's loop, in double).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/utr_hip.h"
+#include "utr_fft.hpp"
+#include "utr_kernels.hip"
+#include "utr_plan.hpp"
+
+using utr::cplx;
+
+namespace {
+
+thread_local std::string g_err;
+
+int set_err(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define UTR_HIP_TRY(expr)                                                                      \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return set_err(e_ == hipErrorOutOfMemory ? UTR_E_NOMEM : UTR_E_DEVICE,             \
+                           std::string(#expr) + ": " + hipGetErrorString(e_));                 \
+    } while (0)
+
+bool fft_size_supported(int n) {
+    switch (n) {
+        case 16: case 32: case 48: case 64: case 96: case 128: case 160: case 192:
+        case 256: case 320: case 384: case 512:
+            return true;
+        default:
+            return false;
+    }
+}
+
+// Dispatches F<N>() for the supported FFT lengths.
+#define UTR_DISPATCH_N(n, F)                  \
+    switch (n) {                              \
+        case 16: F(16); break;                \
+        case 32: F(32); break;                \
+        case 48: F(48); break;                \
+        case 64: F(64); break;                \
+        case 96: F(96); break;                \
+        case 128: F(128); break;              \
+        case 160: F(160); break;              \
+        case 192: F(192); break;              \
+        case 256: F(256); break;              \
+        case 320: F(320); break;              \
+        case 384: F(384); break;              \
+        case 512: F(512); break;              \
+        default: break;                       \
+    }
+
+template <typename T>
+std::vector<cplx<T>> make_twiddles(int n) {
+    const utr::FacList F = utr::factorize(n);
+    std::vector<cplx<T>> tw(F.ntw > 0 ? F.ntw : 1);
+    for (int s = 0; s < F.nf; s++) {
+        const int p = F.p[s], R = F.f[s];
+        for (int k = 0; k < p; k++)
+            for (int r = 1; r < R; r++) {
+                const double ang = -2.0 * utr::kPi * (double)r * k / ((double)p * R);
+                tw[F.off[s] + k * (R - 1) + (r - 1)] = {T(std::cos(ang)), T(std::sin(ang))};
+            }
+    }
+    return tw;
+}
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    hipError_t alloc(size_t count) {
+        if (p && n >= count) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        hipError_t e = hipMalloc(&p, sizeof(T) * (count ? count : 1));
+        if (e == hipSuccess) n = count;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+constexpr int kStageRows = 32;  // rows per host->device staging batch
+
+}  // namespace
+
+struct utr_ctx {
+    utr::Plan plan;
+    int precision = UTR_FP32;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[8] = {};
+    std::vector<char> added;
+    int n_added = 0;
+    int keep_sampling = 0;
+    bool finalized = false;
+    utr_timings tim = {};
+    float add_ms = 0;  // accumulated device time of add_images (2D stage + FP64 scatter)
+    // shared
+    DevBuf<utr::RowParams> rows;
+    DevBuf<float> img_stage;
+    // FP32 form
+    DevBuf<cplx<float>> spec32, V32, H32, twx32, twy32, twz32;
+    DevBuf<float> S32, vol32;
+    // FP64 form
+    DevBuf<cplx<double>> spec64, V64, H64, twx64, twy64, twz64;
+    DevBuf<double> S64, vol64;
+};
+
+namespace {
+
+size_t vox(const utr_ctx* c) {
+    const auto& g = c->plan.g;
+    return (size_t)g.nx * g.ny * g.nz;
+}
+
+template <typename T>
+int upload_twiddles(utr_ctx* c, DevBuf<cplx<T>>& bx, DevBuf<cplx<T>>& by, DevBuf<cplx<T>>& bz) {
+    const auto& g = c->plan.g;
+    const int ns[3] = {g.nx, g.ny, g.nz};
+    DevBuf<cplx<T>>* bs[3] = {&bx, &by, &bz};
+    for (int d = 0; d < 3; d++) {
+        auto tw = make_twiddles<T>(ns[d]);
+        UTR_HIP_TRY(bs[d]->alloc(tw.size()));
+        UTR_HIP_TRY(hipMemcpy(bs[d]->p, tw.data(), sizeof(cplx<T>) * tw.size(), hipMemcpyHostToDevice));
+    }
+    return UTR_OK;
+}
+
+// 2D stage for `count` rows whose images are at d_img (device).
+template <typename T>
+int launch_2d(utr_ctx* c, const float* d_img, int row0, int count, cplx<T>* d_spec,
+              const cplx<T>* twx, const cplx<T>* twy) {
+    const auto& g = c->plan.g;
+    if (count <= 0) return UTR_OK;
+    bool ok = false;
+#define UTR_ROWS(N)                                                                             \
+    {                                                                                           \
+        constexpr int NB = utr::lds_batch(N, sizeof(cplx<T>));                                  \
+        dim3 grid((g.ny + NB - 1) / NB, count);                                                 \
+        utr::k_rows<T, N><<<grid, 256, 0, c->stream>>>(d_img, d_spec, g.ny, g, twx);            \
+        ok = true;                                                                              \
+    }
+    UTR_DISPATCH_N(g.nx, UTR_ROWS)
+#undef UTR_ROWS
+    if (!ok) return set_err(UTR_E_UNSUPPORTED, "unsupported image width for the FFT");
+    ok = false;
+#define UTR_COLS(N)                                                                             \
+    {                                                                                           \
+        constexpr int NB = utr::lds_batch(N, sizeof(cplx<T>));                                  \
+        dim3 grid((g.nx + NB - 1) / NB, count);                                                 \
+        utr::k_cols<T, N><<<grid, 256, 0, c->stream>>>(d_spec, g.nx, g, c->rows.p, row0, twy);  \
+        ok = true;                                                                              \
+    }
+    UTR_DISPATCH_N(g.ny, UTR_COLS)
+#undef UTR_COLS
+    if (!ok) return set_err(UTR_E_UNSUPPORTED, "unsupported image height for the FFT");
+    UTR_HIP_TRY(hipGetLastError());
+    return UTR_OK;
+}
+
+// 3D inverse: pack + z, y, x c2r with mask and scaling.
+template <typename T, typename TO>
+int launch_3d(utr_ctx* c, const cplx<T>* d_V, cplx<T>* d_H, TO* d_vol, const cplx<T>* twx,
+              const cplx<T>* twy, const cplx<T>* twz) {
+    const auto& g = c->plan.g;
+    const int W = g.nx / 2 + 1;
+    bool ok = false;
+#define UTR_PZ(N)                                                                               \
+    {                                                                                           \
+        constexpr int CW = utr::lds_batch(N, sizeof(cplx<T>));                                  \
+        dim3 grid((W + CW - 1) / CW, g.ny);                                                     \
+        utr::k_pack_z<T, N><<<grid, 256, 0, c->stream>>>(d_V, d_H, g, twz);                     \
+        ok = true;                                                                              \
+    }
+    UTR_DISPATCH_N(g.nz, UTR_PZ)
+#undef UTR_PZ
+    if (!ok) return set_err(UTR_E_UNSUPPORTED, "unsupported volume depth for the FFT");
+    ok = false;
+#define UTR_FY(N)                                                                               \
+    {                                                                                           \
+        constexpr int CW = utr::lds_batch(N, sizeof(cplx<T>));                                  \
+        dim3 grid((W + CW - 1) / CW, g.nz);                                                     \
+        utr::k_fft_y<T, N><<<grid, 256, 0, c->stream>>>(d_H, g, twy);                           \
+        ok = true;                                                                              \
+    }
+    UTR_DISPATCH_N(g.ny, UTR_FY)
+#undef UTR_FY
+    if (!ok) return set_err(UTR_E_UNSUPPORTED, "unsupported image height for the FFT");
+    ok = false;
+#define UTR_CX(N)                                                                               \
+    {                                                                                           \
+        constexpr int NB = utr::lds_batch(N, sizeof(cplx<T>));                                  \
+        dim3 grid((g.ny / 2 + NB - 1) / NB, g.nz);                                              \
+        utr::k_c2r_x<T, N, TO><<<grid, 256, 0, c->stream>>>(d_H, d_vol, g, twx);                \
+        ok = true;                                                                              \
+    }
+    UTR_DISPATCH_N(g.nx, UTR_CX)
+#undef UTR_CX
+    if (!ok) return set_err(UTR_E_UNSUPPORTED, "unsupported image width for the FFT");
+    UTR_HIP_TRY(hipGetLastError());
+    return UTR_OK;
+}
+
+int add_rows(utr_ctx* c, int first_row, int count, const float* d_img) {
+    const auto& g = c->plan.g;
+    hipEvent_t e0 = c->ev[6], e1 = c->ev[7];
+    UTR_HIP_TRY(hipEventRecord(e0, c->stream));
+    if (c->precision == UTR_FP32) {
+        int rc = launch_2d<float>(c, d_img, first_row, count,
+                                  c->spec32.p + (size_t)first_row * g.ny * g.nx, c->twx32.p,
+                                  c->twy32.p);
+        if (rc) return rc;
+    } else {
+        // plain form: one row at a time, 2D stage then scatter
+        const size_t n2 = (size_t)g.nx * g.ny;
+        for (int r = 0; r < count; r++) {
+            int rc = launch_2d<double>(c, d_img + r * n2, first_row + r, 1, c->spec64.p,
+                                       c->twx64.p, c->twy64.p);
+            if (rc) return rc;
+            const int threads = 256;
+            const int blocks = (int)((n2 + threads - 1) / threads);
+            utr::k_scatter<<<blocks, threads, 0, c->stream>>>(
+                c->spec64.p, c->rows.p, first_row + r, g, (double*)c->V64.p, c->S64.p);
+            UTR_HIP_TRY(hipGetLastError());
+        }
+    }
+    UTR_HIP_TRY(hipEventRecord(e1, c->stream));
+    UTR_HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0;
+    UTR_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    c->add_ms += ms;
+    for (int r = 0; r < count; r++) {
+        if (!c->added[first_row + r]) c->n_added++;
+        c->added[first_row + r] = 1;
+    }
+    return UTR_OK;
+}
+
+int check_rows(utr_ctx* c, int first_row, int count) {
+    if (!c) return set_err(UTR_E_INVALID, "null context");
+    if (first_row < 0 || count < 0 || first_row + count > c->plan.nrows)
+        return set_err(UTR_E_INVALID, "row range outside the orientation table");
+    return UTR_OK;
+}
+
+int do_finalize(utr_ctx* c, float* d_vol_out, double* d_vol64_out) {
+    const auto& g = c->plan.g;
+    if (c->n_added != c->plan.nrows)
+        return set_err(UTR_E_STATE, "utr_finalize: " + std::to_string(c->plan.nrows - c->n_added) +
+                                        " orientation rows have no image");
+    const size_t N = vox(c);
+    UTR_HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+    if (c->precision == UTR_FP32) {
+        if (c->keep_sampling) UTR_HIP_TRY(c->S32.alloc(N));
+        const int nbx = (g.nx + utr::GBX - 1) / utr::GBX, nby = (g.ny + utr::GBY - 1) / utr::GBY,
+                  nbz = (g.nz + utr::GBZ - 1) / utr::GBZ;
+        const long nblocks = (long)nbx * nby * nbz;
+        if (nblocks <= 0 || nblocks > 0x7fffffffL) return set_err(UTR_E_INVALID, "volume too large");
+        utr::k_gather<<<(unsigned)nblocks, 256, 0, c->stream>>>(
+            c->spec32.p, c->rows.p, c->plan.nrows, g, c->V32.p, c->keep_sampling ? c->S32.p : nullptr);
+        UTR_HIP_TRY(hipGetLastError());
+        UTR_HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+        int rc = launch_3d<float, float>(c, c->V32.p, c->H32.p, d_vol_out, c->twx32.p, c->twy32.p,
+                                         c->twz32.p);
+        if (rc) return rc;
+    } else {
+        const int threads = 256;
+        const unsigned blocks = (unsigned)((N + threads - 1) / threads);
+        utr::k_normalize<double><<<blocks, threads, 0, c->stream>>>(c->V64.p, c->S64.p, g);
+        UTR_HIP_TRY(hipGetLastError());
+        UTR_HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+        int rc;
+        if (d_vol64_out)
+            rc = launch_3d<double, double>(c, c->V64.p, c->H64.p, d_vol64_out, c->twx64.p,
+                                           c->twy64.p, c->twz64.p);
+        else
+            rc = launch_3d<double, float>(c, c->V64.p, c->H64.p, d_vol_out, c->twx64.p,
+                                          c->twy64.p, c->twz64.p);
+        if (rc) return rc;
+    }
+    UTR_HIP_TRY(hipEventRecord(c->ev[2], c->stream));
+    UTR_HIP_TRY(hipEventSynchronize(c->ev[2]));
+    float a = 0, b = 0;
+    UTR_HIP_TRY(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
+    UTR_HIP_TRY(hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
+    c->tim.backprop_ms = a;
+    c->tim.fft3d_ms = b;
+    c->tim.fft2d_ctf_ms = c->add_ms;
+    c->tim.total_ms = c->add_ms + a + b;
+    c->finalized = true;
+    return UTR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* utr_last_error(void) { return g_err.c_str(); }
+
+int utr_abi_version(void) { return UTR_ABI_VERSION; }
+
+int utr_angle_coeffs(const utr_params* params, double angZ_deg, double angY_deg, double* m9) {
+    const double z = 0.0, zero = 0.0;
+    utr::Plan P;
+    std::string err;
+    if (!utr::build_plan(params, 1, &z, &zero, nullptr, &zero, nullptr, P, err))
+        return set_err(UTR_E_INVALID, err);
+    utr::angle_coeffs(P.g, angZ_deg, angY_deg, m9);
+    return UTR_OK;
+}
+
+int utr_create(const utr_params* params, int nrows, const double* angZ_deg, const double* angY_deg,
+               const double* angZ2_deg, const double* defocus, const float* weights, int precision,
+               utr_ctx** out) {
+    if (!out) return set_err(UTR_E_INVALID, "null output pointer");
+    *out = nullptr;
+    if (precision != UTR_FP32 && precision != UTR_FP64)
+        return set_err(UTR_E_INVALID, "precision must be UTR_FP32 or UTR_FP64");
+    utr_ctx* c = new utr_ctx();
+    std::string err;
+    if (!utr::build_plan(params, nrows, angZ_deg, angY_deg, angZ2_deg, defocus, weights, c->plan, err)) {
+        delete c;
+        return set_err(UTR_E_INVALID, err);
+    }
+    const auto& g = c->plan.g;
+    if (!fft_size_supported(g.nx) || !fft_size_supported(g.ny) || !fft_size_supported(g.nz)) {
+        delete c;
+        return set_err(UTR_E_UNSUPPORTED,
+                       "volume edge not supported by the FFT kernels (supported: 16 32 48 64 96 128 "
+                       "160 192 256 320 384 512)");
+    }
+    c->precision = precision;
+    c->added.assign(nrows, 0);
+    auto fail = [&](int rc) {
+        utr_destroy(c);
+        return rc;
+    };
+    if (hipGetDevice(&c->device) != hipSuccess) return fail(set_err(UTR_E_DEVICE, "no HIP device"));
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(set_err(UTR_E_DEVICE, "hipStreamCreate failed"));
+    for (auto& e : c->ev)
+        if (hipEventCreate(&e) != hipSuccess) return fail(set_err(UTR_E_DEVICE, "hipEventCreate failed"));
+    if (c->rows.alloc(nrows) != hipSuccess) return fail(set_err(UTR_E_NOMEM, "row table allocation"));
+    if (hipMemcpy(c->rows.p, c->plan.rows.data(), sizeof(utr::RowParams) * nrows,
+                  hipMemcpyHostToDevice) != hipSuccess)
+        return fail(set_err(UTR_E_DEVICE, "row table upload"));
+    const size_t N = vox(c), n2 = (size_t)g.nx * g.ny;
+    const size_t nh = (size_t)g.nz * g.ny * (g.nx / 2 + 1);
+    int rc;
+    if (precision == UTR_FP32) {
+        if ((rc = upload_twiddles<float>(c, c->twx32, c->twy32, c->twz32))) return fail(rc);
+        if (c->spec32.alloc((size_t)nrows * n2) != hipSuccess || c->V32.alloc(N) != hipSuccess ||
+            c->H32.alloc(nh) != hipSuccess)
+            return fail(set_err(UTR_E_NOMEM, "device allocation of the FP32 buffers failed"));
+    } else {
+        if ((rc = upload_twiddles<double>(c, c->twx64, c->twy64, c->twz64))) return fail(rc);
+        if (c->spec64.alloc(n2) != hipSuccess || c->V64.alloc(N) != hipSuccess ||
+            c->S64.alloc(N) != hipSuccess || c->H64.alloc(nh) != hipSuccess)
+            return fail(set_err(UTR_E_NOMEM, "device allocation of the FP64 buffers failed"));
+        if (hipMemset(c->V64.p, 0, sizeof(cplx<double>) * N) != hipSuccess ||
+            hipMemset(c->S64.p, 0, sizeof(double) * N) != hipSuccess)
+            return fail(set_err(UTR_E_DEVICE, "hipMemset failed"));
+    }
+    *out = c;
+    return UTR_OK;
+}
+
+int utr_add_images_device(utr_ctx* c, int first_row, int count, const float* d_images) {
+    int rc = check_rows(c, first_row, count);
+    if (rc) return rc;
+    if (count == 0) return UTR_OK;
+    if (!d_images) return set_err(UTR_E_INVALID, "null image pointer");
+    return add_rows(c, first_row, count, d_images);
+}
+
+int utr_add_images(utr_ctx* c, int first_row, int count, const float* images) {
+    int rc = check_rows(c, first_row, count);
+    if (rc) return rc;
+    if (count == 0) return UTR_OK;
+    if (!images) return set_err(UTR_E_INVALID, "null image pointer");
+    const auto& g = c->plan.g;
+    const size_t n2 = (size_t)g.nx * g.ny;
+    UTR_HIP_TRY(c->img_stage.alloc(n2 * kStageRows));
+    for (int r = 0; r < count; r += kStageRows) {
+        const int k = std::min(kStageRows, count - r);
+        UTR_HIP_TRY(hipMemcpyAsync(c->img_stage.p, images + (size_t)r * n2, sizeof(float) * n2 * k,
+                                   hipMemcpyHostToDevice, c->stream));
+        rc = add_rows(c, first_row + r, k, c->img_stage.p);
+        if (rc) return rc;
+    }
+    return UTR_OK;
+}
+
+int utr_finalize_device(utr_ctx* c, float* d_vol) {
+    if (!c) return set_err(UTR_E_INVALID, "null context");
+    if (!d_vol) return set_err(UTR_E_INVALID, "null output pointer");
+    return do_finalize(c, d_vol, nullptr);
+}
+
+int utr_finalize(utr_ctx* c, float* vol) {
+    if (!c) return set_err(UTR_E_INVALID, "null context");
+    if (!vol) return set_err(UTR_E_INVALID, "null output pointer");
+    UTR_HIP_TRY(c->vol32.alloc(vox(c)));
+    int rc = do_finalize(c, c->vol32.p, nullptr);
+    if (rc) return rc;
+    UTR_HIP_TRY(hipMemcpy(vol, c->vol32.p, sizeof(float) * vox(c), hipMemcpyDeviceToHost));
+    return UTR_OK;
+}
+
+int utr_finalize_f64(utr_ctx* c, double* vol) {
+    if (!c) return set_err(UTR_E_INVALID, "null context");
+    if (!vol) return set_err(UTR_E_INVALID, "null output pointer");
+    if (c->precision != UTR_FP64) return set_err(UTR_E_INVALID, "utr_finalize_f64 needs a UTR_FP64 context");
+    UTR_HIP_TRY(c->vol64.alloc(vox(c)));
+    int rc = do_finalize(c, nullptr, c->vol64.p);
+    if (rc) return rc;
+    UTR_HIP_TRY(hipMemcpy(vol, c->vol64.p, sizeof(double) * vox(c), hipMemcpyDeviceToHost));
+    return UTR_OK;
+}
+
+int utr_keep_sampling(utr_ctx* c, int on) {
+    if (!c) return set_err(UTR_E_INVALID, "null context");
+    c->keep_sampling = on ? 1 : 0;
+    return UTR_OK;
+}
+
+int utr_get_sampling(utr_ctx* c, float* samp) {
+    if (!c || !samp) return set_err(UTR_E_INVALID, "null argument");
+    if (!c->finalized) return set_err(UTR_E_STATE, "utr_get_sampling before utr_finalize");
+    const size_t N = vox(c);
+    if (c->precision == UTR_FP32) {
+        if (!c->keep_sampling || !c->S32.p)
+            return set_err(UTR_E_STATE, "sampling matrix not kept (call utr_keep_sampling before utr_finalize)");
+        UTR_HIP_TRY(hipMemcpy(samp, c->S32.p, sizeof(float) * N, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<double> tmp(N);
+        UTR_HIP_TRY(hipMemcpy(tmp.data(), c->S64.p, sizeof(double) * N, hipMemcpyDeviceToHost));
+        for (size_t t = 0; t < N; t++) samp[t] = (float)tmp[t];
+    }
+    return UTR_OK;
+}
+
+int utr_get_spectrum(utr_ctx* c, int row, float* spec) {
+    if (!c || !spec) return set_err(UTR_E_INVALID, "null argument");
+    if (row < 0 || row >= c->plan.nrows) return set_err(UTR_E_INVALID, "row outside the table");
+    if (c->precision != UTR_FP32) return set_err(UTR_E_INVALID, "utr_get_spectrum needs a UTR_FP32 context");
+    if (!c->added[row]) return set_err(UTR_E_STATE, "row has no image yet");
+    const size_t n2 = (size_t)c->plan.g.nx * c->plan.g.ny;
+    UTR_HIP_TRY(hipStreamSynchronize(c->stream));
+    UTR_HIP_TRY(hipMemcpy(spec, c->spec32.p + (size_t)row * n2, sizeof(cplx<float>) * n2,
+                          hipMemcpyDeviceToHost));
+    return UTR_OK;
+}
+
+int utr_get_fourier_volume(utr_ctx* c, float* out) {
+    if (!c || !out) return set_err(UTR_E_INVALID, "null argument");
+    if (!c->finalized) return set_err(UTR_E_STATE, "utr_get_fourier_volume before utr_finalize");
+    const size_t N = vox(c);
+    if (c->precision == UTR_FP32) {
+        UTR_HIP_TRY(hipMemcpy(out, c->V32.p, sizeof(cplx<float>) * N, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<cplx<double>> tmp(N);
+        UTR_HIP_TRY(hipMemcpy(tmp.data(), c->V64.p, sizeof(cplx<double>) * N, hipMemcpyDeviceToHost));
+        for (size_t t = 0; t < N; t++) {
+            out[2 * t] = (float)tmp[t].x;
+            out[2 * t + 1] = (float)tmp[t].y;
+        }
+    }
+    return UTR_OK;
+}
+
+int utr_reset(utr_ctx* c) {
+    if (!c) return set_err(UTR_E_INVALID, "null context");
+    std::fill(c->added.begin(), c->added.end(), 0);
+    c->n_added = 0;
+    c->add_ms = 0;
+    c->finalized = false;
+    if (c->precision == UTR_FP64) {
+        UTR_HIP_TRY(hipMemsetAsync(c->V64.p, 0, sizeof(cplx<double>) * vox(c), c->stream));
+        UTR_HIP_TRY(hipMemsetAsync(c->S64.p, 0, sizeof(double) * vox(c), c->stream));
+        UTR_HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    return UTR_OK;
+}
+
+int utr_dims(const utr_ctx* c, int* nx, int* ny, int* nz) {
+    if (!c) return set_err(UTR_E_INVALID, "null context");
+    if (nx) *nx = c->plan.g.nx;
+    if (ny) *ny = c->plan.g.ny;
+    if (nz) *nz = c->plan.g.nz;
+    return UTR_OK;
+}
+
+int utr_get_timings(const utr_ctx* c, utr_timings* t) {
+    if (!c || !t) return set_err(UTR_E_INVALID, "null argument");
+    *t = c->tim;
+    return UTR_OK;
+}
+
+int utr_destroy(utr_ctx* c) {
+    if (!c) return UTR_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->rows.release();
+    c->img_stage.release();
+    c->spec32.release(); c->V32.release(); c->H32.release(); c->S32.release(); c->vol32.release();
+    c->twx32.release(); c->twy32.release(); c->twz32.release();
+    c->spec64.release(); c->V64.release(); c->H64.release(); c->S64.release(); c->vol64.release();
+    c->twx64.release(); c->twy64.release(); c->twz64.release();
+    for (auto& e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return UTR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,596 @@
+// utr_kernels.hip -- gfx950 kernels of the UTR reconstruction path.
+//
+// Data layout in HBM (all row-major, x fastest, "centred" as the reference's
+// shuffled arrays: zero frequency / zero coordinate at index n/2):
+//   images   [row][ny][nx]            float      input intensities
+//   spectra  [row][ny][nx]            cplx<T>    K2four of every row
+//   V        [nz][ny][nx]             cplx<T>    normalised, filtered V3
+//   S        [nz][ny][nx]             T          Samp3 (optional output)
+//   H        [nz][ny][nx/2+1]         cplx<T>    half spectrum during the 3D
+//                                                inverse FFT (x frequency
+//                                                wrapped 0..nx/2)
+//   vol      [nz][ny][nx]             float/f64  K3out
+//
+// Reference stages (MultisliceCppTest/PhaseRetrieval):
+//   k_rows / k_cols        UTR.cpp:951-1019  (Log0, Shuffle, FFT, 2D filters)
+//   k_gather               UTR.h:188-307 gridding + UTR.cpp:1050-1201
+//   k_scatter, k_normalize plain FP64 form of the same
+//   k_pack_z / k_fft_y / k_c2r_x   UTR.cpp:1203-1279
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "utr_fft.hpp"
+#include "utr_plan.hpp"
+
+namespace utr {
+
+// ------------------------------------------------------------ helpers
+
+template <typename T>
+__device__ __forceinline__ T dev_log(float x);
+template <>
+__device__ __forceinline__ float dev_log<float>(float x) { return logf(x); }
+template <>
+__device__ __forceinline__ double dev_log<double>(float x) { return log((double)x); }
+
+__host__ __device__ constexpr int lds_batch(int n, int elem_bytes) {
+    // sequences per LDS buffer: at most 16 KiB per buffer, between 1 and 16
+    return (16384 / (n * elem_bytes)) < 1 ? 1 : ((16384 / (n * elem_bytes)) > 16 ? 16 : (16384 / (n * elem_bytes)));
+}
+
+// Sign (-1)^v for integer v.
+__device__ __forceinline__ int par_sign(int v) { return (v & 1) ? -1 : 1; }
+
+// ------------------------------------------------------------- 2D stage
+
+// Rows: renormalise, clamp, Log0 (UTR.cpp:951-958, 991), checkerboard
+// (-1)^(i+j) (the input Shuffle of UTR.cpp:993) and forward FFT along x.
+template <typename T, int N>
+__global__ void __launch_bounds__(256) k_rows(const float* __restrict__ img,
+                                              cplx<T>* __restrict__ out, int ny, GeomDev g,
+                                              const cplx<T>* __restrict__ tw) {
+    constexpr int NB = lds_batch(N, sizeof(cplx<T>));
+    __shared__ cplx<T> sm[2 * NB * N];
+    cplx<T>* a = sm;
+    cplx<T>* b = sm + NB * N;
+    const int row = blockIdx.y;
+    const int j0 = blockIdx.x * NB;
+    const int nb = min(NB, ny - j0);
+    const float* src = img + ((size_t)row * ny + j0) * N;
+    for (int t = threadIdx.x; t < nb * N; t += blockDim.x) {
+        const int jj = t / N, i = t - jj * N;
+        float x = src[t];
+        if (g.use_f1) x = x / g.f1;
+        if (g.use_f2) x = x + g.f2;
+        if (x < 0.f) x = 0.f;
+        T f = (x > 0.f) ? dev_log<T>(x) : T(0);
+        if ((i + j0 + jj) & 1) f = -f;
+        a[t] = {f, T(0)};
+    }
+    __syncthreads();
+    fft_lds<T, N>(a, b, nb, tw, -1, threadIdx.x, blockDim.x);
+    cplx<T>* dst = out + ((size_t)row * ny + j0) * N;
+    for (int t = threadIdx.x; t < nb * N; t += blockDim.x) dst[t] = a[t];
+}
+
+// Columns: forward FFT along y, output Shuffle as (-1)^(kx+ky+nx/2+ny/2),
+// optional 2D filter (modes 1/3, restated) and subseries weight.
+template <typename T, int N>
+__global__ void __launch_bounds__(256) k_cols(cplx<T>* __restrict__ spec, int nx, GeomDev g,
+                                              const RowParams* __restrict__ rows, int row0,
+                                              const cplx<T>* __restrict__ tw) {
+    constexpr int NB = lds_batch(N, sizeof(cplx<T>));
+    __shared__ cplx<T> sm[2 * NB * N];
+    cplx<T>* a = sm;
+    cplx<T>* b = sm + NB * N;
+    const int ny = N;
+    const int row = blockIdx.y;
+    const int x0 = blockIdx.x * NB;
+    const int nb = min(NB, nx - x0);
+    cplx<T>* base = spec + (size_t)row * ny * nx + x0;
+    for (int t = threadIdx.x; t < nb * N; t += blockDim.x) {
+        const int y = t / nb, c = t - y * nb;
+        a[c * N + y] = base[(size_t)y * nx + c];
+    }
+    __syncthreads();
+    fft_lds<T, N>(a, b, nb, tw, -1, threadIdx.x, blockDim.x);
+    const RowParams& R = rows[row0 + row];
+    const int gsgn = par_sign(g.nxd2 + g.nyd2);
+    const double xap = g.xhi - g.xlo, yap = g.yhi - g.ylo;
+    const double pi = 3.14159265358979323846;
+    for (int t = threadIdx.x; t < nb * N; t += blockDim.x) {
+        const int ky = t / nb, c = t - ky * nb;
+        const int kx = x0 + c;
+        cplx<T> v = a[c * N + ky];
+        const T s = T(par_sign(kx + ky) * gsgn);
+        v.x *= s;
+        v.y *= s;
+        if (g.ctf_mode == 1 && R.zout != 0) {
+            // 2D TIE-Hom restated from UTR.h:159-181 in 2D (unpinned)
+            const double i1 = kx - g.nxd2, j1 = ky - g.nyd2;
+            const double q2 = 4.0 * pi * pi * (i1 * i1 / (xap * xap) + j1 * j1 / (yap * yap));
+            const T f = T(g.two_d_norm / (g.two_d_alpha + q2));
+            v.x *= f;
+            v.y *= f;
+        } else if (g.ctf_mode == 3) {
+            // 2D CTF inversion restated from UTR.h:70-128 in 2D (unpinned)
+            const double dk2 = 1.0 / (xap * xap), de2 = 1.0 / (yap * yap);
+            const double i1 = kx - g.nxd2, j1 = ky - g.nyd2;
+            const double q2 = dk2 * i1 * i1 + de2 * j1 * j1;
+            if (!(g.k2maxo > 0) || q2 < g.k2maxo) {
+                const double fac2 = pi * g.wl * R.zout;
+                double eps = g.epsilon;
+                if (eps <= 0) {
+                    const double mn = dk2 < de2 ? dk2 : de2;
+                    eps = 0.1 * (fabs(fac2) * mn) * (fabs(fac2) * mn);
+                }
+                double chi = fac2 * q2 - atan(g.asigma);
+                if (g.cs3 != 0 || g.cs5 != 0)
+                    chi += pi * g.wl * g.wl * g.wl / 2.0 * g.cs3 * q2 * q2 +
+                           pi * g.wl * g.wl * g.wl * g.wl * g.wl / 3.0 * g.cs5 * q2 * q2 * q2;
+                double sn, cs;
+                sincos(chi, &sn, &cs);
+                const T sr = T(sn / (sn * sn + eps));
+                const T cr = g.escc ? T(cs / (cs * cs + eps)) : T(0);
+                const cplx<T> w = {v.x * sr + v.y * cr, v.y * sr - v.x * cr};
+                v = w;
+            }
+        }
+        if (R.weight != 1.0f) {
+            v.x *= T(R.weight);
+            v.y *= T(R.weight);
+        }
+        base[(size_t)ky * nx + c] = v;
+    }
+}
+
+// ------------------------------------------------- gridding coordinates
+
+// Grid position of spectrum point (a, b) = (i - nx/2, j - ny/2) of one row:
+// the rotation of UTR.h:243-256 in grid units, with explicit fma so that the
+// CPU restatement evaluates bit-identical coordinates.
+struct GridPos {
+    double dx1, dy1, dz1, ox, oy, oz;
+};
+
+__device__ __forceinline__ GridPos grid_pos(const GeomDev& g, const double* m, int ia, int ib) {
+    const double a = (double)ia, b = (double)ib;
+    const double qx = __dmul_rn(g.fxst, a), qy = __dmul_rn(g.fyst, b);
+    const double zz = g.escc ? __dmul_rn(-g.wl2, __fma_rn(qx, qx, __dmul_rn(qy, qy))) : 0.0;
+    GridPos p;
+    p.ox = __fma_rn(a, m[0], __fma_rn(b, m[1], __dmul_rn(zz, m[2])));
+    p.oy = __fma_rn(a, m[3], __fma_rn(b, m[4], __dmul_rn(zz, m[5])));
+    p.oz = __fma_rn(a, m[6], __dmul_rn(zz, m[8]));
+    p.dx1 = __dadd_rn((double)g.nxd2, p.ox);
+    p.dy1 = __dadd_rn((double)g.nyd2, p.oy);
+    p.dz1 = __dadd_rn((double)g.nzd2, p.oz);
+    return p;
+}
+
+// --------------------------------------------------- voxel epilogue
+
+// Sampling normalisation + noise filter (UTR.cpp:1050-1128) and the 3D
+// filters (UTR.cpp:1177-1201; UTR.h:15-185) for voxel (i, j, k).
+template <typename T>
+__device__ __forceinline__ cplx<T> voxel_epilogue(const GeomDev& g, int i, int j, int k,
+                                                  cplx<T> v, T x) {
+    const double pi = 3.14159265358979323846;
+    const T pif = T(pi);
+    const T target = T(kTargetSNR);
+    if (g.nsr2 == 0) {
+        if (x == T(0)) return {T(0), T(0)};
+        const T r = T(1) / x;
+        v.x *= r;
+        v.y *= r;
+    } else {
+        if (x == T(0)) return {T(0), T(0)};
+        T fnsr;
+        if (g.ctf_mode != 1) {
+            fnsr = T(g.nsr2 * (double)sqrt(x));
+        } else {
+            const double norm = sqrt(1.0 + g.asigma * g.asigma) * g.nsr2;
+            const double fac2 = pi * g.wl * g.zout_aver;
+            const double dksi2 = fac2 / ((g.xhi - g.xlo) * (g.xhi - g.xlo));
+            const double deta2 = fac2 / ((g.yhi - g.ylo) * (g.yhi - g.ylo));
+            const double dzeta2 = fac2 / ((g.zhi - g.zlo) * (g.zhi - g.zlo));
+            const int k1 = k - g.nzd2, j1 = j - g.nyd2, i1 = i - g.nxd2;
+            const double zeta2 = k1 * k1 * dzeta2 - g.asigma;
+            const double eta2 = j1 * j1 * deta2 + zeta2;
+            fnsr = T(norm * (double)sqrt(x) / (dksi2 * i1 * i1 + eta2 + g.epsilon));
+        }
+        const T y = sqrt(v.x * v.x + v.y * v.y) / fnsr;
+        const T f = (T(1) / x) * (T(0.5) + atan(T(100) * (y - target)) / pif);
+        v.x *= f;
+        v.y *= f;
+    }
+    const int k1 = k - g.nzd2, j1 = j - g.nyd2, i1 = i - g.nxd2;
+    if (g.do_tie3d || g.do_invlap) {
+        // InverseMLaplacian3D(V3, zhi-zlo, yhi-ylo, xhi-ylo, alpha, norm)
+        const double fact = 4.0 * pi * pi;
+        const double xa = g.xhi - g.ylo, ya = g.yhi - g.ylo, za = g.zhi - g.zlo;
+        const double base = fact * (double)i1 * i1 / (xa * xa) + fact * (double)j1 * j1 / (ya * ya) +
+                            fact * (double)k1 * k1 / (za * za);
+        if (g.do_tie3d) {
+            const T f = T(g.tie_norm / (base + g.tie_alpha));
+            v.x *= f;
+            v.y *= f;
+        }
+        if (g.do_invlap) {
+            const T f = T(g.epsilon / (base + g.epsilon));
+            v.x *= f;
+            v.y *= f;
+        }
+    }
+    if (g.do_ctf3d) {
+        // CTFcorrection3D (UTR.h:86-131); phase argument in FP64
+        const double xap2 = (g.xhi - g.xlo) * (g.xhi - g.xlo);
+        const double yap2 = (g.yhi - g.ylo) * (g.yhi - g.ylo);
+        const double zap2 = (g.zhi - g.zlo) * (g.zhi - g.zlo);
+        const double fac2 = pi * g.wl * g.zout_aver;
+        const double zeta2 = (1.0 / zap2) * k1 * k1;
+        const double zeta2fac2 = fac2 * zeta2 - atan(g.asigma);
+        double eta2 = (1.0 / yap2) * j1 * j1;
+        const double eta2fac2 = fac2 * eta2 + zeta2fac2;
+        eta2 += zeta2;
+        double ksi2 = (1.0 / xap2) * i1 * i1;
+        const double ksi2fac2 = fac2 * ksi2 + eta2fac2;
+        ksi2 += eta2;
+        if (!(g.k2maxo > 0) || ksi2 < g.k2maxo) {
+            double dt = ksi2fac2;
+            if (g.cs3 != 0 || g.cs5 != 0)
+                dt += pi * g.wl * g.wl * g.wl / 2.0 * g.cs3 * ksi2 * ksi2 +
+                      pi * g.wl * g.wl * g.wl * g.wl * g.wl / 3.0 * g.cs5 * ksi2 * ksi2 * ksi2;
+            double sn, cs;
+            sincos(dt, &sn, &cs);
+            const T sr = T(sn / (sn * sn + g.ctf_eps));
+            const T cr = g.escc ? T(cs / (cs * cs + g.ctf_eps)) : T(0);
+            const cplx<T> w = {v.x * sr + v.y * cr, v.y * sr - v.x * cr};
+            v = w;
+        }
+    }
+    return v;
+}
+
+// ------------------------------------------------ gather gridding (FP32)
+
+constexpr int GBX = 8, GBY = 8, GBZ = 4;  // brick of 256 voxels, one per thread
+constexpr int LIST_CAP = 1024;
+
+// Conservative test: may any point of this row touch a voxel of the brick
+// centred at grid offset (cx, cy, cz) with half extents (ex, ey, ez)?
+__device__ __forceinline__ bool brick_touch(const GeomDev& g, const RowParams& R, float cx,
+                                            float cy, float cz, float ex, float ey, float ez,
+                                            float fx2, float fy2, float wl2f) {
+    const float* mi = R.minv;
+    const float s = mi[0] * cx + mi[1] * cy + mi[2] * cz;
+    const float t = mi[3] * cx + mi[4] * cy + mi[5] * cz;
+    const float h = mi[6] * cx + mi[7] * cy + mi[8] * cz;
+    const float Wa = fabsf(mi[0]) * ex + fabsf(mi[1]) * ey + fabsf(mi[2]) * ez + 0.01f;
+    const float Wb = fabsf(mi[3]) * ex + fabsf(mi[4]) * ey + fabsf(mi[5]) * ez + 0.01f;
+    const float Wz = (fabsf(mi[6]) * ex + fabsf(mi[7]) * ey + fabsf(mi[8]) * ez) * 1.01f + 0.01f * R.wz;
+    float alo = fmaxf(s - Wa, (float)(-g.nxd2)), ahi = fminf(s + Wa, (float)(g.nx - 1 - g.nxd2));
+    float blo = fmaxf(t - Wb, (float)(-g.nyd2)), bhi = fminf(t + Wb, (float)(g.ny - 1 - g.nyd2));
+    if (alo > ahi || blo > bhi) return false;
+    if (!g.escc) return fabsf(h) <= Wz;
+    const float a2min = (alo <= 0.f && ahi >= 0.f) ? 0.f : fminf(alo * alo, ahi * ahi);
+    const float a2max = fmaxf(alo * alo, ahi * ahi);
+    const float b2min = (blo <= 0.f && bhi >= 0.f) ? 0.f : fminf(blo * blo, bhi * bhi);
+    const float b2max = fmaxf(blo * blo, bhi * bhi);
+    const float zz_hi = -wl2f * (fx2 * a2min + fy2 * b2min);
+    const float zz_lo = -wl2f * (fx2 * a2max + fy2 * b2max);
+    const float tol = 1e-3f * (fabsf(zz_lo) + Wz);
+    return h >= zz_lo - Wz - tol && h <= zz_hi + Wz + tol;
+}
+
+__device__ __forceinline__ void kahan_add(float& sum, float& c, float v) {
+    const float y = v - c;
+    const float t = sum + y;
+    c = (t - sum) - y;
+    sum = t;
+}
+
+// One workgroup per brick; every thread owns one voxel and accumulates the
+// contributions of all rows in row order (then j, then i, as UTR.h:230-240
+// visits them) with compensated FP32 sums.  Deterministic, no atomics.
+__global__ void __launch_bounds__(256) k_gather(const cplx<float>* __restrict__ spec,
+                                                const RowParams* __restrict__ rows, int nrows,
+                                                GeomDev g, cplx<float>* __restrict__ Vout,
+                                                float* __restrict__ Sout) {
+    __shared__ int s_list[LIST_CAP];
+    __shared__ int s_cnt[4];
+    __shared__ int s_n;
+    const int tid = threadIdx.x;
+    const int nbx = (g.nx + GBX - 1) / GBX, nby = (g.ny + GBY - 1) / GBY;
+    const int bid = blockIdx.x;
+    const int bx = bid % nbx, by = (bid / nbx) % nby, bz = bid / (nbx * nby);
+    const int vx = bx * GBX + (tid % GBX);
+    const int vy = by * GBY + ((tid / GBX) % GBY);
+    const int vz = bz * GBZ + tid / (GBX * GBY);
+    const float fx2 = (float)(g.fxst * g.fxst), fy2 = (float)(g.fyst * g.fyst);
+    const float wl2f = (float)g.wl2;
+    // brick centre and half extents in grid offsets, grown by one voxel
+    const float bcx = bx * GBX + 0.5f * (GBX - 1) - g.nxd2;
+    const float bcy = by * GBY + 0.5f * (GBY - 1) - g.nyd2;
+    const float bcz = bz * GBZ + 0.5f * (GBZ - 1) - g.nzd2;
+    const float ex = 0.5f * (GBX - 1) + 1.f, ey = 0.5f * (GBY - 1) + 1.f, ez = 0.5f * (GBZ - 1) + 1.f;
+    const float dxg = (float)(vx - g.nxd2), dyg = (float)(vy - g.nyd2), dzg = (float)(vz - g.nzd2);
+    const bool inside = vx < g.nx && vy < g.ny && vz < g.nz;
+
+    float sr = 0.f, cr = 0.f, si = 0.f, ci = 0.f, ss = 0.f, cs = 0.f;
+    const int lane = tid & 63, wave = tid >> 6;
+
+    for (int base = 0; base < nrows; base += LIST_CAP) {
+        const int nchunk = min(LIST_CAP, nrows - base);
+        if (tid == 0) s_n = 0;
+        __syncthreads();
+        for (int r0 = 0; r0 < nchunk; r0 += 256) {
+            const int r = r0 + tid;
+            bool pass = false;
+            if (r < nchunk)
+                pass = brick_touch(g, rows[base + r], bcx, bcy, bcz, ex, ey, ez, fx2, fy2, wl2f);
+            const unsigned long long mask = __ballot(pass);
+            const int pre = __popcll(mask & ((1ull << lane) - 1ull));
+            if (lane == 0) s_cnt[wave] = __popcll(mask);
+            __syncthreads();
+            int off = s_n;
+            for (int w = 0; w < wave; w++) off += s_cnt[w];
+            if (pass) s_list[off + pre] = base + r;
+            __syncthreads();
+            if (tid == 0) s_n += s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+            __syncthreads();
+        }
+        const int n = s_n;
+        for (int q = 0; q < n; q++) {
+            const int idx = __builtin_amdgcn_readfirstlane(s_list[q]);
+            const RowParams& R = rows[idx];
+            if (!inside) continue;
+            const float* mi = R.minv;
+            const float s = mi[0] * dxg + mi[1] * dyg + mi[2] * dzg;
+            const float t = mi[3] * dxg + mi[4] * dyg + mi[5] * dzg;
+            const float h = mi[6] * dxg + mi[7] * dyg + mi[8] * dzg;
+            float zzs = 0.f, G = 0.f;
+            if (g.escc) {
+                zzs = -wl2f * (fx2 * s * s + fy2 * t * t);
+                G = 2.f * wl2f * (fx2 * (fabsf(s) + R.wa) * R.wa + fy2 * (fabsf(t) + R.wb) * R.wb);
+            }
+            if (fabsf(zzs - h) > R.wz * 1.001f + G * 1.01f + 1e-3f * R.wz) continue;
+            const float eps = 1e-3f;
+            int alo = (int)ceilf(s - R.wa - eps), ahi = (int)floorf(s + R.wa + eps);
+            alo = max(alo, -g.nxd2);
+            ahi = min(ahi, g.nx - 1 - g.nxd2);
+            int blo, bhi;
+            if (R.y_exact) {
+                blo = bhi = vy - g.nyd2;
+            } else {
+                blo = (int)ceilf(t - R.wb - eps);
+                bhi = (int)floorf(t + R.wb + eps);
+                blo = max(blo, -g.nyd2);
+                bhi = min(bhi, g.ny - 1 - g.nyd2);
+            }
+            const cplx<float>* K = spec + (size_t)idx * g.ny * g.nx;
+            for (int b = blo; b <= bhi; b++) {
+                for (int a = alo; a <= ahi; a++) {
+                    const GridPos P = grid_pos(g, R.m, a, b);
+                    const double fnn = floor(P.dz1), fii = floor(P.dx1), fjj = floor(P.dy1);
+                    if (fnn < 0.0 || fnn > (double)(g.nz - 2)) continue;
+                    if (fii < 0.0 || fii > (double)(g.nx - 2)) continue;
+                    if (fjj < 0.0 || fjj > (double)(g.ny - 2)) continue;
+                    const int ox = vx - (int)fii, oy = vy - (int)fjj, oz = vz - (int)fnn;
+                    if ((unsigned)ox > 1u || (unsigned)oy > 1u || (unsigned)oz > 1u) continue;
+                    const double fx = P.dx1 - fii, fy = P.dy1 - fjj, fz = P.dz1 - fnn;
+                    const double wx = ox ? fx : 1.0 - fx;
+                    const double wy = oy ? fy : 1.0 - fy;
+                    const double wz = oz ? fz : 1.0 - fz;
+                    const float w = (float)(wx * wy * wz);
+                    cplx<float> kv = K[(size_t)(b + g.nyd2) * g.nx + (a + g.nxd2)];
+                    if (g.rot_centre_shift) {
+                        const double ph = -(R.ph[0] * P.ox + R.ph[1] * P.oy + R.ph[2] * P.oz);
+                        double sn, c;
+                        sincos(ph, &sn, &c);
+                        const float cf = (float)c, sf = (float)sn;
+                        kv = {kv.x * cf - kv.y * sf, kv.x * sf + kv.y * cf};
+                    }
+                    kahan_add(sr, cr, w * kv.x);
+                    kahan_add(si, ci, w * kv.y);
+                    kahan_add(ss, cs, w);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (!inside) return;
+    const size_t vidx = ((size_t)vz * g.ny + vy) * g.nx + vx;
+    if (Sout) Sout[vidx] = ss;
+    Vout[vidx] = voxel_epilogue<float>(g, vx, vy, vz, cplx<float>{sr, si}, ss);
+}
+
+// ------------------------------------------- plain FP64 form: scatter
+
+// One thread per spectrum point of one row; atomics into V (interleaved
+// re/im) and S, as UTR.h:230-305 does from its OpenMP threads.
+__global__ void __launch_bounds__(256) k_scatter(const cplx<double>* __restrict__ K,
+                                                 const RowParams* __restrict__ rows, int row,
+                                                 GeomDev g, double* __restrict__ V,
+                                                 double* __restrict__ S) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= g.nx * g.ny) return;
+    const int j = p / g.nx, i = p - j * g.nx;
+    const RowParams& R = rows[row];
+    const GridPos P = grid_pos(g, R.m, i - g.nxd2, j - g.nyd2);
+    const double fnn = floor(P.dz1), fii = floor(P.dx1), fjj = floor(P.dy1);
+    if (fnn < 0.0 || fnn > (double)(g.nz - 2)) return;
+    if (fii < 0.0 || fii > (double)(g.nx - 2)) return;
+    if (fjj < 0.0 || fjj > (double)(g.ny - 2)) return;
+    const long ii = (long)fii, jj = (long)fjj, nn = (long)fnn;
+    const double fx = P.dx1 - fii, fy = P.dy1 - fjj, fz = P.dz1 - fnn;
+    const double wxs[2] = {1.0 - fx, fx}, wys[2] = {1.0 - fy, fy}, wzs[2] = {1.0 - fz, fz};
+    cplx<double> kv = K[p];
+    if (g.rot_centre_shift) {
+        const double ph = -(R.ph[0] * P.ox + R.ph[1] * P.oy + R.ph[2] * P.oz);
+        double sn, c;
+        sincos(ph, &sn, &c);
+        kv = {kv.x * c - kv.y * sn, kv.x * sn + kv.y * c};
+    }
+    const long nxny = (long)g.nx * g.ny;
+    for (int cz = 0; cz < 2; cz++)
+        for (int cy = 0; cy < 2; cy++)
+            for (int cx = 0; cx < 2; cx++) {
+                const double w = wxs[cx] * wys[cy] * wzs[cz];
+                const long idx = (nn + cz) * nxny + (jj + cy) * g.nx + (ii + cx);
+                atomicAdd(&S[idx], w);
+                atomicAdd(&V[2 * idx], w * kv.x);
+                atomicAdd(&V[2 * idx + 1], w * kv.y);
+            }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_normalize(cplx<T>* __restrict__ V,
+                                                   const T* __restrict__ S, GeomDev g) {
+    const size_t n = (size_t)g.nx * g.ny * g.nz;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int i = (int)(t % g.nx), j = (int)((t / g.nx) % g.ny), k = (int)(t / ((size_t)g.nx * g.ny));
+    V[t] = voxel_epilogue<T>(g, i, j, k, V[t], S[t]);
+}
+
+// ------------------------------------------------------ 3D inverse FFT
+
+constexpr int CW_MAX = 16;
+
+// Hermitian combination of V and its mirror (the symmetrisation of
+// UTR.cpp:1203-1217 followed by the half the c2r plan consumes), checkerboard
+// (-1)^(kz+ky+w) for the centred z/y indices, and inverse FFT along z.
+// Nyquist x-plane (centred i = 0): the serial loop order of the reference
+// gives H = (3a + 5 conj b)/8 when (k,j) precedes its mirror, (5a + 3 conj b)/8
+// when it follows it, and Re(a) when it is its own mirror.
+template <typename T, int NZ>
+__global__ void __launch_bounds__(256) k_pack_z(const cplx<T>* __restrict__ V,
+                                                cplx<T>* __restrict__ H, GeomDev g,
+                                                const cplx<T>* __restrict__ tw) {
+    constexpr int CW = lds_batch(NZ, sizeof(cplx<T>));
+    __shared__ cplx<T> sm[2 * CW * NZ];
+    cplx<T>* a = sm;
+    cplx<T>* b = sm + CW * NZ;
+    const int nx = g.nx, ny = g.ny, W = nx / 2 + 1;
+    const int j = blockIdx.y;
+    const int w0 = blockIdx.x * CW;
+    const int nb = min(CW, W - w0);
+    const int ja = (j == 0) ? 0 : ny - j;
+    for (int t = threadIdx.x; t < nb * NZ; t += blockDim.x) {
+        const int k = t / nb, c = t - k * nb;
+        const int w = w0 + c;
+        const int i = (w == nx / 2) ? 0 : w + g.nxd2;
+        const int ia = (i == 0) ? 0 : nx - i;
+        const int ka = (k == 0) ? 0 : NZ - k;
+        const cplx<T> va = V[((size_t)k * ny + j) * nx + i];
+        const cplx<T> vb = V[((size_t)ka * ny + ja) * nx + ia];
+        cplx<T> h;
+        if (i != 0) {
+            h = {T(0.5) * (va.x + vb.x), T(0.5) * (va.y - vb.y)};
+        } else if (k == ka && j == ja) {
+            h = {va.x, T(0)};
+        } else if (k < ka || (k == ka && j < ja)) {
+            h = {(T(3) * va.x + T(5) * vb.x) * T(0.125), (T(3) * va.y - T(5) * vb.y) * T(0.125)};
+        } else {
+            h = {(T(5) * va.x + T(3) * vb.x) * T(0.125), (T(5) * va.y - T(3) * vb.y) * T(0.125)};
+        }
+        if ((k + j + w) & 1) {
+            h.x = -h.x;
+            h.y = -h.y;
+        }
+        a[c * NZ + k] = h;
+    }
+    __syncthreads();
+    fft_lds<T, NZ>(a, b, nb, tw, +1, threadIdx.x, blockDim.x);
+    for (int t = threadIdx.x; t < nb * NZ; t += blockDim.x) {
+        const int k = t / nb, c = t - k * nb;
+        H[((size_t)k * ny + j) * W + w0 + c] = a[c * NZ + k];
+    }
+}
+
+// Inverse FFT along y of the half spectrum, in place.
+template <typename T, int NY>
+__global__ void __launch_bounds__(256) k_fft_y(cplx<T>* __restrict__ H, GeomDev g,
+                                               const cplx<T>* __restrict__ tw) {
+    constexpr int CW = lds_batch(NY, sizeof(cplx<T>));
+    __shared__ cplx<T> sm[2 * CW * NY];
+    cplx<T>* a = sm;
+    cplx<T>* b = sm + CW * NY;
+    const int W = g.nx / 2 + 1;
+    const int k = blockIdx.y;
+    const int w0 = blockIdx.x * CW;
+    const int nb = min(CW, W - w0);
+    cplx<T>* base = H + (size_t)k * NY * W + w0;
+    for (int t = threadIdx.x; t < nb * NY; t += blockDim.x) {
+        const int y = t / nb, c = t - y * nb;
+        a[c * NY + y] = base[(size_t)y * W + c];
+    }
+    __syncthreads();
+    fft_lds<T, NY>(a, b, nb, tw, +1, threadIdx.x, blockDim.x);
+    for (int t = threadIdx.x; t < nb * NY; t += blockDim.x) {
+        const int y = t / nb, c = t - y * nb;
+        base[(size_t)y * W + c] = a[c * NY + y];
+    }
+}
+
+// Complex-to-real inverse FFT along x of two rows at once (z = x1 + i x2),
+// then (-1)^(k+j+(nz+ny)/2)/N, sphere mask, cupping compensation and final
+// scale (UTR.cpp:1235-1279).
+template <typename T, int NX, typename TO>
+__global__ void __launch_bounds__(256) k_c2r_x(const cplx<T>* __restrict__ H,
+                                               TO* __restrict__ vol, GeomDev g,
+                                               const cplx<T>* __restrict__ tw) {
+    constexpr int NB = lds_batch(NX, sizeof(cplx<T>));
+    __shared__ cplx<T> sm[2 * NB * NX];
+    cplx<T>* a = sm;
+    cplx<T>* b = sm + NB * NX;
+    const int ny = g.ny, W = NX / 2 + 1;
+    const int k = blockIdx.y;
+    const int p0 = blockIdx.x * NB;  // first row pair
+    const int np = min(NB, ny / 2 - p0);
+    for (int t = threadIdx.x; t < np * NX; t += blockDim.x) {
+        const int pp = t / NX, w = t - pp * NX;
+        const int j = 2 * (p0 + pp);
+        const cplx<T>* r1 = H + ((size_t)k * ny + j) * W;
+        const cplx<T>* r2 = r1 + W;
+        cplx<T> x1, x2;
+        if (w <= NX / 2) {
+            x1 = r1[w];
+            x2 = r2[w];
+        } else {
+            x1 = r1[NX - w];
+            x2 = r2[NX - w];
+            x1.y = -x1.y;
+            x2.y = -x2.y;
+        }
+        a[t] = {x1.x - x2.y, x1.y + x2.x};
+    }
+    __syncthreads();
+    fft_lds<T, NX>(a, b, np, tw, +1, threadIdx.x, blockDim.x);
+    const double invN = 1.0 / ((double)NX * ny * g.nz);
+    const int gs = par_sign((g.nz + ny) / 2);
+    for (int t = threadIdx.x; t < np * NX; t += blockDim.x) {
+        const int pp = t / NX, m = t - pp * NX;
+        const int j = 2 * (p0 + pp);
+        const cplx<T> z = a[t];
+        for (int h = 0; h < 2; h++) {
+            const int jj = j + h;
+            const T val = h ? z.y : z.x;
+            long ii2 = (long)(m - g.ic) * (m - g.ic);
+            if (g.mask_j) ii2 += (long)(jj - g.jc) * (jj - g.jc);
+            if (g.mask_k) ii2 += (long)(k - g.kc) * (k - g.kc);
+            TO o;
+            if (ii2 > g.nR2) {
+                o = TO(0);
+            } else {
+                const double f = exp(g.mask_dtemp * (double)ii2) * g.fnorm * invN *
+                                 (double)(par_sign(k + jj) * gs);
+                o = TO((double)val * f);
+            }
+            vol[((size_t)k * ny + jj) * NX + m] = o;
+        }
+    }
+}
+
+}  // namespace utr

@@ -1,0 +1,124 @@
+"""Seeded synthetic UTR inputs (ctypes over _lib/libutr_synth.so).
+
+Atom sets and projection images of the shape and statistics the UTR
+configurations describe (flat-field corrected intensities around 1, optional
+Poisson noise), standing in for MsctKirkland output.  Atom rotation follows
+TemsimLibrary/autosliccmd.cpp:734-748.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+SYNTH_PATH = os.path.join(_PKG, "_lib", "libutr_synth.so")
+_lib = None
+
+# species of the 3000-atom configuration (C:N:O:S = 62:17:19:2)
+CNOS = ([6, 7, 8, 16], [0.62, 0.17, 0.19, 0.02])
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(SYNTH_PATH):
+            raise RuntimeError("libutr_synth.so is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(SYNTH_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.utr_synth_atoms.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                      ctypes.c_double, ip, dp, ctypes.c_int, dp, ip]
+        L.utr_synth_project.argtypes = [dp, ip, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_double, ctypes.c_int, dp, dp, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_double, ctypes.c_uint64,
+                                        ctypes.c_void_p, ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def atoms(seed, natoms, shape="sphere", size=80.0, min_sep=0.0, species=CNOS):
+    """Returns (xyz[natoms,3] in UL about the volume centre, Z[natoms])."""
+    L = _load()
+    zs = np.ascontiguousarray(species[0], dtype=np.int32)
+    fr = np.ascontiguousarray(species[1], dtype=np.float64)
+    xyz = np.zeros((natoms, 3))
+    Z = np.zeros(natoms, dtype=np.int32)
+    rc = L.utr_synth_atoms(seed, natoms, 0 if shape == "cube" else 1, size, min_sep,
+                           zs.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                           fr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(zs),
+                           xyz.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                           Z.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    if rc:
+        raise RuntimeError("utr_synth_atoms failed (%d)" % rc)
+    return xyz, Z
+
+
+def project(xyz, Z, nx, ny, xst, angZ, angY, width=0.5, contrast=0.05, dose=0.0, seed=1,
+            nthreads=None):
+    """Projection images [nrows, ny, nx] float32 (I = exp(-contrast*sum))."""
+    L = _load()
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    Z = np.ascontiguousarray(Z, dtype=np.int32)
+    angZ = np.ascontiguousarray(angZ, dtype=np.float64)
+    angY = np.ascontiguousarray(angY, dtype=np.float64)
+    n = len(angZ)
+    out = np.empty((n, ny, nx), dtype=np.float32)
+    if nthreads is None:
+        nthreads = min(16, os.cpu_count() or 1)
+    rc = L.utr_synth_project(xyz.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                             Z.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), len(Z), nx, ny, xst, n,
+                             angZ.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                             angY.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), width, contrast,
+                             dose, seed, out.ctypes.data, nthreads)
+    if rc:
+        raise RuntimeError("utr_synth_project failed (%d)" % rc)
+    return out
+
+
+# --------------------------------------------------------------- configs
+
+def config(name, seed=1):
+    """(params, rows, atoms-spec) of a named workload.
+
+    metric : 256^2 images, 360 rows about y (1 deg steps), 200 nm defocus,
+             256^3 volume, mode 0 with Ewald correction, NSR 1/6.
+    c1     : 128^3 from 36 rows about y over [0,180), 50 nm defocus.
+    c2     : metric geometry with 720 rows (defocus 0 and 2000 A per angle).
+    c3     : 512^3 from 720 rows about y over [0,360).
+    """
+    from . import default_params
+    if name == "c1":
+        p = default_params(nx=128, ny=128, xst=0.4, zlo=-25.6, zhi=25.6, wl=0.02507934,
+                           escc=1, nsr=0.0, ctf_mode=0)
+        angY = np.arange(36) * 5.0
+        rows = dict(angZ=np.zeros(36), angY=angY, defocus=np.full(36, 500.0))
+        at = dict(natoms=400, shape="cube", size=40.0, min_sep=1.4, species=([6], [1.0]))
+    elif name in ("metric", "c2"):
+        p = default_params(nx=256, ny=256, xst=0.6, zlo=-76.8, zhi=76.8, wl=0.01968749,
+                           escc=1, nsr=1.0 / 6.0, ctf_mode=0)
+        if name == "metric":
+            angY = np.arange(360) * 1.0
+            rows = dict(angZ=np.zeros(360), angY=angY, defocus=np.full(360, 2000.0))
+        else:
+            angY = np.repeat(np.arange(360) * 1.0, 2)
+            df = np.tile([0.0, 2000.0], 360)
+            rows = dict(angZ=np.zeros(720), angY=angY, defocus=df)
+        at = dict(natoms=3000, shape="sphere", size=80.0, min_sep=0.0, species=CNOS)
+    elif name == "c3":
+        p = default_params(nx=512, ny=512, xst=0.6, zlo=-153.6, zhi=153.6, wl=0.01968749,
+                           escc=1, nsr=1.0 / 6.0, ctf_mode=0)
+        angY = np.arange(720) * 0.5
+        rows = dict(angZ=np.zeros(720), angY=angY, defocus=np.full(720, 2000.0))
+        at = dict(natoms=12000, shape="sphere", size=160.0, min_sep=0.0, species=CNOS)
+    else:
+        raise KeyError(name)
+    return p, rows, at
+
+
+def images_for(params, rows, at, seed=1, dose=None):
+    """Synthetic images for a config: dose defaults to (1/nsr)^2 counts/px."""
+    xyz, Z = atoms(seed, at["natoms"], at["shape"], at["size"], at["min_sep"], at["species"])
+    if dose is None:
+        dose = (1.0 / params["nsr"]) ** 2 if params["nsr"] > 0 else 0.0
+    return project(xyz, Z, params["nx"], params["ny"], params["xst"], rows["angZ"], rows["angY"],
+                   dose=dose, seed=seed)
