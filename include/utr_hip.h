@@ -34,8 +34,9 @@ extern "C" {
 #define UTR_E_UNSUPPORTED (-5)
 
 /* precision forms */
-#define UTR_FP32 0  /* batched FP32 path: gather gridding with compensated sums */
-#define UTR_FP64 1  /* plain FP64 path: per-row launches, scatter gridding      */
+#define UTR_FP32 0         /* batched FP32 path: brick scatter, exact fixed-point sums  */
+#define UTR_FP64 1         /* plain FP64 path: per-row launches, atomic scatter         */
+#define UTR_FP32_GATHER 2  /* batched FP32 path: per-voxel gather, Kahan-compensated    */
 
 /*
  * Numeric parameters of the UTR parameter file (ReadmeUTR.txt:33-165, read at

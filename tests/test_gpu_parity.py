@@ -20,7 +20,10 @@ FP64_TOL = 1e-10
 
 
 def rel(a, b):
-    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+    dt = np.complex128 if np.iscomplexobj(a) or np.iscomplexobj(b) else np.float64
+    a = np.asarray(a, dt)
+    b = np.asarray(b, dt)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
 def case(n=64, nrows=24, seed=1, escc=1, nsr=0.05, natoms=80, general=False, **kw):
@@ -35,6 +38,7 @@ def case(n=64, nrows=24, seed=1, escc=1, nsr=0.05, natoms=80, general=False, **k
         angZ = np.zeros(nrows)
         angY = np.arange(nrows) * (180.0 / nrows)
     rows = dict(angZ=angZ, angY=angY, defocus=np.full(nrows, 300.0))
+    natoms = min(natoms, max(4, n // 2))
     xyz, Z = synth.atoms(seed, natoms, "cube", n * xst * 0.4, 1.4, ([6], [1.0]))
     dose = (1.0 / nsr) ** 2 if nsr > 0 else 0.0
     imgs = synth.project(xyz, Z, n, n, xst, angZ, angY, dose=dose, seed=seed)
@@ -42,11 +46,18 @@ def case(n=64, nrows=24, seed=1, escc=1, nsr=0.05, natoms=80, general=False, **k
 
 
 def run_both(p, rows, imgs):
+    """Oracle, the FP32 forms and the FP64 form on the same inputs; the
+    two FP32 forms are checked against the oracle here, the caller checks
+    the default FP32 result and the FP64 result."""
     print("case: %s rows=%d" % ({k: p[k] for k in ("nx", "ny", "ctf_mode", "escc", "nsr")},
                                 len(rows["angY"])), flush=True)
-    ref = oracle.reconstruct(p, rows, imgs)
+    ref = oracle.reconstruct(p, rows, imgs, weights=rows.get("weights"))
     v32 = U.reconstruct(p, rows, imgs, U.UTR_FP32)
+    vg = U.reconstruct(p, rows, imgs, U.UTR_FP32_GATHER)
+    assert rel(vg, ref) < FP32_TOL, ("gather form", rel(vg, ref))
     v64 = U.reconstruct(p, rows, imgs, U.UTR_FP64, f64_out=True)
+    print("  rel-L2 vs oracle: fp32 %.2e  fp32-gather %.2e  fp64 %.2e"
+          % (rel(v32, ref), rel(vg, ref), rel(v64, ref)), flush=True)
     return ref, v32, v64
 
 
@@ -92,9 +103,7 @@ def test_reconstruct_filter_modes(mode, extra):
 def test_reconstruct_centre_shift_and_weights():
     p, rows, imgs = case(n=48, nrows=20, seed=6, dxc=0.8, dyc=-0.4, dzc=0.6)
     rows["weights"] = np.linspace(0.5, 1.5, 20).astype(np.float32)
-    ref = oracle.reconstruct(p, rows, imgs, weights=rows["weights"])
-    v32 = U.reconstruct(p, rows, imgs, U.UTR_FP32)
-    v64 = U.reconstruct(p, rows, imgs, U.UTR_FP64, f64_out=True)
+    ref, v32, v64 = run_both(p, rows, imgs)
     assert rel(v64, ref) < FP64_TOL
     assert rel(v32, ref) < FP32_TOL
 
@@ -121,10 +130,11 @@ def test_c1_config_128():
     assert np.abs(v32 - ref).max() < 1e-4 * np.abs(ref).max()
 
 
-def test_sampling_matrix_matches_oracle():
+@pytest.mark.parametrize("form", [0, 2])
+def test_sampling_matrix_matches_oracle(form):
     p, rows, imgs = case(n=48, nrows=16, seed=9)
     _, S = oracle.reconstruct(p, rows, imgs, samp=True)
-    with U.Reconstructor(p, rows, U.UTR_FP32) as r:
+    with U.Reconstructor(p, rows, form) as r:
         r.keep_sampling()
         r.add_images(imgs)
         r.finalize()
@@ -133,9 +143,35 @@ def test_sampling_matrix_matches_oracle():
     assert np.array_equal(S32 == 0, S == 0)
 
 
-def test_fp32_deterministic_and_reset_reuse():
-    p, rows, imgs = case(n=48, nrows=16, seed=10)
+def test_axis_aligned_rows_tiny_weights():
+    # rows at 90/180/270 degrees (cos 90 deg = 6e-17) leave weights of order
+    # 1e-15 on some voxels, a row at 89.999999 degrees weights of order 1e-8;
+    # a voxel reached only by those must still count as sampled (Samp3 != 0)
+    # and keep V3/Samp3 accurate, as in the reference
+    p, rows, imgs = case(n=48, nrows=5, seed=14, nsr=0.0, escc=0)
+    rows = dict(angZ=np.zeros(5), angY=np.array([0.0, 90.0, 180.0, 270.0, 89.999999]),
+                defocus=np.full(5, 300.0))
+    _, S = oracle.reconstruct(p, rows, imgs, samp=True)
+    assert ((S > 0) & (S < 1e-12)).sum() > 100
+    assert ((S > 1e-9) & (S < 1e-6)).sum() > 100
+    ref, v32, v64 = run_both(p, rows, imgs)
     with U.Reconstructor(p, rows, U.UTR_FP32) as r:
+        r.keep_sampling()
+        r.add_images(imgs)
+        r.finalize()
+        S32 = r.sampling()
+    assert np.array_equal(S32 == 0, S == 0)
+    assert rel(v64, ref) < FP64_TOL
+    assert rel(v32, ref) < FP32_TOL
+
+
+@pytest.mark.parametrize("form", [0, 2])
+def test_fp32_reproducible_and_reset_reuse(form):
+    """Both FP32 forms are deterministic: the gather form sums in a fixed
+    order, the brick scatter form in exact fixed point whatever the order of
+    its atomics.  Rows added out of order in two calls give the same bits."""
+    p, rows, imgs = case(n=48, nrows=16, seed=10)
+    with U.Reconstructor(p, rows, form) as r:
         r.add_images(imgs)
         a = r.finalize()
         r.reset()

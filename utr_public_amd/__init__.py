@@ -15,8 +15,9 @@ _PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_PKG, "_lib", "libutr_hip.so")
 SYNTH_PATH = os.path.join(_PKG, "_lib", "libutr_synth.so")
 
-UTR_FP32 = 0
-UTR_FP64 = 1
+UTR_FP32 = 0          # brick scatter, exact fixed-point sums (default)
+UTR_FP64 = 1          # plain FP64 form
+UTR_FP32_GATHER = 2   # per-voxel gather, Kahan-compensated FP32 sums
 
 PARAM_FIELDS = [
     ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("modality", ctypes.c_int),
@@ -80,6 +81,14 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError("libutr_hip.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+    # torch ships its own HIP runtime under the same soname (libamdhip64.so.7).
+    # Loading torch first makes this library bind to that already-loaded
+    # runtime, so torch tensors and this library share one HIP runtime in the
+    # process; loaded the other way round, torch would start a second runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     vp = ctypes.c_void_p
     dp = ctypes.POINTER(ctypes.c_double)

@@ -2,13 +2,17 @@
 //
 // Replaces the FFTW plans and the OpenMP orientation loop of
 // MultisliceCppTest/PhaseRetrieval/UTR.cpp:840-1279 (see include/utr_hip.h).
-// Two forms share the FFT and epilogue kernels:
-//   UTR_FP32  rows are transformed in batches as they arrive; finalize runs
-//             the gather gridding over all rows (compensated FP32 sums,
-//             deterministic), the fused normalisation/filters and the 3D c2r.
-//   UTR_FP64  the plain form: one row at a time, scatter gridding with FP64
-//             atomics, then separate normalisation, symmetrisation and c2r
-//             passes (the structure of the reference's loop, in double).
+// Three forms share the FFT and epilogue kernels:
+//   UTR_FP32         rows are transformed in batches as they arrive; finalize
+//                    runs the brick scatter (exact fixed-point sums in LDS,
+//                    deterministic), the fused normalisation/filters and the
+//                    3D c2r.
+//   UTR_FP32_GATHER  as UTR_FP32 with a per-voxel gather and Kahan-compensated
+//                    FP32 sums instead of the brick scatter.
+//   UTR_FP64         the plain form: one row at a time, scatter gridding with
+//                    FP64 atomics, then separate normalisation, symmetrisation
+//                    and c2r passes (the structure of the reference's loop, in
+//                    double).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -126,6 +130,7 @@ struct utr_ctx {
     // FP32 form
     DevBuf<cplx<float>> spec32, V32, H32, twx32, twy32, twz32;
     DevBuf<float> S32, vol32;
+    DevBuf<unsigned> rowmax;  // per-row max |K| as float bits (FP32 forms)
     // FP64 form
     DevBuf<cplx<double>> spec64, V64, H64, twx64, twy64, twz64;
     DevBuf<double> S64, vol64;
@@ -173,7 +178,8 @@ int launch_2d(utr_ctx* c, const float* d_img, int row0, int count, cplx<T>* d_sp
     {                                                                                           \
         constexpr int NB = utr::lds_batch(N, sizeof(cplx<T>));                                  \
         dim3 grid((g.nx + NB - 1) / NB, count);                                                 \
-        utr::k_cols<T, N><<<grid, 256, 0, c->stream>>>(d_spec, g.nx, g, c->rows.p, row0, twy);  \
+        utr::k_cols<T, N><<<grid, 256, 0, c->stream>>>(d_spec, g.nx, g, c->rows.p, row0, twy,   \
+                                                        c->rowmax.p);                           \
         ok = true;                                                                              \
     }
     UTR_DISPATCH_N(g.ny, UTR_COLS)
@@ -230,7 +236,7 @@ int add_rows(utr_ctx* c, int first_row, int count, const float* d_img) {
     const auto& g = c->plan.g;
     hipEvent_t e0 = c->ev[6], e1 = c->ev[7];
     UTR_HIP_TRY(hipEventRecord(e0, c->stream));
-    if (c->precision == UTR_FP32) {
+    if (c->precision != UTR_FP64) {
         int rc = launch_2d<float>(c, d_img, first_row, count,
                                   c->spec32.p + (size_t)first_row * g.ny * g.nx, c->twx32.p,
                                   c->twy32.p);
@@ -275,14 +281,22 @@ int do_finalize(utr_ctx* c, float* d_vol_out, double* d_vol64_out) {
                                         " orientation rows have no image");
     const size_t N = vox(c);
     UTR_HIP_TRY(hipEventRecord(c->ev[0], c->stream));
-    if (c->precision == UTR_FP32) {
+    if (c->precision != UTR_FP64) {
         if (c->keep_sampling) UTR_HIP_TRY(c->S32.alloc(N));
-        const int nbx = (g.nx + utr::GBX - 1) / utr::GBX, nby = (g.ny + utr::GBY - 1) / utr::GBY,
-                  nbz = (g.nz + utr::GBZ - 1) / utr::GBZ;
-        const long nblocks = (long)nbx * nby * nbz;
-        if (nblocks <= 0 || nblocks > 0x7fffffffL) return set_err(UTR_E_INVALID, "volume too large");
-        utr::k_gather<<<(unsigned)nblocks, 256, 0, c->stream>>>(
-            c->spec32.p, c->rows.p, c->plan.nrows, g, c->V32.p, c->keep_sampling ? c->S32.p : nullptr);
+        float* d_S = c->keep_sampling ? c->S32.p : nullptr;
+        if (c->precision == UTR_FP32) {
+            const long nblocks = (long)((g.nx + utr::SBX - 1) / utr::SBX) *
+                                 ((g.ny + utr::SBY - 1) / utr::SBY) * ((g.nz + utr::SBZ - 1) / utr::SBZ);
+            if (nblocks <= 0 || nblocks > 0x7fffffffL) return set_err(UTR_E_INVALID, "volume too large");
+            utr::k_grid_fx<<<(unsigned)nblocks, 256, 0, c->stream>>>(
+                c->spec32.p, c->rows.p, c->plan.nrows, c->rowmax.p, g, c->V32.p, d_S);
+        } else {
+            const long nblocks = (long)((g.nx + utr::GBX - 1) / utr::GBX) *
+                                 ((g.ny + utr::GBY - 1) / utr::GBY) * ((g.nz + utr::GBZ - 1) / utr::GBZ);
+            if (nblocks <= 0 || nblocks > 0x7fffffffL) return set_err(UTR_E_INVALID, "volume too large");
+            utr::k_gather<<<(unsigned)nblocks, 256, 0, c->stream>>>(c->spec32.p, c->rows.p,
+                                                                   c->plan.nrows, g, c->V32.p, d_S);
+        }
         UTR_HIP_TRY(hipGetLastError());
         UTR_HIP_TRY(hipEventRecord(c->ev[1], c->stream));
         int rc = launch_3d<float, float>(c, c->V32.p, c->H32.p, d_vol_out, c->twx32.p, c->twy32.p,
@@ -339,8 +353,8 @@ int utr_create(const utr_params* params, int nrows, const double* angZ_deg, cons
                utr_ctx** out) {
     if (!out) return set_err(UTR_E_INVALID, "null output pointer");
     *out = nullptr;
-    if (precision != UTR_FP32 && precision != UTR_FP64)
-        return set_err(UTR_E_INVALID, "precision must be UTR_FP32 or UTR_FP64");
+    if (precision != UTR_FP32 && precision != UTR_FP64 && precision != UTR_FP32_GATHER)
+        return set_err(UTR_E_INVALID, "precision must be UTR_FP32, UTR_FP64 or UTR_FP32_GATHER");
     utr_ctx* c = new utr_ctx();
     std::string err;
     if (!utr::build_plan(params, nrows, angZ_deg, angY_deg, angZ2_deg, defocus, weights, c->plan, err)) {
@@ -372,11 +386,13 @@ int utr_create(const utr_params* params, int nrows, const double* angZ_deg, cons
     const size_t N = vox(c), n2 = (size_t)g.nx * g.ny;
     const size_t nh = (size_t)g.nz * g.ny * (g.nx / 2 + 1);
     int rc;
-    if (precision == UTR_FP32) {
+    if (precision != UTR_FP64) {
         if ((rc = upload_twiddles<float>(c, c->twx32, c->twy32, c->twz32))) return fail(rc);
         if (c->spec32.alloc((size_t)nrows * n2) != hipSuccess || c->V32.alloc(N) != hipSuccess ||
-            c->H32.alloc(nh) != hipSuccess)
+            c->H32.alloc(nh) != hipSuccess || c->rowmax.alloc(nrows) != hipSuccess)
             return fail(set_err(UTR_E_NOMEM, "device allocation of the FP32 buffers failed"));
+        if (hipMemset(c->rowmax.p, 0, sizeof(unsigned) * nrows) != hipSuccess)
+            return fail(set_err(UTR_E_DEVICE, "hipMemset failed"));
     } else {
         if ((rc = upload_twiddles<double>(c, c->twx64, c->twy64, c->twz64))) return fail(rc);
         if (c->spec64.alloc(n2) != hipSuccess || c->V64.alloc(N) != hipSuccess ||
@@ -453,7 +469,7 @@ int utr_get_sampling(utr_ctx* c, float* samp) {
     if (!c || !samp) return set_err(UTR_E_INVALID, "null argument");
     if (!c->finalized) return set_err(UTR_E_STATE, "utr_get_sampling before utr_finalize");
     const size_t N = vox(c);
-    if (c->precision == UTR_FP32) {
+    if (c->precision != UTR_FP64) {
         if (!c->keep_sampling || !c->S32.p)
             return set_err(UTR_E_STATE, "sampling matrix not kept (call utr_keep_sampling before utr_finalize)");
         UTR_HIP_TRY(hipMemcpy(samp, c->S32.p, sizeof(float) * N, hipMemcpyDeviceToHost));
@@ -468,7 +484,7 @@ int utr_get_sampling(utr_ctx* c, float* samp) {
 int utr_get_spectrum(utr_ctx* c, int row, float* spec) {
     if (!c || !spec) return set_err(UTR_E_INVALID, "null argument");
     if (row < 0 || row >= c->plan.nrows) return set_err(UTR_E_INVALID, "row outside the table");
-    if (c->precision != UTR_FP32) return set_err(UTR_E_INVALID, "utr_get_spectrum needs a UTR_FP32 context");
+    if (c->precision == UTR_FP64) return set_err(UTR_E_INVALID, "utr_get_spectrum needs an FP32 context");
     if (!c->added[row]) return set_err(UTR_E_STATE, "row has no image yet");
     const size_t n2 = (size_t)c->plan.g.nx * c->plan.g.ny;
     UTR_HIP_TRY(hipStreamSynchronize(c->stream));
@@ -481,7 +497,7 @@ int utr_get_fourier_volume(utr_ctx* c, float* out) {
     if (!c || !out) return set_err(UTR_E_INVALID, "null argument");
     if (!c->finalized) return set_err(UTR_E_STATE, "utr_get_fourier_volume before utr_finalize");
     const size_t N = vox(c);
-    if (c->precision == UTR_FP32) {
+    if (c->precision != UTR_FP64) {
         UTR_HIP_TRY(hipMemcpy(out, c->V32.p, sizeof(cplx<float>) * N, hipMemcpyDeviceToHost));
     } else {
         std::vector<cplx<double>> tmp(N);
@@ -504,6 +520,8 @@ int utr_reset(utr_ctx* c) {
         UTR_HIP_TRY(hipMemsetAsync(c->V64.p, 0, sizeof(cplx<double>) * vox(c), c->stream));
         UTR_HIP_TRY(hipMemsetAsync(c->S64.p, 0, sizeof(double) * vox(c), c->stream));
         UTR_HIP_TRY(hipStreamSynchronize(c->stream));
+    } else {
+        UTR_HIP_TRY(hipMemsetAsync(c->rowmax.p, 0, sizeof(unsigned) * c->plan.nrows, c->stream));
     }
     return UTR_OK;
 }
@@ -528,7 +546,7 @@ int utr_destroy(utr_ctx* c) {
     c->rows.release();
     c->img_stage.release();
     c->spec32.release(); c->V32.release(); c->H32.release(); c->S32.release(); c->vol32.release();
-    c->twx32.release(); c->twy32.release(); c->twz32.release();
+    c->twx32.release(); c->twy32.release(); c->twz32.release(); c->rowmax.release();
     c->spec64.release(); c->V64.release(); c->H64.release(); c->S64.release(); c->vol64.release();
     c->twx64.release(); c->twy64.release(); c->twz64.release();
     for (auto& e : c->ev)

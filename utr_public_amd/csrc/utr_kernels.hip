@@ -76,10 +76,14 @@ __global__ void __launch_bounds__(256) k_rows(const float* __restrict__ img,
 
 // Columns: forward FFT along y, output Shuffle as (-1)^(kx+ky+nx/2+ny/2),
 // optional 2D filter (modes 1/3, restated) and subseries weight.
+// rowmax (optional): per-row max(|re|, |im|) of the result as float bits
+// (non-negative floats order like their bit patterns); it sets the
+// fixed-point scale of the brick gridding.
 template <typename T, int N>
 __global__ void __launch_bounds__(256) k_cols(cplx<T>* __restrict__ spec, int nx, GeomDev g,
                                               const RowParams* __restrict__ rows, int row0,
-                                              const cplx<T>* __restrict__ tw) {
+                                              const cplx<T>* __restrict__ tw,
+                                              unsigned* __restrict__ rowmax) {
     constexpr int NB = lds_batch(N, sizeof(cplx<T>));
     __shared__ cplx<T> sm[2 * NB * N];
     cplx<T>* a = sm;
@@ -99,6 +103,7 @@ __global__ void __launch_bounds__(256) k_cols(cplx<T>* __restrict__ spec, int nx
     const int gsgn = par_sign(g.nxd2 + g.nyd2);
     const double xap = g.xhi - g.xlo, yap = g.yhi - g.ylo;
     const double pi = 3.14159265358979323846;
+    float vmax = 0.f;
     for (int t = threadIdx.x; t < nb * N; t += blockDim.x) {
         const int ky = t / nb, c = t - ky * nb;
         const int kx = x0 + c;
@@ -142,6 +147,11 @@ __global__ void __launch_bounds__(256) k_cols(cplx<T>* __restrict__ spec, int nx
             v.y *= T(R.weight);
         }
         base[(size_t)ky * nx + c] = v;
+        vmax = fmaxf(vmax, fmaxf(fabsf((float)v.x), fabsf((float)v.y)));
+    }
+    if (rowmax) {
+        for (int off = 32; off > 0; off >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, off));
+        if ((threadIdx.x & 63) == 0) atomicMax(&rowmax[row0 + row], __float_as_uint(vmax));
     }
 }
 
@@ -403,6 +413,287 @@ __global__ void __launch_bounds__(256) k_gather(const cplx<float>* __restrict__ 
     const size_t vidx = ((size_t)vz * g.ny + vy) * g.nx + vx;
     if (Sout) Sout[vidx] = ss;
     Vout[vidx] = voxel_epilogue<float>(g, vx, vy, vz, cplx<float>{sr, si}, ss);
+}
+
+// --------------------- FP32 scatter into bricks, exact fixed-point sums
+//
+// One workgroup owns a brick of SBX x SBY x SBZ voxels.  It lists the rows
+// whose Ewald sheet can reach the brick (in row order) and, for each listed
+// row, the detector lines b whose points can splat into it.  Lines are
+// processed 256 at a time, one per thread: the thread bounds the a-interval
+// of its line that can land in the brick, evaluates each point's grid
+// position in FP64 (the same expressions as the CPU restatement), and adds
+// w*K (FP32 products) for the point's corners inside the brick.
+//
+// The sums are kept in LDS as 64-bit fixed-point integers, added with integer
+// LDS atomics (on gfx950 these cost no more than plain stores, while float
+// LDS atomics are several times slower).  Integer addition is associative, so
+// the sums are exact, independent of the order in which points arrive, and
+// the result is deterministic; the only rounding is that of each w*K product
+// and of its conversion to the fixed-point grid.  The scales come from the
+// largest spectrum component and the row count so that no sum can overflow:
+// V3 is resolved to about 1e-11 (in units of K), Samp3 to about 1e-15.
+// Weights below 2^-20 (rare: cells crossed within a millionth of a grid
+// plane, mostly at exact grid alignments) go instead to float accumulators,
+// so a voxel reached only by such weights keeps the relative precision that
+// V3/Samp3 needs there and counts as sampled exactly as in the reference.
+// Corners of weight exactly zero are skipped (adding them changes nothing).
+//
+// The fixed-point accumulators use odd strides (17 along y, 153 along z) so
+// that lanes walking along x, y or z, as the points of a detector line do for
+// any orientation, address distinct LDS banks.
+
+constexpr int SBX = 16, SBY = 8, SBZ = 8;
+constexpr int SVOX = SBX * SBY * SBZ;
+constexpr int SOWN = SVOX / 256;  // voxels finalised per thread
+static_assert(SVOX % 256 == 0, "brick must be a multiple of the workgroup");
+constexpr int SPY = SBX + 1;           // LDS stride along y
+constexpr int SPZ = SPY * (SBY + 1);   // LDS stride along z
+constexpr int SPAD = SPZ * SBZ;        // LDS entries per component
+constexpr int S_LIST = 256;            // rows tested per chunk
+constexpr float S_TINY_W = 9.5367431640625e-07f;  // 2^-20
+
+__device__ __forceinline__ int lds_idx(int lx, int ly, int lz) { return lx + SPY * ly + SPZ * lz; }
+
+// round(d) as a 64-bit integer for |d| < 2^51 (1.5 * 2^52 aligns the units
+// digit with the last mantissa bit; round to nearest even)
+__device__ __forceinline__ long long dround_i64(double d) {
+    const double y = d + 6755399441055744.0;
+    return (long long)(__double_as_longlong(y) - 0x4338000000000000LL);
+}
+
+__global__ void __launch_bounds__(256) k_grid_fx(const cplx<float>* __restrict__ spec,
+                                                 const RowParams* __restrict__ rows, int nrows,
+                                                 const unsigned* __restrict__ rowmax, GeomDev g,
+                                                 cplx<float>* __restrict__ Vout,
+                                                 float* __restrict__ Sout) {
+    __shared__ unsigned long long acc[3 * SPAD];
+    __shared__ float tiny[3 * SVOX];
+    __shared__ float s_red[4];
+    __shared__ int s_list[S_LIST];     // rows reaching the brick, in row order
+    __shared__ int s_lo[S_LIST];       // first detector line of each listed row
+    __shared__ int s_pre[S_LIST + 1];  // line counts, then their exclusive prefix
+    __shared__ int s_cnt[4], s_wsum[4];
+    __shared__ int s_n;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nbx = (g.nx + SBX - 1) / SBX, nby = (g.ny + SBY - 1) / SBY;
+    const int bid = blockIdx.x;
+    const int bx = bid % nbx, by = (bid / nbx) % nby, bz = bid / (nbx * nby);
+    const int x0 = bx * SBX, y0 = by * SBY, z0 = bz * SBZ;
+    for (int t = tid; t < 3 * SPAD; t += 256) acc[t] = 0ull;
+    for (int t = tid; t < 3 * SVOX; t += 256) tiny[t] = 0.f;
+    // fixed-point scales: 2^sv for V3, 2^sw for Samp3.  Per row a voxel
+    // receives a total weight below 8, so |sum| < 8 nrows max|K| 2^sv < 2^62;
+    // each term stays below 2^51 for the rounding above.
+    float mk = 0.f;
+    for (int r = tid; r < nrows; r += 256) mk = fmaxf(mk, __uint_as_float(rowmax[r]));
+    for (int off = 32; off > 0; off >>= 1) mk = fmaxf(mk, __shfl_xor(mk, off));
+    if (lane == 0) s_red[wave] = mk;
+    __syncthreads();
+    mk = fmaxf(fmaxf(s_red[0], s_red[1]), fmaxf(s_red[2], s_red[3]));
+    int ek = 0, ea = 0;
+    frexpf(mk > 0.f ? mk : 1.f, &ek);       // max|K| < 2^ek
+    frexpf((float)(8 * nrows), &ea);         // 8 nrows < 2^ea
+    const int sv = min(61 - ek - ea, 50 - ek);
+    const int sw = min(61 - ea, 50);
+    const double vscale = ldexp(1.0, sv), wscale = ldexp(1.0, sw);
+
+    const float fx2 = (float)(g.fxst * g.fxst), fy2 = (float)(g.fyst * g.fyst);
+    const float wl2f = (float)g.wl2;
+    const float bcx = x0 + 0.5f * (SBX - 1) - g.nxd2;
+    const float bcy = y0 + 0.5f * (SBY - 1) - g.nyd2;
+    const float bcz = z0 + 0.5f * (SBZ - 1) - g.nzd2;
+    const float ex = 0.5f * (SBX - 1) + 1.f, ey = 0.5f * (SBY - 1) + 1.f, ez = 0.5f * (SBZ - 1) + 1.f;
+    const float bmin[3] = {(float)x0, (float)y0, (float)z0};
+    const float bmax[3] = {(float)(x0 + SBX - 1), (float)(y0 + SBY - 1), (float)(z0 + SBZ - 1)};
+    const float cgrid[3] = {(float)g.nxd2, (float)g.nyd2, (float)g.nzd2};
+
+    for (int base = 0; base < nrows; base += S_LIST) {
+        const int nchunk = min(S_LIST, nrows - base);
+        if (tid == 0) s_n = 0;
+        __syncthreads();
+        // rows whose sheet can reach the brick, with their detector lines
+        for (int r0 = 0; r0 < nchunk; r0 += 256) {
+            const int r = r0 + tid;
+            bool pass = false;
+            int blo = 0, nl = 0;
+            if (r < nchunk) {
+                const RowParams& R = rows[base + r];
+                pass = brick_touch(g, R, bcx, bcy, bcz, ex, ey, ez, fx2, fy2, wl2f);
+                if (pass) {
+                    int bhi;
+                    if (R.y_exact) {
+                        // y grid coordinate is ny/2 + b exactly: lines outside the
+                        // brick's y-range only add weight 0 to it
+                        blo = max(y0 - g.nyd2, -g.nyd2);
+                        bhi = min(y0 + SBY - 1 - g.nyd2, g.ny - 1 - g.nyd2);
+                    } else {
+                        const float* mi = R.minv;
+                        const float t = mi[3] * bcx + mi[4] * bcy + mi[5] * bcz;
+                        const float Wb = fabsf(mi[3]) * ex + fabsf(mi[4]) * ey + fabsf(mi[5]) * ez + 0.01f;
+                        blo = max((int)ceilf(t - Wb), -g.nyd2);
+                        bhi = min((int)floorf(t + Wb), g.ny - 1 - g.nyd2);
+                    }
+                    nl = bhi - blo + 1;
+                    pass = nl > 0;
+                }
+            }
+            const unsigned long long mask = __ballot(pass);
+            const int pre = __popcll(mask & ((1ull << lane) - 1ull));
+            if (lane == 0) s_cnt[wave] = __popcll(mask);
+            __syncthreads();
+            int off = s_n;
+            for (int w = 0; w < wave; w++) off += s_cnt[w];
+            if (pass) {
+                s_list[off + pre] = base + r;
+                s_lo[off + pre] = blo;
+                s_pre[off + pre] = nl;
+            }
+            __syncthreads();
+            if (tid == 0) s_n += s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+            __syncthreads();
+        }
+        const int n = s_n;
+        // exclusive prefix of the line counts (two entries per thread)
+        {
+            const int c0 = (2 * tid < n) ? s_pre[2 * tid] : 0;
+            const int c1 = (2 * tid + 1 < n) ? s_pre[2 * tid + 1] : 0;
+            const int pair = c0 + c1;
+            int incl = pair;
+            for (int o = 1; o < 64; o <<= 1) {
+                const int v = __shfl_up(incl, o);
+                if (lane >= o) incl += v;
+            }
+            if (lane == 63) s_wsum[wave] = incl;
+            __syncthreads();
+            int woff = 0;
+            for (int w = 0; w < wave; w++) woff += s_wsum[w];
+            const int excl = woff + incl - pair;
+            if (2 * tid < n) s_pre[2 * tid] = excl;
+            if (2 * tid + 1 < n) s_pre[2 * tid + 1] = excl + c0;
+            if (tid == 0) s_pre[n] = s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
+            __syncthreads();
+        }
+        const int total = s_pre[n];
+        for (int G = 0; G < total; G += 256) {
+            const int gl = G + tid;
+            if (gl < total) {
+                // listed row q holding global line gl: last q with s_pre[q] <= gl
+                int lo = 0, hi = n - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (s_pre[mid] <= gl) lo = mid;
+                    else hi = mid - 1;
+                }
+                const int r = s_list[lo];
+                const int b = s_lo[lo] + (gl - s_pre[lo]);
+                const RowParams& R = rows[r];
+                const float* mi = R.minv;
+                // a-interval of this line whose points can land in the brick
+                const float s = mi[0] * bcx + mi[1] * bcy + mi[2] * bcz;
+                const float Wa = fabsf(mi[0]) * ex + fabsf(mi[1]) * ey + fabsf(mi[2]) * ez + 0.01f;
+                float alo = fmaxf(s - Wa, (float)(-g.nxd2));
+                float ahi = fminf(s + Wa, (float)(g.nx - 1 - g.nxd2));
+                double m[9];
+#pragma unroll
+                for (int t = 0; t < 9; t++) m[t] = R.m[t];
+                if (alo <= ahi) {
+                    const float ac = 0.5f * (alo + ahi), W = 0.5f * (ahi - alo);
+                    const float fb = (float)b;
+                    const float zz0 = g.escc ? -wl2f * (fx2 * ac * ac + fy2 * fb * fb) : 0.f;
+                    const float dzz = g.escc ? -2.f * wl2f * fx2 * ac : 0.f;
+                    const float curv = g.escc ? wl2f * fx2 * W * W : 0.f;
+#pragma unroll
+                    for (int d = 0; d < 3; d++) {
+                        const float m0 = (float)m[3 * d], m1 = (float)m[3 * d + 1], m2 = (float)m[3 * d + 2];
+                        const float Pc = cgrid[d] + m0 * ac + m1 * fb + m2 * zz0;
+                        const float Pd = m0 + m2 * dzz;
+                        const float mg = fabsf(m2) * curv * 1.01f + 2e-4f * (1.f + fabsf(Pc)) + 0.01f;
+                        const float lo_d = bmin[d] - 1.f - mg - Pc, hi_d = bmax[d] + 1.f + mg - Pc;
+                        if (fabsf(Pd) < 1e-6f) {
+                            if (lo_d > 0.f || hi_d < 0.f) alo = ahi + 1.f;
+                        } else {
+                            float t0 = lo_d / Pd, t1 = hi_d / Pd;
+                            if (Pd < 0.f) { const float tt = t0; t0 = t1; t1 = tt; }
+                            alo = fmaxf(alo, ac + t0);
+                            ahi = fminf(ahi, ac + t1);
+                        }
+                    }
+                }
+                int a0 = 0, a1 = -1;
+                if (alo <= ahi) {
+                    a0 = max((int)ceilf(alo - 1e-3f), -g.nxd2);
+                    a1 = min((int)floorf(ahi + 1e-3f), g.nx - 1 - g.nxd2);
+                }
+                const cplx<float>* Kline = spec + ((size_t)r * g.ny + (b + g.nyd2)) * g.nx + g.nxd2;
+                for (int a = a0; a <= a1; a++) {
+                    const cplx<float> kraw = Kline[a];
+                    const GridPos P = grid_pos(g, m, a, b);
+                    const double fnn = floor(P.dz1), fii = floor(P.dx1), fjj = floor(P.dy1);
+                    if (fnn < 0.0 || fnn > (double)(g.nz - 2)) continue;
+                    if (fii < 0.0 || fii > (double)(g.nx - 2)) continue;
+                    if (fjj < 0.0 || fjj > (double)(g.ny - 2)) continue;
+                    const int ii = (int)fii - x0, jj = (int)fjj - y0, nn = (int)fnn - z0;
+                    if (ii < -1 || ii >= SBX || jj < -1 || jj >= SBY || nn < -1 || nn >= SBZ) continue;
+                    const double fx = P.dx1 - fii, fy = P.dy1 - fjj, fz = P.dz1 - fnn;
+                    const double wxs[2] = {1.0 - fx, fx}, wys[2] = {1.0 - fy, fy}, wzs[2] = {1.0 - fz, fz};
+                    cplx<float> kv = kraw;
+                    if (g.rot_centre_shift) {
+                        const double ph = -(R.ph[0] * P.ox + R.ph[1] * P.oy + R.ph[2] * P.oz);
+                        double sn, c;
+                        sincos(ph, &sn, &c);
+                        const float cf = (float)c, sf = (float)sn;
+                        kv = {kraw.x * cf - kraw.y * sf, kraw.x * sf + kraw.y * cf};
+                    }
+#pragma unroll
+                    for (int cz = 0; cz < 2; cz++) {
+                        const int lz = nn + cz;
+                        if (lz < 0 || lz >= SBZ) continue;
+#pragma unroll
+                        for (int cy = 0; cy < 2; cy++) {
+                            const int ly = jj + cy;
+                            if (ly < 0 || ly >= SBY) continue;
+#pragma unroll
+                            for (int cx = 0; cx < 2; cx++) {
+                                const int lx = ii + cx;
+                                if (lx < 0 || lx >= SBX) continue;
+                                const float w = (float)(wxs[cx] * wys[cy] * wzs[cz]);
+                                if (w == 0.f) continue;
+                                if (w < S_TINY_W) {
+                                    const int v = lx + SBX * (ly + SBY * lz);
+                                    atomicAdd(&tiny[v], w * kv.x);
+                                    atomicAdd(&tiny[SVOX + v], w * kv.y);
+                                    atomicAdd(&tiny[2 * SVOX + v], w);
+                                    continue;
+                                }
+                                const int li = lds_idx(lx, ly, lz);
+                                atomicAdd(&acc[li], (unsigned long long)dround_i64((double)(w * kv.x) * vscale));
+                                atomicAdd(&acc[SPAD + li], (unsigned long long)dround_i64((double)(w * kv.y) * vscale));
+                                atomicAdd(&acc[2 * SPAD + li], (unsigned long long)dround_i64((double)w * wscale));
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    const double vinv = ldexp(1.0, -sv), winv = ldexp(1.0, -sw);
+#pragma unroll
+    for (int k = 0; k < SOWN; k++) {
+        const int v = tid + 256 * k;
+        const int lx = v % SBX, ly = (v / SBX) % SBY, lz = v / (SBX * SBY);
+        const int vx = x0 + lx, vy = y0 + ly, vz = z0 + lz;
+        if (vx >= g.nx || vy >= g.ny || vz >= g.nz) continue;
+        const int li = lds_idx(lx, ly, lz);
+        const float vr = (float)((double)(long long)acc[li] * vinv + (double)tiny[v]);
+        const float vi = (float)((double)(long long)acc[SPAD + li] * vinv + (double)tiny[SVOX + v]);
+        const float S = (float)((double)(long long)acc[2 * SPAD + li] * winv + (double)tiny[2 * SVOX + v]);
+        const size_t vidx = ((size_t)vz * g.ny + vy) * g.nx + vx;
+        if (Sout) Sout[vidx] = S;
+        Vout[vidx] = voxel_epilogue<float>(g, vx, vy, vz, cplx<float>{vr, vi}, S);
+    }
 }
 
 // ------------------------------------------- plain FP64 form: scatter
