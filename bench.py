@@ -30,6 +30,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` on this workload, from the committed
+    counter summary profiles/<kernel>_metric_traffic.txt (FETCH_SIZE +
+    WRITE_SIZE, each scaled by the same counter's reading for a plain copy of
+    known size); None if no summary is committed."""
+    path = os.path.join(ROOT, "profiles", "%s_metric_traffic.txt" % kernel)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -37,6 +51,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="first run a plain 256 MiB device copy (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -49,6 +65,13 @@ def main():
     from utr_public_amd import synth
 
     torch.cuda.set_device(0)
+    if args.calib_copy:
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     p, rows, at = synth.config("metric")
     A = len(rows["angY"])
     t0 = time.time()
@@ -83,14 +106,17 @@ def main():
     value = work / (elapsed / args.steps)
     vol32 = d_vol.cpu().numpy()
 
-    # dominant kernel: the gather gridding (k_gather).  Algorithmic bytes per
-    # launch: every spectrum point read once (8 B) + V written once (8 B).
+    # dominant kernel: the brick gridding (k_grid_fx), timed with HIP events on
+    # the library's stream.  Algorithmic bytes per launch: every spectrum point
+    # read once (8 B) + the filtered V3 written once (8 B per voxel).
     g_ms = float(np.mean(grid_ms))
     g_bytes = 8.0 * A * nx * ny + 8.0 * nx * ny * nz
     achieved = g_bytes / (g_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_gather", "kernel_ms": round(g_ms, 4)}
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic("k_grid_fx"),
+                "kernel": "k_grid_fx", "kernel_ms": round(g_ms, 4),
+                "algorithmic_bytes": g_bytes}
 
     extra = {"phase_ms": {"fft2d_ctf": round(float(np.mean(fft2_ms)), 4),
                           "gridding_filters": round(g_ms, 4),

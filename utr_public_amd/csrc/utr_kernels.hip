@@ -150,8 +150,15 @@ __global__ void __launch_bounds__(256) k_cols(cplx<T>* __restrict__ spec, int nx
         vmax = fmaxf(vmax, fmaxf(fabsf((float)v.x), fabsf((float)v.y)));
     }
     if (rowmax) {
+        // one atomic per workgroup (rowmax is a kernel argument: uniform branch)
+        __shared__ float s_wmax[4];
         for (int off = 32; off > 0; off >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, off));
-        if ((threadIdx.x & 63) == 0) atomicMax(&rowmax[row0 + row], __float_as_uint(vmax));
+        if ((threadIdx.x & 63) == 0) s_wmax[threadIdx.x >> 6] = vmax;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const float m = fmaxf(fmaxf(s_wmax[0], s_wmax[1]), fmaxf(s_wmax[2], s_wmax[3]));
+            atomicMax(&rowmax[row0 + row], __float_as_uint(m));
+        }
     }
 }
 
