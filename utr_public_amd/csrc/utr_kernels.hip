@@ -469,7 +469,7 @@ __device__ __forceinline__ long long dround_i64(double d) {
     return (long long)(__double_as_longlong(y) - 0x4338000000000000LL);
 }
 
-__global__ void __launch_bounds__(256) k_grid_fx(const cplx<float>* __restrict__ spec,
+__global__ void __launch_bounds__(256, 3) k_grid_fx(const cplx<float>* __restrict__ spec,
                                                  const RowParams* __restrict__ rows, int nrows,
                                                  const unsigned* __restrict__ rowmax, GeomDev g,
                                                  cplx<float>* __restrict__ Vout,
@@ -634,8 +634,18 @@ __global__ void __launch_bounds__(256) k_grid_fx(const cplx<float>* __restrict__
                     a1 = min((int)floorf(ahi + 1e-3f), g.nx - 1 - g.nxd2);
                 }
                 const cplx<float>* Kline = spec + ((size_t)r * g.ny + (b + g.nyd2)) * g.nx + g.nxd2;
+                double ph[3] = {0.0, 0.0, 0.0};
+                if (g.rot_centre_shift) {
+                    ph[0] = R.ph[0];
+                    ph[1] = R.ph[1];
+                    ph[2] = R.ph[2];
+                }
+                // the next point's value is loaded while this one is splatted
+                cplx<float> knext = {0.f, 0.f};
+                if (a0 <= a1) knext = Kline[a0];
                 for (int a = a0; a <= a1; a++) {
-                    const cplx<float> kraw = Kline[a];
+                    const cplx<float> kraw = knext;
+                    if (a < a1) knext = Kline[a + 1];
                     const GridPos P = grid_pos(g, m, a, b);
                     const double fnn = floor(P.dz1), fii = floor(P.dx1), fjj = floor(P.dy1);
                     if (fnn < 0.0 || fnn > (double)(g.nz - 2)) continue;
@@ -647,38 +657,55 @@ __global__ void __launch_bounds__(256) k_grid_fx(const cplx<float>* __restrict__
                     const double wxs[2] = {1.0 - fx, fx}, wys[2] = {1.0 - fy, fy}, wzs[2] = {1.0 - fz, fz};
                     cplx<float> kv = kraw;
                     if (g.rot_centre_shift) {
-                        const double ph = -(R.ph[0] * P.ox + R.ph[1] * P.oy + R.ph[2] * P.oz);
+                        const double phs = -(ph[0] * P.ox + ph[1] * P.oy + ph[2] * P.oz);
                         double sn, c;
-                        sincos(ph, &sn, &c);
+                        sincos(phs, &sn, &c);
                         const float cf = (float)c, sf = (float)sn;
                         kv = {kraw.x * cf - kraw.y * sf, kraw.x * sf + kraw.y * cf};
                     }
+                    // the eight corners' terms are formed first, without branches, so
+                    // that their independent chains overlap; then each corner inside
+                    // the brick adds its terms
+                    long long tr[8], ti[8], tw[8];
+                    float wc[8];
+                    bool big[8];
+                    bool any_tiny = false;
 #pragma unroll
-                    for (int cz = 0; cz < 2; cz++) {
-                        const int lz = nn + cz;
-                        if (lz < 0 || lz >= SBZ) continue;
+                    for (int c = 0; c < 8; c++) {
+                        const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
+                        const bool inb = (unsigned)(ii + cx) < (unsigned)SBX &&
+                                         (unsigned)(jj + cy) < (unsigned)SBY &&
+                                         (unsigned)(nn + cz) < (unsigned)SBZ;
+                        const float w = (float)(wxs[cx] * wys[cy] * wzs[cz]);
+                        big[c] = inb && w >= S_TINY_W;
+                        any_tiny |= inb && w != 0.f && w < S_TINY_W;
+                        wc[c] = w;
+                        tr[c] = dround_i64((double)(w * kv.x) * vscale);
+                        ti[c] = dround_i64((double)(w * kv.y) * vscale);
+                        tw[c] = dround_i64((double)w * wscale);
+                    }
+                    const int li0 = lds_idx(ii, jj, nn);
 #pragma unroll
-                        for (int cy = 0; cy < 2; cy++) {
-                            const int ly = jj + cy;
-                            if (ly < 0 || ly >= SBY) continue;
+                    for (int c = 0; c < 8; c++) {
+                        if (!big[c]) continue;
+                        const int li = li0 + (c & 1) + SPY * ((c >> 1) & 1) + SPZ * (c >> 2);
+                        atomicAdd(&acc[li], (unsigned long long)tr[c]);
+                        atomicAdd(&acc[SPAD + li], (unsigned long long)ti[c]);
+                        atomicAdd(&acc[2 * SPAD + li], (unsigned long long)tw[c]);
+                    }
+                    if (any_tiny) {
+                        // weights below 2^-20: float accumulators (rare)
 #pragma unroll
-                            for (int cx = 0; cx < 2; cx++) {
-                                const int lx = ii + cx;
-                                if (lx < 0 || lx >= SBX) continue;
-                                const float w = (float)(wxs[cx] * wys[cy] * wzs[cz]);
-                                if (w == 0.f) continue;
-                                if (w < S_TINY_W) {
-                                    const int v = lx + SBX * (ly + SBY * lz);
-                                    atomicAdd(&tiny[v], w * kv.x);
-                                    atomicAdd(&tiny[SVOX + v], w * kv.y);
-                                    atomicAdd(&tiny[2 * SVOX + v], w);
-                                    continue;
-                                }
-                                const int li = lds_idx(lx, ly, lz);
-                                atomicAdd(&acc[li], (unsigned long long)dround_i64((double)(w * kv.x) * vscale));
-                                atomicAdd(&acc[SPAD + li], (unsigned long long)dround_i64((double)(w * kv.y) * vscale));
-                                atomicAdd(&acc[2 * SPAD + li], (unsigned long long)dround_i64((double)w * wscale));
-                            }
+                        for (int c = 0; c < 8; c++) {
+                            const int lx = ii + (c & 1), ly = jj + ((c >> 1) & 1), lz = nn + (c >> 2);
+                            const float w = wc[c];
+                            if ((unsigned)lx >= (unsigned)SBX || (unsigned)ly >= (unsigned)SBY ||
+                                (unsigned)lz >= (unsigned)SBZ || w == 0.f || w >= S_TINY_W)
+                                continue;
+                            const int v = lx + SBX * (ly + SBY * lz);
+                            atomicAdd(&tiny[v], w * kv.x);
+                            atomicAdd(&tiny[SVOX + v], w * kv.y);
+                            atomicAdd(&tiny[2 * SVOX + v], w);
                         }
                     }
                 }
