@@ -122,6 +122,19 @@ def main():
                           "gridding_filters": round(g_ms, 4),
                           "fft3d_mask_scale": round(float(np.mean(fft3_ms)), 4)}}
 
+    # host-memory rate: images from pageable host memory in, volume back to
+    # host memory (the PCIe-inclusive path of utr_add_images / utr_finalize)
+    host_vol = np.empty((nz, ny, nx), dtype=np.float32)
+    reps = 3
+    ts = time.perf_counter()
+    for _ in range(reps):
+        rec.reset()
+        rec.add_images(imgs)
+        rec.finalize(host_vol)
+    th = (time.perf_counter() - ts) / reps
+    extra["host_io"] = {"value": work / th, "ms_per_step": round(th * 1e3, 3),
+                        "note": "images in host memory, volume to host memory"}
+
     if not args.no_fp64:
         rec64 = U.Reconstructor(p, rows, U.UTR_FP64)
         vol64 = torch.empty((nz, ny, nx), dtype=torch.float32, device="cuda")

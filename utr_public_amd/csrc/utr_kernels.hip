@@ -13,7 +13,7 @@
 //
 // Reference stages (MultisliceCppTest/PhaseRetrieval):
 //   k_rows / k_cols        UTR.cpp:951-1019  (Log0, Shuffle, FFT, 2D filters)
-//   k_gather               UTR.h:188-307 gridding + UTR.cpp:1050-1201
+//   k_grid_fx, k_gather    UTR.h:188-307 gridding + UTR.cpp:1050-1201
 //   k_scatter, k_normalize plain FP64 form of the same
 //   k_pack_z / k_fft_y / k_c2r_x   UTR.cpp:1203-1279
 #include <hip/hip_runtime.h>
