@@ -1,0 +1,47 @@
+/*
+ * utr_io.h -- C ABI of the host-side file formats of the UTR command-line
+ * program (libutr_io.so; plain C++, no GPU).  It lets a caller prepare the
+ * inputs of `utr` and read back its outputs in the formats UTR.exe uses.
+ * Every function returns 0 on success and -1 on failure, with the message
+ * in utr_io_last_error() (thread-local).
+ */
+#ifndef UTR_IO_H
+#define UTR_IO_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/*
+ * Reads one image; the format follows the extension: .grd ("DSBB"/"DSAA",
+ * ReadmeUTR.txt:561-579), .tif/.tiff (uncompressed grey), .raw (header bytes,
+ * element 4 or 8 bytes, endianness as parameter 6 of UTR.txt; nx, ny give
+ * its size).  Replaces XArData::ReadFileGRD / ReadFileRAW / TIFFReadFile
+ * (UTR.cpp:918-920).  cap: capacity of out in floats; the size read is
+ * returned in *out_nx, *out_ny.  With out == NULL only the size is returned.
+ */
+int utr_io_read_image(const char* path, long nx, long ny, long header, long element,
+                      int big_endian, float* out, long cap, long* out_nx, long* out_ny);
+
+/*
+ * Writes one slice; the format follows the extension: .grd (binary "DSBB",
+ * with the x/y extents given), .tif/.tiff (32-bit float, or 16-bit when
+ * maxout > 0 with the clamp-and-scale of ReadmeUTR.txt:448-454), .dcm (16-bit
+ * DICOM; maxout must be > 0).  Replaces one slice of WriteFileStackGRD /
+ * TIFFWriteFileStack / WriteFileStackDICOM (UTR.cpp:1526-1541).
+ */
+int utr_io_write_slice(const char* path, long nx, long ny, const float* v, double xlo, double xhi,
+                       double ylo, double yhi, float minin, float maxin, long maxout);
+
+/*
+ * File name number `index` of a template with `count` entries (the
+ * numbering of FileNames, ReadmeUTR.txt:241-255): written to out (cap bytes).
+ */
+int utr_io_file_name(const char* tmpl, long count, long index, char* out, long cap);
+
+const char* utr_io_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* UTR_IO_H */

@@ -1,0 +1,335 @@
+"""The command-line drop-in (utr_public_amd/bin/utr) and the host file formats
+(libutr_io.so), on the CPU: parameter grammar and validation as UTR.cpp:55-784,
+orientation / subseries files, FileNames numbering (ReadmeUTR.txt:241-255,
+429-436), GRD / TIFF / RAW / DICOM round trips.  No GPU call is made here:
+`utr ... --check` stops after parsing and validation.
+"""
+import ctypes
+import json
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "utr_public_amd", "bin", "utr")
+IOLIB = os.path.join(ROOT, "utr_public_amd", "_lib", "libutr_io.so")
+
+TXT_LINES = [
+    ("1.Modality_and_units_of_length:_TEM(0)_or_X-ray(1):", "0"),
+    ("2.Verbose_output:", "1"),
+    ("3.Input_file_with_rotation_angles_and_defocus_distances:", "{orient}"),
+    ("4.Optional_file_with_indexes_and_weights_or_NONE:", "{subseries}"),
+    ("5.Filename_base_of_input_images:", "{inp}"),
+    ("6.HeaderLength_Endianness_ElementLength_(RAW):", "0 0 4"),
+    ("7.Input_data_normalization_factors_f1_f2:", "1.0 0.0"),
+    ("8.Width_and_height_of_input_images_in_pixels:", "{nx} {ny}"),
+    ("9.Pixel_size_in_UL:", "{xst}"),
+    ("10.Output_defocus_distances_min_max_offset_in_UL:", "{zlo} {zhi} 0.0"),
+    ("11.Centre_of_rotation_x_y_z_shifts_in_UL:", "0 0 0"),
+    ("12.Wavelength_in_UL:", "{wl}"),
+    ("13.Objective_aperture_in_mrad:", "0.0"),
+    ("14.Spherical_aberration_Cs3_Cs5_in_mm:", "0.0 0.0"),
+    ("15.Phase_retrieval_method:", "{mode}"),
+    ("16.Ewald_sphere_curvature_correction:", "1"),
+    ("17.Absorption_beta/delta:", "{asigma}"),
+    ("18.Average_noise-to-signal_ratio:", "{nsr}"),
+    ("19.Tikhonov_regularization_parameter:", "0.0001"),
+    ("20.Enforce_symmetry:", "0"),
+    ("21.Input_file_with_symmetry_angles:", "Sym.txt"),
+    ("22.Inverse_3D_Laplacian_filter:", "0"),
+    ("23.Low-pass_filter_width_background_threshold:", "0.0 0.0 -1000000.0"),
+    ("24.Peak_localization_mode:", "0"),
+    ("25.Side_lengths_for_peak_localization:", "0.8 3.2"),
+    ("26.Output_file_name_base:", "{out}"),
+    ("27.Min-in_max-in_max-out_for_16-bit_output:", "{b16}"),
+    ("28.Euler_rotation_angles_for_3D_output:", "0 0 0"),
+    ("29.Slice_output_3D_volume:", "0"),
+    ("30.Flip_axes:", "0 0 0"),
+    ("31.Thick_output_slices:", "0 1.0"),
+    ("32.Save_3D_sampling_matrix:", "0"),
+    ("33.Import_and_reprocess:", "0"),
+    ("34.Folder_name_for_auxiliary_files:", "{aux}"),
+    ("35.Number_of_parallel_threads:", "4"),
+]
+XRI_SKIP = {1, 10, 13, 14, 20, 21, 24, 25}
+
+
+def write_params(path, xri=False, override=None, **kw):
+    d = dict(orient="orient.txt", subseries="NONE", inp="img.grd", nx=64, ny=64, xst=0.4,
+             zlo=-12.8, zhi=12.8, wl=0.02507934, mode=0, asigma=0.0, nsr=0.0, out="out/vol.grd",
+             b16="0 0 0", aux="aux")
+    d.update(kw)
+    lines = ["//*****Input parameter file*****"]
+    for i, (title, val) in enumerate(TXT_LINES, start=1):
+        if xri and i in XRI_SKIP:
+            continue
+        if override and i in override:
+            val = override[i]
+        lines.append("%s %s" % (title, val.format(**d)))
+    lines.append("//*****")
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+@pytest.fixture
+def work(tmp_path):
+    os.makedirs(tmp_path / "out")
+    os.makedirs(tmp_path / "aux")
+    with open(tmp_path / "orient.txt", "w") as f:
+        f.write("//Z Y' Z'' defocus\n")
+        for k in range(36):
+            f.write("0 %g 0 500\n" % (5.0 * k))
+    return tmp_path
+
+
+def run_cli(cwd, *args):
+    r = subprocess.run([EXE] + list(args), cwd=str(cwd), stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT, text=True, stdin=subprocess.DEVNULL, timeout=60)
+    return r.returncode, r.stdout
+
+
+def check_json(cwd, pfile):
+    rc, out = run_cli(cwd, pfile, "--check")
+    assert rc == 0, out
+    line = [l for l in out.splitlines() if l.startswith("{")][-1]
+    return json.loads(line)
+
+
+def test_exports_and_exe_exist():
+    assert os.access(EXE, os.X_OK)
+    lib = ctypes.CDLL(IOLIB)
+    for sym in ("utr_io_read_image", "utr_io_write_slice", "utr_io_file_name", "utr_io_last_error"):
+        assert hasattr(lib, sym)
+
+
+def test_txt_parameter_file_derived_geometry(work):
+    write_params(work / "UTR.txt", mode=4, asigma=0.07, nsr=0.05)
+    j = check_json(work, "UTR.txt")
+    assert (j["nx"], j["ny"], j["nz"]) == (64, 64, 64)          # nz = int((zhi-zlo)/xst + 0.5)
+    assert j["modality"] == 0 and j["ctf_mode"] == 4 and j["escc"] == 1
+    assert j["asigma"] == 0.07 and j["nsr"] == 0.05 and j["rows"] == 36
+    assert j["angY0"] == 0.0 and j["defocus0"] == 500.0
+
+
+def test_default_parameter_file_name(work):
+    write_params(work / "UTR.txt")
+    rc, out = run_cli(work, "--check")
+    assert rc == 0 and "Reading input parameter file UTR.txt" in out
+
+
+def test_xri_parameter_file(work):
+    # .xri: 27 lines; X-ray modality, z range = image x range, asigma = -1/(delta/beta)
+    write_params(work / "p.xri", xri=True, asigma=275.0, mode=2, wl=0.000038745052, xst=100.0,
+                 zlo=0, zhi=0)
+    j = check_json(work, "p.xri")
+    assert j["modality"] == 1
+    assert j["nz"] == 64 and j["zlo"] == -3200.0 and j["zhi"] == 3200.0
+    assert j["asigma"] == pytest.approx(-1.0 / 275.0, rel=1e-15)
+    assert j["aobj_mrad"] == 0.0 and j["cs3_mm"] == 0.0
+
+
+def test_zlo_zhi_swapped_and_odd_nz(work):
+    write_params(work / "UTR.txt", zlo=12.8, zhi=-12.8)
+    assert check_json(work, "UTR.txt")["zlo"] == -12.8
+    write_params(work / "UTR.txt", zlo=-12.6, zhi=12.6)     # nz = 63
+    rc, out = run_cli(work, "UTR.txt", "--check")
+    assert rc == 1 and "Error: number of z points must be even." in out
+
+
+@pytest.mark.parametrize("override,msg", [
+    ({12: "2.0"}, "Error: wavelength value appears to be wrong."),
+    ({9: "-1"}, "Error: pixel size must be positive in input parameter file."),
+    ({8: "63 64"}, "Error: width and height parameters in input parameter file must be even."),
+    ({7: "0 0"}, "The first normalization factor cannot be zero (leads to division by zero)."),
+    ({15: "7"}, "Error: unknown value for phase retrieval parameter in input parameter file."),
+    ({16: "2"}, "Error: unknown value for Ewald sphere curvature correction switch in input parameter file."),
+    ({17: "-0.1"}, "Error: beta/delta must be non-negative in TEM mode."),
+    ({15: "1"}, "Error: beta/delta must be negative in TIE-Hom phase retrieval."),
+    ({18: "-1"}, "Error: NSR in input images must be non-negative."),
+    ({19: "0"}, "Error: when beta/delta == 0, epsilon must be positive."),
+    ({20: "1"}, "Error: symmetry enforcing mode has not been implemented in this program yet."),
+    ({5: "img.png"}, "Error: input filename extension must be TIF, GRD or RAW."),
+    ({6: "0 0 2"}, "Unrecognised ElementLength in input parameter file"),
+    ({26: "out/vol.png"}, "Error: output filename extension must be TIF/TIFF, DCM or GRD."),
+    ({26: "out/vol.dcm"}, "only 16-bit data can be saved in DICOM files."),
+    ({27: "1 2 100"}, "16-bit output cannot be saved in GRD files."),
+    ({26: "out/vol.tif", 27: "2 1 100"}, "Unsuitable values for minimum-in, maximum-in or maximum-out"),
+    ({29: "6"}, "Error: unknown value for the output 3D volume reslicing parameter."),
+    ({30: "0 2 0"}, "Unknown value of Y axis flipping parameter in input parameter file."),
+    ({31: "1 0.4"}, "Slice thickness is smaller than or equal to the voxel thickness."),
+    ({35: "0"}, "Error: the number of parallel threads in input parameter file should be >= 1."),
+    ({26: "nofolder/vol.grd"}, "Error: the specified file folder for 3D potential or beta does not seem to exist."),
+    ({34: "noaux"}, "Error: the specified auxiliary file folder does not seem to exist."),
+    ({4: "sel.dat"}, "Error: unrecognised filename extension in the file with indexes and weights"),
+])
+def test_validation_messages(work, override, msg):
+    write_params(work / "UTR.txt", override=override)
+    rc, out = run_cli(work, "UTR.txt", "--check")
+    assert rc == 1
+    assert "!!!Exception: " + msg in out, out
+
+
+def test_truncated_parameter_file(work):
+    write_params(work / "UTR.txt")
+    lines = open(work / "UTR.txt").read().splitlines()
+    with open(work / "UTR.txt", "w") as f:
+        f.write("\n".join(lines[:12]) + "\n")
+    rc, out = run_cli(work, "UTR.txt", "--check")
+    assert rc == 1 and "Error reading wavelength from input parameter file." in out
+
+
+def test_parameter_file_extension(work):
+    rc, out = run_cli(work, "params.cfg", "--check")
+    assert rc == 1 and "unknown filename extension in the command-line argument" in out
+
+
+def test_two_defocus_distances_rejected(work):
+    with open(work / "orient.txt", "w") as f:
+        f.write("0 0 0 500 0 600\n0 10 0 500 0 600\n")
+    write_params(work / "UTR.txt")
+    rc, out = run_cli(work, "UTR.txt", "--check")
+    assert rc == 1 and "one defocus distance per orientation" in out
+
+
+def test_relionnew_signs(work):
+    # all numbers except the astigmatism angle change sign (ReadmeUTR.txt:215-220)
+    with open(work / "o.relionnew", "w") as f:
+        f.write("1 a.mrc 0 0 -300 -300 0 -12 -34 0\n2 b.mrc 0 0 -300 -300 0 -13 -35 0\n")
+    write_params(work / "UTR.txt", orient="o.relionnew")
+    j = check_json(work, "UTR.txt")
+    assert (j["angZ0"], j["angY0"], j["defocus0"]) == (12.0, 34.0, 300.0)
+
+
+def test_subseries_selection(work):
+    with open(work / "sel.txt", "w") as f:
+        f.write("3 0.5\n7 1.5\n11 1.0\n")
+    write_params(work / "UTR.txt", subseries="sel.txt")
+    j = check_json(work, "UTR.txt")
+    assert j["rows"] == 3 and j["weights"] == 3
+    assert j["angY0"] == 15.0
+    with open(work / "sel.txt", "w") as f:
+        f.write("40 1.0\n")
+    rc, out = run_cli(work, "UTR.txt", "--check")
+    assert rc == 1 and "larger than the number of entries - 1" in out
+
+
+# ---------------------------------------------------------------- file IO
+
+@pytest.fixture(scope="module")
+def io():
+    L = ctypes.CDLL(IOLIB)
+    L.utr_io_read_image.argtypes = [ctypes.c_char_p, ctypes.c_long, ctypes.c_long, ctypes.c_long,
+                                    ctypes.c_long, ctypes.c_int, ctypes.c_void_p, ctypes.c_long,
+                                    ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long)]
+    L.utr_io_write_slice.argtypes = [ctypes.c_char_p, ctypes.c_long, ctypes.c_long, ctypes.c_void_p,
+                                     ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_float, ctypes.c_float, ctypes.c_long]
+    L.utr_io_file_name.argtypes = [ctypes.c_char_p, ctypes.c_long, ctypes.c_long, ctypes.c_char_p,
+                                   ctypes.c_long]
+    L.utr_io_last_error.restype = ctypes.c_char_p
+    return L
+
+
+def read(io, path, nx=0, ny=0, header=0, element=4, be=0):
+    w, h = ctypes.c_long(), ctypes.c_long()
+    assert io.utr_io_read_image(str(path).encode(), nx, ny, header, element, be, None, 0,
+                                ctypes.byref(w), ctypes.byref(h)) == 0, io.utr_io_last_error()
+    out = np.empty((h.value, w.value), np.float32)
+    assert io.utr_io_read_image(str(path).encode(), nx, ny, header, element, be,
+                                out.ctypes.data, out.size, None, None) == 0
+    return out
+
+
+def write(io, path, a, ext=(0, 1, 0, 1), b16=(0, 0, 0)):
+    a = np.ascontiguousarray(a, np.float32)
+    return io.utr_io_write_slice(str(path).encode(), a.shape[1], a.shape[0], a.ctypes.data,
+                                 *ext, *b16)
+
+
+@pytest.mark.parametrize("tmpl,count,expect", [
+    ("ag.grd", 36, ["ag00.grd", "ag01.grd", "ag35.grd"]),
+    ("ag1.grd", 108, ["ag1.grd", "ag2.grd", "ag108.grd"]),
+    ("ag001.grd", 108, ["ag001.grd", "ag002.grd", "ag108.grd"]),
+    ("out/vol.grd", 256, ["out/vol000.grd", "out/vol001.grd", "out/vol255.grd"]),
+    ("d2.0/p.tif", 10, ["d2.0/p0.tif", "d2.0/p1.tif", "d2.0/p9.tif"]),
+    ("x.grd", 1, ["x0.grd", "x0.grd", "x0.grd"]),
+])
+def test_file_names(io, tmpl, count, expect):
+    buf = ctypes.create_string_buffer(256)
+    got = []
+    for idx in (0, 1 if count > 1 else 0, count - 1):
+        assert io.utr_io_file_name(tmpl.encode(), count, idx, buf, 256) == 0
+        got.append(buf.value.decode())
+    assert got == expect
+
+
+def test_grd_round_trip_and_header(io, tmp_path):
+    a = np.random.default_rng(0).standard_normal((6, 10)).astype(np.float32)
+    p = tmp_path / "a.grd"
+    assert write(io, p, a, ext=(0.0, 4.0, 0.0, 2.4)) == 0
+    raw = open(p, "rb").read()
+    assert raw[:4] == b"DSBB" and len(raw) == 56 + 4 * a.size
+    nx, ny = struct.unpack("<hh", raw[4:8])
+    h = struct.unpack("<6d", raw[8:56])
+    assert (nx, ny) == (10, 6)
+    assert h[:4] == (0.0, 4.0, 0.0, 2.4) and h[4] == a.min() and h[5] == a.max()
+    assert np.array_equal(read(io, p), a)
+
+
+def test_grd_ascii_read(io, tmp_path):
+    p = tmp_path / "t.grd"
+    with open(p, "w") as f:
+        f.write("DSAA\n3 2\n0 1\n0 1\n0 5\n0 1 2\n3 4 5\n")
+    assert np.array_equal(read(io, p), np.arange(6, dtype=np.float32).reshape(2, 3))
+
+
+def test_tiff_f32_and_u16(io, tmp_path):
+    a = np.linspace(-1, 3, 35, dtype=np.float32).reshape(5, 7)
+    p = tmp_path / "a.tif"
+    assert write(io, p, a) == 0
+    assert np.array_equal(read(io, p), a)
+    q = tmp_path / "b.tiff"
+    assert write(io, q, a, b16=(0.0, 2.0, 4095)) == 0
+    expect = np.round(np.clip(a, 0, 2) * (4095 / 2.0))
+    assert np.array_equal(read(io, q), expect.astype(np.float32))
+
+
+def test_raw_endianness_header_double(io, tmp_path):
+    a = np.random.default_rng(1).standard_normal((4, 6))
+    p = tmp_path / "a.raw"
+    with open(p, "wb") as f:
+        f.write(b"\x00" * 16 + a.astype(">f8").tobytes())
+    assert np.array_equal(read(io, p, 6, 4, 16, 8, 1), a.astype(np.float32))
+    with open(p, "wb") as f:
+        f.write(a.astype("<f4").tobytes())
+    assert np.array_equal(read(io, p, 6, 4, 0, 4, 0), a.astype(np.float32))
+
+
+def test_dicom_16bit(io, tmp_path):
+    a = np.linspace(0, 1, 12, dtype=np.float32).reshape(3, 4)
+    p = tmp_path / "a.dcm"
+    assert write(io, p, a, ext=(0, 4, 0, 3), b16=(0.0, 1.0, 1000)) == 0
+    raw = open(p, "rb").read()
+    assert raw[128:132] == b"DICM"
+    # pixel data element (7FE0,0010) OW is the last one
+    tag = struct.pack("<HH", 0x7FE0, 0x0010) + b"OW\x00\x00"
+    k = raw.rfind(tag)
+    n = struct.unpack("<I", raw[k + 8:k + 12])[0]
+    pix = np.frombuffer(raw[k + 12:k + 12 + n], "<u2").reshape(3, 4)
+    assert np.array_equal(pix, np.round(a * 1000).astype(np.uint16))
+    assert b"MONOCHROME2" in raw
+    assert write(io, p, a) == -1
+    assert b"only 16-bit" in io.utr_io_last_error()
+
+
+def test_io_errors(io, tmp_path):
+    assert write(io, tmp_path / "x.png", np.zeros((2, 2))) == -1
+    p = tmp_path / "bad.grd"
+    open(p, "wb").write(b"XXXX")
+    w = ctypes.c_long()
+    assert io.utr_io_read_image(str(p).encode(), 0, 0, 0, 4, 0, None, 0, ctypes.byref(w),
+                                ctypes.byref(w)) == -1
+    assert b"not a GRD file" in io.utr_io_last_error()

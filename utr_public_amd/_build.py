@@ -3,6 +3,7 @@
 Outputs (git-ignored, shipped with the tree):
   utr_public_amd/_lib/libutr_hip.so    HIP kernels + C ABI (gfx950)
   utr_public_amd/_lib/libutr_synth.so  seeded synthetic inputs (host C++)
+  utr_public_amd/_lib/libutr_io.so     file formats of the command-line program (host C++)
   utr_public_amd/bin/utr               command-line drop-in for UTR.exe
 """
 import os
@@ -48,14 +49,17 @@ def build(verbose=False):
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-fopenmp", "-shared", "-o", synth + ".tmp",
           os.path.join(CSRC, "synth.cpp")], verbose)
     os.replace(synth + ".tmp", synth)
-    cli_src = os.path.join(CSRC, "utr_cli.cpp")
-    if os.path.exists(cli_src):
-        exe = os.path.join(BINDIR, "utr")
-        _run(["g++", "-O2", "-std=c++17", "-fopenmp", "-o", exe + ".tmp", cli_src,
-              os.path.join(CSRC, "utr_io.cpp"),
-              "-I", os.path.join(ROOT, "include"), "-L", LIBDIR, "-lutr_hip",
-              "-Wl,-rpath,$ORIGIN/../_lib"], verbose)
-        os.replace(exe + ".tmp", exe)
+    inc = ["-I", os.path.join(ROOT, "include")]
+    io_lib = os.path.join(LIBDIR, "libutr_io.so")
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-o", io_lib + ".tmp",
+          os.path.join(CSRC, "utr_io.cpp")] + inc, verbose)
+    os.replace(io_lib + ".tmp", io_lib)
+    exe = os.path.join(BINDIR, "utr")
+    _run(["g++", "-O2", "-std=c++17", "-fopenmp", "-Wall", "-o", exe + ".tmp",
+          os.path.join(CSRC, "utr_cli.cpp"), os.path.join(CSRC, "utr_io.cpp")] + inc +
+         ["-L", LIBDIR, "-lutr_hip", "-Wl,-rpath,$ORIGIN/../_lib",
+          "-Wl,-rpath-link,/opt/rocm/lib"], verbose)
+    os.replace(exe + ".tmp", exe)
     return hip_lib
 
 
