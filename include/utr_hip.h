@@ -164,6 +164,60 @@ int utr_get_fourier_volume(utr_ctx* ctx, float* vol_c);
  */
 int utr_angle_coeffs(const utr_params* params, double angZ_deg, double angY_deg, double* m9);
 
+/* ------------------------------------------------------------------ DHT
+ * The single-defocus reconstructions of the DHT program
+ * (MultisliceCppTest/PhaseRetrieval/DHT.cpp): method 3 (phase estimate
+ * -Fact*0.5*(I-1), Fresnel back-propagation to every output plane,
+ * DHT.cpp:919-1006), method 4 (variable-distance weak-object CTF inversion,
+ * DHT.cpp:1010-1055), both followed by the real-space rotate-and-splat of
+ * DHT.cpp:1081-1164 and the scale of DHT.cpp:1325-1340; method 5 (sCTF on the
+ * Ewald sphere with the Filt3 filter, DHT.cpp:1056-1067, 1195-1416).
+ * Methods 1 and 2 (IWFR, CTFL2: several defocus distances per orientation)
+ * are UTR_E_UNSUPPORTED.
+ */
+typedef struct dht_params {
+    int nx, ny;              /* 6: image width/height (powers of 2; even for method 5) */
+    double xst;              /* 7: pixel size (Angstrom)                     */
+    double zlo, zhi;         /* 19: output defocus range                     */
+    double dzextra;          /* 20: extra defocus (method 3 only, non-zero)  */
+    double dxc, dyc, dzc;    /* 8: centre-of-rotation shifts                 */
+    double wl;               /* 10: wavelength                               */
+    double aobj_mrad;        /* 11: objective aperture                       */
+    double cs3_mm, cs5_mm;   /* 12: spherical aberrations                    */
+    double asigma;           /* 13: beta/delta (methods 4, 5)                */
+    int method;              /* 14: 3, 4 or 5                                */
+    int escc;                /* 15: Ewald sphere curvature (per-plane back-propagation) */
+    double critical_snr;     /* 17: method 5                                 */
+    double epsilon;          /* 18: Fact (method 3) or Tikhonov parameter    */
+    int inv_laplace;         /* 23: inverse 3D Laplacian (method 5)          */
+    double alpha;            /* 24: its regularisation parameter             */
+    double norm_f1, norm_f2; /* 9: input -> input/f1 + f2                    */
+} dht_params;
+
+typedef struct dht_ctx dht_ctx;
+
+/* Replaces the set-up of DHT.cpp:36-720 (validation, geometry) and the 3D
+ * array allocation of DHT.cpp:716-740.  Rows as for utr_create (one defocus
+ * distance per orientation, Z'' must be zero); weights may be NULL. */
+int dht_create(const dht_params* params, int nrows, const double* angZ_deg,
+               const double* angY_deg, const double* defocus, const float* weights,
+               dht_ctx** out);
+
+/* The per-orientation phase retrieval input (DHT.cpp:787-947): images of rows
+ * [first_row, first_row+count), count*ny*nx floats in host / device memory. */
+int dht_add_images(dht_ctx* ctx, int first_row, int count, const float* images);
+int dht_add_images_device(dht_ctx* ctx, int first_row, int count, const float* d_images);
+
+/* Back-propagation or CTF correction per plane, splat and scale
+ * (DHT.cpp:957-1416): nz*ny*nx floats to host / device memory. */
+int dht_finalize(dht_ctx* ctx, float* vol);
+int dht_finalize_device(dht_ctx* ctx, float* d_vol);
+
+int dht_reset(dht_ctx* ctx);
+int dht_dims(const dht_ctx* ctx, int* nx, int* ny, int* nz);
+int dht_get_timings(const dht_ctx* ctx, utr_timings* t);
+int dht_destroy(dht_ctx* ctx);
+
 #ifdef __cplusplus
 }
 #endif

@@ -213,3 +213,91 @@ def last_timings():
     t = np.zeros(4)
     lib().ora_last_timings(_dp(t))
     return {"fft2d_ctf_ms": t[0], "backprop_ms": t[1], "fft3d_ms": t[2], "total_ms": t[3]}
+
+
+# ------------------------------------------------------------------- DHT
+
+DHT_FIELDS = [
+    ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("xst", ctypes.c_double),
+    ("zlo", ctypes.c_double), ("zhi", ctypes.c_double), ("dzextra", ctypes.c_double),
+    ("dxc", ctypes.c_double), ("dyc", ctypes.c_double), ("dzc", ctypes.c_double),
+    ("wl", ctypes.c_double), ("aobj_mrad", ctypes.c_double), ("cs3_mm", ctypes.c_double),
+    ("cs5_mm", ctypes.c_double), ("asigma", ctypes.c_double), ("method", ctypes.c_int),
+    ("escc", ctypes.c_int), ("critical_snr", ctypes.c_double), ("epsilon", ctypes.c_double),
+    ("inv_laplace", ctypes.c_int), ("alpha", ctypes.c_double),
+    ("norm_f1", ctypes.c_double), ("norm_f2", ctypes.c_double),
+]
+
+
+class OraDhtParams(ctypes.Structure):
+    _fields_ = DHT_FIELDS
+
+
+def _dht_lib():
+    L = lib()
+    if not getattr(L, "_dht_ready", False):
+        dp = ctypes.POINTER(ctypes.c_double)
+        fp = ctypes.POINTER(ctypes.c_float)
+        P = ctypes.POINTER(OraDhtParams)
+        L.ora_dht_nz.argtypes = [P]
+        L.ora_dht_planes.argtypes = [P, fp, ctypes.c_double, ctypes.c_double, dp,
+                                     ctypes.POINTER(ctypes.c_int)]
+        L.ora_dht_splat.argtypes = [P, dp, ctypes.c_int, ctypes.c_double, ctypes.c_double, dp]
+        L.ora_dht_reconstruct.argtypes = [P, ctypes.c_int, dp, dp, dp, fp, fp, dp, ctypes.c_int]
+        L._dht_ready = True
+    return L
+
+
+def make_dht_params(d):
+    p = OraDhtParams()
+    for name, _ in DHT_FIELDS:
+        setattr(p, name, d[name])
+    return p
+
+
+def dht_nz(params):
+    p = make_dht_params(params)
+    nz = _dht_lib().ora_dht_nz(ctypes.byref(p))
+    if nz < 0:
+        raise OracleError(lib().ora_last_error().decode())
+    return nz
+
+
+def dht_planes(params, img, zout, weight=1.0):
+    """Plane stack (nplanes, ny, nx) of one orientation (methods 3, 4)."""
+    p = make_dht_params(params)
+    nz = dht_nz(params)
+    img = np.ascontiguousarray(img, dtype=np.float32)
+    out = np.zeros((nz,) + img.shape)
+    n = ctypes.c_int()
+    if _dht_lib().ora_dht_planes(ctypes.byref(p), _fp(img), zout, weight, _dp(out), ctypes.byref(n)):
+        raise OracleError(lib().ora_last_error().decode())
+    return out[:n.value].copy()
+
+
+def dht_splat(params, planes, angZ_deg, angY_deg, K3out=None):
+    p = make_dht_params(params)
+    planes = np.ascontiguousarray(planes, dtype=np.float64)
+    nz = dht_nz(params)
+    if K3out is None:
+        K3out = np.zeros((nz,) + planes.shape[1:])
+    if _dht_lib().ora_dht_splat(ctypes.byref(p), _dp(planes), planes.shape[0], angZ_deg, angY_deg,
+                                _dp(K3out)):
+        raise OracleError(lib().ora_last_error().decode())
+    return K3out
+
+
+def dht_reconstruct(params, rows, images, weights=None, nthreads=1):
+    p = make_dht_params(params)
+    angZ = np.ascontiguousarray(rows["angZ"], dtype=np.float64)
+    angY = np.ascontiguousarray(rows["angY"], dtype=np.float64)
+    defoc = np.ascontiguousarray(rows["defocus"], dtype=np.float64)
+    images = np.ascontiguousarray(images, dtype=np.float32)
+    nz = dht_nz(params)
+    vol = np.zeros((nz,) + images.shape[1:])
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float32)
+    if _dht_lib().ora_dht_reconstruct(ctypes.byref(p), len(angZ), _dp(angZ), _dp(angY), _dp(defoc),
+                                      _fp(w) if w is not None else None, _fp(images), _dp(vol),
+                                      nthreads):
+        raise OracleError(lib().ora_last_error().decode())
+    return vol

@@ -43,6 +43,8 @@ static int fail(const char* msg) {
 
 void ora_last_timings(double* t4) { memcpy(t4, g_timings, sizeof g_timings); }
 
+int ora_set_error(const char* msg) { return fail(msg); }
+
 static double now_ms(void) {
     struct timespec ts;
     clock_gettime(CLOCK_MONOTONIC, &ts);

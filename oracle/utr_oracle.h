@@ -135,6 +135,68 @@ int ora_reconstruct(const ora_params* p, int nrows, const double* angZ_deg,
  * [3] total. */
 void ora_last_timings(double* t4);
 
+/* ------------------------------------------------------------------ DHT
+ * FP64 restatement of the single-defocus reconstructions of the DHT program
+ * (MultisliceCppTest/PhaseRetrieval/DHT.cpp:700-1416), in oracle/dht_oracle.c.
+ * Test infrastructure like the rest of this header.  Parity status:
+ *   ora_dht_splat      in-tree source restated line by line (DHT.cpp:1081-1164,
+ *                      including the (int) truncation); pinned by a brute-force
+ *                      Python transcription and weight conservation
+ *                      (tests/test_dht_oracle.py).
+ *   ora_dht_planes     method 3: the phase estimate of ReadmePhaseRetrieval.txt
+ *                      (Phase = Fact * -0.5 * (I/I0 - 1)) with the Fresnel
+ *                      propagator of the decision register (absent
+ *                      XArray2DFFT::Fresnel) -- propagator PARITY UNPINNED,
+ *                      pinned against closed forms (free-space unitarity,
+ *                      propagation of a Gaussian, zero-distance identity).
+ *                      method 4: weak-object CTF inversion (absent
+ *                      XA_IWFR::InvertPhaseCTF) -- PARITY UNPINNED; pinned by
+ *                      the forward weak-object model it inverts.
+ *   ora_dht_reconstruct method 5 reuses ora_spectrum (mode 3 restatement of the
+ *                      absent InvertCTF_DT1 -- PARITY UNPINNED) and ora_grid,
+ *                      with the Filt3 filter, Hermitian fix, c2r and scale of
+ *                      DHT.cpp:1341-1416 restated line by line.
+ */
+typedef struct ora_dht_params {
+    int nx, ny;            /* param 6 */
+    double xst;            /* param 7 (Angstrom) */
+    double zlo, zhi;       /* param 19 */
+    double dzextra;        /* param 20 */
+    double dxc, dyc, dzc;  /* param 8 */
+    double wl;             /* param 10 */
+    double aobj_mrad;      /* param 11 */
+    double cs3_mm, cs5_mm; /* param 12 */
+    double asigma;         /* param 13 */
+    int method;            /* param 14: 3, 4 or 5 */
+    int escc;              /* param 15 */
+    double critical_snr;   /* param 17 (method 5) */
+    double epsilon;        /* param 18 */
+    int inv_laplace;       /* param 23 (method 5) */
+    double alpha;          /* param 24 */
+    double norm_f1, norm_f2; /* param 9 */
+} ora_dht_params;
+
+/* Output plane stack of one orientation (DHT.cpp:957-1055): method 3 or 4,
+ * nplanes = nz with escc, 1 (the central plane, copied by the splat) without.
+ * planes: nplanes*ny*nx doubles, already multiplied by `weight`. */
+int ora_dht_planes(const ora_dht_params* p, const float* img, double zout, double weight,
+                   double* planes, int* nplanes);
+
+/* Real-space rotate-and-splat of a plane stack into K3out (nz*ny*nx),
+ * DHT.cpp:1081-1164. */
+int ora_dht_splat(const ora_dht_params* p, const double* planes, int nplanes,
+                  double angZ_deg, double angY_deg, double* K3out);
+
+/* Whole reconstruction (methods 3, 4, 5) including the final normalisation
+ * (DHT.cpp:1325-1416).  vol: nz*ny*nx doubles.  nthreads > 1: orientation-
+ * parallel with private accumulators (timing only). */
+int ora_dht_reconstruct(const ora_dht_params* p, int nrows, const double* angZ_deg,
+                        const double* angY_deg, const double* defocus, const float* weights,
+                        const float* images, double* vol, int nthreads);
+
+/* Volume depth of a DHT reconstruction (DHT.cpp:379). */
+int ora_dht_nz(const ora_dht_params* p);
+
 #ifdef __cplusplus
 }
 #endif

@@ -18,13 +18,13 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def declared_symbols():
     txt = open(os.path.join(ROOT, "include", "utr_hip.h")).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(utr_[a-z0-9_]+)\s*\(", txt)))
+    return sorted(set(re.findall(r"\b((?:utr|dht)_[a-z0-9_]+)\s*\(", txt)))
 
 
 def test_library_exports_every_declared_symbol():
     L = U.load()
     syms = declared_symbols()
-    assert len(syms) >= 15
+    assert len(syms) >= 26
     for s in syms:
         assert hasattr(L, s), s
     assert sorted(U.EXPORTS) == syms
@@ -63,3 +63,25 @@ def test_invalid_parameters_rejected(bad, msg):
     with pytest.raises(U.UTRError, match=msg) as e:
         U.angle_coeffs(p, 0.0, 0.0)
     assert e.value.code == -1
+
+
+def test_dht_param_struct_layout_matches_oracle():
+    assert [f for f, _ in U.DHT_FIELDS] == [f for f, _ in oracle.DHT_FIELDS]
+    assert ctypes.sizeof(U.DhtParams) == ctypes.sizeof(oracle.OraDhtParams)
+
+
+@pytest.mark.parametrize("bad,msg", [
+    (dict(method=1), "IWFR and CTFL2"),
+    (dict(method=7), "unknown phase retrieval method"),
+    (dict(nx=48, ny=48), "integer powers of 2"),
+    (dict(method=4, dzextra=0.0, epsilon=0.0), "epsilon = 0"),
+    (dict(dzextra=0.0), "extra defocus = 0"),
+    (dict(method=4), "extra defocus != 0"),
+    (dict(asigma=0.1), "non-zero absorption coefficient"),
+    (dict(wl=2.0), "wavelength value appears to be wrong"),
+])
+def test_dht_invalid_parameters_rejected(bad, msg):
+    p = U.default_dht_params(**bad)
+    rows = dict(angZ=np.zeros(2), angY=np.array([0.0, 30.0]), defocus=np.full(2, 100.0))
+    with pytest.raises(U.UTRError, match=msg):
+        U.DhtReconstructor(p, rows)

@@ -1,0 +1,220 @@
+"""Pins the FP64 restatement of the DHT program's single-defocus methods
+(oracle/dht_oracle.c, DHT.cpp:700-1416) by closed forms and independent
+constructions:
+  - splat: identity and 90-degree orientations are exact permutations; the
+    splat is the adjoint of trilinear interpolation (scipy) at the same
+    positions; weight conservation;
+  - method 3 planes: zero propagation distance gives 1 - I; free-space
+    propagation conserves sum |psi|^2; a Gaussian spreads as the paraxial
+    closed form;
+  - method 4 planes: one Fourier mode is divided by the weak-object transfer
+    function T/(2(T^2 + eps));
+  - reconstruct: the driver equals the composition of the pinned pieces with
+    the scales of DHT.cpp:1325-1340; method 5 equals the UTR oracle's pieces
+    with the Filt3 filter of DHT.cpp:1350-1365.
+"""
+import numpy as np
+import pytest
+from scipy import ndimage
+
+import oracle
+import utr_public_amd as U
+
+
+def params(**kw):
+    d = U.default_dht_params(nx=16, ny=16, xst=0.5, zlo=-4.0, zhi=4.0, dzextra=50.0)
+    d.update(kw)
+    return d
+
+
+def positions(p, angZ, angY):
+    """Grid positions (dz1, dy1, dx1) of every sample (n, j, i): the rotation of
+    DHT.cpp:1108-1147 written as a matrix product."""
+    nx, ny, xst = p["nx"], p["ny"], p["xst"]
+    nz = oracle.dht_nz(p)
+    xlo, ylo = -0.5 * xst * nx, -0.5 * xst * ny
+    zlo, zhi = sorted((p["zlo"], p["zhi"]))
+    zlodz, zhidz = zlo + p["dzextra"], zhi + p["dzextra"]
+    xc = (xlo + (xlo + xst * nx)) / 2 + p["dxc"]
+    yc = (ylo + (ylo + xst * ny)) / 2 + p["dyc"]
+    zc = (zlodz + zhidz) / 2 + p["dzc"]
+    n, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    X, Y, Z = xlo + xst * i - xc, ylo + xst * j - yc, zlodz + xst * n - zc
+    aY, aZ = np.radians(angY), np.radians(angZ)
+    Ry = np.array([[np.cos(aY), 0, -np.sin(aY)], [0, 1, 0], [np.sin(aY), 0, np.cos(aY)]])
+    Rz = np.array([[np.cos(aZ), np.sin(aZ), 0], [-np.sin(aZ), np.cos(aZ), 0], [0, 0, 1]])
+    v = np.stack([X, Y, Z], -1) @ (Rz @ Ry).T
+    return (v[..., 2] + zc - zlodz) / xst, (v[..., 1] + yc - ylo) / xst, (v[..., 0] + xc - xlo) / xst
+
+
+def test_splat_identity_orientation():
+    p = params()
+    nz = oracle.dht_nz(p)
+    P = np.random.default_rng(0).standard_normal((nz, 16, 16))
+    K = oracle.dht_splat(p, P, 0.0, 0.0)
+    assert np.allclose(K[1:-2, 1:-2, 1:-2], P[1:-2, 1:-2, 1:-2], rtol=0, atol=1e-12)
+
+
+def test_splat_90_degrees_is_a_permutation():
+    # rotation about y by 90 degrees: x' = -z, z' = x (in centred grid units)
+    p = params(zlo=-4.0, zhi=4.0)
+    nz = oracle.dht_nz(p)
+    assert nz == 16
+    P = np.random.default_rng(1).standard_normal((nz, 16, 16))
+    K = oracle.dht_splat(p, P, 0.0, 90.0)
+    for n in range(2, 14):
+        for i in range(2, 14):
+            assert np.allclose(K[i, 2:14, 16 - n], P[n, 2:14, i], atol=1e-9)
+
+
+@pytest.mark.parametrize("angZ,angY,shift", [(0.0, 23.0, 0.0), (37.0, 61.0, 0.0), (121.0, 17.0, 0.7)])
+def test_splat_is_adjoint_of_trilinear_interpolation(angZ, angY, shift):
+    p = params(dxc=shift, dyc=-shift, dzc=0.5 * shift)
+    nz = oracle.dht_nz(p)
+    rng = np.random.default_rng(2)
+    P = rng.standard_normal((nz, 16, 16))
+    V = rng.standard_normal((nz, 16, 16))
+    dz, dy, dx = positions(p, angZ, angY)
+    # keep samples whose truncated cell is a proper interior cell (there
+    # truncation is floor and every corner exists)
+    ok = np.ones(dz.shape, bool)
+    for d, n in ((dz, nz), (dy, 16), (dx, 16)):
+        ok &= (d >= 0) & (np.floor(d) <= n - 2)
+        fr = d - np.floor(d)
+        ok &= (fr > 1e-9) & (fr < 1 - 1e-9)  # away from ties
+    P = np.where(ok, P, 0.0)
+    K = oracle.dht_splat(p, P, angZ, angY)
+    interp = ndimage.map_coordinates(V, [dz[ok], dy[ok], dx[ok]], order=1, mode="nearest")
+    lhs = float(np.sum(K * V))
+    rhs = float(np.sum(P[ok] * interp))
+    assert abs(lhs - rhs) < 1e-9 * (abs(rhs) + 1)
+    # conservation: each accepted sample distributes exactly its value
+    assert np.isclose(K.sum(), P.sum(), rtol=1e-12, atol=1e-9)
+
+
+def test_method3_zero_distance_plane_is_one_minus_intensity():
+    p = params(epsilon=0.7)
+    nz = oracle.dht_nz(p)
+    rng = np.random.default_rng(3)
+    I = (1.0 + 0.1 * rng.standard_normal((16, 16))).astype(np.float32)
+    k = 5
+    zout = (-4.0 + p["dzextra"]) + p["xst"] * k  # voutdefocus[k]
+    P = oracle.dht_planes(p, I, zout)
+    assert P.shape == (nz, 16, 16)
+    assert np.allclose(P[k], 1.0 - I.astype(np.float64), atol=1e-12)
+    # no aperture: free-space propagation conserves sum |psi|^2 on every plane
+    tot = float(I.astype(np.float64).sum())
+    for q in range(nz):
+        assert np.isclose(np.sum(1.0 - P[q]), tot, rtol=1e-12)
+
+
+def test_method3_flat_ewald_is_the_central_plane():
+    p = params(escc=0)
+    I = (1.0 + 0.1 * np.random.default_rng(4).standard_normal((16, 16))).astype(np.float32)
+    P1 = oracle.dht_planes(p, I, 60.0)
+    Pn = oracle.dht_planes(params(), I, 60.0)
+    assert P1.shape == (1, 16, 16)
+    assert np.allclose(P1[0], Pn[Pn.shape[0] // 2], atol=1e-14)
+
+
+def test_method3_gaussian_spreads_as_the_paraxial_closed_form():
+    n, xst, wl = 64, 0.4, 0.02507934
+    p = params(nx=n, ny=n, xst=xst, zlo=-12.8, zhi=12.8, dzextra=3000.0, epsilon=0.0, wl=wl)
+    x = (np.arange(n) - n // 2) * xst
+    s = 2.0
+    r2 = x[None, :] ** 2 + x[:, None] ** 2
+    I = np.exp(-r2 / s ** 2)
+    # the centred Gaussian sits at index n/2; roll it to index 0 so that the
+    # natural-order DFT sees a symmetric field
+    I0 = np.roll(I, (-(n // 2), -(n // 2)), (0, 1)).astype(np.float32)
+    zout = 3000.0 + 12.8  # the last plane is closest; plane 0 is 25.6 A further
+    P = oracle.dht_planes(p, I0, zout)
+    for q in (0, 10, 40):
+        d = (-12.8 + 3000.0 + xst * q) - zout
+        sd2 = s ** 2 * (1 + (wl * d / (2 * np.pi * s ** 2)) ** 2)
+        expect = (s ** 2 / sd2) * np.exp(-r2 / sd2)
+        got = np.roll(1.0 - P[q], (n // 2, n // 2), (0, 1))
+        assert np.allclose(got, expect, atol=2e-6), (q, np.abs(got - expect).max())
+
+
+@pytest.mark.parametrize("k,asigma,eps", [(3, 0.0, 1e-3), (5, 0.08, 1e-2)])
+def test_method4_single_mode_is_divided_by_the_transfer_function(k, asigma, eps):
+    n, xst, wl = 32, 0.5, 0.02507934
+    p = params(nx=n, ny=n, xst=xst, zlo=-8.0, zhi=8.0, dzextra=0.0, method=4, asigma=asigma,
+               epsilon=eps)
+    x = np.arange(n) * xst
+    A = 0.02
+    f = k / (n * xst)
+    I = (1.0 + A * np.cos(2 * np.pi * f * x))[None, :].repeat(n, 0).astype(np.float32)
+    zout = 300.0
+    P = oracle.dht_planes(p, I, zout)
+    G = np.fft.fft2(I.astype(np.float64) - 1.0)
+    for q in (0, 7, 31):
+        z = -8.0 + xst * q
+        chi = np.pi * wl * (zout - z) * f * f
+        T = np.sin(chi) - asigma * np.cos(chi)
+        F = np.fft.fft2(P[q])
+        # the mode is scaled by T/(2(T^2 + eps)) (the float32 rounding noise of
+        # the input sits in the other frequencies)
+        assert np.isclose((F[0, k] / G[0, k]).real, T / (2 * (T * T + eps)), rtol=1e-12)
+        assert abs((F[0, k] / G[0, k]).imag) < 1e-12
+        assert np.isclose(F[0, n - k] / G[0, n - k], F[0, k] / G[0, k], rtol=1e-12)
+
+
+def _rows(nrows, seed, general=False):
+    rng = np.random.default_rng(seed)
+    angZ = rng.uniform(0, 360, nrows) if general else np.zeros(nrows)
+    return dict(angZ=angZ, angY=rng.uniform(0, 180, nrows), defocus=rng.uniform(40, 80, nrows))
+
+
+@pytest.mark.parametrize("method,escc", [(3, 1), (3, 0), (4, 1), (4, 0)])
+def test_reconstruct_is_the_composition_with_the_reference_scales(method, escc):
+    p = params(method=method, escc=escc, dzextra=50.0 if method == 3 else 0.0,
+               asigma=0.05 if method == 4 else 0.0, epsilon=0.01 if method == 4 else 0.8)
+    rows = _rows(3, 5, general=True)
+    rng = np.random.default_rng(6)
+    imgs = (1.0 + 0.05 * rng.standard_normal((3, 16, 16))).astype(np.float32)
+    w = np.array([0.5, 1.0, 1.5], np.float32)
+    vol = oracle.dht_reconstruct(p, rows, imgs, weights=w)
+    K = None
+    for r in range(3):
+        P = oracle.dht_planes(p, imgs[r], rows["defocus"][r], float(w[r]))
+        K = oracle.dht_splat(p, P, rows["angZ"][r], rows["angY"][r], K)
+    g = oracle.derive(U.default_params(nx=16, ny=16, xst=0.5, zlo=-4, zhi=4, wl=p["wl"]),
+                      dict(angZ=[0.0], angY=[0.0], defocus=[0.0]))
+    EE, wl = g.EE, p["wl"]
+    if method == 3:
+        dnorm = EE * 0.1 / 1.0 * (-wl / np.pi) / float(w.astype(np.float64).sum())
+    else:
+        dnorm = wl * EE / np.pi / (1.0 * 8.0) / float(w.astype(np.float64).sum())
+    assert np.allclose(vol, K * dnorm, rtol=1e-12, atol=1e-14 * np.abs(vol).max())
+
+
+def test_sctf_is_the_ewald_gridding_with_the_filt3_filter():
+    p = params(nx=16, ny=16, method=5, dzextra=0.0, asigma=0.07, epsilon=0.01, critical_snr=0.4,
+               escc=1)
+    rows = _rows(6, 7)
+    imgs = (1.0 + 0.05 * np.random.default_rng(8).standard_normal((6, 16, 16))).astype(np.float32)
+    vol = oracle.dht_reconstruct(p, rows, imgs)
+    q = U.default_params(nx=16, ny=16, xst=0.5, zlo=-4.0, zhi=4.0, wl=p["wl"], ctf_mode=3,
+                         escc=1, asigma=0.0, epsilon=1.0)
+    g = oracle.derive(q, rows)
+    g.asigma, g.epsilon = 0.07, 0.01
+    V = np.zeros((g.nz, 16, 16), np.complex128)
+    S = np.zeros((g.nz, 16, 16))
+    for r in range(6):
+        K = oracle.spectrum(g, imgs[r], zout=rows["defocus"][r])
+        V, S = oracle.grid(g, K, 0.0, rows["angY"][r], V, S)
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+        f = np.where(S != 0, (1 / S) * np.exp((S - 0.4) / S ** 4), 1.0)
+    V = V * f
+    V = oracle.hermitian(g, V)
+    out = oracle.c2r(g, V) * (p["wl"] * g.EE / (2 * np.pi * np.sqrt(1 + 0.07 ** 2)))
+    assert np.allclose(vol, out, rtol=1e-10, atol=1e-12 * np.abs(out).max())
+
+
+def test_invalid_parameters():
+    with pytest.raises(oracle.OracleError, match="extra defocus = 0"):
+        oracle.dht_nz(params(dzextra=0.0))
+    with pytest.raises(oracle.OracleError, match="IWFR"):
+        oracle.dht_nz(params(method=2))

@@ -36,11 +36,28 @@ EXPORTS = [
     "utr_finalize_device", "utr_finalize_f64", "utr_keep_sampling", "utr_get_sampling",
     "utr_reset", "utr_dims", "utr_get_timings", "utr_destroy", "utr_last_error",
     "utr_abi_version", "utr_angle_coeffs", "utr_get_spectrum", "utr_get_fourier_volume",
+    "dht_create", "dht_add_images", "dht_add_images_device", "dht_finalize", "dht_finalize_device",
+    "dht_reset", "dht_dims", "dht_get_timings", "dht_destroy",
+]
+
+DHT_FIELDS = [
+    ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("xst", ctypes.c_double),
+    ("zlo", ctypes.c_double), ("zhi", ctypes.c_double), ("dzextra", ctypes.c_double),
+    ("dxc", ctypes.c_double), ("dyc", ctypes.c_double), ("dzc", ctypes.c_double),
+    ("wl", ctypes.c_double), ("aobj_mrad", ctypes.c_double), ("cs3_mm", ctypes.c_double),
+    ("cs5_mm", ctypes.c_double), ("asigma", ctypes.c_double), ("method", ctypes.c_int),
+    ("escc", ctypes.c_int), ("critical_snr", ctypes.c_double), ("epsilon", ctypes.c_double),
+    ("inv_laplace", ctypes.c_int), ("alpha", ctypes.c_double),
+    ("norm_f1", ctypes.c_double), ("norm_f2", ctypes.c_double),
 ]
 
 
 class UtrParams(ctypes.Structure):
     _fields_ = PARAM_FIELDS
+
+
+class DhtParams(ctypes.Structure):
+    _fields_ = DHT_FIELDS
 
 
 class UtrTimings(ctypes.Structure):
@@ -60,6 +77,17 @@ def default_params(**kw):
              dxc=0.0, dyc=0.0, dzc=0.0, wl=0.02507934, aobj_mrad=0.0, cs3_mm=0.0,
              cs5_mm=0.0, ctf_mode=0, escc=1, asigma=0.0, nsr=0.0, epsilon=1e-4,
              inv_laplace=0, norm_f1=1.0, norm_f2=0.0)
+    d.update(kw)
+    return d
+
+
+def default_dht_params(**kw):
+    """DHT parameter set: method 3 (Min05LogAmp + Fresnel back-propagation)
+    on a 64^2 x 64 grid at 0.4 A, 200 keV, 100 A extra defocus."""
+    d = dict(nx=64, ny=64, xst=0.4, zlo=-12.8, zhi=12.8, dzextra=100.0, dxc=0.0, dyc=0.0,
+             dzc=0.0, wl=0.02507934, aobj_mrad=0.0, cs3_mm=0.0, cs5_mm=0.0, asigma=0.0,
+             method=3, escc=1, critical_snr=0.5, epsilon=1.0, inv_laplace=0, alpha=0.0,
+             norm_f1=1.0, norm_f2=0.0)
     d.update(kw)
     return d
 
@@ -110,6 +138,17 @@ def load():
     L.utr_angle_coeffs.argtypes = [ctypes.POINTER(UtrParams), ctypes.c_double, ctypes.c_double, dp]
     L.utr_get_spectrum.argtypes = [vp, ctypes.c_int, vp]
     L.utr_get_fourier_volume.argtypes = [vp, vp]
+    L.dht_create.argtypes = [ctypes.POINTER(DhtParams), ctypes.c_int, dp, dp, dp, vp,
+                             ctypes.POINTER(vp)]
+    for f in ("dht_add_images", "dht_add_images_device"):
+        getattr(L, f).argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
+    L.dht_finalize.argtypes = [vp, vp]
+    L.dht_finalize_device.argtypes = [vp, vp]
+    L.dht_reset.argtypes = [vp]
+    L.dht_dims.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                           ctypes.POINTER(ctypes.c_int)]
+    L.dht_get_timings.argtypes = [vp, ctypes.POINTER(UtrTimings)]
+    L.dht_destroy.argtypes = [vp]
     _lib = L
     return L
 
@@ -238,6 +277,83 @@ def reconstruct(params, rows, images, precision=UTR_FP32, f64_out=False):
         r.add_images(images)
         if f64_out:
             return r.finalize_f64()
+        return r.finalize()
+
+
+class DhtReconstructor:
+    """One reconstruction of the DHT program's single-defocus methods (3, 4,
+    5; MultisliceCppTest/PhaseRetrieval/DHT.cpp) on the current HIP device.
+    rows: angZ, angY (degrees), defocus (Angstrom), optional weights."""
+
+    def __init__(self, params, rows):
+        L = load()
+        self._L = L
+        self.params = dict(params)
+        self.angZ = np.ascontiguousarray(rows["angZ"], dtype=np.float64)
+        self.angY = np.ascontiguousarray(rows["angY"], dtype=np.float64)
+        self.defocus = np.ascontiguousarray(rows["defocus"], dtype=np.float64)
+        self.weights = None
+        if rows.get("weights") is not None:
+            self.weights = np.ascontiguousarray(rows["weights"], dtype=np.float32)
+        p = DhtParams()
+        for name, _ in DHT_FIELDS:
+            setattr(p, name, self.params[name])
+        ctx = ctypes.c_void_p()
+        _check(L.dht_create(ctypes.byref(p), len(self.angZ), _dp(self.angZ), _dp(self.angY),
+                            _dp(self.defocus),
+                            self.weights.ctypes.data if self.weights is not None else None,
+                            ctypes.byref(ctx)))
+        self._ctx = ctx
+        nx, ny, nz = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(L.dht_dims(ctx, ctypes.byref(nx), ctypes.byref(ny), ctypes.byref(nz)))
+        self.shape = (nz.value, ny.value, nx.value)
+
+    def add_images(self, images, first_row=0):
+        images = np.ascontiguousarray(images, dtype=np.float32)
+        assert images.shape[1:] == self.shape[1:], (images.shape, self.shape)
+        _check(self._L.dht_add_images(self._ctx, first_row, images.shape[0], images.ctypes.data))
+
+    def add_images_device(self, ptr, count, first_row=0):
+        _check(self._L.dht_add_images_device(self._ctx, first_row, count, ctypes.c_void_p(ptr)))
+
+    def finalize(self):
+        out = np.empty(self.shape, dtype=np.float32)
+        _check(self._L.dht_finalize(self._ctx, out.ctypes.data))
+        return out
+
+    def finalize_device(self, ptr):
+        _check(self._L.dht_finalize_device(self._ctx, ctypes.c_void_p(ptr)))
+
+    def reset(self):
+        _check(self._L.dht_reset(self._ctx))
+
+    def timings(self):
+        t = UtrTimings()
+        _check(self._L.dht_get_timings(self._ctx, ctypes.byref(t)))
+        return {k: getattr(t, k) for k, _ in t._fields_}
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            self._L.dht_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def dht_reconstruct(params, rows, images):
+    """Convenience: one DHT reconstruction with host input and output."""
+    with DhtReconstructor(params, rows) as r:
+        r.add_images(images)
         return r.finalize()
 
 

@@ -1,0 +1,805 @@
+// dht.hip -- the single-defocus reconstructions of the DHT program on gfx950
+// (MultisliceCppTest/PhaseRetrieval/DHT.cpp), behind the dht_* C ABI of
+// include/utr_hip.h.  Included by utr_api.hip (one translation unit).
+//
+// Methods 3 and 4 (real-space path):
+//   k_dht_rows / k_dht_cols    phase-retrieval input and forward 2D FFT of each
+//                              orientation (DHT.cpp:850-947)
+//   k_dht_prop_rows / _cols    per output plane: Fresnel propagator (method 3)
+//                              or weak-object CTF inversion (method 4), inverse
+//                              2D FFT, 1-|psi|^2 or Re (DHT.cpp:957-1055); phase
+//                              arguments in FP64
+//   k_dht_splat                real-space inverse rotation and trilinear splat
+//                              with (int) truncation (DHT.cpp:1081-1164): one
+//                              workgroup per 16x8x8 brick, exact 64-bit
+//                              fixed-point sums in LDS, coordinates evaluated
+//                              in FP64 in the reference's operation order
+//   k_dht_finish               scale of DHT.cpp:1325-1340
+// Orientations are processed in batches whose plane stacks fit a memory
+// budget; each batch adds its exact brick sums to an FP64 volume.
+//
+// Method 5 (sCTF) runs the UTR Ewald-sphere kernels with the DHT voxel
+// filter, no sphere mask and the DHT scale (DHT.cpp:1056-1067, 1195-1416).
+
+namespace utr {
+
+struct DhtGeom {
+    int nx, ny, nz, nplanes, method, escc, use_f1, use_f2;
+    float f1, f2;
+    double xst, wl, k2maxo, c3, c5, asigma, epsilon, zlodz;
+    double xlo, ylo, xc, yc, zc;
+    int nx2, ny2, nz2;
+};
+
+// One orientation of the DHT path.
+struct alignas(16) DhtRow {
+    double sinY, cosY, sinZ, cosZ;
+    double zout;
+    float weight;
+    float c0[3];   // grid position (dx1, dy1, dz1) of source sample (i, j, n) = (0, 0, 0)
+    float ui[3];   // its derivative along i, j, n
+    float uj[3];
+    float un[3];
+    float inv[9];  // inverse of the 3x3 map [ui uj un]
+    int pad;
+};
+
+constexpr int DHT_BX = 16, DHT_BY = 8, DHT_BZ = 8;
+constexpr int DHT_PY = DHT_BX + 1, DHT_PZ = DHT_PY * (DHT_BY + 1), DHT_PAD = DHT_PZ * DHT_BZ;
+constexpr int DHT_VOX = DHT_BX * DHT_BY * DHT_BZ;
+
+__device__ __forceinline__ double dht_freq(int k, int n, double xst) {
+    return (double)(k <= n / 2 ? k : k - n) / ((double)n * xst);
+}
+
+// Image rows: renormalise and clamp (DHT.cpp:850-858), the field of the
+// method (3: sqrt(I) exp(i Fact (-0.5)(I - 1)) - 1; 4: I - 1) and forward FFT
+// along x in natural order.
+template <int N>
+__global__ void __launch_bounds__(256) k_dht_rows(const float* __restrict__ img,
+                                                  cplx<float>* __restrict__ out, int ny, DhtGeom g,
+                                                  const cplx<float>* __restrict__ tw) {
+    constexpr int NB = lds_batch(N, sizeof(cplx<float>));
+    __shared__ cplx<float> sm[2 * NB * N];
+    cplx<float>* a = sm;
+    cplx<float>* b = sm + NB * N;
+    const int row = blockIdx.y;
+    const int j0 = blockIdx.x * NB;
+    const int nb = min(NB, ny - j0);
+    const float* src = img + ((size_t)row * ny + j0) * N;
+    for (int t = threadIdx.x; t < nb * N; t += blockDim.x) {
+        double x = (double)src[t];
+        if (g.use_f1) x = x / (double)g.f1;
+        if (g.use_f2) x = x + (double)g.f2;
+        if (x < 0.0) x = 0.0;
+        if (g.method == 3) {
+            // the scattered part u = psi - 1 is carried instead of psi (the
+            // propagators leave the constant 1 unchanged), so that 1 - |psi|^2
+            // = -2 Re u - |u|^2 keeps its relative precision in FP32
+            const double amp = sqrt(x), phi = g.epsilon * (-0.5) * (x - 1.0);
+            double s, c;
+            sincos(phi, &s, &c);
+            a[t] = {(float)(amp * c - 1.0), (float)(amp * s)};
+        } else {
+            a[t] = {(float)(x - 1.0), 0.f};
+        }
+    }
+    __syncthreads();
+    fft_lds<float, N>(a, b, nb, tw, -1, threadIdx.x, blockDim.x);
+    cplx<float>* dst = out + ((size_t)row * ny + j0) * N;
+    for (int t = threadIdx.x; t < nb * N; t += blockDim.x) dst[t] = a[t];
+}
+
+// Forward (sign -1) or inverse (+1) FFT along y of `count` ny x nx arrays, in place.
+template <int N>
+__global__ void __launch_bounds__(256) k_dht_cols(cplx<float>* __restrict__ data, int nx,
+                                                  const cplx<float>* __restrict__ tw, int sign) {
+    constexpr int NB = lds_batch(N, sizeof(cplx<float>));
+    __shared__ cplx<float> sm[2 * NB * N];
+    cplx<float>* a = sm;
+    cplx<float>* b = sm + NB * N;
+    const int x0 = blockIdx.x * NB;
+    const int nb = min(NB, nx - x0);
+    cplx<float>* base = data + (size_t)blockIdx.y * N * nx + x0;
+    for (int t = threadIdx.x; t < nb * N; t += blockDim.x) {
+        const int y = t / nb, c = t - y * nb;
+        a[c * N + y] = base[(size_t)y * nx + c];
+    }
+    __syncthreads();
+    fft_lds<float, N>(a, b, nb, tw, sign, threadIdx.x, blockDim.x);
+    for (int t = threadIdx.x; t < nb * N; t += blockDim.x) {
+        const int y = t / nb, c = t - y * nb;
+        base[(size_t)y * nx + c] = a[c * N + y];
+    }
+}
+
+// Per output plane p of orientation a0+blockIdx.z: spectrum row times the
+// plane's filter, inverse FFT along x.  grid (rows/NB, nplanes, na).
+template <int N>
+__global__ void __launch_bounds__(256) k_dht_prop_rows(const cplx<float>* __restrict__ S,
+                                                       cplx<float>* __restrict__ T, DhtGeom g,
+                                                       const DhtRow* __restrict__ rows, int a0,
+                                                       const cplx<float>* __restrict__ tw) {
+    constexpr int NB = lds_batch(N, sizeof(cplx<float>));
+    __shared__ cplx<float> sm[2 * NB * N];
+    cplx<float>* a = sm;
+    cplx<float>* b = sm + NB * N;
+    const int ny = g.ny;
+    const int p = blockIdx.y, ab = blockIdx.z;
+    const int k0 = blockIdx.x * NB;
+    const int nb = min(NB, ny - k0);
+    const DhtRow& R = rows[a0 + ab];
+    const int n = g.escc ? p : g.nz / 2;
+    const double zn = g.zlodz + g.xst * n;  // voutdefocus[n], DHT.cpp:388
+    const double d = g.method == 3 ? zn - R.zout : R.zout - zn;
+    const double pi = 3.14159265358979323846, twopi = 2.0 * pi;
+    const double f2 = pi * g.wl * d, f4 = pi * g.wl * g.wl * g.wl * g.c3 / 2.0,
+                 f6 = pi * pow(g.wl, 5) * g.c5 / 3.0;
+    const cplx<float>* src = S + ((size_t)(a0 + ab) * ny + k0) * N;
+    for (int t = threadIdx.x; t < nb * N; t += blockDim.x) {
+        const int ky = k0 + t / N, kx = t % N;
+        const double fx = dht_freq(kx, N, g.xst), fy = dht_freq(ky, ny, g.xst);
+        const double q2 = fx * fx + fy * fy;
+        float fr = 0.f, fi = 0.f;
+        if (!(g.k2maxo > 0) || q2 < g.k2maxo) {
+            double chi = f2 * q2 + f4 * q2 * q2 + f6 * q2 * q2 * q2;
+            chi -= twopi * rint(chi / twopi);  // FP64 reduction, then FP32 sincos
+            float s, c;
+            sincosf((float)chi, &s, &c);
+            if (g.method == 3) {
+                fr = c;
+                fi = -s;
+            } else {
+                const float tt = s - (float)g.asigma * c;
+                fr = tt / (2.f * (tt * tt + (float)g.epsilon));
+            }
+        }
+        const cplx<float> v = src[t];
+        a[t] = {v.x * fr - v.y * fi, v.x * fi + v.y * fr};
+    }
+    __syncthreads();
+    fft_lds<float, N>(a, b, nb, tw, +1, threadIdx.x, blockDim.x);
+    cplx<float>* dst = T + (((size_t)ab * g.nplanes + p) * ny + k0) * N;
+    for (int t = threadIdx.x; t < nb * N; t += blockDim.x) dst[t] = a[t];
+}
+
+// Inverse FFT along y of each plane, then 1 - |psi|^2 = -2 Re u - |u|^2
+// (method 3) or the real part (method 4), times the orientation's weight.  grid (nx/NB, nplanes, na).
+// pmax: max |value| of the batch as float bits.
+template <int N>
+__global__ void __launch_bounds__(256) k_dht_prop_cols(const cplx<float>* __restrict__ T,
+                                                       float* __restrict__ P, DhtGeom g,
+                                                       const DhtRow* __restrict__ rows, int a0,
+                                                       const cplx<float>* __restrict__ tw,
+                                                       unsigned* __restrict__ pmax) {
+    constexpr int NB = lds_batch(N, sizeof(cplx<float>));
+    __shared__ cplx<float> sm[2 * NB * N];
+    __shared__ float s_wmax[4];
+    cplx<float>* a = sm;
+    cplx<float>* b = sm + NB * N;
+    const int nx = g.nx;
+    const int p = blockIdx.y, ab = blockIdx.z;
+    const int x0 = blockIdx.x * NB;
+    const int nb = min(NB, nx - x0);
+    const size_t plane = ((size_t)ab * g.nplanes + p) * N * nx;
+    const cplx<float>* base = T + plane + x0;
+    for (int t = threadIdx.x; t < nb * N; t += blockDim.x) {
+        const int y = t / nb, c = t - y * nb;
+        a[c * N + y] = base[(size_t)y * nx + c];
+    }
+    __syncthreads();
+    fft_lds<float, N>(a, b, nb, tw, +1, threadIdx.x, blockDim.x);
+    const float inv = (float)(1.0 / ((double)nx * N));
+    const float w = rows[a0 + ab].weight;
+    float vmax = 0.f;
+    for (int t = threadIdx.x; t < nb * N; t += blockDim.x) {
+        const int y = t / nb, c = t - y * nb;
+        const cplx<float> z = a[c * N + y];
+        const float re = z.x * inv, im = z.y * inv;
+        const float v = (g.method == 3 ? -2.f * re - (re * re + im * im) : re) * w;
+        P[plane + (size_t)y * nx + x0 + c] = v;
+        vmax = fmaxf(vmax, fabsf(v));
+    }
+    for (int off = 32; off > 0; off >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, off));
+    if ((threadIdx.x & 63) == 0) s_wmax[threadIdx.x >> 6] = vmax;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        atomicMax(pmax, __float_as_uint(fmaxf(fmaxf(s_wmax[0], s_wmax[1]), fmaxf(s_wmax[2], s_wmax[3]))));
+}
+
+__device__ __forceinline__ int dht_lds(int lx, int ly, int lz) { return lx + DHT_PY * ly + DHT_PZ * lz; }
+
+// Splat of the batch's plane stacks (orientations a0 .. a0+na-1) into one
+// brick, added to the FP64 volume acc3 (the brick's own voxels only).
+// xs/ys/zs: per orientation (x_i - xc) sinY, (x_i - xc) cosY; (y_j - yc) sinZ,
+// (y_j - yc) cosZ; (z_n - zc) sinY, (z_n - zc) cosY, as DHT.cpp:1083-1107.
+__global__ void __launch_bounds__(256) k_dht_splat(const float* __restrict__ P, DhtGeom g,
+                                                   const DhtRow* __restrict__ rows,
+                                                   const double* __restrict__ xs,
+                                                   const double* __restrict__ ys,
+                                                   const double* __restrict__ zs, int a0, int na,
+                                                   const unsigned* __restrict__ pmax,
+                                                   double* __restrict__ acc3) {
+    __shared__ unsigned long long acc[DHT_PAD];
+    const int tid = threadIdx.x;
+    const int nbx = (g.nx + DHT_BX - 1) / DHT_BX, nby = (g.ny + DHT_BY - 1) / DHT_BY;
+    const int bid = blockIdx.x;
+    const int bx = bid % nbx, by = (bid / nbx) % nby, bz = bid / (nbx * nby);
+    const int x0 = bx * DHT_BX, y0 = by * DHT_BY, z0 = bz * DHT_BZ;
+    for (int t = tid; t < DHT_PAD; t += 256) acc[t] = 0ull;
+    // fixed-point scale: per orientation a voxel takes at most 27 samples with
+    // |weight| <= 8 (truncation allows dz1 in (-1, 0)), so |sum| < 2^8 na max|P|
+    int ek = 0, ea = 0;
+    const float mk = __uint_as_float(*pmax);
+    frexpf(mk > 0.f ? mk : 1.f, &ek);
+    frexpf((float)na, &ea);
+    const int s = min(53 - ek - ea, 47 - ek);
+    const double scale = ldexp(1.0, s);
+    __syncthreads();
+
+    const int np_stride = g.nplanes;
+    const size_t n2 = (size_t)g.nx * g.ny;
+    for (int ab = 0; ab < na; ab++) {
+        const DhtRow& R = rows[a0 + ab];
+        // source-index box of the brick's preimage: samples whose truncated
+        // cell can touch the brick lie in the open box (x0-1, x0+BX) etc.
+        float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
+        const float bl[3] = {(float)x0 - 1.f, (float)y0 - 1.f, (float)z0 - 1.f};
+        const float bh[3] = {(float)(x0 + DHT_BX), (float)(y0 + DHT_BY), (float)(z0 + DHT_BZ)};
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            const float q[3] = {((c & 1) ? bh[0] : bl[0]) - R.c0[0], ((c & 2) ? bh[1] : bl[1]) - R.c0[1],
+                                ((c & 4) ? bh[2] : bl[2]) - R.c0[2]};
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const float v = R.inv[3 * d] * q[0] + R.inv[3 * d + 1] * q[1] + R.inv[3 * d + 2] * q[2];
+                lo[d] = fminf(lo[d], v);
+                hi[d] = fmaxf(hi[d], v);
+            }
+        }
+        // lo/hi over (i, j, n); one sample of margin
+        const int jlo = max((int)floorf(lo[1]) - 1, 0), jhi = min((int)ceilf(hi[1]) + 1, g.ny - 1);
+        const int nlo = max((int)floorf(lo[2]) - 1, 0), nhi = min((int)ceilf(hi[2]) + 1, g.nz - 1);
+        const int ilo0 = max((int)floorf(lo[0]) - 1, 0), ihi0 = min((int)ceilf(hi[0]) + 1, g.nx - 1);
+        if (jlo > jhi || nlo > nhi || ilo0 > ihi0) continue;
+        const int nj = jhi - jlo + 1;
+        const int nlines = nj * (nhi - nlo + 1);
+        const double* X = xs + (size_t)(a0 + ab) * 2 * g.nx;
+        const double* Y = ys + (size_t)(a0 + ab) * 2 * g.ny;
+        const double* Z = zs + (size_t)(a0 + ab) * 2 * g.nz;
+        for (int L = tid; L < nlines; L += 256) {
+            const int n = nlo + L / nj, j = jlo + L % nj;
+            // i-interval of this line inside the box (slab test, one sample of margin)
+            float ilo = (float)ilo0, ihi = (float)ihi0;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const float base = R.c0[d] + R.uj[d] * j + R.un[d] * n;
+                const float u = R.ui[d];
+                const float l = bl[d] - 0.05f - base, h = bh[d] + 0.05f - base;
+                if (fabsf(u) < 1e-6f) {
+                    if (l > 0.f || h < 0.f) ihi = ilo - 1.f;
+                } else {
+                    float t0 = l / u, t1 = h / u;
+                    if (u < 0.f) { const float tt = t0; t0 = t1; t1 = tt; }
+                    ilo = fmaxf(ilo, floorf(t0) - 1.f);
+                    ihi = fminf(ihi, ceilf(t1) + 1.f);
+                }
+            }
+            ilo = fminf(ilo, (float)ihi0 + 1.f);  // keep the conversions in range
+            ihi = fmaxf(ihi, (float)ilo0 - 1.f);
+            const int i0 = (int)ilo, i1 = (int)ihi;
+            const float* Kp = P + ((size_t)ab * np_stride + (g.escc ? n : 0)) * n2 + (size_t)j * g.nx;
+            const double zsin = Z[2 * n], zcos = Z[2 * n + 1];
+            const double ysin = Y[2 * j], ycos = Y[2 * j + 1];
+            for (int i = i0; i <= i1; i++) {
+                // DHT.cpp:1119-1150, operation by operation
+                const double zzz = __dadd_rn(__dadd_rn(g.zc, zcos), X[2 * i]);
+                double dz1 = __ddiv_rn(__dsub_rn(zzz, g.zlodz), g.xst);
+                const int nn = (int)dz1;
+                if (nn < 0 || nn > g.nz2) continue;
+                dz1 = __dsub_rn(dz1, (double)nn);
+                const double dz0 = __dsub_rn(1.0, dz1);
+                const double xx = __dadd_rn(-zsin, X[2 * i + 1]);
+                const double xx_sinZ = __dmul_rn(xx, R.sinZ), xx_cosZ = __dmul_rn(xx, R.cosZ);
+                const double xxx = __dadd_rn(__dadd_rn(g.xc, ysin), xx_cosZ);
+                double dx1 = __ddiv_rn(__dsub_rn(xxx, g.xlo), g.xst);
+                const int ii = (int)dx1;
+                if (ii < 0 || ii > g.nx2) continue;
+                dx1 = __dsub_rn(dx1, (double)ii);
+                const double dx0 = __dsub_rn(1.0, dx1);
+                const double yyy = __dsub_rn(__dadd_rn(g.yc, ycos), xx_sinZ);
+                double dy1 = __ddiv_rn(__dsub_rn(yyy, g.ylo), g.xst);
+                const int jj = (int)dy1;
+                if (jj < 0 || jj > g.ny2) continue;
+                dy1 = __dsub_rn(dy1, (double)jj);
+                const double dy0 = __dsub_rn(1.0, dy1);
+                const int lx = ii - x0, ly = jj - y0, lz = nn - z0;
+                if (lx < -1 || lx >= DHT_BX || ly < -1 || ly >= DHT_BY || lz < -1 || lz >= DHT_BZ) continue;
+                const double k = (double)Kp[i];
+                const double dz0K = dz0 * k, dz1K = dz1 * k;
+                const double wx[2] = {dx0, dx1}, wy[2] = {dy0, dy1}, wz[2] = {dz0K, dz1K};
+#pragma unroll
+                for (int c = 0; c < 8; c++) {
+                    const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
+                    const int vx = lx + cx, vy = ly + cy, vz = lz + cz;
+                    if ((unsigned)vx >= (unsigned)DHT_BX || (unsigned)vy >= (unsigned)DHT_BY ||
+                        (unsigned)vz >= (unsigned)DHT_BZ)
+                        continue;
+                    const double w = wx[cx] * wy[cy] * wz[cz];
+                    atomicAdd(&acc[dht_lds(vx, vy, vz)], (unsigned long long)dround_i64(w * scale));
+                }
+            }
+        }
+    }
+    __syncthreads();
+    const double inv = ldexp(1.0, -s);
+    for (int v = tid; v < DHT_VOX; v += 256) {
+        const int lx = v % DHT_BX, ly = (v / DHT_BX) % DHT_BY, lz = v / (DHT_BX * DHT_BY);
+        const int vx = x0 + lx, vy = y0 + ly, vz = z0 + lz;
+        if (vx >= g.nx || vy >= g.ny || vz >= g.nz) continue;
+        const size_t idx = ((size_t)vz * g.ny + vy) * g.nx + vx;
+        acc3[idx] += (double)(long long)acc[dht_lds(lx, ly, lz)] * inv;
+    }
+}
+
+__global__ void k_dht_zero_u32(unsigned* __restrict__ v) {
+    if (threadIdx.x == 0) v[0] = 0u;
+}
+
+__global__ void k_dht_fill(double* __restrict__ v, size_t n) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) v[t] = 0.0;
+}
+
+__global__ void k_dht_finish(const double* __restrict__ acc3, float* __restrict__ vol, double dnorm,
+                             size_t n) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) vol[t] = (float)(acc3[t] * dnorm);
+}
+
+}  // namespace utr
+
+struct dht_ctx {
+    int method = 3;
+    utr_ctx* sctf = nullptr;  // method 5
+    // methods 3 and 4
+    utr::DhtGeom g{};
+    int nrows = 0;
+    int batch = 1;
+    double dnorm = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {};
+    std::vector<char> added;
+    int n_added = 0;
+    float add_ms = 0;
+    utr_timings tim = {};
+    DevBuf<utr::DhtRow> rows;
+    DevBuf<double> xs, ys, zs, acc3;
+    DevBuf<cplx<float>> spec, T, twx, twy;
+    DevBuf<float> P, stage, vol;
+    DevBuf<unsigned> pmax;
+};
+
+namespace {
+
+// plane-stack budget per batch of orientations (UTR_DHT_BATCH_BYTES overrides)
+size_t dht_batch_bytes() {
+    const char* e = std::getenv("UTR_DHT_BATCH_BYTES");
+    if (e && *e) {
+        const long long v = std::atoll(e);
+        if (v > 0) return (size_t)v;
+    }
+    return size_t(2) << 30;
+}
+
+double tem_energy(double wl) {
+    // UTR.cpp:352-366 (the same relation DHT.cpp uses)
+    constexpr double hp = 6.62607004e-34, cc = 299792458, ee = 1.602176634e-19, m0 = 9.1093837015e-31;
+    constexpr long double mc2 = m0 * cc * cc;
+    long double chl2 = (long double)(cc * cc * hp * hp) / (long double)(wl * wl * 1.e-20);
+    long double abra = std::sqrt(mc2 * mc2 + chl2);
+    return (double)((-mc2 + abra) / (long double)(ee));
+}
+
+bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+// Validation of DHT.cpp:36-720 for the numeric path; "" if valid.
+std::string dht_validate(const dht_params* p) {
+    if (p->method == 1 || p->method == 2)
+        return "IWFR and CTFL2 (methods 1, 2) need several defocus distances per orientation; not supported";
+    if (p->method < 3 || p->method > 5) return "Error: unknown phase retrieval method in the input parameter file.";
+    if (p->asigma != 0 && !(p->method == 4 || p->method == 5))
+        return "Error: selected phase retrieval method is incompatibe non-zero absorption coefficient in the input parameter file.";
+    if (p->method != 5) {
+        if (!is_pow2(p->nx) || !is_pow2(p->ny))
+            return "Error: Width and Height parameters in input parameter file must be integer powers of 2.";
+    } else if (p->nx <= 0 || p->ny <= 0 || p->nx % 2 || p->ny % 2) {
+        return "Error: Width and Height parameters in input parameter file must be even.";
+    }
+    if (!(p->xst > 0)) return "Error: pixel size must be positive in input parameter file.";
+    if (!(p->wl > 0) || p->wl > 1) return "Error: wavelength value appears to be wrong.";
+    if (p->aobj_mrad < 0 || p->aobj_mrad > 1000) return "Error: objective aperture value appears to be wrong.";
+    if (p->escc != 0 && p->escc != 1)
+        return "Error: unknown value for Ewald sphere curvature correction switch in input parameter file.";
+    if (p->norm_f1 == 0) return "The first normalization factor cannot be zero (leads to division by zero).";
+    if (p->method == 5 && p->critical_snr < 0) return "Error: the critical SNR value must be non-negative.";
+    if (p->epsilon < 0 && p->method != 3)
+        return "Error: minimal phase error / regulalization parameter must be non-negative, unless -0.5LogAmp phase retrieval method is selected.";
+    if (p->method == 4 && p->epsilon == 0)
+        return "Error: the estimated regularization parameter (epsilon = 0) is not supported; give epsilon > 0.";
+    if (p->inv_laplace && p->alpha < 0)
+        return "Error: regularization parameter for 3D Laplacian filter must be non-negative.";
+    if (p->dzextra == 0 && p->method == 3)
+        return "Error: extra defocus = 0, which is unexpected for the selected reconstruction (phase retrieval) method.";
+    if (p->dzextra != 0 && (p->method == 4 || p->method == 5))
+        return "ERROR: extra defocus != 0, which is unexpected for the phase retrieval method selected above.";
+    return "";
+}
+
+// Per-orientation tables of DHT.cpp:1083-1107, evaluated as the reference
+// does (no contraction into fused multiply-adds).
+void dht_tables(const utr::DhtGeom& g, double angZ_deg, double angY_deg, double* xs, double* ys,
+                double* zs, utr::DhtRow& R) {
+#pragma clang fp contract(off)
+    const double aZ = angZ_deg * utr::kPi180, aY = angY_deg * utr::kPi180;
+    R.sinY = std::sin(aY);
+    R.cosY = std::cos(aY);
+    R.sinZ = std::sin(aZ);
+    R.cosZ = std::cos(aZ);
+    for (int i = 0; i < g.nx; i++) {
+        const double xxx = g.xlo + g.xst * i - g.xc;
+        xs[2 * i] = xxx * R.sinY;
+        xs[2 * i + 1] = xxx * R.cosY;
+    }
+    for (int j = 0; j < g.ny; j++) {
+        const double yyy = g.ylo + g.xst * j - g.yc;
+        ys[2 * j] = yyy * R.sinZ;
+        ys[2 * j + 1] = yyy * R.cosZ;
+    }
+    for (int n = 0; n < g.nz; n++) {
+        const double zzz = (g.zlodz + g.xst * n) - g.zc;
+        zs[2 * n] = zzz * R.sinY;
+        zs[2 * n + 1] = zzz * R.cosY;
+    }
+    // affine map (i, j, n) -> (dx1, dy1, dz1) for the brick bounds
+    auto pos = [&](double i, double j, double n, double* o) {
+        const double X = g.xlo + g.xst * i - g.xc, Y = g.ylo + g.xst * j - g.yc,
+                     Zz = (g.zlodz + g.xst * n) - g.zc;
+        const double xx = X * R.cosY - Zz * R.sinY;
+        o[0] = (g.xc + Y * R.sinZ + xx * R.cosZ - g.xlo) / g.xst;
+        o[1] = (g.yc + Y * R.cosZ - xx * R.sinZ - g.ylo) / g.xst;
+        o[2] = (g.zc + Zz * R.cosY + X * R.sinY - g.zlodz) / g.xst;
+    };
+    double c[3], ci[3], cj[3], cn[3];
+    pos(0, 0, 0, c);
+    pos(1, 0, 0, ci);
+    pos(0, 1, 0, cj);
+    pos(0, 0, 1, cn);
+    double M[9];
+    for (int d = 0; d < 3; d++) {
+        R.c0[d] = (float)c[d];
+        R.ui[d] = (float)(ci[d] - c[d]);
+        R.uj[d] = (float)(cj[d] - c[d]);
+        R.un[d] = (float)(cn[d] - c[d]);
+        M[3 * d] = ci[d] - c[d];
+        M[3 * d + 1] = cj[d] - c[d];
+        M[3 * d + 2] = cn[d] - c[d];
+    }
+    // inverse of a rotation (in index units): its transpose, from the exact matrix
+    const double det = M[0] * (M[4] * M[8] - M[5] * M[7]) - M[1] * (M[3] * M[8] - M[5] * M[6]) +
+                       M[2] * (M[3] * M[7] - M[4] * M[6]);
+    const double id = 1.0 / det;
+    const double inv[9] = {(M[4] * M[8] - M[5] * M[7]) * id, (M[2] * M[7] - M[1] * M[8]) * id,
+                           (M[1] * M[5] - M[2] * M[4]) * id, (M[5] * M[6] - M[3] * M[8]) * id,
+                           (M[0] * M[8] - M[2] * M[6]) * id, (M[2] * M[3] - M[0] * M[5]) * id,
+                           (M[3] * M[7] - M[4] * M[6]) * id, (M[1] * M[6] - M[0] * M[7]) * id,
+                           (M[0] * M[4] - M[1] * M[3]) * id};
+    for (int t = 0; t < 9; t++) R.inv[t] = (float)inv[t];
+}
+
+int dht_create_sctf(const dht_params* p, int nrows, const double* angZ, const double* angY,
+                    const double* defocus, const float* weights, dht_ctx* c) {
+    // the UTR Ewald-sphere path with the DHT filter, no mask and the DHT scale
+    utr_params q{};
+    q.nx = p->nx; q.ny = p->ny; q.modality = 0; q.xst = p->xst;
+    q.zlo = p->zlo; q.zhi = p->zhi; q.dzextra = 0;
+    q.dxc = p->dxc; q.dyc = p->dyc; q.dzc = p->dzc; q.wl = p->wl;
+    q.aobj_mrad = p->aobj_mrad; q.cs3_mm = p->cs3_mm; q.cs5_mm = p->cs5_mm;
+    q.ctf_mode = 3; q.escc = p->escc; q.asigma = 0; q.nsr = 0; q.epsilon = 1;
+    q.inv_laplace = 0; q.norm_f1 = p->norm_f1; q.norm_f2 = p->norm_f2;
+    int rc = utr_create(&q, nrows, angZ, angY, nullptr, defocus, weights, UTR_FP32, &c->sctf);
+    if (rc) return rc;
+    utr::GeomDev& g = c->sctf->plan.g;
+    g.asigma = p->asigma;
+    g.epsilon = p->epsilon;
+    g.dht_sctf = 1;
+    g.critical_snr = p->critical_snr;
+    g.no_mask = 1;
+    g.do_invlap = p->inv_laplace;
+    g.invlap_eps = p->alpha;
+    g.fnorm = p->wl * tem_energy(p->wl) / (2.0 * utr::kPi * std::sqrt(1.0 + p->asigma * p->asigma));
+    return UTR_OK;
+}
+
+int dht_add_rows(dht_ctx* c, int first_row, int count, const float* d_img) {
+    const auto& g = c->g;
+    UTR_HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+    bool ok = false;
+#define DHT_ROWS(N)                                                                              \
+    {                                                                                            \
+        constexpr int NB = utr::lds_batch(N, sizeof(cplx<float>));                               \
+        utr::k_dht_rows<N><<<dim3((g.ny + NB - 1) / NB, count), 256, 0, c->stream>>>(            \
+            d_img, c->spec.p + (size_t)first_row * g.ny * g.nx, g.ny, g, c->twx.p);              \
+        ok = true;                                                                               \
+    }
+    UTR_DISPATCH_N(g.nx, DHT_ROWS)
+#undef DHT_ROWS
+    if (!ok) return set_err(UTR_E_UNSUPPORTED, "unsupported image width for the FFT");
+    ok = false;
+#define DHT_COLS(N)                                                                              \
+    {                                                                                            \
+        constexpr int NB = utr::lds_batch(N, sizeof(cplx<float>));                               \
+        utr::k_dht_cols<N><<<dim3((g.nx + NB - 1) / NB, count), 256, 0, c->stream>>>(            \
+            c->spec.p + (size_t)first_row * g.ny * g.nx, g.nx, c->twy.p, -1);                    \
+        ok = true;                                                                               \
+    }
+    UTR_DISPATCH_N(g.ny, DHT_COLS)
+#undef DHT_COLS
+    if (!ok) return set_err(UTR_E_UNSUPPORTED, "unsupported image height for the FFT");
+    UTR_HIP_TRY(hipGetLastError());
+    UTR_HIP_TRY(hipEventRecord(c->ev[1], c->stream));
+    UTR_HIP_TRY(hipEventSynchronize(c->ev[1]));
+    float ms = 0;
+    UTR_HIP_TRY(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    c->add_ms += ms;
+    for (int r = 0; r < count; r++) {
+        if (!c->added[first_row + r]) c->n_added++;
+        c->added[first_row + r] = 1;
+    }
+    return UTR_OK;
+}
+
+int dht_run(dht_ctx* c, float* d_vol) {
+    const auto& g = c->g;
+    if (c->n_added != c->nrows)
+        return set_err(UTR_E_STATE, "dht_finalize: " + std::to_string(c->nrows - c->n_added) +
+                                        " orientation rows have no image");
+    const size_t N = (size_t)g.nx * g.ny * g.nz;
+    UTR_HIP_TRY(hipEventRecord(c->ev[2], c->stream));
+    utr::k_dht_fill<<<(unsigned)((N + 255) / 256), 256, 0, c->stream>>>(c->acc3.p, N);
+    const long nbricks = (long)((g.nx + utr::DHT_BX - 1) / utr::DHT_BX) *
+                         ((g.ny + utr::DHT_BY - 1) / utr::DHT_BY) * ((g.nz + utr::DHT_BZ - 1) / utr::DHT_BZ);
+    for (int a0 = 0; a0 < c->nrows; a0 += c->batch) {
+        const int na = std::min(c->batch, c->nrows - a0);
+        utr::k_dht_zero_u32<<<1, 64, 0, c->stream>>>(c->pmax.p);
+        bool ok = false;
+#define DHT_PR(N)                                                                                \
+    {                                                                                            \
+        constexpr int NB = utr::lds_batch(N, sizeof(cplx<float>));                               \
+        utr::k_dht_prop_rows<N><<<dim3((g.ny + NB - 1) / NB, g.nplanes, na), 256, 0, c->stream>>>( \
+            c->spec.p, c->T.p, g, c->rows.p, a0, c->twx.p);                                      \
+        ok = true;                                                                               \
+    }
+        UTR_DISPATCH_N(g.nx, DHT_PR)
+#undef DHT_PR
+        if (!ok) return set_err(UTR_E_UNSUPPORTED, "unsupported image width for the FFT");
+        ok = false;
+#define DHT_PC(N)                                                                                \
+    {                                                                                            \
+        constexpr int NB = utr::lds_batch(N, sizeof(cplx<float>));                               \
+        utr::k_dht_prop_cols<N><<<dim3((g.nx + NB - 1) / NB, g.nplanes, na), 256, 0, c->stream>>>( \
+            c->T.p, c->P.p, g, c->rows.p, a0, c->twy.p, c->pmax.p);                              \
+        ok = true;                                                                               \
+    }
+        UTR_DISPATCH_N(g.ny, DHT_PC)
+#undef DHT_PC
+        if (!ok) return set_err(UTR_E_UNSUPPORTED, "unsupported image height for the FFT");
+        utr::k_dht_splat<<<(unsigned)nbricks, 256, 0, c->stream>>>(c->P.p, g, c->rows.p, c->xs.p, c->ys.p,
+                                                                  c->zs.p, a0, na, c->pmax.p, c->acc3.p);
+        UTR_HIP_TRY(hipGetLastError());
+    }
+    UTR_HIP_TRY(hipEventRecord(c->ev[3], c->stream));
+    utr::k_dht_finish<<<(unsigned)((N + 255) / 256), 256, 0, c->stream>>>(c->acc3.p, d_vol, c->dnorm, N);
+    UTR_HIP_TRY(hipGetLastError());
+    UTR_HIP_TRY(hipEventSynchronize(c->ev[3]));
+    UTR_HIP_TRY(hipStreamSynchronize(c->stream));
+    float ms = 0;
+    UTR_HIP_TRY(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+    c->tim.fft2d_ctf_ms = c->add_ms;
+    c->tim.backprop_ms = ms;
+    c->tim.fft3d_ms = 0;
+    c->tim.total_ms = c->add_ms + ms;
+    return UTR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dht_create(const dht_params* p, int nrows, const double* angZ_deg, const double* angY_deg,
+               const double* defocus, const float* weights, dht_ctx** out) {
+    if (!out) return set_err(UTR_E_INVALID, "null output pointer");
+    *out = nullptr;
+    if (!p) return set_err(UTR_E_INVALID, "null parameters");
+    if (nrows < 1 || !angZ_deg || !angY_deg || !defocus)
+        return set_err(UTR_E_INVALID, "Error: the orientation file contains no rows.");
+    std::string err = dht_validate(p);
+    if (!err.empty())
+        return set_err(p->method == 1 || p->method == 2 ? UTR_E_UNSUPPORTED : UTR_E_INVALID, err);
+    dht_ctx* c = new dht_ctx();
+    c->method = p->method;
+    if (p->method == 5) {
+        int rc = dht_create_sctf(p, nrows, angZ_deg, angY_deg, defocus, weights, c);
+        if (rc) {
+            delete c;
+            return rc;
+        }
+        *out = c;
+        return UTR_OK;
+    }
+    auto fail = [&](int rc) {
+        dht_destroy(c);
+        return rc;
+    };
+    utr::DhtGeom& g = c->g;
+    g.nx = p->nx; g.ny = p->ny; g.method = p->method; g.escc = p->escc;
+    g.use_f1 = p->norm_f1 != 1.0; g.use_f2 = p->norm_f2 != 0.0;
+    g.f1 = (float)p->norm_f1; g.f2 = (float)p->norm_f2;
+    g.xst = p->xst; g.wl = p->wl;
+    g.k2maxo = std::pow(p->aobj_mrad * (double)0.001f / p->wl, 2.0);  // DHT.cpp:250
+    const double cs3 = p->cs3_mm * 1.e+7, cs5 = p->cs5_mm * 1.e+7;
+    g.c3 = p->method == 3 ? -cs3 : cs3;  // iSign = -1 reverts the aberrations (DHT.cpp:962)
+    g.c5 = p->method == 3 ? -cs5 : cs5;
+    g.asigma = p->asigma; g.epsilon = p->epsilon;
+    g.xlo = -0.5 * p->xst * p->nx; g.ylo = -0.5 * p->xst * p->ny;
+    const double xhi = g.xlo + p->xst * p->nx, yhi = g.ylo + p->xst * p->ny;
+    double zlo = p->zlo, zhi = p->zhi;
+    if (zlo > zhi) std::swap(zlo, zhi);
+    g.zlodz = zlo + p->dzextra;
+    const double zhidz = zhi + p->dzextra;
+    const long nz = (long)((zhidz - g.zlodz) / p->xst + 0.5);
+    if (nz <= 0) return fail(set_err(UTR_E_INVALID, "Error: number of output defocus planes must be positive."));
+    g.nz = (int)nz;
+    g.nplanes = p->escc ? g.nz : 1;
+    g.xc = (xhi + g.xlo) / 2.0 + p->dxc;
+    g.yc = (yhi + g.ylo) / 2.0 + p->dyc;
+    g.zc = (zhidz + g.zlodz) / 2.0 + p->dzc;
+    g.nx2 = g.nx - 2; g.ny2 = g.ny - 2; g.nz2 = g.nz - 2;
+    if (!fft_size_supported(g.nx) || !fft_size_supported(g.ny))
+        return fail(set_err(UTR_E_UNSUPPORTED, "image size not supported by the FFT kernels"));
+    c->nrows = nrows;
+    c->added.assign(nrows, 0);
+    // DHT.cpp:1325-1340
+    double total = 0;
+    if (weights) for (int r = 0; r < nrows; r++) total += weights[r];
+    else total = nrows;
+    const double EE = tem_energy(p->wl), dAtomWidth = 1.0, dGamma = 0.1;
+    if (p->method == 3) c->dnorm = EE * dGamma / dAtomWidth * (-p->wl / utr::kPi) / total;
+    else c->dnorm = p->wl * EE / utr::kPi / (dAtomWidth * 8.0) / total;
+    // per-orientation rows and tables
+    std::vector<utr::DhtRow> rows(nrows);
+    std::vector<double> xs((size_t)nrows * 2 * g.nx), ys((size_t)nrows * 2 * g.ny), zs((size_t)nrows * 2 * g.nz);
+    for (int r = 0; r < nrows; r++) {
+        std::memset(&rows[r], 0, sizeof rows[r]);
+        dht_tables(g, angZ_deg[r], angY_deg[r], xs.data() + (size_t)r * 2 * g.nx,
+                   ys.data() + (size_t)r * 2 * g.ny, zs.data() + (size_t)r * 2 * g.nz, rows[r]);
+        rows[r].zout = defocus[r];
+        rows[r].weight = weights ? weights[r] : 1.0f;
+    }
+    const size_t n2 = (size_t)g.nx * g.ny, N = n2 * g.nz;
+    const size_t per_angle = (size_t)g.nplanes * n2 * (sizeof(cplx<float>) + sizeof(float));
+    c->batch = (int)std::max<size_t>(1, std::min<size_t>(std::min<size_t>((size_t)nrows, 65535),
+                                                         dht_batch_bytes() / per_angle));
+    if (g.nplanes > 65535) return fail(set_err(UTR_E_UNSUPPORTED, "too many output planes"));
+    if (hipGetDevice(&c->device) != hipSuccess) return fail(set_err(UTR_E_DEVICE, "no HIP device"));
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(set_err(UTR_E_DEVICE, "hipStreamCreate failed"));
+    for (auto& e : c->ev)
+        if (hipEventCreate(&e) != hipSuccess) return fail(set_err(UTR_E_DEVICE, "hipEventCreate failed"));
+    if (c->rows.alloc(nrows) || c->xs.alloc(xs.size()) || c->ys.alloc(ys.size()) || c->zs.alloc(zs.size()) ||
+        c->acc3.alloc(N) || c->spec.alloc((size_t)nrows * n2) || c->pmax.alloc(1) ||
+        c->T.alloc((size_t)c->batch * g.nplanes * n2) || c->P.alloc((size_t)c->batch * g.nplanes * n2))
+        return fail(set_err(UTR_E_NOMEM, "device allocation of the DHT buffers failed"));
+    if (hipMemcpy(c->rows.p, rows.data(), sizeof(utr::DhtRow) * nrows, hipMemcpyHostToDevice) ||
+        hipMemcpy(c->xs.p, xs.data(), sizeof(double) * xs.size(), hipMemcpyHostToDevice) ||
+        hipMemcpy(c->ys.p, ys.data(), sizeof(double) * ys.size(), hipMemcpyHostToDevice) ||
+        hipMemcpy(c->zs.p, zs.data(), sizeof(double) * zs.size(), hipMemcpyHostToDevice))
+        return fail(set_err(UTR_E_DEVICE, "upload of the orientation tables failed"));
+    auto twx = make_twiddles<float>(g.nx), twy = make_twiddles<float>(g.ny);
+    if (c->twx.alloc(twx.size()) || c->twy.alloc(twy.size()) ||
+        hipMemcpy(c->twx.p, twx.data(), sizeof(cplx<float>) * twx.size(), hipMemcpyHostToDevice) ||
+        hipMemcpy(c->twy.p, twy.data(), sizeof(cplx<float>) * twy.size(), hipMemcpyHostToDevice))
+        return fail(set_err(UTR_E_DEVICE, "twiddle upload failed"));
+    *out = c;
+    return UTR_OK;
+}
+
+int dht_add_images_device(dht_ctx* c, int first_row, int count, const float* d_images) {
+    if (!c) return set_err(UTR_E_INVALID, "null context");
+    if (c->sctf) return utr_add_images_device(c->sctf, first_row, count, d_images);
+    if (first_row < 0 || count < 0 || first_row + count > c->nrows)
+        return set_err(UTR_E_INVALID, "row range outside the orientation table");
+    if (count == 0) return UTR_OK;
+    if (!d_images) return set_err(UTR_E_INVALID, "null image pointer");
+    return dht_add_rows(c, first_row, count, d_images);
+}
+
+int dht_add_images(dht_ctx* c, int first_row, int count, const float* images) {
+    if (!c) return set_err(UTR_E_INVALID, "null context");
+    if (c->sctf) return utr_add_images(c->sctf, first_row, count, images);
+    if (first_row < 0 || count < 0 || first_row + count > c->nrows)
+        return set_err(UTR_E_INVALID, "row range outside the orientation table");
+    if (count == 0) return UTR_OK;
+    if (!images) return set_err(UTR_E_INVALID, "null image pointer");
+    const size_t n2 = (size_t)c->g.nx * c->g.ny;
+    UTR_HIP_TRY(c->stage.alloc(n2 * kStageRows));
+    for (int r = 0; r < count; r += kStageRows) {
+        const int k = std::min(kStageRows, count - r);
+        UTR_HIP_TRY(hipMemcpyAsync(c->stage.p, images + (size_t)r * n2, sizeof(float) * n2 * k,
+                                   hipMemcpyHostToDevice, c->stream));
+        int rc = dht_add_rows(c, first_row + r, k, c->stage.p);
+        if (rc) return rc;
+    }
+    return UTR_OK;
+}
+
+int dht_finalize_device(dht_ctx* c, float* d_vol) {
+    if (!c) return set_err(UTR_E_INVALID, "null context");
+    if (c->sctf) return utr_finalize_device(c->sctf, d_vol);
+    if (!d_vol) return set_err(UTR_E_INVALID, "null output pointer");
+    return dht_run(c, d_vol);
+}
+
+int dht_finalize(dht_ctx* c, float* vol) {
+    if (!c) return set_err(UTR_E_INVALID, "null context");
+    if (c->sctf) return utr_finalize(c->sctf, vol);
+    if (!vol) return set_err(UTR_E_INVALID, "null output pointer");
+    const size_t N = (size_t)c->g.nx * c->g.ny * c->g.nz;
+    UTR_HIP_TRY(c->vol.alloc(N));
+    int rc = dht_run(c, c->vol.p);
+    if (rc) return rc;
+    UTR_HIP_TRY(hipMemcpy(vol, c->vol.p, sizeof(float) * N, hipMemcpyDeviceToHost));
+    return UTR_OK;
+}
+
+int dht_reset(dht_ctx* c) {
+    if (!c) return set_err(UTR_E_INVALID, "null context");
+    if (c->sctf) return utr_reset(c->sctf);
+    std::fill(c->added.begin(), c->added.end(), 0);
+    c->n_added = 0;
+    c->add_ms = 0;
+    return UTR_OK;
+}
+
+int dht_dims(const dht_ctx* c, int* nx, int* ny, int* nz) {
+    if (!c) return set_err(UTR_E_INVALID, "null context");
+    if (c->sctf) return utr_dims(c->sctf, nx, ny, nz);
+    if (nx) *nx = c->g.nx;
+    if (ny) *ny = c->g.ny;
+    if (nz) *nz = c->g.nz;
+    return UTR_OK;
+}
+
+int dht_get_timings(const dht_ctx* c, utr_timings* t) {
+    if (!c || !t) return set_err(UTR_E_INVALID, "null argument");
+    if (c->sctf) return utr_get_timings(c->sctf, t);
+    *t = c->tim;
+    return UTR_OK;
+}
+
+int dht_destroy(dht_ctx* c) {
+    if (!c) return UTR_OK;
+    if (c->sctf) utr_destroy(c->sctf);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->rows.release(); c->xs.release(); c->ys.release(); c->zs.release(); c->acc3.release();
+    c->spec.release(); c->T.release(); c->twx.release(); c->twy.release();
+    c->P.release(); c->stage.release(); c->vol.release(); c->pmax.release();
+    for (auto& e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return UTR_OK;
+}
+
+}  // extern "C"

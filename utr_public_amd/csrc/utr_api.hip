@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -557,3 +558,6 @@ int utr_destroy(utr_ctx* c) {
 }
 
 }  // extern "C"
+
+// DHT variants (methods 3-5) share the helpers above
+#include "dht.hip"

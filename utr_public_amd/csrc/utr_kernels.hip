@@ -195,7 +195,15 @@ __device__ __forceinline__ cplx<T> voxel_epilogue(const GeomDev& g, int i, int j
     const double pi = 3.14159265358979323846;
     const T pif = T(pi);
     const T target = T(kTargetSNR);
-    if (g.nsr2 == 0) {
+    if (g.dht_sctf) {
+        // DHT.cpp:1350-1365: x -> (1/x) exp((x - SNRc)/x^4) where sampled
+        if (x != T(0)) {
+            const double xd = (double)x;
+            const T f = T((1.0 / xd) * exp((xd - g.critical_snr) / (xd * xd * xd * xd)));
+            v.x *= f;
+            v.y *= f;
+        }
+    } else if (g.nsr2 == 0) {
         if (x == T(0)) return {T(0), T(0)};
         const T r = T(1) / x;
         v.x *= r;
@@ -234,7 +242,7 @@ __device__ __forceinline__ cplx<T> voxel_epilogue(const GeomDev& g, int i, int j
             v.y *= f;
         }
         if (g.do_invlap) {
-            const T f = T(g.epsilon / (base + g.epsilon));
+            const T f = T(g.invlap_eps / (base + g.invlap_eps));
             v.x *= f;
             v.y *= f;
         }
@@ -906,7 +914,9 @@ __global__ void __launch_bounds__(256) k_c2r_x(const cplx<T>* __restrict__ H,
             if (g.mask_j) ii2 += (long)(jj - g.jc) * (jj - g.jc);
             if (g.mask_k) ii2 += (long)(k - g.kc) * (k - g.kc);
             TO o;
-            if (ii2 > g.nR2) {
+            if (g.no_mask) {
+                o = TO((double)val * (g.fnorm * invN * (double)(par_sign(k + jj) * gs)));
+            } else if (ii2 > g.nR2) {
                 o = TO(0);
             } else {
                 const double f = exp(g.mask_dtemp * (double)ii2) * g.fnorm * invN *

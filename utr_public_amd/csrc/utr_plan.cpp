@@ -210,6 +210,7 @@ bool build_plan(const utr_params* p, int nrows, const double* angZ, const double
         g.ctf_eps = p->epsilon;
         if (g.ctf_eps <= 0) g.ctf_eps = 0.1 * std::pow(std::fabs(fac2) * std::min(dk2, std::min(de2, dz2)), 2);
     }
+    g.invlap_eps = p->epsilon;
     if (g.inv_laplace && g.ctf_mode != 1 && g.ctf_mode != 2) {
         g.do_invlap = 1;
         if (p->epsilon <= 0) { err = "InverseMLaplacian3D() cannot be called with alpha <= 0"; return false; }

@@ -38,6 +38,11 @@ struct GeomDev {
     double tie_alpha, tie_norm;  // InverseMLaplacian3D arguments for mode 2
     double ctf_eps;              // epsilon actually used by CTFcorrection3D
     double two_d_alpha, two_d_norm;  // 2D TIE-Hom (mode 1)
+    double invlap_eps;               // regularisation of the inverse Laplacian (parameter 22)
+    // sCTF of the DHT program: Filt3 filter instead of the noise gate, no
+    // sphere mask (DHT.cpp:1350-1416)
+    int dht_sctf, no_mask;
+    double critical_snr;
 };
 
 // Per orientation row.  m: grid-unit forward map (rows dx1, dy1, dz1 over
