@@ -51,6 +51,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-dht", action="store_true",
+                    help="skip the DHT variants (reported under 'dht', not in value)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="first run a plain 256 MiB device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -157,6 +159,34 @@ def main():
                              "rel_l2_fp32_vs_fp64": relL2,
                              "max_abs_fp32_vs_fp64": float(np.abs(vol32 - v64).max())}
         rec64.close()
+
+    if not args.no_dht:
+        # DHT variants on the same images (methods 3, 4: per-plane propagation
+        # and real-space splat, O(A N^3); method 5: Ewald-sphere sCTF)
+        dht = {}
+        for m, kw in ((3, dict(dzextra=100.0, epsilon=1.0)),
+                      (4, dict(dzextra=0.0, asigma=0.05, epsilon=0.02)),
+                      (5, dict(dzextra=0.0, asigma=0.05, epsilon=0.02, critical_snr=0.5))):
+            dp = U.default_dht_params(nx=nx, ny=ny, xst=p["xst"], zlo=p["zlo"], zhi=p["zhi"],
+                                      wl=p["wl"], method=m, escc=1, **kw)
+            r = U.DhtReconstructor(dp, rows)
+            dvol = torch.empty((nz, ny, nx), dtype=torch.float32, device="cuda")
+            ts = []
+            for it in range(3):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                r.reset()
+                r.add_images_device(d_img.data_ptr(), A)
+                r.finalize_device(dvol.data_ptr())
+                torch.cuda.synchronize()
+                ts.append(time.perf_counter() - t0)
+            tm = r.timings()
+            r.close()
+            t = float(np.mean(ts[1:]))
+            dht["method%d" % m] = {"ms_per_step": round(t * 1e3, 2), "value": work / t,
+                                   "phase_ms": {k: round(v, 2) for k, v in tm.items()}}
+            log("dht method %d: %.1f ms" % (m, t * 1e3))
+        extra["dht"] = dht
 
     cpu = None
     if not args.no_cpu_baseline:
