@@ -164,6 +164,46 @@ int utr_get_fourier_volume(utr_ctx* ctx, float* vol_c);
  */
 int utr_angle_coeffs(const utr_params* params, double angZ_deg, double angY_deg, double* m9);
 
+/* ----------------------------------------------------------- batched form
+ * Several independent reconstructions of uneven size run as one HIP graph:
+ * each job's launches (2D stage of all its rows, gridding, 3D inverse FFT)
+ * form one branch, the branches run concurrently, and the graph is captured
+ * once at utr_batch_create and replayed by every utr_batch_run.  No
+ * counterpart in the reference, which runs one reconstruction per process
+ * (UTR.cpp:main); it is the batched form of the project's target.
+ */
+typedef struct utr_job {
+    utr_params params;
+    int nrows;
+    const double* angZ_deg;  /* per row, as utr_create                      */
+    const double* angY_deg;
+    const double* defocus;
+    const float* weights;    /* may be NULL                                 */
+    const float* images;     /* nrows*ny*nx                                 */
+    float* vol;              /* nz*ny*nx result                             */
+    int images_on_device;    /* 0: host memory (copied before each run)     */
+    int vol_on_device;       /* 0: host memory (copied after each run)      */
+    int precision;           /* UTR_FP32 or UTR_FP32_GATHER; ignored when autotuned */
+} utr_job;
+
+typedef struct utr_batch utr_batch;
+
+/* Creates the contexts and captures the graph.  Device pointers in jobs must
+ * stay valid while the batch exists; host buffers are read / written at each
+ * utr_batch_run.  autotune != 0: for every distinct job shape (nx, ny, nz,
+ * nrows) both gridding forms are timed once on that job's images and the
+ * faster one is captured (the choice is cached for the process). */
+int utr_batch_create(const utr_job* jobs, int njobs, int autotune, utr_batch** out);
+
+/* Runs all jobs: host images in, one graph launch, host volumes out.
+ * graph_ms (may be NULL): device time of the graph launch. */
+int utr_batch_run(utr_batch* b, double* graph_ms);
+
+/* Form chosen for job j (UTR_FP32 or UTR_FP32_GATHER). */
+int utr_batch_job_form(const utr_batch* b, int j);
+
+int utr_batch_destroy(utr_batch* b);
+
 /* ------------------------------------------------------------------ DHT
  * The single-defocus reconstructions of the DHT program
  * (MultisliceCppTest/PhaseRetrieval/DHT.cpp): method 3 (phase estimate

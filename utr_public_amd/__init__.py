@@ -36,6 +36,7 @@ EXPORTS = [
     "utr_finalize_device", "utr_finalize_f64", "utr_keep_sampling", "utr_get_sampling",
     "utr_reset", "utr_dims", "utr_get_timings", "utr_destroy", "utr_last_error",
     "utr_abi_version", "utr_angle_coeffs", "utr_get_spectrum", "utr_get_fourier_volume",
+    "utr_batch_create", "utr_batch_run", "utr_batch_job_form", "utr_batch_destroy",
     "dht_create", "dht_add_images", "dht_add_images_device", "dht_finalize", "dht_finalize_device",
     "dht_reset", "dht_dims", "dht_get_timings", "dht_destroy",
 ]
@@ -58,6 +59,15 @@ class UtrParams(ctypes.Structure):
 
 class DhtParams(ctypes.Structure):
     _fields_ = DHT_FIELDS
+
+
+class UtrJob(ctypes.Structure):
+    _fields_ = [("params", UtrParams), ("nrows", ctypes.c_int),
+                ("angZ_deg", ctypes.c_void_p), ("angY_deg", ctypes.c_void_p),
+                ("defocus", ctypes.c_void_p), ("weights", ctypes.c_void_p),
+                ("images", ctypes.c_void_p), ("vol", ctypes.c_void_p),
+                ("images_on_device", ctypes.c_int), ("vol_on_device", ctypes.c_int),
+                ("precision", ctypes.c_int)]
 
 
 class UtrTimings(ctypes.Structure):
@@ -138,6 +148,11 @@ def load():
     L.utr_angle_coeffs.argtypes = [ctypes.POINTER(UtrParams), ctypes.c_double, ctypes.c_double, dp]
     L.utr_get_spectrum.argtypes = [vp, ctypes.c_int, vp]
     L.utr_get_fourier_volume.argtypes = [vp, vp]
+    L.utr_batch_create.argtypes = [ctypes.POINTER(UtrJob), ctypes.c_int, ctypes.c_int,
+                                   ctypes.POINTER(vp)]
+    L.utr_batch_run.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
+    L.utr_batch_job_form.argtypes = [vp, ctypes.c_int]
+    L.utr_batch_destroy.argtypes = [vp]
     L.dht_create.argtypes = [ctypes.POINTER(DhtParams), ctypes.c_int, dp, dp, dp, vp,
                              ctypes.POINTER(vp)]
     for f in ("dht_add_images", "dht_add_images_device"):
@@ -278,6 +293,70 @@ def reconstruct(params, rows, images, precision=UTR_FP32, f64_out=False):
         if f64_out:
             return r.finalize_f64()
         return r.finalize()
+
+
+class Batch:
+    """Independent reconstructions of uneven size run as one HIP graph
+    (utr_batch_* of include/utr_hip.h).  jobs: list of (params, rows, images)
+    with host images; volumes are returned by run()."""
+
+    def __init__(self, jobs, autotune=False, precision=UTR_FP32):
+        L = load()
+        self._L = L
+        self._keep = []
+        self.shapes = []
+        arr = (UtrJob * len(jobs))()
+        self.outputs = []
+        for k, (params, rows, images) in enumerate(jobs):
+            J = arr[k]
+            J.params = make_params(params)
+            cols = [np.ascontiguousarray(rows[c], dtype=np.float64) for c in ("angZ", "angY", "defocus")]
+            w = rows.get("weights")
+            w = None if w is None else np.ascontiguousarray(w, dtype=np.float32)
+            images = np.ascontiguousarray(images, dtype=np.float32)
+            nz = int((max(params["zlo"], params["zhi"]) - min(params["zlo"], params["zhi"])) /
+                     params["xst"] + 0.5)
+            out = np.empty((nz, params["ny"], params["nx"]), dtype=np.float32)
+            self._keep += cols + [w, images, out]
+            J.nrows = len(cols[0])
+            J.angZ_deg, J.angY_deg, J.defocus = (c.ctypes.data for c in cols)
+            J.weights = w.ctypes.data if w is not None else None
+            J.images = images.ctypes.data
+            J.vol = out.ctypes.data
+            J.images_on_device = 0
+            J.vol_on_device = 0
+            J.precision = precision
+            self.outputs.append(out)
+        self._jobs = arr
+        b = ctypes.c_void_p()
+        _check(L.utr_batch_create(arr, len(jobs), 1 if autotune else 0, ctypes.byref(b)))
+        self._b = b
+
+    def run(self):
+        ms = ctypes.c_double()
+        _check(self._L.utr_batch_run(self._b, ctypes.byref(ms)))
+        self.graph_ms = ms.value
+        return self.outputs
+
+    def form(self, j):
+        return self._L.utr_batch_job_form(self._b, j)
+
+    def close(self):
+        if getattr(self, "_b", None):
+            self._L.utr_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 class DhtReconstructor:

@@ -559,5 +559,225 @@ int utr_destroy(utr_ctx* c) {
 
 }  // extern "C"
 
+// ---------------------------------------------------------- batched form
+
+#include <map>
+#include <mutex>
+#include <tuple>
+
+struct utr_batch {
+    std::vector<utr_job> jobs;
+    std::vector<utr_ctx*> ctx;
+    std::vector<DevBuf<float>> img, vol;  // device copies for host-side jobs
+    std::vector<const float*> d_img;
+    std::vector<float*> d_vol;
+    std::vector<hipEvent_t> join;
+    hipStream_t s0 = nullptr;
+    hipEvent_t fork = nullptr, t0 = nullptr, t1 = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+// The launches of one whole reconstruction on the context's stream, without
+// host synchronisation or memset nodes (capturable).
+int enqueue_job(utr_ctx* c, const float* d_img, float* d_vol) {
+    const auto& g = c->plan.g;
+    const int nrows = c->plan.nrows;
+    utr::k_fill_u32<<<(nrows + 255) / 256, 256, 0, c->stream>>>(c->rowmax.p, nrows);
+    int rc = launch_2d<float>(c, d_img, 0, nrows, c->spec32.p, c->twx32.p, c->twy32.p);
+    if (rc) return rc;
+    if (c->precision == UTR_FP32) {
+        const long nblocks = (long)((g.nx + utr::SBX - 1) / utr::SBX) *
+                             ((g.ny + utr::SBY - 1) / utr::SBY) * ((g.nz + utr::SBZ - 1) / utr::SBZ);
+        utr::k_grid_fx<<<(unsigned)nblocks, 256, 0, c->stream>>>(c->spec32.p, c->rows.p, nrows,
+                                                                 c->rowmax.p, g, c->V32.p, nullptr);
+    } else {
+        const long nblocks = (long)((g.nx + utr::GBX - 1) / utr::GBX) *
+                             ((g.ny + utr::GBY - 1) / utr::GBY) * ((g.nz + utr::GBZ - 1) / utr::GBZ);
+        utr::k_gather<<<(unsigned)nblocks, 256, 0, c->stream>>>(c->spec32.p, c->rows.p, nrows, g,
+                                                               c->V32.p, nullptr);
+    }
+    UTR_HIP_TRY(hipGetLastError());
+    return launch_3d<float, float>(c, c->V32.p, c->H32.p, d_vol, c->twx32.p, c->twy32.p, c->twz32.p);
+}
+
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int, int>, int> g_tune;  // shape -> chosen form
+
+int time_job(utr_ctx* c, const float* d_img, float* d_vol, float* ms) {
+    hipEvent_t a = c->ev[4], b = c->ev[5];
+    float best = 1e30f;
+    for (int rep = 0; rep < 2; rep++) {
+        UTR_HIP_TRY(hipEventRecord(a, c->stream));
+        int rc = enqueue_job(c, d_img, d_vol);
+        if (rc) return rc;
+        UTR_HIP_TRY(hipEventRecord(b, c->stream));
+        UTR_HIP_TRY(hipEventSynchronize(b));
+        float t = 0;
+        UTR_HIP_TRY(hipEventElapsedTime(&t, a, b));
+        best = std::min(best, t);
+    }
+    *ms = best;
+    return UTR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int utr_batch_destroy(utr_batch* b) {
+    if (!b) return UTR_OK;
+    if (b->s0) (void)hipStreamSynchronize(b->s0);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    for (auto* c : b->ctx) utr_destroy(c);
+    for (auto& d : b->img) d.release();
+    for (auto& d : b->vol) d.release();
+    for (auto e : b->join)
+        if (e) (void)hipEventDestroy(e);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->t0) (void)hipEventDestroy(b->t0);
+    if (b->t1) (void)hipEventDestroy(b->t1);
+    if (b->s0) (void)hipStreamDestroy(b->s0);
+    delete b;
+    return UTR_OK;
+}
+
+int utr_batch_create(const utr_job* jobs, int njobs, int autotune, utr_batch** out) {
+    if (!out) return set_err(UTR_E_INVALID, "null output pointer");
+    *out = nullptr;
+    if (!jobs || njobs < 1) return set_err(UTR_E_INVALID, "no jobs");
+    utr_batch* b = new utr_batch();
+    auto fail = [&](int rc) {
+        utr_batch_destroy(b);
+        return rc;
+    };
+    b->jobs.assign(jobs, jobs + njobs);
+    b->ctx.assign(njobs, nullptr);
+    b->img.resize(njobs);
+    b->vol.resize(njobs);
+    b->d_img.assign(njobs, nullptr);
+    b->d_vol.assign(njobs, nullptr);
+    b->join.assign(njobs, nullptr);
+    if (hipStreamCreateWithFlags(&b->s0, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&b->fork) != hipSuccess || hipEventCreate(&b->t0) != hipSuccess ||
+        hipEventCreate(&b->t1) != hipSuccess)
+        return fail(set_err(UTR_E_DEVICE, "stream/event creation failed"));
+    for (int j = 0; j < njobs; j++) {
+        const utr_job& J = jobs[j];
+        if (!J.images || !J.vol) return fail(set_err(UTR_E_INVALID, "job " + std::to_string(j) + ": null buffer"));
+        const int form = J.precision == UTR_FP32_GATHER ? UTR_FP32_GATHER : UTR_FP32;
+        int rc = utr_create(&J.params, J.nrows, J.angZ_deg, J.angY_deg, nullptr, J.defocus, J.weights,
+                            form, &b->ctx[j]);
+        if (rc) return fail(set_err(rc, "job " + std::to_string(j) + ": " + g_err));
+        utr_ctx* c = b->ctx[j];
+        const auto& g = c->plan.g;
+        const size_t n2 = (size_t)g.nx * g.ny, N = n2 * g.nz;
+        if (J.images_on_device) {
+            b->d_img[j] = J.images;
+        } else {
+            if (b->img[j].alloc(n2 * J.nrows) != hipSuccess) return fail(set_err(UTR_E_NOMEM, "image buffer"));
+            b->d_img[j] = b->img[j].p;
+            if (hipMemcpy(b->img[j].p, J.images, sizeof(float) * n2 * J.nrows, hipMemcpyHostToDevice) != hipSuccess)
+                return fail(set_err(UTR_E_DEVICE, "image upload"));
+        }
+        if (J.vol_on_device) {
+            b->d_vol[j] = J.vol;
+        } else {
+            if (b->vol[j].alloc(N) != hipSuccess) return fail(set_err(UTR_E_NOMEM, "volume buffer"));
+            b->d_vol[j] = b->vol[j].p;
+        }
+        if (hipEventCreateWithFlags(&b->join[j], hipEventDisableTiming) != hipSuccess)
+            return fail(set_err(UTR_E_DEVICE, "event creation failed"));
+        if (autotune) {
+            const auto key = std::make_tuple(g.nx, g.ny, g.nz, J.nrows);
+            int chosen = -1;
+            {
+                std::lock_guard<std::mutex> lk(g_tune_mu);
+                auto it = g_tune.find(key);
+                if (it != g_tune.end()) chosen = it->second;
+            }
+            if (chosen < 0) {
+                float t_fx = 0, t_ga = 0;
+                c->precision = UTR_FP32;
+                if ((rc = time_job(c, b->d_img[j], b->d_vol[j], &t_fx))) return fail(rc);
+                c->precision = UTR_FP32_GATHER;
+                if ((rc = time_job(c, b->d_img[j], b->d_vol[j], &t_ga))) return fail(rc);
+                chosen = t_ga < t_fx ? UTR_FP32_GATHER : UTR_FP32;
+                std::lock_guard<std::mutex> lk(g_tune_mu);
+                g_tune[key] = chosen;
+            }
+            c->precision = chosen;
+        }
+    }
+    // capture: fork from s0, one branch per job on its own stream, join on s0
+    for (auto* c : b->ctx) UTR_HIP_TRY(hipStreamSynchronize(c->stream));
+    UTR_HIP_TRY(hipStreamBeginCapture(b->s0, hipStreamCaptureModeThreadLocal));
+    int rc = UTR_OK;
+    if (hipEventRecord(b->fork, b->s0) != hipSuccess) rc = set_err(UTR_E_DEVICE, "capture: fork");
+    for (int j = 0; j < njobs && !rc; j++) {
+        utr_ctx* c = b->ctx[j];
+        if (hipStreamWaitEvent(c->stream, b->fork, 0) != hipSuccess) {
+            rc = set_err(UTR_E_DEVICE, "capture: wait");
+            break;
+        }
+        rc = enqueue_job(c, b->d_img[j], b->d_vol[j]);
+        if (rc) break;
+        if (hipEventRecord(b->join[j], c->stream) != hipSuccess ||
+            hipStreamWaitEvent(b->s0, b->join[j], 0) != hipSuccess)
+            rc = set_err(UTR_E_DEVICE, "capture: join");
+    }
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(b->s0, &graph);
+    if (rc) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return fail(rc);
+    }
+    if (e != hipSuccess) return fail(set_err(UTR_E_DEVICE, std::string("graph capture failed: ") + hipGetErrorString(e)));
+    b->graph = graph;
+    if (hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess)
+        return fail(set_err(UTR_E_DEVICE, "graph instantiation failed"));
+    *out = b;
+    return UTR_OK;
+}
+
+int utr_batch_run(utr_batch* b, double* graph_ms) {
+    if (!b) return set_err(UTR_E_INVALID, "null batch");
+    const int njobs = (int)b->jobs.size();
+    for (int j = 0; j < njobs; j++) {
+        const utr_job& J = b->jobs[j];
+        if (J.images_on_device) continue;
+        const auto& g = b->ctx[j]->plan.g;
+        UTR_HIP_TRY(hipMemcpyAsync(b->img[j].p, J.images, sizeof(float) * (size_t)g.nx * g.ny * J.nrows,
+                                   hipMemcpyHostToDevice, b->s0));
+    }
+    UTR_HIP_TRY(hipEventRecord(b->t0, b->s0));
+    UTR_HIP_TRY(hipGraphLaunch(b->exec, b->s0));
+    UTR_HIP_TRY(hipEventRecord(b->t1, b->s0));
+    for (int j = 0; j < njobs; j++) {
+        const utr_job& J = b->jobs[j];
+        if (J.vol_on_device) continue;
+        const auto& g = b->ctx[j]->plan.g;
+        UTR_HIP_TRY(hipMemcpyAsync(J.vol, b->vol[j].p, sizeof(float) * (size_t)g.nx * g.ny * g.nz,
+                                   hipMemcpyDeviceToHost, b->s0));
+    }
+    UTR_HIP_TRY(hipStreamSynchronize(b->s0));
+    if (graph_ms) {
+        float ms = 0;
+        UTR_HIP_TRY(hipEventElapsedTime(&ms, b->t0, b->t1));
+        *graph_ms = ms;
+    }
+    return UTR_OK;
+}
+
+int utr_batch_job_form(const utr_batch* b, int j) {
+    if (!b || j < 0 || j >= (int)b->ctx.size()) return set_err(UTR_E_INVALID, "no such job");
+    return b->ctx[j]->precision;
+}
+
+}  // extern "C"
+
 // DHT variants (methods 3-5) share the helpers above
 #include "dht.hip"

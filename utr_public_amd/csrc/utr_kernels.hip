@@ -738,6 +738,13 @@ __global__ void __launch_bounds__(256, 3) k_grid_fx(const cplx<float>* __restric
     }
 }
 
+// Zero fill of a small counter array (kept out of hipMemsetAsync so that
+// launch sequences containing it can be captured into a graph).
+__global__ void k_fill_u32(unsigned* __restrict__ p, int n) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = 0u;
+}
+
 // ------------------------------------------- plain FP64 form: scatter
 
 // One thread per spectrum point of one row; atomics into V (interleaved
