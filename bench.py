@@ -51,6 +51,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batched 16-job workload (reported under 'batched')")
     ap.add_argument("--no-dht", action="store_true",
                     help="skip the DHT variants (reported under 'dht', not in value)")
     ap.add_argument("--calib-copy", action="store_true",
@@ -187,6 +189,46 @@ def main():
                                    "phase_ms": {k: round(v, 2) for k, v in tm.items()}}
             log("dht method %d: %.1f ms" % (m, t * 1e3))
         extra["dht"] = dht
+
+    if not args.no_batch:
+        # 16 uneven independent reconstructions: one graph vs one at a time
+        jobs = []
+        for k, (jp, jrows, jat) in enumerate(synth.c4_jobs(seed=1)):
+            jobs.append((jp, jrows, synth.images_for(jp, jrows, jat, seed=k + 1)))
+        bwork = sum(float(len(jr["angY"])) * jp["nx"] * jp["ny"] *
+                    int((jp["zhi"] - jp["zlo"]) / jp["xst"] + 0.5) for jp, jr, _ in jobs)
+        B = U.Batch(jobs, autotune=True)
+        B.run()
+        gms = []
+        for _ in range(3):
+            B.run()
+            gms.append(B.graph_ms)
+        forms = [B.form(j) for j in range(len(jobs))]
+        B.close()
+        seq = 0.0
+        for jp, jr, ji in jobs:
+            r = U.Reconstructor(jp, jr, U.UTR_FP32)
+            dimg = torch.from_numpy(ji).cuda()
+            dvol = torch.empty(r.shape, dtype=torch.float32, device="cuda")
+            for it in range(2):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                r.reset()
+                r.add_images_device(dimg.data_ptr(), len(jr["angY"]))
+                r.finalize_device(dvol.data_ptr())
+                torch.cuda.synchronize()
+                t = time.perf_counter() - t0
+            seq += t
+            r.close()
+            del dimg, dvol
+        g_ms = float(np.mean(gms))
+        extra["batched"] = {"jobs": len(jobs), "edges": [jp["nx"] for jp, _, _ in jobs],
+                            "rows": [len(jr["angY"]) for _, jr, _ in jobs],
+                            "graph_ms": round(g_ms, 3), "value": bwork / (g_ms * 1e-3),
+                            "sequential_ms": round(seq * 1e3, 3),
+                            "speedup_vs_sequential": round(seq * 1e3 / g_ms, 2),
+                            "forms": forms}
+        log("batched: graph %.2f ms, sequential %.2f ms" % (g_ms, seq * 1e3))
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -115,6 +115,33 @@ def config(name, seed=1):
     return p, rows, at
 
 
+def c4_jobs(seed=1):
+    """The batched workload: 16 independent reconstructions with cube edges
+    drawn from {64, 96, 128, 192, 256, 384}, angle counts from {30, 90, 180,
+    360} about y, 1-3 defocus distances in [0, 300] nm per angle (flattened to
+    rows), random C/N/O/S atoms at 0.05 atoms/nm^3.  Returns a list of
+    (params, rows, atoms-spec)."""
+    from . import default_params
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for k in range(16):
+        n = int(rng.choice([64, 96, 128, 192, 256, 384]))
+        na = int(rng.choice([30, 90, 180, 360]))
+        nd = int(rng.integers(1, 4))
+        xst = 0.6
+        half = n * xst / 2
+        p = default_params(nx=n, ny=n, xst=xst, zlo=-half, zhi=half, wl=0.01968749, escc=1,
+                           nsr=1.0 / 6.0, ctf_mode=0)
+        ang = np.arange(na) * (360.0 / na)
+        df = np.sort(rng.uniform(0.0, 3000.0, nd))
+        rows = dict(angZ=np.zeros(na * nd), angY=np.repeat(ang, nd), defocus=np.tile(df, na))
+        side_nm = n * xst / 10.0 * 0.8
+        natoms = max(8, int(0.05 * side_nm ** 3))
+        at = dict(natoms=natoms, shape="cube", size=n * xst * 0.8, min_sep=0.0, species=CNOS)
+        jobs.append((p, rows, at))
+    return jobs
+
+
 def images_for(params, rows, at, seed=1, dose=None):
     """Synthetic images for a config: dose defaults to (1/nsr)^2 counts/px."""
     xyz, Z = atoms(seed, at["natoms"], at["shape"], at["size"], at["min_sep"], at["species"])
