@@ -178,9 +178,9 @@ int launch_2d(utr_ctx* c, const float* d_img, int row0, int count, cplx<T>* d_sp
 #define UTR_COLS(N)                                                                             \
     {                                                                                           \
         constexpr int NB = utr::lds_batch(N, sizeof(cplx<T>));                                  \
-        dim3 grid((g.nx + NB - 1) / NB, count);                                                 \
-        utr::k_cols<T, N><<<grid, 256, 0, c->stream>>>(d_spec, g.nx, g, c->rows.p, row0, twy,   \
-                                                        c->rowmax.p);                           \
+        const int gx = (g.nx + NB - 1) / NB;                                                    \
+        utr::k_cols<T, N><<<utr::xcd_grid(gx * count), 256, 0, c->stream>>>(                   \
+            d_spec, g.nx, g, c->rows.p, row0, twy, c->rowmax.p, gx, count);                     \
         ok = true;                                                                              \
     }
     UTR_DISPATCH_N(g.ny, UTR_COLS)
@@ -200,8 +200,9 @@ int launch_3d(utr_ctx* c, const cplx<T>* d_V, cplx<T>* d_H, TO* d_vol, const cpl
 #define UTR_PZ(N)                                                                               \
     {                                                                                           \
         constexpr int CW = utr::lds_batch(N, sizeof(cplx<T>));                                  \
-        dim3 grid((W + CW - 1) / CW, g.ny);                                                     \
-        utr::k_pack_z<T, N><<<grid, 256, 0, c->stream>>>(d_V, d_H, g, twz);                     \
+        const int gx = (W + CW - 1) / CW;                                                       \
+        utr::k_pack_z<T, N><<<utr::xcd_grid(gx * g.ny), 256, 0, c->stream>>>(d_V, d_H, g, twz,  \
+                                                                             gx, g.ny);         \
         ok = true;                                                                              \
     }
     UTR_DISPATCH_N(g.nz, UTR_PZ)
@@ -211,8 +212,9 @@ int launch_3d(utr_ctx* c, const cplx<T>* d_V, cplx<T>* d_H, TO* d_vol, const cpl
 #define UTR_FY(N)                                                                               \
     {                                                                                           \
         constexpr int CW = utr::lds_batch(N, sizeof(cplx<T>));                                  \
-        dim3 grid((W + CW - 1) / CW, g.nz);                                                     \
-        utr::k_fft_y<T, N><<<grid, 256, 0, c->stream>>>(d_H, g, twy);                           \
+        const int gx = (W + CW - 1) / CW;                                                       \
+        utr::k_fft_y<T, N><<<utr::xcd_grid(gx * g.nz), 256, 0, c->stream>>>(d_H, g, twy, gx,    \
+                                                                            g.nz);              \
         ok = true;                                                                              \
     }
     UTR_DISPATCH_N(g.ny, UTR_FY)
@@ -222,8 +224,9 @@ int launch_3d(utr_ctx* c, const cplx<T>* d_V, cplx<T>* d_H, TO* d_vol, const cpl
 #define UTR_CX(N)                                                                               \
     {                                                                                           \
         constexpr int NB = utr::lds_batch(N, sizeof(cplx<T>));                                  \
-        dim3 grid((g.ny / 2 + NB - 1) / NB, g.nz);                                              \
-        utr::k_c2r_x<T, N, TO><<<grid, 256, 0, c->stream>>>(d_H, d_vol, g, twx);                \
+        const int gx = (g.ny / 2 + NB - 1) / NB;                                                \
+        utr::k_c2r_x<T, N, TO><<<utr::xcd_grid(gx * g.nz), 256, 0, c->stream>>>(d_H, d_vol, g,  \
+                                                                               twx, gx, g.nz);  \
         ok = true;                                                                              \
     }
     UTR_DISPATCH_N(g.nx, UTR_CX)

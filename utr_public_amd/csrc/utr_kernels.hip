@@ -42,6 +42,22 @@ __host__ __device__ constexpr int lds_batch(int n, int elem_bytes) {
 // Sign (-1)^v for integer v.
 __device__ __forceinline__ int par_sign(int v) { return (v & 1) ? -1 : 1; }
 
+// XCD-aware tiling of a 1D launch: workgroups are dealt to the 8 XCDs round
+// robin (workgroup L runs on XCD L % 8), so each XCD is given one contiguous
+// range of tiles (x fastest).  Neighbouring column groups, which share
+// 128-byte lines, are then fetched through the same L2.  Launch
+// xcd_grid(gx * gy) workgroups; xcd_tile returns false for the padding.
+__host__ __device__ constexpr unsigned xcd_grid(unsigned tiles) { return (tiles + 7u) & ~7u; }
+
+__device__ __forceinline__ bool xcd_tile(int gx, int gy, int& tx, int& ty) {
+    const unsigned L = blockIdx.x, T = gridDim.x;
+    const unsigned w = (L & 7u) * (T >> 3) + (L >> 3);
+    if (w >= (unsigned)gx * (unsigned)gy) return false;
+    tx = (int)(w % (unsigned)gx);
+    ty = (int)(w / (unsigned)gx);
+    return true;
+}
+
 // ------------------------------------------------------------- 2D stage
 
 // Rows: renormalise, clamp, Log0 (UTR.cpp:951-958, 991), checkerboard
@@ -83,14 +99,16 @@ template <typename T, int N>
 __global__ void __launch_bounds__(256) k_cols(cplx<T>* __restrict__ spec, int nx, GeomDev g,
                                               const RowParams* __restrict__ rows, int row0,
                                               const cplx<T>* __restrict__ tw,
-                                              unsigned* __restrict__ rowmax) {
+                                              unsigned* __restrict__ rowmax, int gx, int gy) {
     constexpr int NB = lds_batch(N, sizeof(cplx<T>));
     __shared__ cplx<T> sm[2 * NB * N];
     cplx<T>* a = sm;
     cplx<T>* b = sm + NB * N;
     const int ny = N;
-    const int row = blockIdx.y;
-    const int x0 = blockIdx.x * NB;
+    int tx, ty;
+    if (!xcd_tile(gx, gy, tx, ty)) return;
+    const int row = ty;
+    const int x0 = tx * NB;
     const int nb = min(NB, nx - x0);
     cplx<T>* base = spec + (size_t)row * ny * nx + x0;
     for (int t = threadIdx.x; t < nb * N; t += blockDim.x) {
@@ -648,6 +666,87 @@ __global__ void __launch_bounds__(256, 3) k_grid_fx(const cplx<float>* __restric
                     ph[1] = R.ph[1];
                     ph[2] = R.ph[2];
                 }
+                if (R.y_exact) {
+                    // No rotation about Z: the point's y grid coordinate is ny/2 + b
+                    // exactly (weight 1 on its own plane, 0 on the next), so only the
+                    // four (x, z) corners can receive weight.  Same expressions as
+                    // grid_pos, so the result is bit-identical to the general path.
+                    const int jjg = b + g.nyd2;
+                    const int ly = jjg - y0;
+                    if (jjg <= g.ny - 2 && ly >= 0 && ly < SBY) {
+                        const double bd = (double)b;
+                        const double qy = __dmul_rn(g.fyst, bd);
+                        const double qy2 = __dmul_rn(qy, qy);
+                        cplx<float> kn = {0.f, 0.f};
+                        if (a0 <= a1) kn = Kline[a0];
+                        for (int a = a0; a <= a1; a++) {
+                            const cplx<float> kraw = kn;
+                            if (a < a1) kn = Kline[a + 1];
+                            const double ad = (double)a;
+                            const double qx = __dmul_rn(g.fxst, ad);
+                            const double zz = g.escc ? __dmul_rn(-g.wl2, __fma_rn(qx, qx, qy2)) : 0.0;
+                            const double ox = __fma_rn(ad, m[0], __fma_rn(bd, m[1], __dmul_rn(zz, m[2])));
+                            const double oz = __fma_rn(ad, m[6], __dmul_rn(zz, m[8]));
+                            const double dx1 = __dadd_rn((double)g.nxd2, ox);
+                            const double dz1 = __dadd_rn((double)g.nzd2, oz);
+                            const double fnn = floor(dz1), fii = floor(dx1);
+                            if (fnn < 0.0 || fnn > (double)(g.nz - 2)) continue;
+                            if (fii < 0.0 || fii > (double)(g.nx - 2)) continue;
+                            const int ii = (int)fii - x0, nn = (int)fnn - z0;
+                            if (ii < -1 || ii >= SBX || nn < -1 || nn >= SBZ) continue;
+                            const double fx = dx1 - fii, fz = dz1 - fnn;
+                            const double wxs[2] = {1.0 - fx, fx}, wzs[2] = {1.0 - fz, fz};
+                            cplx<float> kv = kraw;
+                            if (g.rot_centre_shift) {
+                                const double phs = -(ph[0] * ox + ph[1] * bd + ph[2] * oz);
+                                double sn, c;
+                                sincos(phs, &sn, &c);
+                                const float cf = (float)c, sf = (float)sn;
+                                kv = {kraw.x * cf - kraw.y * sf, kraw.x * sf + kraw.y * cf};
+                            }
+                            long long tr[4], ti[4], tw[4];
+                            float wc[4];
+                            bool big[4];
+                            bool any_tiny = false;
+#pragma unroll
+                            for (int c = 0; c < 4; c++) {
+                                const int cx = c & 1, cz = c >> 1;
+                                const bool inb = (unsigned)(ii + cx) < (unsigned)SBX &&
+                                                 (unsigned)(nn + cz) < (unsigned)SBZ;
+                                const float w = (float)(wxs[cx] * 1.0 * wzs[cz]);
+                                big[c] = inb && w >= S_TINY_W;
+                                any_tiny |= inb && w != 0.f && w < S_TINY_W;
+                                wc[c] = w;
+                                tr[c] = dround_i64((double)(w * kv.x) * vscale);
+                                ti[c] = dround_i64((double)(w * kv.y) * vscale);
+                                tw[c] = dround_i64((double)w * wscale);
+                            }
+                            const int li0 = lds_idx(ii, ly, nn);
+#pragma unroll
+                            for (int c = 0; c < 4; c++) {
+                                if (!big[c]) continue;
+                                const int li = li0 + (c & 1) + SPZ * (c >> 1);
+                                atomicAdd(&acc[li], (unsigned long long)tr[c]);
+                                atomicAdd(&acc[SPAD + li], (unsigned long long)ti[c]);
+                                atomicAdd(&acc[2 * SPAD + li], (unsigned long long)tw[c]);
+                            }
+                            if (any_tiny) {
+#pragma unroll
+                                for (int c = 0; c < 4; c++) {
+                                    const int lx = ii + (c & 1), lz = nn + (c >> 1);
+                                    const float w = wc[c];
+                                    if ((unsigned)lx >= (unsigned)SBX || (unsigned)lz >= (unsigned)SBZ ||
+                                        w == 0.f || w >= S_TINY_W)
+                                        continue;
+                                    const int v = lx + SBX * (ly + SBY * lz);
+                                    atomicAdd(&tiny[v], w * kv.x);
+                                    atomicAdd(&tiny[SVOX + v], w * kv.y);
+                                    atomicAdd(&tiny[2 * SVOX + v], w);
+                                }
+                            }
+                        }
+                    }
+                } else {
                 // the next point's value is loaded while this one is splatted
                 cplx<float> knext = {0.f, 0.f};
                 if (a0 <= a1) knext = Kline[a0];
@@ -716,6 +815,7 @@ __global__ void __launch_bounds__(256, 3) k_grid_fx(const cplx<float>* __restric
                             atomicAdd(&tiny[2 * SVOX + v], w);
                         }
                     }
+                }
                 }
             }
         }
@@ -807,14 +907,16 @@ constexpr int CW_MAX = 16;
 template <typename T, int NZ>
 __global__ void __launch_bounds__(256) k_pack_z(const cplx<T>* __restrict__ V,
                                                 cplx<T>* __restrict__ H, GeomDev g,
-                                                const cplx<T>* __restrict__ tw) {
+                                                const cplx<T>* __restrict__ tw, int gx, int gy) {
     constexpr int CW = lds_batch(NZ, sizeof(cplx<T>));
     __shared__ cplx<T> sm[2 * CW * NZ];
     cplx<T>* a = sm;
     cplx<T>* b = sm + CW * NZ;
     const int nx = g.nx, ny = g.ny, W = nx / 2 + 1;
-    const int j = blockIdx.y;
-    const int w0 = blockIdx.x * CW;
+    int tx, ty;
+    if (!xcd_tile(gx, gy, tx, ty)) return;
+    const int j = ty;
+    const int w0 = tx * CW;
     const int nb = min(CW, W - w0);
     const int ja = (j == 0) ? 0 : ny - j;
     for (int t = threadIdx.x; t < nb * NZ; t += blockDim.x) {
@@ -852,14 +954,16 @@ __global__ void __launch_bounds__(256) k_pack_z(const cplx<T>* __restrict__ V,
 // Inverse FFT along y of the half spectrum, in place.
 template <typename T, int NY>
 __global__ void __launch_bounds__(256) k_fft_y(cplx<T>* __restrict__ H, GeomDev g,
-                                               const cplx<T>* __restrict__ tw) {
+                                               const cplx<T>* __restrict__ tw, int gx, int gy) {
     constexpr int CW = lds_batch(NY, sizeof(cplx<T>));
     __shared__ cplx<T> sm[2 * CW * NY];
     cplx<T>* a = sm;
     cplx<T>* b = sm + CW * NY;
     const int W = g.nx / 2 + 1;
-    const int k = blockIdx.y;
-    const int w0 = blockIdx.x * CW;
+    int tx, ty;
+    if (!xcd_tile(gx, gy, tx, ty)) return;
+    const int k = ty;
+    const int w0 = tx * CW;
     const int nb = min(CW, W - w0);
     cplx<T>* base = H + (size_t)k * NY * W + w0;
     for (int t = threadIdx.x; t < nb * NY; t += blockDim.x) {
@@ -880,14 +984,16 @@ __global__ void __launch_bounds__(256) k_fft_y(cplx<T>* __restrict__ H, GeomDev 
 template <typename T, int NX, typename TO>
 __global__ void __launch_bounds__(256) k_c2r_x(const cplx<T>* __restrict__ H,
                                                TO* __restrict__ vol, GeomDev g,
-                                               const cplx<T>* __restrict__ tw) {
+                                               const cplx<T>* __restrict__ tw, int gx, int gy) {
     constexpr int NB = lds_batch(NX, sizeof(cplx<T>));
     __shared__ cplx<T> sm[2 * NB * NX];
     cplx<T>* a = sm;
     cplx<T>* b = sm + NB * NX;
     const int ny = g.ny, W = NX / 2 + 1;
-    const int k = blockIdx.y;
-    const int p0 = blockIdx.x * NB;  // first row pair
+    int tx, ty;
+    if (!xcd_tile(gx, gy, tx, ty)) return;
+    const int k = ty;
+    const int p0 = tx * NB;  // first row pair
     const int np = min(NB, ny / 2 - p0);
     for (int t = threadIdx.x; t < np * NX; t += blockDim.x) {
         const int pp = t / NX, w = t - pp * NX;
