@@ -110,16 +110,20 @@ def main():
     value = work / (elapsed / args.steps)
     vol32 = d_vol.cpu().numpy()
 
-    # dominant kernel: the brick gridding (k_grid_fx), timed with HIP events on
-    # the library's stream.  Algorithmic bytes per launch: every spectrum point
-    # read once (8 B) + the filtered V3 written once (8 B per voxel).
+    # dominant kernel: the gridding, timed with HIP events on the library's
+    # stream.  With every row about y the library grids in y-plane tiles
+    # (k_grid_yp), otherwise in bricks (k_grid_fx).  Algorithmic bytes per
+    # launch: every spectrum point read once (8 B) + the filtered V3 written
+    # once (8 B per voxel).
+    gk = "k_grid_yp" if (np.all(np.asarray(rows["angZ"]) == 0)
+                         and os.environ.get("UTR_GRID_FORM") != "brick") else "k_grid_fx"
     g_ms = float(np.mean(grid_ms))
     g_bytes = 8.0 * A * nx * ny + 8.0 * nx * ny * nz
     achieved = g_bytes / (g_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": measured_traffic("k_grid_fx"),
-                "kernel": "k_grid_fx", "kernel_ms": round(g_ms, 4),
+                "traffic": measured_traffic(gk),
+                "kernel": gk, "kernel_ms": round(g_ms, 4),
                 "algorithmic_bytes": g_bytes}
 
     extra = {"phase_ms": {"fft2d_ctf": round(float(np.mean(fft2_ms)), 4),

@@ -59,7 +59,7 @@ template <int N>
 __global__ void __launch_bounds__(256) k_dht_rows(const float* __restrict__ img,
                                                   cplx<float>* __restrict__ out, int ny, DhtGeom g,
                                                   const cplx<float>* __restrict__ tw) {
-    constexpr int NB = lds_batch(N, sizeof(cplx<float>));
+    constexpr int NB = lds_batch(N, sizeof(cplx<float>), LDS_DHT);
     __shared__ cplx<float> sm[2 * NB * N];
     cplx<float>* a = sm;
     cplx<float>* b = sm + NB * N;
@@ -94,7 +94,7 @@ __global__ void __launch_bounds__(256) k_dht_rows(const float* __restrict__ img,
 template <int N>
 __global__ void __launch_bounds__(256) k_dht_cols(cplx<float>* __restrict__ data, int nx,
                                                   const cplx<float>* __restrict__ tw, int sign) {
-    constexpr int NB = lds_batch(N, sizeof(cplx<float>));
+    constexpr int NB = lds_batch(N, sizeof(cplx<float>), LDS_DHT);
     __shared__ cplx<float> sm[2 * NB * N];
     cplx<float>* a = sm;
     cplx<float>* b = sm + NB * N;
@@ -120,7 +120,7 @@ __global__ void __launch_bounds__(256) k_dht_prop_rows(const cplx<float>* __rest
                                                        cplx<float>* __restrict__ T, DhtGeom g,
                                                        const DhtRow* __restrict__ rows, int a0,
                                                        const cplx<float>* __restrict__ tw) {
-    constexpr int NB = lds_batch(N, sizeof(cplx<float>));
+    constexpr int NB = lds_batch(N, sizeof(cplx<float>), LDS_DHT);
     __shared__ cplx<float> sm[2 * NB * N];
     cplx<float>* a = sm;
     cplx<float>* b = sm + NB * N;
@@ -172,7 +172,7 @@ __global__ void __launch_bounds__(256) k_dht_prop_cols(const cplx<float>* __rest
                                                        const DhtRow* __restrict__ rows, int a0,
                                                        const cplx<float>* __restrict__ tw,
                                                        unsigned* __restrict__ pmax) {
-    constexpr int NB = lds_batch(N, sizeof(cplx<float>));
+    constexpr int NB = lds_batch(N, sizeof(cplx<float>), LDS_DHT);
     __shared__ cplx<float> sm[2 * NB * N];
     __shared__ float s_wmax[4];
     cplx<float>* a = sm;
@@ -528,7 +528,7 @@ int dht_add_rows(dht_ctx* c, int first_row, int count, const float* d_img) {
     bool ok = false;
 #define DHT_ROWS(N)                                                                              \
     {                                                                                            \
-        constexpr int NB = utr::lds_batch(N, sizeof(cplx<float>));                               \
+        constexpr int NB = utr::lds_batch(N, sizeof(cplx<float>), utr::LDS_DHT);                               \
         utr::k_dht_rows<N><<<dim3((g.ny + NB - 1) / NB, count), 256, 0, c->stream>>>(            \
             d_img, c->spec.p + (size_t)first_row * g.ny * g.nx, g.ny, g, c->twx.p);              \
         ok = true;                                                                               \
@@ -539,7 +539,7 @@ int dht_add_rows(dht_ctx* c, int first_row, int count, const float* d_img) {
     ok = false;
 #define DHT_COLS(N)                                                                              \
     {                                                                                            \
-        constexpr int NB = utr::lds_batch(N, sizeof(cplx<float>));                               \
+        constexpr int NB = utr::lds_batch(N, sizeof(cplx<float>), utr::LDS_DHT);                               \
         utr::k_dht_cols<N><<<dim3((g.nx + NB - 1) / NB, count), 256, 0, c->stream>>>(            \
             c->spec.p + (size_t)first_row * g.ny * g.nx, g.nx, c->twy.p, -1);                    \
         ok = true;                                                                               \
@@ -576,7 +576,7 @@ int dht_run(dht_ctx* c, float* d_vol) {
         bool ok = false;
 #define DHT_PR(N)                                                                                \
     {                                                                                            \
-        constexpr int NB = utr::lds_batch(N, sizeof(cplx<float>));                               \
+        constexpr int NB = utr::lds_batch(N, sizeof(cplx<float>), utr::LDS_DHT);                               \
         utr::k_dht_prop_rows<N><<<dim3((g.ny + NB - 1) / NB, g.nplanes, na), 256, 0, c->stream>>>( \
             c->spec.p, c->T.p, g, c->rows.p, a0, c->twx.p);                                      \
         ok = true;                                                                               \
@@ -587,7 +587,7 @@ int dht_run(dht_ctx* c, float* d_vol) {
         ok = false;
 #define DHT_PC(N)                                                                                \
     {                                                                                            \
-        constexpr int NB = utr::lds_batch(N, sizeof(cplx<float>));                               \
+        constexpr int NB = utr::lds_batch(N, sizeof(cplx<float>), utr::LDS_DHT);                               \
         utr::k_dht_prop_cols<N><<<dim3((g.nx + NB - 1) / NB, g.nplanes, na), 256, 0, c->stream>>>( \
             c->T.p, c->P.p, g, c->rows.p, a0, c->twy.p, c->pmax.p);                              \
         ok = true;                                                                               \

@@ -166,7 +166,7 @@ int launch_2d(utr_ctx* c, const float* d_img, int row0, int count, cplx<T>* d_sp
     bool ok = false;
 #define UTR_ROWS(N)                                                                             \
     {                                                                                           \
-        constexpr int NB = utr::lds_batch(N, sizeof(cplx<T>));                                  \
+        constexpr int NB = utr::lds_batch(N, sizeof(cplx<T>), utr::LDS_ROWS);                      \
         dim3 grid((g.ny + NB - 1) / NB, count);                                                 \
         utr::k_rows<T, N><<<grid, 256, 0, c->stream>>>(d_img, d_spec, g.ny, g, twx);            \
         ok = true;                                                                              \
@@ -177,7 +177,7 @@ int launch_2d(utr_ctx* c, const float* d_img, int row0, int count, cplx<T>* d_sp
     ok = false;
 #define UTR_COLS(N)                                                                             \
     {                                                                                           \
-        constexpr int NB = utr::lds_batch(N, sizeof(cplx<T>));                                  \
+        constexpr int NB = utr::lds_batch(N, sizeof(cplx<T>), utr::LDS_COLS);                      \
         const int gx = (g.nx + NB - 1) / NB;                                                    \
         utr::k_cols<T, N><<<utr::xcd_grid(gx * count), 256, 0, c->stream>>>(                   \
             d_spec, g.nx, g, c->rows.p, row0, twy, c->rowmax.p, gx, count);                     \
@@ -199,7 +199,7 @@ int launch_3d(utr_ctx* c, const cplx<T>* d_V, cplx<T>* d_H, TO* d_vol, const cpl
     bool ok = false;
 #define UTR_PZ(N)                                                                               \
     {                                                                                           \
-        constexpr int CW = utr::lds_batch(N, sizeof(cplx<T>));                                  \
+        constexpr int CW = utr::lds_batch(N, sizeof(cplx<T>), utr::LDS_PACKZ);                     \
         const int gx = (W + CW - 1) / CW;                                                       \
         utr::k_pack_z<T, N><<<utr::xcd_grid(gx * g.ny), 256, 0, c->stream>>>(d_V, d_H, g, twz,  \
                                                                              gx, g.ny);         \
@@ -211,7 +211,7 @@ int launch_3d(utr_ctx* c, const cplx<T>* d_V, cplx<T>* d_H, TO* d_vol, const cpl
     ok = false;
 #define UTR_FY(N)                                                                               \
     {                                                                                           \
-        constexpr int CW = utr::lds_batch(N, sizeof(cplx<T>));                                  \
+        constexpr int CW = utr::lds_batch(N, sizeof(cplx<T>), utr::LDS_FFTY);                      \
         const int gx = (W + CW - 1) / CW;                                                       \
         utr::k_fft_y<T, N><<<utr::xcd_grid(gx * g.nz), 256, 0, c->stream>>>(d_H, g, twy, gx,    \
                                                                             g.nz);              \
@@ -223,7 +223,7 @@ int launch_3d(utr_ctx* c, const cplx<T>* d_V, cplx<T>* d_H, TO* d_vol, const cpl
     ok = false;
 #define UTR_CX(N)                                                                               \
     {                                                                                           \
-        constexpr int NB = utr::lds_batch(N, sizeof(cplx<T>));                                  \
+        constexpr int NB = utr::lds_batch(N, sizeof(cplx<T>), utr::LDS_C2R);                       \
         const int gx = (g.ny / 2 + NB - 1) / NB;                                                \
         utr::k_c2r_x<T, N, TO><<<utr::xcd_grid(gx * g.nz), 256, 0, c->stream>>>(d_H, d_vol, g,  \
                                                                                twx, gx, g.nz);  \
@@ -278,6 +278,34 @@ int check_rows(utr_ctx* c, int first_row, int count) {
     return UTR_OK;
 }
 
+// Gridding of the FP32 form on the context's stream: y-plane tiles when no
+// row rotates about Z, bricks otherwise.  UTR_GRID_FORM=brick in the
+// environment selects the bricks for every job (both give the same sums).
+int launch_grid32(utr_ctx* c, float* d_S) {
+    const auto& g = c->plan.g;
+    const char* form = std::getenv("UTR_GRID_FORM");
+    const bool brick = form && std::strcmp(form, "brick") == 0;
+    if (c->plan.all_y_exact && !brick) {
+        const long ntiles = (long)((g.nx + utr::PX - 1) / utr::PX) * ((g.nz + utr::PZ - 1) / utr::PZ);
+        const long nwg = ntiles * g.ny;
+        if (nwg <= 0 || nwg > 0x7ffffff0L) return set_err(UTR_E_INVALID, "volume too large");
+        if (g.rot_centre_shift)
+            utr::k_grid_yp<true><<<utr::xcd_grid((unsigned)nwg), 256, 0, c->stream>>>(
+                c->spec32.p, c->rows.p, c->plan.nrows, c->rowmax.p, g, c->V32.p, d_S, (int)ntiles);
+        else
+            utr::k_grid_yp<false><<<utr::xcd_grid((unsigned)nwg), 256, 0, c->stream>>>(
+                c->spec32.p, c->rows.p, c->plan.nrows, c->rowmax.p, g, c->V32.p, d_S, (int)ntiles);
+    } else {
+        const long nblocks = (long)((g.nx + utr::SBX - 1) / utr::SBX) *
+                             ((g.ny + utr::SBY - 1) / utr::SBY) * ((g.nz + utr::SBZ - 1) / utr::SBZ);
+        if (nblocks <= 0 || nblocks > 0x7fffffffL) return set_err(UTR_E_INVALID, "volume too large");
+        utr::k_grid_fx<<<(unsigned)nblocks, 256, 0, c->stream>>>(c->spec32.p, c->rows.p, c->plan.nrows,
+                                                                 c->rowmax.p, g, c->V32.p, d_S);
+    }
+    UTR_HIP_TRY(hipGetLastError());
+    return UTR_OK;
+}
+
 int do_finalize(utr_ctx* c, float* d_vol_out, double* d_vol64_out) {
     const auto& g = c->plan.g;
     if (c->n_added != c->plan.nrows)
@@ -289,11 +317,8 @@ int do_finalize(utr_ctx* c, float* d_vol_out, double* d_vol64_out) {
         if (c->keep_sampling) UTR_HIP_TRY(c->S32.alloc(N));
         float* d_S = c->keep_sampling ? c->S32.p : nullptr;
         if (c->precision == UTR_FP32) {
-            const long nblocks = (long)((g.nx + utr::SBX - 1) / utr::SBX) *
-                                 ((g.ny + utr::SBY - 1) / utr::SBY) * ((g.nz + utr::SBZ - 1) / utr::SBZ);
-            if (nblocks <= 0 || nblocks > 0x7fffffffL) return set_err(UTR_E_INVALID, "volume too large");
-            utr::k_grid_fx<<<(unsigned)nblocks, 256, 0, c->stream>>>(
-                c->spec32.p, c->rows.p, c->plan.nrows, c->rowmax.p, g, c->V32.p, d_S);
+            const int rc = launch_grid32(c, d_S);
+            if (rc) return rc;
         } else {
             const long nblocks = (long)((g.nx + utr::GBX - 1) / utr::GBX) *
                                  ((g.ny + utr::GBY - 1) / utr::GBY) * ((g.nz + utr::GBZ - 1) / utr::GBZ);
@@ -395,17 +420,21 @@ int utr_create(const utr_params* params, int nrows, const double* angZ_deg, cons
         if (c->spec32.alloc((size_t)nrows * n2) != hipSuccess || c->V32.alloc(N) != hipSuccess ||
             c->H32.alloc(nh) != hipSuccess || c->rowmax.alloc(nrows) != hipSuccess)
             return fail(set_err(UTR_E_NOMEM, "device allocation of the FP32 buffers failed"));
-        if (hipMemset(c->rowmax.p, 0, sizeof(unsigned) * nrows) != hipSuccess)
-            return fail(set_err(UTR_E_DEVICE, "hipMemset failed"));
+        // on the context's stream: it is non-blocking, so a null-stream memset
+        // would not be ordered before the kernels launched on it
+        if (hipMemsetAsync(c->rowmax.p, 0, sizeof(unsigned) * nrows, c->stream) != hipSuccess)
+            return fail(set_err(UTR_E_DEVICE, "hipMemsetAsync failed"));
     } else {
         if ((rc = upload_twiddles<double>(c, c->twx64, c->twy64, c->twz64))) return fail(rc);
         if (c->spec64.alloc(n2) != hipSuccess || c->V64.alloc(N) != hipSuccess ||
             c->S64.alloc(N) != hipSuccess || c->H64.alloc(nh) != hipSuccess)
             return fail(set_err(UTR_E_NOMEM, "device allocation of the FP64 buffers failed"));
-        if (hipMemset(c->V64.p, 0, sizeof(cplx<double>) * N) != hipSuccess ||
-            hipMemset(c->S64.p, 0, sizeof(double) * N) != hipSuccess)
-            return fail(set_err(UTR_E_DEVICE, "hipMemset failed"));
+        if (hipMemsetAsync(c->V64.p, 0, sizeof(cplx<double>) * N, c->stream) != hipSuccess ||
+            hipMemsetAsync(c->S64.p, 0, sizeof(double) * N, c->stream) != hipSuccess)
+            return fail(set_err(UTR_E_DEVICE, "hipMemsetAsync failed"));
     }
+    if (hipStreamSynchronize(c->stream) != hipSuccess)
+        return fail(set_err(UTR_E_DEVICE, "hipStreamSynchronize failed"));
     *out = c;
     return UTR_OK;
 }
@@ -592,10 +621,8 @@ int enqueue_job(utr_ctx* c, const float* d_img, float* d_vol) {
     int rc = launch_2d<float>(c, d_img, 0, nrows, c->spec32.p, c->twx32.p, c->twy32.p);
     if (rc) return rc;
     if (c->precision == UTR_FP32) {
-        const long nblocks = (long)((g.nx + utr::SBX - 1) / utr::SBX) *
-                             ((g.ny + utr::SBY - 1) / utr::SBY) * ((g.nz + utr::SBZ - 1) / utr::SBZ);
-        utr::k_grid_fx<<<(unsigned)nblocks, 256, 0, c->stream>>>(c->spec32.p, c->rows.p, nrows,
-                                                                 c->rowmax.p, g, c->V32.p, nullptr);
+        rc = launch_grid32(c, nullptr);
+        if (rc) return rc;
     } else {
         const long nblocks = (long)((g.nx + utr::GBX - 1) / utr::GBX) *
                              ((g.ny + utr::GBY - 1) / utr::GBY) * ((g.nz + utr::GBZ - 1) / utr::GBZ);

@@ -13,7 +13,8 @@
 //
 // Reference stages (MultisliceCppTest/PhaseRetrieval):
 //   k_rows / k_cols        UTR.cpp:951-1019  (Log0, Shuffle, FFT, 2D filters)
-//   k_grid_fx, k_gather    UTR.h:188-307 gridding + UTR.cpp:1050-1201
+//   k_grid_yp, k_grid_fx,  UTR.h:188-307 gridding + UTR.cpp:1050-1201
+//   k_gather               (y-plane tiles, bricks, per-voxel gather)
 //   k_scatter, k_normalize plain FP64 form of the same
 //   k_pack_z / k_fft_y / k_c2r_x   UTR.cpp:1203-1279
 #include <hip/hip_runtime.h>
@@ -34,9 +35,15 @@ __device__ __forceinline__ float dev_log<float>(float x) { return logf(x); }
 template <>
 __device__ __forceinline__ double dev_log<double>(float x) { return log((double)x); }
 
-__host__ __device__ constexpr int lds_batch(int n, int elem_bytes) {
-    // sequences per LDS buffer: at most 16 KiB per buffer, between 1 and 16
-    return (16384 / (n * elem_bytes)) < 1 ? 1 : ((16384 / (n * elem_bytes)) > 16 ? 16 : (16384 / (n * elem_bytes)));
+// LDS bytes per FFT buffer, per kernel (measured on gfx950 at 256^3: small
+// batches, and so more resident workgroups, win everywhere except the
+// Hermitian pack, which reads mirrored column pairs)
+constexpr int LDS_ROWS = 8192, LDS_COLS = 8192, LDS_PACKZ = 16384, LDS_FFTY = 8192, LDS_C2R = 8192,
+              LDS_DHT = 16384;
+
+// sequences per LDS buffer of at most `budget` bytes, between 1 and 16
+__host__ __device__ constexpr int lds_batch(int n, int elem_bytes, int budget) {
+    return (budget / (n * elem_bytes)) < 1 ? 1 : ((budget / (n * elem_bytes)) > 16 ? 16 : (budget / (n * elem_bytes)));
 }
 
 // Sign (-1)^v for integer v.
@@ -66,7 +73,7 @@ template <typename T, int N>
 __global__ void __launch_bounds__(256) k_rows(const float* __restrict__ img,
                                               cplx<T>* __restrict__ out, int ny, GeomDev g,
                                               const cplx<T>* __restrict__ tw) {
-    constexpr int NB = lds_batch(N, sizeof(cplx<T>));
+    constexpr int NB = lds_batch(N, sizeof(cplx<T>), LDS_ROWS);
     __shared__ cplx<T> sm[2 * NB * N];
     cplx<T>* a = sm;
     cplx<T>* b = sm + NB * N;
@@ -100,7 +107,7 @@ __global__ void __launch_bounds__(256) k_cols(cplx<T>* __restrict__ spec, int nx
                                               const RowParams* __restrict__ rows, int row0,
                                               const cplx<T>* __restrict__ tw,
                                               unsigned* __restrict__ rowmax, int gx, int gy) {
-    constexpr int NB = lds_batch(N, sizeof(cplx<T>));
+    constexpr int NB = lds_batch(N, sizeof(cplx<T>), LDS_COLS);
     __shared__ cplx<T> sm[2 * NB * N];
     cplx<T>* a = sm;
     cplx<T>* b = sm + NB * N;
@@ -647,7 +654,9 @@ __global__ void __launch_bounds__(256, 3) k_grid_fx(const cplx<float>* __restric
                         if (fabsf(Pd) < 1e-6f) {
                             if (lo_d > 0.f || hi_d < 0.f) alo = ahi + 1.f;
                         } else {
-                            float t0 = lo_d / Pd, t1 = hi_d / Pd;
+                            // the margins above exceed the 1-ulp error of the reciprocal
+                            const float rp = __builtin_amdgcn_rcpf(Pd);
+                            float t0 = lo_d * rp, t1 = hi_d * rp;
                             if (Pd < 0.f) { const float tt = t0; t0 = t1; t1 = tt; }
                             alo = fmaxf(alo, ac + t0);
                             ahi = fminf(ahi, ac + t1);
@@ -677,11 +686,8 @@ __global__ void __launch_bounds__(256, 3) k_grid_fx(const cplx<float>* __restric
                         const double bd = (double)b;
                         const double qy = __dmul_rn(g.fyst, bd);
                         const double qy2 = __dmul_rn(qy, qy);
-                        cplx<float> kn = {0.f, 0.f};
-                        if (a0 <= a1) kn = Kline[a0];
-                        for (int a = a0; a <= a1; a++) {
-                            const cplx<float> kraw = kn;
-                            if (a < a1) kn = Kline[a + 1];
+                        // one detector point: position, corner terms, LDS adds
+                        auto splat = [&](const int a, const cplx<float> kraw) {
                             const double ad = (double)a;
                             const double qx = __dmul_rn(g.fxst, ad);
                             const double zz = g.escc ? __dmul_rn(-g.wl2, __fma_rn(qx, qx, qy2)) : 0.0;
@@ -690,10 +696,10 @@ __global__ void __launch_bounds__(256, 3) k_grid_fx(const cplx<float>* __restric
                             const double dx1 = __dadd_rn((double)g.nxd2, ox);
                             const double dz1 = __dadd_rn((double)g.nzd2, oz);
                             const double fnn = floor(dz1), fii = floor(dx1);
-                            if (fnn < 0.0 || fnn > (double)(g.nz - 2)) continue;
-                            if (fii < 0.0 || fii > (double)(g.nx - 2)) continue;
+                            if (fnn < 0.0 || fnn > (double)(g.nz - 2)) return;
+                            if (fii < 0.0 || fii > (double)(g.nx - 2)) return;
                             const int ii = (int)fii - x0, nn = (int)fnn - z0;
-                            if (ii < -1 || ii >= SBX || nn < -1 || nn >= SBZ) continue;
+                            if (ii < -1 || ii >= SBX || nn < -1 || nn >= SBZ) return;
                             const double fx = dx1 - fii, fz = dz1 - fnn;
                             const double wxs[2] = {1.0 - fx, fx}, wzs[2] = {1.0 - fz, fz};
                             cplx<float> kv = kraw;
@@ -744,6 +750,19 @@ __global__ void __launch_bounds__(256, 3) k_grid_fx(const cplx<float>* __restric
                                     atomicAdd(&tiny[2 * SVOX + v], w);
                                 }
                             }
+                        };
+                        // two points in flight without register moves between the
+                        // loads and their use (clamped indices keep loads in the line)
+                        cplx<float> kA = {0.f, 0.f}, kB = {0.f, 0.f};
+                        if (a0 <= a1) {
+                            kA = Kline[a0];
+                            kB = Kline[min(a0 + 1, a1)];
+                        }
+                        for (int a = a0; a <= a1; a += 2) {
+                            splat(a, kA);
+                            kA = Kline[min(a + 2, a1)];
+                            if (a + 1 <= a1) splat(a + 1, kB);
+                            kB = Kline[min(a + 3, a1)];
                         }
                     }
                 } else {
@@ -838,6 +857,274 @@ __global__ void __launch_bounds__(256, 3) k_grid_fx(const cplx<float>* __restric
     }
 }
 
+// ------------- FP32 scatter into y-plane tiles (rows without Z rotation)
+//
+// When no row rotates about Z, a detector line b reaches exactly one y plane
+// of the volume (j = ny/2 + b, weight 1) and its points move in that plane's
+// (x, z) coordinates.  Each workgroup then owns a PX x PZ tile of one y plane
+// and takes, from every row, the one line that lands in the plane: the line
+// is clipped to the tile once, cut into segments of PSEG points, and the
+// segments are spread over the threads, so that threads get nearly equal
+// work.  Point positions, corner weights and the fixed-point sums are those
+// of k_grid_fx (the same expressions in the same order), so both kernels
+// produce the same V3 and Samp3 sums.  Tiles of one plane are numbered
+// contiguously per XCD, so the plane's spectrum lines are fetched into that
+// XCD's L2 once.
+
+constexpr int PX = 32, PZ = 32;
+constexpr int PVOX = PX * PZ;
+constexpr int POWN = PVOX / 256;
+static_assert(PVOX % 256 == 0, "tile must be a multiple of the workgroup");
+constexpr int PSX = PX + 1;          // LDS stride along z (odd: lanes walking along z spread over banks)
+constexpr int PPAD = PSX * PZ;       // LDS entries per component
+constexpr int P_LIST = 256;          // rows tested per chunk
+constexpr int PSEG = 4;              // points per segment (short segments balance the threads)
+
+// Fixed-point scales 2^sv (V3) and 2^sw (Samp3) from the largest spectrum
+// component and the row count.  Per row a voxel receives a total weight
+// below 8, so |sum| < 8 nrows max|K| 2^sv < 2^62, and each term stays below
+// 2^51 for dround_i64.  Ends with a barrier.
+__device__ __forceinline__ void fx_scales(const unsigned* __restrict__ rowmax, int nrows,
+                                          float* s_red, int& sv, int& sw) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    float mk = 0.f;
+    for (int r = tid; r < nrows; r += blockDim.x) mk = fmaxf(mk, __uint_as_float(rowmax[r]));
+    for (int off = 32; off > 0; off >>= 1) mk = fmaxf(mk, __shfl_xor(mk, off));
+    if (lane == 0) s_red[wave] = mk;
+    __syncthreads();
+    mk = fmaxf(fmaxf(s_red[0], s_red[1]), fmaxf(s_red[2], s_red[3]));
+    int ek = 0, ea = 0;
+    frexpf(mk > 0.f ? mk : 1.f, &ek);    // max|K| < 2^ek
+    frexpf((float)(8 * nrows), &ea);      // 8 nrows < 2^ea
+    sv = min(61 - ek - ea, 50 - ek);
+    sw = min(61 - ea, 50);
+}
+
+// ROT: rows carry the centre-shift phase (compiled separately: its FP64
+// sincos would otherwise set the register budget of the common case).
+template <bool ROT>
+__global__ void __launch_bounds__(256) k_grid_yp(const cplx<float>* __restrict__ spec,
+                                                 const RowParams* __restrict__ rows, int nrows,
+                                                 const unsigned* __restrict__ rowmax, GeomDev g,
+                                                 cplx<float>* __restrict__ Vout,
+                                                 float* __restrict__ Sout, int ntiles) {
+    __shared__ unsigned long long acc[3 * PPAD];
+    __shared__ float tiny[3 * PVOX];
+    __shared__ float s_red[4];
+    __shared__ unsigned short s_row[P_LIST];  // rows whose line reaches the tile
+    __shared__ short s_a0[P_LIST], s_a1[P_LIST];  // clipped a-interval of that line
+    __shared__ int s_pre[P_LIST + 1];             // segment counts, then their prefix
+    __shared__ int s_cnt[4], s_wsum[4];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int tile, j;
+    if (!xcd_tile(ntiles, g.ny, tile, j)) return;
+    const int ntx = (g.nx + PX - 1) / PX;
+    const int x0 = (tile % ntx) * PX, z0 = (tile / ntx) * PZ;
+    for (int t = tid; t < 3 * PPAD; t += 256) acc[t] = 0ull;
+    for (int t = tid; t < 3 * PVOX; t += 256) tiny[t] = 0.f;
+    int sv, sw;
+    fx_scales(rowmax, nrows, s_red, sv, sw);
+    const double vscale = ldexp(1.0, sv), wscale = ldexp(1.0, sw);
+
+    const float fx2 = (float)(g.fxst * g.fxst), fy2 = (float)(g.fyst * g.fyst);
+    const float wl2f = (float)g.wl2;
+    // the plane's line: b = j - ny/2; the last plane gets no weight (its
+    // cells would need plane ny)
+    const int b = j - g.nyd2;
+    const bool plane_ok = j <= g.ny - 2;
+    const float fb = (float)b;
+    const float bmin[2] = {(float)x0, (float)z0};
+    const float bmax[2] = {(float)(x0 + PX - 1), (float)(z0 + PZ - 1)};
+    const float cgrid[2] = {(float)g.nxd2, (float)g.nzd2};
+    const double bd = (double)b;
+    const double qy = __dmul_rn(g.fyst, bd);
+    const double qy2 = __dmul_rn(qy, qy);
+
+    for (int base = 0; base < nrows && plane_ok; base += P_LIST) {
+        const int nchunk = min(P_LIST, nrows - base);
+        // clip each row's line to the tile: x(a) and z(a) are linear in a up
+        // to the Ewald term m2*zz(a), whose deviation from its tangent at the
+        // interval centre is at most wl2*fx2*W^2; two passes, the second
+        // about the first's interval
+        bool pass = false;
+        int a0 = 0, a1 = -1;
+        if (tid < nchunk) {
+            const RowParams& R = rows[base + tid];
+            float alo = (float)(-g.nxd2), ahi = (float)(g.nx - 1 - g.nxd2);
+            for (int it = 0; it < 2 && alo <= ahi; it++) {
+                const float ac = 0.5f * (alo + ahi), W = 0.5f * (ahi - alo);
+                const float zz0 = g.escc ? -wl2f * (fx2 * ac * ac + fy2 * fb * fb) : 0.f;
+                const float dzz = g.escc ? -2.f * wl2f * fx2 * ac : 0.f;
+                const float curv = g.escc ? wl2f * fx2 * W * W : 0.f;
+#pragma unroll
+                for (int d = 0; d < 2; d++) {
+                    const int rd = d ? 6 : 0;
+                    const float m0 = (float)R.m[rd], m1 = (float)R.m[rd + 1], m2 = (float)R.m[rd + 2];
+                    const float Pc = cgrid[d] + m0 * ac + (d ? 0.f : m1 * fb) + m2 * zz0;
+                    const float Pd = m0 + m2 * dzz;
+                    const float mg = fabsf(m2) * curv * 1.01f + 2e-4f * (1.f + fabsf(Pc)) + 0.01f;
+                    const float lo_d = bmin[d] - 1.f - mg - Pc, hi_d = bmax[d] + 1.f + mg - Pc;
+                    if (fabsf(Pd) < 1e-6f) {
+                        if (lo_d > 0.f || hi_d < 0.f) alo = ahi + 1.f;
+                    } else {
+                        // the margins above exceed the 1-ulp error of the reciprocal
+                        const float rp = __builtin_amdgcn_rcpf(Pd);
+                        float t0 = lo_d * rp, t1 = hi_d * rp;
+                        if (Pd < 0.f) { const float tt = t0; t0 = t1; t1 = tt; }
+                        alo = fmaxf(alo, ac + t0);
+                        ahi = fminf(ahi, ac + t1);
+                    }
+                }
+            }
+            if (alo <= ahi) {
+                a0 = max((int)ceilf(alo - 1e-3f), -g.nxd2);
+                a1 = min((int)floorf(ahi + 1e-3f), g.nx - 1 - g.nxd2);
+            }
+            pass = a0 <= a1;
+        }
+        const unsigned long long mask = __ballot(pass);
+        const int pre = __popcll(mask & ((1ull << lane) - 1ull));
+        if (lane == 0) s_cnt[wave] = __popcll(mask);
+        __syncthreads();
+        int off = 0;
+        for (int w = 0; w < wave; w++) off += s_cnt[w];
+        const int nsg = pass ? (a1 - a0 + PSEG) / PSEG : 0;
+        if (pass) {
+            s_row[off + pre] = (unsigned short)(base + tid);
+            s_a0[off + pre] = (short)a0;
+            s_a1[off + pre] = (short)a1;
+        }
+        // exclusive prefix of the segment counts, in list order (the listed
+        // rows keep their thread order, so the prefix follows the ballot)
+        int incl = nsg;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int v = __shfl_up(incl, o);
+            if (lane >= o) incl += v;
+        }
+        if (lane == 63) s_wsum[wave] = incl;
+        __syncthreads();
+        int woff = 0;
+        for (int w = 0; w < wave; w++) woff += s_wsum[w];
+        if (pass) s_pre[off + pre] = woff + incl - nsg;
+        const int n = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+        const int total = s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
+        if (tid == 0) s_pre[n] = total;
+        __syncthreads();
+        for (int G = 0; G < total; G += 256) {
+            const int gl = G + tid;
+            if (gl >= total) continue;
+            // listed row q holding segment gl: last q with s_pre[q] <= gl
+            int lo = 0, hi = n - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_pre[mid] <= gl) lo = mid;
+                else hi = mid - 1;
+            }
+            const int r = s_row[lo];
+            const int as = s_a0[lo] + (gl - s_pre[lo]) * PSEG;
+            const int ae = min(as + PSEG - 1, (int)s_a1[lo]);
+            const RowParams& R = rows[r];
+            const double m0 = R.m[0], m1 = R.m[1], m2 = R.m[2], m6 = R.m[6], m8 = R.m[8];
+            double ph[3] = {0.0, 0.0, 0.0};
+            if (ROT) {
+                ph[0] = R.ph[0];
+                ph[1] = R.ph[1];
+                ph[2] = R.ph[2];
+            }
+            const cplx<float>* Kline = spec + ((size_t)r * g.ny + j) * g.nx + g.nxd2;
+            auto splat = [&](const int a, const cplx<float> kraw) {
+                const double ad = (double)a;
+                const double qx = __dmul_rn(g.fxst, ad);
+                const double zz = g.escc ? __dmul_rn(-g.wl2, __fma_rn(qx, qx, qy2)) : 0.0;
+                const double ox = __fma_rn(ad, m0, __fma_rn(bd, m1, __dmul_rn(zz, m2)));
+                const double oz = __fma_rn(ad, m6, __dmul_rn(zz, m8));
+                const double dx1 = __dadd_rn((double)g.nxd2, ox);
+                const double dz1 = __dadd_rn((double)g.nzd2, oz);
+                const double fnn = floor(dz1), fii = floor(dx1);
+                if (fnn < 0.0 || fnn > (double)(g.nz - 2)) return;
+                if (fii < 0.0 || fii > (double)(g.nx - 2)) return;
+                const int ii = (int)fii - x0, nn = (int)fnn - z0;
+                if (ii < -1 || ii >= PX || nn < -1 || nn >= PZ) return;
+                const double fx = dx1 - fii, fz = dz1 - fnn;
+                const double wxs[2] = {1.0 - fx, fx}, wzs[2] = {1.0 - fz, fz};
+                cplx<float> kv = kraw;
+                if (ROT) {
+                    const double phs = -(ph[0] * ox + ph[1] * bd + ph[2] * oz);
+                    double sn, c;
+                    sincos(phs, &sn, &c);
+                    const float cf = (float)c, sf = (float)sn;
+                    kv = {kraw.x * cf - kraw.y * sf, kraw.x * sf + kraw.y * cf};
+                }
+                long long tr[4], ti[4], tw[4];
+                float wc[4];
+                bool big[4];
+                bool any_tiny = false;
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    const int cx = c & 1, cz = c >> 1;
+                    const bool inb = (unsigned)(ii + cx) < (unsigned)PX && (unsigned)(nn + cz) < (unsigned)PZ;
+                    const float w = (float)(wxs[cx] * 1.0 * wzs[cz]);
+                    big[c] = inb && w >= S_TINY_W;
+                    any_tiny |= inb && w != 0.f && w < S_TINY_W;
+                    wc[c] = w;
+                    tr[c] = dround_i64((double)(w * kv.x) * vscale);
+                    ti[c] = dround_i64((double)(w * kv.y) * vscale);
+                    tw[c] = dround_i64((double)w * wscale);
+                }
+                const int li0 = ii + PSX * nn;
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    if (!big[c]) continue;
+                    const int li = li0 + (c & 1) + PSX * (c >> 1);
+                    atomicAdd(&acc[li], (unsigned long long)tr[c]);
+                    atomicAdd(&acc[PPAD + li], (unsigned long long)ti[c]);
+                    atomicAdd(&acc[2 * PPAD + li], (unsigned long long)tw[c]);
+                }
+                if (any_tiny) {
+                    // weights below 2^-20: float accumulators (rare)
+#pragma unroll
+                    for (int c = 0; c < 4; c++) {
+                        const int lx = ii + (c & 1), lz = nn + (c >> 1);
+                        const float w = wc[c];
+                        if ((unsigned)lx >= (unsigned)PX || (unsigned)lz >= (unsigned)PZ || w == 0.f ||
+                            w >= S_TINY_W)
+                            continue;
+                        const int v = lx + PX * lz;
+                        atomicAdd(&tiny[v], w * kv.x);
+                        atomicAdd(&tiny[PVOX + v], w * kv.y);
+                        atomicAdd(&tiny[2 * PVOX + v], w);
+                    }
+                }
+            };
+            // two points in flight (clamped indices keep the loads in the line)
+            cplx<float> kA = Kline[as], kB = Kline[min(as + 1, ae)];
+            for (int a = as; a <= ae; a += 2) {
+                splat(a, kA);
+                kA = Kline[min(a + 2, ae)];
+                if (a + 1 <= ae) splat(a + 1, kB);
+                kB = Kline[min(a + 3, ae)];
+            }
+        }
+        __syncthreads();
+    }
+    const double vinv = ldexp(1.0, -sv), winv = ldexp(1.0, -sw);
+#pragma unroll
+    for (int k = 0; k < POWN; k++) {
+        const int v = tid + 256 * k;
+        const int lx = v % PX, lz = v / PX;
+        const int vx = x0 + lx, vz = z0 + lz;
+        if (vx >= g.nx || vz >= g.nz) continue;
+        const int li = lx + PSX * lz;
+        const float vr = (float)((double)(long long)acc[li] * vinv + (double)tiny[v]);
+        const float vi = (float)((double)(long long)acc[PPAD + li] * vinv + (double)tiny[PVOX + v]);
+        const float S = (float)((double)(long long)acc[2 * PPAD + li] * winv + (double)tiny[2 * PVOX + v]);
+        const size_t vidx = ((size_t)vz * g.ny + j) * g.nx + vx;
+        if (Sout) Sout[vidx] = S;
+        Vout[vidx] = voxel_epilogue<float>(g, vx, j, vz, cplx<float>{vr, vi}, S);
+    }
+}
+
 // Zero fill of a small counter array (kept out of hipMemsetAsync so that
 // launch sequences containing it can be captured into a graph).
 __global__ void k_fill_u32(unsigned* __restrict__ p, int n) {
@@ -908,7 +1195,7 @@ template <typename T, int NZ>
 __global__ void __launch_bounds__(256) k_pack_z(const cplx<T>* __restrict__ V,
                                                 cplx<T>* __restrict__ H, GeomDev g,
                                                 const cplx<T>* __restrict__ tw, int gx, int gy) {
-    constexpr int CW = lds_batch(NZ, sizeof(cplx<T>));
+    constexpr int CW = lds_batch(NZ, sizeof(cplx<T>), LDS_PACKZ);
     __shared__ cplx<T> sm[2 * CW * NZ];
     cplx<T>* a = sm;
     cplx<T>* b = sm + CW * NZ;
@@ -955,7 +1242,7 @@ __global__ void __launch_bounds__(256) k_pack_z(const cplx<T>* __restrict__ V,
 template <typename T, int NY>
 __global__ void __launch_bounds__(256) k_fft_y(cplx<T>* __restrict__ H, GeomDev g,
                                                const cplx<T>* __restrict__ tw, int gx, int gy) {
-    constexpr int CW = lds_batch(NY, sizeof(cplx<T>));
+    constexpr int CW = lds_batch(NY, sizeof(cplx<T>), LDS_FFTY);
     __shared__ cplx<T> sm[2 * CW * NY];
     cplx<T>* a = sm;
     cplx<T>* b = sm + CW * NY;
@@ -985,7 +1272,7 @@ template <typename T, int NX, typename TO>
 __global__ void __launch_bounds__(256) k_c2r_x(const cplx<T>* __restrict__ H,
                                                TO* __restrict__ vol, GeomDev g,
                                                const cplx<T>* __restrict__ tw, int gx, int gy) {
-    constexpr int NB = lds_batch(NX, sizeof(cplx<T>));
+    constexpr int NB = lds_batch(NX, sizeof(cplx<T>), LDS_C2R);
     __shared__ cplx<T> sm[2 * NB * NX];
     cplx<T>* a = sm;
     cplx<T>* b = sm + NB * NX;
@@ -1016,6 +1303,23 @@ __global__ void __launch_bounds__(256) k_c2r_x(const cplx<T>* __restrict__ H,
     fft_lds<T, NX>(a, b, np, tw, +1, threadIdx.x, blockDim.x);
     const double invN = 1.0 / ((double)NX * ny * g.nz);
     const int gs = par_sign((g.nz + ny) / 2);
+    // The cupping factor exp(dtemp*(di^2 + dj^2 + dk^2)) is the product of one
+    // factor per axis: x factors go to the free FFT buffer, the block's 2*NB
+    // y factors to a small table, and the z factor is one value per block.
+    __shared__ double s_ey[2 * NB];
+    double* s_ex = reinterpret_cast<double*>(b);
+    static_assert(sizeof(double) <= NB * sizeof(cplx<T>), "x factors must fit the FFT scratch buffer");
+    if (!g.no_mask) {
+        for (int m = threadIdx.x; m < NX; m += blockDim.x)
+            s_ex[m] = exp(g.mask_dtemp * (double)((long)(m - g.ic) * (m - g.ic)));
+        if (threadIdx.x < 2 * np) {
+            const int jj = 2 * p0 + threadIdx.x;
+            s_ey[threadIdx.x] = g.mask_j ? exp(g.mask_dtemp * (double)((long)(jj - g.jc) * (jj - g.jc))) : 1.0;
+        }
+        __syncthreads();
+    }
+    const double ez = (g.mask_k && !g.no_mask) ? exp(g.mask_dtemp * (double)((long)(k - g.kc) * (k - g.kc))) : 1.0;
+    const double base_f = g.fnorm * invN * (double)gs;
     for (int t = threadIdx.x; t < np * NX; t += blockDim.x) {
         const int pp = t / NX, m = t - pp * NX;
         const int j = 2 * (p0 + pp);
@@ -1026,14 +1330,14 @@ __global__ void __launch_bounds__(256) k_c2r_x(const cplx<T>* __restrict__ H,
             long ii2 = (long)(m - g.ic) * (m - g.ic);
             if (g.mask_j) ii2 += (long)(jj - g.jc) * (jj - g.jc);
             if (g.mask_k) ii2 += (long)(k - g.kc) * (k - g.kc);
+            const double sgn = (double)par_sign(k + jj);
             TO o;
             if (g.no_mask) {
-                o = TO((double)val * (g.fnorm * invN * (double)(par_sign(k + jj) * gs)));
+                o = TO((double)val * (base_f * sgn));
             } else if (ii2 > g.nR2) {
                 o = TO(0);
             } else {
-                const double f = exp(g.mask_dtemp * (double)ii2) * g.fnorm * invN *
-                                 (double)(par_sign(k + jj) * gs);
+                const double f = s_ex[m] * s_ey[2 * pp + h] * ez * base_f * sgn;
                 o = TO((double)val * f);
             }
             vol[((size_t)k * ny + jj) * NX + m] = o;

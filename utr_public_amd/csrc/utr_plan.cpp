@@ -247,6 +247,9 @@ bool build_plan(const utr_params* p, int nrows, const double* angZ, const double
         R.zout = defocus[r] - p->dzextra;
         R.weight = P.weights[r];
     }
+    P.all_y_exact = nrows > 0 && nrows <= 65535;
+    for (int r = 0; r < nrows; r++)
+        if (!P.rows[r].y_exact) P.all_y_exact = 0;
     return true;
 }
 

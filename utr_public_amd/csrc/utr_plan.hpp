@@ -68,6 +68,7 @@ struct Plan {
     double EE = 0, xst = 0, zst = 0;
     double xc = 0, yc = 0, zc = 0;
     int rot_y_only = 1, rot_z_only = 1, rot_z2_only = 1;
+    int all_y_exact = 0;  // no row rotates about Z (y-plane gridding applies)
 };
 
 // Validates and derives; on failure returns false and sets err.
