@@ -863,11 +863,12 @@ __global__ void __launch_bounds__(256, 3) k_grid_fx(const cplx<float>* __restric
 // of the volume (j = ny/2 + b, weight 1) and its points move in that plane's
 // (x, z) coordinates.  Each workgroup then owns a PX x PZ tile of one y plane
 // and takes, from every row, the one line that lands in the plane: the line
-// is clipped to the tile once, cut into segments of PSEG points, and the
-// segments are spread over the threads, so that threads get nearly equal
-// work.  Point positions, corner weights and the fixed-point sums are those
-// of k_grid_fx (the same expressions in the same order), so both kernels
-// produce the same V3 and Samp3 sums.  Tiles of one plane are numbered
+// is clipped to the tile once, the clipped lines are listed in row order, and
+// each thread takes an equal run of the listed points (crossing from one line
+// to the next, with the next line's coefficients loaded one line ahead), so
+// that all threads get the same work.  Point positions, corner weights and
+// the fixed-point sums are those of k_grid_fx (the same expressions in the
+// same order), so both kernels produce the same V3 and Samp3 sums.  Tiles of one plane are numbered
 // contiguously per XCD, so the plane's spectrum lines are fetched into that
 // XCD's L2 once.
 
@@ -878,7 +879,6 @@ static_assert(PVOX % 256 == 0, "tile must be a multiple of the workgroup");
 constexpr int PSX = PX + 1;          // LDS stride along z (odd: lanes walking along z spread over banks)
 constexpr int PPAD = PSX * PZ;       // LDS entries per component
 constexpr int P_LIST = 256;          // rows tested per chunk
-constexpr int PSEG = 4;              // points per segment (short segments balance the threads)
 
 // Fixed-point scales 2^sv (V3) and 2^sw (Samp3) from the largest spectrum
 // component and the row count.  Per row a voxel receives a total weight
@@ -901,9 +901,10 @@ __device__ __forceinline__ void fx_scales(const unsigned* __restrict__ rowmax, i
 }
 
 // ROT: rows carry the centre-shift phase (compiled separately: its FP64
-// sincos would otherwise set the register budget of the common case).
+// sincos would otherwise set the register budget of the common case, which
+// fits 4 workgroups per CU).
 template <bool ROT>
-__global__ void __launch_bounds__(256) k_grid_yp(const cplx<float>* __restrict__ spec,
+__global__ void __launch_bounds__(256, ROT ? 1 : 4) k_grid_yp(const cplx<float>* __restrict__ spec,
                                                  const RowParams* __restrict__ rows, int nrows,
                                                  const unsigned* __restrict__ rowmax, GeomDev g,
                                                  cplx<float>* __restrict__ Vout,
@@ -912,8 +913,8 @@ __global__ void __launch_bounds__(256) k_grid_yp(const cplx<float>* __restrict__
     __shared__ float tiny[3 * PVOX];
     __shared__ float s_red[4];
     __shared__ unsigned short s_row[P_LIST];  // rows whose line reaches the tile
-    __shared__ short s_a0[P_LIST], s_a1[P_LIST];  // clipped a-interval of that line
-    __shared__ int s_pre[P_LIST + 1];             // segment counts, then their prefix
+    __shared__ short s_a0[P_LIST];                // first a of its clipped line
+    __shared__ int s_pre[P_LIST + 1];             // point counts, then their prefix
     __shared__ int s_cnt[4], s_wsum[4];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -989,15 +990,14 @@ __global__ void __launch_bounds__(256) k_grid_yp(const cplx<float>* __restrict__
         __syncthreads();
         int off = 0;
         for (int w = 0; w < wave; w++) off += s_cnt[w];
-        const int nsg = pass ? (a1 - a0 + PSEG) / PSEG : 0;
+        const int npt = pass ? a1 - a0 + 1 : 0;  // points of the line
         if (pass) {
             s_row[off + pre] = (unsigned short)(base + tid);
             s_a0[off + pre] = (short)a0;
-            s_a1[off + pre] = (short)a1;
         }
-        // exclusive prefix of the segment counts, in list order (the listed
+        // exclusive prefix of the point counts, in list order (the listed
         // rows keep their thread order, so the prefix follows the ballot)
-        int incl = nsg;
+        int incl = npt;
         for (int o = 1; o < 64; o <<= 1) {
             const int v = __shfl_up(incl, o);
             if (lane >= o) incl += v;
@@ -1006,104 +1006,138 @@ __global__ void __launch_bounds__(256) k_grid_yp(const cplx<float>* __restrict__
         __syncthreads();
         int woff = 0;
         for (int w = 0; w < wave; w++) woff += s_wsum[w];
-        if (pass) s_pre[off + pre] = woff + incl - nsg;
+        if (pass) s_pre[off + pre] = woff + incl - npt;
         const int n = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
         const int total = s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
         if (tid == 0) s_pre[n] = total;
         __syncthreads();
-        for (int G = 0; G < total; G += 256) {
-            const int gl = G + tid;
-            if (gl >= total) continue;
-            // listed row q holding segment gl: last q with s_pre[q] <= gl
-            int lo = 0, hi = n - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (s_pre[mid] <= gl) lo = mid;
-                else hi = mid - 1;
-            }
-            const int r = s_row[lo];
-            const int as = s_a0[lo] + (gl - s_pre[lo]) * PSEG;
-            const int ae = min(as + PSEG - 1, (int)s_a1[lo]);
-            const RowParams& R = rows[r];
-            const double m0 = R.m[0], m1 = R.m[1], m2 = R.m[2], m6 = R.m[6], m8 = R.m[8];
-            double ph[3] = {0.0, 0.0, 0.0};
-            if (ROT) {
-                ph[0] = R.ph[0];
-                ph[1] = R.ph[1];
-                ph[2] = R.ph[2];
-            }
-            const cplx<float>* Kline = spec + ((size_t)r * g.ny + j) * g.nx + g.nxd2;
-            auto splat = [&](const int a, const cplx<float> kraw) {
-                const double ad = (double)a;
-                const double qx = __dmul_rn(g.fxst, ad);
-                const double zz = g.escc ? __dmul_rn(-g.wl2, __fma_rn(qx, qx, qy2)) : 0.0;
-                const double ox = __fma_rn(ad, m0, __fma_rn(bd, m1, __dmul_rn(zz, m2)));
-                const double oz = __fma_rn(ad, m6, __dmul_rn(zz, m8));
-                const double dx1 = __dadd_rn((double)g.nxd2, ox);
-                const double dz1 = __dadd_rn((double)g.nzd2, oz);
-                const double fnn = floor(dz1), fii = floor(dx1);
-                if (fnn < 0.0 || fnn > (double)(g.nz - 2)) return;
-                if (fii < 0.0 || fii > (double)(g.nx - 2)) return;
-                const int ii = (int)fii - x0, nn = (int)fnn - z0;
-                if (ii < -1 || ii >= PX || nn < -1 || nn >= PZ) return;
-                const double fx = dx1 - fii, fz = dz1 - fnn;
-                const double wxs[2] = {1.0 - fx, fx}, wzs[2] = {1.0 - fz, fz};
-                cplx<float> kv = kraw;
+        {
+            // each thread takes an equal run of the listed points, in list order
+            const int p0 = (int)(((long long)tid * total) >> 8);
+            const int p1 = (int)(((long long)(tid + 1) * total) >> 8);
+            if (p0 < p1) {
+                int lo = 0, hi = n - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (s_pre[mid] <= p0) lo = mid;
+                    else hi = mid - 1;
+                }
+                int q = lo;
+                // the current row, and the next listed row loaded one row ahead
+                int r = s_row[q];
+                int abase = s_a0[q] - s_pre[q], pend = s_pre[q + 1];
+                double m0 = rows[r].m[0], m1 = rows[r].m[1], m2 = rows[r].m[2], m6 = rows[r].m[6], m8 = rows[r].m[8];
+                double ph[3] = {0.0, 0.0, 0.0};
                 if (ROT) {
-                    const double phs = -(ph[0] * ox + ph[1] * bd + ph[2] * oz);
-                    double sn, c;
-                    sincos(phs, &sn, &c);
-                    const float cf = (float)c, sf = (float)sn;
-                    kv = {kraw.x * cf - kraw.y * sf, kraw.x * sf + kraw.y * cf};
+                    ph[0] = rows[r].ph[0];
+                    ph[1] = rows[r].ph[1];
+                    ph[2] = rows[r].ph[2];
                 }
-                long long tr[4], ti[4], tw[4];
-                float wc[4];
-                bool big[4];
-                bool any_tiny = false;
-#pragma unroll
-                for (int c = 0; c < 4; c++) {
-                    const int cx = c & 1, cz = c >> 1;
-                    const bool inb = (unsigned)(ii + cx) < (unsigned)PX && (unsigned)(nn + cz) < (unsigned)PZ;
-                    const float w = (float)(wxs[cx] * 1.0 * wzs[cz]);
-                    big[c] = inb && w >= S_TINY_W;
-                    any_tiny |= inb && w != 0.f && w < S_TINY_W;
-                    wc[c] = w;
-                    tr[c] = dround_i64((double)(w * kv.x) * vscale);
-                    ti[c] = dround_i64((double)(w * kv.y) * vscale);
-                    tw[c] = dround_i64((double)w * wscale);
-                }
-                const int li0 = ii + PSX * nn;
-#pragma unroll
-                for (int c = 0; c < 4; c++) {
-                    if (!big[c]) continue;
-                    const int li = li0 + (c & 1) + PSX * (c >> 1);
-                    atomicAdd(&acc[li], (unsigned long long)tr[c]);
-                    atomicAdd(&acc[PPAD + li], (unsigned long long)ti[c]);
-                    atomicAdd(&acc[2 * PPAD + li], (unsigned long long)tw[c]);
-                }
-                if (any_tiny) {
-                    // weights below 2^-20: float accumulators (rare)
+                const cplx<float>* Kl = spec + ((size_t)r * g.ny + j) * g.nx + g.nxd2 + abase;
+                int abn = abase, pendn = pend;
+                double n0 = m0, n1 = m1, n2 = m2, n6 = m6, n8 = m8, nph[3] = {ph[0], ph[1], ph[2]};
+                const cplx<float>* Kn = Kl;
+                auto load_next = [&]() {
+                    if (q + 1 < n) {
+                        const int rn = s_row[q + 1];
+                        abn = s_a0[q + 1] - s_pre[q + 1];
+                        pendn = s_pre[q + 2];
+                        const RowParams& Rn = rows[rn];
+                        n0 = Rn.m[0]; n1 = Rn.m[1]; n2 = Rn.m[2]; n6 = Rn.m[6]; n8 = Rn.m[8];
+                        if (ROT) {
+                            nph[0] = Rn.ph[0];
+                            nph[1] = Rn.ph[1];
+                            nph[2] = Rn.ph[2];
+                        }
+                        Kn = spec + ((size_t)rn * g.ny + j) * g.nx + g.nxd2 + abn;
+                    }
+                };
+                load_next();
+                auto splat = [&](const int a, const cplx<float> kraw) {
+                    const double ad = (double)a;
+                    const double qx = __dmul_rn(g.fxst, ad);
+                    const double zz = g.escc ? __dmul_rn(-g.wl2, __fma_rn(qx, qx, qy2)) : 0.0;
+                    const double ox = __fma_rn(ad, m0, __fma_rn(bd, m1, __dmul_rn(zz, m2)));
+                    const double oz = __fma_rn(ad, m6, __dmul_rn(zz, m8));
+                    const double dx1 = __dadd_rn((double)g.nxd2, ox);
+                    const double dz1 = __dadd_rn((double)g.nzd2, oz);
+                    const double fnn = floor(dz1), fii = floor(dx1);
+                    if (fnn < 0.0 || fnn > (double)(g.nz - 2)) return;
+                    if (fii < 0.0 || fii > (double)(g.nx - 2)) return;
+                    const int ii = (int)fii - x0, nn = (int)fnn - z0;
+                    if (ii < -1 || ii >= PX || nn < -1 || nn >= PZ) return;
+                    const double fx = dx1 - fii, fz = dz1 - fnn;
+                    const double wxs[2] = {1.0 - fx, fx}, wzs[2] = {1.0 - fz, fz};
+                    cplx<float> kv = kraw;
+                    if (ROT) {
+                        const double phs = -(ph[0] * ox + ph[1] * bd + ph[2] * oz);
+                        double sn, c;
+                        sincos(phs, &sn, &c);
+                        const float cf = (float)c, sf = (float)sn;
+                        kv = {kraw.x * cf - kraw.y * sf, kraw.x * sf + kraw.y * cf};
+                    }
+                    long long tr[4], ti[4], tw[4];
+                    float wc[4];
+                    bool big[4];
+                    bool any_tiny = false;
 #pragma unroll
                     for (int c = 0; c < 4; c++) {
-                        const int lx = ii + (c & 1), lz = nn + (c >> 1);
-                        const float w = wc[c];
-                        if ((unsigned)lx >= (unsigned)PX || (unsigned)lz >= (unsigned)PZ || w == 0.f ||
-                            w >= S_TINY_W)
-                            continue;
-                        const int v = lx + PX * lz;
-                        atomicAdd(&tiny[v], w * kv.x);
-                        atomicAdd(&tiny[PVOX + v], w * kv.y);
-                        atomicAdd(&tiny[2 * PVOX + v], w);
+                        const int cx = c & 1, cz = c >> 1;
+                        const bool inb = (unsigned)(ii + cx) < (unsigned)PX && (unsigned)(nn + cz) < (unsigned)PZ;
+                        const float w = (float)(wxs[cx] * 1.0 * wzs[cz]);
+                        big[c] = inb && w >= S_TINY_W;
+                        any_tiny |= inb && w != 0.f && w < S_TINY_W;
+                        wc[c] = w;
+                        tr[c] = dround_i64((double)(w * kv.x) * vscale);
+                        ti[c] = dround_i64((double)(w * kv.y) * vscale);
+                        tw[c] = dround_i64((double)w * wscale);
                     }
+                    const int li0 = ii + PSX * nn;
+#pragma unroll
+                    for (int c = 0; c < 4; c++) {
+                        if (!big[c]) continue;
+                        const int li = li0 + (c & 1) + PSX * (c >> 1);
+                        atomicAdd(&acc[li], (unsigned long long)tr[c]);
+                        atomicAdd(&acc[PPAD + li], (unsigned long long)ti[c]);
+                        atomicAdd(&acc[2 * PPAD + li], (unsigned long long)tw[c]);
+                    }
+                    if (any_tiny) {
+                        // weights below 2^-20: float accumulators (rare)
+#pragma unroll
+                        for (int c = 0; c < 4; c++) {
+                                const int lx = ii + (c & 1), lz = nn + (c >> 1);
+                                const float w = wc[c];
+                                if ((unsigned)lx >= (unsigned)PX || (unsigned)lz >= (unsigned)PZ || w == 0.f ||
+                                    w >= S_TINY_W)
+                                    continue;
+                                const int v = lx + PX * lz;
+                                atomicAdd(&tiny[v], w * kv.x);
+                                atomicAdd(&tiny[PVOX + v], w * kv.y);
+                                atomicAdd(&tiny[2 * PVOX + v], w);
+                        }
+                    }
+                };
+                cplx<float> kv = Kl[p0];
+                for (int p = p0; p < p1; p++) {
+                    if (p == pend) {
+                        // enter the next listed row
+                        q++;
+                        abase = abn;
+                        pend = pendn;
+                        m0 = n0; m1 = n1; m2 = n2; m6 = n6; m8 = n8;
+                        if (ROT) {
+                            ph[0] = nph[0];
+                            ph[1] = nph[1];
+                            ph[2] = nph[2];
+                        }
+                        Kl = Kn;
+                        load_next();
+                    }
+                    const cplx<float> kraw = kv;
+                    // the next point's value: this row, or the next one
+                    if (p + 1 < p1) kv = (p + 1 < pend) ? Kl[p + 1] : Kn[p + 1];
+                    splat(abase + p, kraw);
                 }
-            };
-            // two points in flight (clamped indices keep the loads in the line)
-            cplx<float> kA = Kline[as], kB = Kline[min(as + 1, ae)];
-            for (int a = as; a <= ae; a += 2) {
-                splat(a, kA);
-                kA = Kline[min(a + 2, ae)];
-                if (a + 1 <= ae) splat(a + 1, kB);
-                kB = Kline[min(a + 3, ae)];
             }
         }
         __syncthreads();
