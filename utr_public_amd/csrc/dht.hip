@@ -60,9 +60,9 @@ __global__ void __launch_bounds__(256) k_dht_rows(const float* __restrict__ img,
                                                   cplx<float>* __restrict__ out, int ny, DhtGeom g,
                                                   const cplx<float>* __restrict__ tw) {
     constexpr int NB = lds_batch(N, sizeof(cplx<float>), LDS_DHT);
-    __shared__ cplx<float> sm[2 * NB * N];
+    __shared__ cplx<float> sm[2 * lds_len(NB * N)];
     cplx<float>* a = sm;
-    cplx<float>* b = sm + NB * N;
+    cplx<float>* b = sm + lds_len(NB * N);
     const int row = blockIdx.y;
     const int j0 = blockIdx.x * NB;
     const int nb = min(NB, ny - j0);
@@ -79,15 +79,15 @@ __global__ void __launch_bounds__(256) k_dht_rows(const float* __restrict__ img,
             const double amp = sqrt(x), phi = g.epsilon * (-0.5) * (x - 1.0);
             double s, c;
             sincos(phi, &s, &c);
-            a[t] = {(float)(amp * c - 1.0), (float)(amp * s)};
+            a[lx(t)] = {(float)(amp * c - 1.0), (float)(amp * s)};
         } else {
-            a[t] = {(float)(x - 1.0), 0.f};
+            a[lx(t)] = {(float)(x - 1.0), 0.f};
         }
     }
     __syncthreads();
     fft_lds<float, N>(a, b, nb, tw, -1, threadIdx.x, blockDim.x);
     cplx<float>* dst = out + ((size_t)row * ny + j0) * N;
-    for (int t = threadIdx.x; t < nb * N; t += blockDim.x) dst[t] = a[t];
+    for (int t = threadIdx.x; t < nb * N; t += blockDim.x) dst[t] = a[lx(t)];
 }
 
 // Forward (sign -1) or inverse (+1) FFT along y of `count` ny x nx arrays, in place.
@@ -95,21 +95,21 @@ template <int N>
 __global__ void __launch_bounds__(256) k_dht_cols(cplx<float>* __restrict__ data, int nx,
                                                   const cplx<float>* __restrict__ tw, int sign) {
     constexpr int NB = lds_batch(N, sizeof(cplx<float>), LDS_DHT);
-    __shared__ cplx<float> sm[2 * NB * N];
+    __shared__ cplx<float> sm[2 * lds_len(NB * N)];
     cplx<float>* a = sm;
-    cplx<float>* b = sm + NB * N;
+    cplx<float>* b = sm + lds_len(NB * N);
     const int x0 = blockIdx.x * NB;
     const int nb = min(NB, nx - x0);
     cplx<float>* base = data + (size_t)blockIdx.y * N * nx + x0;
     for (int t = threadIdx.x; t < nb * N; t += blockDim.x) {
         const int y = t / nb, c = t - y * nb;
-        a[c * N + y] = base[(size_t)y * nx + c];
+        a[lx(c * N + y)] = base[(size_t)y * nx + c];
     }
     __syncthreads();
     fft_lds<float, N>(a, b, nb, tw, sign, threadIdx.x, blockDim.x);
     for (int t = threadIdx.x; t < nb * N; t += blockDim.x) {
         const int y = t / nb, c = t - y * nb;
-        base[(size_t)y * nx + c] = a[c * N + y];
+        base[(size_t)y * nx + c] = a[lx(c * N + y)];
     }
 }
 
@@ -121,9 +121,9 @@ __global__ void __launch_bounds__(256) k_dht_prop_rows(const cplx<float>* __rest
                                                        const DhtRow* __restrict__ rows, int a0,
                                                        const cplx<float>* __restrict__ tw) {
     constexpr int NB = lds_batch(N, sizeof(cplx<float>), LDS_DHT);
-    __shared__ cplx<float> sm[2 * NB * N];
+    __shared__ cplx<float> sm[2 * lds_len(NB * N)];
     cplx<float>* a = sm;
-    cplx<float>* b = sm + NB * N;
+    cplx<float>* b = sm + lds_len(NB * N);
     const int ny = g.ny;
     const int p = blockIdx.y, ab = blockIdx.z;
     const int k0 = blockIdx.x * NB;
@@ -155,12 +155,12 @@ __global__ void __launch_bounds__(256) k_dht_prop_rows(const cplx<float>* __rest
             }
         }
         const cplx<float> v = src[t];
-        a[t] = {v.x * fr - v.y * fi, v.x * fi + v.y * fr};
+        a[lx(t)] = {v.x * fr - v.y * fi, v.x * fi + v.y * fr};
     }
     __syncthreads();
     fft_lds<float, N>(a, b, nb, tw, +1, threadIdx.x, blockDim.x);
     cplx<float>* dst = T + (((size_t)ab * g.nplanes + p) * ny + k0) * N;
-    for (int t = threadIdx.x; t < nb * N; t += blockDim.x) dst[t] = a[t];
+    for (int t = threadIdx.x; t < nb * N; t += blockDim.x) dst[t] = a[lx(t)];
 }
 
 // Inverse FFT along y of each plane, then 1 - |psi|^2 = -2 Re u - |u|^2
@@ -173,10 +173,10 @@ __global__ void __launch_bounds__(256) k_dht_prop_cols(const cplx<float>* __rest
                                                        const cplx<float>* __restrict__ tw,
                                                        unsigned* __restrict__ pmax) {
     constexpr int NB = lds_batch(N, sizeof(cplx<float>), LDS_DHT);
-    __shared__ cplx<float> sm[2 * NB * N];
+    __shared__ cplx<float> sm[2 * lds_len(NB * N)];
     __shared__ float s_wmax[4];
     cplx<float>* a = sm;
-    cplx<float>* b = sm + NB * N;
+    cplx<float>* b = sm + lds_len(NB * N);
     const int nx = g.nx;
     const int p = blockIdx.y, ab = blockIdx.z;
     const int x0 = blockIdx.x * NB;
@@ -185,7 +185,7 @@ __global__ void __launch_bounds__(256) k_dht_prop_cols(const cplx<float>* __rest
     const cplx<float>* base = T + plane + x0;
     for (int t = threadIdx.x; t < nb * N; t += blockDim.x) {
         const int y = t / nb, c = t - y * nb;
-        a[c * N + y] = base[(size_t)y * nx + c];
+        a[lx(c * N + y)] = base[(size_t)y * nx + c];
     }
     __syncthreads();
     fft_lds<float, N>(a, b, nb, tw, +1, threadIdx.x, blockDim.x);
@@ -194,7 +194,7 @@ __global__ void __launch_bounds__(256) k_dht_prop_cols(const cplx<float>* __rest
     float vmax = 0.f;
     for (int t = threadIdx.x; t < nb * N; t += blockDim.x) {
         const int y = t / nb, c = t - y * nb;
-        const cplx<float> z = a[c * N + y];
+        const cplx<float> z = a[lx(c * N + y)];
         const float re = z.x * inv, im = z.y * inv;
         const float v = (g.method == 3 ? -2.f * re - (re * re + im * im) : re) * w;
         P[plane + (size_t)y * nx + x0 + c] = v;

@@ -15,6 +15,7 @@
 //                    double).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstdio>
@@ -110,6 +111,7 @@ struct DevBuf {
 };
 
 constexpr int kStageRows = 32;  // rows per host->device staging batch
+constexpr int kRowsWgPerCu = 8; // resident workgroups of the persistent row FFT
 
 }  // namespace
 
@@ -119,6 +121,7 @@ struct utr_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev[8] = {};
+    int ncu = 256;  // compute units of the device (persistent grids)
     std::vector<char> added;
     int n_added = 0;
     int keep_sampling = 0;
@@ -144,6 +147,13 @@ size_t vox(const utr_ctx* c) {
     return (size_t)g.nx * g.ny * g.nz;
 }
 
+// Grid of a persistent column-type kernel: wpc resident workgroups per CU,
+// rounded down to a multiple of 8 (the XCD count) as its tile order needs.
+unsigned persistent_cap(const utr_ctx* c, int wpc) {
+    const unsigned n = (unsigned)(c->ncu * wpc) & ~7u;
+    return n ? n : 8u;
+}
+
 template <typename T>
 int upload_twiddles(utr_ctx* c, DevBuf<cplx<T>>& bx, DevBuf<cplx<T>>& by, DevBuf<cplx<T>>& bz) {
     const auto& g = c->plan.g;
@@ -167,8 +177,9 @@ int launch_2d(utr_ctx* c, const float* d_img, int row0, int count, cplx<T>* d_sp
 #define UTR_ROWS(N)                                                                             \
     {                                                                                           \
         constexpr int NB = utr::lds_batch(N, sizeof(cplx<T>), utr::LDS_ROWS);                      \
-        dim3 grid((g.ny + NB - 1) / NB, count);                                                 \
-        utr::k_rows<T, N><<<grid, 256, 0, c->stream>>>(d_img, d_spec, g.ny, g, twx);            \
+        const int nbr = (g.ny + NB - 1) / NB, nbt = nbr * count;                                \
+        const int grid = std::min(nbt, c->ncu * kRowsWgPerCu);                                  \
+        utr::k_rows<T, N><<<grid, 256, 0, c->stream>>>(d_img, d_spec, g.ny, g, twx, nbr, nbt);  \
         ok = true;                                                                              \
     }
     UTR_DISPATCH_N(g.nx, UTR_ROWS)
@@ -179,8 +190,9 @@ int launch_2d(utr_ctx* c, const float* d_img, int row0, int count, cplx<T>* d_sp
     {                                                                                           \
         constexpr int NB = utr::lds_batch(N, sizeof(cplx<T>), utr::LDS_COLS);                      \
         const int gx = (g.nx + NB - 1) / NB;                                                    \
-        utr::k_cols<T, N><<<utr::xcd_grid(gx * count), 256, 0, c->stream>>>(                   \
-            d_spec, g.nx, g, c->rows.p, row0, twy, c->rowmax.p, gx, count);                     \
+        const unsigned grid = std::min(utr::xcd_grid(gx * count), persistent_cap(c, 8));        \
+        utr::k_cols<T, N><<<grid, 256, 0, c->stream>>>(d_spec, g.nx, g, c->rows.p, row0, twy,  \
+                                                       c->rowmax.p, gx, count);                 \
         ok = true;                                                                              \
     }
     UTR_DISPATCH_N(g.ny, UTR_COLS)
@@ -404,6 +416,9 @@ int utr_create(const utr_params* params, int nrows, const double* angZ_deg, cons
         return rc;
     };
     if (hipGetDevice(&c->device) != hipSuccess) return fail(set_err(UTR_E_DEVICE, "no HIP device"));
+    if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess ||
+        c->ncu <= 0)
+        return fail(set_err(UTR_E_DEVICE, "hipDeviceGetAttribute failed"));
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return fail(set_err(UTR_E_DEVICE, "hipStreamCreate failed"));
     for (auto& e : c->ev)

@@ -36,6 +36,14 @@ __device__ __forceinline__ cplx<T> mul_si(cplx<T> a, int sign) {
     return sign > 0 ? cplx<T>{-a.y, a.x} : cplx<T>{a.y, -a.x};
 }
 
+// ------------------------------------------------------------- LDS layout
+// One pad entry after every 8: the strided writes of the first Stockham
+// passes (stride R, then R*p for p < 64) and the transposed column loads
+// then fall on distinct LDS banks instead of 4 to 16 lanes sharing one.
+// A buffer of n logical entries holds lds_len(n); entry i sits at lx(i).
+__host__ __device__ constexpr int lds_len(int n) { return n + (n >> 3); }
+__device__ __forceinline__ int lx(int i) { return i + (i >> 3); }
+
 // ---------------------------------------------------------------- factors
 struct FacList {
     int nf;
@@ -155,10 +163,10 @@ __device__ __forceinline__ void fft_pass(const cplx<T>* __restrict__ src, cplx<T
         const int seq = t / M;
         const int i = t - seq * M;
         const int k = i % p;
-        const cplx<T>* s = src + seq * N;
+        const int s0 = seq * N + i;
         cplx<T> u[R];
 #pragma unroll
-        for (int r = 0; r < R; r++) u[r] = s[i + r * M];
+        for (int r = 0; r < R; r++) u[r] = src[lx(s0 + r * M)];
         if (p > 1) {
             const cplx<T>* w = tw + k * (R - 1);
 #pragma unroll
@@ -169,9 +177,9 @@ __device__ __forceinline__ void fft_pass(const cplx<T>* __restrict__ src, cplx<T
             }
         }
         dft_small<T, R>(u, sign);
-        cplx<T>* d = dst + seq * N + (i / p) * p * R + k;
+        const int d0 = seq * N + (i / p) * p * R + k;
 #pragma unroll
-        for (int q = 0; q < R; q++) d[q * p] = u[q];
+        for (int q = 0; q < R; q++) dst[lx(d0 + q * p)] = u[q];
     }
 }
 
@@ -191,8 +199,8 @@ __device__ __forceinline__ void fft_passes(cplx<T>*& a, cplx<T>*& b, int nb,
     }
 }
 
-// Transforms nb sequences of length N held in `a` (LDS).  `b` is scratch of
-// the same size.  On return `a` points at the buffer holding the result.
+// Transforms nb sequences of length N held in `a` (LDS, entry i of the
+// concatenated sequences at lx(i)).  `b` is scratch of the same size.  On return `a` points at the buffer holding the result.
 // The caller must __syncthreads() before the call (data loaded) and reads
 // the result after it (this function ends with a barrier).
 // tw: forward-sign twiddle table for length N (FacList layout).

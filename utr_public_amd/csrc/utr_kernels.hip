@@ -69,32 +69,54 @@ __device__ __forceinline__ bool xcd_tile(int gx, int gy, int& tx, int& ty) {
 
 // Rows: renormalise, clamp, Log0 (UTR.cpp:951-958, 991), checkerboard
 // (-1)^(i+j) (the input Shuffle of UTR.cpp:993) and forward FFT along x.
+// Persistent: each workgroup walks batches of NB rows (nbt batches in all,
+// nbr per image) and loads the next batch's pixels into registers while it
+// transforms the current one.
 template <typename T, int N>
 __global__ void __launch_bounds__(256) k_rows(const float* __restrict__ img,
                                               cplx<T>* __restrict__ out, int ny, GeomDev g,
-                                              const cplx<T>* __restrict__ tw) {
+                                              const cplx<T>* __restrict__ tw, int nbr, int nbt) {
     constexpr int NB = lds_batch(N, sizeof(cplx<T>), LDS_ROWS);
-    __shared__ cplx<T> sm[2 * NB * N];
-    cplx<T>* a = sm;
-    cplx<T>* b = sm + NB * N;
-    const int row = blockIdx.y;
-    const int j0 = blockIdx.x * NB;
-    const int nb = min(NB, ny - j0);
-    const float* src = img + ((size_t)row * ny + j0) * N;
-    for (int t = threadIdx.x; t < nb * N; t += blockDim.x) {
-        const int jj = t / N, i = t - jj * N;
-        float x = src[t];
-        if (g.use_f1) x = x / g.f1;
-        if (g.use_f2) x = x + g.f2;
-        if (x < 0.f) x = 0.f;
-        T f = (x > 0.f) ? dev_log<T>(x) : T(0);
-        if ((i + j0 + jj) & 1) f = -f;
-        a[t] = {f, T(0)};
+    constexpr int PER = (NB * N + 255) / 256;  // pixels per thread and batch
+    __shared__ cplx<T> sm[2 * lds_len(NB * N)];
+    float xn[PER];
+    auto fetch = [&](int bt) {
+        const int row = bt / nbr, j0 = (bt - row * nbr) * NB, nb = min(NB, ny - j0);
+        const float* src = img + ((size_t)row * ny + j0) * N;
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int t = threadIdx.x + 256 * k;
+            xn[k] = t < nb * N ? src[t] : 0.f;
+        }
+    };
+    if ((int)blockIdx.x < nbt) fetch(blockIdx.x);
+    for (int bt = blockIdx.x; bt < nbt; bt += gridDim.x) {
+        float xc[PER];
+#pragma unroll
+        for (int k = 0; k < PER; k++) xc[k] = xn[k];
+        if (bt + (int)gridDim.x < nbt) fetch(bt + gridDim.x);
+        const int row = bt / nbr, j0 = (bt - row * nbr) * NB, nb = min(NB, ny - j0);
+        cplx<T>* a = sm;
+        cplx<T>* b = sm + lds_len(NB * N);
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            const int t = threadIdx.x + 256 * k;
+            if (t >= nb * N) break;
+            const int jj = t / N, i = t - jj * N;
+            float x = xc[k];
+            if (g.use_f1) x = x / g.f1;
+            if (g.use_f2) x = x + g.f2;
+            if (x < 0.f) x = 0.f;
+            T f = (x > 0.f) ? dev_log<T>(x) : T(0);
+            if ((i + j0 + jj) & 1) f = -f;
+            a[lx(t)] = {f, T(0)};
+        }
+        __syncthreads();
+        fft_lds<T, N>(a, b, nb, tw, -1, threadIdx.x, blockDim.x);
+        cplx<T>* dst = out + ((size_t)row * ny + j0) * N;
+        for (int t = threadIdx.x; t < nb * N; t += blockDim.x) dst[t] = a[lx(t)];
+        __syncthreads();  // the next batch overwrites both buffers
     }
-    __syncthreads();
-    fft_lds<T, N>(a, b, nb, tw, -1, threadIdx.x, blockDim.x);
-    cplx<T>* dst = out + ((size_t)row * ny + j0) * N;
-    for (int t = threadIdx.x; t < nb * N; t += blockDim.x) dst[t] = a[t];
 }
 
 // Columns: forward FFT along y, output Shuffle as (-1)^(kx+ky+nx/2+ny/2),
@@ -108,82 +130,116 @@ __global__ void __launch_bounds__(256) k_cols(cplx<T>* __restrict__ spec, int nx
                                               const cplx<T>* __restrict__ tw,
                                               unsigned* __restrict__ rowmax, int gx, int gy) {
     constexpr int NB = lds_batch(N, sizeof(cplx<T>), LDS_COLS);
-    __shared__ cplx<T> sm[2 * NB * N];
-    cplx<T>* a = sm;
-    cplx<T>* b = sm + NB * N;
+    constexpr int PER = (NB * N + 255) / 256;  // elements per thread and tile
+    __shared__ cplx<T> sm[2 * lds_len(NB * N)];
     const int ny = N;
-    int tx, ty;
-    if (!xcd_tile(gx, gy, tx, ty)) return;
-    const int row = ty;
-    const int x0 = tx * NB;
-    const int nb = min(NB, nx - x0);
-    cplx<T>* base = spec + (size_t)row * ny * nx + x0;
-    for (int t = threadIdx.x; t < nb * N; t += blockDim.x) {
-        const int y = t / nb, c = t - y * nb;
-        a[c * N + y] = base[(size_t)y * nx + c];
-    }
-    __syncthreads();
-    fft_lds<T, N>(a, b, nb, tw, -1, threadIdx.x, blockDim.x);
-    const RowParams& R = rows[row0 + row];
-    const int gsgn = par_sign(g.nxd2 + g.nyd2);
-    const double xap = g.xhi - g.xlo, yap = g.yhi - g.ylo;
-    const double pi = 3.14159265358979323846;
-    float vmax = 0.f;
-    for (int t = threadIdx.x; t < nb * N; t += blockDim.x) {
-        const int ky = t / nb, c = t - ky * nb;
-        const int kx = x0 + c;
-        cplx<T> v = a[c * N + ky];
-        const T s = T(par_sign(kx + ky) * gsgn);
-        v.x *= s;
-        v.y *= s;
-        if (g.ctf_mode == 1 && R.zout != 0) {
-            // 2D TIE-Hom restated from UTR.h:159-181 in 2D (unpinned)
-            const double i1 = kx - g.nxd2, j1 = ky - g.nyd2;
-            const double q2 = 4.0 * pi * pi * (i1 * i1 / (xap * xap) + j1 * j1 / (yap * yap));
-            const T f = T(g.two_d_norm / (g.two_d_alpha + q2));
-            v.x *= f;
-            v.y *= f;
-        } else if (g.ctf_mode == 3) {
-            // 2D CTF inversion restated from UTR.h:70-128 in 2D (unpinned)
-            const double dk2 = 1.0 / (xap * xap), de2 = 1.0 / (yap * yap);
-            const double i1 = kx - g.nxd2, j1 = ky - g.nyd2;
-            const double q2 = dk2 * i1 * i1 + de2 * j1 * j1;
-            if (!(g.k2maxo > 0) || q2 < g.k2maxo) {
-                const double fac2 = pi * g.wl * R.zout;
-                double eps = g.epsilon;
-                if (eps <= 0) {
-                    const double mn = dk2 < de2 ? dk2 : de2;
-                    eps = 0.1 * (fabs(fac2) * mn) * (fabs(fac2) * mn);
-                }
-                double chi = fac2 * q2 - atan(g.asigma);
-                if (g.cs3 != 0 || g.cs5 != 0)
-                    chi += pi * g.wl * g.wl * g.wl / 2.0 * g.cs3 * q2 * q2 +
-                           pi * g.wl * g.wl * g.wl * g.wl * g.wl / 3.0 * g.cs5 * q2 * q2 * q2;
-                double sn, cs;
-                sincos(chi, &sn, &cs);
-                const T sr = T(sn / (sn * sn + eps));
-                const T cr = g.escc ? T(cs / (cs * cs + eps)) : T(0);
-                const cplx<T> w = {v.x * sr + v.y * cr, v.y * sr - v.x * cr};
-                v = w;
+    const int total = gx * gy;
+    // Persistent: in round k this workgroup takes tile tile_of(k); within a
+    // round the tiles of one XCD are contiguous, so neighbouring column groups
+    // share that XCD's L2.  The next tile's columns are loaded into registers
+    // while the current one is transformed.  gridDim.x is a multiple of 8.
+    const unsigned L = blockIdx.x, G = gridDim.x;
+    auto tile_of = [&](int k) { return (int)(k * G + (L & 7u) * (G >> 3) + (L >> 3)); };
+    cplx<T> vn[PER];
+    auto fetch = [&](int w) {
+        const int row = w / gx, x0 = (w - row * gx) * NB, nb = min(NB, nx - x0);
+        const cplx<T>* src = spec + (size_t)row * ny * nx + x0;
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int t = threadIdx.x + 256 * q;
+            if (t < nb * N) {
+                const int y = t / nb, c = t - y * nb;
+                vn[q] = src[(size_t)y * nx + c];
             }
         }
-        if (R.weight != 1.0f) {
-            v.x *= T(R.weight);
-            v.y *= T(R.weight);
+    };
+    const int rounds = (total + (int)G - 1) / (int)G;
+    if (tile_of(0) < total) fetch(tile_of(0));
+    for (int rk = 0; rk < rounds; rk++) {
+        const int wt = tile_of(rk);
+        cplx<T> vc[PER];
+#pragma unroll
+        for (int q = 0; q < PER; q++) vc[q] = vn[q];
+        if (rk + 1 < rounds && tile_of(rk + 1) < total) fetch(tile_of(rk + 1));
+        if (wt >= total) continue;  // uniform over the workgroup
+        cplx<T>* a = sm;
+        cplx<T>* b = sm + lds_len(NB * N);
+        const int row = wt / gx;
+        const int x0 = (wt - row * gx) * NB;
+        const int nb = min(NB, nx - x0);
+        cplx<T>* base = spec + (size_t)row * ny * nx + x0;
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int t = threadIdx.x + 256 * q;
+            if (t < nb * N) {
+                const int y = t / nb, c = t - y * nb;
+                a[lx(c * N + y)] = vc[q];
+            }
         }
-        base[(size_t)ky * nx + c] = v;
-        vmax = fmaxf(vmax, fmaxf(fabsf((float)v.x), fabsf((float)v.y)));
-    }
-    if (rowmax) {
-        // one atomic per workgroup (rowmax is a kernel argument: uniform branch)
-        __shared__ float s_wmax[4];
-        for (int off = 32; off > 0; off >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, off));
-        if ((threadIdx.x & 63) == 0) s_wmax[threadIdx.x >> 6] = vmax;
         __syncthreads();
-        if (threadIdx.x == 0) {
-            const float m = fmaxf(fmaxf(s_wmax[0], s_wmax[1]), fmaxf(s_wmax[2], s_wmax[3]));
-            atomicMax(&rowmax[row0 + row], __float_as_uint(m));
+        fft_lds<T, N>(a, b, nb, tw, -1, threadIdx.x, blockDim.x);
+        const RowParams& R = rows[row0 + row];
+        const int gsgn = par_sign(g.nxd2 + g.nyd2);
+        const double xap = g.xhi - g.xlo, yap = g.yhi - g.ylo;
+        const double pi = 3.14159265358979323846;
+        float vmax = 0.f;
+        for (int t = threadIdx.x; t < nb * N; t += blockDim.x) {
+            const int ky = t / nb, c = t - ky * nb;
+            const int kx = x0 + c;
+            cplx<T> v = a[lx(c * N + ky)];
+            const T s = T(par_sign(kx + ky) * gsgn);
+            v.x *= s;
+            v.y *= s;
+            if (g.ctf_mode == 1 && R.zout != 0) {
+                // 2D TIE-Hom restated from UTR.h:159-181 in 2D (unpinned)
+                const double i1 = kx - g.nxd2, j1 = ky - g.nyd2;
+                const double q2 = 4.0 * pi * pi * (i1 * i1 / (xap * xap) + j1 * j1 / (yap * yap));
+                const T f = T(g.two_d_norm / (g.two_d_alpha + q2));
+                v.x *= f;
+                v.y *= f;
+            } else if (g.ctf_mode == 3) {
+                // 2D CTF inversion restated from UTR.h:70-128 in 2D (unpinned)
+                const double dk2 = 1.0 / (xap * xap), de2 = 1.0 / (yap * yap);
+                const double i1 = kx - g.nxd2, j1 = ky - g.nyd2;
+                const double q2 = dk2 * i1 * i1 + de2 * j1 * j1;
+                if (!(g.k2maxo > 0) || q2 < g.k2maxo) {
+                    const double fac2 = pi * g.wl * R.zout;
+                    double eps = g.epsilon;
+                    if (eps <= 0) {
+                        const double mn = dk2 < de2 ? dk2 : de2;
+                        eps = 0.1 * (fabs(fac2) * mn) * (fabs(fac2) * mn);
+                    }
+                    double chi = fac2 * q2 - atan(g.asigma);
+                    if (g.cs3 != 0 || g.cs5 != 0)
+                        chi += pi * g.wl * g.wl * g.wl / 2.0 * g.cs3 * q2 * q2 +
+                               pi * g.wl * g.wl * g.wl * g.wl * g.wl / 3.0 * g.cs5 * q2 * q2 * q2;
+                    double sn, cs;
+                    sincos(chi, &sn, &cs);
+                    const T sr = T(sn / (sn * sn + eps));
+                    const T cr = g.escc ? T(cs / (cs * cs + eps)) : T(0);
+                    const cplx<T> w = {v.x * sr + v.y * cr, v.y * sr - v.x * cr};
+                    v = w;
+                }
+            }
+            if (R.weight != 1.0f) {
+                v.x *= T(R.weight);
+                v.y *= T(R.weight);
+            }
+            base[(size_t)ky * nx + c] = v;
+            vmax = fmaxf(vmax, fmaxf(fabsf((float)v.x), fabsf((float)v.y)));
         }
+        if (rowmax) {
+            // one atomic per workgroup (rowmax is a kernel argument: uniform branch)
+            __shared__ float s_wmax[4];
+            for (int off = 32; off > 0; off >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, off));
+            if ((threadIdx.x & 63) == 0) s_wmax[threadIdx.x >> 6] = vmax;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                const float m = fmaxf(fmaxf(s_wmax[0], s_wmax[1]), fmaxf(s_wmax[2], s_wmax[3]));
+                atomicMax(&rowmax[row0 + row], __float_as_uint(m));
+            }
+        }
+        __syncthreads();  // the next tile overwrites both buffers
     }
 }
 
@@ -1230,9 +1286,9 @@ __global__ void __launch_bounds__(256) k_pack_z(const cplx<T>* __restrict__ V,
                                                 cplx<T>* __restrict__ H, GeomDev g,
                                                 const cplx<T>* __restrict__ tw, int gx, int gy) {
     constexpr int CW = lds_batch(NZ, sizeof(cplx<T>), LDS_PACKZ);
-    __shared__ cplx<T> sm[2 * CW * NZ];
+    __shared__ cplx<T> sm[2 * lds_len(CW * NZ)];
     cplx<T>* a = sm;
-    cplx<T>* b = sm + CW * NZ;
+    cplx<T>* b = sm + lds_len(CW * NZ);
     const int nx = g.nx, ny = g.ny, W = nx / 2 + 1;
     int tx, ty;
     if (!xcd_tile(gx, gy, tx, ty)) return;
@@ -1262,13 +1318,13 @@ __global__ void __launch_bounds__(256) k_pack_z(const cplx<T>* __restrict__ V,
             h.x = -h.x;
             h.y = -h.y;
         }
-        a[c * NZ + k] = h;
+        a[lx(c * NZ + k)] = h;
     }
     __syncthreads();
     fft_lds<T, NZ>(a, b, nb, tw, +1, threadIdx.x, blockDim.x);
     for (int t = threadIdx.x; t < nb * NZ; t += blockDim.x) {
         const int k = t / nb, c = t - k * nb;
-        H[((size_t)k * ny + j) * W + w0 + c] = a[c * NZ + k];
+        H[((size_t)k * ny + j) * W + w0 + c] = a[lx(c * NZ + k)];
     }
 }
 
@@ -1277,9 +1333,9 @@ template <typename T, int NY>
 __global__ void __launch_bounds__(256) k_fft_y(cplx<T>* __restrict__ H, GeomDev g,
                                                const cplx<T>* __restrict__ tw, int gx, int gy) {
     constexpr int CW = lds_batch(NY, sizeof(cplx<T>), LDS_FFTY);
-    __shared__ cplx<T> sm[2 * CW * NY];
+    __shared__ cplx<T> sm[2 * lds_len(CW * NY)];
     cplx<T>* a = sm;
-    cplx<T>* b = sm + CW * NY;
+    cplx<T>* b = sm + lds_len(CW * NY);
     const int W = g.nx / 2 + 1;
     int tx, ty;
     if (!xcd_tile(gx, gy, tx, ty)) return;
@@ -1289,13 +1345,13 @@ __global__ void __launch_bounds__(256) k_fft_y(cplx<T>* __restrict__ H, GeomDev 
     cplx<T>* base = H + (size_t)k * NY * W + w0;
     for (int t = threadIdx.x; t < nb * NY; t += blockDim.x) {
         const int y = t / nb, c = t - y * nb;
-        a[c * NY + y] = base[(size_t)y * W + c];
+        a[lx(c * NY + y)] = base[(size_t)y * W + c];
     }
     __syncthreads();
     fft_lds<T, NY>(a, b, nb, tw, +1, threadIdx.x, blockDim.x);
     for (int t = threadIdx.x; t < nb * NY; t += blockDim.x) {
         const int y = t / nb, c = t - y * nb;
-        base[(size_t)y * W + c] = a[c * NY + y];
+        base[(size_t)y * W + c] = a[lx(c * NY + y)];
     }
 }
 
@@ -1307,9 +1363,9 @@ __global__ void __launch_bounds__(256) k_c2r_x(const cplx<T>* __restrict__ H,
                                                TO* __restrict__ vol, GeomDev g,
                                                const cplx<T>* __restrict__ tw, int gx, int gy) {
     constexpr int NB = lds_batch(NX, sizeof(cplx<T>), LDS_C2R);
-    __shared__ cplx<T> sm[2 * NB * NX];
+    __shared__ cplx<T> sm[2 * lds_len(NB * NX)];
     cplx<T>* a = sm;
-    cplx<T>* b = sm + NB * NX;
+    cplx<T>* b = sm + lds_len(NB * NX);
     const int ny = g.ny, W = NX / 2 + 1;
     int tx, ty;
     if (!xcd_tile(gx, gy, tx, ty)) return;
@@ -1331,7 +1387,7 @@ __global__ void __launch_bounds__(256) k_c2r_x(const cplx<T>* __restrict__ H,
             x1.y = -x1.y;
             x2.y = -x2.y;
         }
-        a[t] = {x1.x - x2.y, x1.y + x2.x};
+        a[lx(t)] = {x1.x - x2.y, x1.y + x2.x};
     }
     __syncthreads();
     fft_lds<T, NX>(a, b, np, tw, +1, threadIdx.x, blockDim.x);
@@ -1357,7 +1413,7 @@ __global__ void __launch_bounds__(256) k_c2r_x(const cplx<T>* __restrict__ H,
     for (int t = threadIdx.x; t < np * NX; t += blockDim.x) {
         const int pp = t / NX, m = t - pp * NX;
         const int j = 2 * (p0 + pp);
-        const cplx<T> z = a[t];
+        const cplx<T> z = a[lx(t)];
         for (int h = 0; h < 2; h++) {
             const int jj = j + h;
             const T val = h ? z.y : z.x;
