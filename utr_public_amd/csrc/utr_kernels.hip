@@ -547,7 +547,7 @@ constexpr int SPY = SBX + 1;           // LDS stride along y
 constexpr int SPZ = SPY * (SBY + 1);   // LDS stride along z
 constexpr int SPAD = SPZ * SBZ;        // LDS entries per component
 constexpr int S_LIST = 256;            // rows tested per chunk
-constexpr float S_TINY_W = 9.5367431640625e-07f;  // 2^-20
+constexpr double S_TINY_D = 9.5367431640625e-07;  // 2^-20
 
 __device__ __forceinline__ int lds_idx(int lx, int ly, int lz) { return lx + SPY * ly + SPZ * lz; }
 
@@ -555,6 +555,13 @@ __device__ __forceinline__ int lds_idx(int lx, int ly, int lz) { return lx + SPY
 // digit with the last mantissa bit; round to nearest even)
 __device__ __forceinline__ long long dround_i64(double d) {
     const double y = d + 6755399441055744.0;
+    return (long long)(__double_as_longlong(y) - 0x4338000000000000LL);
+}
+
+// round(w * k) as a 64-bit integer for |w * k| < 2^51: one fma rounds the
+// exact product once, as dround_i64 of the exact product would
+__device__ __forceinline__ long long fx_term(double w, double k) {
+    const double y = __fma_rn(w, k, 6755399441055744.0);
     return (long long)(__double_as_longlong(y) - 0x4338000000000000LL);
 }
 
@@ -581,7 +588,7 @@ __global__ void __launch_bounds__(256, 3) k_grid_fx(const cplx<float>* __restric
     for (int t = tid; t < 3 * SVOX; t += 256) tiny[t] = 0.f;
     // fixed-point scales: 2^sv for V3, 2^sw for Samp3.  Per row a voxel
     // receives a total weight below 8, so |sum| < 8 nrows max|K| 2^sv < 2^62;
-    // each term stays below 2^51 for the rounding above.
+    // each term stays below 2^51 for fx_term.
     float mk = 0.f;
     for (int r = tid; r < nrows; r += 256) mk = fmaxf(mk, __uint_as_float(rowmax[r]));
     for (int off = 32; off > 0; off >>= 1) mk = fmaxf(mk, __shfl_xor(mk, off));
@@ -766,22 +773,25 @@ __global__ void __launch_bounds__(256, 3) k_grid_fx(const cplx<float>* __restric
                                 const float cf = (float)c, sf = (float)sn;
                                 kv = {kraw.x * cf - kraw.y * sf, kraw.x * sf + kraw.y * cf};
                             }
+                            // the spectrum value on the V3 fixed-point grid (exact: a power of two)
+                            const double kxs = (double)kv.x * vscale, kys = (double)kv.y * vscale;
                             long long tr[4], ti[4], tw[4];
                             float wc[4];
-                            bool big[4];
+                            bool big[4], tny[4];
                             bool any_tiny = false;
 #pragma unroll
                             for (int c = 0; c < 4; c++) {
                                 const int cx = c & 1, cz = c >> 1;
                                 const bool inb = (unsigned)(ii + cx) < (unsigned)SBX &&
                                                  (unsigned)(nn + cz) < (unsigned)SBZ;
-                                const float w = (float)(wxs[cx] * 1.0 * wzs[cz]);
-                                big[c] = inb && w >= S_TINY_W;
-                                any_tiny |= inb && w != 0.f && w < S_TINY_W;
-                                wc[c] = w;
-                                tr[c] = dround_i64((double)(w * kv.x) * vscale);
-                                ti[c] = dround_i64((double)(w * kv.y) * vscale);
-                                tw[c] = dround_i64((double)w * wscale);
+                                const double wd = wxs[cx] * 1.0 * wzs[cz];
+                                big[c] = inb && wd >= S_TINY_D;
+                                tny[c] = inb && wd != 0.0 && wd < S_TINY_D;
+                                any_tiny |= tny[c];
+                                wc[c] = (float)wd;
+                                tr[c] = fx_term(wd, kxs);
+                                ti[c] = fx_term(wd, kys);
+                                tw[c] = fx_term(wd, wscale);
                             }
                             const int li0 = lds_idx(ii, ly, nn);
 #pragma unroll
@@ -797,9 +807,7 @@ __global__ void __launch_bounds__(256, 3) k_grid_fx(const cplx<float>* __restric
                                 for (int c = 0; c < 4; c++) {
                                     const int lx = ii + (c & 1), lz = nn + (c >> 1);
                                     const float w = wc[c];
-                                    if ((unsigned)lx >= (unsigned)SBX || (unsigned)lz >= (unsigned)SBZ ||
-                                        w == 0.f || w >= S_TINY_W)
-                                        continue;
+                                    if (!tny[c]) continue;
                                     const int v = lx + SBX * (ly + SBY * lz);
                                     atomicAdd(&tiny[v], w * kv.x);
                                     atomicAdd(&tiny[SVOX + v], w * kv.y);
@@ -848,9 +856,11 @@ __global__ void __launch_bounds__(256, 3) k_grid_fx(const cplx<float>* __restric
                     // the eight corners' terms are formed first, without branches, so
                     // that their independent chains overlap; then each corner inside
                     // the brick adds its terms
+                    // the spectrum value on the V3 fixed-point grid (exact: a power of two)
+                    const double kxs = (double)kv.x * vscale, kys = (double)kv.y * vscale;
                     long long tr[8], ti[8], tw[8];
                     float wc[8];
-                    bool big[8];
+                    bool big[8], tny[8];
                     bool any_tiny = false;
 #pragma unroll
                     for (int c = 0; c < 8; c++) {
@@ -858,13 +868,14 @@ __global__ void __launch_bounds__(256, 3) k_grid_fx(const cplx<float>* __restric
                         const bool inb = (unsigned)(ii + cx) < (unsigned)SBX &&
                                          (unsigned)(jj + cy) < (unsigned)SBY &&
                                          (unsigned)(nn + cz) < (unsigned)SBZ;
-                        const float w = (float)(wxs[cx] * wys[cy] * wzs[cz]);
-                        big[c] = inb && w >= S_TINY_W;
-                        any_tiny |= inb && w != 0.f && w < S_TINY_W;
-                        wc[c] = w;
-                        tr[c] = dround_i64((double)(w * kv.x) * vscale);
-                        ti[c] = dround_i64((double)(w * kv.y) * vscale);
-                        tw[c] = dround_i64((double)w * wscale);
+                        const double wd = wxs[cx] * wys[cy] * wzs[cz];
+                        big[c] = inb && wd >= S_TINY_D;
+                        tny[c] = inb && wd != 0.0 && wd < S_TINY_D;
+                        any_tiny |= tny[c];
+                        wc[c] = (float)wd;
+                        tr[c] = fx_term(wd, kxs);
+                        ti[c] = fx_term(wd, kys);
+                        tw[c] = fx_term(wd, wscale);
                     }
                     const int li0 = lds_idx(ii, jj, nn);
 #pragma unroll
@@ -881,9 +892,7 @@ __global__ void __launch_bounds__(256, 3) k_grid_fx(const cplx<float>* __restric
                         for (int c = 0; c < 8; c++) {
                             const int lx = ii + (c & 1), ly = jj + ((c >> 1) & 1), lz = nn + (c >> 2);
                             const float w = wc[c];
-                            if ((unsigned)lx >= (unsigned)SBX || (unsigned)ly >= (unsigned)SBY ||
-                                (unsigned)lz >= (unsigned)SBZ || w == 0.f || w >= S_TINY_W)
-                                continue;
+                            if (!tny[c]) continue;
                             const int v = lx + SBX * (ly + SBY * lz);
                             atomicAdd(&tiny[v], w * kv.x);
                             atomicAdd(&tiny[SVOX + v], w * kv.y);
@@ -939,7 +948,7 @@ constexpr int P_LIST = 256;          // rows tested per chunk
 // Fixed-point scales 2^sv (V3) and 2^sw (Samp3) from the largest spectrum
 // component and the row count.  Per row a voxel receives a total weight
 // below 8, so |sum| < 8 nrows max|K| 2^sv < 2^62, and each term stays below
-// 2^51 for dround_i64.  Ends with a barrier.
+// 2^51 for fx_term.  Ends with a barrier.
 __device__ __forceinline__ void fx_scales(const unsigned* __restrict__ rowmax, int nrows,
                                           float* s_red, int& sv, int& sw) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -1132,21 +1141,24 @@ __global__ void __launch_bounds__(256, ROT ? 1 : 4) k_grid_yp(const cplx<float>*
                         const float cf = (float)c, sf = (float)sn;
                         kv = {kraw.x * cf - kraw.y * sf, kraw.x * sf + kraw.y * cf};
                     }
+                    // the spectrum value on the V3 fixed-point grid (exact: a power of two)
+                    const double kxs = (double)kv.x * vscale, kys = (double)kv.y * vscale;
                     long long tr[4], ti[4], tw[4];
                     float wc[4];
-                    bool big[4];
+                    bool big[4], tny[4];
                     bool any_tiny = false;
 #pragma unroll
                     for (int c = 0; c < 4; c++) {
                         const int cx = c & 1, cz = c >> 1;
                         const bool inb = (unsigned)(ii + cx) < (unsigned)PX && (unsigned)(nn + cz) < (unsigned)PZ;
-                        const float w = (float)(wxs[cx] * 1.0 * wzs[cz]);
-                        big[c] = inb && w >= S_TINY_W;
-                        any_tiny |= inb && w != 0.f && w < S_TINY_W;
-                        wc[c] = w;
-                        tr[c] = dround_i64((double)(w * kv.x) * vscale);
-                        ti[c] = dround_i64((double)(w * kv.y) * vscale);
-                        tw[c] = dround_i64((double)w * wscale);
+                        const double wd = wxs[cx] * 1.0 * wzs[cz];
+                        big[c] = inb && wd >= S_TINY_D;
+                        tny[c] = inb && wd != 0.0 && wd < S_TINY_D;
+                        any_tiny |= tny[c];
+                        wc[c] = (float)wd;
+                        tr[c] = fx_term(wd, kxs);
+                        ti[c] = fx_term(wd, kys);
+                        tw[c] = fx_term(wd, wscale);
                     }
                     const int li0 = ii + PSX * nn;
 #pragma unroll
@@ -1161,15 +1173,12 @@ __global__ void __launch_bounds__(256, ROT ? 1 : 4) k_grid_yp(const cplx<float>*
                         // weights below 2^-20: float accumulators (rare)
 #pragma unroll
                         for (int c = 0; c < 4; c++) {
-                                const int lx = ii + (c & 1), lz = nn + (c >> 1);
-                                const float w = wc[c];
-                                if ((unsigned)lx >= (unsigned)PX || (unsigned)lz >= (unsigned)PZ || w == 0.f ||
-                                    w >= S_TINY_W)
-                                    continue;
-                                const int v = lx + PX * lz;
-                                atomicAdd(&tiny[v], w * kv.x);
-                                atomicAdd(&tiny[PVOX + v], w * kv.y);
-                                atomicAdd(&tiny[2 * PVOX + v], w);
+                            if (!tny[c]) continue;
+                            const int v = ii + (c & 1) + PX * (nn + (c >> 1));
+                            const float w = wc[c];
+                            atomicAdd(&tiny[v], w * kv.x);
+                            atomicAdd(&tiny[PVOX + v], w * kv.y);
+                            atomicAdd(&tiny[2 * PVOX + v], w);
                         }
                     }
                 };
