@@ -129,6 +129,30 @@ def main():
     extra = {"phase_ms": {"fft2d_ctf": round(float(np.mean(fft2_ms)), 4),
                           "gridding_filters": round(g_ms, 4),
                           "fft3d_mask_scale": round(float(np.mean(fft3_ms)), 4)}}
+    # FFT rates as conventionally counted (5 N log2 N per complex transform,
+    # half that for the real-output 3D inverse), over each phase's time
+    n2 = float(nx * ny)
+    nvox = float(nx * ny * nz)
+    extra["fft_gflops"] = {
+        "fft2d": round(5.0 * n2 * np.log2(n2) * A / (np.mean(fft2_ms) * 1e-3) / 1e9, 1),
+        "fft3d": round(2.5 * nvox * np.log2(nvox) / (np.mean(fft3_ms) * 1e-3) / 1e9, 1)}
+    # this box's HBM stream rate: a 1 GiB device-to-device copy (read + write)
+    src = torch.empty(256 << 20, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    dst.copy_(src)
+    e0.record()
+    for _ in range(5):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    extra["hbm_stream_GBs"] = round(2 * 5 * src.numel() * 4 / (e0.elapsed_time(e1) * 1e-3) / 1e9, 1)
+    del src, dst
+    # whole-step algorithmic bytes: images read once, the volume written once,
+    # and the 3D c2r's one round trip of the half spectrum (4 A n^2 + 12 N^3)
+    b_alg = 4.0 * A * nx * ny + 12.0 * nx * ny * nz
+    extra["step_algorithmic"] = {"bytes": b_alg,
+                                 "GBs": round(b_alg / (ms_step * 1e-3) / 1e9, 1)}
 
     # host-memory rate: images from pageable host memory in, volume back to
     # host memory (the PCIe-inclusive path of utr_add_images / utr_finalize)
