@@ -164,6 +164,18 @@ int utr_get_fourier_volume(utr_ctx* ctx, float* vol_c);
  */
 int utr_angle_coeffs(const utr_params* params, double angZ_deg, double angY_deg, double* m9);
 
+/* ------------------------------------------------------- post-processing
+ * High-pass filter of a reconstructed volume (UTR.cpp:1357-1378, parameter
+ * 23): vol -= Gaussian(vol) with sigma = width / 2.355, the Gaussian applied
+ * in Fourier space on the periodic grid (the reference's GaussFilter, whose
+ * code is in the absent XArrayLibrary, restated as exp(-2 pi^2 sigma^2 q^2)).
+ * vol is nz*ny*nx floats (x fastest) with steps xst, yst, zst and width in
+ * the same length unit; on the device (d_vol) or in host memory.  Edges must
+ * be FFT lengths the library supports. */
+int utr_highpass3d_device(float* d_vol, int nx, int ny, int nz, double xst, double yst, double zst,
+                          double width);
+int utr_highpass3d(float* vol, int nx, int ny, int nz, double xst, double yst, double zst, double width);
+
 /* ----------------------------------------------------------- batched form
  * Several independent reconstructions of uneven size run as one HIP graph:
  * each job's launches (2D stage of all its rows, gridding, 3D inverse FFT)

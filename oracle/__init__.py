@@ -301,3 +301,20 @@ def dht_reconstruct(params, rows, images, weights=None, nthreads=1):
                                       nthreads):
         raise OracleError(lib().ora_last_error().decode())
     return vol
+
+
+def highpass3d(vol, xst, yst, zst, width):
+    """CPU restatement of the high-pass filter of UTR.cpp:1357-1378
+    (parameter 23): vol - G(vol), G the Gaussian of FWHM `width`
+    (sigma = width / 2.355) applied in Fourier space on the periodic grid,
+    exp(-2 pi^2 sigma^2 |q|^2) with q in cycles per length unit.  The
+    reference's GaussFilter is in the absent XArrayLibrary; this is the
+    textbook form (parity unpinned), checked against closed forms."""
+    v = np.asarray(vol, dtype=np.float64)
+    nz, ny, nx = v.shape
+    sigma = width / 2.355
+    qz = np.fft.fftfreq(nz, d=zst)[:, None, None]
+    qy = np.fft.fftfreq(ny, d=yst)[None, :, None]
+    qx = np.fft.fftfreq(nx, d=xst)[None, None, :]
+    g = np.exp(-2.0 * np.pi ** 2 * sigma ** 2 * (qx ** 2 + qy ** 2 + qz ** 2))
+    return v - np.real(np.fft.ifftn(np.fft.fftn(v) * g))

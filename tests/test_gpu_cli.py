@@ -153,3 +153,15 @@ def test_cli_subseries_weights(tmp_path, io):
     r2["weights"] = np.asarray(w, np.float32)
     ref = oracle.reconstruct(p, r2, imgs[sel], weights=r2["weights"])
     assert rel(read_stack(io, tmp_path, "out/vol.grd", 64), ref) < TOL
+
+
+def test_cli_highpass_filter(tmp_path, io):
+    # parameter 23: low-pass width 2 A -> the volume minus its Gaussian low-pass
+    # (UTR.cpp:1357-1378), no threshold
+    p, rows, imgs = setup_case(tmp_path, io, override={23: "2.0 0.0 -1000000.0"}, seed=9)
+    out = run(tmp_path)
+    assert "High-pass filtering" in out
+    ref = oracle.reconstruct(p, rows, imgs)
+    ref = oracle.highpass3d(ref, 0.4, 0.4, 0.4, 2.0)
+    vol = read_stack(io, tmp_path, "out/vol.grd", 64)
+    assert rel(vol, ref) < 2e-5

@@ -39,6 +39,7 @@ EXPORTS = [
     "utr_batch_create", "utr_batch_run", "utr_batch_job_form", "utr_batch_destroy",
     "dht_create", "dht_add_images", "dht_add_images_device", "dht_finalize", "dht_finalize_device",
     "dht_reset", "dht_dims", "dht_get_timings", "dht_destroy",
+    "utr_highpass3d", "utr_highpass3d_device",
 ]
 
 DHT_FIELDS = [
@@ -164,6 +165,9 @@ def load():
                            ctypes.POINTER(ctypes.c_int)]
     L.dht_get_timings.argtypes = [vp, ctypes.POINTER(UtrTimings)]
     L.dht_destroy.argtypes = [vp]
+    d = ctypes.c_double
+    for f in ("utr_highpass3d", "utr_highpass3d_device"):
+        getattr(L, f).argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, d, d, d, d]
     _lib = L
     return L
 
@@ -427,6 +431,16 @@ class DhtReconstructor:
 
     def __exit__(self, *a):
         self.close()
+
+
+def highpass3d(vol, xst, yst, zst, width):
+    """High-pass filter of a (nz, ny, nx) float32 volume on the GPU:
+    vol - Gaussian(vol) with sigma = width / 2.355 (UTR.cpp:1357-1378).
+    Returns a new array."""
+    out = np.ascontiguousarray(vol, dtype=np.float32).copy()
+    nz, ny, nx = out.shape
+    _check(load().utr_highpass3d(out.ctypes.data, nx, ny, nz, xst, yst, zst, width))
+    return out
 
 
 def dht_reconstruct(params, rows, images):

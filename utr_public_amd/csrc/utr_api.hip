@@ -826,3 +826,5 @@ int utr_batch_job_form(const utr_batch* b, int j) {
 
 // DHT variants (methods 3-5) share the helpers above
 #include "dht.hip"
+// post-processing of the reconstructed volume
+#include "post.hip"

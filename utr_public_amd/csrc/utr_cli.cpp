@@ -186,7 +186,6 @@ int run(int argc, char** argv) {
     // features of the reference this build does not provide
     if (c.reprocess) unsupported("importing and reprocessing an existing 3D volume (parameter 33)");
     if (c.symmetry_mode == 2) unsupported("post-applied symmetry (parameter 20 = 2)");
-    if (c.lp_filter > 0 && c.lp_filter < (c.xhi - c.xlo)) unsupported("the 3D high-pass filter (parameter 23)");
     if (c.rot_out[0] != 0 || c.rot_out[1] != 0 || c.rot_out[2] != 0)
         unsupported("rotation of the output volume (parameter 28)");
     if (c.peaks) unsupported("peak localization (parameter 24)");
@@ -321,7 +320,13 @@ int run(int argc, char** argv) {
         write_stack(cg, S, fm, ext, ax, t_write);
     }
 
-    // post-processing (UTR.cpp:1383-1490)
+    // post-processing (UTR.cpp:1357-1490)
+    if (c.lp_filter > 0 && c.lp_filter < (c.xhi - c.xlo)) {
+        // high-pass: the volume minus its Gaussian low-pass (UTR.cpp:1368-1374), on the GPU
+        printf("\nHigh-pass filtering the reconstructed 3D object ...");
+        const double zst = (c.zhi - c.zlo) / (double)nz;
+        check(utr_highpass3d(vol.v.data(), nx, ny, nz, c.xst, c.xst, zst, c.lp_filter));
+    }
     float vmin = vol.v.empty() ? 0.f : *std::min_element(vol.v.begin(), vol.v.end());
     if (c.background != 0 || vmin < c.threshold) {
         printf("\nThresholding the reconstructed 3D object ...");
