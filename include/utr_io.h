@@ -39,6 +39,16 @@ int utr_io_write_slice(const char* path, long nx, long ny, const float* v, doubl
  */
 int utr_io_file_name(const char* tmpl, long count, long index, char* out, long cap);
 
+/*
+ * Peak localization of FindPeaks.cpp:14-171 (UTR parameters 24, 25) on an
+ * nz*ny*nx volume with extent6 = {xlo, xhi, ylo, yhi, zlo, zhi}: vol is
+ * replaced by the isolated peaks; their (i, j, k) indices and heights, in
+ * descending height, go to out_ijk (3 ints each) and out_height (up to cap
+ * entries; either may be NULL); *npeaks receives their number.
+ */
+int utr_io_find_peaks(float* vol, long nx, long ny, long nz, const double* extent6, double atom_xy,
+                      double atom_z, int* out_ijk, float* out_height, long cap, long* npeaks);
+
 const char* utr_io_last_error(void);
 
 #ifdef __cplusplus

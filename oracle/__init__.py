@@ -318,3 +318,60 @@ def highpass3d(vol, xst, yst, zst, width):
     qx = np.fft.fftfreq(nx, d=xst)[None, None, :]
     g = np.exp(-2.0 * np.pi ** 2 * sigma ** 2 * (qx ** 2 + qy ** 2 + qz ** 2))
     return v - np.real(np.fft.ifftn(np.fft.fftn(v) * g))
+
+
+def find_peaks(vol, extent, atom_xy, atom_z):
+    """Restatement of FindPeaks.cpp:14-171 for small volumes (test oracle).
+    Returns (peak volume, [(i, j, k, height), ...] in descending height).
+    Points outside the sphere of mean-square radius of the box take the
+    volume minimum (:38-63); each atom-sized box keeps its first largest
+    positive value (:65-100); the smaller of two peaks closer than atom_xy is
+    dropped (:102-152).  The reference's thread order of the found peaks is
+    replaced by (k, j, i) order with a stable sort (ties: later peak first)."""
+    v = np.array(vol, dtype=np.float32)
+    nz, ny, nx = v.shape
+    xlo, xhi, ylo, yhi, zlo, zhi = extent
+    xst, yst, zst = (xhi - xlo) / nx, (yhi - ylo) / ny, (zhi - zlo) / nz
+    xR, yR, zR = (xhi - xlo) / 2.0, (yhi - ylo) / 2.0, (zhi - zlo) / 2.0
+    R2 = (xR * xR + yR * yR + zR * zR) / 3.0
+    vmin = v.min()
+    zz = (-zR + zst * np.arange(nz)) ** 2
+    yy = (-yR + yst * np.arange(ny)) ** 2
+    xx = (-xR + xst * np.arange(nx)) ** 2
+    v[(zz[:, None, None] + yy[None, :, None]) + xx[None, None, :] > R2] = vmin
+    ka, ja, ia = int(atom_z / zst + 0.5), int(atom_xy / yst + 0.5), int(atom_xy / xst + 0.5)
+    found = []
+    for k in range(ka, nz - 2 * ka, ka):
+        for j in range(ja, ny - 2 * ja, ja):
+            for i in range(ia, nx - 2 * ia, ia):
+                box = v[k:k + ka, j:j + ja, i:i + ia]
+                m = box.max()
+                if m > 0:
+                    kk, jj, ii = np.unravel_index(int(np.argmax(box)), box.shape)
+                    box[...] = 0
+                    box[kk, jj, ii] = m
+                    found.append((i + ii, j + jj, k + kk, np.float32(m)))
+                else:
+                    box[...] = 0
+    order = sorted(range(len(found)), key=lambda n: found[n][3])
+    h = [found[n][3] for n in order]
+    pos = [(np.float32(xlo + xst * found[n][0]), np.float32(ylo + yst * found[n][1]),
+            np.float32(zlo + zst * found[n][2])) for n in order]
+    d2 = np.float32(atom_xy * atom_xy)
+    kept = []
+    for nn in range(len(order) - 1, -1, -1):
+        if h[nn] == 0:
+            continue
+        kept.append(found[order[nn]])
+        a = pos[nn]
+        for mm in range(nn - 1, -1, -1):
+            if h[mm] == 0:
+                continue
+            b = pos[mm]
+            dx, dy, dz = a[0] - b[0], a[1] - b[1], a[2] - b[2]
+            if np.float32(dx * dx + dy * dy + dz * dz) < d2:
+                h[mm] = np.float32(0)
+    out = np.zeros_like(v)
+    for i, j, k, m in kept:
+        out[k, j, i] = m
+    return out, kept

@@ -165,3 +165,27 @@ def test_cli_highpass_filter(tmp_path, io):
     ref = oracle.highpass3d(ref, 0.4, 0.4, 0.4, 2.0)
     vol = read_stack(io, tmp_path, "out/vol.grd", 64)
     assert rel(vol, ref) < 2e-5
+
+
+def test_cli_peak_localization(tmp_path, io):
+    # parameters 24-25: the volume becomes its isolated peaks (FindPeaks.cpp),
+    # listed in out/vol.xyz and written to aux/volA*.grd
+    p, rows, imgs = setup_case(tmp_path, io, override={24: "1", 25: "1.2 1.2"}, seed=10)
+    out = run(tmp_path)
+    assert "peak positions have been found" in out
+    ref = oracle.reconstruct(p, rows, imgs)
+    n, xst = 64, 0.4
+    ref_peaks_vol, ref_peaks = oracle.find_peaks(ref, (0.0, n * xst, 0.0, n * xst, 0.0, n * xst), 1.2, 1.2)
+    vol = read_stack(io, tmp_path, "out/vol.grd", 64)
+    aux = read_stack(io, tmp_path, "aux/volA.grd", 64)
+    assert np.array_equal(vol, aux)
+    got = {tuple(int(x) for x in np.argwhere(vol != 0)[q]) for q in range(np.count_nonzero(vol))}
+    want = {(k, j, i) for i, j, k, _ in ref_peaks}
+    # FP32 against FP64: allow a few near-tie decisions to differ
+    assert len(got ^ want) <= max(2, len(want) // 50), (len(got), len(want), len(got ^ want))
+    common = sorted(got & want)
+    a = np.array([vol[c] for c in common], np.float64)
+    b = np.array([ref_peaks_vol[c] for c in common], np.float64)
+    assert np.abs(a - b).max() < 1e-4 * np.abs(b).max()
+    lines = open(tmp_path / "out" / "vol.xyz").read().split("\n")
+    assert lines[-2] == "-1" and len(lines) - 4 == len(got)

@@ -131,7 +131,7 @@ int run(int argc, char** argv) {
         else if (a == "--gather") precision = UTR_FP32_GATHER;
         else throw std::runtime_error("Error: unknown command-line option " + a);
     }
-    long t_read = 0, t_write = 0;
+    long t_read = 0, t_write = 0, t_peaks = 0;
 
     printf("\nReading input parameter file %s ...", pfile.c_str());
     Config c = read_config(pfile);
@@ -188,7 +188,6 @@ int run(int argc, char** argv) {
     if (c.symmetry_mode == 2) unsupported("post-applied symmetry (parameter 20 = 2)");
     if (c.rot_out[0] != 0 || c.rot_out[1] != 0 || c.rot_out[2] != 0)
         unsupported("rotation of the output volume (parameter 28)");
-    if (c.peaks) unsupported("peak localization (parameter 24)");
     for (const auto& o : rows) {
         if (o.shift_x != 0 || o.shift_y != 0) unsupported("image shifts from .relionnew files");
         if (astig) unsupported("astigmatism from .relionnew files");
@@ -369,6 +368,42 @@ int run(int argc, char** argv) {
         zhi -= zlo;
         zlo = 0;
     }
+    if (c.peaks) {
+        // peak localization (UTR.cpp:1427-1446, FindPeaks.cpp:14-171): the volume
+        // becomes the isolated peaks, listed in an XYZ file beside the output
+        // and written as slices into the auxiliary folder with "A" appended
+        printf("\nSearching for peak positions in the 3D distribution of the electrostatic potential or beta ...");
+        const auto tp = Clock::now();
+        const auto peaks = find_peaks(vol, xlo, xhi, ylo, yhi, zlo, zhi, c.atom_xy, c.atom_z);
+        if (peaks.empty()) {
+            printf("\n\n!!WARNING: no peaks have been found!");
+        } else {
+            printf("\n%d peak positions have been found in total.", (int)peaks.size());
+            constexpr size_t kMaxAtoms = 500000;  // natommax, UTR.h:10
+            std::string xyz = c.output_template;
+            xyz.replace(xyz.find_last_of('.') + 1, std::string::npos, "xyz");
+            if (peaks.size() > kMaxAtoms) {
+                printf("\nNot saving the reconstructed peak-localized 3D object into an XYZ file, since too many peaks (%d) have been found.",
+                       (int)peaks.size());
+            } else {
+                printf("\nSaving the reconstructed peak-localized 3D object into an XYZ file %s ...", xyz.c_str());
+                write_xyz(xyz, (float)(xhi - xlo), (float)(zhi - zlo), peaks);
+            }
+        }
+        std::string base = std::filesystem::path(c.output_template).filename().string();
+        base.insert(base.find_last_of('.'), "A");
+        const std::string e = file_extension(base);
+        if (e != ".TIF" && e != ".TIFF") base.replace(base.find_last_of('.'), std::string::npos, ".grd");
+        auto names = file_names(c.aux_folder + base, vol.nz);
+        printf("\nSaving the reconstructed peak-localized 3D object into output files %s, etc. ...", names[0].c_str());
+        Config cg = c;
+        cg.b16_minin = cg.b16_maxin = 0;
+        cg.b16_maxout = 0;
+        const double pext[3][2] = {{xlo, xhi}, {ylo, yhi}, {zlo, zhi}};
+        const int pax[3] = {0, 1, 2};
+        write_stack(cg, vol, names, pext, pax, t_write);
+        t_peaks = std::chrono::duration_cast<std::chrono::milliseconds>(Clock::now() - tp).count();
+    }
     static const int axes_of[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 2, 0}, {1, 0, 2}, {2, 0, 1}, {2, 1, 0}};
     if (c.reslice || c.flip[0] || c.flip[1] || c.flip[2]) {
         printf("\nReslicing the output 3D volume ...");
@@ -389,7 +424,7 @@ int run(int argc, char** argv) {
         printf("\nTotal 2D FFT and CTF correction time = %ld (ms)", (long)std::lround(tm.fft2d_ctf_ms));
         printf("\nTotal backpropagation time = %ld (ms)", (long)std::lround(tm.backprop_ms));
         printf("\nTotal 3D FFT time = %ld (ms)", (long)std::lround(tm.fft3d_ms));
-        printf("\nTotal peak localization time = %ld (ms)", 0L);
+        printf("\nTotal peak localization time = %ld (ms)", t_peaks);
     }
     return 0;
 }

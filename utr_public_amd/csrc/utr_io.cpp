@@ -911,6 +911,114 @@ int io_fail(const std::exception& e) {
 }
 }  // namespace
 
+// ------------------------------------------------------------ peak search
+
+namespace utrio {
+
+std::vector<Peak> find_peaks(Volume& vol, double xlo, double xhi, double ylo, double yhi, double zlo,
+                             double zhi, double atom_xy, double atom_z) {
+    const long nx = vol.nx, ny = vol.ny, nz = vol.nz;
+    const double xst = (xhi - xlo) / nx, yst = (yhi - ylo) / ny, zst = (zhi - zlo) / nz;
+    // points outside the sphere of mean-square radius of the box are set to
+    // the volume minimum, which excludes them (FindPeaks.cpp:38-63)
+    const double xR = (xhi - xlo) / 2.0, yR = (yhi - ylo) / 2.0, zR = (zhi - zlo) / 2.0;
+    const double R2 = (xR * xR + yR * yR + zR * zR) / 3.0;
+    const float vmin = vol.v.empty() ? 0.f : *std::min_element(vol.v.begin(), vol.v.end());
+    for (long k = 0; k < nz; k++) {
+        double zz = -zR + zst * k;
+        zz *= zz;
+        for (long j = 0; j < ny; j++) {
+            double yy = -yR + yst * j;
+            yy = yy * yy + zz;
+            for (long i = 0; i < nx; i++) {
+                double xx = -xR + xst * i;
+                xx = xx * xx + yy;
+                if (xx > R2) vol.at(k, j, i) = vmin;
+            }
+        }
+    }
+    // the largest positive value of every atom-sized box, boxes tiled from one
+    // box in from the low edges to two boxes from the high edges; everything
+    // else in a box is zeroed (FindPeaks.cpp:65-100).  The reference zeroes
+    // the previous running maximum, starting from [0][0][0]; that voxel never
+    // lies in a box and is overwritten below, so it is left out here.
+    const int katom = (int)(atom_z / zst + 0.5), jatom = (int)(atom_xy / yst + 0.5),
+              iatom = (int)(atom_xy / xst + 0.5);
+    std::vector<Peak> found;
+    if (katom <= 0 || jatom <= 0 || iatom <= 0) return found;
+    for (long k = katom; k < nz - katom * 2; k += katom)
+        for (long j = jatom; j < ny - jatom * 2; j += jatom)
+            for (long i = iatom; i < nx - iatom * 2; i += iatom) {
+                double vmax = 0.0;
+                long km = -1, jm = -1, im = -1;
+                for (long kk = k; kk < k + katom; kk++)
+                    for (long jj = j; jj < j + jatom; jj++)
+                        for (long ii = i; ii < i + iatom; ii++) {
+                            float& x = vol.at(kk, jj, ii);
+                            if (x > vmax) {
+                                vmax = x;
+                                if (km >= 0) vol.at(km, jm, im) = 0.f;
+                                km = kk;
+                                jm = jj;
+                                im = ii;
+                            } else {
+                                x = 0.f;
+                            }
+                        }
+                if (vmax > 0) {
+                    Peak p;
+                    p.i = (int)im;
+                    p.j = (int)jm;
+                    p.k = (int)km;
+                    p.x = (float)(xlo + xst * im);
+                    p.y = (float)(ylo + yst * jm);
+                    p.z = (float)(zlo + zst * km);
+                    p.height = vol.at(km, jm, im);
+                    found.push_back(p);
+                }
+            }
+    // keep peaks in descending height; drop any smaller peak closer than
+    // atom_xy to a kept one (FindPeaks.cpp:102-152).  The reference sorts the
+    // peaks of its threads with qsort; here they are found in (k, j, i) order
+    // and sorted stably by height, ties keeping the later peak first.
+    std::vector<int> order(found.size());
+    for (size_t n = 0; n < order.size(); n++) order[n] = (int)n;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return found[a].height < found[b].height; });
+    std::vector<float> v(order.size());
+    for (size_t n = 0; n < order.size(); n++) v[n] = found[order[n]].height;
+    const float d2 = (float)(atom_xy * atom_xy);
+    std::vector<Peak> kept;
+    for (long nn = (long)order.size() - 1; nn >= 0; nn--) {
+        if (v[nn] == 0.f) continue;
+        const Peak& a = found[order[nn]];
+        kept.push_back(a);
+        for (long mm = nn - 1; mm >= 0; mm--) {
+            if (v[mm] == 0.f) continue;
+            const Peak& b = found[order[mm]];
+            if ((a.x - b.x) * (a.x - b.x) + (a.y - b.y) * (a.y - b.y) + (a.z - b.z) * (a.z - b.z) < d2)
+                v[mm] = 0.f;
+        }
+    }
+    // the volume becomes the isolated peaks (FindPeaks.cpp:154-157)
+    std::fill(vol.v.begin(), vol.v.end(), 0.f);
+    for (const Peak& p : kept) vol.at(p.k, p.j, p.i) = p.height;
+    return kept;
+}
+
+void write_xyz(const std::string& path, float a, float c, const std::vector<Peak>& peaks) {
+    FILE* f = std::fopen(path.c_str(), "w");
+    if (!f) throw std::runtime_error("Error: cannot open output file " + path + ".");
+    std::fprintf(f, "Peak positions in the reconstructed 3D distribution\n");
+    std::fprintf(f, "%.7g %.7g %.7g\n", a, a, c);
+    for (const Peak& p : peaks)
+        std::fprintf(f, "%d %.7g %.7g %.7g %.7g %.7g\n", 6, p.x, p.y, p.z, p.height, 0.0);
+    std::fprintf(f, "-1\n");
+    if (std::fclose(f) != 0) throw std::runtime_error("Error: cannot write output file " + path + ".");
+}
+
+}  // namespace utrio
+
 extern "C" int utr_io_read_image(const char* path, long nx, long ny, long header, long element,
                                  int big_endian, float* out, long cap, long* out_nx, long* out_ny) {
     try {
@@ -973,6 +1081,35 @@ extern "C" int utr_io_file_name(const char* tmpl, long count, long index, char* 
         if ((long)s.size() + 1 > cap) throw std::runtime_error("Error: file name buffer too small.");
         std::copy(s.begin(), s.end(), out);
         out[s.size()] = '\0';
+        return 0;
+    } catch (std::exception& ex) {
+        return io_fail(ex);
+    }
+}
+
+extern "C" int utr_io_find_peaks(float* vol, long nx, long ny, long nz, const double* extent6,
+                                 double atom_xy, double atom_z, int* out_ijk, float* out_height,
+                                 long cap, long* npeaks) {
+    try {
+        if (!vol || !extent6 || !npeaks || nx <= 0 || ny <= 0 || nz <= 0)
+            throw std::runtime_error("Error: invalid arguments to utr_io_find_peaks.");
+        utrio::Volume v;
+        v.nx = nx;
+        v.ny = ny;
+        v.nz = nz;
+        v.v.assign(vol, vol + (size_t)nx * ny * nz);
+        const auto peaks = utrio::find_peaks(v, extent6[0], extent6[1], extent6[2], extent6[3],
+                                             extent6[4], extent6[5], atom_xy, atom_z);
+        std::copy(v.v.begin(), v.v.end(), vol);
+        *npeaks = (long)peaks.size();
+        for (long n = 0; n < (long)peaks.size() && n < cap; n++) {
+            if (out_ijk) {
+                out_ijk[3 * n] = peaks[n].i;
+                out_ijk[3 * n + 1] = peaks[n].j;
+                out_ijk[3 * n + 2] = peaks[n].k;
+            }
+            if (out_height) out_height[n] = peaks[n].height;
+        }
         return 0;
     } catch (std::exception& ex) {
         return io_fail(ex);

@@ -135,4 +135,18 @@ void threshold_low(Volume& vol, float a, float b);
 Volume reslice(const Volume& vol, int mode, bool fx, bool fy, bool fz);
 Volume thick_slices(const Volume& vol, int mode, int step);
 
+// One located peak: grid indices, coordinates and height.
+struct Peak {
+    int i, j, k;
+    float x, y, z, height;
+};
+// Peak localization of FindPeaks.cpp:14-171 (parameters 24, 25): at most one
+// peak per atom-sized box inside the inscribed sphere, the smaller of any two
+// peaks closer than atom_xy removed; vol is replaced by the isolated peaks.
+std::vector<Peak> find_peaks(Volume& vol, double xlo, double xhi, double ylo, double yhi, double zlo,
+                             double zhi, double atom_xy, double atom_z);
+// Kirkland's XYZ format: title, cell a b c, "Z x y z occupancy wobble" per
+// atom (carbon, occupancy = peak height, wobble 0), "-1".
+void write_xyz(const std::string& path, float a, float c, const std::vector<Peak>& peaks);
+
 }  // namespace utrio
