@@ -36,9 +36,10 @@ template <>
 __device__ __forceinline__ double dev_log<double>(float x) { return log((double)x); }
 
 // LDS bytes per FFT buffer, per kernel (measured on gfx950 at 256^3: small
-// batches, and so more resident workgroups, win everywhere except the
+// batches, and so more resident workgroups, win for the row and 3D passes;
+// the persistent column pass prefers 64-byte column segments, as does the
 // Hermitian pack, which reads mirrored column pairs)
-constexpr int LDS_ROWS = 8192, LDS_COLS = 8192, LDS_PACKZ = 16384, LDS_FFTY = 8192, LDS_C2R = 8192,
+constexpr int LDS_ROWS = 8192, LDS_COLS = 16384, LDS_PACKZ = 16384, LDS_FFTY = 8192, LDS_C2R = 8192,
               LDS_DHT = 16384;
 
 // sequences per LDS buffer of at most `budget` bytes, between 1 and 16
