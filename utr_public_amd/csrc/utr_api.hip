@@ -430,7 +430,7 @@ int utr_create(const utr_params* params, int nrows, const double* angZ_deg, cons
                   hipMemcpyHostToDevice) != hipSuccess)
         return fail(set_err(UTR_E_DEVICE, "row table upload"));
     const size_t N = vox(c), n2 = (size_t)g.nx * g.ny;
-    const size_t nh = (size_t)g.nz * g.ny * (g.nx / 2 + 1);
+    const size_t nh = (size_t)g.nz * g.ny * utr::h_stride(g.nx);
     int rc;
     if (precision != UTR_FP64) {
         if ((rc = upload_twiddles<float>(c, c->twx32, c->twy32, c->twz32))) return fail(rc);

@@ -1285,6 +1285,11 @@ __global__ void __launch_bounds__(256) k_normalize(cplx<T>* __restrict__ V,
 
 constexpr int CW_MAX = 16;
 
+// Row stride of the half spectrum H: nx/2 + 1 entries rounded up to 8, so
+// that every row starts on a 64-byte boundary and the column segments the 3D
+// passes move stay whole cache lines.
+__host__ __device__ constexpr int h_stride(int nx) { return (nx / 2 + 1 + 7) & ~7; }
+
 // Hermitian combination of V and its mirror (the symmetrisation of
 // UTR.cpp:1203-1217 followed by the half the c2r plan consumes), checkerboard
 // (-1)^(kz+ky+w) for the centred z/y indices, and inverse FFT along z.
@@ -1299,7 +1304,7 @@ __global__ void __launch_bounds__(256) k_pack_z(const cplx<T>* __restrict__ V,
     __shared__ cplx<T> sm[2 * lds_len(CW * NZ)];
     cplx<T>* a = sm;
     cplx<T>* b = sm + lds_len(CW * NZ);
-    const int nx = g.nx, ny = g.ny, W = nx / 2 + 1;
+    const int nx = g.nx, ny = g.ny, W = nx / 2 + 1, WP = h_stride(nx);
     int tx, ty;
     if (!xcd_tile(gx, gy, tx, ty)) return;
     const int j = ty;
@@ -1334,7 +1339,7 @@ __global__ void __launch_bounds__(256) k_pack_z(const cplx<T>* __restrict__ V,
     fft_lds<T, NZ>(a, b, nb, tw, +1, threadIdx.x, blockDim.x);
     for (int t = threadIdx.x; t < nb * NZ; t += blockDim.x) {
         const int k = t / nb, c = t - k * nb;
-        H[((size_t)k * ny + j) * W + w0 + c] = a[lx(c * NZ + k)];
+        H[((size_t)k * ny + j) * WP + w0 + c] = a[lx(c * NZ + k)];
     }
 }
 
@@ -1346,22 +1351,22 @@ __global__ void __launch_bounds__(256) k_fft_y(cplx<T>* __restrict__ H, GeomDev 
     __shared__ cplx<T> sm[2 * lds_len(CW * NY)];
     cplx<T>* a = sm;
     cplx<T>* b = sm + lds_len(CW * NY);
-    const int W = g.nx / 2 + 1;
+    const int W = g.nx / 2 + 1, WP = h_stride(g.nx);
     int tx, ty;
     if (!xcd_tile(gx, gy, tx, ty)) return;
     const int k = ty;
     const int w0 = tx * CW;
     const int nb = min(CW, W - w0);
-    cplx<T>* base = H + (size_t)k * NY * W + w0;
+    cplx<T>* base = H + (size_t)k * NY * WP + w0;
     for (int t = threadIdx.x; t < nb * NY; t += blockDim.x) {
         const int y = t / nb, c = t - y * nb;
-        a[lx(c * NY + y)] = base[(size_t)y * W + c];
+        a[lx(c * NY + y)] = base[(size_t)y * WP + c];
     }
     __syncthreads();
     fft_lds<T, NY>(a, b, nb, tw, +1, threadIdx.x, blockDim.x);
     for (int t = threadIdx.x; t < nb * NY; t += blockDim.x) {
         const int y = t / nb, c = t - y * nb;
-        base[(size_t)y * W + c] = a[lx(c * NY + y)];
+        base[(size_t)y * WP + c] = a[lx(c * NY + y)];
     }
 }
 
@@ -1376,7 +1381,7 @@ __global__ void __launch_bounds__(256) k_c2r_x(const cplx<T>* __restrict__ H,
     __shared__ cplx<T> sm[2 * lds_len(NB * NX)];
     cplx<T>* a = sm;
     cplx<T>* b = sm + lds_len(NB * NX);
-    const int ny = g.ny, W = NX / 2 + 1;
+    const int ny = g.ny, WP = h_stride(NX);
     int tx, ty;
     if (!xcd_tile(gx, gy, tx, ty)) return;
     const int k = ty;
@@ -1385,8 +1390,8 @@ __global__ void __launch_bounds__(256) k_c2r_x(const cplx<T>* __restrict__ H,
     for (int t = threadIdx.x; t < np * NX; t += blockDim.x) {
         const int pp = t / NX, w = t - pp * NX;
         const int j = 2 * (p0 + pp);
-        const cplx<T>* r1 = H + ((size_t)k * ny + j) * W;
-        const cplx<T>* r2 = r1 + W;
+        const cplx<T>* r1 = H + ((size_t)k * ny + j) * WP;
+        const cplx<T>* r2 = r1 + WP;
         cplx<T> x1, x2;
         if (w <= NX / 2) {
             x1 = r1[w];
