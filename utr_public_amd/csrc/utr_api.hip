@@ -135,6 +135,7 @@ struct utr_ctx {
     DevBuf<cplx<float>> spec32, V32, H32, twx32, twy32, twz32;
     DevBuf<float> S32, vol32;
     DevBuf<unsigned> rowmax;  // per-row max |K| as float bits (FP32 forms)
+    DevBuf<int> scales;       // fixed-point scales of the plane gridding
     // FP64 form
     DevBuf<cplx<double>> spec64, V64, H64, twx64, twy64, twz64;
     DevBuf<double> S64, vol64;
@@ -303,12 +304,13 @@ int launch_grid32(utr_ctx* c, float* d_S) {
         const long ntiles = (long)((g.nx + utr::PX - 1) / utr::PX) * ((g.nz + utr::PZ - 1) / utr::PZ);
         const long nwg = ntiles * g.ny;
         if (nwg <= 0 || nwg > 0x7ffffff0L) return set_err(UTR_E_INVALID, "volume too large");
+        utr::k_fx_scales<<<1, 256, 0, c->stream>>>(c->rowmax.p, c->plan.nrows, c->scales.p);
         if (g.rot_centre_shift)
             utr::k_grid_yp<true><<<utr::xcd_grid((unsigned)nwg), 256, 0, c->stream>>>(
-                c->spec32.p, c->rows.p, c->plan.nrows, c->rowmax.p, g, c->V32.p, d_S, (int)ntiles);
+                c->spec32.p, c->rows.p, c->plan.nrows, c->scales.p, g, c->V32.p, d_S, (int)ntiles);
         else
             utr::k_grid_yp<false><<<utr::xcd_grid((unsigned)nwg), 256, 0, c->stream>>>(
-                c->spec32.p, c->rows.p, c->plan.nrows, c->rowmax.p, g, c->V32.p, d_S, (int)ntiles);
+                c->spec32.p, c->rows.p, c->plan.nrows, c->scales.p, g, c->V32.p, d_S, (int)ntiles);
     } else {
         const long nblocks = (long)((g.nx + utr::SBX - 1) / utr::SBX) *
                              ((g.ny + utr::SBY - 1) / utr::SBY) * ((g.nz + utr::SBZ - 1) / utr::SBZ);
@@ -435,7 +437,8 @@ int utr_create(const utr_params* params, int nrows, const double* angZ_deg, cons
     if (precision != UTR_FP64) {
         if ((rc = upload_twiddles<float>(c, c->twx32, c->twy32, c->twz32))) return fail(rc);
         if (c->spec32.alloc((size_t)nrows * n2) != hipSuccess || c->V32.alloc(N) != hipSuccess ||
-            c->H32.alloc(nh) != hipSuccess || c->rowmax.alloc(nrows) != hipSuccess)
+            c->H32.alloc(nh) != hipSuccess || c->rowmax.alloc(nrows) != hipSuccess ||
+            c->scales.alloc(2) != hipSuccess)
             return fail(set_err(UTR_E_NOMEM, "device allocation of the FP32 buffers failed"));
         // on the context's stream: it is non-blocking, so a null-stream memset
         // would not be ordered before the kernels launched on it
@@ -596,7 +599,7 @@ int utr_destroy(utr_ctx* c) {
     c->rows.release();
     c->img_stage.release();
     c->spec32.release(); c->V32.release(); c->H32.release(); c->S32.release(); c->vol32.release();
-    c->twx32.release(); c->twy32.release(); c->twz32.release(); c->rowmax.release();
+    c->twx32.release(); c->twy32.release(); c->twz32.release(); c->rowmax.release(); c->scales.release();
     c->spec64.release(); c->V64.release(); c->H64.release(); c->S64.release(); c->vol64.release();
     c->twx64.release(); c->twy64.release(); c->twz64.release();
     for (auto& e : c->ev)

@@ -966,18 +966,29 @@ __device__ __forceinline__ void fx_scales(const unsigned* __restrict__ rowmax, i
     sw = min(61 - ea, 50);
 }
 
+// The fixed-point scales of a job, once: scales[0] = sv, scales[1] = sw.
+__global__ void __launch_bounds__(256) k_fx_scales(const unsigned* __restrict__ rowmax, int nrows,
+                                                   int* __restrict__ scales) {
+    __shared__ float s_red[4];
+    int sv, sw;
+    fx_scales(rowmax, nrows, s_red, sv, sw);
+    if (threadIdx.x == 0) {
+        scales[0] = sv;
+        scales[1] = sw;
+    }
+}
+
 // ROT: rows carry the centre-shift phase (compiled separately: its FP64
 // sincos would otherwise set the register budget of the common case, which
 // fits 4 workgroups per CU).
 template <bool ROT>
 __global__ void __launch_bounds__(256, ROT ? 1 : 4) k_grid_yp(const cplx<float>* __restrict__ spec,
                                                  const RowParams* __restrict__ rows, int nrows,
-                                                 const unsigned* __restrict__ rowmax, GeomDev g,
+                                                 const int* __restrict__ scales, GeomDev g,
                                                  cplx<float>* __restrict__ Vout,
                                                  float* __restrict__ Sout, int ntiles) {
     __shared__ unsigned long long acc[3 * PPAD];
     __shared__ float tiny[3 * PVOX];
-    __shared__ float s_red[4];
     __shared__ unsigned short s_row[P_LIST];  // rows whose line reaches the tile
     __shared__ short s_a0[P_LIST];                // first a of its clipped line
     __shared__ int s_pre[P_LIST + 1];             // point counts, then their prefix
@@ -990,9 +1001,9 @@ __global__ void __launch_bounds__(256, ROT ? 1 : 4) k_grid_yp(const cplx<float>*
     const int x0 = (tile % ntx) * PX, z0 = (tile / ntx) * PZ;
     for (int t = tid; t < 3 * PPAD; t += 256) acc[t] = 0ull;
     for (int t = tid; t < 3 * PVOX; t += 256) tiny[t] = 0.f;
-    int sv, sw;
-    fx_scales(rowmax, nrows, s_red, sv, sw);
+    const int sv = scales[0], sw = scales[1];
     const double vscale = ldexp(1.0, sv), wscale = ldexp(1.0, sw);
+    __syncthreads();  // LDS zeroed
 
     const float fx2 = (float)(g.fxst * g.fxst), fy2 = (float)(g.fyst * g.fyst);
     const float wl2f = (float)g.wl2;
