@@ -251,29 +251,33 @@ int launch_3d(utr_ctx* c, const cplx<T>* d_V, cplx<T>* d_H, TO* d_vol, const cpl
     return UTR_OK;
 }
 
-int add_rows(utr_ctx* c, int first_row, int count, const float* d_img) {
+// Launches the 2D stage (and, in the plain FP64 form, the scatter) of `count`
+// rows whose images are at d_img (device), without waiting.
+int enqueue_rows(utr_ctx* c, int first_row, int count, const float* d_img) {
     const auto& g = c->plan.g;
-    hipEvent_t e0 = c->ev[6], e1 = c->ev[7];
-    UTR_HIP_TRY(hipEventRecord(e0, c->stream));
     if (c->precision != UTR_FP64) {
-        int rc = launch_2d<float>(c, d_img, first_row, count,
-                                  c->spec32.p + (size_t)first_row * g.ny * g.nx, c->twx32.p,
-                                  c->twy32.p);
-        if (rc) return rc;
-    } else {
-        // plain form: one row at a time, 2D stage then scatter
-        const size_t n2 = (size_t)g.nx * g.ny;
-        for (int r = 0; r < count; r++) {
-            int rc = launch_2d<double>(c, d_img + r * n2, first_row + r, 1, c->spec64.p,
-                                       c->twx64.p, c->twy64.p);
-            if (rc) return rc;
-            const int threads = 256;
-            const int blocks = (int)((n2 + threads - 1) / threads);
-            utr::k_scatter<<<blocks, threads, 0, c->stream>>>(
-                c->spec64.p, c->rows.p, first_row + r, g, (double*)c->V64.p, c->S64.p);
-            UTR_HIP_TRY(hipGetLastError());
-        }
+        return launch_2d<float>(c, d_img, first_row, count,
+                                c->spec32.p + (size_t)first_row * g.ny * g.nx, c->twx32.p, c->twy32.p);
     }
+    // plain form: one row at a time, 2D stage then scatter
+    const size_t n2 = (size_t)g.nx * g.ny;
+    for (int r = 0; r < count; r++) {
+        int rc = launch_2d<double>(c, d_img + r * n2, first_row + r, 1, c->spec64.p, c->twx64.p,
+                                   c->twy64.p);
+        if (rc) return rc;
+        const int threads = 256;
+        const int blocks = (int)((n2 + threads - 1) / threads);
+        utr::k_scatter<<<blocks, threads, 0, c->stream>>>(c->spec64.p, c->rows.p, first_row + r, g,
+                                                          (double*)c->V64.p, c->S64.p);
+        UTR_HIP_TRY(hipGetLastError());
+    }
+    return UTR_OK;
+}
+
+// Waits for the rows enqueued since event e0, adds their time to the 2D
+// phase and marks them added.
+int finish_rows(utr_ctx* c, hipEvent_t e0, int first_row, int count) {
+    hipEvent_t e1 = c->ev[7];
     UTR_HIP_TRY(hipEventRecord(e1, c->stream));
     UTR_HIP_TRY(hipEventSynchronize(e1));
     float ms = 0;
@@ -284,6 +288,14 @@ int add_rows(utr_ctx* c, int first_row, int count, const float* d_img) {
         c->added[first_row + r] = 1;
     }
     return UTR_OK;
+}
+
+int add_rows(utr_ctx* c, int first_row, int count, const float* d_img) {
+    hipEvent_t e0 = c->ev[6];
+    UTR_HIP_TRY(hipEventRecord(e0, c->stream));
+    const int rc = enqueue_rows(c, first_row, count, d_img);
+    if (rc) return rc;
+    return finish_rows(c, e0, first_row, count);
 }
 
 int check_rows(utr_ctx* c, int first_row, int count) {
@@ -473,16 +485,24 @@ int utr_add_images(utr_ctx* c, int first_row, int count, const float* images) {
     if (count == 0) return UTR_OK;
     if (!images) return set_err(UTR_E_INVALID, "null image pointer");
     const auto& g = c->plan.g;
-    const size_t n2 = (size_t)g.nx * g.ny;
-    UTR_HIP_TRY(c->img_stage.alloc(n2 * kStageRows));
-    for (int r = 0; r < count; r += kStageRows) {
-        const int k = std::min(kStageRows, count - r);
-        UTR_HIP_TRY(hipMemcpyAsync(c->img_stage.p, images + (size_t)r * n2, sizeof(float) * n2 * k,
+    const size_t n2 = (size_t)g.nx * g.ny, slot = n2 * kStageRows;
+    // two device staging slots: the runtime's copy of batch b+1 from the
+    // caller's (pageable) memory overlaps the 2D stage of batch b.  (A pinned
+    // bounce buffer filled by one host thread measured slower.)
+    UTR_HIP_TRY(c->img_stage.alloc(2 * slot));
+    hipEvent_t e0 = c->ev[6];
+    UTR_HIP_TRY(hipEventRecord(e0, c->stream));
+    for (int r = 0, b = 0; r < count; r += kStageRows, b++) {
+        const int k = std::min(kStageRows, count - r), s = b & 1;
+        UTR_HIP_TRY(hipMemcpyAsync(c->img_stage.p + s * slot, images + (size_t)r * n2, sizeof(float) * n2 * k,
                                    hipMemcpyHostToDevice, c->stream));
-        rc = add_rows(c, first_row + r, k, c->img_stage.p);
-        if (rc) return rc;
+        rc = enqueue_rows(c, first_row + r, k, c->img_stage.p + s * slot);
+        if (rc) {
+            (void)hipStreamSynchronize(c->stream);
+            return rc;
+        }
     }
-    return UTR_OK;
+    return finish_rows(c, e0, first_row, count);
 }
 
 int utr_finalize_device(utr_ctx* c, float* d_vol) {
