@@ -359,6 +359,213 @@ __global__ void k_dht_finish(const double* __restrict__ acc3, float* __restrict_
 
 }  // namespace utr
 
+namespace utr {
+
+// Real-space splat in y-plane tiles, for batches whose orientations have no
+// rotation about Z (sinZ = 0, cosZ = 1).  Then a sample's x and z grid
+// coordinates do not depend on its source line j, and its y coordinate
+// depends on j alone: source line j reaches planes jj(j) and jj(j) + 1 with
+// the weights dy0(j), dy1(j).  Each workgroup owns a DP_X x DP_Z (x, z) tile
+// of one plane J, takes from each orientation the source lines j with
+// jj(j) in {J - 1, J}, clips every source plane n's i-line to the tile and
+// deals the listed samples to its threads in equal runs.  Coordinates,
+// weights and fixed-point terms are those of k_dht_splat expression by
+// expression (the j-dependent terms it adds are zeros here), so both kernels
+// form the same sums.
+constexpr int DP_X = 32, DP_Z = 32, DP_SX = DP_X + 1, DP_PAD = DP_SX * DP_Z, DP_VOX = DP_X * DP_Z;
+
+__global__ void __launch_bounds__(256) k_dht_splat_yp(const float* __restrict__ P, DhtGeom g,
+                                                      const DhtRow* __restrict__ rows,
+                                                      const double* __restrict__ xs,
+                                                      const double* __restrict__ ys,
+                                                      const double* __restrict__ zs, int a0, int na,
+                                                      const unsigned* __restrict__ pmax,
+                                                      double* __restrict__ acc3, int ntiles) {
+    __shared__ unsigned long long acc[DP_PAD];
+    __shared__ unsigned short s_n[256];   // listed source planes n
+    __shared__ short s_i0[256];           // first i of each listed line
+    __shared__ int s_pre[257];            // sample counts, then their prefix
+    __shared__ int s_cnt[4], s_wsum[4];
+    __shared__ int s_j[4];                // source lines reaching plane J
+    __shared__ double s_wy[4];            // their y weights
+    __shared__ int s_nj;                  // how many
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int tile, J;
+    if (!xcd_tile(ntiles, g.ny, tile, J)) return;
+    const int ntx = (g.nx + DP_X - 1) / DP_X;
+    const int x0 = (tile % ntx) * DP_X, z0 = (tile / ntx) * DP_Z;
+    for (int t = tid; t < DP_PAD; t += 256) acc[t] = 0ull;
+    // the scale of k_dht_splat
+    int ek = 0, ea = 0;
+    const float mk = __uint_as_float(*pmax);
+    frexpf(mk > 0.f ? mk : 1.f, &ek);
+    frexpf((float)na, &ea);
+    const int sc = min(53 - ek - ea, 47 - ek);
+    const double scale = ldexp(1.0, sc);
+    const size_t n2 = (size_t)g.nx * g.ny;
+    const int nsrc = g.nz;  // source planes n
+    for (int ab = 0; ab < na; ab++) {
+        const DhtRow& R = rows[a0 + ab];
+        const double* X = xs + (size_t)(a0 + ab) * 2 * g.nx;
+        const double* Y = ys + (size_t)(a0 + ab) * 2 * g.ny;
+        const double* Z = zs + (size_t)(a0 + ab) * 2 * g.nz;
+        __syncthreads();  // previous orientation's lists are consumed
+        if (tid == 0) {
+            // y of source line j as DHT.cpp:1143-1148 forms it (xx sinZ = 0).
+            // Rounding puts line j in cell j or j - 1, so the lines reaching
+            // plane J lie in J - 2 .. J + 1; a line whose lower cell is J
+            // adds with weight dy0, one whose lower cell is J - 1 with dy1.
+            int cnt = 0;
+            for (int j = max(J - 2, 0); j <= min(J + 1, g.ny - 1); j++) {
+                const double yyy = __dsub_rn(__dadd_rn(g.yc, Y[2 * j + 1]), __dmul_rn(0.0, R.sinZ));
+                double dy1 = __ddiv_rn(__dsub_rn(yyy, g.ylo), g.xst);
+                const int jj = (int)dy1;
+                if (jj < 0 || jj > g.ny2) continue;
+                dy1 = __dsub_rn(dy1, (double)jj);
+                if (jj == J) {
+                    s_j[cnt] = j;
+                    s_wy[cnt++] = __dsub_rn(1.0, dy1);
+                } else if (jj == J - 1) {
+                    s_j[cnt] = j;
+                    s_wy[cnt++] = dy1;
+                }
+            }
+            s_nj = cnt;
+        }
+        __syncthreads();  // s_j, s_wy, s_nj written
+        const int nj = s_nj;
+        if (nj == 0) continue;  // uniform: no source line reaches plane J
+        // clip the source planes' i-lines to the tile (one sample of margin),
+        // 256 planes at a time
+        const float bl[2] = {(float)x0 - 1.f, (float)z0 - 1.f};
+        const float bh[2] = {(float)(x0 + DP_X), (float)(z0 + DP_Z)};
+        for (int nb = 0; nb < nsrc; nb += 256) {
+        bool pass = false;
+        int i0 = 0, i1 = -1;
+        if (nb + tid < nsrc) {
+            const int n = nb + tid;
+            float ilo = 0.f, ihi = (float)(g.nx - 1);
+            const int dd[2] = {0, 2};
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                const int d = dd[q];
+                const float base = R.c0[d] + R.un[d] * n;
+                const float u = R.ui[d];
+                const float l = bl[q] - 0.05f - base, h = bh[q] + 0.05f - base;
+                if (fabsf(u) < 1e-6f) {
+                    if (l > 0.f || h < 0.f) ihi = ilo - 1.f;
+                } else {
+                    float t0 = l / u, t1 = h / u;
+                    if (u < 0.f) { const float tt = t0; t0 = t1; t1 = tt; }
+                    ilo = fmaxf(ilo, floorf(t0) - 1.f);
+                    ihi = fminf(ihi, ceilf(t1) + 1.f);
+                }
+            }
+            ilo = fmaxf(ilo, 0.f);
+            ihi = fminf(ihi, (float)(g.nx - 1));
+            if (ilo <= ihi) {
+                i0 = (int)ilo;
+                i1 = (int)ihi;
+            }
+            pass = i0 <= i1;
+        }
+        const unsigned long long mask = __ballot(pass);
+        const int pre = __popcll(mask & ((1ull << lane) - 1ull));
+        if (lane == 0) s_cnt[wave] = __popcll(mask);
+        __syncthreads();
+        int off = 0;
+        for (int w = 0; w < wave; w++) off += s_cnt[w];
+        const int npt = pass ? i1 - i0 + 1 : 0;
+        if (pass) {
+            s_n[off + pre] = (unsigned short)(nb + tid);
+            s_i0[off + pre] = (short)i0;
+        }
+        int incl = npt;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int v = __shfl_up(incl, o);
+            if (lane >= o) incl += v;
+        }
+        if (lane == 63) s_wsum[wave] = incl;
+        __syncthreads();
+        int woff = 0;
+        for (int w = 0; w < wave; w++) woff += s_wsum[w];
+        if (pass) s_pre[off + pre] = woff + incl - npt;
+        const int nl = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+        const int total = s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
+        if (tid == 0) s_pre[nl] = total;
+        __syncthreads();
+        const float* Pab = P + (size_t)ab * g.nplanes * n2;
+        const double ysin = Y[2 * s_j[0]];  // y_j sinZ: a zero of either sign, same sum
+        const int p0 = (int)(((long long)tid * total) >> 8);
+        const int p1 = (int)(((long long)(tid + 1) * total) >> 8);
+        if (p0 < p1) {
+            int lo = 0, hi = nl - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_pre[mid] <= p0) lo = mid;
+                else hi = mid - 1;
+            }
+            int q = lo;
+            int n = s_n[q], ibase = s_i0[q] - s_pre[q], pend = s_pre[q + 1];
+            for (int p = p0; p < p1; p++) {
+                if (p == pend) {
+                    q++;
+                    n = s_n[q];
+                    ibase = s_i0[q] - s_pre[q];
+                    pend = s_pre[q + 1];
+                }
+                const int i = ibase + p;
+                // DHT.cpp:1119-1150, operation by operation
+                const double zcos = Z[2 * n + 1], zsin = Z[2 * n];
+                const double zzz = __dadd_rn(__dadd_rn(g.zc, zcos), X[2 * i]);
+                double dz1 = __ddiv_rn(__dsub_rn(zzz, g.zlodz), g.xst);
+                const int nn = (int)dz1;
+                if (nn < 0 || nn > g.nz2) continue;
+                dz1 = __dsub_rn(dz1, (double)nn);
+                const double dz0 = __dsub_rn(1.0, dz1);
+                const double xx = __dadd_rn(-zsin, X[2 * i + 1]);
+                const double xx_cosZ = __dmul_rn(xx, R.cosZ);
+                const double xxx = __dadd_rn(__dadd_rn(g.xc, ysin), xx_cosZ);
+                double dx1 = __ddiv_rn(__dsub_rn(xxx, g.xlo), g.xst);
+                const int ii = (int)dx1;
+                if (ii < 0 || ii > g.nx2) continue;
+                dx1 = __dsub_rn(dx1, (double)ii);
+                const double dx0 = __dsub_rn(1.0, dx1);
+                const int lx = ii - x0, lz = nn - z0;
+                if (lx < -1 || lx >= DP_X || lz < -1 || lz >= DP_Z) continue;
+                const size_t po = (size_t)(g.escc ? n : 0) * n2 + i;
+                const double wx[2] = {dx0, dx1};
+                for (int h = 0; h < nj; h++) {
+                    const double wyv = s_wy[h];
+                    const double k = (double)Pab[po + (size_t)s_j[h] * g.nx];
+                    const double wz[2] = {dz0 * k, dz1 * k};
+#pragma unroll
+                    for (int c = 0; c < 4; c++) {
+                        const int cx = c & 1, cz = c >> 1;
+                        const int vx = lx + cx, vz = lz + cz;
+                        if ((unsigned)vx >= (unsigned)DP_X || (unsigned)vz >= (unsigned)DP_Z) continue;
+                        const double w = wx[cx] * wyv * wz[cz];
+                        atomicAdd(&acc[vx + DP_SX * vz], (unsigned long long)dround_i64(w * scale));
+                    }
+                }
+            }
+        }
+        __syncthreads();  // the lists are rewritten for the next planes
+        }
+    }
+    __syncthreads();
+    const double inv = ldexp(1.0, -sc);
+    for (int v = tid; v < DP_VOX; v += 256) {
+        const int lx = v % DP_X, lz = v / DP_X;
+        const int vx = x0 + lx, vz = z0 + lz;
+        if (vx >= g.nx || vz >= g.nz) continue;
+        const size_t idx = ((size_t)vz * g.ny + J) * g.nx + vx;
+        acc3[idx] += (double)(long long)acc[lx + DP_SX * lz] * inv;
+    }
+}
+
+}  // namespace utr
+
 struct dht_ctx {
     int method = 3;
     utr_ctx* sctf = nullptr;  // method 5
@@ -379,6 +586,7 @@ struct dht_ctx {
     DevBuf<cplx<float>> spec, T, twx, twy;
     DevBuf<float> P, stage, vol;
     DevBuf<unsigned> pmax;
+    int all_z0 = 0;  // no orientation rotates about Z (y-plane splat applies)
 };
 
 namespace {
@@ -570,6 +778,11 @@ int dht_run(dht_ctx* c, float* d_vol) {
     utr::k_dht_fill<<<(unsigned)((N + 255) / 256), 256, 0, c->stream>>>(c->acc3.p, N);
     const long nbricks = (long)((g.nx + utr::DHT_BX - 1) / utr::DHT_BX) *
                          ((g.ny + utr::DHT_BY - 1) / utr::DHT_BY) * ((g.nz + utr::DHT_BZ - 1) / utr::DHT_BZ);
+    // y-plane tiles when no orientation rotates about Z (UTR_GRID_FORM=brick
+    // selects the bricks; both form the same sums)
+    const char* form = std::getenv("UTR_GRID_FORM");
+    const bool plane = c->all_z0 && !(form && std::strcmp(form, "brick") == 0);
+    const long ntiles = (long)((g.nx + utr::DP_X - 1) / utr::DP_X) * ((g.nz + utr::DP_Z - 1) / utr::DP_Z);
     for (int a0 = 0; a0 < c->nrows; a0 += c->batch) {
         const int na = std::min(c->batch, c->nrows - a0);
         utr::k_dht_zero_u32<<<1, 64, 0, c->stream>>>(c->pmax.p);
@@ -595,8 +808,13 @@ int dht_run(dht_ctx* c, float* d_vol) {
         UTR_DISPATCH_N(g.ny, DHT_PC)
 #undef DHT_PC
         if (!ok) return set_err(UTR_E_UNSUPPORTED, "unsupported image height for the FFT");
-        utr::k_dht_splat<<<(unsigned)nbricks, 256, 0, c->stream>>>(c->P.p, g, c->rows.p, c->xs.p, c->ys.p,
-                                                                  c->zs.p, a0, na, c->pmax.p, c->acc3.p);
+        if (plane) {
+            utr::k_dht_splat_yp<<<utr::xcd_grid((unsigned)(ntiles * g.ny)), 256, 0, c->stream>>>(
+                c->P.p, g, c->rows.p, c->xs.p, c->ys.p, c->zs.p, a0, na, c->pmax.p, c->acc3.p, (int)ntiles);
+        } else {
+            utr::k_dht_splat<<<(unsigned)nbricks, 256, 0, c->stream>>>(c->P.p, g, c->rows.p, c->xs.p, c->ys.p,
+                                                                      c->zs.p, a0, na, c->pmax.p, c->acc3.p);
+        }
         UTR_HIP_TRY(hipGetLastError());
     }
     UTR_HIP_TRY(hipEventRecord(c->ev[3], c->stream));
@@ -680,7 +898,9 @@ int dht_create(const dht_params* p, int nrows, const double* angZ_deg, const dou
     // per-orientation rows and tables
     std::vector<utr::DhtRow> rows(nrows);
     std::vector<double> xs((size_t)nrows * 2 * g.nx), ys((size_t)nrows * 2 * g.ny), zs((size_t)nrows * 2 * g.nz);
+    c->all_z0 = 1;
     for (int r = 0; r < nrows; r++) {
+        if (angZ_deg[r] != 0.0 || g.nz > 65535) c->all_z0 = 0;  // plane numbers are listed as u16
         std::memset(&rows[r], 0, sizeof rows[r]);
         dht_tables(g, angZ_deg[r], angY_deg[r], xs.data() + (size_t)r * 2 * g.nx,
                    ys.data() + (size_t)r * 2 * g.ny, zs.data() + (size_t)r * 2 * g.nz, rows[r]);
