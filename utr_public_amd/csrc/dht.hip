@@ -143,7 +143,7 @@ __global__ void __launch_bounds__(256) k_dht_prop_rows(const cplx<float>* __rest
         float fr = 0.f, fi = 0.f;
         if (!(g.k2maxo > 0) || q2 < g.k2maxo) {
             double chi = f2 * q2 + f4 * q2 * q2 + f6 * q2 * q2 * q2;
-            chi -= twopi * rint(chi / twopi);  // FP64 reduction, then FP32 sincos
+            chi -= twopi * rint(chi * (1.0 / twopi));  // FP64 reduction, then FP32 sincos
             float s, c;
             sincosf((float)chi, &s, &c);
             if (g.method == 3) {
@@ -366,13 +366,16 @@ namespace utr {
 // coordinates do not depend on its source line j, and its y coordinate
 // depends on j alone: source line j reaches planes jj(j) and jj(j) + 1 with
 // the weights dy0(j), dy1(j).  Each workgroup owns a DP_X x DP_Z (x, z) tile
-// of one plane J, takes from each orientation the source lines j with
-// jj(j) in {J - 1, J}, clips every source plane n's i-line to the tile and
-// deals the listed samples to its threads in equal runs.  Coordinates,
-// weights and fixed-point terms are those of k_dht_splat expression by
-// expression (the j-dependent terms it adds are zeros here), so both kernels
-// form the same sums.
+// of DP_G consecutive planes J0 .. J0 + DP_G - 1, takes from each orientation
+// the source lines j with jj(j) in J0 - 1 .. J0 + DP_G - 1, clips every
+// source plane n's i-line to the tile and deals the listed samples to its
+// threads in equal runs; a sample's x and z coordinates are formed once for
+// all its lines.  Coordinates, weights and fixed-point terms are those of
+// k_dht_splat expression by expression (the j-dependent terms it adds are
+// zeros here), so both kernels form the same sums.
 constexpr int DP_X = 32, DP_Z = 32, DP_SX = DP_X + 1, DP_PAD = DP_SX * DP_Z, DP_VOX = DP_X * DP_Z;
+constexpr int DP_G = 4;            // planes per workgroup
+constexpr int DP_NC = DP_G + 3;    // candidate source lines J0 - 2 .. J0 + DP_G
 
 __global__ void __launch_bounds__(256) k_dht_splat_yp(const float* __restrict__ P, DhtGeom g,
                                                       const DhtRow* __restrict__ rows,
@@ -381,20 +384,21 @@ __global__ void __launch_bounds__(256) k_dht_splat_yp(const float* __restrict__ 
                                                       const double* __restrict__ zs, int a0, int na,
                                                       const unsigned* __restrict__ pmax,
                                                       double* __restrict__ acc3, int ntiles) {
-    __shared__ unsigned long long acc[DP_PAD];
+    __shared__ unsigned long long acc[DP_G * DP_PAD];
     __shared__ unsigned short s_n[256];   // listed source planes n
     __shared__ short s_i0[256];           // first i of each listed line
     __shared__ int s_pre[257];            // sample counts, then their prefix
     __shared__ int s_cnt[4], s_wsum[4];
-    __shared__ int s_j[4];                // source lines reaching plane J
-    __shared__ double s_wy[4];            // their y weights
-    __shared__ int s_nj;                  // how many
+    __shared__ int s_lp[DP_NC];           // candidate line's lower plane jj - J0 (-2: none)
+    __shared__ double s_dy1[DP_NC];       // its dy1
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    int tile, J;
-    if (!xcd_tile(ntiles, g.ny, tile, J)) return;
+    const int ngy = (g.ny + DP_G - 1) / DP_G;
+    int tile, Jg;
+    if (!xcd_tile(ntiles, ngy, tile, Jg)) return;
+    const int J0 = Jg * DP_G;
     const int ntx = (g.nx + DP_X - 1) / DP_X;
     const int x0 = (tile % ntx) * DP_X, z0 = (tile / ntx) * DP_Z;
-    for (int t = tid; t < DP_PAD; t += 256) acc[t] = 0ull;
+    for (int t = tid; t < DP_G * DP_PAD; t += 256) acc[t] = 0ull;
     // the scale of k_dht_splat
     int ek = 0, ea = 0;
     const float mk = __uint_as_float(*pmax);
@@ -410,31 +414,30 @@ __global__ void __launch_bounds__(256) k_dht_splat_yp(const float* __restrict__ 
         const double* Y = ys + (size_t)(a0 + ab) * 2 * g.ny;
         const double* Z = zs + (size_t)(a0 + ab) * 2 * g.nz;
         __syncthreads();  // previous orientation's lists are consumed
-        if (tid == 0) {
+        if (tid < DP_NC) {
             // y of source line j as DHT.cpp:1143-1148 forms it (xx sinZ = 0).
             // Rounding puts line j in cell j or j - 1, so the lines reaching
-            // plane J lie in J - 2 .. J + 1; a line whose lower cell is J
-            // adds with weight dy0, one whose lower cell is J - 1 with dy1.
-            int cnt = 0;
-            for (int j = max(J - 2, 0); j <= min(J + 1, g.ny - 1); j++) {
+            // planes J0 .. J0 + DP_G - 1 lie in J0 - 2 .. J0 + DP_G.
+            const int j = J0 - 2 + tid;
+            int lp = -2;
+            double dy1 = 0.0;
+            if (j >= 0 && j < g.ny) {
                 const double yyy = __dsub_rn(__dadd_rn(g.yc, Y[2 * j + 1]), __dmul_rn(0.0, R.sinZ));
-                double dy1 = __ddiv_rn(__dsub_rn(yyy, g.ylo), g.xst);
+                dy1 = __ddiv_rn(__dsub_rn(yyy, g.ylo), g.xst);
                 const int jj = (int)dy1;
-                if (jj < 0 || jj > g.ny2) continue;
-                dy1 = __dsub_rn(dy1, (double)jj);
-                if (jj == J) {
-                    s_j[cnt] = j;
-                    s_wy[cnt++] = __dsub_rn(1.0, dy1);
-                } else if (jj == J - 1) {
-                    s_j[cnt] = j;
-                    s_wy[cnt++] = dy1;
+                if (jj >= 0 && jj <= g.ny2 && jj >= J0 - 1 && jj < J0 + DP_G) {
+                    dy1 = __dsub_rn(dy1, (double)jj);
+                    lp = jj - J0;
                 }
             }
-            s_nj = cnt;
+            s_lp[tid] = lp;
+            s_dy1[tid] = dy1;
         }
-        __syncthreads();  // s_j, s_wy, s_nj written
-        const int nj = s_nj;
-        if (nj == 0) continue;  // uniform: no source line reaches plane J
+        __syncthreads();  // s_lp, s_dy1 written
+        int jmask = 0;
+#pragma unroll
+        for (int h = 0; h < DP_NC; h++) jmask |= (s_lp[h] > -2) << h;
+        if (jmask == 0) continue;  // uniform: no source line reaches the planes
         // clip the source planes' i-lines to the tile (one sample of margin),
         // 256 planes at a time
         const float bl[2] = {(float)x0 - 1.f, (float)z0 - 1.f};
@@ -495,7 +498,8 @@ __global__ void __launch_bounds__(256) k_dht_splat_yp(const float* __restrict__ 
         if (tid == 0) s_pre[nl] = total;
         __syncthreads();
         const float* Pab = P + (size_t)ab * g.nplanes * n2;
-        const double ysin = Y[2 * s_j[0]];  // y_j sinZ: a zero of either sign, same sum
+        const int jfirst = __builtin_ctz(jmask);
+        const double ysin = Y[2 * (J0 - 2 + jfirst)];  // y_j sinZ: a zero of either sign, same sum
         const int p0 = (int)(((long long)tid * total) >> 8);
         const int p1 = (int)(((long long)(tid + 1) * total) >> 8);
         if (p0 < p1) {
@@ -535,17 +539,27 @@ __global__ void __launch_bounds__(256) k_dht_splat_yp(const float* __restrict__ 
                 if (lx < -1 || lx >= DP_X || lz < -1 || lz >= DP_Z) continue;
                 const size_t po = (size_t)(g.escc ? n : 0) * n2 + i;
                 const double wx[2] = {dx0, dx1};
-                for (int h = 0; h < nj; h++) {
-                    const double wyv = s_wy[h];
-                    const double k = (double)Pab[po + (size_t)s_j[h] * g.nx];
+#pragma unroll
+                for (int h = 0; h < DP_NC; h++) {
+                    if (!((jmask >> h) & 1)) continue;
+                    const int lp = s_lp[h];
+                    const double dy1 = s_dy1[h];
+                    const double k = (double)Pab[po + (size_t)(J0 - 2 + h) * g.nx];  // line j >= 0
                     const double wz[2] = {dz0 * k, dz1 * k};
 #pragma unroll
-                    for (int c = 0; c < 4; c++) {
-                        const int cx = c & 1, cz = c >> 1;
-                        const int vx = lx + cx, vz = lz + cz;
-                        if ((unsigned)vx >= (unsigned)DP_X || (unsigned)vz >= (unsigned)DP_Z) continue;
-                        const double w = wx[cx] * wyv * wz[cz];
-                        atomicAdd(&acc[vx + DP_SX * vz], (unsigned long long)dround_i64(w * scale));
+                    for (int cy = 0; cy < 2; cy++) {
+                        const int pl = lp + cy;  // plane J0 + pl takes weight dy0 (cy 0) or dy1
+                        if (pl < 0 || pl >= DP_G) continue;
+                        const double wyv = cy ? dy1 : __dsub_rn(1.0, dy1);
+                        unsigned long long* ap = acc + pl * DP_PAD;
+#pragma unroll
+                        for (int c = 0; c < 4; c++) {
+                            const int cx = c & 1, cz = c >> 1;
+                            const int vx = lx + cx, vz = lz + cz;
+                            if ((unsigned)vx >= (unsigned)DP_X || (unsigned)vz >= (unsigned)DP_Z) continue;
+                            const double w = wx[cx] * wyv * wz[cz];
+                            atomicAdd(&ap[vx + DP_SX * vz], (unsigned long long)dround_i64(w * scale));
+                        }
                     }
                 }
             }
@@ -555,12 +569,13 @@ __global__ void __launch_bounds__(256) k_dht_splat_yp(const float* __restrict__ 
     }
     __syncthreads();
     const double inv = ldexp(1.0, -sc);
-    for (int v = tid; v < DP_VOX; v += 256) {
-        const int lx = v % DP_X, lz = v / DP_X;
-        const int vx = x0 + lx, vz = z0 + lz;
-        if (vx >= g.nx || vz >= g.nz) continue;
+    for (int v = tid; v < DP_G * DP_VOX; v += 256) {
+        const int pl = v / DP_VOX, r = v - pl * DP_VOX;
+        const int lx = r % DP_X, lz = r / DP_X;
+        const int vx = x0 + lx, vz = z0 + lz, J = J0 + pl;
+        if (vx >= g.nx || vz >= g.nz || J >= g.ny) continue;
         const size_t idx = ((size_t)vz * g.ny + J) * g.nx + vx;
-        acc3[idx] += (double)(long long)acc[lx + DP_SX * lz] * inv;
+        acc3[idx] += (double)(long long)acc[pl * DP_PAD + lx + DP_SX * lz] * inv;
     }
 }
 
@@ -809,7 +824,8 @@ int dht_run(dht_ctx* c, float* d_vol) {
 #undef DHT_PC
         if (!ok) return set_err(UTR_E_UNSUPPORTED, "unsupported image height for the FFT");
         if (plane) {
-            utr::k_dht_splat_yp<<<utr::xcd_grid((unsigned)(ntiles * g.ny)), 256, 0, c->stream>>>(
+            utr::k_dht_splat_yp<<<utr::xcd_grid((unsigned)(ntiles * ((g.ny + utr::DP_G - 1) / utr::DP_G))), 256, 0,
+                                  c->stream>>>(
                 c->P.p, g, c->rows.p, c->xs.p, c->ys.p, c->zs.p, a0, na, c->pmax.p, c->acc3.p, (int)ntiles);
         } else {
             utr::k_dht_splat<<<(unsigned)nbricks, 256, 0, c->stream>>>(c->P.p, g, c->rows.p, c->xs.p, c->ys.p,
