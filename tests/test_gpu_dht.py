@@ -60,6 +60,11 @@ def test_method3_aperture_and_aberrations():
                 cs5_mm=2.0, seed=3))
 
 
+def test_method4_aperture_and_aberrations():
+    check(*case(method=4, escc=1, dzextra=0.0, asigma=0.06, epsilon=0.02, aobj_mrad=20.0, cs3_mm=1.2,
+                cs5_mm=2.0, seed=8))
+
+
 @pytest.mark.parametrize("escc", [1, 0])
 def test_method4_vctf(escc):
     check(*case(method=4, escc=escc, dzextra=0.0, asigma=0.06, epsilon=0.02, seed=4))

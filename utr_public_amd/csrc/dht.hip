@@ -29,6 +29,7 @@ struct DhtGeom {
     double xst, wl, k2maxo, c3, c5, asigma, epsilon, zlodz;
     double xlo, ylo, xc, yc, zc;
     int nx2, ny2, nz2;
+    int use_ap;  // aperture or Cs terms present: filters read the table ap
 };
 
 // One orientation of the DHT path.
@@ -115,95 +116,174 @@ __global__ void __launch_bounds__(256) k_dht_cols(cplx<float>* __restrict__ data
 
 // Per output plane p of orientation a0+blockIdx.z: spectrum row times the
 // plane's filter, inverse FFT along x.  grid (rows/NB, nplanes, na).
+// The phase chi = f2 q^2 + f4 q^4 + f6 q^6 (DHT.cpp:962-1000) is split as
+// exp(-i chi) = ex(fx) ey(fy) ap(fx, fy): ex = exp(-i f2 fx^2) and
+// ey = exp(-i f2 fy^2) depend on the plane's defocus and are formed per
+// workgroup with FP64 phases; ap = exp(-i (f4 q^4 + f6 q^6)), zero outside
+// the objective aperture, is one table per job (read when g.use_ap).
 template <int N>
 __global__ void __launch_bounds__(256) k_dht_prop_rows(const cplx<float>* __restrict__ S,
                                                        cplx<float>* __restrict__ T, DhtGeom g,
                                                        const DhtRow* __restrict__ rows, int a0,
-                                                       const cplx<float>* __restrict__ tw) {
+                                                       const cplx<float>* __restrict__ tw,
+                                                       const cplx<float>* __restrict__ ap) {
     constexpr int NB = lds_batch(N, sizeof(cplx<float>), LDS_DHT);
+    constexpr int PER = (NB * N + 255) / 256;  // elements per thread
     __shared__ cplx<float> sm[2 * lds_len(NB * N)];
+    __shared__ cplx<float> s_ex[N], s_ey[NB];
     cplx<float>* a = sm;
     cplx<float>* b = sm + lds_len(NB * N);
     const int ny = g.ny;
     const int p = blockIdx.y, ab = blockIdx.z;
     const int k0 = blockIdx.x * NB;
     const int nb = min(NB, ny - k0);
+    const int tid = threadIdx.x;
     const DhtRow& R = rows[a0 + ab];
     const int n = g.escc ? p : g.nz / 2;
     const double zn = g.zlodz + g.xst * n;  // voutdefocus[n], DHT.cpp:388
     const double d = g.method == 3 ? zn - R.zout : R.zout - zn;
     const double pi = 3.14159265358979323846, twopi = 2.0 * pi;
-    const double f2 = pi * g.wl * d, f4 = pi * g.wl * g.wl * g.wl * g.c3 / 2.0,
-                 f6 = pi * pow(g.wl, 5) * g.c5 / 3.0;
+    const double f2 = pi * g.wl * d;
     const cplx<float>* src = S + ((size_t)(a0 + ab) * ny + k0) * N;
-    for (int t = threadIdx.x; t < nb * N; t += blockDim.x) {
-        const int ky = k0 + t / N, kx = t % N;
-        const double fx = dht_freq(kx, N, g.xst), fy = dht_freq(ky, ny, g.xst);
-        const double q2 = fx * fx + fy * fy;
-        float fr = 0.f, fi = 0.f;
-        if (!(g.k2maxo > 0) || q2 < g.k2maxo) {
-            double chi = f2 * q2 + f4 * q2 * q2 + f6 * q2 * q2 * q2;
-            chi -= twopi * rint(chi * (1.0 / twopi));  // FP64 reduction, then FP32 sincos
-            float s, c;
-            sincosf((float)chi, &s, &c);
-            if (g.method == 3) {
-                fr = c;
-                fi = -s;
-            } else {
-                const float tt = s - (float)g.asigma * c;
-                fr = tt / (2.f * (tt * tt + (float)g.epsilon));
-            }
+    const cplx<float>* aps = ap + (size_t)k0 * N;
+    cplx<float> v[PER], h[PER];
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const int t = tid + 256 * q;
+        if (t < nb * N) {
+            v[q] = src[t];
+            h[q] = g.use_ap ? aps[t] : cplx<float>{1.f, 0.f};
         }
-        const cplx<float> v = src[t];
-        a[lx(t)] = {v.x * fr - v.y * fi, v.x * fi + v.y * fr};
+    }
+    for (int k = tid; k < N + nb; k += 256) {
+        const double f = k < N ? dht_freq(k, N, g.xst) : dht_freq(k0 + k - N, ny, g.xst);
+        double chi = f2 * (f * f);
+        chi -= twopi * rint(chi * (1.0 / twopi));
+        double sn, cs;
+        sincos(chi, &sn, &cs);
+        const cplx<float> e = {(float)cs, (float)-sn};
+        if (k < N) s_ex[k] = e;
+        else s_ey[k - N] = e;
     }
     __syncthreads();
-    fft_lds<float, N>(a, b, nb, tw, +1, threadIdx.x, blockDim.x);
+#pragma unroll
+    for (int q = 0; q < PER; q++) {
+        const int t = tid + 256 * q;
+        if (t < nb * N) {
+            const int r = t / N, kx = t - r * N;
+            const cplx<float> H = cmul(cmul(s_ex[kx], s_ey[r]), h[q]);  // cos chi - i sin chi
+            float fr = 0.f, fi = 0.f;
+            if (g.method == 3) {
+                fr = H.x;
+                fi = H.y;
+            } else if (h[q].x != 0.f || h[q].y != 0.f) {  // |ap| = 1 inside the aperture, 0 outside
+                const float tt = -H.y - (float)g.asigma * H.x;  // sin chi - asigma cos chi
+                fr = tt / (2.f * (tt * tt + (float)g.epsilon));
+            }
+            a[lx(t)] = {v[q].x * fr - v[q].y * fi, v[q].x * fi + v[q].y * fr};
+        }
+    }
+    __syncthreads();
+    fft_lds<float, N>(a, b, nb, tw, +1, tid, blockDim.x);
     cplx<float>* dst = T + (((size_t)ab * g.nplanes + p) * ny + k0) * N;
-    for (int t = threadIdx.x; t < nb * N; t += blockDim.x) dst[t] = a[lx(t)];
+    for (int t = tid; t < nb * N; t += blockDim.x) dst[t] = a[lx(t)];
 }
 
 // Inverse FFT along y of each plane, then 1 - |psi|^2 = -2 Re u - |u|^2
-// (method 3) or the real part (method 4), times the orientation's weight.  grid (nx/NB, nplanes, na).
+// (method 3) or the real part (method 4), times the orientation's weight.
+// Tile w = NB adjacent columns of one plane: column group w % gx, plane
+// w / gx of the batch (orientation-major).  Persistent: in round k this
+// workgroup takes tile k G + (L & 7) G/8 + L/8, so the tiles of one XCD in a
+// round are contiguous and neighbouring column groups share its L2; the
+// next tile's columns are loaded into registers while the current one is
+// transformed.  gridDim.x is a multiple of 8.
 // pmax: max |value| of the batch as float bits.
 template <int N>
 __global__ void __launch_bounds__(256) k_dht_prop_cols(const cplx<float>* __restrict__ T,
                                                        float* __restrict__ P, DhtGeom g,
                                                        const DhtRow* __restrict__ rows, int a0,
                                                        const cplx<float>* __restrict__ tw,
-                                                       unsigned* __restrict__ pmax) {
+                                                       unsigned* __restrict__ pmax, int gx, int total) {
     constexpr int NB = lds_batch(N, sizeof(cplx<float>), LDS_DHT);
+    constexpr int PER = (NB * N + 255) / 256;  // elements per thread
     __shared__ cplx<float> sm[2 * lds_len(NB * N)];
     __shared__ float s_wmax[4];
-    cplx<float>* a = sm;
-    cplx<float>* b = sm + lds_len(NB * N);
     const int nx = g.nx;
-    const int p = blockIdx.y, ab = blockIdx.z;
-    const int x0 = blockIdx.x * NB;
-    const int nb = min(NB, nx - x0);
-    const size_t plane = ((size_t)ab * g.nplanes + p) * N * nx;
-    const cplx<float>* base = T + plane + x0;
-    for (int t = threadIdx.x; t < nb * N; t += blockDim.x) {
-        const int y = t / nb, c = t - y * nb;
-        a[lx(c * N + y)] = base[(size_t)y * nx + c];
-    }
-    __syncthreads();
-    fft_lds<float, N>(a, b, nb, tw, +1, threadIdx.x, blockDim.x);
-    const float inv = (float)(1.0 / ((double)nx * N));
-    const float w = rows[a0 + ab].weight;
+    const int tid = threadIdx.x;
+    const unsigned L = blockIdx.x, G = gridDim.x;
+    auto tile_of = [&](int k) { return (int)(k * G + (L & 7u) * (G >> 3) + (L >> 3)); };
+    // element t of a tile is (row t / cols, column t % cols); a full tile
+    // (the common case) divides by the constant NB
+    cplx<float> vn[PER];
+    auto fetch_cols = [&](const cplx<float>* base, const int cols) {
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int t = tid + 256 * q;
+            if (t < cols * N) {
+                const int y = t / cols, c = t - y * cols;
+                vn[q] = base[(size_t)y * nx + c];
+            }
+        }
+    };
+    auto fetch = [&](int w) {
+        const int pa = w / gx, x0 = (w - pa * gx) * NB, nb = min(NB, nx - x0);
+        const cplx<float>* base = T + (size_t)pa * N * nx + x0;
+        if (nb == NB) fetch_cols(base, NB);
+        else fetch_cols(base, nb);
+    };
     float vmax = 0.f;
-    for (int t = threadIdx.x; t < nb * N; t += blockDim.x) {
-        const int y = t / nb, c = t - y * nb;
-        const cplx<float> z = a[lx(c * N + y)];
-        const float re = z.x * inv, im = z.y * inv;
-        const float v = (g.method == 3 ? -2.f * re - (re * re + im * im) : re) * w;
-        P[plane + (size_t)y * nx + x0 + c] = v;
-        vmax = fmaxf(vmax, fabsf(v));
+    const int rounds = (total + (int)G - 1) / (int)G;
+    if (tile_of(0) < total) fetch(tile_of(0));
+    for (int rk = 0; rk < rounds; rk++) {
+        const int wt = tile_of(rk);
+        cplx<float> vc[PER];
+#pragma unroll
+        for (int q = 0; q < PER; q++) vc[q] = vn[q];
+        if (rk + 1 < rounds && tile_of(rk + 1) < total) fetch(tile_of(rk + 1));
+        if (wt >= total) continue;  // uniform over the workgroup
+        cplx<float>* a = sm;
+        cplx<float>* b = sm + lds_len(NB * N);
+        const int pa = wt / gx, x0 = (wt - pa * gx) * NB, nb = min(NB, nx - x0);
+        const int ab = pa / g.nplanes;
+        const size_t plane = (size_t)pa * N * nx;
+        auto put = [&](const int cols) {
+#pragma unroll
+            for (int q = 0; q < PER; q++) {
+                const int t = tid + 256 * q;
+                if (t < cols * N) {
+                    const int y = t / cols, c = t - y * cols;
+                    a[lx(c * N + y)] = vc[q];
+                }
+            }
+        };
+        if (nb == NB) put(NB);
+        else put(nb);
+        __syncthreads();
+        fft_lds<float, N>(a, b, nb, tw, +1, tid, blockDim.x);
+        const float inv = (float)(1.0 / ((double)nx * N));
+        const float w = rows[a0 + ab].weight;
+        auto store = [&](const int cols) {
+#pragma unroll
+            for (int q = 0; q < PER; q++) {
+                const int t = tid + 256 * q;
+                if (t < cols * N) {
+                    const int y = t / cols, c = t - y * cols;
+                    const cplx<float> z = a[lx(c * N + y)];
+                    const float re = z.x * inv, im = z.y * inv;
+                    const float v = (g.method == 3 ? -2.f * re - (re * re + im * im) : re) * w;
+                    P[plane + (size_t)y * nx + x0 + c] = v;
+                    vmax = fmaxf(vmax, fabsf(v));
+                }
+            }
+        };
+        if (nb == NB) store(NB);
+        else store(nb);
+        __syncthreads();  // the next tile overwrites both buffers
     }
     for (int off = 32; off > 0; off >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, off));
-    if ((threadIdx.x & 63) == 0) s_wmax[threadIdx.x >> 6] = vmax;
+    if ((tid & 63) == 0) s_wmax[tid >> 6] = vmax;
     __syncthreads();
-    if (threadIdx.x == 0)
+    if (tid == 0)
         atomicMax(pmax, __float_as_uint(fmaxf(fmaxf(s_wmax[0], s_wmax[1]), fmaxf(s_wmax[2], s_wmax[3]))));
 }
 
@@ -376,6 +456,7 @@ namespace utr {
 constexpr int DP_X = 32, DP_Z = 32, DP_SX = DP_X + 1, DP_PAD = DP_SX * DP_Z, DP_VOX = DP_X * DP_Z;
 constexpr int DP_G = 4;            // planes per workgroup
 constexpr int DP_NC = DP_G + 3;    // candidate source lines J0 - 2 .. J0 + DP_G
+constexpr int DP_NXMAX = 512;      // widest image (x table staged in LDS)
 
 __global__ void __launch_bounds__(256) k_dht_splat_yp(const float* __restrict__ P, DhtGeom g,
                                                       const DhtRow* __restrict__ rows,
@@ -385,7 +466,9 @@ __global__ void __launch_bounds__(256) k_dht_splat_yp(const float* __restrict__ 
                                                       const unsigned* __restrict__ pmax,
                                                       double* __restrict__ acc3, int ntiles) {
     __shared__ unsigned long long acc[DP_G * DP_PAD];
-    __shared__ unsigned short s_n[256];   // listed source planes n
+    __shared__ double s_X[2 * DP_NXMAX];  // the orientation's (x_i - xc) sinY, cosY
+    __shared__ double s_Z[2 * 256];       // (z_n - zc) sinY, cosY of the current 256 planes
+    __shared__ unsigned short s_n[256];   // listed source planes n (within the chunk)
     __shared__ short s_i0[256];           // first i of each listed line
     __shared__ int s_pre[257];            // sample counts, then their prefix
     __shared__ int s_cnt[4], s_wsum[4];
@@ -413,7 +496,8 @@ __global__ void __launch_bounds__(256) k_dht_splat_yp(const float* __restrict__ 
         const double* X = xs + (size_t)(a0 + ab) * 2 * g.nx;
         const double* Y = ys + (size_t)(a0 + ab) * 2 * g.ny;
         const double* Z = zs + (size_t)(a0 + ab) * 2 * g.nz;
-        __syncthreads();  // previous orientation's lists are consumed
+        __syncthreads();  // previous orientation's lists and tables are consumed
+        for (int t = tid; t < 2 * g.nx; t += 256) s_X[t] = X[t];
         if (tid < DP_NC) {
             // y of source line j as DHT.cpp:1143-1148 forms it (xx sinZ = 0).
             // Rounding puts line j in cell j or j - 1, so the lines reaching
@@ -433,11 +517,19 @@ __global__ void __launch_bounds__(256) k_dht_splat_yp(const float* __restrict__ 
             s_lp[tid] = lp;
             s_dy1[tid] = dy1;
         }
-        __syncthreads();  // s_lp, s_dy1 written
-        int jmask = 0;
+        __syncthreads();  // s_lp, s_dy1, s_X written
+        // the listed lines, compacted: h-th line j = J0 - 2 + hj[h]
+        int nvl = 0, hj[DP_NC];
 #pragma unroll
-        for (int h = 0; h < DP_NC; h++) jmask |= (s_lp[h] > -2) << h;
-        if (jmask == 0) continue;  // uniform: no source line reaches the planes
+        for (int h = 0; h < DP_NC; h++) {
+            hj[h] = 0;
+        }
+#pragma unroll
+        for (int h = 0; h < DP_NC; h++)
+            if (s_lp[h] > -2) hj[nvl++] = h;
+        if (nvl == 0) continue;  // uniform: no source line reaches the planes
+        const double ysin = Y[2 * (J0 - 2 + hj[0])];  // y_j sinZ: a zero of either sign, same sum
+        const float* Pab = P + (size_t)ab * g.nplanes * n2;
         // clip the source planes' i-lines to the tile (one sample of margin),
         // 256 planes at a time
         const float bl[2] = {(float)x0 - 1.f, (float)z0 - 1.f};
@@ -447,6 +539,8 @@ __global__ void __launch_bounds__(256) k_dht_splat_yp(const float* __restrict__ 
         int i0 = 0, i1 = -1;
         if (nb + tid < nsrc) {
             const int n = nb + tid;
+            s_Z[2 * tid] = Z[2 * n];
+            s_Z[2 * tid + 1] = Z[2 * n + 1];
             float ilo = 0.f, ihi = (float)(g.nx - 1);
             const int dd[2] = {0, 2};
 #pragma unroll
@@ -480,7 +574,7 @@ __global__ void __launch_bounds__(256) k_dht_splat_yp(const float* __restrict__ 
         for (int w = 0; w < wave; w++) off += s_cnt[w];
         const int npt = pass ? i1 - i0 + 1 : 0;
         if (pass) {
-            s_n[off + pre] = (unsigned short)(nb + tid);
+            s_n[off + pre] = (unsigned short)tid;
             s_i0[off + pre] = (short)i0;
         }
         int incl = npt;
@@ -496,10 +590,7 @@ __global__ void __launch_bounds__(256) k_dht_splat_yp(const float* __restrict__ 
         const int nl = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
         const int total = s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
         if (tid == 0) s_pre[nl] = total;
-        __syncthreads();
-        const float* Pab = P + (size_t)ab * g.nplanes * n2;
-        const int jfirst = __builtin_ctz(jmask);
-        const double ysin = Y[2 * (J0 - 2 + jfirst)];  // y_j sinZ: a zero of either sign, same sum
+        __syncthreads();  // lists and s_Z written
         const int p0 = (int)(((long long)tid * total) >> 8);
         const int p1 = (int)(((long long)(tid + 1) * total) >> 8);
         if (p0 < p1) {
@@ -510,24 +601,30 @@ __global__ void __launch_bounds__(256) k_dht_splat_yp(const float* __restrict__ 
                 else hi = mid - 1;
             }
             int q = lo;
-            int n = s_n[q], ibase = s_i0[q] - s_pre[q], pend = s_pre[q + 1];
+            int nc = s_n[q], ibase = s_i0[q] - s_pre[q], pend = s_pre[q + 1];
             for (int p = p0; p < p1; p++) {
                 if (p == pend) {
                     q++;
-                    n = s_n[q];
+                    nc = s_n[q];
                     ibase = s_i0[q] - s_pre[q];
                     pend = s_pre[q + 1];
                 }
-                const int i = ibase + p;
+                const int i = ibase + p, n = nb + nc;
+                // the source values of all listed lines first (one wait)
+                const size_t po = (size_t)(g.escc ? n : 0) * n2 + i;
+                float kv[DP_NC];
+#pragma unroll
+                for (int h = 0; h < DP_NC; h++)
+                    kv[h] = Pab[po + (size_t)(J0 - 2 + hj[h < nvl ? h : 0]) * g.nx];  // line j >= 0
                 // DHT.cpp:1119-1150, operation by operation
-                const double zcos = Z[2 * n + 1], zsin = Z[2 * n];
-                const double zzz = __dadd_rn(__dadd_rn(g.zc, zcos), X[2 * i]);
+                const double zcos = s_Z[2 * nc + 1], zsin = s_Z[2 * nc];
+                const double zzz = __dadd_rn(__dadd_rn(g.zc, zcos), s_X[2 * i]);
                 double dz1 = __ddiv_rn(__dsub_rn(zzz, g.zlodz), g.xst);
                 const int nn = (int)dz1;
                 if (nn < 0 || nn > g.nz2) continue;
                 dz1 = __dsub_rn(dz1, (double)nn);
                 const double dz0 = __dsub_rn(1.0, dz1);
-                const double xx = __dadd_rn(-zsin, X[2 * i + 1]);
+                const double xx = __dadd_rn(-zsin, s_X[2 * i + 1]);
                 const double xx_cosZ = __dmul_rn(xx, R.cosZ);
                 const double xxx = __dadd_rn(__dadd_rn(g.xc, ysin), xx_cosZ);
                 double dx1 = __ddiv_rn(__dsub_rn(xxx, g.xlo), g.xst);
@@ -537,14 +634,13 @@ __global__ void __launch_bounds__(256) k_dht_splat_yp(const float* __restrict__ 
                 const double dx0 = __dsub_rn(1.0, dx1);
                 const int lx = ii - x0, lz = nn - z0;
                 if (lx < -1 || lx >= DP_X || lz < -1 || lz >= DP_Z) continue;
-                const size_t po = (size_t)(g.escc ? n : 0) * n2 + i;
                 const double wx[2] = {dx0, dx1};
 #pragma unroll
                 for (int h = 0; h < DP_NC; h++) {
-                    if (!((jmask >> h) & 1)) continue;
-                    const int lp = s_lp[h];
-                    const double dy1 = s_dy1[h];
-                    const double k = (double)Pab[po + (size_t)(J0 - 2 + h) * g.nx];  // line j >= 0
+                    if (h >= nvl) break;
+                    const int lp = s_lp[hj[h]];
+                    const double dy1 = s_dy1[hj[h]];
+                    const double k = (double)kv[h];
                     const double wz[2] = {dz0 * k, dz1 * k};
 #pragma unroll
                     for (int cy = 0; cy < 2; cy++) {
@@ -564,7 +660,7 @@ __global__ void __launch_bounds__(256) k_dht_splat_yp(const float* __restrict__ 
                 }
             }
         }
-        __syncthreads();  // the lists are rewritten for the next planes
+        __syncthreads();  // the lists and s_Z are rewritten for the next planes
         }
     }
     __syncthreads();
@@ -590,6 +686,7 @@ struct dht_ctx {
     int batch = 1;
     double dnorm = 0;
     int device = 0;
+    int ncu = 256;  // compute units (persistent grids)
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {};
     std::vector<char> added;
@@ -598,7 +695,7 @@ struct dht_ctx {
     utr_timings tim = {};
     DevBuf<utr::DhtRow> rows;
     DevBuf<double> xs, ys, zs, acc3;
-    DevBuf<cplx<float>> spec, T, twx, twy;
+    DevBuf<cplx<float>> spec, T, twx, twy, ap;
     DevBuf<float> P, stage, vol;
     DevBuf<unsigned> pmax;
     int all_z0 = 0;  // no orientation rotates about Z (y-plane splat applies)
@@ -801,27 +898,39 @@ int dht_run(dht_ctx* c, float* d_vol) {
     for (int a0 = 0; a0 < c->nrows; a0 += c->batch) {
         const int na = std::min(c->batch, c->nrows - a0);
         utr::k_dht_zero_u32<<<1, 64, 0, c->stream>>>(c->pmax.p);
+        // per orientation: its plane stack's row pass into T, then the column
+        // pass into its slot of P; T (one orientation) stays in the
+        // memory-side cache between the two passes
         bool ok = false;
+        for (int ab = 0; ab < na; ab++) {
+            ok = false;
 #define DHT_PR(N)                                                                                \
     {                                                                                            \
-        constexpr int NB = utr::lds_batch(N, sizeof(cplx<float>), utr::LDS_DHT);                               \
-        utr::k_dht_prop_rows<N><<<dim3((g.ny + NB - 1) / NB, g.nplanes, na), 256, 0, c->stream>>>( \
-            c->spec.p, c->T.p, g, c->rows.p, a0, c->twx.p);                                      \
+        constexpr int NB = utr::lds_batch(N, sizeof(cplx<float>), utr::LDS_DHT);                 \
+        utr::k_dht_prop_rows<N><<<dim3((g.ny + NB - 1) / NB, g.nplanes, 1), 256, 0, c->stream>>>( \
+            c->spec.p, c->T.p, g, c->rows.p, a0 + ab, c->twx.p, c->ap.p);                        \
         ok = true;                                                                               \
     }
-        UTR_DISPATCH_N(g.nx, DHT_PR)
+            UTR_DISPATCH_N(g.nx, DHT_PR)
 #undef DHT_PR
-        if (!ok) return set_err(UTR_E_UNSUPPORTED, "unsupported image width for the FFT");
-        ok = false;
+            if (!ok) return set_err(UTR_E_UNSUPPORTED, "unsupported image width for the FFT");
+            ok = false;
 #define DHT_PC(N)                                                                                \
     {                                                                                            \
-        constexpr int NB = utr::lds_batch(N, sizeof(cplx<float>), utr::LDS_DHT);                               \
-        utr::k_dht_prop_cols<N><<<dim3((g.nx + NB - 1) / NB, g.nplanes, na), 256, 0, c->stream>>>( \
-            c->T.p, c->P.p, g, c->rows.p, a0, c->twy.p, c->pmax.p);                              \
+        constexpr int NB = utr::lds_batch(N, sizeof(cplx<float>), utr::LDS_DHT);                 \
+        constexpr int lds = 2 * utr::lds_len(NB * N) * (int)sizeof(cplx<float>);                \
+        const int gx = (g.nx + NB - 1) / NB, total = gx * g.nplanes;                             \
+        const unsigned cap = (unsigned)(c->ncu * std::max(1, std::min(8, 163840 / lds))) & ~7u;  \
+        const unsigned grid = std::min(utr::xcd_grid((unsigned)total), cap ? cap : 8u);          \
+        utr::k_dht_prop_cols<N><<<grid, 256, 0, c->stream>>>(                                    \
+            c->T.p, c->P.p + (size_t)ab * g.nplanes * g.nx * g.ny, g, c->rows.p, a0 + ab,        \
+            c->twy.p, c->pmax.p, gx, total);                                                     \
         ok = true;                                                                               \
     }
-        UTR_DISPATCH_N(g.ny, DHT_PC)
+            UTR_DISPATCH_N(g.ny, DHT_PC)
 #undef DHT_PC
+            if (!ok) break;
+        }
         if (!ok) return set_err(UTR_E_UNSUPPORTED, "unsupported image height for the FFT");
         if (plane) {
             utr::k_dht_splat_yp<<<utr::xcd_grid((unsigned)(ntiles * ((g.ny + utr::DP_G - 1) / utr::DP_G))), 256, 0,
@@ -916,7 +1025,7 @@ int dht_create(const dht_params* p, int nrows, const double* angZ_deg, const dou
     std::vector<double> xs((size_t)nrows * 2 * g.nx), ys((size_t)nrows * 2 * g.ny), zs((size_t)nrows * 2 * g.nz);
     c->all_z0 = 1;
     for (int r = 0; r < nrows; r++) {
-        if (angZ_deg[r] != 0.0 || g.nz > 65535) c->all_z0 = 0;  // plane numbers are listed as u16
+        if (angZ_deg[r] != 0.0 || g.nx > utr::DP_NXMAX) c->all_z0 = 0;  // x table staged in LDS
         std::memset(&rows[r], 0, sizeof rows[r]);
         dht_tables(g, angZ_deg[r], angY_deg[r], xs.data() + (size_t)r * 2 * g.nx,
                    ys.data() + (size_t)r * 2 * g.ny, zs.data() + (size_t)r * 2 * g.nz, rows[r]);
@@ -924,24 +1033,52 @@ int dht_create(const dht_params* p, int nrows, const double* angZ_deg, const dou
         rows[r].weight = weights ? weights[r] : 1.0f;
     }
     const size_t n2 = (size_t)g.nx * g.ny, N = n2 * g.nz;
-    const size_t per_angle = (size_t)g.nplanes * n2 * (sizeof(cplx<float>) + sizeof(float));
+    const size_t per_angle = (size_t)g.nplanes * n2 * sizeof(float);  // its plane stack in P
     c->batch = (int)std::max<size_t>(1, std::min<size_t>(std::min<size_t>((size_t)nrows, 65535),
                                                          dht_batch_bytes() / per_angle));
     if (g.nplanes > 65535) return fail(set_err(UTR_E_UNSUPPORTED, "too many output planes"));
     if (hipGetDevice(&c->device) != hipSuccess) return fail(set_err(UTR_E_DEVICE, "no HIP device"));
+    if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess ||
+        c->ncu <= 0)
+        c->ncu = 256;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return fail(set_err(UTR_E_DEVICE, "hipStreamCreate failed"));
     for (auto& e : c->ev)
         if (hipEventCreate(&e) != hipSuccess) return fail(set_err(UTR_E_DEVICE, "hipEventCreate failed"));
     if (c->rows.alloc(nrows) || c->xs.alloc(xs.size()) || c->ys.alloc(ys.size()) || c->zs.alloc(zs.size()) ||
         c->acc3.alloc(N) || c->spec.alloc((size_t)nrows * n2) || c->pmax.alloc(1) ||
-        c->T.alloc((size_t)c->batch * g.nplanes * n2) || c->P.alloc((size_t)c->batch * g.nplanes * n2))
+        c->T.alloc((size_t)g.nplanes * n2) || c->P.alloc((size_t)c->batch * g.nplanes * n2))
         return fail(set_err(UTR_E_NOMEM, "device allocation of the DHT buffers failed"));
     if (hipMemcpy(c->rows.p, rows.data(), sizeof(utr::DhtRow) * nrows, hipMemcpyHostToDevice) ||
         hipMemcpy(c->xs.p, xs.data(), sizeof(double) * xs.size(), hipMemcpyHostToDevice) ||
         hipMemcpy(c->ys.p, ys.data(), sizeof(double) * ys.size(), hipMemcpyHostToDevice) ||
         hipMemcpy(c->zs.p, zs.data(), sizeof(double) * zs.size(), hipMemcpyHostToDevice))
         return fail(set_err(UTR_E_DEVICE, "upload of the orientation tables failed"));
+    {
+        // aperture and Cs phase table of the propagators (DHT.cpp:962-1000):
+        // ap = exp(-i (f4 q^4 + f6 q^6)) inside q^2 < k2maxo, 0 outside
+        const double pi = 3.14159265358979323846, twopi = 2.0 * pi;
+        const double f4 = pi * g.wl * g.wl * g.wl * g.c3 / 2.0, f6 = pi * std::pow(g.wl, 5) * g.c5 / 3.0;
+        g.use_ap = (g.k2maxo > 0 || f4 != 0.0 || f6 != 0.0) ? 1 : 0;
+        std::vector<cplx<float>> ap(g.use_ap ? n2 : 1, cplx<float>{1.f, 0.f});
+        auto freq = [&](int k, int n) { return (double)(k <= n / 2 ? k : k - n) / ((double)n * g.xst); };
+        if (g.use_ap)
+            for (int ky = 0; ky < g.ny; ky++)
+                for (int kx = 0; kx < g.nx; kx++) {
+                    const double fx = freq(kx, g.nx), fy = freq(ky, g.ny), q2 = fx * fx + fy * fy;
+                    cplx<float>& e = ap[(size_t)ky * g.nx + kx];
+                    if (g.k2maxo > 0 && !(q2 < g.k2maxo)) {
+                        e = {0.f, 0.f};
+                        continue;
+                    }
+                    double chi = f4 * q2 * q2 + f6 * q2 * q2 * q2;
+                    chi -= twopi * std::rint(chi / twopi);
+                    e = {(float)std::cos(chi), (float)-std::sin(chi)};
+                }
+        if (c->ap.alloc(ap.size()) ||
+            hipMemcpy(c->ap.p, ap.data(), sizeof(cplx<float>) * ap.size(), hipMemcpyHostToDevice))
+            return fail(set_err(UTR_E_DEVICE, "upload of the aperture table failed"));
+    }
     auto twx = make_twiddles<float>(g.nx), twy = make_twiddles<float>(g.ny);
     if (c->twx.alloc(twx.size()) || c->twy.alloc(twy.size()) ||
         hipMemcpy(c->twx.p, twx.data(), sizeof(cplx<float>) * twx.size(), hipMemcpyHostToDevice) ||
@@ -1029,7 +1166,7 @@ int dht_destroy(dht_ctx* c) {
     if (c->sctf) utr_destroy(c->sctf);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     c->rows.release(); c->xs.release(); c->ys.release(); c->zs.release(); c->acc3.release();
-    c->spec.release(); c->T.release(); c->twx.release(); c->twy.release();
+    c->spec.release(); c->T.release(); c->twx.release(); c->twy.release(); c->ap.release();
     c->P.release(); c->stage.release(); c->vol.release(); c->pmax.release();
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
