@@ -5,18 +5,21 @@
 // Methods 3 and 4 (real-space path):
 //   k_dht_rows / k_dht_cols    phase-retrieval input and forward 2D FFT of each
 //                              orientation (DHT.cpp:850-947)
-//   k_dht_prop_rows / _cols    per output plane: Fresnel propagator (method 3)
-//                              or weak-object CTF inversion (method 4), inverse
-//                              2D FFT, 1-|psi|^2 or Re (DHT.cpp:957-1055); phase
+//   k_dht_prop_rows / _cols    per orientation and output plane: Fresnel
+//                              propagator (method 3) or weak-object CTF
+//                              inversion (method 4), inverse 2D FFT,
+//                              1-|psi|^2 or Re (DHT.cpp:957-1055); phase
 //                              arguments in FP64
 //   k_dht_splat                real-space inverse rotation and trilinear splat
 //                              with (int) truncation (DHT.cpp:1081-1164): one
 //                              workgroup per 16x8x8 brick, exact 64-bit
 //                              fixed-point sums in LDS, coordinates evaluated
 //                              in FP64 in the reference's operation order
+//   k_dht_splat_yp             the same sums in y-plane tiles when no
+//                              orientation rotates about Z
 //   k_dht_finish               scale of DHT.cpp:1325-1340
 // Orientations are processed in batches whose plane stacks fit a memory
-// budget; each batch adds its exact brick sums to an FP64 volume.
+// budget; each batch adds its exact sums to an FP64 volume.
 //
 // Method 5 (sCTF) runs the UTR Ewald-sphere kernels with the DHT voxel
 // filter, no sphere mask and the DHT scale (DHT.cpp:1056-1067, 1195-1416).
