@@ -78,9 +78,11 @@ typedef struct utr_timings {
  * Replaces the set-up at UTR.cpp:840-867 (Fftwd2fc plan, V3/Samp3
  * allocation) together with the validation of UTR.cpp:55-784.
  * rows: nrows orientation rows (UTR.cpp:122 ReadDefocusParamsFile output):
- * angZ_deg, angY_deg (columns 1-2), angZ2_deg (column 3, may be NULL; non-zero
- * values are UTR_E_UNSUPPORTED), defocus (column 4, UL), weights (subseries
- * weights, UTR.cpp:1019; may be NULL).  precision: UTR_FP32 or UTR_FP64.
+ * angZ_deg, angY_deg (columns 1-2), angZ2_deg (column 3, may be NULL: the
+ * Z'' angle, undone on each image before its 2D stage by a rotation about the
+ * rotation centre, UTR.cpp:960-977), defocus (column 4, UL), weights
+ * (subseries weights, UTR.cpp:1019; may be NULL).  precision: UTR_FP32 or
+ * UTR_FP64.
  */
 int utr_create(const utr_params* params, int nrows, const double* angZ_deg,
                const double* angY_deg, const double* angZ2_deg, const double* defocus,
@@ -116,6 +118,13 @@ int utr_finalize_device(utr_ctx* ctx, float* d_vol);
 /* Same, writing double precision to host memory (UTR_FP64 contexts only);
  * for precision studies against an FP64 CPU computation. */
 int utr_finalize_f64(utr_ctx* ctx, double* vol);
+
+/* RELION image shifts (UL, per orientation row, as read from a .relionnew
+ * file), undone on each image before its 2D stage: the image content moves
+ * by (floor(-shift_x/xst + 0.5), floor(-shift_y/xst + 0.5)) pixels, the
+ * edge mean filling what is uncovered.  Call before any image is added.
+ * Replaces the XArray2DMove::Move call of UTR.cpp:961-969. */
+int utr_set_image_shifts(utr_ctx* ctx, const double* shift_x, const double* shift_y);
 
 /* Keep the sampling matrix Samp3 of the next utr_finalize in device memory
  * (parameter 32, UTR.cpp:706-712; off by default). */

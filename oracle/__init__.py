@@ -6,6 +6,7 @@ precision.  Only ``tests/``, ``__graft_entry__.smoke()`` and the cpu_baseline
 leg of ``bench.py`` import this module; the product package never does.
 """
 import ctypes
+import math
 import os
 import subprocess
 
@@ -188,13 +189,99 @@ def mask_scale(g, vol):
     return vol
 
 
+def keys_weights(t):
+    """Cubic convolution weights of Keys (1981, a = -1/2) of the neighbours at
+    offsets -1, 0, 1, 2 for fractional positions t in [0, 1): shape t.shape+(4,)."""
+    t = np.asarray(t, dtype=np.float64)
+    t2, t3 = t * t, t * t * t
+    return np.stack([-0.5 * t3 + t2 - 0.5 * t, 1.5 * t3 - 2.5 * t2 + 1.0,
+                     -1.5 * t3 + 2.0 * t2 + 0.5 * t, 0.5 * t3 - 0.5 * t2], axis=-1)
+
+
+def edge_mean(s, e=5):
+    """NormAverEdge(e) restated: the mean of the pixels within e of the border."""
+    ny, nx = s.shape
+    e = min(e, min(nx, ny) // 2)
+    m = np.zeros((ny, nx), dtype=bool)
+    m[:e, :] = m[ny - e:, :] = m[:, :e] = m[:, nx - e:] = True
+    return float(np.mean(s[m], dtype=np.float64))
+
+
+def shift_rotate(s, dx, dy, ang_deg, px, py, bg):
+    """The image correction of UTR.cpp:960-977, restated (the XArrayLibrary
+    routines are absent): content shifted by (dx, dy) pixels (XArray2DMove::
+    Move), then turned by ang_deg counter-clockwise in (x = column, y = row)
+    about (px, py) (XArray2DSpln::Rotate) with Keys cubic convolution; bg
+    wherever a neighbour falls outside either image.  FP64."""
+    ny, nx = s.shape
+    a = math.radians(ang_deg)
+    cs, sn = math.cos(a), math.sin(a)
+    j, i = np.mgrid[0:ny, 0:nx].astype(np.float64)
+    sx = px + (i - px) * cs + (j - py) * sn
+    sy = py - (i - px) * sn + (j - py) * cs
+    fx, fy = np.floor(sx), np.floor(sy)
+    wx, wy = keys_weights(sx - fx), keys_weights(sy - fy)
+    out = np.zeros((ny, nx))
+    for b in range(4):
+        yy = fy.astype(np.int64) - 1 + b
+        for c in range(4):
+            xx = fx.astype(np.int64) - 1 + c
+            xs, ys = xx - dx, yy - dy
+            ok = ((xx >= 0) & (xx < nx) & (yy >= 0) & (yy < ny) & (xs >= 0) & (xs < nx) &
+                  (ys >= 0) & (ys < ny))
+            v = np.full((ny, nx), bg)
+            v[ok] = s[ys[ok], xs[ok]]
+            out += wx[..., c] * wy[..., b] * v
+    return out
+
+
+def prep_images(params, rows, images):
+    """Images after the input correction of UTR.cpp:960-977 (Z'' rotation
+    undone, RELION shifts undone), expressed before the renormalisation that
+    the 2D stage applies (so that it yields the corrected image)."""
+    angZ, angY, angZ2, defoc = _rows(rows)
+    n = len(angZ)
+    z = np.zeros(n)
+    shx = np.asarray(rows.get("shift_x", z), dtype=np.float64)
+    shy = np.asarray(rows.get("shift_y", z), dtype=np.float64)
+    xst = float(params["xst"])
+    nx, ny = int(params["nx"]), int(params["ny"])
+    f1, f2 = np.float32(params.get("norm_f1", 1.0)), np.float32(params.get("norm_f2", 0.0))
+    px = nx / 2 + float(params.get("dxc", 0.0)) / xst
+    py = ny / 2 + float(params.get("dyc", 0.0)) / xst
+    out = np.array(images, dtype=np.float32, copy=True)
+    for r in range(n):
+        dx = int(np.floor(-shx[r] / xst + 0.5))
+        dy = int(np.floor(-shy[r] / xst + 0.5))
+        if angZ2[r] == 0 and dx == 0 and dy == 0:
+            continue
+        s = images[r].astype(np.float32)
+        if f1 != 1:
+            s = s / f1
+        if f2 != 0:
+            s = s + f2
+        s = np.maximum(s, np.float32(0)).astype(np.float64)
+        y = shift_rotate(s, dx, dy, -angZ2[r], px, py, edge_mean(s))
+        if f2 != 0:
+            y = y - float(f2)
+        if f1 != 1:
+            y = y * float(f1)
+        out[r] = y.astype(np.float32)
+    return out
+
+
 def reconstruct(params, rows, images, weights=None, nthreads=1, samp=False):
     """Full FP64 reconstruction.  Returns the volume (nz, ny, nx) float64 and,
-    if samp is True, the sampling matrix."""
+    if samp is True, the sampling matrix.  Rows with a Z'' angle or a RELION
+    shift (rows['shift_x'], rows['shift_y']) have their images corrected
+    first (prep_images)."""
     angZ, angY, angZ2, defoc = _rows(rows)
     g = derive(params, rows)
     images = np.ascontiguousarray(images, dtype=np.float32)
     assert images.shape == (len(angZ), g.ny, g.nx), images.shape
+    if np.any(angZ2 != 0) or np.any(np.asarray(rows.get("shift_x", 0.0)) != 0) or \
+            np.any(np.asarray(rows.get("shift_y", 0.0)) != 0):
+        images = np.ascontiguousarray(prep_images(params, rows, images))
     vol = np.zeros((g.nz, g.ny, g.nx))
     S = np.zeros((g.nz, g.ny, g.nx)) if samp else None
     w = None

@@ -190,9 +190,6 @@ int ora_derive(const ora_params* p, int nrows, const double* angZ_deg,
     if (p->epsilon == 0 && p->asigma == 0) return fail("Error: when beta/delta == 0, epsilon must be positive.");
     if (p->inv_laplace != 0 && p->inv_laplace != 1)
         return fail("Error: unknown value for 3D Laplacian filter mode in input parameter file.");
-    if (angZ2_deg)
-        for (int i = 0; i < nrows; i++)
-            if (angZ2_deg[i] != 0) return fail("Z'' image rotation is not supported");
 
     g->nx = p->nx; g->ny = p->ny;
     g->nxd2 = p->nx / 2; g->nyd2 = p->ny / 2;

@@ -189,3 +189,23 @@ def test_cli_peak_localization(tmp_path, io):
     assert np.abs(a - b).max() < 1e-4 * np.abs(b).max()
     lines = open(tmp_path / "out" / "vol.xyz").read().split("\n")
     assert lines[-2] == "-1" and len(lines) - 4 == len(got)
+
+
+def test_cli_z2_rotation_and_relion_shifts(tmp_path, io):
+    """Z'' angles and RELION shifts in a .relionnew file: the program undoes
+    them on each image before its 2D stage (UTR.cpp:960-977) and matches the
+    oracle given the same rows."""
+    nrows = 16
+    p, rows, imgs = setup_case(tmp_path, io, nrows=nrows, orient="o.relionnew")
+    rng = np.random.default_rng(9)
+    z2 = rng.uniform(-40, 40, nrows)
+    sx, sy = rng.uniform(-1.2, 1.2, nrows), rng.uniform(-1.2, 1.2, nrows)
+    with open(tmp_path / "o.relionnew", "w") as f:
+        for i, a in enumerate(rows["angY"]):
+            # every number but the astigmatism angle changes sign on reading
+            f.write("%d img%02d.mrc %.10g %.10g -300 -300 0 0 %.10g %.10g\n"
+                    % (i + 1, i, sx[i], sy[i], -a, -z2[i]))
+    run(tmp_path)
+    ref = oracle.reconstruct(p, dict(rows, angZ2=z2, shift_x=-sx, shift_y=-sy), imgs)
+    vol = read_stack(io, tmp_path, "out/vol.grd", 64)
+    assert rel(vol, ref) < TOL

@@ -193,8 +193,8 @@ def test_missing_rows_and_bad_ranges_are_errors():
             r.add_images(imgs[:2], first_row=5)
     with pytest.raises(U.UTRError, match="not supported"):
         U.Reconstructor(U.default_params(nx=34, ny=34, zlo=-6.8, zhi=6.8), rows)
-    with pytest.raises(U.UTRError, match="Z''"):
-        U.Reconstructor(p, dict(rows, angZ2=np.ones(6)))
+    with pytest.raises(U.UTRError, match="finite"):
+        U.Reconstructor(p, dict(rows, shift_x=np.full(6, np.nan), shift_y=np.zeros(6)))
 
 
 def test_single_row():

@@ -34,6 +34,7 @@ PARAM_FIELDS = [
 EXPORTS = [
     "utr_create", "utr_add_images", "utr_add_images_device", "utr_finalize",
     "utr_finalize_device", "utr_finalize_f64", "utr_keep_sampling", "utr_get_sampling",
+    "utr_set_image_shifts",
     "utr_reset", "utr_dims", "utr_get_timings", "utr_destroy", "utr_last_error",
     "utr_abi_version", "utr_angle_coeffs", "utr_get_spectrum", "utr_get_fourier_volume",
     "utr_batch_create", "utr_batch_run", "utr_batch_job_form", "utr_batch_destroy",
@@ -139,6 +140,7 @@ def load():
     L.utr_finalize_device.argtypes = [vp, vp]
     L.utr_finalize_f64.argtypes = [vp, vp]
     L.utr_keep_sampling.argtypes = [vp, ctypes.c_int]
+    L.utr_set_image_shifts.argtypes = [vp, dp, dp]
     L.utr_get_sampling.argtypes = [vp, vp]
     L.utr_reset.argtypes = [vp]
     L.utr_dims.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
@@ -192,7 +194,8 @@ class Reconstructor:
     """One UTR reconstruction context on the current HIP device.
 
     rows: dict with arrays angZ, angY (degrees), defocus (UL), optional angZ2
-    and weights -- the columns of the orientation file read at UTR.cpp:122.
+    and weights -- the columns of the orientation file read at UTR.cpp:122 --
+    and optional shift_x, shift_y (UL, the image shifts of a .relionnew file).
     """
 
     def __init__(self, params, rows, precision=UTR_FP32):
@@ -218,6 +221,12 @@ class Reconstructor:
                             self.weights.ctypes.data if self.weights is not None else None,
                             precision, ctypes.byref(ctx)))
         self._ctx = ctx
+        if rows.get("shift_x") is not None or rows.get("shift_y") is not None:
+            z = np.zeros(self.nrows)
+            sx = np.ascontiguousarray(rows.get("shift_x", z), dtype=np.float64)
+            sy = np.ascontiguousarray(rows.get("shift_y", z), dtype=np.float64)
+            assert sx.shape == (self.nrows,) and sy.shape == (self.nrows,)
+            _check(L.utr_set_image_shifts(ctx, _dp(sx), _dp(sy)))
         nx, ny, nz = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         _check(L.utr_dims(ctx, ctypes.byref(nx), ctypes.byref(ny), ctypes.byref(nz)))
         self.shape = (nz.value, ny.value, nx.value)
@@ -387,6 +396,12 @@ class DhtReconstructor:
                             self.weights.ctypes.data if self.weights is not None else None,
                             ctypes.byref(ctx)))
         self._ctx = ctx
+        if rows.get("shift_x") is not None or rows.get("shift_y") is not None:
+            z = np.zeros(self.nrows)
+            sx = np.ascontiguousarray(rows.get("shift_x", z), dtype=np.float64)
+            sy = np.ascontiguousarray(rows.get("shift_y", z), dtype=np.float64)
+            assert sx.shape == (self.nrows,) and sy.shape == (self.nrows,)
+            _check(L.utr_set_image_shifts(ctx, _dp(sx), _dp(sy)))
         nx, ny, nz = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         _check(L.dht_dims(ctx, ctypes.byref(nx), ctypes.byref(ny), ctypes.byref(nz)))
         self.shape = (nz.value, ny.value, nx.value)

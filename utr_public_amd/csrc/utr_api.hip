@@ -131,6 +131,9 @@ struct utr_ctx {
     // shared
     DevBuf<utr::RowParams> rows;
     DevBuf<float> img_stage;
+    // input image correction (Z'' rotation, RELION shift) of the rows that need it
+    DevBuf<utr::PrepRow> prep;
+    DevBuf<float> img_prep, prep_aver;
     // FP32 form
     DevBuf<cplx<float>> spec32, V32, H32, twx32, twy32, twz32;
     DevBuf<float> S32, vol32;
@@ -253,7 +256,60 @@ int launch_3d(utr_ctx* c, const cplx<T>* d_V, cplx<T>* d_H, TO* d_vol, const cpl
 
 // Launches the 2D stage (and, in the plain FP64 form, the scatter) of `count`
 // rows whose images are at d_img (device), without waiting.
-int enqueue_rows(utr_ctx* c, int first_row, int count, const float* d_img) {
+constexpr int kPrepRows = 64;  // images corrected per pass (img_prep holds this many)
+
+// Device buffers of the input image correction, when some row needs it.
+int alloc_prep(utr_ctx* c) {
+    bool any = false;
+    for (const auto& q : c->plan.prep) any |= q.on != 0;
+    if (!any) return UTR_OK;
+    const auto& g = c->plan.g;
+    const int k = std::min(kPrepRows, c->plan.nrows);
+    UTR_HIP_TRY(c->img_prep.alloc((size_t)k * g.nx * g.ny));
+    UTR_HIP_TRY(c->prep_aver.alloc(k));
+    return UTR_OK;
+}
+
+// Input image correction of count <= kPrepRows rows (UTR.cpp:960-977) into
+// img_prep: the edge mean of each image, then its shifted and rotated copy
+// (rows without a correction are copied unchanged).
+int prep_images(utr_ctx* c, int first_row, int count, const float* d_img) {
+    const auto& g = c->plan.g;
+    const size_t n2 = (size_t)g.nx * g.ny;
+    if (count > kPrepRows || c->img_prep.n < n2 * count || c->prep_aver.n < (size_t)count)
+        return set_err(UTR_E_STATE, "image correction buffers not allocated");
+    utr::k_prep_aver<<<count, 256, 0, c->stream>>>(d_img, c->prep.p, first_row, g, c->prep_aver.p);
+    UTR_HIP_TRY(hipGetLastError());
+    // rotation centre (UTR.cpp:342-343) in pixels: pixel i sits at xlo + i xst
+    const double px = (c->plan.xc - g.xlo) / c->plan.xst;
+    const double py = (c->plan.yc - g.ylo) / c->plan.xst;
+    const dim3 grid((unsigned)((n2 + 255) / 256), (unsigned)count);
+    utr::k_prep_img<<<grid, 256, 0, c->stream>>>(d_img, c->img_prep.p, c->prep.p, first_row, g, px, py,
+                                                 c->prep_aver.p);
+    UTR_HIP_TRY(hipGetLastError());
+    return UTR_OK;
+}
+
+// stage(first, count, images) for rows [first_row, first_row + count) whose
+// images are at d_img, through the input image correction where a row
+// needs it.
+template <typename F>
+int with_prep(utr_ctx* c, int first_row, int count, const float* d_img, F&& stage) {
+    bool any = false;
+    for (int r = 0; r < count; r++) any |= c->plan.prep[first_row + r].on != 0;
+    if (!any) return stage(first_row, count, d_img);
+    const size_t n2 = (size_t)c->plan.g.nx * c->plan.g.ny;
+    for (int r = 0; r < count; r += kPrepRows) {
+        const int k = std::min(kPrepRows, count - r);
+        int rc = prep_images(c, first_row + r, k, d_img + r * n2);
+        if (rc) return rc;
+        rc = stage(first_row + r, k, c->img_prep.p);
+        if (rc) return rc;
+    }
+    return UTR_OK;
+}
+
+int enqueue_plain(utr_ctx* c, int first_row, int count, const float* d_img) {
     const auto& g = c->plan.g;
     if (c->precision != UTR_FP64) {
         return launch_2d<float>(c, d_img, first_row, count,
@@ -272,6 +328,11 @@ int enqueue_rows(utr_ctx* c, int first_row, int count, const float* d_img) {
         UTR_HIP_TRY(hipGetLastError());
     }
     return UTR_OK;
+}
+
+int enqueue_rows(utr_ctx* c, int first_row, int count, const float* d_img) {
+    return with_prep(c, first_row, count, d_img,
+                     [&](int f, int k, const float* im) { return enqueue_plain(c, f, k, im); });
 }
 
 // Waits for the rows enqueued since event e0, adds their time to the 2D
@@ -443,6 +504,11 @@ int utr_create(const utr_params* params, int nrows, const double* angZ_deg, cons
     if (hipMemcpy(c->rows.p, c->plan.rows.data(), sizeof(utr::RowParams) * nrows,
                   hipMemcpyHostToDevice) != hipSuccess)
         return fail(set_err(UTR_E_DEVICE, "row table upload"));
+    if (c->prep.alloc(nrows) != hipSuccess ||
+        hipMemcpy(c->prep.p, c->plan.prep.data(), sizeof(utr::PrepRow) * nrows, hipMemcpyHostToDevice) !=
+            hipSuccess)
+        return fail(set_err(UTR_E_DEVICE, "image correction table upload"));
+    if (alloc_prep(c)) return fail(set_err(UTR_E_NOMEM, "image correction buffers"));
     const size_t N = vox(c), n2 = (size_t)g.nx * g.ny;
     const size_t nh = (size_t)g.nz * g.ny * utr::h_stride(g.nx);
     int rc;
@@ -532,6 +598,27 @@ int utr_finalize_f64(utr_ctx* c, double* vol) {
     return UTR_OK;
 }
 
+int utr_set_image_shifts(utr_ctx* c, const double* shift_x, const double* shift_y) {
+    if (!c) return set_err(UTR_E_INVALID, "null context");
+    if (!shift_x || !shift_y) return set_err(UTR_E_INVALID, "null shift array");
+    if (c->n_added) return set_err(UTR_E_STATE, "utr_set_image_shifts after images were added");
+    auto& P = c->plan;
+    for (int i = 0; i < P.nrows; i++) {
+        if (!std::isfinite(shift_x[i]) || !std::isfinite(shift_y[i]))
+            return set_err(UTR_E_INVALID, "image shifts must be finite");
+        // UTR.cpp:965-969: Move(floor(-shift_y / yst + 0.5), floor(-shift_x / xst + 0.5))
+        const double dx = std::floor(-shift_x[i] / P.xst + 0.5), dy = std::floor(-shift_y[i] / P.xst + 0.5);
+        if (std::fabs(dx) > 1e6 || std::fabs(dy) > 1e6) return set_err(UTR_E_INVALID, "image shift too large");
+        utr::PrepRow& q = P.prep[i];
+        q.dx = (int)dx;
+        q.dy = (int)dy;
+        q.on = (P.angZ2[i] != 0 || q.dx != 0 || q.dy != 0) ? 1 : 0;
+    }
+    UTR_HIP_TRY(hipStreamSynchronize(c->stream));
+    UTR_HIP_TRY(hipMemcpy(c->prep.p, P.prep.data(), sizeof(utr::PrepRow) * P.nrows, hipMemcpyHostToDevice));
+    return alloc_prep(c);
+}
+
 int utr_keep_sampling(utr_ctx* c, int on) {
     if (!c) return set_err(UTR_E_INVALID, "null context");
     c->keep_sampling = on ? 1 : 0;
@@ -618,6 +705,9 @@ int utr_destroy(utr_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     c->rows.release();
     c->img_stage.release();
+    c->prep.release();
+    c->img_prep.release();
+    c->prep_aver.release();
     c->spec32.release(); c->V32.release(); c->H32.release(); c->S32.release(); c->vol32.release();
     c->twx32.release(); c->twy32.release(); c->twz32.release(); c->rowmax.release(); c->scales.release();
     c->spec64.release(); c->V64.release(); c->H64.release(); c->S64.release(); c->vol64.release();
@@ -658,7 +748,10 @@ int enqueue_job(utr_ctx* c, const float* d_img, float* d_vol) {
     const auto& g = c->plan.g;
     const int nrows = c->plan.nrows;
     utr::k_fill_u32<<<(nrows + 255) / 256, 256, 0, c->stream>>>(c->rowmax.p, nrows);
-    int rc = launch_2d<float>(c, d_img, 0, nrows, c->spec32.p, c->twx32.p, c->twy32.p);
+    const size_t n2 = (size_t)g.nx * g.ny;
+    int rc = with_prep(c, 0, nrows, d_img, [&](int f, int k, const float* im) {
+        return launch_2d<float>(c, im, f, k, c->spec32.p + (size_t)f * n2, c->twx32.p, c->twy32.p);
+    });
     if (rc) return rc;
     if (c->precision == UTR_FP32) {
         rc = launch_grid32(c, nullptr);

@@ -188,10 +188,7 @@ int run(int argc, char** argv) {
     if (c.symmetry_mode == 2) unsupported("post-applied symmetry (parameter 20 = 2)");
     if (c.rot_out[0] != 0 || c.rot_out[1] != 0 || c.rot_out[2] != 0)
         unsupported("rotation of the output volume (parameter 28)");
-    for (const auto& o : rows) {
-        if (o.shift_x != 0 || o.shift_y != 0) unsupported("image shifts from .relionnew files");
-        if (astig) unsupported("astigmatism from .relionnew files");
-    }
+    if (astig) unsupported("astigmatism from .relionnew files");
     if ((c.ctf_mode == 2 || c.ctf_mode == 4) && !single_defocus) {
         printf("\n\nWARNING: different defocus distances and/or non-zero astigmatism have been detected in the input file %s.",
                c.defocus_file.c_str());
@@ -247,6 +244,17 @@ int run(int argc, char** argv) {
         utr_ctx* c;
         ~Guard() { utr_destroy(c); }
     } guard{ctx};
+    {
+        // RELION image shifts, undone before the 2D stage (UTR.cpp:961-969)
+        std::vector<double> sx(nrows), sy(nrows);
+        bool any = false;
+        for (long i = 0; i < nrows; i++) {
+            sx[i] = rows[i].shift_x;
+            sy[i] = rows[i].shift_y;
+            any |= sx[i] != 0 || sy[i] != 0;
+        }
+        if (any) check(utr_set_image_shifts(ctx, sx.data(), sy.data()));
+    }
     int nx = 0, ny = 0, nz = 0;
     check(utr_dims(ctx, &nx, &ny, &nz));
     if (c.save_sampling == 1) check(utr_keep_sampling(ctx, 1));

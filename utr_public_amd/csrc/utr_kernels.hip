@@ -120,6 +120,118 @@ __global__ void __launch_bounds__(256) k_rows(const float* __restrict__ img,
     }
 }
 
+// ----------------------------------------- input image correction (a6)
+//
+// UTR.cpp:960-977 undoes, before the 2D stage, the Z'' rotation of an input
+// image (XArray2DSpln::Rotate(int0, -Z'', yc, xc, aver)) and the shift of a
+// RELION row (XArray2DMove::Move(dy, dx, aver), applied first), with aver =
+// int0.NormAverEdge(5) of the renormalised, clamped image.  Both routines
+// live in the absent XArrayLibrary; restated here:
+//   aver   the mean of the pixels within 5 of the image border;
+//   Move   content shifted by (dx, dy) pixels, aver where nothing lands;
+//   Rotate content turned by the angle about the rotation centre (px, py)
+//          in pixel units, counter-clockwise in (x = column, y = row) as the
+//          forward model turns atoms about Z (autosliccmd.cpp:734-740); the
+//          value at a source position is the cubic convolution of Keys
+//          (1981, a = -1/2) of its 4 x 4 neighbours, aver outside the image.
+// The output goes back through the inverse of the renormalisation, since
+// k_rows renormalises (and clamps) every pixel it reads.
+
+__device__ __forceinline__ float renorm_clamp(float x, const GeomDev& g) {
+    if (g.use_f1) x = x / g.f1;
+    if (g.use_f2) x = x + g.f2;
+    return x < 0.f ? 0.f : x;
+}
+
+// one workgroup per image of the batch: aver of image r
+__global__ void __launch_bounds__(256) k_prep_aver(const float* __restrict__ img,
+                                                   const PrepRow* __restrict__ prep, int row0,
+                                                   GeomDev g, float* __restrict__ aver) {
+    __shared__ double s_red[4];
+    const int r = blockIdx.x, tid = threadIdx.x;
+    if (!prep[row0 + r].on) return;  // uniform
+    const int nx = g.nx, ny = g.ny, e = min(5, min(nx, ny) / 2);
+    const float* src = img + (size_t)r * nx * ny;
+    // frame: e full rows at the top and bottom, e pixels at both ends of the rows between
+    const int nrow = 2 * e * nx, nside = 2 * e * (ny - 2 * e), total = nrow + nside;
+    double sum = 0.0;
+    for (int t = tid; t < total; t += 256) {
+        int i, j;
+        if (t < nrow) {
+            const int k = t / nx;
+            i = t - k * nx;
+            j = k < e ? k : ny - 2 * e + k;
+        } else {
+            const int u = t - nrow, k = u / (2 * e), c = u - k * 2 * e;
+            j = e + k;
+            i = c < e ? c : nx - 2 * e + c;
+        }
+        sum += (double)renorm_clamp(src[(size_t)j * nx + i], g);
+    }
+    for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+    if ((tid & 63) == 0) s_red[tid >> 6] = sum;
+    __syncthreads();
+    if (tid == 0) aver[r] = (float)((s_red[0] + s_red[1] + s_red[2] + s_red[3]) / (double)total);
+}
+
+// Keys cubic convolution weights of the neighbours at offsets -1, 0, 1, 2
+// for fractional position t in [0, 1), FP64, each expression rounded in the
+// order written (no contraction), as the oracle evaluates it
+__device__ __forceinline__ void keys_weights(double t, double w[4]) {
+    const double t2 = __dmul_rn(t, t), t3 = __dmul_rn(t2, t);
+    w[0] = __dsub_rn(__dadd_rn(__dmul_rn(-0.5, t3), t2), __dmul_rn(0.5, t));
+    w[1] = __dadd_rn(__dsub_rn(__dmul_rn(1.5, t3), __dmul_rn(2.5, t2)), 1.0);
+    w[2] = __dadd_rn(__dadd_rn(__dmul_rn(-1.5, t3), __dmul_rn(2.0, t2)), __dmul_rn(0.5, t));
+    w[3] = __dsub_rn(__dmul_rn(0.5, t3), __dmul_rn(0.5, t2));
+}
+
+// grid (pixels / 256, images of the batch); (px, py): rotation centre in
+// pixels.  FP64 throughout, one rounding to float at the end.
+__global__ void __launch_bounds__(256) k_prep_img(const float* __restrict__ img, float* __restrict__ out,
+                                                  const PrepRow* __restrict__ prep, int row0, GeomDev g,
+                                                  double px, double py, const float* __restrict__ aver) {
+    const int r = blockIdx.y;
+    const int nx = g.nx, ny = g.ny;
+    const size_t n2 = (size_t)nx * ny;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n2) return;
+    const float* src = img + (size_t)r * n2;
+    const PrepRow P = prep[row0 + r];
+    if (!P.on) {
+        out[(size_t)r * n2 + t] = src[t];
+        return;
+    }
+    const int i = (int)(t % nx), j = (int)(t / nx);
+    const double bg = (double)aver[r];
+    // source position in the shifted image
+    const double dxp = __dsub_rn((double)i, px), dyp = __dsub_rn((double)j, py);
+    const double sx = __dadd_rn(__dadd_rn(px, __dmul_rn(dxp, P.cs)), __dmul_rn(dyp, P.sn));
+    const double sy = __dadd_rn(__dsub_rn(py, __dmul_rn(dxp, P.sn)), __dmul_rn(dyp, P.cs));
+    const double fx0 = floor(sx), fy0 = floor(sy);
+    double wx[4], wy[4];
+    keys_weights(__dsub_rn(sx, fx0), wx);
+    keys_weights(__dsub_rn(sy, fy0), wy);
+    const int ix = (int)fx0 - 1, iy = (int)fy0 - 1;
+    double acc = 0.0;
+#pragma unroll
+    for (int b = 0; b < 4; b++) {
+        const int yy = iy + b, ys = yy - P.dy;
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+            const int xx = ix + a, xs = xx - P.dx;
+            double v = bg;
+            if ((unsigned)xx < (unsigned)nx && (unsigned)yy < (unsigned)ny && (unsigned)xs < (unsigned)nx &&
+                (unsigned)ys < (unsigned)ny)
+                v = (double)renorm_clamp(src[(size_t)ys * nx + xs], g);
+            acc = __dadd_rn(acc, __dmul_rn(__dmul_rn(wx[a], wy[b]), v));
+        }
+    }
+    // back through the inverse renormalisation (k_rows applies it again)
+    if (g.use_f2) acc = __dsub_rn(acc, (double)g.f2);
+    if (g.use_f1) acc = __dmul_rn(acc, (double)g.f1);
+    out[(size_t)r * n2 + t] = (float)acc;
+}
+
 // Columns: forward FFT along y, output Shuffle as (-1)^(kx+ky+nx/2+ny/2),
 // optional 2D filter (modes 1/3, restated) and subseries weight.
 // rowmax (optional): per-row max(|re|, |im|) of the result as float bits

@@ -96,18 +96,22 @@ bool build_plan(const utr_params* p, int nrows, const double* angZ, const double
         err = "Error: unknown value for 3D Laplacian filter mode in input parameter file.";
         return false;
     }
-    if (angZ2)
-        for (int i = 0; i < nrows; i++)
-            if (angZ2[i] != 0) {
-                err = "Z'' image rotation (column 3 of the orientation file) is not supported";
-                return false;
-            }
-
     P.p = *p;
     P.nrows = nrows;
     P.angZ.assign(angZ, angZ + nrows);
     P.angY.assign(angY, angY + nrows);
     P.defocus.assign(defocus, defocus + nrows);
+    P.angZ2.assign(nrows, 0.0);
+    if (angZ2) P.angZ2.assign(angZ2, angZ2 + nrows);
+    // UTR.cpp:971-976: Rotate(int0, -Z'') turns the image content by -Z''
+    P.prep.assign(nrows, PrepRow{});
+    for (int i = 0; i < nrows; i++) {
+        PrepRow& q = P.prep[i];
+        const double a = -P.angZ2[i] * (3.14159265358979323846 / 180.0);
+        q.cs = std::cos(a);
+        q.sn = std::sin(a);
+        q.on = P.angZ2[i] != 0 ? 1 : 0;
+    }
     P.weights.assign(nrows, 1.0f);
     if (weights)
         for (int i = 0; i < nrows; i++) P.weights[i] = weights[i];

@@ -58,11 +58,22 @@ struct alignas(16) RowParams {
     int pad[3];
 };
 
+// Per orientation row: the in-plane correction of its input image before
+// the 2D stage (UTR.cpp:960-977): content rotation by -Z'' about the
+// rotation centre and the integer shift of a RELION row.
+struct PrepRow {
+    double cs, sn;  // cos, sin of the content rotation angle
+    int dx, dy;     // content shift in pixels (columns, rows)
+    int on;         // 0: the image passes unchanged
+    int pad;
+};
+
 struct Plan {
     utr_params p;
     GeomDev g;
     int nrows = 0;
-    std::vector<double> angZ, angY, defocus;
+    std::vector<double> angZ, angY, angZ2, defocus;
+    std::vector<PrepRow> prep;
     std::vector<float> weights;
     std::vector<RowParams> rows;
     double EE = 0, xst = 0, zst = 0;
