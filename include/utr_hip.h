@@ -185,6 +185,26 @@ int utr_highpass3d_device(float* d_vol, int nx, int ny, int nz, double xst, doub
                           double width);
 int utr_highpass3d(float* vol, int nx, int ny, int nz, double xst, double yst, double zst, double width);
 
+/* 3D rotation of a volume, replacing XArray3DSpln::Rotate3(K3out, Z, Y', Z'')
+ * of UTR.cpp:1387-1394 (parameter 28; the reference passes the angles
+ * negated on reading, in radians -- here in degrees).  The content turns by
+ * R = Rz(angZ2) Ry(angY) Rz(angZ) about the array centre (nx/2, ny/2, nz/2),
+ * the rotation pdb.exe applies to atoms (pdb_run.cpp:419-432); values are
+ * tricubic (Keys) interpolations, 0 outside.  The spline of the absent
+ * library is restated.  _device: d_out must not alias d_in. */
+int utr_rotate3d_device(const float* d_in, float* d_out, int nx, int ny, int nz, double angZ_deg,
+                        double angY_deg, double angZ2_deg);
+int utr_rotate3d(float* vol, int nx, int ny, int nz, double angZ_deg, double angY_deg, double angZ2_deg);
+
+/* Symmetry average of UTR.cpp:1341-1355 (parameter 20 = 2): vol becomes the
+ * mean of itself and its rotations turning back symmetry rows 1 .. nsym-1
+ * (Rotate3 with -Z'', -Y', -Z of each row of the parameter-21 file, whose
+ * columns 1-3 are angZ, angY, angZ2 in degrees). */
+int utr_symmetrize3d_device(float* d_vol, int nx, int ny, int nz, int nsym, const double* angZ_deg,
+                            const double* angY_deg, const double* angZ2_deg);
+int utr_symmetrize3d(float* vol, int nx, int ny, int nz, int nsym, const double* angZ_deg,
+                     const double* angY_deg, const double* angZ2_deg);
+
 /* ----------------------------------------------------------- batched form
  * Several independent reconstructions of uneven size run as one HIP graph:
  * each job's launches (2D stage of all its rows, gridding, 3D inverse FFT)

@@ -235,6 +235,57 @@ def shift_rotate(s, dx, dy, ang_deg, px, py, bg):
     return out
 
 
+def rot3_matrix(aZ_deg, aY_deg, aZ2_deg):
+    """R = Rz(aZ2) Ry(aY) Rz(aZ) of pdb_run.cpp:419-432: Rz turns (x, y) by
+    (x cos - y sin, x sin + y cos), Ry turns (x, z) by (x cos + z sin,
+    -x sin + z cos)."""
+    def rz(a):
+        c, s = math.cos(math.radians(a)), math.sin(math.radians(a))
+        return np.array([[c, -s, 0.0], [s, c, 0.0], [0.0, 0.0, 1.0]])
+
+    c, s = math.cos(math.radians(aY_deg)), math.sin(math.radians(aY_deg))
+    ry = np.array([[c, 0.0, s], [0.0, 1.0, 0.0], [-s, 0.0, c]])
+    return rz(aZ2_deg) @ ry @ rz(aZ_deg)
+
+
+def rotate3(vol, aZ_deg, aY_deg, aZ2_deg):
+    """XArray3DSpln::Rotate3 restated (the library is absent): the content of
+    the (nz, ny, nx) volume turns by rot3_matrix about (nx/2, ny/2, nz/2);
+    values by tricubic Keys convolution, 0 outside.  FP64."""
+    v = np.asarray(vol, dtype=np.float64)
+    nz, ny, nx = v.shape
+    Mi = rot3_matrix(aZ_deg, aY_deg, aZ2_deg).T
+    k, j, i = np.mgrid[0:nz, 0:ny, 0:nx].astype(np.float64)
+    c = np.array([nx / 2, ny / 2, nz / 2])
+    d = np.stack([i - c[0], j - c[1], k - c[2]])
+    q = [c[r] + Mi[r, 0] * d[0] + Mi[r, 1] * d[1] + Mi[r, 2] * d[2] for r in range(3)]
+    f = [np.floor(x) for x in q]
+    w = [keys_weights(q[r] - f[r]) for r in range(3)]
+    base = [f[r].astype(np.int64) - 1 for r in range(3)]
+    out = np.zeros_like(v)
+    for cz in range(4):
+        zz = base[2] + cz
+        for b in range(4):
+            yy = base[1] + b
+            for a in range(4):
+                xx = base[0] + a
+                ok = (xx >= 0) & (xx < nx) & (yy >= 0) & (yy < ny) & (zz >= 0) & (zz < nz)
+                val = np.zeros_like(v)
+                val[ok] = v[zz[ok], yy[ok], xx[ok]]
+                out += w[0][..., a] * w[1][..., b] * w[2][..., cz] * val
+    return out
+
+
+def symmetrize3(vol, angZ, angY, angZ2):
+    """UTR.cpp:1341-1355: the mean of the volume and its rotations turning
+    back symmetry rows 1 .. n-1 (Rotate3 with -Z'', -Y', -Z)."""
+    v = np.asarray(vol, dtype=np.float64)
+    acc = v.copy()
+    for s in range(1, len(angZ)):
+        acc += rotate3(v, -angZ2[s], -angY[s], -angZ[s])
+    return acc / len(angZ)
+
+
 def prep_images(params, rows, images):
     """Images after the input correction of UTR.cpp:960-977 (Z'' rotation
     undone, RELION shifts undone), expressed before the renormalisation that

@@ -209,3 +209,21 @@ def test_cli_z2_rotation_and_relion_shifts(tmp_path, io):
     ref = oracle.reconstruct(p, dict(rows, angZ2=z2, shift_x=-sx, shift_y=-sy), imgs)
     vol = read_stack(io, tmp_path, "out/vol.grd", 64)
     assert rel(vol, ref) < TOL
+
+
+def test_cli_symmetry_and_output_rotation(tmp_path, io):
+    """Parameters 20 = 2 with a symmetry file (21) and 28: the reconstruction
+    is averaged over the symmetry rows turned back (UTR.cpp:1341-1355), then
+    rotated for output by the negated angles (UTR.cpp:656-661, 1387-1394);
+    compared with the oracle reconstruction passed through the oracle's
+    Rotate3 restatement."""
+    p, rows, imgs = setup_case(tmp_path, io, override={20: "2", 21: "Sym.txt", 28: "10 -20 35"})
+    with open(tmp_path / "Sym.txt", "w") as f:
+        f.write("0 0 0 0\n90 0 0 0\n180 0 0 0\n270 0 0 0\n")
+    out = run(tmp_path)
+    assert "Enforcing known symmetries" in out
+    ref = oracle.reconstruct(p, rows, imgs)
+    ref = oracle.symmetrize3(ref, np.array([0.0, 90, 180, 270]), np.zeros(4), np.zeros(4))
+    ref = oracle.rotate3(ref, -10.0, 20.0, -35.0)
+    vol = read_stack(io, tmp_path, "out/vol.grd", 64)
+    assert rel(vol, ref) < TOL

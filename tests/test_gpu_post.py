@@ -36,3 +36,47 @@ def test_highpass_rejects_bad_input():
         U.highpass3d(np.zeros((30, 30, 30), np.float32), 0.4, 0.4, 0.4, 2.0)
     with pytest.raises(U.UTRError, match="width"):
         U.highpass3d(np.zeros((16, 16, 16), np.float32), 0.4, 0.4, 0.4, 0.0)
+
+
+def smooth_volume(shape, seed):
+    rng = np.random.default_rng(seed)
+    z, y, x = np.meshgrid(*[np.arange(n, dtype=np.float64) for n in shape], indexing="ij")
+    v = np.zeros(shape)
+    for _ in range(6):
+        c = [rng.uniform(0.25, 0.75) * n for n in shape]
+        v += rng.uniform(0.5, 2) * np.exp(-((x - c[2]) ** 2 + (y - c[1]) ** 2 + (z - c[0]) ** 2) /
+                                          (2 * rng.uniform(1.5, 3) ** 2))
+    return (v + 0.05 * rng.standard_normal(shape)).astype(np.float32)
+
+
+@pytest.mark.parametrize("shape,angles", [((32, 32, 32), (30.0, 0.0, 0.0)),
+                                          ((32, 32, 32), (0.0, 90.0, 0.0)),
+                                          ((24, 40, 36), (20.0, 35.0, -50.0)),
+                                          ((40, 24, 32), (-110.0, 12.5, 200.0))])
+def test_rotate3d_matches_restatement(shape, angles):
+    """Rotate3 (parameter 28) on the GPU against the oracle's tricubic restatement."""
+    print("rotate3d", shape, angles, flush=True)
+    v = smooth_volume(shape, 3)
+    got = U.rotate3d(v, *angles)
+    ref = oracle.rotate3(v, *angles)
+    assert rel(got, ref) < 1e-6
+
+
+def test_symmetrize3d_matches_restatement():
+    """The symmetry average of UTR.cpp:1341-1355 (parameter 20 = 2)."""
+    v = smooth_volume((32, 32, 32), 4)
+    angZ = np.array([0.0, 120.0, 240.0, 0.0])
+    angY = np.array([0.0, 0.0, 0.0, 180.0])
+    angZ2 = np.array([0.0, 0.0, 0.0, 30.0])
+    got = U.symmetrize3d(v, angZ, angY, angZ2)
+    ref = oracle.symmetrize3(v, angZ, angY, angZ2)
+    assert rel(got, ref) < 1e-6
+    # one row: identity
+    assert np.array_equal(U.symmetrize3d(v, [0.0], [0.0], [0.0]), v)
+
+
+def test_rotate3d_rejects_bad_input():
+    with pytest.raises(U.UTRError, match="finite"):
+        U.rotate3d(np.zeros((8, 8, 8), np.float32), np.nan, 0, 0)
+    with pytest.raises(U.UTRError, match="nsym"):
+        U.symmetrize3d(np.zeros((8, 8, 8), np.float32), [], [], [])

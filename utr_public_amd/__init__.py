@@ -40,7 +40,8 @@ EXPORTS = [
     "utr_batch_create", "utr_batch_run", "utr_batch_job_form", "utr_batch_destroy",
     "dht_create", "dht_add_images", "dht_add_images_device", "dht_finalize", "dht_finalize_device",
     "dht_reset", "dht_dims", "dht_get_timings", "dht_destroy",
-    "utr_highpass3d", "utr_highpass3d_device",
+    "utr_highpass3d", "utr_highpass3d_device", "utr_rotate3d", "utr_rotate3d_device",
+    "utr_symmetrize3d", "utr_symmetrize3d_device",
 ]
 
 DHT_FIELDS = [
@@ -168,8 +169,13 @@ def load():
     L.dht_get_timings.argtypes = [vp, ctypes.POINTER(UtrTimings)]
     L.dht_destroy.argtypes = [vp]
     d = ctypes.c_double
+    i = ctypes.c_int
     for f in ("utr_highpass3d", "utr_highpass3d_device"):
-        getattr(L, f).argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, d, d, d, d]
+        getattr(L, f).argtypes = [vp, i, i, i, d, d, d, d]
+    L.utr_rotate3d.argtypes = [vp, i, i, i, d, d, d]
+    L.utr_rotate3d_device.argtypes = [vp, vp, i, i, i, d, d, d]
+    for f in ("utr_symmetrize3d", "utr_symmetrize3d_device"):
+        getattr(L, f).argtypes = [vp, i, i, i, i, dp, dp, dp]
     _lib = L
     return L
 
@@ -455,6 +461,28 @@ def highpass3d(vol, xst, yst, zst, width):
     out = np.ascontiguousarray(vol, dtype=np.float32).copy()
     nz, ny, nx = out.shape
     _check(load().utr_highpass3d(out.ctypes.data, nx, ny, nz, xst, yst, zst, width))
+    return out
+
+
+def rotate3d(vol, angZ, angY, angZ2):
+    """3D rotation (degrees) of a (nz, ny, nx) float32 volume on the GPU: the
+    content turns by Rz(angZ2) Ry(angY) Rz(angZ) about the array centre
+    (Rotate3 of UTR.cpp:1393, parameter 28).  Returns a new array."""
+    out = np.ascontiguousarray(vol, dtype=np.float32).copy()
+    nz, ny, nx = out.shape
+    _check(load().utr_rotate3d(out.ctypes.data, nx, ny, nz, float(angZ), float(angY), float(angZ2)))
+    return out
+
+
+def symmetrize3d(vol, angZ, angY, angZ2):
+    """Symmetry average of UTR.cpp:1341-1355 (parameter 20 = 2) on the GPU;
+    angZ, angY, angZ2: the rows of the symmetry file (degrees).  Returns a
+    new array."""
+    out = np.ascontiguousarray(vol, dtype=np.float32).copy()
+    nz, ny, nx = out.shape
+    a = [np.ascontiguousarray(x, dtype=np.float64) for x in (angZ, angY, angZ2)]
+    assert a[0].shape == a[1].shape == a[2].shape and a[0].ndim == 1
+    _check(load().utr_symmetrize3d(out.ctypes.data, nx, ny, nz, len(a[0]), *(_dp(x) for x in a)))
     return out
 
 

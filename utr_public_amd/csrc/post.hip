@@ -1,7 +1,11 @@
 // post.hip -- post-processing of the reconstructed volume on gfx950 (the
 // post-processing block of MultisliceCppTest/PhaseRetrieval/UTR.cpp, behind
-// the utr_highpass3d* C ABI of include/utr_hip.h).  Included by utr_api.hip
+// the utr_highpass3d*, utr_rotate3d* and utr_symmetrize3d* C ABI of
+// include/utr_hip.h).  Included by utr_api.hip
 // (one translation unit).
+//
+// 3D rotation (Rotate3, parameters 28 and 20 = 2): k_rotate3, restated
+// below with the symmetry average of UTR.cpp:1341-1355.
 //
 // High-pass filter (UTR.cpp:1357-1378, parameter 23): K3out -= G * K3out with
 // G the Gaussian of FWHM w (sigma = w / 2.355), applied in Fourier space as
@@ -87,6 +91,78 @@ __global__ void k_post_subtract(float* __restrict__ vol, const cplx<float>* __re
     if (t < n) vol[t] -= low[t].x;
 }
 
+// 3D rotation of a volume (XArray3DSpln::Rotate3, UTR.cpp:1349 and 1393;
+// the library is absent, restated): the content turns by
+// R = Rz(a2) Ry(b) Rz(a) about the array centre (nx/2, ny/2, nz/2), the
+// rotation pdb.exe applies to atoms (pdb_run.cpp:419-432), so
+// out(p) = in(R^T (p - c) + c); the value at a source position is the
+// tricubic convolution of Keys (1981, a = -1/2) of its 4 x 4 x 4
+// neighbours, 0 outside the volume.  minv: R^T row-major.  FP64, one
+// rounding to float.
+struct Rot3 {
+    double m[9];
+};
+
+__device__ __forceinline__ void keys_w64(double t, double w[4]) {
+    const double t2 = t * t, t3 = t2 * t;
+    w[0] = -0.5 * t3 + t2 - 0.5 * t;
+    w[1] = 1.5 * t3 - 2.5 * t2 + 1.0;
+    w[2] = -1.5 * t3 + 2.0 * t2 + 0.5 * t;
+    w[3] = 0.5 * t3 - 0.5 * t2;
+}
+
+__global__ void __launch_bounds__(256) k_rotate3(const float* __restrict__ in, float* __restrict__ out,
+                                                 int nx, int ny, int nz, Rot3 M) {
+    const size_t n = (size_t)nx * ny * nz;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int i = (int)(t % nx), j = (int)((t / nx) % ny), k = (int)(t / ((size_t)nx * ny));
+    const double cx = 0.5 * nx, cy = 0.5 * ny, cz = 0.5 * nz;
+    const double dx = i - cx, dy = j - cy, dz = k - cz;
+    const double q[3] = {cx + M.m[0] * dx + M.m[1] * dy + M.m[2] * dz,
+                         cy + M.m[3] * dx + M.m[4] * dy + M.m[5] * dz,
+                         cz + M.m[6] * dx + M.m[7] * dy + M.m[8] * dz};
+    // far outside: every neighbour is 0
+    if (!(q[0] > -2.0 && q[0] < nx + 1.0 && q[1] > -2.0 && q[1] < ny + 1.0 && q[2] > -2.0 && q[2] < nz + 1.0)) {
+        out[t] = 0.f;
+        return;
+    }
+    const double f[3] = {floor(q[0]), floor(q[1]), floor(q[2])};
+    double wx[4], wy[4], wz[4];
+    keys_w64(q[0] - f[0], wx);
+    keys_w64(q[1] - f[1], wy);
+    keys_w64(q[2] - f[2], wz);
+    const int ix = (int)f[0] - 1, iy = (int)f[1] - 1, iz = (int)f[2] - 1;
+    double acc = 0.0;
+    for (int c = 0; c < 4; c++) {
+        const int zz = iz + c;
+        if ((unsigned)zz >= (unsigned)nz) continue;
+        for (int b = 0; b < 4; b++) {
+            const int yy = iy + b;
+            if ((unsigned)yy >= (unsigned)ny) continue;
+            const float* rowp = in + ((size_t)zz * ny + yy) * nx;
+            const double wyz = wy[b] * wz[c];
+#pragma unroll
+            for (int a = 0; a < 4; a++) {
+                const int xx = ix + a;
+                if ((unsigned)xx < (unsigned)nx) acc += wx[a] * wyz * (double)rowp[xx];
+            }
+        }
+    }
+    out[t] = (float)acc;
+}
+
+// out += w * in (the symmetry sum), elementwise
+__global__ void k_post_axpy(float* __restrict__ out, const float* __restrict__ in, float w, size_t n) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) out[t] += w * in[t];
+}
+
+__global__ void k_post_scale(float* __restrict__ v, float w, size_t n) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) v[t] *= w;
+}
+
 }  // namespace utr
 
 namespace {
@@ -126,9 +202,106 @@ int post_fft_axis(utr::cplx<float>* d, int nx, int ny, int nz, int axis, int sig
     return UTR_OK;
 }
 
+// R^T of R = Rz(a2) Ry(b) Rz(a) (angles in degrees), the rotation of
+// pdb_run.cpp:419-432: Rz(a) (x, y) -> (x cos a - y sin a, x sin a + y cos a),
+// Ry(b) (x, z) -> (x cos b + z sin b, -x sin b + z cos b).
+utr::Rot3 rot3_inverse(double aZ_deg, double aY_deg, double aZ2_deg) {
+    const double d = 3.14159265358979323846 / 180.0;
+    const double ca = std::cos(aZ_deg * d), sa = std::sin(aZ_deg * d);
+    const double cb = std::cos(aY_deg * d), sb = std::sin(aY_deg * d);
+    const double cg = std::cos(aZ2_deg * d), sg = std::sin(aZ2_deg * d);
+    const double R[9] = {cg * cb * ca - sg * sa, -cg * cb * sa - sg * ca, cg * sb,
+                         sg * cb * ca + cg * sa, -sg * cb * sa + cg * ca, sg * sb,
+                         -sb * ca, sb * sa, cb};
+    utr::Rot3 M;
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) M.m[3 * r + c] = R[3 * c + r];
+    return M;
+}
+
+int check_vol(const void* p, int nx, int ny, int nz) {
+    if (!p) return set_err(UTR_E_INVALID, "null volume pointer");
+    if (nx <= 0 || ny <= 0 || nz <= 0) return set_err(UTR_E_INVALID, "volume sizes must be positive");
+    return UTR_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int utr_rotate3d_device(const float* d_in, float* d_out, int nx, int ny, int nz, double angZ_deg,
+                        double angY_deg, double angZ2_deg) {
+    int rc = check_vol(d_in, nx, ny, nz);
+    if (rc) return rc;
+    if (!d_out || d_out == d_in) return set_err(UTR_E_INVALID, "utr_rotate3d_device: output must be a distinct buffer");
+    if (!std::isfinite(angZ_deg) || !std::isfinite(angY_deg) || !std::isfinite(angZ2_deg))
+        return set_err(UTR_E_INVALID, "rotation angles must be finite");
+    const size_t n = (size_t)nx * ny * nz;
+    utr::k_rotate3<<<(unsigned)((n + 255) / 256), 256>>>(d_in, d_out, nx, ny, nz,
+                                                         rot3_inverse(angZ_deg, angY_deg, angZ2_deg));
+    UTR_HIP_TRY(hipGetLastError());
+    UTR_HIP_TRY(hipDeviceSynchronize());
+    return UTR_OK;
+}
+
+int utr_rotate3d(float* vol, int nx, int ny, int nz, double angZ_deg, double angY_deg, double angZ2_deg) {
+    int rc = check_vol(vol, nx, ny, nz);
+    if (rc) return rc;
+    const size_t n = (size_t)nx * ny * nz;
+    DevBuf<float> a, b;
+    UTR_HIP_TRY(a.alloc(n));
+    UTR_HIP_TRY(b.alloc(n));
+    UTR_HIP_TRY(hipMemcpy(a.p, vol, sizeof(float) * n, hipMemcpyHostToDevice));
+    rc = utr_rotate3d_device(a.p, b.p, nx, ny, nz, angZ_deg, angY_deg, angZ2_deg);
+    if (rc) return rc;
+    UTR_HIP_TRY(hipMemcpy(vol, b.p, sizeof(float) * n, hipMemcpyDeviceToHost));
+    return UTR_OK;
+}
+
+int utr_symmetrize3d_device(float* d_vol, int nx, int ny, int nz, int nsym, const double* angZ_deg,
+                            const double* angY_deg, const double* angZ2_deg) {
+    int rc = check_vol(d_vol, nx, ny, nz);
+    if (rc) return rc;
+    if (nsym < 1 || !angZ_deg || !angY_deg || !angZ2_deg)
+        return set_err(UTR_E_INVALID, "utr_symmetrize3d: need nsym >= 1 rows of angles");
+    for (int s = 0; s < nsym; s++)
+        if (!std::isfinite(angZ_deg[s]) || !std::isfinite(angY_deg[s]) || !std::isfinite(angZ2_deg[s]))
+            return set_err(UTR_E_INVALID, "rotation angles must be finite");
+    if (nsym == 1) return UTR_OK;
+    const size_t n = (size_t)nx * ny * nz;
+    DevBuf<float> sum, rot;
+    UTR_HIP_TRY(sum.alloc(n));
+    UTR_HIP_TRY(rot.alloc(n));
+    UTR_HIP_TRY(hipMemcpy(sum.p, d_vol, sizeof(float) * n, hipMemcpyDeviceToDevice));
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    // UTR.cpp:1341-1355: rows 1 .. nsym-1 turned back (Rotate3 with -Z'', -Y', -Z)
+    for (int s = 1; s < nsym; s++) {
+        utr::k_rotate3<<<blocks, 256>>>(d_vol, rot.p, nx, ny, nz,
+                                        rot3_inverse(-angZ2_deg[s], -angY_deg[s], -angZ_deg[s]));
+        UTR_HIP_TRY(hipGetLastError());
+        utr::k_post_axpy<<<blocks, 256>>>(sum.p, rot.p, 1.f, n);
+        UTR_HIP_TRY(hipGetLastError());
+    }
+    utr::k_post_scale<<<blocks, 256>>>(sum.p, 1.f / (float)nsym, n);
+    UTR_HIP_TRY(hipGetLastError());
+    UTR_HIP_TRY(hipMemcpy(d_vol, sum.p, sizeof(float) * n, hipMemcpyDeviceToDevice));
+    UTR_HIP_TRY(hipDeviceSynchronize());
+    return UTR_OK;
+}
+
+int utr_symmetrize3d(float* vol, int nx, int ny, int nz, int nsym, const double* angZ_deg,
+                     const double* angY_deg, const double* angZ2_deg) {
+    int rc = check_vol(vol, nx, ny, nz);
+    if (rc) return rc;
+    const size_t n = (size_t)nx * ny * nz;
+    DevBuf<float> a;
+    UTR_HIP_TRY(a.alloc(n));
+    UTR_HIP_TRY(hipMemcpy(a.p, vol, sizeof(float) * n, hipMemcpyHostToDevice));
+    rc = utr_symmetrize3d_device(a.p, nx, ny, nz, nsym, angZ_deg, angY_deg, angZ2_deg);
+    if (rc) return rc;
+    UTR_HIP_TRY(hipMemcpy(vol, a.p, sizeof(float) * n, hipMemcpyDeviceToHost));
+    return UTR_OK;
+}
 
 int utr_highpass3d_device(float* d_vol, int nx, int ny, int nz, double xst, double yst, double zst,
                           double width) {

@@ -185,9 +185,16 @@ int run(int argc, char** argv) {
 
     // features of the reference this build does not provide
     if (c.reprocess) unsupported("importing and reprocessing an existing 3D volume (parameter 33)");
-    if (c.symmetry_mode == 2) unsupported("post-applied symmetry (parameter 20 = 2)");
-    if (c.rot_out[0] != 0 || c.rot_out[1] != 0 || c.rot_out[2] != 0)
-        unsupported("rotation of the output volume (parameter 28)");
+    // symmetry rows of parameter 21 (UTR.cpp:501-506), read before the reconstruction
+    std::vector<double> symZ, symY, symZ2;
+    if (c.symmetry_mode == 2 && !c.symmetry_file.empty()) {
+        printf("\nReading symmetry enforcing file %s ...", c.symmetry_file.c_str());
+        for (const auto& o : read_orientations(c.symmetry_file)) {
+            symZ.push_back(o.angZ);
+            symY.push_back(o.angY);
+            symZ2.push_back(o.defocus.empty() ? 0.0 : o.defocus[0].first);
+        }
+    }
     if (astig) unsupported("astigmatism from .relionnew files");
     if ((c.ctf_mode == 2 || c.ctf_mode == 4) && !single_defocus) {
         printf("\n\nWARNING: different defocus distances and/or non-zero astigmatism have been detected in the input file %s.",
@@ -327,7 +334,13 @@ int run(int argc, char** argv) {
         write_stack(cg, S, fm, ext, ax, t_write);
     }
 
-    // post-processing (UTR.cpp:1357-1490)
+    // post-processing (UTR.cpp:1341-1490)
+    if (symZ.size() > 1) {
+        // known symmetry (UTR.cpp:1341-1355), on the GPU
+        printf("\nEnforcing known symmetries of the reconstructed 3D potential or beta ...");
+        check(utr_symmetrize3d(vol.v.data(), nx, ny, nz, (int)symZ.size(), symZ.data(), symY.data(),
+                               symZ2.data()));
+    }
     if (c.lp_filter > 0 && c.lp_filter < (c.xhi - c.xlo)) {
         // high-pass: the volume minus its Gaussian low-pass (UTR.cpp:1368-1374), on the GPU
         printf("\nHigh-pass filtering the reconstructed 3D object ...");
@@ -338,6 +351,12 @@ int run(int argc, char** argv) {
     if (c.background != 0 || vmin < c.threshold) {
         printf("\nThresholding the reconstructed 3D object ...");
         threshold_low(vol, (float)c.background, (float)c.threshold);
+    }
+    if (c.rot_out[0] != 0 || c.rot_out[1] != 0 || c.rot_out[2] != 0) {
+        // output rotation (UTR.cpp:656-661, 1387-1394): the angles as written, negated
+        const double aZ = -c.rot_out[0], aY = -c.rot_out[1], aZ2 = -c.rot_out[2];
+        printf("\nRotating around Z by %g (deg), around Y' by %g (deg) and around Z'' by %g (deg) ...", aZ, aY, aZ2);
+        check(utr_rotate3d(vol.v.data(), nx, ny, nz, aZ, aY, aZ2));
     }
     if (padded) {
         // trim back to the size of the input images (UTR.cpp:1406-1424)
