@@ -185,6 +185,18 @@ int utr_highpass3d_device(float* d_vol, int nx, int ny, int nz, double xst, doub
                           double width);
 int utr_highpass3d(float* vol, int nx, int ny, int nz, double xst, double yst, double zst, double width);
 
+/* Regularised inverse (-Laplacian) of a real volume, replacing
+ * fft3f.InverseMLaplacian(K3out, epsilon, epsilon) of UTR.cpp:1363-1367
+ * (parameter 22 with parameter 33): the 3D FFT times
+ * norm / (alpha + 4 pi^2 (kx^2/Lx^2 + ky^2/Ly^2 + kz^2/Lz^2)), L = n * step,
+ * as InverseMLaplacian3D (UTR.h:135-185) does in Fourier space, then the
+ * inverse FFT (Fftwd3frc's member is in the absent library; restated).
+ * alpha must be positive.  Edges must be supported FFT lengths. */
+int utr_invlaplace3d_device(float* d_vol, int nx, int ny, int nz, double xst, double yst, double zst,
+                            double alpha, double norm);
+int utr_invlaplace3d(float* vol, int nx, int ny, int nz, double xst, double yst, double zst, double alpha,
+                     double norm);
+
 /* 3D rotation of a volume, replacing XArray3DSpln::Rotate3(K3out, Z, Y', Z'')
  * of UTR.cpp:1387-1394 (parameter 28; the reference passes the angles
  * negated on reading, in radians -- here in degrees).  The content turns by

@@ -458,6 +458,21 @@ def highpass3d(vol, xst, yst, zst, width):
     return v - np.real(np.fft.ifftn(np.fft.fftn(v) * g))
 
 
+def invlaplace3d(vol, xst, yst, zst, alpha, norm):
+    """CPU restatement of fft3f.InverseMLaplacian(K3out, alpha, norm)
+    (UTR.cpp:1363-1367): the periodic 3D FFT times norm / (alpha +
+    4 pi^2 |q|^2), q in cycles per length unit -- the filter of
+    InverseMLaplacian3D (UTR.h:135-185) with apertures n * step -- then the
+    inverse FFT.  Fftwd3frc's member is absent (parity unpinned)."""
+    v = np.asarray(vol, dtype=np.float64)
+    nz, ny, nx = v.shape
+    qz = np.fft.fftfreq(nz, d=zst)[:, None, None]
+    qy = np.fft.fftfreq(ny, d=yst)[None, :, None]
+    qx = np.fft.fftfreq(nx, d=xst)[None, None, :]
+    f = norm / (alpha + 4.0 * np.pi ** 2 * (qx ** 2 + qy ** 2 + qz ** 2))
+    return np.real(np.fft.ifftn(np.fft.fftn(v) * f))
+
+
 def find_peaks(vol, extent, atom_xy, atom_z):
     """Restatement of FindPeaks.cpp:14-171 for small volumes (test oracle).
     Returns (peak volume, [(i, j, k, height), ...] in descending height).

@@ -333,3 +333,37 @@ def test_io_errors(io, tmp_path):
     assert io.utr_io_read_image(str(p).encode(), 0, 0, 0, 4, 0, None, 0, ctypes.byref(w),
                                 ctypes.byref(w)) == -1
     assert b"not a GRD file" in io.utr_io_last_error()
+
+
+def _names(io, tmpl, count):
+    buf = ctypes.create_string_buffer(512)
+    out = []
+    for i in range(count):
+        assert io.utr_io_file_name(tmpl.encode(), count, i, buf, 512) == 0
+        out.append(buf.value.decode())
+    return out
+
+
+def test_reprocess_reads_output_slices(work, io):
+    """Parameter 33 = 1 (UTR.cpp:729-749, 1282-1338, 1502-1512): the slices
+    named by the output template are read back (smaller ones padded with 0),
+    pass the post-processing (none here) and are written to the auxiliary
+    folder with "R" appended.  No GPU call is made on this path."""
+    rng = np.random.default_rng(4)
+    vol = rng.uniform(-1, 1, (64, 60, 62)).astype(np.float32)
+    for k, fn in enumerate(_names(io, str(work / "out" / "vol.grd"), 64)):
+        assert write(io, fn, vol[k], ext=(0, 62 * 0.4, 0, 60 * 0.4)) == 0
+    write_params(work / "UTR.txt", override={33: "1"})
+    rc, out = run_cli(work, "UTR.txt")
+    assert rc == 0, out
+    assert "Reading pre-existing 3D distribution" in out
+    got = np.stack([read(io, fn) for fn in _names(io, str(work / "aux" / "volR.grd"), 64)])
+    ref = np.zeros((64, 64, 64), np.float32)
+    ref[:, 2:62, 1:63] = vol
+    assert np.array_equal(got, ref)
+
+
+def test_reprocess_rejects_other_extensions(work):
+    write_params(work / "UTR.txt", override={33: "1"}, out="out/vol.png")
+    rc, out = run_cli(work, "UTR.txt")
+    assert rc == 1

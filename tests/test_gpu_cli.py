@@ -227,3 +227,38 @@ def test_cli_symmetry_and_output_rotation(tmp_path, io):
     ref = oracle.rotate3(ref, -10.0, 20.0, -35.0)
     vol = read_stack(io, tmp_path, "out/vol.grd", 64)
     assert rel(vol, ref) < TOL
+
+
+def test_cli_reprocess_with_filters(tmp_path, io):
+    """Parameter 33 = 1 with symmetry (20 = 2), the real-space inverse
+    Laplacian (22 = 1, UTR.cpp:1363-1367), the high-pass filter (23) and the
+    output rotation (28), in the reference's order, against the oracle's
+    restatements applied to the same imported volume."""
+    os.makedirs(tmp_path / "out", exist_ok=True)
+    os.makedirs(tmp_path / "aux", exist_ok=True)
+    with open(tmp_path / "orient.txt", "w") as f:
+        f.write("0 0 0 300\n0 90 0 300\n")
+    with open(tmp_path / "Sym.txt", "w") as f:
+        f.write("0 0 0 0\n90 0 0 0\n180 0 0 0\n270 0 0 0\n")
+    rng = np.random.default_rng(8)
+    z, y, x = np.meshgrid(*[np.arange(64.0)] * 3, indexing="ij")
+    vol = np.zeros((64, 64, 64))
+    for _ in range(5):
+        c = rng.uniform(20, 44, 3)
+        vol += np.exp(-((x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2) / 8.0)
+    vol = vol.astype(np.float32)
+    for k, fn in enumerate(names(io, str(tmp_path / "out" / "vol.grd"), 64)):
+        assert write(io, fn, vol[k], ext=(0, 64 * 0.4, 0, 64 * 0.4)) == 0
+    # epsilon (parameter 19) 0.5: with a tiny epsilon the inverse Laplacian
+    # leaves the mean times 1 and every other mode times < 2e-3, and the
+    # high-pass then subtracts that mean: an ill-conditioned comparison
+    write_params(tmp_path / "UTR.txt", override={33: "1", 22: "1", 20: "2", 21: "Sym.txt", 19: "0.5",
+                                                   23: "2.0 0.0 -1000000.0", 28: "10 -20 35"})
+    out = run(tmp_path)
+    assert "Inverse Laplace filtering" in out
+    ref = oracle.symmetrize3(vol, np.array([0.0, 90, 180, 270]), np.zeros(4), np.zeros(4))
+    ref = oracle.invlaplace3d(ref, 0.4, 0.4, 0.4, 0.5, 0.5)
+    ref = oracle.highpass3d(ref, 0.4, 0.4, 0.4, 2.0)
+    ref = oracle.rotate3(ref, -10.0, 20.0, -35.0)
+    got = read_stack(io, tmp_path, "aux/volR.grd", 64)
+    assert rel(got, ref) < TOL

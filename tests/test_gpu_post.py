@@ -80,3 +80,15 @@ def test_rotate3d_rejects_bad_input():
         U.rotate3d(np.zeros((8, 8, 8), np.float32), np.nan, 0, 0)
     with pytest.raises(U.UTRError, match="nsym"):
         U.symmetrize3d(np.zeros((8, 8, 8), np.float32), [], [], [])
+
+
+@pytest.mark.parametrize("shape,steps,alpha", [((32, 32, 32), (0.4, 0.4, 0.4), 1e-2),
+                                               ((48, 64, 32), (0.5, 0.3, 0.6), 0.5)])
+def test_invlaplace_matches_restatement(shape, steps, alpha):
+    """The real-space inverse Laplacian of the reprocess path (UTR.cpp:1363-1367)."""
+    v = smooth_volume(shape, 5)
+    got = U.invlaplace3d(v, steps[0], steps[1], steps[2], alpha, alpha)
+    ref = oracle.invlaplace3d(v, steps[0], steps[1], steps[2], alpha, alpha)
+    assert rel(got, ref) < 1e-5
+    with pytest.raises(U.UTRError, match="alpha"):
+        U.invlaplace3d(v, 0.4, 0.4, 0.4, 0.0, 1.0)

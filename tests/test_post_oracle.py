@@ -36,3 +36,16 @@ def test_gaussian_blob_low_pass_widens_as_convolution():
     st2 = s0 ** 2 + s ** 2
     expect = (s0 ** 2 / st2) ** 1.5 * np.exp(-r2 / (2 * st2))
     assert np.abs(low - expect).max() < 1e-9
+
+
+def test_invlaplace_closed_forms():
+    """norm / (alpha + 4 pi^2 q^2) on one Fourier mode and on a constant."""
+    n, st = 32, 0.5
+    z, y, x = np.meshgrid(*[np.arange(n) * st] * 3, indexing="ij")
+    L = n * st
+    mode = np.cos(2 * np.pi * 3 * x / L) * np.cos(2 * np.pi * 2 * z / L)
+    out = oracle.invlaplace3d(mode, st, st, st, 0.7, 1.3)
+    q2 = (3 / L) ** 2 + (2 / L) ** 2
+    assert np.allclose(out, mode * 1.3 / (0.7 + 4 * np.pi ** 2 * q2), atol=1e-12)
+    const = np.full((n, n, n), 2.0)
+    assert np.allclose(oracle.invlaplace3d(const, st, st, st, 0.5, 0.25), 1.0, atol=1e-12)

@@ -41,7 +41,7 @@ EXPORTS = [
     "dht_create", "dht_add_images", "dht_add_images_device", "dht_finalize", "dht_finalize_device",
     "dht_reset", "dht_dims", "dht_get_timings", "dht_destroy",
     "utr_highpass3d", "utr_highpass3d_device", "utr_rotate3d", "utr_rotate3d_device",
-    "utr_symmetrize3d", "utr_symmetrize3d_device",
+    "utr_symmetrize3d", "utr_symmetrize3d_device", "utr_invlaplace3d", "utr_invlaplace3d_device",
 ]
 
 DHT_FIELDS = [
@@ -172,6 +172,8 @@ def load():
     i = ctypes.c_int
     for f in ("utr_highpass3d", "utr_highpass3d_device"):
         getattr(L, f).argtypes = [vp, i, i, i, d, d, d, d]
+    for f in ("utr_invlaplace3d", "utr_invlaplace3d_device"):
+        getattr(L, f).argtypes = [vp, i, i, i, d, d, d, d, d]
     L.utr_rotate3d.argtypes = [vp, i, i, i, d, d, d]
     L.utr_rotate3d_device.argtypes = [vp, vp, i, i, i, d, d, d]
     for f in ("utr_symmetrize3d", "utr_symmetrize3d_device"):
@@ -461,6 +463,15 @@ def highpass3d(vol, xst, yst, zst, width):
     out = np.ascontiguousarray(vol, dtype=np.float32).copy()
     nz, ny, nx = out.shape
     _check(load().utr_highpass3d(out.ctypes.data, nx, ny, nz, xst, yst, zst, width))
+    return out
+
+
+def invlaplace3d(vol, xst, yst, zst, alpha, norm):
+    """Regularised inverse (-Laplacian) of a (nz, ny, nx) float32 volume on
+    the GPU (UTR.cpp:1363-1367).  Returns a new array."""
+    out = np.ascontiguousarray(vol, dtype=np.float32).copy()
+    nz, ny, nx = out.shape
+    _check(load().utr_invlaplace3d(out.ctypes.data, nx, ny, nz, xst, yst, zst, alpha, norm))
     return out
 
 

@@ -86,6 +86,25 @@ __global__ void k_post_gauss(cplx<float>* __restrict__ data, int nx, int ny, int
     data[t].y *= g;
 }
 
+// regularised inverse (-Laplacian): norm / (alpha + 4 pi^2 |q|^2) / N, with
+// dk = 4 pi^2 / aperture^2 per axis (UTR.h:135-185 in the wrapped layout)
+__global__ void k_post_invlap(cplx<float>* __restrict__ data, int nx, int ny, int nz, double dksi2,
+                              double deta2, double dzeta2, double alpha, double normN) {
+    const size_t n = (size_t)nx * ny * nz;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int i = (int)(t % nx), j = (int)((t / nx) % ny), k = (int)(t / ((size_t)nx * ny));
+    const double i1 = post_freq(i, nx), j1 = post_freq(j, ny), k1 = post_freq(k, nz);
+    const float f = (float)(normN / (dksi2 * i1 * i1 + deta2 * j1 * j1 + dzeta2 * k1 * k1 + alpha));
+    data[t].x *= f;
+    data[t].y *= f;
+}
+
+__global__ void k_post_real(float* __restrict__ vol, const cplx<float>* __restrict__ v, size_t n) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) vol[t] = v[t].x;
+}
+
 __global__ void k_post_subtract(float* __restrict__ vol, const cplx<float>* __restrict__ low, size_t n) {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) vol[t] -= low[t].x;
@@ -303,12 +322,14 @@ int utr_symmetrize3d(float* vol, int nx, int ny, int nz, int nsym, const double*
     return UTR_OK;
 }
 
-int utr_highpass3d_device(float* d_vol, int nx, int ny, int nz, double xst, double yst, double zst,
-                          double width) {
-    if (!d_vol) return set_err(UTR_E_INVALID, "null volume pointer");
-    if (nx <= 0 || ny <= 0 || nz <= 0 || !(xst > 0) || !(yst > 0) || !(zst > 0))
-        return set_err(UTR_E_INVALID, "utr_highpass3d: sizes and steps must be positive");
-    if (!(width > 0)) return set_err(UTR_E_INVALID, "utr_highpass3d: the filter width must be positive");
+}  // extern "C"
+
+namespace {
+
+// d_vol (real, nz*ny*nx) -> 3D FFT -> filter(scratch, blocks, stream) ->
+// inverse FFT -> d_vol -= Re (subtract) or d_vol = Re.
+template <typename F>
+int post_fourier(float* d_vol, int nx, int ny, int nz, F&& filter, bool subtract) {
     if (!fft_size_supported(nx) || !fft_size_supported(ny) || !fft_size_supported(nz))
         return set_err(UTR_E_UNSUPPORTED,
                        "volume edge not supported by the FFT kernels (supported: 16 32 48 64 96 128 "
@@ -334,18 +355,14 @@ int utr_highpass3d_device(float* d_vol, int nx, int ny, int nz, double xst, doub
             const int rc = post_fft_axis(scratch.p, nx, ny, nz, ax, -1, tws[ax]->p, s);
             if (rc) return rc;
         }
-        const double pi = 3.14159265358979323846;
-        const double sigma = width / 2.355;
-        const double k2 = 2.0 * pi * pi * sigma * sigma;
-        utr::k_post_gauss<<<blocks, 256, 0, s>>>(scratch.p, nx, ny, nz, k2 / ((nx * xst) * (nx * xst)),
-                                                 k2 / ((ny * yst) * (ny * yst)),
-                                                 k2 / ((nz * zst) * (nz * zst)), 1.0 / (double)n);
+        filter(scratch.p, blocks, s);
         UTR_HIP_TRY(hipGetLastError());
         for (int ax = 2; ax >= 0; ax--) {
             const int rc = post_fft_axis(scratch.p, nx, ny, nz, ax, +1, tws[ax]->p, s);
             if (rc) return rc;
         }
-        utr::k_post_subtract<<<blocks, 256, 0, s>>>(d_vol, scratch.p, n);
+        if (subtract) utr::k_post_subtract<<<blocks, 256, 0, s>>>(d_vol, scratch.p, n);
+        else utr::k_post_real<<<blocks, 256, 0, s>>>(d_vol, scratch.p, n);
         UTR_HIP_TRY(hipGetLastError());
         UTR_HIP_TRY(hipStreamSynchronize(s));
         return UTR_OK;
@@ -354,6 +371,58 @@ int utr_highpass3d_device(float* d_vol, int nx, int ny, int nz, double xst, doub
     (void)hipStreamSynchronize(s);
     (void)hipStreamDestroy(s);
     return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int utr_highpass3d_device(float* d_vol, int nx, int ny, int nz, double xst, double yst, double zst,
+                          double width) {
+    if (!d_vol) return set_err(UTR_E_INVALID, "null volume pointer");
+    if (nx <= 0 || ny <= 0 || nz <= 0 || !(xst > 0) || !(yst > 0) || !(zst > 0))
+        return set_err(UTR_E_INVALID, "utr_highpass3d: sizes and steps must be positive");
+    if (!(width > 0)) return set_err(UTR_E_INVALID, "utr_highpass3d: the filter width must be positive");
+    const size_t n = (size_t)nx * ny * nz;
+    const double pi = 3.14159265358979323846;
+    const double sigma = width / 2.355;
+    const double k2 = 2.0 * pi * pi * sigma * sigma;
+    return post_fourier(d_vol, nx, ny, nz, [&](utr::cplx<float>* d, unsigned blocks, hipStream_t s) {
+        utr::k_post_gauss<<<blocks, 256, 0, s>>>(d, nx, ny, nz, k2 / ((nx * xst) * (nx * xst)),
+                                                 k2 / ((ny * yst) * (ny * yst)),
+                                                 k2 / ((nz * zst) * (nz * zst)), 1.0 / (double)n);
+    }, true);
+}
+
+int utr_invlaplace3d_device(float* d_vol, int nx, int ny, int nz, double xst, double yst, double zst,
+                            double alpha, double norm) {
+    if (!d_vol) return set_err(UTR_E_INVALID, "null volume pointer");
+    if (nx <= 0 || ny <= 0 || nz <= 0 || !(xst > 0) || !(yst > 0) || !(zst > 0))
+        return set_err(UTR_E_INVALID, "utr_invlaplace3d: sizes and steps must be positive");
+    // UTR.h:148-149
+    if (!(alpha > 0)) return set_err(UTR_E_INVALID, "InverseMLaplacian3D() cannot be called with alpha <= 0");
+    const size_t n = (size_t)nx * ny * nz;
+    const double fact = 4.0 * 3.14159265358979323846 * 3.14159265358979323846;
+    const double xa = nx * xst, ya = ny * yst, za = nz * zst;
+    return post_fourier(d_vol, nx, ny, nz, [&](utr::cplx<float>* d, unsigned blocks, hipStream_t s) {
+        utr::k_post_invlap<<<blocks, 256, 0, s>>>(d, nx, ny, nz, fact / (xa * xa), fact / (ya * ya),
+                                                  fact / (za * za), alpha, norm / (double)n);
+    }, false);
+}
+
+int utr_invlaplace3d(float* vol, int nx, int ny, int nz, double xst, double yst, double zst, double alpha,
+                     double norm) {
+    if (!vol) return set_err(UTR_E_INVALID, "null volume pointer");
+    if (nx <= 0 || ny <= 0 || nz <= 0)
+        return set_err(UTR_E_INVALID, "utr_invlaplace3d: sizes and steps must be positive");
+    const size_t n = (size_t)nx * ny * nz;
+    DevBuf<float> d;
+    UTR_HIP_TRY(d.alloc(n));
+    UTR_HIP_TRY(hipMemcpy(d.p, vol, sizeof(float) * n, hipMemcpyHostToDevice));
+    const int rc = utr_invlaplace3d_device(d.p, nx, ny, nz, xst, yst, zst, alpha, norm);
+    if (rc) return rc;
+    UTR_HIP_TRY(hipMemcpy(vol, d.p, sizeof(float) * n, hipMemcpyDeviceToHost));
+    return UTR_OK;
 }
 
 int utr_highpass3d(float* vol, int nx, int ny, int nz, double xst, double yst, double zst, double width) {

@@ -183,8 +183,6 @@ int run(int argc, char** argv) {
         return 0;
     }
 
-    // features of the reference this build does not provide
-    if (c.reprocess) unsupported("importing and reprocessing an existing 3D volume (parameter 33)");
     // symmetry rows of parameter 21 (UTR.cpp:501-506), read before the reconstruction
     std::vector<double> symZ, symY, symZ2;
     if (c.symmetry_mode == 2 && !c.symmetry_file.empty()) {
@@ -195,8 +193,9 @@ int run(int argc, char** argv) {
             symZ2.push_back(o.defocus.empty() ? 0.0 : o.defocus[0].first);
         }
     }
-    if (astig) unsupported("astigmatism from .relionnew files");
-    if ((c.ctf_mode == 2 || c.ctf_mode == 4) && !single_defocus) {
+    // a feature of the reference this build does not provide
+    if (astig && !c.reprocess) unsupported("astigmatism from .relionnew files");
+    if (!c.reprocess && (c.ctf_mode == 2 || c.ctf_mode == 4) && !single_defocus) {
         printf("\n\nWARNING: different defocus distances and/or non-zero astigmatism have been detected in the input file %s.",
                c.defocus_file.c_str());
         printf("\n3D CTF correction have only been implemented for constant defocus distance and no astigmatism, possibly leading to inaccurate results in this case.");
@@ -204,6 +203,81 @@ int run(int argc, char** argv) {
             throw std::runtime_error("Error: stopped at the user's request.");
     }
 
+    Volume vol;
+    utr_timings tm{};
+    int nx = 0, ny = 0, nz = 0;
+    long nx0 = 0, ny0 = 0;
+    bool padded = false;
+    utr_ctx* ctx = nullptr;
+    struct Guard {
+        utr_ctx* c;
+        ~Guard() {
+            if (c) utr_destroy(c);
+        }
+    } guard{nullptr};
+    std::string out_template = c.output_template;
+    if (c.reprocess) {
+        // import the slices written by an earlier run (UTR.cpp:729-749, 1282-1338):
+        // file names and format from the output template, padded with 0 or
+        // trimmed to the volume's x-y size; outputs go to the auxiliary folder
+        // with "R" appended (UTR.cpp:1502-1512)
+        std::string tin = c.output_template;
+        std::string e = file_extension(tin);
+        if (e == ".DCM") {
+            printf("\nWARNING: input filename extension (in this mode - it is taken from the output filename template) must be TIF, GRD or RAW.");
+            const char* q = "\nPress 't' if you want to substitute TIFF extension for input files, 'g' for GRD, 'r' for RAW or any other symbol to exit: ";
+            int ch = 0;
+            if (isatty(0)) {
+                printf("%s", q);
+                fflush(stdout);
+                ch = getchar();
+            } else {
+                printf("%s(stdin is not a terminal)\n", q);
+            }
+            const char* ext = ch == 't' || ch == 'T' ? ".tif" : ch == 'g' || ch == 'G' ? ".grd"
+                            : ch == 'r' || ch == 'R' ? ".raw" : nullptr;
+            if (!ext) std::exit(1);
+            tin.replace(tin.find_last_of('.'), std::string::npos, ext);
+            e = file_extension(tin);
+        } else if (e != ".TIF" && e != ".TIFF" && e != ".GRD" && e != ".RAW") {
+            throw std::runtime_error("Error: input filename extension (in this mode - it is taken from the output filename template) must be TIF, GRD, DCM or RAW.");
+        }
+        nx = (int)c.nx;
+        ny = (int)c.ny;
+        // UTR.cpp:308-329
+        double zl = c.zlo, zh = c.zhi;
+        if (zl > zh) std::swap(zl, zh);
+        const long nzl = (long)((zh - zl) / c.xst + 0.5);
+        if (nzl <= 0) throw std::runtime_error("Error: number of z steps is not positive.");
+        if (nzl % 2) throw std::runtime_error("Error: number of z points must be even.");
+        nz = (int)nzl;
+        printf("\n\nReading pre-existing 3D distribution of the electrostatic potential or beta from files ...");
+        const auto t0 = Clock::now();
+        vol.nx = nx;
+        vol.ny = ny;
+        vol.nz = nz;
+        vol.v.assign((size_t)nx * ny * nz, 0.f);
+        const std::vector<std::string> vin = file_names(tin, nz);
+        std::string err;
+        omp_set_num_threads(c.threads);
+#pragma omp parallel for schedule(dynamic)
+        for (long k = 0; k < nz; k++) {
+            try {
+                Image im = read_image(c, vin[k]);
+                pad_or_trim(im, c.nx, c.ny, 0.0f);
+                std::copy(im.v.begin(), im.v.end(), vol.v.begin() + (size_t)k * nx * ny);
+            } catch (std::exception& ex) {
+#pragma omp critical
+                err = ex.what();
+            }
+        }
+        if (!err.empty()) throw std::runtime_error(err);
+        t_read += ms_since(t0);
+        std::string base = std::filesystem::path(c.output_template).filename().string();
+        base.insert(base.find_last_of('.'), "R");
+        out_template = c.aux_folder + base;
+        printf("\nOutput file name base for the renormalized 3D potential or beta = %s", out_template.c_str());
+    } else {
     // input file names (one image per orientation row)
     std::vector<std::string> names = file_names(c.input_template, nfull);
     if (!sel.empty()) {
@@ -244,13 +318,9 @@ int run(int argc, char** argv) {
         aZ2[i] = rows[i].defocus[0].first;
         dz[i] = rows[i].defocus[0].second;
     }
-    utr_ctx* ctx = nullptr;
     check(utr_create(&P, (int)nrows, aZ.data(), aY.data(), aZ2.data(), dz.data(),
                      weights.empty() ? nullptr : weights.data(), precision, &ctx));
-    struct Guard {
-        utr_ctx* c;
-        ~Guard() { utr_destroy(c); }
-    } guard{ctx};
+    guard.c = ctx;
     {
         // RELION image shifts, undone before the 2D stage (UTR.cpp:961-969)
         std::vector<double> sx(nrows), sy(nrows);
@@ -262,7 +332,6 @@ int run(int argc, char** argv) {
         }
         if (any) check(utr_set_image_shifts(ctx, sx.data(), sy.data()));
     }
-    int nx = 0, ny = 0, nz = 0;
     check(utr_dims(ctx, &nx, &ny, &nz));
     if (c.save_sampling == 1) check(utr_keep_sampling(ctx, 1));
 
@@ -272,8 +341,6 @@ int run(int argc, char** argv) {
     const long B = std::max<long>(16, std::min<long>(128, 2 * c.threads));
     const size_t n2 = (size_t)c.nx * c.ny;
     std::vector<float> batch(B * n2);
-    long nx0 = 0, ny0 = 0;
-    bool padded = false;
     for (long r0 = 0; r0 < nrows; r0 += B) {
         const long cnt = std::min(B, nrows - r0);
         const auto t0 = Clock::now();
@@ -309,17 +376,16 @@ int run(int argc, char** argv) {
         check(utr_add_images(ctx, (int)r0, (int)cnt, batch.data()));
     }
 
-    Volume vol;
     vol.nx = nx;
     vol.ny = ny;
     vol.nz = nz;
     vol.v.resize((size_t)nx * ny * nz);
     check(utr_finalize(ctx, vol.v.data()));
-    utr_timings tm{};
     check(utr_get_timings(ctx, &tm));
+    }
 
     double xlo = c.xlo, xhi = c.xhi, ylo = c.ylo, yhi = c.yhi, zlo = c.zlo, zhi = c.zhi;
-    if (c.save_sampling == 1) {
+    if (c.save_sampling == 1 && ctx) {
         Volume S = vol;
         check(utr_get_sampling(ctx, S.v.data()));
         std::string base = std::filesystem::path(c.output_template).filename().string();
@@ -340,6 +406,12 @@ int run(int argc, char** argv) {
         printf("\nEnforcing known symmetries of the reconstructed 3D potential or beta ...");
         check(utr_symmetrize3d(vol.v.data(), nx, ny, nz, (int)symZ.size(), symZ.data(), symY.data(),
                                symZ2.data()));
+    }
+    const double zst_v = (c.zhi - c.zlo) / (double)nz;
+    if (c.reprocess && c.inv_laplace) {
+        // UTR.cpp:1363-1367: the regularised inverse (-Laplacian) in real space, on the GPU
+        printf("\nInverse Laplace filtering the reconstructed 3D object ...");
+        check(utr_invlaplace3d(vol.v.data(), nx, ny, nz, c.xst, c.xst, std::fabs(zst_v), c.epsilon, c.epsilon));
     }
     if (c.lp_filter > 0 && c.lp_filter < (c.xhi - c.xlo)) {
         // high-pass: the volume minus its Gaussian low-pass (UTR.cpp:1368-1374), on the GPU
@@ -441,7 +513,7 @@ int run(int argc, char** argv) {
         vol = thick_slices(vol, c.thick_mode, (int)(c.thick_UL / c.xst + 0.5));
     }
     const double ext[3][2] = {{xlo, xhi}, {ylo, yhi}, {zlo, zhi}};
-    auto outnames = file_names(c.output_template, vol.nz);
+    auto outnames = file_names(out_template, vol.nz);
     printf("\n\nSaving the reconstructed 3D object into output files %s, etc. ...", outnames[0].c_str());
     write_stack(c, vol, outnames, ext, axes_of[c.reslice], t_write);
 
