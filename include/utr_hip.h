@@ -126,6 +126,14 @@ int utr_finalize_f64(utr_ctx* ctx, double* vol);
  * Replaces the XArray2DMove::Move call of UTR.cpp:961-969. */
 int utr_set_image_shifts(utr_ctx* ctx, const double* shift_x, const double* shift_y);
 
+/* Astigmatism per orientation row (defocus difference dz in UL and angle in
+ * degrees, as read from a .relionnew file into vastigm), used by the 2D CTF
+ * inversion of ctf_mode 3 (UTR.cpp:1011-1016; the other modes ignore it, as
+ * the reference does): the defocus at azimuth phi of the spatial frequency
+ * is zout + dz cos 2(phi - angle).  Restated (InvertCTF_DT2 is in the absent
+ * XArrayLibrary).  Call before any image is added. */
+int utr_set_astigmatism(utr_ctx* ctx, const double* dz, const double* angle_deg);
+
 /* Keep the sampling matrix Samp3 of the next utr_finalize in device memory
  * (parameter 32, UTR.cpp:706-712; off by default). */
 int utr_keep_sampling(utr_ctx* ctx, int on);

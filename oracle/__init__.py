@@ -79,6 +79,11 @@ def lib():
         L.ora_fft.argtypes = [dp, ctypes.c_int, ctypes.c_int]
         L.ora_angle_coeffs.argtypes = [ctypes.POINTER(OraGeom), ctypes.c_double,
                                        ctypes.c_double, dp]
+        L.ora_set_astigmatism.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                          ctypes.POINTER(ctypes.c_double)]
+        L.ora_set_astigmatism.restype = None
+        L.ora_spectrum_astig.argtypes = [ctypes.POINTER(OraGeom), fp, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
         L.ora_spectrum.argtypes = [ctypes.POINTER(OraGeom), fp, ctypes.c_double,
                                    ctypes.c_double, dp]
         L.ora_grid.argtypes = [ctypes.POINTER(OraGeom), dp, ctypes.c_double, ctypes.c_double,
@@ -144,10 +149,12 @@ def angle_coeffs(g, angZ_deg, angY_deg):
     return m.reshape(3, 3)
 
 
-def spectrum(g, img, zout=0.0, weight=1.0):
+def spectrum(g, img, zout=0.0, weight=1.0, astig=(0.0, 0.0)):
+    """2D stage of one image; astig = (dz UL, angle degrees) for mode 3."""
     img = np.ascontiguousarray(img, dtype=np.float32)
     K = np.zeros((g.ny, g.nx), dtype=np.complex128)
-    lib().ora_spectrum(ctypes.byref(g), _fp(img), zout, weight, _dp(K.view(np.float64)))
+    lib().ora_spectrum_astig(ctypes.byref(g), _fp(img), zout, weight, float(astig[0]), float(astig[1]),
+                             _dp(K.view(np.float64)))
     return K
 
 
@@ -339,9 +346,18 @@ def reconstruct(params, rows, images, weights=None, nthreads=1, samp=False):
     if weights is not None:
         w = np.ascontiguousarray(weights, dtype=np.float32)
     p = make_params(params)
-    rc = lib().ora_reconstruct(ctypes.byref(p), len(angZ), _dp(angZ), _dp(angY), _dp(angZ2),
-                               _dp(defoc), _fp(w) if w is not None else None, _fp(images),
-                               _dp(vol), _dp(S) if S is not None else None, nthreads)
+    adz = rows.get("astig_dz")
+    aang = rows.get("astig_angle")
+    if adz is not None:
+        adz = np.ascontiguousarray(adz, dtype=np.float64)
+        aang = np.ascontiguousarray(aang if aang is not None else np.zeros_like(adz), dtype=np.float64)
+        lib().ora_set_astigmatism(len(adz), _dp(adz), _dp(aang))
+    try:
+        rc = lib().ora_reconstruct(ctypes.byref(p), len(angZ), _dp(angZ), _dp(angY), _dp(angZ2),
+                                   _dp(defoc), _fp(w) if w is not None else None, _fp(images),
+                                   _dp(vol), _dp(S) if S is not None else None, nthreads)
+    finally:
+        lib().ora_set_astigmatism(0, None, None)
     if rc:
         raise OracleError(lib().ora_last_error().decode())
     return (vol, S) if samp else vol

@@ -299,8 +299,27 @@ void ora_angle_coeffs(const ora_geom* g, double angZ_deg, double angY_deg, doubl
 
 /* ------------------------------------------------------------ 2D stage */
 
+/* per-row astigmatism for the mode-3 filter (ora_set_astigmatism) */
+static const double* g_astig_dz = NULL;
+static const double* g_astig_ang = NULL;
+static int g_astig_n = 0;
+
+void ora_set_astigmatism(int n, const double* dz, const double* angle_deg) {
+    g_astig_n = (dz && angle_deg) ? n : 0;
+    g_astig_dz = dz;
+    g_astig_ang = angle_deg;
+}
+
+int ora_spectrum_astig(const ora_geom* g, const float* img, double zout, double weight,
+                       double adz, double aang_deg, double* K);
+
 int ora_spectrum(const ora_geom* g, const float* img, double zout, double weight,
                  double* K) {
+    return ora_spectrum_astig(g, img, zout, weight, 0.0, 0.0, K);
+}
+
+int ora_spectrum_astig(const ora_geom* g, const float* img, double zout, double weight,
+                       double adz, double aang_deg, double* K) {
     const int nx = g->nx, ny = g->ny;
     /* UTR.cpp:951-958 and Log0 (UTR.cpp:991) */
     const float f1 = (float)g->norm_f1, f2 = (float)g->norm_f2;
@@ -359,12 +378,19 @@ int ora_spectrum(const ora_geom* g, const float* img, double zout, double weight
         double fac5 = ORA_PI * pow(g->wl, 5) / 3.0 * g->cs5;
         int bC35 = (g->cs3 != 0) || (g->cs5 != 0);
         int aper = g->k2maxo > 0;
+        /* astigmatism (UTR.cpp:1013, restated): defocus zout + dz cos 2(phi - phi0)
+         * at azimuth phi of (qx, qy), i.e. chi += pi wl dz ((qx^2 - qy^2) cos 2phi0
+         * + 2 qx qy sin 2phi0) */
+        double a2 = 2.0 * aang_deg * ORA_PI / 180.0;
+        double adc = ORA_PI * g->wl * adz * cos(a2), ads = ORA_PI * g->wl * adz * sin(a2);
+        double dxy = 1.0 / (xap * yap);
         for (int ky = 0; ky < ny; ky++)
             for (int kx = 0; kx < nx; kx++) {
                 double j1 = ky - g->nyd2, i1 = kx - g->nxd2;
                 double q2 = dk2 * i1 * i1 + de2 * j1 * j1;
                 if (aper && !(q2 < g->k2maxo)) continue;
                 double chi = fac2 * q2 - omega;
+                if (adz != 0) chi += adc * (dk2 * i1 * i1 - de2 * j1 * j1) + 2.0 * ads * dxy * i1 * j1;
                 if (bC35) chi += fac3 * q2 * q2 + fac5 * q2 * q2 * q2;
                 double sn = sin(chi), cs = cos(chi);
                 double sr = sn / (sn * sn + eps);
@@ -679,8 +705,9 @@ int ora_reconstruct(const ora_params* p, int nrows, const double* angZ_deg,
         double* K = (double*)malloc(sizeof(double) * 2 * n2);
         for (int r = 0; r < nrows; r++) {
             double a = now_ms();
-            ora_spectrum(&g, images + (size_t)r * n2, defocus[r] - g.dzextra,
-                         weights ? (double)weights[r] : 1.0, K);
+            ora_spectrum_astig(&g, images + (size_t)r * n2, defocus[r] - g.dzextra,
+                           weights ? (double)weights[r] : 1.0, r < g_astig_n ? g_astig_dz[r] : 0.0,
+                           r < g_astig_n ? g_astig_ang[r] : 0.0, K);
             double b = now_ms();
             ora_grid(&g, K, angZ_deg[r], angY_deg[r], V, S);
             t_grid += now_ms() - b;
@@ -702,8 +729,9 @@ int ora_reconstruct(const ora_params* p, int nrows, const double* angZ_deg,
             double* K = (double*)malloc(sizeof(double) * 2 * n2);
 #pragma omp for schedule(static)
             for (int r = 0; r < nrows; r++) {
-                ora_spectrum(&g, images + (size_t)r * n2, defocus[r] - g.dzextra,
-                             weights ? (double)weights[r] : 1.0, K);
+                ora_spectrum_astig(&g, images + (size_t)r * n2, defocus[r] - g.dzextra,
+                               weights ? (double)weights[r] : 1.0, r < g_astig_n ? g_astig_dz[r] : 0.0,
+                               r < g_astig_n ? g_astig_ang[r] : 0.0, K);
                 ora_grid(&g, K, angZ_deg[r], angY_deg[r], Vt, St);
             }
             free(K);

@@ -97,6 +97,15 @@ void ora_angle_coeffs(const ora_geom* g, double angZ_deg, double angY_deg,
 /* Per-row 2D stage (UTR.cpp:951-1019): renormalise, clamp, ln, centred 2D
  * DFT, optional 2D filter (modes 1/3), subseries weight.  K is ny*nx
  * interleaved complex, centred (zero frequency at (ny/2, nx/2)). */
+/* Per-row astigmatism (defocus difference dz in UL, angle in degrees) used
+ * by the mode-3 filter of the following ora_reconstruct calls; n = 0 or NULL
+ * arrays clear it.  The arrays must stay valid while it is set. */
+void ora_set_astigmatism(int n, const double* dz, const double* angle_deg);
+
+/* ora_spectrum with astigmatism (dz UL, angle degrees) in the mode-3 filter. */
+int ora_spectrum_astig(const ora_geom* g, const float* img, double zout, double weight,
+                       double adz, double aang_deg, double* K);
+
 int ora_spectrum(const ora_geom* g, const float* img, double zout,
                  double weight, double* K);
 

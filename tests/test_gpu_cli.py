@@ -262,3 +262,22 @@ def test_cli_reprocess_with_filters(tmp_path, io):
     ref = oracle.rotate3(ref, -10.0, 20.0, -35.0)
     got = read_stack(io, tmp_path, "aux/volR.grd", 64)
     assert rel(got, ref) < TOL
+
+
+def test_cli_relion_astigmatism_mode3(tmp_path, io):
+    """Astigmatism of .relionnew rows (defocus U/V and angle) reaches the
+    mode-3 2D CTF inversion."""
+    nrows = 12
+    p, rows, imgs = setup_case(tmp_path, io, nrows=nrows, orient="o.relionnew", mode=3, asigma=0.08)
+    p = dict(p, ctf_mode=3, asigma=0.08, epsilon=0.0001)
+    rng = np.random.default_rng(12)
+    ang = rng.uniform(0, 180, nrows)
+    with open(tmp_path / "o.relionnew", "w") as f:
+        for i, a in enumerate(rows["angY"]):
+            # defocus U = 430, V = 370: mean 400, dz = 30; every number but the angle changes sign
+            f.write("%d img%02d.mrc 0 0 -430 -370 %.10g 0 %.10g 0\n" % (i + 1, i, ang[i], -a))
+    run(tmp_path)
+    ref = oracle.reconstruct(p, dict(rows, defocus=np.full(nrows, 400.0), astig_dz=np.full(nrows, 30.0),
+                                     astig_angle=ang), imgs)
+    vol = read_stack(io, tmp_path, "out/vol.grd", 64)
+    assert rel(vol, ref) < TOL

@@ -121,3 +121,23 @@ def test_turned_projections_reconstruct_like_plain_ones():
     print("turned vs plain: corrected %.3f, opposite sense %.3f" % (e_fixed, e_wrong), flush=True)
     assert e_fixed < 0.05
     assert e_wrong > 0.5
+
+
+@pytest.mark.parametrize("precision", [U.UTR_FP32, U.UTR_FP64])
+def test_mode3_astigmatism_parity(precision):
+    """Astigmatic rows in the mode-3 2D CTF inversion (utr_set_astigmatism)."""
+    n, nrows, xst = 64, 16, 0.4
+    p = U.default_params(nx=n, ny=n, xst=xst, zlo=-n * xst / 2, zhi=n * xst / 2, ctf_mode=3, escc=1,
+                         asigma=0.08, epsilon=0.05, nsr=0.05)
+    rng = np.random.default_rng(17)
+    angY = np.arange(nrows) * (180.0 / nrows)
+    rows = dict(angZ=np.zeros(nrows), angY=angY, defocus=np.full(nrows, 400.0),
+                astig_dz=rng.uniform(-80, 80, nrows), astig_angle=rng.uniform(0, 180, nrows))
+    xyz, Z = synth.atoms(17, 40, "cube", n * xst * 0.35, 1.4, ([6], [1.0]))
+    imgs = synth.project(xyz, Z, n, n, xst, rows["angZ"], angY, dose=400.0, seed=17)
+    ref = oracle.reconstruct(p, rows, imgs)
+    got = U.reconstruct(p, rows, imgs, precision)
+    plain = oracle.reconstruct(p, dict(rows, astig_dz=np.zeros(nrows)), imgs)
+    print("astigmatism: rel-L2 %.2e (vs no astigmatism %.2e)" % (rel(got, ref), rel(plain, ref)), flush=True)
+    assert rel(got, ref) < TOL
+    assert rel(plain, ref) > 100 * TOL

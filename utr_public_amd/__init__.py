@@ -34,7 +34,7 @@ PARAM_FIELDS = [
 EXPORTS = [
     "utr_create", "utr_add_images", "utr_add_images_device", "utr_finalize",
     "utr_finalize_device", "utr_finalize_f64", "utr_keep_sampling", "utr_get_sampling",
-    "utr_set_image_shifts",
+    "utr_set_image_shifts", "utr_set_astigmatism",
     "utr_reset", "utr_dims", "utr_get_timings", "utr_destroy", "utr_last_error",
     "utr_abi_version", "utr_angle_coeffs", "utr_get_spectrum", "utr_get_fourier_volume",
     "utr_batch_create", "utr_batch_run", "utr_batch_job_form", "utr_batch_destroy",
@@ -142,6 +142,7 @@ def load():
     L.utr_finalize_f64.argtypes = [vp, vp]
     L.utr_keep_sampling.argtypes = [vp, ctypes.c_int]
     L.utr_set_image_shifts.argtypes = [vp, dp, dp]
+    L.utr_set_astigmatism.argtypes = [vp, dp, dp]
     L.utr_get_sampling.argtypes = [vp, vp]
     L.utr_reset.argtypes = [vp]
     L.utr_dims.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
@@ -203,7 +204,8 @@ class Reconstructor:
 
     rows: dict with arrays angZ, angY (degrees), defocus (UL), optional angZ2
     and weights -- the columns of the orientation file read at UTR.cpp:122 --
-    and optional shift_x, shift_y (UL, the image shifts of a .relionnew file).
+    and optional shift_x, shift_y (UL, the image shifts of a .relionnew file)
+    and astig_dz, astig_angle (its astigmatism, used by ctf_mode 3).
     """
 
     def __init__(self, params, rows, precision=UTR_FP32):
@@ -235,6 +237,11 @@ class Reconstructor:
             sy = np.ascontiguousarray(rows.get("shift_y", z), dtype=np.float64)
             assert sx.shape == (self.nrows,) and sy.shape == (self.nrows,)
             _check(L.utr_set_image_shifts(ctx, _dp(sx), _dp(sy)))
+        if rows.get("astig_dz") is not None:
+            adz = np.ascontiguousarray(rows["astig_dz"], dtype=np.float64)
+            aang = np.ascontiguousarray(rows.get("astig_angle", np.zeros(self.nrows)), dtype=np.float64)
+            assert adz.shape == (self.nrows,) and aang.shape == (self.nrows,)
+            _check(L.utr_set_astigmatism(ctx, _dp(adz), _dp(aang)))
         nx, ny, nz = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         _check(L.utr_dims(ctx, ctypes.byref(nx), ctypes.byref(ny), ctypes.byref(nz)))
         self.shape = (nz.value, ny.value, nx.value)

@@ -619,6 +619,26 @@ int utr_set_image_shifts(utr_ctx* c, const double* shift_x, const double* shift_
     return alloc_prep(c);
 }
 
+int utr_set_astigmatism(utr_ctx* c, const double* dz, const double* angle_deg) {
+    if (!c) return set_err(UTR_E_INVALID, "null context");
+    if (!dz || !angle_deg) return set_err(UTR_E_INVALID, "null astigmatism array");
+    if (c->n_added) return set_err(UTR_E_STATE, "utr_set_astigmatism after images were added");
+    auto& P = c->plan;
+    for (int i = 0; i < P.nrows; i++)
+        if (!std::isfinite(dz[i]) || !std::isfinite(angle_deg[i]))
+            return set_err(UTR_E_INVALID, "astigmatism values must be finite");
+    const double pi = 3.14159265358979323846;
+    for (int i = 0; i < P.nrows; i++) {
+        // the restated mode-3 filter (UTR.cpp:1013): chi += pi wl dz ((qx^2 - qy^2) cos 2phi0 + 2 qx qy sin 2phi0)
+        const double a2 = 2.0 * angle_deg[i] * pi / 180.0;
+        P.rows[i].astig_c = pi * P.g.wl * dz[i] * std::cos(a2);
+        P.rows[i].astig_s = pi * P.g.wl * dz[i] * std::sin(a2);
+    }
+    UTR_HIP_TRY(hipStreamSynchronize(c->stream));
+    UTR_HIP_TRY(hipMemcpy(c->rows.p, P.rows.data(), sizeof(utr::RowParams) * P.nrows, hipMemcpyHostToDevice));
+    return UTR_OK;
+}
+
 int utr_keep_sampling(utr_ctx* c, int on) {
     if (!c) return set_err(UTR_E_INVALID, "null context");
     c->keep_sampling = on ? 1 : 0;

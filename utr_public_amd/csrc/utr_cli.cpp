@@ -193,9 +193,8 @@ int run(int argc, char** argv) {
             symZ2.push_back(o.defocus.empty() ? 0.0 : o.defocus[0].first);
         }
     }
-    // a feature of the reference this build does not provide
-    if (astig && !c.reprocess) unsupported("astigmatism from .relionnew files");
-    if (!c.reprocess && (c.ctf_mode == 2 || c.ctf_mode == 4) && !single_defocus) {
+    // UTR.cpp:758-766: astigmatism only enters mode 3; modes 2 and 4 warn
+    if (!c.reprocess && (c.ctf_mode == 2 || c.ctf_mode == 4) && (!single_defocus || astig)) {
         printf("\n\nWARNING: different defocus distances and/or non-zero astigmatism have been detected in the input file %s.",
                c.defocus_file.c_str());
         printf("\n3D CTF correction have only been implemented for constant defocus distance and no astigmatism, possibly leading to inaccurate results in this case.");
@@ -331,6 +330,15 @@ int run(int argc, char** argv) {
             any |= sx[i] != 0 || sy[i] != 0;
         }
         if (any) check(utr_set_image_shifts(ctx, sx.data(), sy.data()));
+    }
+    if (astig) {
+        // astigmatism of .relionnew rows (UTR.cpp:1013, mode 3)
+        std::vector<double> adz(nrows), aang(nrows);
+        for (long i = 0; i < nrows; i++) {
+            adz[i] = rows[i].astig_dz;
+            aang[i] = rows[i].astig_angle;
+        }
+        check(utr_set_astigmatism(ctx, adz.data(), aang.data()));
     }
     check(utr_dims(ctx, &nx, &ny, &nz));
     if (c.save_sampling == 1) check(utr_keep_sampling(ctx, 1));

@@ -323,6 +323,10 @@ __global__ void __launch_bounds__(256) k_cols(cplx<T>* __restrict__ spec, int nx
                         eps = 0.1 * (fabs(fac2) * mn) * (fabs(fac2) * mn);
                     }
                     double chi = fac2 * q2 - atan(g.asigma);
+                    // astigmatism: defocus zout + dz cos 2(phi - phi0) at azimuth phi
+                    if (R.astig_c != 0 || R.astig_s != 0)
+                        chi += R.astig_c * (dk2 * i1 * i1 - de2 * j1 * j1) +
+                               2.0 * R.astig_s * (1.0 / (xap * yap)) * i1 * j1;
                     if (g.cs3 != 0 || g.cs5 != 0)
                         chi += pi * g.wl * g.wl * g.wl / 2.0 * g.cs3 * q2 * q2 +
                                pi * g.wl * g.wl * g.wl * g.wl * g.wl / 3.0 * g.cs5 * q2 * q2 * q2;

@@ -51,6 +51,7 @@ struct alignas(16) RowParams {
     double m[9];
     double ph[3];      // centre-shift phase per grid offset (radians)
     double zout;       // defocus of this row minus the offset (UL)
+    double astig_c, astig_s;  // pi wl dz cos 2phi0, pi wl dz sin 2phi0: astigmatism (mode 3)
     float minv[9];
     float wa, wb, wz;  // bounds of |da|,|db|,|dzz| over the unit infinity ball
     float weight;      // subseries weight (1 if none)
