@@ -299,10 +299,19 @@ typedef struct dht_ctx dht_ctx;
 
 /* Replaces the set-up of DHT.cpp:36-720 (validation, geometry) and the 3D
  * array allocation of DHT.cpp:716-740.  Rows as for utr_create (one defocus
- * distance per orientation, Z'' must be zero); weights may be NULL. */
+ * distance per orientation; Z'' angles through dht_set_image_correction);
+ * weights may be NULL. */
 int dht_create(const dht_params* params, int nrows, const double* angZ_deg,
                const double* angY_deg, const double* defocus, const float* weights,
                dht_ctx** out);
+
+/* Z'' angles (degrees) and RELION image shifts (Angstrom) per row, undone on
+ * each renormalised input image before its 2D stage (the Move / Rotate block
+ * of DHT.cpp:862-880, restated as for utr_create's angZ2 and
+ * utr_set_image_shifts).  Any pointer may be NULL (zeros).  Call before any
+ * image is added. */
+int dht_set_image_correction(dht_ctx* ctx, const double* angZ2_deg, const double* shift_x,
+                             const double* shift_y);
 
 /* The per-orientation phase retrieval input (DHT.cpp:787-947): images of rows
  * [first_row, first_row+count), count*ny*nx floats in host / device memory. */

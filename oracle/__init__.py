@@ -447,6 +447,9 @@ def dht_reconstruct(params, rows, images, weights=None, nthreads=1):
     angY = np.ascontiguousarray(rows["angY"], dtype=np.float64)
     defoc = np.ascontiguousarray(rows["defocus"], dtype=np.float64)
     images = np.ascontiguousarray(images, dtype=np.float32)
+    if any(np.any(np.asarray(rows.get(k, 0.0)) != 0) for k in ("angZ2", "shift_x", "shift_y")):
+        # DHT.cpp:862-880: the same correction as UTR.cpp:960-977
+        images = np.ascontiguousarray(prep_images(params, rows, images))
     nz = dht_nz(params)
     vol = np.zeros((nz,) + images.shape[1:])
     w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float32)

@@ -149,3 +149,23 @@ def test_plane_and_brick_splat_agree(name):
     assert np.array_equal(plane.view(np.uint32), brick.view(np.uint32))
     ref = oracle.dht_reconstruct(p, rows, imgs)
     assert rel(plane, ref) < TOL
+
+
+@pytest.mark.parametrize("method", [3, 4, 5])
+def test_z2_rotation_and_shifts(method):
+    """Z'' angles and RELION shifts undone on the input images
+    (dht_set_image_correction, DHT.cpp:862-880) in every method."""
+    kw = {3: dict(dzextra=60.0, epsilon=1.0), 4: dict(dzextra=0.0, asigma=0.06, epsilon=0.02),
+          5: dict(dzextra=0.0, asigma=0.05, epsilon=0.01, critical_snr=0.5)}[method]
+    p, rows, imgs = case(method=method, escc=1, seed=40 + method, general=False, **kw)
+    rng = np.random.default_rng(method)
+    n = len(rows["angY"])
+    rows = dict(rows, angZ2=rng.uniform(-40, 40, n), shift_x=rng.uniform(-1, 1, n),
+                shift_y=rng.uniform(-1, 1, n))
+    rows["angZ2"][::3] = 0.0
+    print("dht z2 case: method %d" % method, flush=True)
+    ref = oracle.dht_reconstruct(p, rows, imgs)
+    got = U.dht_reconstruct(p, rows, imgs)
+    e = rel(got, ref)
+    print("  rel-L2 %.2e" % e, flush=True)
+    assert e < TOL

@@ -39,7 +39,7 @@ EXPORTS = [
     "utr_abi_version", "utr_angle_coeffs", "utr_get_spectrum", "utr_get_fourier_volume",
     "utr_batch_create", "utr_batch_run", "utr_batch_job_form", "utr_batch_destroy",
     "dht_create", "dht_add_images", "dht_add_images_device", "dht_finalize", "dht_finalize_device",
-    "dht_reset", "dht_dims", "dht_get_timings", "dht_destroy",
+    "dht_reset", "dht_dims", "dht_get_timings", "dht_destroy", "dht_set_image_correction",
     "utr_highpass3d", "utr_highpass3d_device", "utr_rotate3d", "utr_rotate3d_device",
     "utr_symmetrize3d", "utr_symmetrize3d_device", "utr_invlaplace3d", "utr_invlaplace3d_device",
 ]
@@ -169,6 +169,7 @@ def load():
                            ctypes.POINTER(ctypes.c_int)]
     L.dht_get_timings.argtypes = [vp, ctypes.POINTER(UtrTimings)]
     L.dht_destroy.argtypes = [vp]
+    L.dht_set_image_correction.argtypes = [vp, dp, dp, dp]
     d = ctypes.c_double
     i = ctypes.c_int
     for f in ("utr_highpass3d", "utr_highpass3d_device"):
@@ -390,7 +391,8 @@ class Batch:
 class DhtReconstructor:
     """One reconstruction of the DHT program's single-defocus methods (3, 4,
     5; MultisliceCppTest/PhaseRetrieval/DHT.cpp) on the current HIP device.
-    rows: angZ, angY (degrees), defocus (Angstrom), optional weights."""
+    rows: angZ, angY (degrees), defocus (Angstrom), optional weights, angZ2
+    (Z'' in degrees) and shift_x, shift_y (RELION image shifts, Angstrom)."""
 
     def __init__(self, params, rows):
         L = load()
@@ -411,12 +413,12 @@ class DhtReconstructor:
                             self.weights.ctypes.data if self.weights is not None else None,
                             ctypes.byref(ctx)))
         self._ctx = ctx
-        if rows.get("shift_x") is not None or rows.get("shift_y") is not None:
-            z = np.zeros(self.nrows)
-            sx = np.ascontiguousarray(rows.get("shift_x", z), dtype=np.float64)
-            sy = np.ascontiguousarray(rows.get("shift_y", z), dtype=np.float64)
-            assert sx.shape == (self.nrows,) and sy.shape == (self.nrows,)
-            _check(L.utr_set_image_shifts(ctx, _dp(sx), _dp(sy)))
+        if any(rows.get(k) is not None for k in ("angZ2", "shift_x", "shift_y")):
+            z = np.zeros(len(self.angZ))
+            cols = [np.ascontiguousarray(rows.get(k) if rows.get(k) is not None else z, dtype=np.float64)
+                    for k in ("angZ2", "shift_x", "shift_y")]
+            assert all(c.shape == z.shape for c in cols)
+            _check(L.dht_set_image_correction(ctx, *(_dp(c) for c in cols)))
         nx, ny, nz = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         _check(L.dht_dims(ctx, ctypes.byref(nx), ctypes.byref(ny), ctypes.byref(nz)))
         self.shape = (nz.value, ny.value, nx.value)

@@ -702,6 +702,10 @@ struct dht_ctx {
     DevBuf<float> P, stage, vol;
     DevBuf<unsigned> pmax;
     int all_z0 = 0;  // no orientation rotates about Z (y-plane splat applies)
+    // input image correction (Z'' rotation, RELION shift; DHT.cpp:862-880)
+    std::vector<utr::PrepRow> prep;
+    DevBuf<utr::PrepRow> prep_d;
+    DevBuf<float> img_prep, prep_aver;
 };
 
 namespace {
@@ -845,7 +849,7 @@ int dht_create_sctf(const dht_params* p, int nrows, const double* angZ, const do
     return UTR_OK;
 }
 
-int dht_add_rows(dht_ctx* c, int first_row, int count, const float* d_img) {
+int dht_add_plain(dht_ctx* c, int first_row, int count, const float* d_img) {
     const auto& g = c->g;
     UTR_HIP_TRY(hipEventRecord(c->ev[0], c->stream));
     bool ok = false;
@@ -879,6 +883,41 @@ int dht_add_rows(dht_ctx* c, int first_row, int count, const float* d_img) {
     for (int r = 0; r < count; r++) {
         if (!c->added[first_row + r]) c->n_added++;
         c->added[first_row + r] = 1;
+    }
+    return UTR_OK;
+}
+
+// The rows' images through the input image correction where a row needs it
+// (the kernels of the UTR path with the DHT's renormalisation), in passes of
+// kPrepRows images, then the 2D stage.
+int dht_add_rows(dht_ctx* c, int first_row, int count, const float* d_img) {
+    bool any = false;
+    for (int r = 0; r < count && !c->prep.empty(); r++) any |= c->prep[first_row + r].on != 0;
+    if (!any) return dht_add_plain(c, first_row, count, d_img);
+    const auto& g = c->g;
+    utr::GeomDev gp{};
+    gp.nx = g.nx;
+    gp.ny = g.ny;
+    gp.use_f1 = g.use_f1;
+    gp.use_f2 = g.use_f2;
+    gp.f1 = g.f1;
+    gp.f2 = g.f2;
+    const size_t n2 = (size_t)g.nx * g.ny;
+    // rotation centre (DHT.cpp:820) in pixels
+    const double px = (g.xc - g.xlo) / g.xst, py = (g.yc - g.ylo) / g.xst;
+    for (int r = 0; r < count; r += kPrepRows) {
+        const int k = std::min(kPrepRows, count - r);
+        if (c->img_prep.n < n2 * k || c->prep_aver.n < (size_t)k)
+            return set_err(UTR_E_STATE, "image correction buffers not allocated");
+        utr::k_prep_aver<<<k, 256, 0, c->stream>>>(d_img + r * n2, c->prep_d.p, first_row + r, gp,
+                                                   c->prep_aver.p);
+        UTR_HIP_TRY(hipGetLastError());
+        const dim3 grid((unsigned)((n2 + 255) / 256), (unsigned)k);
+        utr::k_prep_img<<<grid, 256, 0, c->stream>>>(d_img + r * n2, c->img_prep.p, c->prep_d.p, first_row + r,
+                                                     gp, px, py, c->prep_aver.p);
+        UTR_HIP_TRY(hipGetLastError());
+        const int rc = dht_add_plain(c, first_row + r, k, c->img_prep.p);
+        if (rc) return rc;
     }
     return UTR_OK;
 }
@@ -1091,6 +1130,43 @@ int dht_create(const dht_params* p, int nrows, const double* angZ_deg, const dou
     return UTR_OK;
 }
 
+int dht_set_image_correction(dht_ctx* c, const double* angZ2_deg, const double* shift_x,
+                             const double* shift_y) {
+    if (!c) return set_err(UTR_E_INVALID, "null context");
+    if (c->sctf) return set_image_correction(c->sctf, angZ2_deg, shift_x, shift_y);
+    if (c->n_added) return set_err(UTR_E_STATE, "dht_set_image_correction after images were added");
+    const auto& g = c->g;
+    std::vector<utr::PrepRow> prep(c->nrows);
+    bool any = false;
+    for (int i = 0; i < c->nrows; i++) {
+        const double z2 = angZ2_deg ? angZ2_deg[i] : 0.0, sx = shift_x ? shift_x[i] : 0.0,
+                     sy = shift_y ? shift_y[i] : 0.0;
+        if (!std::isfinite(z2) || !std::isfinite(sx) || !std::isfinite(sy))
+            return set_err(UTR_E_INVALID, "image correction values must be finite");
+        // DHT.cpp:862-880: Move(floor(-shift_y / yst + 0.5), floor(-shift_x / xst + 0.5)), Rotate(-Z'')
+        const double dx = std::floor(-sx / g.xst + 0.5), dy = std::floor(-sy / g.xst + 0.5);
+        if (std::fabs(dx) > 1e6 || std::fabs(dy) > 1e6) return set_err(UTR_E_INVALID, "image shift too large");
+        const double a = -z2 * (3.14159265358979323846 / 180.0);
+        utr::PrepRow& q = prep[i];
+        q.cs = std::cos(a);
+        q.sn = std::sin(a);
+        q.dx = (int)dx;
+        q.dy = (int)dy;
+        q.on = (z2 != 0 || q.dx != 0 || q.dy != 0) ? 1 : 0;
+        any |= q.on != 0;
+    }
+    UTR_HIP_TRY(hipStreamSynchronize(c->stream));
+    UTR_HIP_TRY(c->prep_d.alloc(c->nrows));
+    UTR_HIP_TRY(hipMemcpy(c->prep_d.p, prep.data(), sizeof(utr::PrepRow) * c->nrows, hipMemcpyHostToDevice));
+    if (any) {
+        const int k = std::min(kPrepRows, c->nrows);
+        UTR_HIP_TRY(c->img_prep.alloc((size_t)k * g.nx * g.ny));
+        UTR_HIP_TRY(c->prep_aver.alloc(k));
+    }
+    c->prep.swap(prep);
+    return UTR_OK;
+}
+
 int dht_add_images_device(dht_ctx* c, int first_row, int count, const float* d_images) {
     if (!c) return set_err(UTR_E_INVALID, "null context");
     if (c->sctf) return utr_add_images_device(c->sctf, first_row, count, d_images);
@@ -1170,6 +1246,7 @@ int dht_destroy(dht_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     c->rows.release(); c->xs.release(); c->ys.release(); c->zs.release(); c->acc3.release();
     c->spec.release(); c->T.release(); c->twx.release(); c->twy.release(); c->ap.release();
+    c->prep_d.release(); c->img_prep.release(); c->prep_aver.release();
     c->P.release(); c->stage.release(); c->vol.release(); c->pmax.release();
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);

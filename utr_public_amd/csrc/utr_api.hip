@@ -598,6 +598,41 @@ int utr_finalize_f64(utr_ctx* c, double* vol) {
     return UTR_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Image correction table of rows from Z'' angles and shifts (either may be
+// NULL: zeros), uploaded; for the DHT sCTF context, whose Z'' angles come
+// after its creation (the UTR context takes them in utr_create).
+int set_image_correction(utr_ctx* c, const double* angZ2, const double* shift_x, const double* shift_y) {
+    if (c->n_added) return set_err(UTR_E_STATE, "image correction set after images were added");
+    auto& P = c->plan;
+    for (int i = 0; i < P.nrows; i++) {
+        const double z2 = angZ2 ? angZ2[i] : 0.0, sx = shift_x ? shift_x[i] : 0.0,
+                     sy = shift_y ? shift_y[i] : 0.0;
+        if (!std::isfinite(z2) || !std::isfinite(sx) || !std::isfinite(sy))
+            return set_err(UTR_E_INVALID, "image correction values must be finite");
+        const double dx = std::floor(-sx / P.xst + 0.5), dy = std::floor(-sy / P.xst + 0.5);
+        if (std::fabs(dx) > 1e6 || std::fabs(dy) > 1e6) return set_err(UTR_E_INVALID, "image shift too large");
+        P.angZ2[i] = z2;
+        const double a = -z2 * (3.14159265358979323846 / 180.0);
+        utr::PrepRow& q = P.prep[i];
+        q.cs = std::cos(a);
+        q.sn = std::sin(a);
+        q.dx = (int)dx;
+        q.dy = (int)dy;
+        q.on = (z2 != 0 || q.dx != 0 || q.dy != 0) ? 1 : 0;
+    }
+    UTR_HIP_TRY(hipStreamSynchronize(c->stream));
+    UTR_HIP_TRY(hipMemcpy(c->prep.p, P.prep.data(), sizeof(utr::PrepRow) * P.nrows, hipMemcpyHostToDevice));
+    return alloc_prep(c);
+}
+
+}  // namespace
+
+extern "C" {
+
 int utr_set_image_shifts(utr_ctx* c, const double* shift_x, const double* shift_y) {
     if (!c) return set_err(UTR_E_INVALID, "null context");
     if (!shift_x || !shift_y) return set_err(UTR_E_INVALID, "null shift array");
