@@ -193,12 +193,13 @@ int launch_2d(utr_ctx* c, const float* d_img, int row0, int count, cplx<T>* d_sp
 #define UTR_COLS(N)                                                                             \
     {                                                                                           \
         constexpr int NB = utr::lds_batch(N, sizeof(cplx<T>), utr::LDS_COLS);                      \
-        const int gx = (g.nx + NB - 1) / NB;                                                    \
+        const int ncol = g.nx / 2 + 1; /* the rest by Hermitian symmetry */                     \
+        const int gx = (ncol + NB - 1) / NB;                                                    \
         const int lds = 2 * utr::lds_len(NB * N) * (int)sizeof(cplx<T>);                          \
         const int wpc = std::max(1, std::min(8, 163840 / lds)); /* resident per CU */           \
         const unsigned grid = std::min(utr::xcd_grid(gx * count), persistent_cap(c, wpc));      \
-        utr::k_cols<T, N><<<grid, 256, 0, c->stream>>>(d_spec, g.nx, g, c->rows.p, row0, twy,  \
-                                                       c->rowmax.p, gx, count);                 \
+        utr::k_cols<T, N><<<grid, 256, 0, c->stream>>>(d_spec, g.nx, ncol, g, c->rows.p, row0,  \
+                                                       twy, c->rowmax.p, gx, count);            \
         ok = true;                                                                              \
     }
     UTR_DISPATCH_N(g.ny, UTR_COLS)

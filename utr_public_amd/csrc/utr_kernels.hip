@@ -114,8 +114,13 @@ __global__ void __launch_bounds__(256) k_rows(const float* __restrict__ img,
         }
         __syncthreads();
         fft_lds<T, N>(a, b, nb, tw, -1, threadIdx.x, blockDim.x);
+        // columns 0 .. N/2 only: k_cols forms the others from the Hermitian symmetry
         cplx<T>* dst = out + ((size_t)row * ny + j0) * N;
-        for (int t = threadIdx.x; t < nb * N; t += blockDim.x) dst[t] = a[lx(t)];
+        constexpr int W = N / 2 + 1;
+        for (int t = threadIdx.x; t < nb * W; t += blockDim.x) {
+            const int jj = t / W, i = t - jj * W;
+            dst[(size_t)jj * N + i] = a[lx(jj * N + i)];
+        }
         __syncthreads();  // the next batch overwrites both buffers
     }
 }
@@ -234,11 +239,18 @@ __global__ void __launch_bounds__(256) k_prep_img(const float* __restrict__ img,
 
 // Columns: forward FFT along y, output Shuffle as (-1)^(kx+ky+nx/2+ny/2),
 // optional 2D filter (modes 1/3, restated) and subseries weight.
+// The image is real, so its (centred) spectrum before the filter is
+// Hermitian: K(nx - kx, (ny - ky) mod ny) = conj K(kx, ky); the sign,
+// filters and weight are even in the frequency.  Only columns 0 .. ncol-1 =
+// nx/2 are transformed (k_rows writes no others); the conjugate of each
+// value goes through the same filter into its mirror in columns
+// nx/2+1 .. nx-1 (the mode-3 filter with ESCC is complex, so the filtered
+// spectrum itself need not be Hermitian).
 // rowmax (optional): per-row max(|re|, |im|) of the result as float bits
 // (non-negative floats order like their bit patterns); it sets the
 // fixed-point scale of the brick gridding.
 template <typename T, int N>
-__global__ void __launch_bounds__(256) k_cols(cplx<T>* __restrict__ spec, int nx, GeomDev g,
+__global__ void __launch_bounds__(256) k_cols(cplx<T>* __restrict__ spec, int nx, int ncol, GeomDev g,
                                               const RowParams* __restrict__ rows, int row0,
                                               const cplx<T>* __restrict__ tw,
                                               unsigned* __restrict__ rowmax, int gx, int gy) {
@@ -255,7 +267,7 @@ __global__ void __launch_bounds__(256) k_cols(cplx<T>* __restrict__ spec, int nx
     auto tile_of = [&](int k) { return (int)(k * G + (L & 7u) * (G >> 3) + (L >> 3)); };
     cplx<T> vn[PER];
     auto fetch = [&](int w) {
-        const int row = w / gx, x0 = (w - row * gx) * NB, nb = min(NB, nx - x0);
+        const int row = w / gx, x0 = (w - row * gx) * NB, nb = min(NB, ncol - x0);
         const cplx<T>* src = spec + (size_t)row * ny * nx + x0;
 #pragma unroll
         for (int q = 0; q < PER; q++) {
@@ -279,8 +291,9 @@ __global__ void __launch_bounds__(256) k_cols(cplx<T>* __restrict__ spec, int nx
         cplx<T>* b = sm + lds_len(NB * N);
         const int row = wt / gx;
         const int x0 = (wt - row * gx) * NB;
-        const int nb = min(NB, nx - x0);
-        cplx<T>* base = spec + (size_t)row * ny * nx + x0;
+        const int nb = min(NB, ncol - x0);
+        cplx<T>* const rowbase = spec + (size_t)row * ny * nx;
+        cplx<T>* base = rowbase + x0;
 #pragma unroll
         for (int q = 0; q < PER; q++) {
             const int t = threadIdx.x + 256 * q;
@@ -303,6 +316,12 @@ __global__ void __launch_bounds__(256) k_cols(cplx<T>* __restrict__ spec, int nx
             const T s = T(par_sign(kx + ky) * gsgn);
             v.x *= s;
             v.y *= s;
+            // the mirror's value before the filter: the image spectrum is
+            // Hermitian and the sign is even; the filter below is even in the
+            // frequency but complex in mode 3 with ESCC, so it is applied to
+            // both values
+            const bool mirror = kx > 0 && kx < g.nxd2;
+            cplx<T> vm = {v.x, -v.y};
             if (g.ctf_mode == 1 && R.zout != 0) {
                 // 2D TIE-Hom restated from UTR.h:159-181 in 2D (unpinned)
                 const double i1 = kx - g.nxd2, j1 = ky - g.nyd2;
@@ -310,12 +329,15 @@ __global__ void __launch_bounds__(256) k_cols(cplx<T>* __restrict__ spec, int nx
                 const T f = T(g.two_d_norm / (g.two_d_alpha + q2));
                 v.x *= f;
                 v.y *= f;
+                vm.x *= f;
+                vm.y *= f;
             } else if (g.ctf_mode == 3) {
-                // 2D CTF inversion restated from UTR.h:70-128 in 2D (unpinned)
+                // 2D CTF inversion restated from UTR.h:70-128 in 2D (unpinned):
+                // at centred frequency (i1, j1), inside the aperture, u -> u (sr - i cr)
                 const double dk2 = 1.0 / (xap * xap), de2 = 1.0 / (yap * yap);
-                const double i1 = kx - g.nxd2, j1 = ky - g.nyd2;
-                const double q2 = dk2 * i1 * i1 + de2 * j1 * j1;
-                if (!(g.k2maxo > 0) || q2 < g.k2maxo) {
+                auto coef = [&](double i1, double j1, T& sr, T& cr) -> bool {
+                    const double q2 = dk2 * i1 * i1 + de2 * j1 * j1;
+                    if (g.k2maxo > 0 && !(q2 < g.k2maxo)) return false;
                     const double fac2 = pi * g.wl * R.zout;
                     double eps = g.epsilon;
                     if (eps <= 0) {
@@ -332,17 +354,37 @@ __global__ void __launch_bounds__(256) k_cols(cplx<T>* __restrict__ spec, int nx
                                pi * g.wl * g.wl * g.wl * g.wl * g.wl / 3.0 * g.cs5 * q2 * q2 * q2;
                     double sn, cs;
                     sincos(chi, &sn, &cs);
-                    const T sr = T(sn / (sn * sn + eps));
-                    const T cr = g.escc ? T(cs / (cs * cs + eps)) : T(0);
-                    const cplx<T> w = {v.x * sr + v.y * cr, v.y * sr - v.x * cr};
-                    v = w;
+                    sr = T(sn / (sn * sn + eps));
+                    cr = g.escc ? T(cs / (cs * cs + eps)) : T(0);
+                    return true;
+                };
+                auto mul = [](cplx<T> u, T sr, T cr) { return cplx<T>{u.x * sr + u.y * cr, u.y * sr - u.x * cr}; };
+                const double i1 = kx - g.nxd2, j1 = ky - g.nyd2;
+                T sr, cr;
+                const bool in = coef(i1, j1, sr, cr);
+                if (in) v = mul(v, sr, cr);
+                // the filter is even, so the mirror (-i1, -j1) takes the same
+                // factor -- except on row ky = 0, which is its own mirror row
+                // (j1 = -ny/2 on both sides): there the astigmatic cross term
+                // i1 j1 changes sign
+                if (mirror && ky == 0 && R.astig_s != 0) {
+                    T sr2, cr2;
+                    if (coef(-i1, j1, sr2, cr2)) vm = mul(vm, sr2, cr2);
+                } else if (in) {
+                    vm = mul(vm, sr, cr);
                 }
             }
             if (R.weight != 1.0f) {
                 v.x *= T(R.weight);
                 v.y *= T(R.weight);
+                vm.x *= T(R.weight);
+                vm.y *= T(R.weight);
             }
             base[(size_t)ky * nx + c] = v;
+            if (mirror) {
+                rowbase[(size_t)(ky == 0 ? 0 : ny - ky) * nx + (nx - kx)] = vm;
+                vmax = fmaxf(vmax, fmaxf(fabsf((float)vm.x), fabsf((float)vm.y)));
+            }
             vmax = fmaxf(vmax, fmaxf(fabsf((float)v.x), fabsf((float)v.y)));
         }
         if (rowmax) {
