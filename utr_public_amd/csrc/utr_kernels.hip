@@ -69,7 +69,8 @@ __device__ __forceinline__ bool xcd_tile(int gx, int gy, int& tx, int& ty) {
 // ------------------------------------------------------------- 2D stage
 
 // Rows: renormalise, clamp, Log0 (UTR.cpp:951-958, 991), checkerboard
-// (-1)^(i+j) (the input Shuffle of UTR.cpp:993) and forward FFT along x.
+// (-1)^(i+j) (the input Shuffle of UTR.cpp:993) and forward FFT along x,
+// two real rows per complex transform.
 // Persistent: each workgroup walks batches of NB rows (nbt batches in all,
 // nbr per image) and loads the next batch's pixels into registers while it
 // transforms the current one.
@@ -110,16 +111,25 @@ __global__ void __launch_bounds__(256) k_rows(const float* __restrict__ img,
             if (x < 0.f) x = 0.f;
             T f = (x > 0.f) ? dev_log<T>(x) : T(0);
             if ((i + j0 + jj) & 1) f = -f;
-            a[lx(t)] = {f, T(0)};
+            // rows 2s and 2s+1 are the real and imaginary parts of sequence s
+            T* e = reinterpret_cast<T*>(&a[lx((jj >> 1) * N + i)]);
+            e[jj & 1] = f;
+            if ((nb & 1) && jj == nb - 1) e[1] = T(0);
         }
         __syncthreads();
-        fft_lds<T, N>(a, b, nb, tw, -1, threadIdx.x, blockDim.x);
+        const int np = (nb + 1) >> 1;
+        fft_lds<T, N>(a, b, np, tw, -1, threadIdx.x, blockDim.x);
+        // the two real rows' spectra from Z = X1 + i X2:
+        // X1(k) = (Z(k) + conj Z(N-k)) / 2, X2(k) = (Z(k) - conj Z(N-k)) / 2i;
         // columns 0 .. N/2 only: k_cols forms the others from the Hermitian symmetry
         cplx<T>* dst = out + ((size_t)row * ny + j0) * N;
         constexpr int W = N / 2 + 1;
-        for (int t = threadIdx.x; t < nb * W; t += blockDim.x) {
-            const int jj = t / W, i = t - jj * W;
-            dst[(size_t)jj * N + i] = a[lx(jj * N + i)];
+        for (int t = threadIdx.x; t < np * W; t += blockDim.x) {
+            const int sq = t / W, k = t - sq * W;
+            const cplx<T> z = a[lx(sq * N + k)], zm = a[lx(sq * N + (k == 0 ? 0 : N - k))];
+            dst[(size_t)(2 * sq) * N + k] = {T(0.5) * (z.x + zm.x), T(0.5) * (z.y - zm.y)};
+            if (2 * sq + 1 < nb)
+                dst[(size_t)(2 * sq + 1) * N + k] = {T(0.5) * (z.y + zm.y), T(-0.5) * (z.x - zm.x)};
         }
         __syncthreads();  // the next batch overwrites both buffers
     }
