@@ -38,8 +38,10 @@ __device__ __forceinline__ cplx<T> mul_si(cplx<T> a, int sign) {
 
 // ------------------------------------------------------------- LDS layout
 // One pad entry after every 8: the strided writes of the first Stockham
-// passes (stride R, then R*p for p < 64) and the transposed column loads
-// then fall on distinct LDS banks instead of 4 to 16 lanes sharing one.
+// passes (stride R, then R*p for p < 64) then fall on distinct LDS banks
+// instead of 4 to 16 lanes sharing one.  (The transposed column loads of
+// 8 sequences of 256 still share banks; two layouts that spread them, an
+// extra pad per 128 entries and an XOR swizzle, measured slower overall.)
 // A buffer of n logical entries holds lds_len(n); entry i sits at lx(i).
 __host__ __device__ constexpr int lds_len(int n) { return n + (n >> 3); }
 __device__ __forceinline__ int lx(int i) { return i + (i >> 3); }
