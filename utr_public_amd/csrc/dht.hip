@@ -117,13 +117,21 @@ __global__ void __launch_bounds__(256) k_dht_cols(cplx<float>* __restrict__ data
     }
 }
 
-// Per output plane p of orientation a0+blockIdx.z: spectrum row times the
-// plane's filter, inverse FFT along x.  grid (rows/NB, nplanes, na).
+// Per slot s = blockIdx.y of orientation a0+blockIdx.z: spectrum row times
+// the filter of the slot's plane(s), inverse FFT along x.  grid (rows/NB,
+// slots, na).  Method 3: slot s is plane s.  Method 4: the filtered spectrum
+// of a plane is Hermitian (the spectrum of the real I - 1 times a real, even
+// filter), so its inverse transform is real, and slot s carries plane 2s in
+// the real part and plane 2s+1 in the imaginary part (dht_slots).
 // The phase chi = f2 q^2 + f4 q^4 + f6 q^6 (DHT.cpp:962-1000) is split as
 // exp(-i chi) = ex(fx) ey(fy) ap(fx, fy): ex = exp(-i f2 fx^2) and
 // ey = exp(-i f2 fy^2) depend on the plane's defocus and are formed per
 // workgroup with FP64 phases; ap = exp(-i (f4 q^4 + f6 q^6)), zero outside
 // the objective aperture, is one table per job (read when g.use_ap).
+__host__ __device__ inline int dht_slots(const DhtGeom& g) {
+    return g.method == 4 ? (g.nplanes + 1) / 2 : g.nplanes;
+}
+
 template <int N>
 __global__ void __launch_bounds__(256) k_dht_prop_rows(const cplx<float>* __restrict__ S,
                                                        cplx<float>* __restrict__ T, DhtGeom g,
@@ -133,20 +141,19 @@ __global__ void __launch_bounds__(256) k_dht_prop_rows(const cplx<float>* __rest
     constexpr int NB = lds_batch(N, sizeof(cplx<float>), LDS_DHT);
     constexpr int PER = (NB * N + 255) / 256;  // elements per thread
     __shared__ cplx<float> sm[2 * lds_len(NB * N)];
-    __shared__ cplx<float> s_ex[N], s_ey[NB];
+    __shared__ cplx<float> s_ex[2][N], s_ey[2][NB];
     cplx<float>* a = sm;
     cplx<float>* b = sm + lds_len(NB * N);
     const int ny = g.ny;
-    const int p = blockIdx.y, ab = blockIdx.z;
+    const int slot = blockIdx.y, ab = blockIdx.z;
+    const bool pack = g.method == 4;
+    const int p0 = pack ? 2 * slot : slot;
+    const int np = pack && p0 + 1 < g.nplanes ? 2 : 1;  // planes in this slot
     const int k0 = blockIdx.x * NB;
     const int nb = min(NB, ny - k0);
     const int tid = threadIdx.x;
     const DhtRow& R = rows[a0 + ab];
-    const int n = g.escc ? p : g.nz / 2;
-    const double zn = g.zlodz + g.xst * n;  // voutdefocus[n], DHT.cpp:388
-    const double d = g.method == 3 ? zn - R.zout : R.zout - zn;
     const double pi = 3.14159265358979323846, twopi = 2.0 * pi;
-    const double f2 = pi * g.wl * d;
     const cplx<float>* src = S + ((size_t)(a0 + ab) * ny + k0) * N;
     const cplx<float>* aps = ap + (size_t)k0 * N;
     cplx<float> v[PER], h[PER];
@@ -158,15 +165,20 @@ __global__ void __launch_bounds__(256) k_dht_prop_rows(const cplx<float>* __rest
             h[q] = g.use_ap ? aps[t] : cplx<float>{1.f, 0.f};
         }
     }
-    for (int k = tid; k < N + nb; k += 256) {
-        const double f = k < N ? dht_freq(k, N, g.xst) : dht_freq(k0 + k - N, ny, g.xst);
+    for (int k = tid; k < np * (N + nb); k += 256) {
+        const int m = k >= N + nb ? 1 : 0, kk = k - m * (N + nb);
+        const int n = g.escc ? p0 + m : g.nz / 2;
+        const double zn = g.zlodz + g.xst * n;  // voutdefocus[n], DHT.cpp:388
+        const double d = g.method == 3 ? zn - R.zout : R.zout - zn;
+        const double f2 = pi * g.wl * d;
+        const double f = kk < N ? dht_freq(kk, N, g.xst) : dht_freq(k0 + kk - N, ny, g.xst);
         double chi = f2 * (f * f);
         chi -= twopi * rint(chi * (1.0 / twopi));
         double sn, cs;
         sincos(chi, &sn, &cs);
         const cplx<float> e = {(float)cs, (float)-sn};
-        if (k < N) s_ex[k] = e;
-        else s_ey[k - N] = e;
+        if (kk < N) s_ex[m][kk] = e;
+        else s_ey[m][kk - N] = e;
     }
     __syncthreads();
 #pragma unroll
@@ -174,27 +186,36 @@ __global__ void __launch_bounds__(256) k_dht_prop_rows(const cplx<float>* __rest
         const int t = tid + 256 * q;
         if (t < nb * N) {
             const int r = t / N, kx = t - r * N;
-            const cplx<float> H = cmul(cmul(s_ex[kx], s_ey[r]), h[q]);  // cos chi - i sin chi
-            float fr = 0.f, fi = 0.f;
-            if (g.method == 3) {
-                fr = H.x;
-                fi = H.y;
-            } else if (h[q].x != 0.f || h[q].y != 0.f) {  // |ap| = 1 inside the aperture, 0 outside
-                const float tt = -H.y - (float)g.asigma * H.x;  // sin chi - asigma cos chi
-                fr = tt / (2.f * (tt * tt + (float)g.epsilon));
+            const bool in = h[q].x != 0.f || h[q].y != 0.f;  // |ap| = 1 inside the aperture, 0 outside
+            if (!pack) {
+                const cplx<float> H = cmul(cmul(s_ex[0][kx], s_ey[0][r]), h[q]);  // cos chi - i sin chi
+                const float fr = H.x, fi = H.y;
+                a[lx(t)] = {v[q].x * fr - v[q].y * fi, v[q].x * fi + v[q].y * fr};
+            } else {
+                float f[2] = {0.f, 0.f};
+#pragma unroll
+                for (int m = 0; m < 2; m++) {
+                    if (m < np && in) {
+                        const cplx<float> H = cmul(cmul(s_ex[m][kx], s_ey[m][r]), h[q]);
+                        const float tt = -H.y - (float)g.asigma * H.x;  // sin chi - asigma cos chi
+                        f[m] = tt / (2.f * (tt * tt + (float)g.epsilon));
+                    }
+                }
+                // plane p0 times f[0] plus i times plane p0+1 times f[1]
+                a[lx(t)] = {v[q].x * f[0] - v[q].y * f[1], v[q].y * f[0] + v[q].x * f[1]};
             }
-            a[lx(t)] = {v[q].x * fr - v[q].y * fi, v[q].x * fi + v[q].y * fr};
         }
     }
     __syncthreads();
     fft_lds<float, N>(a, b, nb, tw, +1, tid, blockDim.x);
-    cplx<float>* dst = T + (((size_t)ab * g.nplanes + p) * ny + k0) * N;
+    cplx<float>* dst = T + (((size_t)ab * dht_slots(g) + slot) * ny + k0) * N;
     for (int t = tid; t < nb * N; t += blockDim.x) dst[t] = a[lx(t)];
 }
 
-// Inverse FFT along y of each plane, then 1 - |psi|^2 = -2 Re u - |u|^2
-// (method 3) or the real part (method 4), times the orientation's weight.
-// Tile w = NB adjacent columns of one plane: column group w % gx, plane
+// Inverse FFT along y of each slot, then 1 - |psi|^2 = -2 Re u - |u|^2
+// (method 3) or the real and imaginary parts as planes 2s and 2s+1
+// (method 4), times the orientation's weight.
+// Tile w = NB adjacent columns of one slot: column group w % gx, slot
 // w / gx of the batch (orientation-major).  Persistent: in round k this
 // workgroup takes tile k G + (L & 7) G/8 + L/8, so the tiles of one XCD in a
 // round are contiguous and neighbouring column groups share its L2; the
@@ -247,8 +268,12 @@ __global__ void __launch_bounds__(256) k_dht_prop_cols(const cplx<float>* __rest
         cplx<float>* a = sm;
         cplx<float>* b = sm + lds_len(NB * N);
         const int pa = wt / gx, x0 = (wt - pa * gx) * NB, nb = min(NB, nx - x0);
-        const int ab = pa / g.nplanes;
-        const size_t plane = (size_t)pa * N * nx;
+        const int nslots = dht_slots(g), ab = pa / nslots, slot = pa - ab * nslots;
+        // method 4: the slot's real part is plane 2 slot, its imaginary part plane 2 slot + 1
+        const bool pack = g.method == 4;
+        const int pl0 = ab * g.nplanes + (pack ? 2 * slot : slot);
+        const bool second = pack && (pack ? 2 * slot : slot) + 1 < g.nplanes;
+        const size_t plane = (size_t)pl0 * N * nx;
         auto put = [&](const int cols) {
 #pragma unroll
             for (int q = 0; q < PER; q++) {
@@ -273,9 +298,14 @@ __global__ void __launch_bounds__(256) k_dht_prop_cols(const cplx<float>* __rest
                     const int y = t / cols, c = t - y * cols;
                     const cplx<float> z = a[lx(c * N + y)];
                     const float re = z.x * inv, im = z.y * inv;
-                    const float v = (g.method == 3 ? -2.f * re - (re * re + im * im) : re) * w;
+                    const float v = (pack ? re : -2.f * re - (re * re + im * im)) * w;
                     P[plane + (size_t)y * nx + x0 + c] = v;
                     vmax = fmaxf(vmax, fabsf(v));
+                    if (second) {
+                        const float v2 = im * w;
+                        P[plane + (size_t)N * nx + (size_t)y * nx + x0 + c] = v2;
+                        vmax = fmaxf(vmax, fabsf(v2));
+                    }
                 }
             }
         };
@@ -949,7 +979,8 @@ int dht_run(dht_ctx* c, float* d_vol) {
 #define DHT_PR(N)                                                                                \
     {                                                                                            \
         constexpr int NB = utr::lds_batch(N, sizeof(cplx<float>), utr::LDS_DHT);                 \
-        utr::k_dht_prop_rows<N><<<dim3((g.ny + NB - 1) / NB, g.nplanes, 1), 256, 0, c->stream>>>( \
+        utr::k_dht_prop_rows<N><<<dim3((g.ny + NB - 1) / NB, utr::dht_slots(g), 1), 256, 0,       \
+                                  c->stream>>>(                                                  \
             c->spec.p, c->T.p, g, c->rows.p, a0 + ab, c->twx.p, c->ap.p);                        \
         ok = true;                                                                               \
     }
@@ -961,7 +992,7 @@ int dht_run(dht_ctx* c, float* d_vol) {
     {                                                                                            \
         constexpr int NB = utr::lds_batch(N, sizeof(cplx<float>), utr::LDS_DHT);                 \
         constexpr int lds = 2 * utr::lds_len(NB * N) * (int)sizeof(cplx<float>);                \
-        const int gx = (g.nx + NB - 1) / NB, total = gx * g.nplanes;                             \
+        const int gx = (g.nx + NB - 1) / NB, total = gx * utr::dht_slots(g);                     \
         const unsigned cap = (unsigned)(c->ncu * std::max(1, std::min(8, 163840 / lds))) & ~7u;  \
         const unsigned grid = std::min(utr::xcd_grid((unsigned)total), cap ? cap : 8u);          \
         utr::k_dht_prop_cols<N><<<grid, 256, 0, c->stream>>>(                                    \
