@@ -328,6 +328,56 @@ int dht_dims(const dht_ctx* ctx, int* nx, int* ny, int* nz);
 int dht_get_timings(const dht_ctx* ctx, utr_timings* t);
 int dht_destroy(dht_ctx* ctx);
 
+/* ---------------------------------------------------------------------------
+ * TEM multislice forward model: the image simulator of TemsimLibrary
+ * (autosliccmd(), autosliccmd.cpp:177-1300, with autoslic::calculate,
+ * autoslic.cpp:209-869), coherent branch only: plane incident wave, one
+ * atom configuration (no thermal wobble), no ice, no crystal tilt, Z'' = 0
+ * and no XY shifts of the images.  One context holds one atom set; each call
+ * simulates any number of orientations and defocus distances.
+ * ------------------------------------------------------------------------- */
+typedef struct msct_params {
+    int nx, ny;             /* pixels (autosliccmd line 12 "nx ny")                 */
+    int mode;               /* 0 multislice, 1 projection, 2 first Born (the slice
+                               loop of both skips propagation, autoslic.cpp:772)   */
+    int output;             /* 0 intensity |psi|^2, 1 complex psi (re, im pairs)   */
+    double ctblength;       /* cube side in Angstrom (the largest cell edge of the
+                               XYZ file, autosliccmd.cpp:629-633)                   */
+    double deltaz;          /* slice thickness, Angstrom (line 14)                 */
+    double kev;             /* beam energy, keV (line 11)                          */
+    double aobj_mrad;       /* objective aperture, mrad; 0 = none (line 8)         */
+    double cs3_mm, cs5_mm;  /* spherical aberrations, mm (line 6)                  */
+    double astig_dz;        /* two-fold astigmatism amplitude, Angstrom            */
+    double astig_angle_deg; /* and its angle, degrees (autosliccmd's astigm pair)  */
+    double dxc, dyc, dzc;   /* rotation centre minus the cube centre, A (line 13)  */
+} msct_params;
+
+typedef struct msct_ctx msct_ctx;
+
+/* One atom set (ReadXYZcoord's Z, x, y, z, occ; natom may be 0) and the
+ * electron scattering-factor parameters fe_params[(Z-1)*12 + i] of Kirkland
+ * (2010) App. C for Z = 1..103 (slicelib.cpp:958-2209, shipped as
+ * utr_public_amd/data/fe_params.txt).  Replaces the set-up half of
+ * autosliccmd() (autosliccmd.cpp:600-690, 870-910). */
+int msct_create(const msct_params* p, const double* fe_params, int natom, const int* Z, const float* x,
+                const float* y, const float* z, const float* occ, msct_ctx** out);
+
+/* Images of nang orientations (angZ, angY in radians, the autosliccmd line
+ * 25 pair: rotation about Z, then about Y', autosliccmd.cpp:733-748) at ndef
+ * defocus distances (Angstrom, autosliccmd.cpp:1065): out holds
+ * nang*ndef*ny*nx floats (x fastest), twice that for complex output,
+ * orientation-major.  An atom leaving the cube after rotation is an error,
+ * as in autosliccmd.cpp:857-860.  Host / device output. */
+int msct_images(msct_ctx* ctx, int nang, const double* angZ_rad, const double* angY_rad, int ndef,
+                const double* defocus, float* out);
+int msct_images_device(msct_ctx* ctx, int nang, const double* angZ_rad, const double* angY_rad, int ndef,
+                       const double* defocus, float* d_out);
+
+/* Slices per orientation: k * deltaz for k = 1, 2, ... while below the cube
+ * side + deltaz (autoslic.cpp:727-744). */
+int msct_slices(const msct_ctx* ctx);
+int msct_destroy(msct_ctx* ctx);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,357 @@
+"""TEST INFRASTRUCTURE, NOT PART OF THE PRODUCT.
+
+FP64 numpy restatement of the reference's TEM multislice forward model
+(TemsimLibrary, the image simulator behind MsctKirkland):
+  * projected atomic potential vzatom() from the 12 fe parameters of
+    Kirkland (2010) App. C, with the modified Bessel functions of
+    Abramowitz & Stegun 9.8.1-9.8.6 (slicelib.cpp:185-260, 2759-2792);
+  * its look-up table vzatomLUT(): 100 log-spaced radii, Akima's
+    quasi-Hermite cubic spline in r^2 (slicelib.cpp:2402-2424, 2537-2603,
+    2812-2889);
+  * atom rotation about Z then Y' around the cube centre, in float
+    (autosliccmd.cpp:709-748);
+  * slicing, transmission function, bandwidth limit and propagator of the
+    coherent multislice loop (autoslic.cpp:325-420, 666-800, 917-1013;
+    slicelib.cpp:457-476, 811-849);
+  * free-space propagation of the exit wave to each defocus distance
+    (autosliccmd.cpp:1060-1087).  FresnelA lives in the absent XArrayLibrary
+    and is restated as in the DHT oracle: spectrum * exp(-i chi),
+    chi = pi wl R q^2 + pi wl^3 Cs3 q^4 / 2 + pi wl^5 Cs5 q^6 / 3 + the
+    two-fold astigmatism term pi wl dz ((qx^2 - qy^2) cos 2a + 2 qx qy sin 2a),
+    zero for q^2 >= k2maxo when an objective aperture is set.
+Discrete decisions (slice membership, pixel windows, bandwidth masks) are
+taken on the same float values the reference computes, so that they agree
+with it; everything else is double.  Only tests/ and bench.py's cpu_baseline
+leg use this module.
+"""
+import math
+import os
+
+import numpy as np
+
+_FE_PATH = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                        "utr_public_amd", "data", "fe_params.txt")
+_FE = None
+
+f32 = np.float32
+
+
+def fe_params():
+    """(104, 12) table, row Z = the 12 parameters of element Z (row 0 unused)."""
+    global _FE
+    if _FE is None:
+        t = np.loadtxt(_FE_PATH, dtype=np.float64)
+        _FE = np.vstack([np.zeros((1, 12)), t])
+    return _FE
+
+
+def wavelength(kev):
+    """Electron wavelength in Angstrom (slicelib.cpp:2891-2900)."""
+    emass, hc = 510.99906, 12.3984244
+    return hc / math.sqrt(kev * (2 * emass + kev))
+
+
+def sigma(kev):
+    """Interaction parameter in rad/(kV A) (slicelib.cpp:2441-2455)."""
+    emass = 510.99906
+    x = (emass + kev) / (2.0 * emass + kev)
+    return 2.0 * math.pi * x / (wavelength(kev) * kev)
+
+
+def bessi0(x):
+    """I0(x), Abramowitz & Stegun 9.8.1 and 9.8.2 (slicelib.cpp:185-212)."""
+    ax = abs(x)
+    if ax <= 3.75:
+        t = (x / 3.75) ** 2
+        c = (1.0, 3.5156229, 3.0899424, 1.2067492, 0.2659732, 0.0360768, 0.0045813)
+        s = c[6]
+        for k in range(5, -1, -1):
+            s = s * t + c[k]
+        return s
+    t = 3.75 / ax
+    c = (0.39894228, 0.01328592, 0.00225319, -0.00157565, 0.00916281, -0.02057706,
+         0.02635537, -0.01647633, 0.00392377)
+    s = c[8]
+    for k in range(7, -1, -1):
+        s = s * t + c[k]
+    return math.exp(ax) * s / math.sqrt(ax)
+
+
+def bessk0(x):
+    """K0(x), Abramowitz & Stegun 9.8.5 and 9.8.6 (slicelib.cpp:217-245);
+    1e20 at x = 0."""
+    ax = abs(x)
+    if 0.0 < ax <= 2.0:
+        t = (ax / 2.0) ** 2
+        c = (-0.57721566, 0.42278420, 0.23069756, 0.03488590, 0.00262698, 0.00010750, 0.00000740)
+        s = c[6]
+        for k in range(5, -1, -1):
+            s = s * t + c[k]
+        return -math.log(ax / 2.0) * bessi0(x) + s
+    if ax > 2.0:
+        t = 2.0 / ax
+        c = (1.25331414, -0.07832358, 0.02189568, -0.01062446, 0.00587872, -0.00251540, 0.00053208)
+        s = c[6]
+        for k in range(5, -1, -1):
+            s = s * t + c[k]
+        return math.exp(-ax) * s / math.sqrt(ax)
+    return 1.0e20
+
+
+def vzatom(Z, r):
+    """Projected potential (V A) of element Z at radius r (slicelib.cpp:2759-2792):
+    al * sum a_i K0(2 pi r sqrt(b_i)) + ag * sum c_i exp(-(pi r)^2 / d_i) / d_i,
+    with Kirkland's constants al = 4 pi^2 a0 e, ag = 2 pi^2 a0 e and the
+    10-digit pi the reference uses."""
+    if Z < 1 or Z > 103:
+        return 0.0
+    fp = fe_params()[Z]
+    al, ag, pi = 300.8242834, 150.4121417, 3.141592654
+    r = max(abs(r), 1.0e-10)
+    x = 2.0 * pi * r
+    suml = sum(fp[i] * bessk0(x * math.sqrt(fp[i + 1])) for i in (0, 2, 4))
+    x = (pi * r) ** 2
+    sumg = sum(fp[i] * math.exp(-x / fp[i + 1]) / fp[i + 1] for i in (6, 8, 10))
+    return al * suml + ag * sumg
+
+
+def splinh(x, y):
+    """Akima quasi-Hermite cubic spline coefficients b, c, d (slicelib.cpp:2537-2603;
+    H. Akima, J. ACM 17 (1970) 589)."""
+    n = len(x)
+    b = np.zeros(n)
+    c = np.zeros(n)
+    d = np.zeros(n)
+    SMALL = 1.0e-25
+    m5 = (y[3] - y[2]) / (x[3] - x[2])
+    m4 = (y[2] - y[1]) / (x[2] - x[1])
+    m3 = (y[1] - y[0]) / (x[1] - x[0])
+    m2 = m3 + m3 - m4
+    m1 = m2 + m2 - m3
+    m54, m43, m32, m21 = abs(m5 - m4), abs(m4 - m3), abs(m3 - m2), abs(m2 - m1)
+    t1 = (m43 * m2 + m21 * m3) / (m43 + m21) if (m43 + m21) > SMALL else 0.5 * (m2 + m3)
+    for i in range(n - 1):
+        t2 = (m54 * m3 + m32 * m4) / (m54 + m32) if (m54 + m32) > SMALL else 0.5 * (m3 + m4)
+        x43 = x[i + 1] - x[i]
+        b[i] = t1
+        c[i] = (3.0 * m3 - t1 - t1 - t2) / x43
+        d[i] = (t1 + t2 - m3 - m3) / (x43 * x43)
+        m1, m2, m3, m4 = m2, m3, m4, m5
+        m5 = (y[i + 4] - y[i + 3]) / (x[i + 4] - x[i + 3]) if i < n - 4 else m4 + m4 - m3
+        m21, m32, m43 = m32, m43, m54
+        m54 = abs(m5 - m4)
+        t1 = t2
+    return b, c, d
+
+
+def seval_index(x, x0):
+    """Interval index of seval() (slicelib.cpp:2402-2424): 0 below x[0],
+    n-2 at or above x[n-2], else the bisection result."""
+    n = len(x)
+    if x0 <= x[0]:
+        return 0
+    if x0 >= x[n - 2]:
+        return n - 2
+    return int(np.searchsorted(x, x0, side="right") - 1)
+
+
+class VzLUT:
+    """vzatomLUT() (slicelib.cpp:2812-2889): r^2 knots (0.01 * (500)^(i/99))^2,
+    i = 0..99, values vzatom(Z, sqrt(knot)), Akima spline in r^2."""
+    RMIN, RMAX, NR = 0.01, 5.0, 100
+
+    def __init__(self):
+        dlnr = math.log(self.RMAX / self.RMIN) / (self.NR - 1)
+        self.x = np.array([(self.RMIN * math.exp(i * dlnr)) ** 2 for i in range(self.NR)])
+        self.tab = {}
+
+    def coeffs(self, Z):
+        if Z not in self.tab:
+            y = np.array([vzatom(Z, math.sqrt(v)) for v in self.x])
+            b, c, d = splinh(self.x, y)
+            self.tab[Z] = (y, b, c, d)
+        return self.tab[Z]
+
+    def __call__(self, Z, rsq):
+        y, b, c, d = self.coeffs(Z)
+        rsq = np.asarray(rsq, np.float64)
+        i = np.searchsorted(self.x, rsq, side="right") - 1
+        i = np.clip(i, 0, self.NR - 2)
+        i = np.where(rsq <= self.x[0], 0, i)
+        z = rsq - self.x[i]
+        return y[i] + (b[i] + (c[i] + d[i] * z) * z) * z
+
+
+def freqn(n, a):
+    """Spatial frequencies of the FFT bins, float (slicelib.cpp:457-476)."""
+    imid = int(n / 2.0 + 0.5)
+    k = np.empty(n, f32)
+    for i in range(n):
+        k[i] = f32(i - n) / f32(a) if i > imid else f32(i) / f32(a)
+    return k, (k * k).astype(f32)
+
+
+def rotate_atoms(x, y, z, angZ, angY, ctb, dxc=0.0, dyc=0.0, dzc=0.0):
+    """Rotation about Z by angZ, then about Y' by angY (radians), around
+    (ctb/2 + dxc, ctb/2 + dyc, ctb/2 + dzc), in float (autosliccmd.cpp:709-748)."""
+    x = np.asarray(x, f32).copy()
+    y = np.asarray(y, f32).copy()
+    z = np.asarray(z, f32).copy()
+    xc, yc, zc = f32(ctb / 2.0 + dxc), f32(ctb / 2.0 + dyc), f32(ctb / 2.0 + dzc)
+    sZ, cZ, sY, cY = f32(math.sin(angZ)), f32(math.cos(angZ)), f32(math.sin(angY)), f32(math.cos(angY))
+    xx = xc + (x - xc) * cZ + (-y + yc) * sZ
+    yy = yc + (x - xc) * sZ + (y - yc) * cZ
+    x, y = xx.astype(f32), yy.astype(f32)
+    xx = xc + (x - xc) * cY + (z - zc) * sY
+    zz = zc + (-x + xc) * sY + (z - zc) * cY
+    return xx.astype(f32), y, zz.astype(f32)
+
+
+def slice_counts(zs, deltaz, zmax):
+    """Number of atoms of each slice for z-sorted coordinates: slice k
+    (k = 1, 2, ...) takes the atoms with z < k * deltaz not yet taken, while
+    zslice < zmax + deltaz (autoslic.cpp:727-744, 794-796)."""
+    dz = float(f32(deltaz))
+    lim = float(f32(zmax)) + dz
+    counts = []
+    zslice = dz
+    istart = 0
+    n = len(zs)
+    while zslice < lim:
+        na = 0
+        while istart + na < n and float(zs[istart + na]) < zslice:
+            na += 1
+        counts.append(na)
+        zslice += dz
+        istart += na
+    return counts
+
+
+def cf_fft(a):
+    """cfpix::fft(): FFTW_BACKWARD (exponent +1), unnormalised (cfpix.cpp:137-148, 225-228)."""
+    return np.fft.ifft2(a) * (a.shape[0] * a.shape[1])
+
+
+def cf_ifft(a):
+    """cfpix::ifft(): exponent -1, times 1/(nx ny) (cfpix.cpp:150-168)."""
+    return np.fft.fft2(a) / (a.shape[0] * a.shape[1])
+
+
+class Model:
+    """One multislice set-up: cube side ctb (A), nx x ny pixels, slice
+    thickness deltaz (A), beam energy kev (keV)."""
+
+    def __init__(self, nx, ny, ctb, deltaz, kev, mode=0):
+        self.nx, self.ny, self.ctb, self.deltaz, self.kev, self.mode = nx, ny, ctb, deltaz, kev, mode
+        self.lut = VzLUT()
+        self.kx, self.kx2 = freqn(nx, ctb)
+        self.ky, self.ky2 = freqn(ny, ctb)
+        # bandwidth limit 2/3 of the smaller Nyquist frequency, squared, in float
+        # (autoslic.cpp:148, 382-393)
+        k2max = f32(nx) / (f32(2.0) * f32(ctb))
+        t = f32(ny) / (f32(2.0) * f32(ctb))
+        if t < k2max:
+            k2max = t
+        k2max = f32(f32(2.0) / f32(3.0)) * k2max
+        self.k2max = f32(k2max * k2max)
+        self.kevf = float(f32(kev))  # the beam energy is a float parameter (autosliccmd.cpp:318)
+        self.wavlen = f32(wavelength(self.kevf))
+        # propagator (autoslic.cpp:395-415), no crystal tilt: float phase, double cos/sin
+        scale = f32(math.pi) * f32(deltaz)
+        tx = (scale * (self.kx2 * self.wavlen)).astype(f32).astype(np.float64)
+        ty = (scale * (self.ky2 * self.wavlen)).astype(f32).astype(np.float64)
+        self.propx = np.exp(-1j * tx)
+        self.propy = np.exp(-1j * ty)
+        k2 = (self.ky2[:, None] + self.kx2[None, :]).astype(f32)
+        self.keep = k2 < self.k2max
+
+    def trlayer(self, x, y, occ, Zn):
+        """Transmission function of one slice (autoslic.cpp:917-1013)."""
+        nx, ny, ax = self.nx, self.ny, float(f32(self.ctb))
+        scale = sigma(self.kevf) / 1000.0
+        sx, sy = float(f32(ax) / f32(nx)), float(f32(ax) / f32(ny))  # float steps, double radii
+        rx, ry = ax / nx, ax / ny
+        rmin = 0.25 * math.sqrt(0.5 * (rx * rx + ry * ry))
+        rminsq = rmin * rmin
+        idx, idy = int(nx * 3.0 / ax) + 1, int(ny * 3.0 / ax) + 1
+        v = np.zeros((ny, nx))
+        for i in range(len(x)):
+            xi, yi = float(x[i]), float(y[i])
+            ixo, iyo = int(xi / sx), int(yi / sy)
+            ix = np.arange(ixo - idx, ixo + idx + 1)
+            iy = np.arange(iyo - idy, iyo + idy + 1)
+            rx2 = (xi - ix * sx) ** 2
+            rsq = rx2[None, :] + ((yi - iy * sy) ** 2)[:, None]
+            m = rsq <= 9.0
+            rsq = np.maximum(rsq, rminsq)
+            vz = float(occ[i]) * self.lut(int(Zn[i]), rsq)
+            np.add.at(v, (np.broadcast_to((iy % ny)[:, None], m.shape)[m],
+                          np.broadcast_to((ix % nx)[None, :], m.shape)[m]), vz[m])
+        t = np.exp(1j * scale * v)
+        T = cf_fft(t)
+        T[~self.keep] = 0
+        return cf_ifft(T)
+
+    def exit_wave(self, x, y, z, occ, Zn):
+        """Coherent multislice of one orientation (autoslic.cpp:666-800):
+        plane wave in, transmit slice by slice, propagate after each slice
+        (multislice) or not at all (projection)."""
+        order = np.argsort(np.asarray(z, f32), kind="stable")
+        x, y, z, occ, Zn = (np.asarray(a)[order] for a in (x, y, z, occ, Zn))
+        if len(z) and (min(x.min(), y.min(), z.min()) < 0 or max(x.max(), y.max(), z.max()) > f32(self.ctb)):
+            raise ValueError("atoms outside the cube")
+        wave = np.ones((self.ny, self.nx), np.complex128)
+        s = 0
+        for na in slice_counts(z, self.deltaz, self.ctb):
+            if na > 0:
+                wave = wave * self.trlayer(x[s:s + na], y[s:s + na], occ[s:s + na], Zn[s:s + na])
+            if self.mode == 0:
+                W = cf_fft(wave)
+                W = W * self.propy[:, None] * self.propx[None, :]
+                W[~self.keep] = 0
+                wave = cf_ifft(W)
+            s += na
+        return wave
+
+    def fresnel(self, wave, R, aobj_rad=0.0, cs3_mm=0.0, cs5_mm=0.0, astig_dz=0.0, astig_ang=0.0):
+        """Free-space propagation by R (A) with aberrations (restated FresnelA)."""
+        ny, nx = wave.shape
+        wl = float(self.wavlen)
+        ax = float(f32(self.ctb))
+        qx = np.fft.fftfreq(nx, ax / nx)
+        qy = np.fft.fftfreq(ny, ax / ny)
+        cs3_mm, cs5_mm = float(f32(cs3_mm)), float(f32(cs5_mm))  # float parameters (autosliccmd.cpp:306)
+        QX, QY = np.meshgrid(qx, qy)
+        q2 = QX * QX + QY * QY
+        chi = math.pi * wl * R * q2 + math.pi * wl ** 3 * cs3_mm * 1e7 * q2 * q2 / 2.0 \
+            + math.pi * wl ** 5 * cs5_mm * 1e7 * q2 ** 3 / 3.0
+        if astig_dz != 0.0:
+            chi = chi + math.pi * wl * astig_dz * ((QX * QX - QY * QY) * math.cos(2 * astig_ang)
+                                                  + 2 * QX * QY * math.sin(2 * astig_ang))
+        F = np.fft.fft2(wave) * np.exp(-1j * chi)
+        if aobj_rad > 0:
+            k2maxo = float(f32(f32(aobj_rad) / f32(wl)) ** 2)
+            F[q2 >= k2maxo] = 0
+        return np.fft.ifft2(F)
+
+
+def images(p, atoms, angZ, angY, defocus, output="intensity"):
+    """Images [orientation][defocus][ny][nx] of an atom set (autosliccmd.cpp:
+    709-748 rotation, autoslic::calculate, 1060-1087 propagation, 1196-1200
+    intensity).  p: dict with nx, ny, ctb, deltaz, kev and optional mode,
+    aobj_mrad, cs3_mm, cs5_mm, astig_dz, astig_angle (deg), dxc, dyc, dzc.
+    atoms: dict with Z, x, y, z, occ."""
+    m = Model(p["nx"], p["ny"], p["ctb"], p["deltaz"], p["kev"], p.get("mode", 0))
+    aobj = float(f32(abs(f32(p.get("aobj_mrad", 0.0)) * f32(0.001))))
+    out = np.zeros((len(angZ), len(defocus), p["ny"], p["nx"]),
+                   np.complex128 if output == "complex" else np.float64)
+    for a in range(len(angZ)):
+        x, y, z = rotate_atoms(atoms["x"], atoms["y"], atoms["z"], angZ[a], angY[a], p["ctb"],
+                               p.get("dxc", 0.0), p.get("dyc", 0.0), p.get("dzc", 0.0))
+        w = m.exit_wave(x, y, z, atoms["occ"], atoms["Z"])
+        for j, d in enumerate(defocus):
+            R = d - 0.5 * p["ctb"]
+            c = m.fresnel(w, R, aobj, p.get("cs3_mm", 0.0), p.get("cs5_mm", 0.0), p.get("astig_dz", 0.0),
+                          math.radians(p.get("astig_angle", 0.0)))
+            out[a, j] = c if output == "complex" else np.abs(c) ** 2
+    return out

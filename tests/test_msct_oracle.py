@@ -1,0 +1,215 @@
+"""Pins the multislice oracle (oracle/msct.py).
+
+* Against the reference itself: TemsimLibrary/slicelib.cpp compiles on its
+  own (oracle/Makefile, target ref -> oracle/_ref/libslicelib_ref.so), so
+  its fe table, Bessel functions, projected potential, look-up table,
+  spline, wavelength, interaction parameter and frequency grid are compared
+  value for value (skipped where the reference tree is absent).
+* Against mathematics: scipy's K0/I0 within the Abramowitz & Stegun error
+  bounds; the 2D Fourier transform of the projected potential equals
+  (al / 2 pi) fe(q) (Kirkland's parameterisation), which fixes the pairing
+  and the constants of vzatom; Akima's spline reproduces straight lines;
+  free-space propagation is unitary and composes; a multislice whose atoms
+  all sit in one slice equals the projection of that slice propagated by
+  the remaining slices; moving every atom by one pixel moves the image by
+  one pixel along the same axis.
+"""
+import ctypes
+import math
+import os
+
+import numpy as np
+import pytest
+import scipy.integrate
+import scipy.special
+
+from oracle import msct as M
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libslicelib_ref.so")
+
+
+@pytest.fixture(scope="module")
+def ref():
+    if not os.path.exists(REF_SO):
+        pytest.skip("reference slicelib not built (oracle/_ref)")
+    L = ctypes.CDLL(REF_SO)
+    d, i = ctypes.c_double, ctypes.c_int
+    for n in ("_Z6bessk0d", "_Z6bessi0d", "_Z10wavelengthd", "_Z5sigmad"):
+        getattr(L, n).restype = d
+        getattr(L, n).argtypes = [d]
+    for n in ("_Z6vzatomid", "_Z9vzatomLUTid"):
+        getattr(L, n).restype = d
+        getattr(L, n).argtypes = [i, d]
+    dp = ctypes.POINTER(d)
+    L._Z6splinhPdS_S_S_S_i.argtypes = [dp, dp, dp, dp, dp, i]
+    L._Z6splinhPdS_S_S_S_i.restype = None
+    fp = ctypes.POINTER(ctypes.c_float)
+    L._Z5freqnPfS_S_id.argtypes = [fp, fp, fp, i, d]
+    L._Z5freqnPfS_S_id.restype = None
+    L._Z11ReadfeTablev.restype = i
+    return L
+
+
+def test_fe_table_is_the_references(ref):
+    ref._Z11ReadfeTablev()
+    fp = ctypes.POINTER(ctypes.POINTER(ctypes.c_double)).in_dll(ref, "fparams")
+    tab = M.fe_params()
+    for Z in range(1, 104):
+        assert [fp[Z][i] for i in range(12)] == list(tab[Z]), Z
+
+
+def test_bessel_wavelength_sigma_match_reference(ref):
+    for x in (1e-6, 0.01, 0.5, 1.0, 1.999, 2.0, 2.001, 3.7, 3.75, 3.8, 10.0, 40.0):
+        assert M.bessk0(x) == ref._Z6bessk0d(x)
+        assert M.bessi0(x) == ref._Z6bessi0d(x)
+    for kev in (80.0, 200.0, 300.0):
+        assert M.wavelength(kev) == ref._Z10wavelengthd(kev)
+        assert M.sigma(kev) == ref._Z5sigmad(kev)
+
+
+def test_potential_and_lut_match_reference(ref):
+    lut = M.VzLUT()
+    for Z in (1, 6, 7, 8, 15, 16, 29, 79, 103):
+        for r in (1e-3, 0.05, 0.3, 1.0, 2.9, 4.9):
+            a, b = ref._Z6vzatomid(Z, r), M.vzatom(Z, r)
+            assert abs(a - b) <= 1e-14 * abs(a)
+        for r2 in (1e-6, 1e-4, 0.0025, 0.3, 1.0, 4.0, 8.99, 9.0, 24.0, 30.0):
+            a, b = ref._Z9vzatomLUTid(Z, r2), float(lut(Z, r2))
+            assert abs(a - b) <= 1e-13 * abs(a), (Z, r2)
+
+
+def test_spline_and_freqn_match_reference(ref):
+    rng = np.random.default_rng(3)
+    x = np.cumsum(rng.uniform(0.1, 1.0, 40))
+    y = np.sin(x) + 0.1 * rng.standard_normal(40)
+    b, c, d = (np.zeros(40) for _ in range(3))
+    dp = ctypes.POINTER(ctypes.c_double)
+    ref._Z6splinhPdS_S_S_S_i(x.ctypes.data_as(dp), y.ctypes.data_as(dp), b.ctypes.data_as(dp),
+                              c.ctypes.data_as(dp), d.ctypes.data_as(dp), 40)
+    b2, c2, d2 = M.splinh(x, y)
+    np.testing.assert_array_equal(b[:-1], b2[:-1])
+    np.testing.assert_array_equal(c[:-1], c2[:-1])
+    np.testing.assert_array_equal(d[:-1], d2[:-1])
+    for n, a in ((64, 51.2), (96, 57.6), (256, 153.6)):
+        k, k2, xo = (np.zeros(n, np.float32) for _ in range(3))
+        fp = ctypes.POINTER(ctypes.c_float)
+        ref._Z5freqnPfS_S_id(k.ctypes.data_as(fp), k2.ctypes.data_as(fp), xo.ctypes.data_as(fp), n, a)
+        kk, kk2 = M.freqn(n, np.float32(a))
+        np.testing.assert_array_equal(k, kk)
+        np.testing.assert_array_equal(k2, kk2)
+
+
+def test_bessel_within_abramowitz_stegun_bounds():
+    for x in np.linspace(0.05, 20, 60):
+        assert abs(M.bessi0(x) - scipy.special.i0(x)) <= 2e-7 * scipy.special.i0(x)
+        assert abs(M.bessk0(x) - scipy.special.k0(x)) <= 2e-7 * max(scipy.special.k0(x), 1e-3) + 1e-8
+
+
+@pytest.mark.parametrize("Z", [6, 8, 16])
+def test_projected_potential_transform_is_kirkland_fe(Z):
+    """int vz(r) J0(2 pi q r) 2 pi r dr = (al / 2 pi) * fe(q), fe(q) =
+    sum a_i / (q^2 + b_i) + sum c_i exp(-d_i q^2)."""
+    fp = M.fe_params()[Z]
+    al = 300.8242834
+    for q in (0.0, 0.5, 1.0):
+        got, _ = scipy.integrate.quad(lambda r: M.vzatom(Z, r) * scipy.special.j0(2 * math.pi * q * r) * 2 * math.pi * r,
+                                      0.0, 25.0, limit=400, points=[0.5, 2.0])
+        fe = sum(fp[i] / (q * q + fp[i + 1]) for i in (0, 2, 4)) + \
+            sum(fp[i] * math.exp(-fp[i + 1] * q * q) for i in (6, 8, 10))
+        want = al / (2 * math.pi) * fe
+        assert abs(got - want) <= 2e-5 * want, (q, got, want)
+
+
+def test_akima_reproduces_lines_and_lut_tracks_potential():
+    x = np.linspace(0.0, 5.0, 12) ** 2
+    b, c, d = M.splinh(x, 3.0 * x - 2.0)
+    assert np.allclose(b[:-1], 3.0) and np.allclose(c[:-1], 0.0, atol=1e-12) and np.allclose(d[:-1], 0.0, atol=1e-12)
+    lut = M.VzLUT()
+    for Z in (6, 16):
+        for r in np.linspace(0.05, 3.0, 25):
+            assert abs(float(lut(Z, r * r)) - M.vzatom(Z, r)) <= 1e-2 * M.vzatom(Z, r)
+
+
+def test_slice_membership():
+    # slice k (1-based) holds z in [(k-1) dz, k dz): 2.5 -> slice 3, 0.0 -> 1
+    zs = np.sort(np.array([0.0, 0.4, 2.5, 2.9999, 3.0, 9.99], np.float32))
+    counts = M.slice_counts(zs, 1.0, 10.0)
+    assert len(counts) == 10 and sum(counts) == 6
+    assert counts[0] == 2 and counts[2] == 2 and counts[3] == 1 and counts[9] == 1
+
+
+def layer(n=12, seed=1, z=(20.2, 20.8)):
+    rng = np.random.default_rng(seed)
+    return dict(Z=np.array([6, 8, 16, 7] * (n // 4)), x=rng.uniform(10, 40, n).astype(np.float32),
+                y=rng.uniform(10, 40, n).astype(np.float32), z=rng.uniform(*z, n).astype(np.float32),
+                occ=np.ones(n, np.float32))
+
+
+P0 = dict(nx=64, ny=64, ctb=51.2, deltaz=1.0, kev=200.0, mode=0)
+
+
+def test_empty_specimen_is_a_plane_wave():
+    at = dict(Z=np.zeros(0, int), x=np.zeros(0, np.float32), y=np.zeros(0, np.float32), z=np.zeros(0, np.float32),
+              occ=np.zeros(0, np.float32))
+    im = M.images(P0, at, [0.3], [1.0], [0.0, 500.0])
+    assert np.abs(im - 1.0).max() < 1e-12
+
+
+def test_one_slice_multislice_is_propagated_projection():
+    """All atoms in slice 21 of 52: the exit wave is that slice's bandwidth-
+    limited transmission function propagated by the 32 slices from there to
+    the exit, so the multislice image at defocus d equals the projection
+    image at d + 32 A."""
+    at = layer()
+    a = M.images(P0, at, [0.0], [0.0], [100.0], output="complex")
+    b = M.images(dict(P0, mode=1), at, [0.0], [0.0], [132.0], output="complex")
+    assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-9
+    assert np.abs(a - 1).max() > 0.01  # the atoms are visible
+
+
+def test_fresnel_unitary_and_composes():
+    m = M.Model(64, 64, 51.2, 1.0, 200.0)
+    rng = np.random.default_rng(5)
+    w = np.exp(1j * 0.3 * rng.standard_normal((64, 64)))
+    a = m.fresnel(w, 300.0)
+    assert abs(np.sum(np.abs(a) ** 2) - np.sum(np.abs(w) ** 2)) < 1e-9 * np.sum(np.abs(w) ** 2)
+    b = m.fresnel(m.fresnel(w, 120.0), 180.0)
+    assert np.linalg.norm(a - b) < 1e-10 * np.linalg.norm(a)
+    # with an aperture only the kept frequencies survive
+    c = m.fresnel(w, 0.0, aobj_rad=0.01)
+    F = np.fft.fft2(c)
+    q = np.hypot(*np.meshgrid(np.fft.fftfreq(64, 0.8), np.fft.fftfreq(64, 0.8)))
+    assert np.abs(F[q >= 0.01 / float(m.wavlen)]).max() < 1e-9
+
+
+def test_pixel_shift_moves_image_along_x_and_y():
+    at = layer(seed=2)
+    a = M.images(P0, at, [0.0], [0.0], [50.0], output="complex")
+    sx = np.float32(np.float32(51.2) / np.float32(64))
+    bx = M.images(P0, dict(at, x=(at["x"] + sx).astype(np.float32)), [0.0], [0.0], [50.0], output="complex")
+    by = M.images(P0, dict(at, y=(at["y"] + sx).astype(np.float32)), [0.0], [0.0], [50.0], output="complex")
+    assert np.linalg.norm(np.roll(a, 1, axis=3) - bx) < 1e-5 * np.linalg.norm(bx - 1)
+    assert np.linalg.norm(np.roll(a, 1, axis=2) - by) < 1e-5 * np.linalg.norm(by - 1)
+
+
+def test_rotation_keeps_distances_and_quarter_turns():
+    rng = np.random.default_rng(4)
+    x, y, z = (rng.uniform(10, 40, 20).astype(np.float32) for _ in range(3))
+    c = np.float32(25.6)
+    for aZ, aY in ((0.4, 1.1), (2.0, -0.7)):
+        X, Y, Z = M.rotate_atoms(x, y, z, aZ, aY, 51.2)
+        r0 = np.sqrt((x - c) ** 2 + (y - c) ** 2 + (z - c) ** 2)
+        r1 = np.sqrt((X - c) ** 2 + (Y - c) ** 2 + (Z - c) ** 2)
+        assert np.allclose(r0, r1, atol=1e-4)
+    # two half turns about Y' return every atom
+    X, Y, Z = M.rotate_atoms(*M.rotate_atoms(x, y, z, 0.0, math.pi, 51.2), 0.0, math.pi, 51.2)
+    assert np.allclose(X, x, atol=1e-4) and np.allclose(Z, z, atol=1e-4) and np.array_equal(Y, y)
+
+
+def test_atoms_leaving_the_cube_are_an_error():
+    # a corner atom turned by 45 degrees about Z lies 34.8 A from the axis
+    at = dict(Z=np.array([6]), x=np.array([1.0], np.float32), y=np.array([1.0], np.float32),
+              z=np.array([25.0], np.float32), occ=np.ones(1, np.float32))
+    with pytest.raises(ValueError):
+        M.images(P0, at, [math.pi / 4], [0.0], [0.0])

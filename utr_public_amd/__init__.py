@@ -42,6 +42,7 @@ EXPORTS = [
     "dht_reset", "dht_dims", "dht_get_timings", "dht_destroy", "dht_set_image_correction",
     "utr_highpass3d", "utr_highpass3d_device", "utr_rotate3d", "utr_rotate3d_device",
     "utr_symmetrize3d", "utr_symmetrize3d_device", "utr_invlaplace3d", "utr_invlaplace3d_device",
+    "msct_create", "msct_images", "msct_images_device", "msct_slices", "msct_destroy",
 ]
 
 DHT_FIELDS = [
@@ -54,6 +55,19 @@ DHT_FIELDS = [
     ("inv_laplace", ctypes.c_int), ("alpha", ctypes.c_double),
     ("norm_f1", ctypes.c_double), ("norm_f2", ctypes.c_double),
 ]
+
+
+MSCT_FIELDS = [
+    ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("mode", ctypes.c_int), ("output", ctypes.c_int),
+    ("ctblength", ctypes.c_double), ("deltaz", ctypes.c_double), ("kev", ctypes.c_double),
+    ("aobj_mrad", ctypes.c_double), ("cs3_mm", ctypes.c_double), ("cs5_mm", ctypes.c_double),
+    ("astig_dz", ctypes.c_double), ("astig_angle_deg", ctypes.c_double),
+    ("dxc", ctypes.c_double), ("dyc", ctypes.c_double), ("dzc", ctypes.c_double),
+]
+
+
+class MsctParams(ctypes.Structure):
+    _fields_ = MSCT_FIELDS
 
 
 class UtrParams(ctypes.Structure):
@@ -180,6 +194,13 @@ def load():
     L.utr_rotate3d_device.argtypes = [vp, vp, i, i, i, d, d, d]
     for f in ("utr_symmetrize3d", "utr_symmetrize3d_device"):
         getattr(L, f).argtypes = [vp, i, i, i, i, dp, dp, dp]
+    fp = ctypes.POINTER(ctypes.c_float)
+    L.msct_create.argtypes = [ctypes.POINTER(MsctParams), dp, i, ctypes.POINTER(ctypes.c_int), fp, fp, fp,
+                              fp, ctypes.POINTER(vp)]
+    L.msct_images.argtypes = [vp, i, dp, dp, i, dp, vp]
+    L.msct_images_device.argtypes = [vp, i, dp, dp, i, dp, vp]
+    L.msct_slices.argtypes = [vp]
+    L.msct_destroy.argtypes = [vp]
     _lib = L
     return L
 
@@ -514,3 +535,88 @@ def dht_reconstruct(params, rows, images):
 
 
 from . import synth  # noqa: E402,F401
+
+
+# --------------------------------------------------------------------------
+# TEM multislice forward model (msct_* C ABI)
+
+FE_PARAMS_PATH = os.path.join(_PKG, "data", "fe_params.txt")
+
+
+def fe_params():
+    """Kirkland (2010) App. C electron scattering-factor parameters,
+    (103, 12) for Z = 1..103 (the table of TemsimLibrary/slicelib.cpp:958-2209)."""
+    return np.loadtxt(FE_PARAMS_PATH, dtype=np.float64).reshape(103, 12)
+
+
+def default_msct_params(**kw):
+    """Multislice set-up: 128^2 pixels over a 51.2 A cube, 1 A slices, 200 keV,
+    no aperture or aberrations (the autosliccmd parameter lines)."""
+    d = dict(nx=128, ny=128, mode=0, output=0, ctblength=51.2, deltaz=1.0, kev=200.0, aobj_mrad=0.0,
+             cs3_mm=0.0, cs5_mm=0.0, astig_dz=0.0, astig_angle_deg=0.0, dxc=0.0, dyc=0.0, dzc=0.0)
+    d.update(kw)
+    return d
+
+
+class Multislice:
+    """One atom set for the multislice forward model (autosliccmd()):
+    atoms = dict(Z, x, y, z[, occ]) in Angstrom inside [0, ctblength]^3."""
+
+    def __init__(self, params, atoms):
+        self._L = load()
+        self.params = dict(params)
+        p = MsctParams()
+        for name, _ in MSCT_FIELDS:
+            setattr(p, name, self.params[name])
+        Z = np.ascontiguousarray(atoms["Z"], dtype=np.int32)
+        n = len(Z)
+        xs = [np.ascontiguousarray(atoms[k], dtype=np.float32) for k in ("x", "y", "z")]
+        occ = np.ascontiguousarray(atoms.get("occ", np.ones(n)), dtype=np.float32)
+        fe = np.ascontiguousarray(fe_params())
+        fpp = ctypes.POINTER(ctypes.c_float)
+        ctx = ctypes.c_void_p()
+        _check(self._L.msct_create(ctypes.byref(p), _dp(fe), n, Z.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                   xs[0].ctypes.data_as(fpp), xs[1].ctypes.data_as(fpp),
+                                   xs[2].ctypes.data_as(fpp), occ.ctypes.data_as(fpp), ctypes.byref(ctx)))
+        self._ctx = ctx
+
+    @property
+    def slices(self):
+        return self._L.msct_slices(self._ctx)
+
+    def images(self, angZ_deg, angY_deg, defocus, out=None):
+        """[orientation][defocus][ny][nx] float32 intensities (complex64 for
+        output 1); angles in degrees about Z then Y', defocus in Angstrom.
+        out: optional torch CUDA tensor of that shape (float32, complex64
+        viewed as float32 pairs) filled on the device."""
+        aZ = np.radians(np.ascontiguousarray(angZ_deg, dtype=np.float64))
+        aY = np.radians(np.ascontiguousarray(angY_deg, dtype=np.float64))
+        df = np.ascontiguousarray(np.atleast_1d(defocus), dtype=np.float64)
+        if len(aZ) != len(aY):
+            raise ValueError("angZ and angY differ in length")
+        nx, ny, cplx_out = self.params["nx"], self.params["ny"], self.params["output"] == 1
+        if out is not None:
+            _check(self._L.msct_images_device(self._ctx, len(aZ), _dp(aZ), _dp(aY), len(df), _dp(df),
+                                              ctypes.c_void_p(out.data_ptr())))
+            return out
+        res = np.empty((len(aZ), len(df), ny, nx), np.complex64 if cplx_out else np.float32)
+        _check(self._L.msct_images(self._ctx, len(aZ), _dp(aZ), _dp(aY), len(df), _dp(df),
+                                   res.ctypes.data_as(ctypes.c_void_p)))
+        return res
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            self._L.msct_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        self.close()
+
+
+def multislice_images(params, atoms, angZ_deg, angY_deg, defocus):
+    """One-shot Multislice(params, atoms).images(...)."""
+    m = Multislice(params, atoms)
+    try:
+        return m.images(angZ_deg, angY_deg, defocus)
+    finally:
+        m.close()

@@ -1,0 +1,789 @@
+// msct.hip -- TEM multislice forward model on gfx950: the coherent branch of
+// TemsimLibrary/autoslic.cpp (autoslic::calculate, lpartl = 0, no thermal
+// configurations), its projected atomic potentials (slicelib.cpp vzatom /
+// vzatomLUT), the per-orientation atom rotation and the free-space
+// propagation of the exit wave to each defocus distance (autosliccmd.cpp),
+// behind the msct_* C ABI of include/utr_hip.h.  Included by utr_api.hip
+// (one translation unit).
+//
+// All orientations of a batch advance through the slices together, so
+// every launch covers B images.  Per slice:
+//   k_ms_potential    projected potential of the slice's atoms, summed per
+//                     pixel (atom order, float) and turned into the
+//                     transmission function exp(i sigma V)
+//                     (autoslic.cpp:917-990)
+//   k_post_lines      x FFT (exponent +1, cfpix::fft) of the transmission
+//   k_ms_cols<0>      y FFT, bandwidth limit k^2 < k2max, y inverse FFT
+//                     (autoslic.cpp:992-1006)
+//   k_ms_mul          x inverse FFTs of transmission and wave, product, x FFT
+//                     of the product (autoslic.cpp:771)
+//   k_ms_cols<1>      y FFT, propagator of the slice with the same limit, y
+//                     inverse FFT (slicelib.cpp:811-849)
+// Between slices the wave stays in the x-transformed state (normalised so
+// that an inverse x FFT returns it), so no plane makes a round trip through
+// the real-space form that the next slice does not need.  After the last
+// slice, per defocus distance R: k_ms_cols<2> multiplies the spectrum by
+// the restated Fresnel factor (autosliccmd.cpp:1072-1087) and k_ms_out
+// returns |psi|^2 (autosliccmd.cpp:1196-1200) or psi.
+
+namespace utr {
+
+constexpr int MS_NR = 100;  // radii of the potential look-up table (slicelib.cpp:2818-2820)
+
+struct alignas(16) MsAtom {
+    float x, y, occ;
+    int zi;        // row of the element in the spline tables
+    int ixo, iyo;  // (int)(x / dx), (int)(y / dy) (autoslic.cpp:950-951)
+    int pad0, pad1;
+};
+
+struct MsGeom {
+    int nx, ny, idx, idy, nsl1;
+    double sx, sy, rminsq, lnx0, inv_dln;
+};
+
+__device__ __forceinline__ int ms_pmod(int a, int n) {
+    const int r = a % n;
+    return r < 0 ? r + n : r;
+}
+
+// vzatomLUT (slicelib.cpp:2868-2886): interval of seval (below the first
+// knot 0, at or above the next-to-last n-2, else x[i] <= r2 < x[i+1]) and the
+// cubic in z = r2 - x[i].
+__device__ __forceinline__ double ms_lut(const double* __restrict__ knots, const double4* __restrict__ coef,
+                                         double r2, const MsGeom& g) {
+    int i;
+    if (r2 <= knots[0]) {
+        i = 0;
+    } else if (r2 >= knots[MS_NR - 2]) {
+        i = MS_NR - 2;
+    } else {
+        i = (int)((logf((float)r2) - (float)g.lnx0) * (float)g.inv_dln);
+        i = min(max(i, 0), MS_NR - 2);
+        while (i < MS_NR - 2 && r2 >= knots[i + 1]) i++;
+        while (i > 0 && r2 < knots[i]) i--;
+    }
+    const double4 c = coef[i];
+    const double z = r2 - knots[i];
+    return c.x + (c.y + (c.z + c.w * z) * z) * z;
+}
+
+// grid (tiles of 16 x 16 pixels, orientation); one thread per pixel.  A
+// pixel takes an atom when its periodic image inside the atom's window
+// [ixo - idx, ixo + idx] x [iyo - idy, iyo + idy] lies within 3 A; windows
+// are narrower than the image (checked on the host), so there is at most
+// one such image.
+__global__ void __launch_bounds__(256) k_ms_potential(const MsAtom* __restrict__ atoms,
+                                                      const int* __restrict__ off, int natom, int s,
+                                                      MsGeom g, const double* __restrict__ knots,
+                                                      const double4* __restrict__ coef, double scale,
+                                                      cplx<float>* __restrict__ trans, int tiles_x) {
+    const int o = blockIdx.y;
+    const int a0 = off[o * g.nsl1 + s], a1 = off[o * g.nsl1 + s + 1];
+    if (a0 == a1) return;  // empty slice: the wave is not transmitted
+    const int tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
+    const int px = tx * 16 + (threadIdx.x & 15), py = ty * 16 + (threadIdx.x >> 4);
+    const bool on = px < g.nx && py < g.ny;
+    __shared__ MsAtom sa[256];
+    const MsAtom* A = atoms + (size_t)o * natom;
+    float acc = 0.f;
+    for (int c0 = a0; c0 < a1; c0 += 256) {
+        const int n = min(256, a1 - c0);
+        __syncthreads();
+        if ((int)threadIdx.x < n) sa[threadIdx.x] = A[c0 + threadIdx.x];
+        __syncthreads();
+        if (!on) continue;
+        for (int k = 0; k < n; k++) {
+            const MsAtom at = sa[k];
+            const int x0 = at.ixo - g.idx;
+            const int ix = x0 + ms_pmod(px - x0, g.nx);
+            if (ix > at.ixo + g.idx) continue;
+            const int y0 = at.iyo - g.idy;
+            const int iy = y0 + ms_pmod(py - y0, g.ny);
+            if (iy > at.iyo + g.idy) continue;
+            const double rx = (double)at.x - (double)ix * g.sx;
+            const double ry = (double)at.y - (double)iy * g.sy;
+            double r2 = rx * rx + ry * ry;
+            if (r2 > 9.0) continue;
+            if (r2 < g.rminsq) r2 = g.rminsq;
+            acc += (float)((double)at.occ * ms_lut(knots, coef + (size_t)at.zi * MS_NR, r2, g));
+        }
+    }
+    if (!on) return;
+    float sn, cs;
+    sincosf((float)(scale * (double)acc), &sn, &cs);
+    trans[((size_t)o * g.ny + py) * g.nx + px] = {cs, sn};
+}
+
+// Column pass over [B][ny][nx] (tile = NB adjacent kx columns of one
+// orientation): y FFT (exponent +1), multiply, y inverse FFT (exponent -1),
+// store into dst.  MODE 0: bandwidth limit, factor = scale where
+// kx2 + ky2 < k2max (float, as autoslic.cpp:1000-1003), else 0.  MODE 1: the
+// same limit times the propagator py * px (slicelib.cpp:827-835).  MODE 2:
+// the Fresnel table F[ky][kx] times scale.
+template <int N, int MODE>
+__global__ void __launch_bounds__(256) k_ms_cols(const cplx<float>* __restrict__ src, cplx<float>* __restrict__ dst,
+                                                 int nx, int gx, int gy, const cplx<float>* __restrict__ tw,
+                                                 const float* __restrict__ kx2, const float* __restrict__ ky2,
+                                                 float k2max, const cplx<float>* __restrict__ px,
+                                                 const cplx<float>* __restrict__ py,
+                                                 const cplx<float>* __restrict__ F, float scale,
+                                                 const int* __restrict__ off, int nsl1, int s) {
+    constexpr int NB = lds_batch(N, sizeof(cplx<float>), LDS_POST);
+    __shared__ cplx<float> sm[2 * lds_len(NB * N)];
+    cplx<float>* a = sm;
+    cplx<float>* b = sm + lds_len(NB * N);
+    int tx, o;
+    if (!xcd_tile(gx, gy, tx, o)) return;
+    if (MODE == 0 && off[o * nsl1 + s] == off[o * nsl1 + s + 1]) return;
+    const int c0 = tx * NB;
+    const int nb = min(NB, nx - c0);
+    const size_t base = (size_t)o * N * nx + c0;
+    for (int t = threadIdx.x; t < nb * N; t += blockDim.x) {
+        const int y = t / nb, c = t - y * nb;
+        a[lx(c * N + y)] = src[base + (size_t)y * nx + c];
+    }
+    __syncthreads();
+    fft_lds<float, N>(a, b, nb, tw, +1, threadIdx.x, blockDim.x);
+    for (int t = threadIdx.x; t < nb * N; t += blockDim.x) {
+        const int c = t / N, y = t - c * N;
+        const int kx = c0 + c;
+        cplx<float> v = a[lx(t)];
+        if (MODE == 2) {
+            v = cmul(v, F[(size_t)y * nx + kx]);
+            v = {v.x * scale, v.y * scale};
+        } else if (kx2[kx] + ky2[y] < k2max) {
+            if (MODE == 1) v = cmul(cmul(v, py[y]), px[kx]);
+            v = {v.x * scale, v.y * scale};
+        } else {
+            v = {0.f, 0.f};
+        }
+        a[lx(t)] = v;
+    }
+    __syncthreads();
+    fft_lds<float, N>(a, b, nb, tw, -1, threadIdx.x, blockDim.x);
+    for (int t = threadIdx.x; t < nb * N; t += blockDim.x) {
+        const int y = t / nb, c = t - y * nb;
+        dst[base + (size_t)y * nx + c] = a[lx(c * N + y)];
+    }
+}
+
+// Lines of [B][ny][nx]: NB lines per workgroup.  Multislice (PROJ false):
+// the wave W and the transmission T are both x-transformed; inverse x FFT of
+// both, W * T where the slice holds atoms for this orientation, x FFT of the
+// product back into W.  Projection (PROJ true): W is real-space; inverse x
+// FFT of T only and W *= T.
+template <int N, bool PROJ>
+__global__ void __launch_bounds__(256) k_ms_mul(cplx<float>* __restrict__ W, const cplx<float>* __restrict__ T,
+                                                int ny, int nlines, const cplx<float>* __restrict__ tw,
+                                                const int* __restrict__ off, int nsl1, int s) {
+    constexpr int NB = lds_batch(N, sizeof(cplx<float>), LDS_POST) / 2;
+    static_assert(NB >= 1, "two sequences per line must fit");
+    __shared__ cplx<float> sm[2 * lds_len(2 * NB * N)];
+    cplx<float>* a = sm;
+    cplx<float>* b = sm + lds_len(2 * NB * N);
+    const int l0 = blockIdx.x * NB;
+    const int nb = min(NB, nlines - l0);
+    const size_t base = (size_t)l0 * N;
+    // lines of one workgroup may straddle two orientations: flag per line
+    if (PROJ) {
+        for (int t = threadIdx.x; t < nb * N; t += blockDim.x) a[lx(t)] = T[base + t];
+        __syncthreads();
+        fft_lds<float, N>(a, b, nb, tw, -1, threadIdx.x, blockDim.x);
+        for (int t = threadIdx.x; t < nb * N; t += blockDim.x) {
+            const int o = (l0 + t / N) / ny;
+            if (off[o * nsl1 + s] == off[o * nsl1 + s + 1]) continue;
+            W[base + t] = cmul(W[base + t], a[lx(t)]);
+        }
+        return;
+    }
+    for (int t = threadIdx.x; t < nb * N; t += blockDim.x) {
+        a[lx(t)] = W[base + t];
+        a[lx(nb * N + t)] = T[base + t];
+    }
+    __syncthreads();
+    fft_lds<float, N>(a, b, 2 * nb, tw, -1, threadIdx.x, blockDim.x);
+    for (int t = threadIdx.x; t < nb * N; t += blockDim.x) {
+        const int o = (l0 + t / N) / ny;
+        if (off[o * nsl1 + s] != off[o * nsl1 + s + 1]) a[lx(t)] = cmul(a[lx(t)], a[lx(nb * N + t)]);
+    }
+    __syncthreads();
+    fft_lds<float, N>(a, b, nb, tw, +1, threadIdx.x, blockDim.x);
+    for (int t = threadIdx.x; t < nb * N; t += blockDim.x) W[base + t] = a[lx(t)];
+}
+
+// Lines of E [B][ny][nx] (x-transformed, normalised): inverse x FFT, then
+// |psi|^2 (complex_out false) or psi into out[(o * ndef + d) * ny + y].
+template <int N>
+__global__ void __launch_bounds__(256) k_ms_out(const cplx<float>* __restrict__ E, float* __restrict__ out,
+                                                int ny, int nlines, int ndef, int d, int complex_out,
+                                                const cplx<float>* __restrict__ tw) {
+    constexpr int NB = lds_batch(N, sizeof(cplx<float>), LDS_POST);
+    __shared__ cplx<float> sm[2 * lds_len(NB * N)];
+    cplx<float>* a = sm;
+    cplx<float>* b = sm + lds_len(NB * N);
+    const int l0 = blockIdx.x * NB;
+    const int nb = min(NB, nlines - l0);
+    for (int t = threadIdx.x; t < nb * N; t += blockDim.x) a[lx(t)] = E[(size_t)l0 * N + t];
+    __syncthreads();
+    fft_lds<float, N>(a, b, nb, tw, -1, threadIdx.x, blockDim.x);
+    for (int t = threadIdx.x; t < nb * N; t += blockDim.x) {
+        const int l = l0 + t / N, x = t % N;
+        const int o = l / ny, y = l - o * ny;
+        const size_t row = ((size_t)o * ndef + d) * ny + y;
+        const cplx<float> v = a[lx(t)];
+        if (complex_out) {
+            out[2 * (row * N + x)] = v.x;
+            out[2 * (row * N + x) + 1] = v.y;
+        } else {
+            out[row * N + x] = v.x * v.x + v.y * v.y;
+        }
+    }
+}
+
+// wave before the first slice: plane wave 1.  x-transformed and normalised
+// that is 1 at kx = 0 and 0 elsewhere; real-space (projection) it is 1.
+__global__ void k_ms_init(cplx<float>* __restrict__ W, int nx, size_t n, int proj) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    W[t] = (proj || t % nx == 0) ? cplx<float>{1.f, 0.f} : cplx<float>{0.f, 0.f};
+}
+
+}  // namespace utr
+
+namespace {
+
+constexpr double kMsPi = 3.14159265358979323846;
+
+// A&S 9.8.1-9.8.2 (slicelib.cpp:185-212)
+double ms_bessi0(double x) {
+    const double ax = std::fabs(x);
+    if (ax <= 3.75) {
+        static const double c[7] = {1.0, 3.5156229, 3.0899424, 1.2067492, 0.2659732, 0.0360768, 0.0045813};
+        const double t = (x / 3.75) * (x / 3.75);
+        double s = c[6];
+        for (int i = 5; i >= 0; i--) s = s * t + c[i];
+        return s;
+    }
+    static const double c[9] = {0.39894228,  0.01328592, 0.00225319,  -0.00157565, 0.00916281,
+                                -0.02057706, 0.02635537, -0.01647633, 0.00392377};
+    const double t = 3.75 / ax;
+    double s = c[8];
+    for (int i = 7; i >= 0; i--) s = s * t + c[i];
+    return std::exp(ax) * s / std::sqrt(ax);
+}
+
+// A&S 9.8.5-9.8.6 (slicelib.cpp:217-245)
+double ms_bessk0(double x) {
+    const double ax = std::fabs(x);
+    if (ax > 0.0 && ax <= 2.0) {
+        static const double c[7] = {-0.57721566, 0.42278420, 0.23069756, 0.03488590,
+                                    0.00262698,  0.00010750, 0.00000740};
+        const double t = (ax / 2.0) * (ax / 2.0);
+        double s = c[6];
+        for (int i = 5; i >= 0; i--) s = s * t + c[i];
+        return -std::log(ax / 2.0) * ms_bessi0(x) + s;
+    }
+    if (ax > 2.0) {
+        static const double c[7] = {1.25331414,  -0.07832358, 0.02189568, -0.01062446,
+                                    0.00587872, -0.00251540, 0.00053208};
+        const double t = 2.0 / ax;
+        double s = c[6];
+        for (int i = 5; i >= 0; i--) s = s * t + c[i];
+        return std::exp(-ax) * s / std::sqrt(ax);
+    }
+    return 1.0e20;
+}
+
+// projected potential of one element from its 12 fe parameters
+// (slicelib.cpp:2759-2792): Kirkland's al = 4 pi^2 a0 e, ag = 2 pi^2 a0 e
+// and the 10-digit pi of that routine.
+double ms_vzatom(const double* fp, double r) {
+    const double al = 300.8242834, ag = 150.4121417, pi = 3.141592654;
+    r = std::fabs(r);
+    if (r < 1.0e-10) r = 1.0e-10;
+    double suml = 0.0, sumg = 0.0;
+    double x = 2.0 * pi * r;
+    for (int i = 0; i < 6; i += 2) suml += fp[i] * ms_bessk0(x * std::sqrt(fp[i + 1]));
+    x = pi * r;
+    x = x * x;
+    for (int i = 6; i < 12; i += 2) sumg += fp[i] * std::exp(-x / fp[i + 1]) / fp[i + 1];
+    return al * suml + ag * sumg;
+}
+
+// Akima quasi-Hermite spline (H. Akima, J. ACM 17 (1970) 589; the
+// reference's splinh, slicelib.cpp:2537-2603): slopes t_i from the weighted
+// neighbouring chords, two extrapolated chords at each end.
+void ms_akima(const double* x, const double* y, double* b, double* c, double* d, int n) {
+    const double SMALL = 1.0e-25;
+    double m5 = (y[3] - y[2]) / (x[3] - x[2]);
+    double m4 = (y[2] - y[1]) / (x[2] - x[1]);
+    double m3 = (y[1] - y[0]) / (x[1] - x[0]);
+    double m2 = m3 + m3 - m4;
+    double m1 = m2 + m2 - m3;
+    double m54 = std::fabs(m5 - m4), m43 = std::fabs(m4 - m3), m32 = std::fabs(m3 - m2),
+           m21 = std::fabs(m2 - m1);
+    double t1 = (m43 + m21) > SMALL ? (m43 * m2 + m21 * m3) / (m43 + m21) : 0.5 * (m2 + m3);
+    for (int i = 0; i < n - 1; i++) {
+        const double t2 = (m54 + m32) > SMALL ? (m54 * m3 + m32 * m4) / (m54 + m32) : 0.5 * (m3 + m4);
+        const double h = x[i + 1] - x[i];
+        b[i] = t1;
+        c[i] = (3.0 * m3 - t1 - t1 - t2) / h;
+        d[i] = (t1 + t2 - m3 - m3) / (h * h);
+        m1 = m2;
+        m2 = m3;
+        m3 = m4;
+        m4 = m5;
+        m5 = i < n - 4 ? (y[i + 4] - y[i + 3]) / (x[i + 4] - x[i + 3]) : m4 + m4 - m3;
+        m21 = m32;
+        m32 = m43;
+        m43 = m54;
+        m54 = std::fabs(m5 - m4);
+        t1 = t2;
+    }
+    b[n - 1] = c[n - 1] = d[n - 1] = 0.0;
+}
+
+// spatial frequency of FFT bin i (slicelib.cpp:457-476), float
+float ms_freq(int i, int n, float a) {
+    const int imid = (int)(n / 2.0 + 0.5);
+    return i > imid ? (float)(i - n) / a : (float)i / a;
+}
+
+}  // namespace
+
+struct msct_ctx {
+    msct_params p{};
+    int natom = 0;
+    std::vector<int> zi;  // element row per atom
+    std::vector<float> x, y, z, occ;
+    int nsl = 0;          // slices per orientation
+    float ctbf = 0.f, k2max = 0.f, wavlen = 0.f, aobj = 0.f;
+    utr::MsGeom g{};
+    double scale = 0.0;   // sigma(kev) / 1000
+    DevBuf<double> knots;
+    DevBuf<double4> coef;
+    DevBuf<float> kx2, ky2;
+    DevBuf<cplx<float>> px, py, twx, twy, F;
+    DevBuf<utr::MsAtom> atoms;
+    DevBuf<int> off;
+    DevBuf<cplx<float>> W, T, E;
+    int batch = 0;
+};
+
+namespace {
+
+int ms_fail_invalid(const char* m) { return set_err(UTR_E_INVALID, m); }
+
+// the reference's wavelength() and sigma() (slicelib.cpp:2441-2455, 2891-2900)
+double ms_wavelength(double kev) { return 12.3984244 / std::sqrt(kev * (2 * 510.99906 + kev)); }
+double ms_sigma(double kev) {
+    const double x = (510.99906 + kev) / (2.0 * 510.99906 + kev);
+    return 2.0 * kMsPi * x / (ms_wavelength(kev) * kev);
+}
+
+// number of slices: zslice = k * deltaz (k = 1, 2, ...) while zslice <
+// zmax + deltaz (autoslic.cpp:727-744, 794), zmax = the cube side
+int ms_slice_count(float ctb, float deltaz) {
+    const double dz = deltaz, lim = (double)ctb + dz;
+    int n = 0;
+    for (double zs = dz; zs < lim; zs += dz) n++;
+    return n;
+}
+
+// Fresnel factor table for the +1-exponent spectrum (restated FresnelA):
+// bin (kx, ky) holds the frequency of bin (-kx, -ky) of the -1-exponent
+// convention, exp(-i chi) with chi of the oracle, 0 outside the objective
+// aperture.
+__global__ void k_ms_fresnel(cplx<float>* __restrict__ F, int nx, int ny, double ctb, double wl, double R,
+                             double cs3A, double cs5A, double adz, double ca, double sa, double k2maxo) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nx * ny) return;
+    const int kx = t % nx, ky = t / nx;
+    const int mx = (nx - kx) % nx, my = (ny - ky) % ny;
+    const double qx = (mx < nx / 2 ? mx : mx - nx) / ctb;
+    const double qy = (my < ny / 2 ? my : my - ny) / ctb;
+    const double q2 = qx * qx + qy * qy;
+    const double pi = 3.14159265358979323846;
+    double chi = pi * wl * R * q2 + pi * wl * wl * wl * cs3A * q2 * q2 / 2.0 +
+                 pi * wl * wl * wl * wl * wl * cs5A * q2 * q2 * q2 / 3.0;
+    if (adz != 0.0) chi += pi * wl * adz * ((qx * qx - qy * qy) * ca + 2.0 * qx * qy * sa);
+    if (k2maxo > 0.0 && q2 >= k2maxo) {
+        F[t] = {0.f, 0.f};
+        return;
+    }
+    double s, c;
+    sincos(chi, &s, &c);
+    F[t] = {(float)c, (float)-s};
+}
+
+int ms_run_cols(int mode, msct_ctx* c, const cplx<float>* src, cplx<float>* dst, int B, float scale, int s,
+                const cplx<float>* F, hipStream_t st) {
+    const int nx = c->g.nx, ny = c->g.ny;
+    bool ok = false;
+#define MS_COLS(N)                                                                                  \
+    {                                                                                               \
+        constexpr int NB = utr::lds_batch(N, sizeof(cplx<float>), utr::LDS_POST);                   \
+        const int gx = (nx + NB - 1) / NB;                                                          \
+        const unsigned grid = utr::xcd_grid((unsigned)(gx * B));                                    \
+        if (mode == 0)                                                                              \
+            utr::k_ms_cols<N, 0><<<grid, 256, 0, st>>>(src, dst, nx, gx, B, c->twy.p, c->kx2.p,     \
+                                                       c->ky2.p, c->k2max, c->px.p, c->py.p,        \
+                                                       F, scale, c->off.p, c->g.nsl1, s);      \
+        else if (mode == 1)                                                                         \
+            utr::k_ms_cols<N, 1><<<grid, 256, 0, st>>>(src, dst, nx, gx, B, c->twy.p, c->kx2.p,     \
+                                                       c->ky2.p, c->k2max, c->px.p, c->py.p,        \
+                                                       F, scale, c->off.p, c->g.nsl1, s);      \
+        else                                                                                        \
+            utr::k_ms_cols<N, 2><<<grid, 256, 0, st>>>(src, dst, nx, gx, B, c->twy.p, c->kx2.p,     \
+                                                       c->ky2.p, c->k2max, c->px.p, c->py.p,        \
+                                                       F, scale, c->off.p, c->g.nsl1, s);      \
+        ok = true;                                                                                  \
+    }
+    UTR_DISPATCH_N(ny, MS_COLS)
+#undef MS_COLS
+    if (!ok) return set_err(UTR_E_UNSUPPORTED, "msct: unsupported image height for the FFT");
+    UTR_HIP_TRY(hipGetLastError());
+    return UTR_OK;
+}
+
+int ms_run_lines(msct_ctx* c, int what, int B, int s, int ndef, int d, float* out, hipStream_t st) {
+    const int nx = c->g.nx, ny = c->g.ny, nlines = B * ny;
+    const bool proj = c->p.mode != 0;
+    bool ok = false;
+#define MS_LINES(N)                                                                                 \
+    {                                                                                               \
+        constexpr int NB = utr::lds_batch(N, sizeof(cplx<float>), utr::LDS_POST);                   \
+        constexpr int NB2 = NB / 2;                                                                 \
+        if (what == 0) {                                                                            \
+            utr::k_post_lines<float, N><<<(nlines + NB - 1) / NB, 256, 0, st>>>(c->T.p, nlines, +1, \
+                                                                               c->twx.p);           \
+        } else if (what == 1) {                                                                     \
+            if (proj)                                                                               \
+                utr::k_ms_mul<N, true><<<(nlines + NB2 - 1) / NB2, 256, 0, st>>>(                   \
+                    c->W.p, c->T.p, ny, nlines, c->twx.p, c->off.p, c->g.nsl1, s);                  \
+            else                                                                                    \
+                utr::k_ms_mul<N, false><<<(nlines + NB2 - 1) / NB2, 256, 0, st>>>(                  \
+                    c->W.p, c->T.p, ny, nlines, c->twx.p, c->off.p, c->g.nsl1, s);                  \
+        } else if (what == 2) {                                                                     \
+            utr::k_post_lines<float, N><<<(nlines + NB - 1) / NB, 256, 0, st>>>(c->W.p, nlines, +1, \
+                                                                               c->twx.p);           \
+        } else {                                                                                    \
+            utr::k_ms_out<N><<<(nlines + NB - 1) / NB, 256, 0, st>>>(c->E.p, out, ny, nlines, ndef, \
+                                                                    d, c->p.output, c->twx.p);      \
+        }                                                                                           \
+        ok = true;                                                                                  \
+    }
+    UTR_DISPATCH_N(nx, MS_LINES)
+#undef MS_LINES
+    if (!ok) return set_err(UTR_E_UNSUPPORTED, "msct: unsupported image width for the FFT");
+    UTR_HIP_TRY(hipGetLastError());
+    return UTR_OK;
+}
+
+// Rotates, checks, z-sorts and slices the atoms of orientations [a0, a0+B)
+// into A (B x natom) and off (B x (nsl + 1)); autosliccmd.cpp:709-748 in
+// float, then the bounds test of autosliccmd.cpp:857-860 and the depth sort
+// and slice membership of autoslic.cpp:720-744.
+int ms_prepare(msct_ctx* c, int a0, int B, const double* angZ, const double* angY,
+               std::vector<utr::MsAtom>& A, std::vector<int>& off) {
+    const int n = c->natom, nsl1 = c->nsl + 1;
+    A.resize((size_t)B * n);
+    off.assign((size_t)B * nsl1, 0);
+    const float ctb = c->ctbf;
+    const float xc = (float)(c->p.ctblength / 2.0 + c->p.dxc), yc = (float)(c->p.ctblength / 2.0 + c->p.dyc),
+                zc = (float)(c->p.ctblength / 2.0 + c->p.dzc);
+    std::vector<float> X(n), Y(n), Z(n);
+    std::vector<int> idx(n);
+    const double dz = (double)(float)c->p.deltaz;
+    for (int b = 0; b < B; b++) {
+        const double aZ = angZ[a0 + b], aY = angY[a0 + b];
+        const float sinZ = (float)std::sin(aZ), cosZ = (float)std::cos(aZ), sinY = (float)std::sin(aY),
+                    cosY = (float)std::cos(aY);
+        for (int k = 0; k < n; k++) {
+            float xx = xc + (c->x[k] - xc) * cosZ + (-c->y[k] + yc) * sinZ;
+            float yy = yc + (c->x[k] - xc) * sinZ + (c->y[k] - yc) * cosZ;
+            float zz = c->z[k];
+            const float x1 = xc + (xx - xc) * cosY + (zz - zc) * sinY;
+            zz = zc + (-xx + xc) * sinY + (zz - zc) * cosY;
+            X[k] = x1;
+            Y[k] = yy;
+            Z[k] = zz;
+            if (!(x1 >= 0.f && yy >= 0.f && zz >= 0.f && x1 <= ctb && yy <= ctb && zz <= ctb))
+                return set_err(UTR_E_INVALID,
+                               "msct: an atom leaves the cube [0, ctblength]^3 at orientation " +
+                                   std::to_string(a0 + b) + " (the reference stops with the same error)");
+        }
+        for (int k = 0; k < n; k++) idx[k] = k;
+        std::stable_sort(idx.begin(), idx.end(), [&](int i, int j) { return Z[i] < Z[j]; });
+        utr::MsAtom* dst = A.data() + (size_t)b * n;
+        for (int k = 0; k < n; k++) {
+            const int i = idx[k];
+            utr::MsAtom& m = dst[k];
+            m.x = X[i];
+            m.y = Y[i];
+            m.occ = c->occ[i];
+            m.zi = c->zi[i];
+            m.ixo = (int)((double)X[i] / c->g.sx);
+            m.iyo = (int)((double)Y[i] / c->g.sy);
+            m.pad0 = m.pad1 = 0;
+        }
+        int* o = off.data() + (size_t)b * nsl1;
+        int istart = 0;
+        double zslice = dz;
+        for (int s = 0; s < c->nsl; s++) {
+            o[s] = istart;
+            while (istart < n && (double)Z[idx[istart]] < zslice) istart++;
+            zslice += dz;
+        }
+        o[c->nsl] = istart;
+    }
+    return UTR_OK;
+}
+
+int ms_images(msct_ctx* c, int nang, const double* angZ, const double* angY, int ndef, const double* defocus,
+              float* out, bool device_out) {
+    if (!c) return ms_fail_invalid("null msct context");
+    if (nang < 1 || ndef < 1 || !angZ || !angY || !defocus || !out)
+        return ms_fail_invalid("msct_images: need nang >= 1 orientations, ndef >= 1 defocus distances and an output");
+    for (int a = 0; a < nang; a++)
+        if (!std::isfinite(angZ[a]) || !std::isfinite(angY[a])) return ms_fail_invalid("msct: angles must be finite");
+    for (int d = 0; d < ndef; d++)
+        if (!std::isfinite(defocus[d])) return ms_fail_invalid("msct: defocus distances must be finite");
+    const int nx = c->g.nx, ny = c->g.ny;
+    const size_t plane = (size_t)nx * ny;
+    const size_t per_img = plane * (c->p.output ? 2 : 1);
+    const int B = std::min(nang, c->batch);
+    UTR_HIP_TRY(c->W.alloc(plane * B));
+    UTR_HIP_TRY(c->T.alloc(plane * B));
+    UTR_HIP_TRY(c->E.alloc(plane * B));
+    UTR_HIP_TRY(c->atoms.alloc((size_t)B * std::max(c->natom, 1)));
+    UTR_HIP_TRY(c->off.alloc((size_t)B * (c->nsl + 1)));
+    UTR_HIP_TRY(c->F.alloc(plane * ndef));
+    LocalBuf<float> dout;
+    if (!device_out) UTR_HIP_TRY(dout.alloc(per_img * ndef * B));
+    hipStream_t st = nullptr;
+    UTR_HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    auto run = [&]() -> int {
+        // Fresnel tables, one per defocus distance: R = defocus - ctblength/2
+        // (autosliccmd.cpp:1065), Cs in mm -> A (autosliccmd.cpp:1049-1050)
+        const double wl = (double)c->wavlen;
+        const double k2maxo = c->aobj > 0.f ? (double)((c->aobj / c->wavlen) * (c->aobj / c->wavlen)) : 0.0;
+        const double aa = c->p.astig_angle_deg * kMsPi / 180.0;
+        for (int d = 0; d < ndef; d++) {
+            k_ms_fresnel<<<(unsigned)((plane + 255) / 256), 256, 0, st>>>(
+                c->F.p + plane * d, nx, ny, (double)c->ctbf, wl, defocus[d] - 0.5 * c->p.ctblength,
+                (double)(float)c->p.cs3_mm * 1.e7, (double)(float)c->p.cs5_mm * 1.e7, c->p.astig_dz,
+                std::cos(2.0 * aa), std::sin(2.0 * aa), k2maxo);
+            UTR_HIP_TRY(hipGetLastError());
+        }
+        std::vector<utr::MsAtom> A;
+        std::vector<int> off;
+        const bool proj = c->p.mode != 0;
+        const int tiles_x = (nx + 15) / 16, tiles = tiles_x * ((ny + 15) / 16);
+        const float inv = 1.f / (float)plane;
+        for (int a0 = 0; a0 < nang; a0 += B) {
+            const int nb = std::min(B, nang - a0);
+            int rc = ms_prepare(c, a0, nb, angZ, angY, A, off);
+            if (rc) return rc;
+            UTR_HIP_TRY(hipStreamSynchronize(st));  // the previous batch no longer reads atoms/off
+            if (c->natom > 0)
+                UTR_HIP_TRY(hipMemcpyAsync(c->atoms.p, A.data(), sizeof(utr::MsAtom) * A.size(),
+                                           hipMemcpyHostToDevice, st));
+            UTR_HIP_TRY(hipMemcpyAsync(c->off.p, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice, st));
+            utr::k_ms_init<<<(unsigned)((plane * nb + 255) / 256), 256, 0, st>>>(c->W.p, nx, plane * nb,
+                                                                                 proj ? 1 : 0);
+            UTR_HIP_TRY(hipGetLastError());
+            for (int s = 0; s < c->nsl; s++) {
+                if (c->natom > 0) {
+                    utr::k_ms_potential<<<dim3(tiles, nb), 256, 0, st>>>(c->atoms.p, c->off.p, c->natom, s, c->g,
+                                                                          c->knots.p, c->coef.p, c->scale, c->T.p,
+                                                                          tiles_x);
+                    UTR_HIP_TRY(hipGetLastError());
+                    if ((rc = ms_run_lines(c, 0, nb, s, 0, 0, nullptr, st))) return rc;
+                    if ((rc = ms_run_cols(0, c, c->T.p, c->T.p, nb, inv, s, nullptr, st))) return rc;
+                    if ((rc = ms_run_lines(c, 1, nb, s, 0, 0, nullptr, st))) return rc;
+                } else if (!proj) {
+                    if ((rc = ms_run_lines(c, 1, nb, s, 0, 0, nullptr, st))) return rc;
+                }
+                if (!proj && (rc = ms_run_cols(1, c, c->W.p, c->W.p, nb, inv, s, nullptr, st))) return rc;
+            }
+            // exit wave -> spectrum along x: already so (multislice, normalised)
+            // or one x FFT of the real-space wave (projection, unnormalised)
+            if (proj && (rc = ms_run_lines(c, 2, nb, 0, 0, 0, nullptr, st))) return rc;
+            const float fs = proj ? inv : 1.f / (float)ny;
+            float* dst = device_out ? out + per_img * ndef * (size_t)a0 : dout.p;
+            for (int d = 0; d < ndef; d++) {
+                if ((rc = ms_run_cols(2, c, c->W.p, c->E.p, nb, fs, 0, c->F.p + plane * d, st))) return rc;
+                if ((rc = ms_run_lines(c, 3, nb, 0, ndef, d, dst, st))) return rc;
+            }
+            if (!device_out)
+                UTR_HIP_TRY(hipMemcpyAsync(out + per_img * ndef * (size_t)a0, dout.p,
+                                           sizeof(float) * per_img * ndef * nb, hipMemcpyDeviceToHost, st));
+        }
+        UTR_HIP_TRY(hipStreamSynchronize(st));
+        return UTR_OK;
+    };
+    const int rc = run();
+    (void)hipStreamSynchronize(st);
+    (void)hipStreamDestroy(st);
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int msct_create(const msct_params* p, const double* fe_params, int natom, const int* Z, const float* x,
+                const float* y, const float* z, const float* occ, msct_ctx** out) {
+    if (!out) return ms_fail_invalid("null output handle");
+    *out = nullptr;
+    if (!p || !fe_params) return ms_fail_invalid("msct_create: null parameters or fe table");
+    if (!fft_size_supported(p->nx) || !fft_size_supported(p->ny))
+        return set_err(UTR_E_UNSUPPORTED, "msct: image size not supported by the FFT kernels (supported: 16 32 48 "
+                                          "64 96 128 160 192 256 320 384 512)");
+    const double vals[] = {p->ctblength, p->deltaz, p->kev, p->aobj_mrad, p->cs3_mm, p->cs5_mm,
+                           p->astig_dz, p->astig_angle_deg, p->dxc, p->dyc, p->dzc};
+    for (double v : vals)
+        if (!std::isfinite(v)) return ms_fail_invalid("msct: parameters must be finite");
+    if (!(p->ctblength > 0) || !(p->deltaz > 0) || !(p->kev > 0))
+        return ms_fail_invalid("msct: cube side, slice thickness and energy must be positive");
+    if (p->mode < 0 || p->mode > 2) return ms_fail_invalid("msct: mode must be 0 (multislice), 1 or 2 (projection)");
+    if (p->output < 0 || p->output > 1) return ms_fail_invalid("msct: output must be 0 (intensity) or 1 (complex)");
+    if (natom < 0 || (natom > 0 && (!Z || !x || !y || !z || !occ))) return ms_fail_invalid("msct: bad atom arrays");
+    for (int k = 0; k < natom; k++) {
+        if (Z[k] < 1 || Z[k] > 103) return ms_fail_invalid("msct: atomic numbers must be in 1..103");
+        if (!std::isfinite(x[k]) || !std::isfinite(y[k]) || !std::isfinite(z[k]) || !std::isfinite(occ[k]))
+            return ms_fail_invalid("msct: atom coordinates and occupancies must be finite");
+    }
+    for (int i = 0; i < 103 * 12; i++)
+        if (!std::isfinite(fe_params[i])) return ms_fail_invalid("msct: fe table must be finite");
+    msct_ctx* c = new msct_ctx;
+    c->p = *p;
+    auto fail = [&](int rc) {
+        msct_destroy(c);
+        return rc;
+    };
+    const int nx = p->nx, ny = p->ny;
+    const float ax = (float)p->ctblength;  // param[pAX] (autosliccmd.cpp:894-895)
+    c->ctbf = ax;
+    const float kevf = (float)p->kev;
+    c->wavlen = (float)ms_wavelength(kevf);
+    c->scale = ms_sigma(kevf) / 1000.0;
+    c->aobj = std::fabs((float)p->aobj_mrad * 0.001f);  // autosliccmd.cpp:312
+    // trlayer geometry (autoslic.cpp:930-941): float steps, double radii
+    utr::MsGeom& g = c->g;
+    g.nx = nx;
+    g.ny = ny;
+    g.sx = (double)(ax / (float)nx);
+    g.sy = (double)(ax / (float)ny);
+    double rmin = ax / (double)nx;
+    const double r = ax / (double)ny;
+    rmin = 0.25 * std::sqrt(0.5 * (rmin * rmin + r * r));
+    g.rminsq = rmin * rmin;
+    g.idx = (int)(nx * 3.0 / ax) + 1;
+    g.idy = (int)(ny * 3.0 / ax) + 1;
+    if (2 * g.idx + 1 > nx || 2 * g.idy + 1 > ny)
+        return fail(ms_fail_invalid("msct: the 3 A atom window is wider than the image (cube side too small)"));
+    c->nsl = ms_slice_count(ax, (float)p->deltaz);
+    g.nsl1 = c->nsl + 1;
+    // bandwidth limit (autoslic.cpp:382-393) and propagator (autoslic.cpp:395-415), float
+    float k2max = (float)nx / (2.0f * ax);
+    const float tcty = (float)ny / (2.0f * ax);
+    if (tcty < k2max) k2max = tcty;
+    k2max = (2.0f / 3.0f) * k2max;
+    c->k2max = k2max * k2max;
+    const float pscale = 3.14159265f * (float)p->deltaz;
+    std::vector<float> kx2(nx), ky2(ny);
+    std::vector<cplx<float>> px(nx), py(ny);
+    for (int i = 0; i < nx; i++) {
+        const float k = ms_freq(i, nx, ax);
+        kx2[i] = k * k;
+        const float t = pscale * (kx2[i] * c->wavlen);
+        px[i] = {(float)std::cos((double)t), (float)-std::sin((double)t)};
+    }
+    for (int i = 0; i < ny; i++) {
+        const float k = ms_freq(i, ny, ax);
+        ky2[i] = k * k;
+        const float t = pscale * (ky2[i] * c->wavlen);
+        py[i] = {(float)std::cos((double)t), (float)-std::sin((double)t)};
+    }
+    // element tables: vzatomLUT knots and Akima coefficients (slicelib.cpp:2831-2866)
+    int zrow[104];
+    for (int i = 0; i < 104; i++) zrow[i] = -1;
+    std::vector<int> elems;
+    c->zi.resize(natom);
+    for (int k = 0; k < natom; k++) {
+        if (zrow[Z[k]] < 0) {
+            zrow[Z[k]] = (int)elems.size();
+            elems.push_back(Z[k]);
+        }
+        c->zi[k] = zrow[Z[k]];
+    }
+    const double RMIN = 0.01, RMAX = 5.0;
+    const double dlnr = std::log(RMAX / RMIN) / (utr::MS_NR - 1);
+    std::vector<double> knots(utr::MS_NR);
+    for (int i = 0; i < utr::MS_NR; i++) {
+        const double rr = RMIN * std::exp(i * dlnr);
+        knots[i] = rr * rr;
+    }
+    g.lnx0 = std::log(knots[0]);
+    g.inv_dln = 1.0 / (2.0 * dlnr);
+    std::vector<double4> coef(std::max<size_t>(1, elems.size() * utr::MS_NR));
+    std::vector<double> yv(utr::MS_NR), bv(utr::MS_NR), cv(utr::MS_NR), dv(utr::MS_NR);
+    for (size_t e = 0; e < elems.size(); e++) {
+        const double* fp = fe_params + (size_t)(elems[e] - 1) * 12;
+        for (int i = 0; i < utr::MS_NR; i++) yv[i] = ms_vzatom(fp, std::sqrt(knots[i]));
+        ms_akima(knots.data(), yv.data(), bv.data(), cv.data(), dv.data(), utr::MS_NR);
+        for (int i = 0; i < utr::MS_NR; i++) coef[e * utr::MS_NR + i] = make_double4(yv[i], bv[i], cv[i], dv[i]);
+    }
+    c->natom = natom;
+    c->x.assign(x, x + natom);
+    c->y.assign(y, y + natom);
+    c->z.assign(z, z + natom);
+    c->occ.assign(occ, occ + natom);
+    auto up = [](auto& buf, const auto& v) -> hipError_t {
+        hipError_t e = buf.alloc(v.size());
+        if (e != hipSuccess) return e;
+        return hipMemcpy(buf.p, v.data(), sizeof(v[0]) * v.size(), hipMemcpyHostToDevice);
+    };
+    hipError_t e = hipSuccess;
+    if ((e = up(c->knots, knots)) != hipSuccess || (e = up(c->coef, coef)) != hipSuccess ||
+        (e = up(c->kx2, kx2)) != hipSuccess || (e = up(c->ky2, ky2)) != hipSuccess ||
+        (e = up(c->px, px)) != hipSuccess || (e = up(c->py, py)) != hipSuccess ||
+        (e = up(c->twx, make_twiddles<float>(nx))) != hipSuccess ||
+        (e = up(c->twy, make_twiddles<float>(ny))) != hipSuccess)
+        return fail(set_err(e == hipErrorOutOfMemory ? UTR_E_NOMEM : UTR_E_DEVICE,
+                            std::string("msct_create: upload failed: ") + hipGetErrorString(e)));
+    // orientations per batch: three complex planes each (wave, transmission,
+    // propagated exit wave); UTR_MSCT_BATCH_BYTES caps the total
+    size_t budget = (size_t)6 << 30;
+    if (const char* s = std::getenv("UTR_MSCT_BATCH_BYTES")) budget = std::max<size_t>(1, std::strtoull(s, nullptr, 10));
+    const size_t per = (size_t)3 * nx * ny * sizeof(cplx<float>);
+    c->batch = (int)std::max<size_t>(1, std::min<size_t>(budget / per, 1 << 16));
+    *out = c;
+    return UTR_OK;
+}
+
+int msct_images(msct_ctx* ctx, int nang, const double* angZ_rad, const double* angY_rad, int ndef,
+                const double* defocus, float* out) {
+    return ms_images(ctx, nang, angZ_rad, angY_rad, ndef, defocus, out, false);
+}
+
+int msct_images_device(msct_ctx* ctx, int nang, const double* angZ_rad, const double* angY_rad, int ndef,
+                       const double* defocus, float* d_out) {
+    return ms_images(ctx, nang, angZ_rad, angY_rad, ndef, defocus, d_out, true);
+}
+
+int msct_slices(const msct_ctx* ctx) { return ctx ? ctx->nsl : -1; }
+
+int msct_destroy(msct_ctx* c) {
+    if (!c) return UTR_OK;
+    c->knots.release(); c->coef.release(); c->kx2.release(); c->ky2.release();
+    c->px.release(); c->py.release(); c->twx.release(); c->twy.release(); c->F.release();
+    c->atoms.release(); c->off.release(); c->W.release(); c->T.release(); c->E.release();
+    delete c;
+    return UTR_OK;
+}
+
+}  // extern "C"

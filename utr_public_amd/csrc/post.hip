@@ -267,7 +267,7 @@ int utr_rotate3d(float* vol, int nx, int ny, int nz, double angZ_deg, double ang
     int rc = check_vol(vol, nx, ny, nz);
     if (rc) return rc;
     const size_t n = (size_t)nx * ny * nz;
-    DevBuf<float> a, b;
+    LocalBuf<float> a, b;
     UTR_HIP_TRY(a.alloc(n));
     UTR_HIP_TRY(b.alloc(n));
     UTR_HIP_TRY(hipMemcpy(a.p, vol, sizeof(float) * n, hipMemcpyHostToDevice));
@@ -288,7 +288,7 @@ int utr_symmetrize3d_device(float* d_vol, int nx, int ny, int nz, int nsym, cons
             return set_err(UTR_E_INVALID, "rotation angles must be finite");
     if (nsym == 1) return UTR_OK;
     const size_t n = (size_t)nx * ny * nz;
-    DevBuf<float> sum, rot;
+    LocalBuf<float> sum, rot;
     UTR_HIP_TRY(sum.alloc(n));
     UTR_HIP_TRY(rot.alloc(n));
     UTR_HIP_TRY(hipMemcpy(sum.p, d_vol, sizeof(float) * n, hipMemcpyDeviceToDevice));
@@ -313,7 +313,7 @@ int utr_symmetrize3d(float* vol, int nx, int ny, int nz, int nsym, const double*
     int rc = check_vol(vol, nx, ny, nz);
     if (rc) return rc;
     const size_t n = (size_t)nx * ny * nz;
-    DevBuf<float> a;
+    LocalBuf<float> a;
     UTR_HIP_TRY(a.alloc(n));
     UTR_HIP_TRY(hipMemcpy(a.p, vol, sizeof(float) * n, hipMemcpyHostToDevice));
     rc = utr_symmetrize3d_device(a.p, nx, ny, nz, nsym, angZ_deg, angY_deg, angZ2_deg);
@@ -335,7 +335,7 @@ int post_fourier(float* d_vol, int nx, int ny, int nz, F&& filter, bool subtract
                        "volume edge not supported by the FFT kernels (supported: 16 32 48 64 96 128 "
                        "160 192 256 320 384 512)");
     const size_t n = (size_t)nx * ny * nz;
-    DevBuf<utr::cplx<float>> scratch, tx, ty, tz;
+    LocalBuf<utr::cplx<float>> scratch, tx, ty, tz;
     UTR_HIP_TRY(scratch.alloc(n));
     const int ns[3] = {nx, ny, nz};
     DevBuf<utr::cplx<float>>* tws[3] = {&tx, &ty, &tz};
@@ -416,7 +416,7 @@ int utr_invlaplace3d(float* vol, int nx, int ny, int nz, double xst, double yst,
     if (nx <= 0 || ny <= 0 || nz <= 0)
         return set_err(UTR_E_INVALID, "utr_invlaplace3d: sizes and steps must be positive");
     const size_t n = (size_t)nx * ny * nz;
-    DevBuf<float> d;
+    LocalBuf<float> d;
     UTR_HIP_TRY(d.alloc(n));
     UTR_HIP_TRY(hipMemcpy(d.p, vol, sizeof(float) * n, hipMemcpyHostToDevice));
     const int rc = utr_invlaplace3d_device(d.p, nx, ny, nz, xst, yst, zst, alpha, norm);
@@ -430,7 +430,7 @@ int utr_highpass3d(float* vol, int nx, int ny, int nz, double xst, double yst, d
     if (nx <= 0 || ny <= 0 || nz <= 0)
         return set_err(UTR_E_INVALID, "utr_highpass3d: sizes and steps must be positive");
     const size_t n = (size_t)nx * ny * nz;
-    DevBuf<float> d;
+    LocalBuf<float> d;
     UTR_HIP_TRY(d.alloc(n));
     UTR_HIP_TRY(hipMemcpy(d.p, vol, sizeof(float) * n, hipMemcpyHostToDevice));
     const int rc = utr_highpass3d_device(d.p, nx, ny, nz, xst, yst, zst, width);

@@ -110,6 +110,16 @@ struct DevBuf {
     }
 };
 
+// DevBuf owned by one scope (function-local scratch): freed on every return
+// path, error returns included.
+template <typename T>
+struct LocalBuf : DevBuf<T> {
+    LocalBuf() = default;
+    LocalBuf(const LocalBuf&) = delete;
+    LocalBuf& operator=(const LocalBuf&) = delete;
+    ~LocalBuf() { this->release(); }
+};
+
 constexpr int kStageRows = 32;  // rows per host->device staging batch
 constexpr int kRowsWgPerCu = 8; // resident workgroups of the persistent row FFT
 
@@ -1002,3 +1012,5 @@ int utr_batch_job_form(const utr_batch* b, int j) {
 #include "dht.hip"
 // post-processing of the reconstructed volume
 #include "post.hip"
+
+#include "msct.hip"
