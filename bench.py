@@ -55,6 +55,8 @@ def main():
                     help="skip the batched 16-job workload (reported under 'batched')")
     ap.add_argument("--no-dht", action="store_true",
                     help="skip the DHT variants (reported under 'dht', not in value)")
+    ap.add_argument("--no-msct", action="store_true",
+                    help="skip the multislice forward model (reported under 'multislice')")
     ap.add_argument("--calib-copy", action="store_true",
                     help="first run a plain 256 MiB device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -258,8 +260,52 @@ def main():
                             "forms": forms}
         log("batched: graph %.2f ms, sequential %.2f ms" % (g_ms, seq * 1e3))
 
+    ms_case = None
+    if not args.no_msct:
+        # Multislice forward model of the same kind of specimen: 3000 C:N:O:S
+        # atoms in a 72 A sphere (the largest that stays inside the 153.6 A
+        # cube under every rotation), 256^2 px at 0.6 A, 1 A slices, 300 keV,
+        # 360 orientations about y over [0, 360), defocus 0 and 2000 A
+        ctb = 256 * 0.6
+        mxyz, mZ = synth.atoms(11, 3000, "sphere", 72.0, 1.4)
+        mat = dict(Z=mZ, x=(mxyz[:, 0] + ctb / 2).astype(np.float32), y=(mxyz[:, 1] + ctb / 2).astype(np.float32),
+                   z=(mxyz[:, 2] + ctb / 2).astype(np.float32), occ=np.ones(len(mZ), np.float32))
+        mp = U.default_msct_params(nx=256, ny=256, ctblength=ctb, deltaz=1.0, kev=300.0)
+        mang = np.arange(360) * 1.0
+        mdef = [0.0, 2000.0]
+        ms = U.Multislice(mp, mat)
+        mout = torch.empty((360, 2, 256, 256), dtype=torch.float32, device="cuda")
+        mts = []
+        for it in range(3):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ms.images(np.zeros(360), mang, mdef, out=mout)
+            torch.cuda.synchronize()
+            mts.append(time.perf_counter() - t0)
+        nsl = ms.slices
+        ms.close()
+        tm = float(np.mean(mts[1:]))
+        extra["multislice"] = {
+            "workload": "3000 atoms C:N:O:S 62:17:19:2 in a 72 A sphere, 256^2 px over a 153.6 A cube, "
+                        "1 A slices, 300 keV, 360 orientations about y, defocus 0 and 2000 A, intensities",
+            "slices": nsl, "ms": round(tm * 1e3, 2), "images_per_s": 720 / tm,
+            "slice_pixels_per_s": 360 * nsl * 256 * 256 / tm}
+        log("multislice: %.1f ms for 360 orientations x 2 defocus (%d slices)" % (tm * 1e3, nsl))
+        ms_case = (mp, mat, mdef)
+
     cpu = None
     if not args.no_cpu_baseline:
+        if ms_case is not None:
+            from oracle import msct as M
+            mp, mat, mdef = ms_case
+            op = dict(nx=256, ny=256, ctb=mp["ctblength"], deltaz=1.0, kev=300.0)
+            ts = time.perf_counter()
+            M.images(op, mat, np.zeros(4), np.radians([0.0, 90.0, 180.0, 270.0]), mdef)
+            tc = time.perf_counter() - ts
+            extra["multislice"]["cpu_baseline"] = {
+                "images_per_s": 8 / tc, "cores": 1, "kind": "port",
+                "sample": "4 of the 360 orientations, both defocus distances, FP64 numpy restatement",
+                "seconds": round(tc, 2)}
         import oracle
         ncores = min(16, os.cpu_count() or 1)
         ts = time.perf_counter()

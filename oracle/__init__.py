@@ -57,9 +57,15 @@ class OraGeom(ctypes.Structure):
     ]
 
 
+REF_SLICELIB = "/root/reference/TemsimLibrary/slicelib.cpp"
+
+
 def build():
-    """Compile liboracle_utr.so with the committed Makefile."""
+    """Compile liboracle_utr.so with the committed Makefile and, where the
+    reference tree is present, the reference's own slicelib into _ref/."""
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    if os.path.exists(REF_SLICELIB):
+        subprocess.run(["make", "-s", "-C", _HERE, "ref"], check=True)
 
 
 _lib = None

@@ -145,3 +145,15 @@ def test_errors():
         U.Multislice(p, dict(corner, Z=np.array([0])))
     with pytest.raises(U.UTRError, match="window"):
         U.Multislice(dict(p, ctblength=4.0), corner)
+
+
+def test_multislice_then_utr_recovers_atom_positions():
+    """The GPU forward model feeding the GPU reconstruction: the geometric
+    round trip of tests/test_roundtrip_oracle.py on the product path."""
+    from test_roundtrip_oracle import A, N, XST, atom_set, check_positions, utr_setup
+    pts, at = atom_set()
+    p = U.default_msct_params(nx=N, ny=N, ctblength=N * XST, deltaz=1.0, kev=200.0)
+    imgs = U.multislice_images(p, at, np.zeros(A), np.arange(A) * 5.0, [300.0])[:, 0]
+    up, rows = utr_setup()
+    vol = U.reconstruct(up, rows, np.ascontiguousarray(imgs), precision=U.UTR_FP32)
+    check_positions(np.asarray(vol, np.float64), pts)

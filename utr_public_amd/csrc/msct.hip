@@ -29,6 +29,11 @@
 namespace utr {
 
 constexpr int MS_NR = 100;  // radii of the potential look-up table (slicelib.cpp:2818-2820)
+// LDS budget (bytes of one buffer) of the column and product kernels.
+// 8192 (4 columns of 256 per workgroup) measured faster than 16384 (8
+// columns, fewer resident workgroups): 96 against 124 us per column pass
+// over 360 images of 256^2.
+constexpr int MS_LDS = 8192;
 
 struct alignas(16) MsAtom {
     float x, y, occ;
@@ -72,7 +77,9 @@ __device__ __forceinline__ double ms_lut(const double* __restrict__ knots, const
 // pixel takes an atom when its periodic image inside the atom's window
 // [ixo - idx, ixo + idx] x [iyo - idy, iyo + idy] lies within 3 A; windows
 // are narrower than the image (checked on the host), so there is at most
-// one such image.
+// one such image.  Each chunk of the slice's atoms is first culled against
+// the tile and compacted in atom order (wave ballots, no atomics), so every
+// pixel still sums its atoms in the order of the z-sorted list.
 __global__ void __launch_bounds__(256) k_ms_potential(const MsAtom* __restrict__ atoms,
                                                       const int* __restrict__ off, int natom, int s,
                                                       MsGeom g, const double* __restrict__ knots,
@@ -82,34 +89,52 @@ __global__ void __launch_bounds__(256) k_ms_potential(const MsAtom* __restrict__
     const int a0 = off[o * g.nsl1 + s], a1 = off[o * g.nsl1 + s + 1];
     if (a0 == a1) return;  // empty slice: the wave is not transmitted
     const int tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
-    const int px = tx * 16 + (threadIdx.x & 15), py = ty * 16 + (threadIdx.x >> 4);
-    const bool on = px < g.nx && py < g.ny;
+    const int X0 = tx * 16, Y0 = ty * 16;
+    const int px = X0 + (threadIdx.x & 15), py = Y0 + (threadIdx.x >> 4);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     __shared__ MsAtom sa[256];
+    __shared__ int wcount[4];
     const MsAtom* A = atoms + (size_t)o * natom;
     float acc = 0.f;
     for (int c0 = a0; c0 < a1; c0 += 256) {
         const int n = min(256, a1 - c0);
+        bool hit = false;
+        MsAtom at{};
+        if ((int)threadIdx.x < n) {
+            at = A[c0 + threadIdx.x];
+            // offset of the tile's first pixel from the window start, mod n:
+            // the 16-pixel tile meets some image of the window when it
+            // starts inside one or the next one starts within the tile
+            const int dx = ms_pmod(X0 - (at.ixo - g.idx), g.nx);
+            const int dy = ms_pmod(Y0 - (at.iyo - g.idy), g.ny);
+            hit = (dx <= 2 * g.idx || g.nx - dx <= 15) && (dy <= 2 * g.idy || g.ny - dy <= 15);
+        }
+        const unsigned long long m = __ballot(hit);
+        const int pre = __popcll(m & ((1ull << lane) - 1ull));
+        __syncthreads();  // the previous chunk's list is no longer read
+        if (lane == 0) wcount[wv] = __popcll(m);
         __syncthreads();
-        if ((int)threadIdx.x < n) sa[threadIdx.x] = A[c0 + threadIdx.x];
+        int base = 0;
+        for (int k = 0; k < wv; k++) base += wcount[k];
+        const int total = wcount[0] + wcount[1] + wcount[2] + wcount[3];
+        if (hit) sa[base + pre] = at;
         __syncthreads();
-        if (!on) continue;
-        for (int k = 0; k < n; k++) {
-            const MsAtom at = sa[k];
-            const int x0 = at.ixo - g.idx;
+        for (int k = 0; k < total; k++) {
+            const MsAtom t = sa[k];
+            const int x0 = t.ixo - g.idx;
             const int ix = x0 + ms_pmod(px - x0, g.nx);
-            if (ix > at.ixo + g.idx) continue;
-            const int y0 = at.iyo - g.idy;
+            if (ix > t.ixo + g.idx) continue;
+            const int y0 = t.iyo - g.idy;
             const int iy = y0 + ms_pmod(py - y0, g.ny);
-            if (iy > at.iyo + g.idy) continue;
-            const double rx = (double)at.x - (double)ix * g.sx;
-            const double ry = (double)at.y - (double)iy * g.sy;
+            if (iy > t.iyo + g.idy) continue;
+            const double rx = (double)t.x - (double)ix * g.sx;
+            const double ry = (double)t.y - (double)iy * g.sy;
             double r2 = rx * rx + ry * ry;
             if (r2 > 9.0) continue;
             if (r2 < g.rminsq) r2 = g.rminsq;
-            acc += (float)((double)at.occ * ms_lut(knots, coef + (size_t)at.zi * MS_NR, r2, g));
+            acc += (float)((double)t.occ * ms_lut(knots, coef + (size_t)t.zi * MS_NR, r2, g));
         }
     }
-    if (!on) return;
     float sn, cs;
     sincosf((float)(scale * (double)acc), &sn, &cs);
     trans[((size_t)o * g.ny + py) * g.nx + px] = {cs, sn};
@@ -119,17 +144,16 @@ __global__ void __launch_bounds__(256) k_ms_potential(const MsAtom* __restrict__
 // orientation): y FFT (exponent +1), multiply, y inverse FFT (exponent -1),
 // store into dst.  MODE 0: bandwidth limit, factor = scale where
 // kx2 + ky2 < k2max (float, as autoslic.cpp:1000-1003), else 0.  MODE 1: the
-// same limit times the propagator py * px (slicelib.cpp:827-835).  MODE 2:
-// the Fresnel table F[ky][kx] times scale.
+// table F[ky][kx] of the same limit times the propagator py * px
+// (slicelib.cpp:827-835), times scale.  MODE 2: the Fresnel table F[ky][kx]
+// times scale.
 template <int N, int MODE>
 __global__ void __launch_bounds__(256) k_ms_cols(const cplx<float>* __restrict__ src, cplx<float>* __restrict__ dst,
                                                  int nx, int gx, int gy, const cplx<float>* __restrict__ tw,
                                                  const float* __restrict__ kx2, const float* __restrict__ ky2,
-                                                 float k2max, const cplx<float>* __restrict__ px,
-                                                 const cplx<float>* __restrict__ py,
-                                                 const cplx<float>* __restrict__ F, float scale,
+                                                 float k2max, const cplx<float>* __restrict__ F, float scale,
                                                  const int* __restrict__ off, int nsl1, int s) {
-    constexpr int NB = lds_batch(N, sizeof(cplx<float>), LDS_POST);
+    constexpr int NB = lds_batch(N, sizeof(cplx<float>), MS_LDS);
     __shared__ cplx<float> sm[2 * lds_len(NB * N)];
     cplx<float>* a = sm;
     cplx<float>* b = sm + lds_len(NB * N);
@@ -149,11 +173,10 @@ __global__ void __launch_bounds__(256) k_ms_cols(const cplx<float>* __restrict__
         const int c = t / N, y = t - c * N;
         const int kx = c0 + c;
         cplx<float> v = a[lx(t)];
-        if (MODE == 2) {
+        if (MODE != 0) {
             v = cmul(v, F[(size_t)y * nx + kx]);
             v = {v.x * scale, v.y * scale};
         } else if (kx2[kx] + ky2[y] < k2max) {
-            if (MODE == 1) v = cmul(cmul(v, py[y]), px[kx]);
             v = {v.x * scale, v.y * scale};
         } else {
             v = {0.f, 0.f};
@@ -177,7 +200,7 @@ template <int N, bool PROJ>
 __global__ void __launch_bounds__(256) k_ms_mul(cplx<float>* __restrict__ W, const cplx<float>* __restrict__ T,
                                                 int ny, int nlines, const cplx<float>* __restrict__ tw,
                                                 const int* __restrict__ off, int nsl1, int s) {
-    constexpr int NB = lds_batch(N, sizeof(cplx<float>), LDS_POST) / 2;
+    constexpr int NB = lds_batch(N, sizeof(cplx<float>), MS_LDS) / 2;
     static_assert(NB >= 1, "two sequences per line must fit");
     __shared__ cplx<float> sm[2 * lds_len(2 * NB * N)];
     cplx<float>* a = sm;
@@ -364,7 +387,7 @@ struct msct_ctx {
     DevBuf<double> knots;
     DevBuf<double4> coef;
     DevBuf<float> kx2, ky2;
-    DevBuf<cplx<float>> px, py, twx, twy, F;
+    DevBuf<cplx<float>> prop, twx, twy, F;  // prop: limit * propagator per (ky, kx)
     DevBuf<utr::MsAtom> atoms;
     DevBuf<int> off;
     DevBuf<cplx<float>> W, T, E;
@@ -423,20 +446,20 @@ int ms_run_cols(int mode, msct_ctx* c, const cplx<float>* src, cplx<float>* dst,
     bool ok = false;
 #define MS_COLS(N)                                                                                  \
     {                                                                                               \
-        constexpr int NB = utr::lds_batch(N, sizeof(cplx<float>), utr::LDS_POST);                   \
+        constexpr int NB = utr::lds_batch(N, sizeof(cplx<float>), utr::MS_LDS);                    \
         const int gx = (nx + NB - 1) / NB;                                                          \
         const unsigned grid = utr::xcd_grid((unsigned)(gx * B));                                    \
         if (mode == 0)                                                                              \
             utr::k_ms_cols<N, 0><<<grid, 256, 0, st>>>(src, dst, nx, gx, B, c->twy.p, c->kx2.p,     \
-                                                       c->ky2.p, c->k2max, c->px.p, c->py.p,        \
+                                                       c->ky2.p, c->k2max,                          \
                                                        F, scale, c->off.p, c->g.nsl1, s);      \
         else if (mode == 1)                                                                         \
             utr::k_ms_cols<N, 1><<<grid, 256, 0, st>>>(src, dst, nx, gx, B, c->twy.p, c->kx2.p,     \
-                                                       c->ky2.p, c->k2max, c->px.p, c->py.p,        \
+                                                       c->ky2.p, c->k2max,                          \
                                                        F, scale, c->off.p, c->g.nsl1, s);      \
         else                                                                                        \
             utr::k_ms_cols<N, 2><<<grid, 256, 0, st>>>(src, dst, nx, gx, B, c->twy.p, c->kx2.p,     \
-                                                       c->ky2.p, c->k2max, c->px.p, c->py.p,        \
+                                                       c->ky2.p, c->k2max,                          \
                                                        F, scale, c->off.p, c->g.nsl1, s);      \
         ok = true;                                                                                  \
     }
@@ -454,7 +477,7 @@ int ms_run_lines(msct_ctx* c, int what, int B, int s, int ndef, int d, float* ou
 #define MS_LINES(N)                                                                                 \
     {                                                                                               \
         constexpr int NB = utr::lds_batch(N, sizeof(cplx<float>), utr::LDS_POST);                   \
-        constexpr int NB2 = NB / 2;                                                                 \
+        constexpr int NB2 = utr::lds_batch(N, sizeof(cplx<float>), utr::MS_LDS) / 2;               \
         if (what == 0) {                                                                            \
             utr::k_post_lines<float, N><<<(nlines + NB - 1) / NB, 256, 0, st>>>(c->T.p, nlines, +1, \
                                                                                c->twx.p);           \
@@ -481,63 +504,87 @@ int ms_run_lines(msct_ctx* c, int what, int B, int s, int ndef, int d, float* ou
     return UTR_OK;
 }
 
-// Rotates, checks, z-sorts and slices the atoms of orientations [a0, a0+B)
-// into A (B x natom) and off (B x (nsl + 1)); autosliccmd.cpp:709-748 in
-// float, then the bounds test of autosliccmd.cpp:857-860 and the depth sort
-// and slice membership of autoslic.cpp:720-744.
-int ms_prepare(msct_ctx* c, int a0, int B, const double* angZ, const double* angY,
+// Rotates, checks, z-sorts and slices the atoms of orientation a into
+// dst (natom) and o (nsl + 1): autosliccmd.cpp:709-748 in float, then the
+// bounds test of autosliccmd.cpp:857-860 and the depth sort and slice
+// membership of autoslic.cpp:720-744.  Returns false when an atom leaves
+// the cube.
+bool ms_prepare_one(const msct_ctx* c, double aZ, double aY, utr::MsAtom* dst, int* o, std::vector<float>& X,
+                    std::vector<float>& Y, std::vector<float>& Z, std::vector<int>& idx) {
+    const int n = c->natom;
+    X.resize(n);
+    Y.resize(n);
+    Z.resize(n);
+    idx.resize(n);
+    const float ctb = c->ctbf;
+    const float xc = (float)(c->p.ctblength / 2.0 + c->p.dxc), yc = (float)(c->p.ctblength / 2.0 + c->p.dyc),
+                zc = (float)(c->p.ctblength / 2.0 + c->p.dzc);
+    const float sinZ = (float)std::sin(aZ), cosZ = (float)std::cos(aZ), sinY = (float)std::sin(aY),
+                cosY = (float)std::cos(aY);
+    for (int k = 0; k < n; k++) {
+        const float xx = xc + (c->x[k] - xc) * cosZ + (-c->y[k] + yc) * sinZ;
+        const float yy = yc + (c->x[k] - xc) * sinZ + (c->y[k] - yc) * cosZ;
+        const float x1 = xc + (xx - xc) * cosY + (c->z[k] - zc) * sinY;
+        const float z1 = zc + (-xx + xc) * sinY + (c->z[k] - zc) * cosY;
+        if (!(x1 >= 0.f && yy >= 0.f && z1 >= 0.f && x1 <= ctb && yy <= ctb && z1 <= ctb)) return false;
+        X[k] = x1;
+        Y[k] = yy;
+        Z[k] = z1;
+        idx[k] = k;
+    }
+    std::stable_sort(idx.begin(), idx.end(), [&](int i, int j) { return Z[i] < Z[j]; });
+    for (int k = 0; k < n; k++) {
+        const int i = idx[k];
+        utr::MsAtom& m = dst[k];
+        m.x = X[i];
+        m.y = Y[i];
+        m.occ = c->occ[i];
+        m.zi = c->zi[i];
+        m.ixo = (int)((double)X[i] / c->g.sx);
+        m.iyo = (int)((double)Y[i] / c->g.sy);
+        m.pad0 = m.pad1 = 0;
+    }
+    const double dz = (double)(float)c->p.deltaz;
+    int istart = 0;
+    double zslice = dz;
+    for (int s = 0; s < c->nsl; s++) {
+        o[s] = istart;
+        while (istart < n && (double)Z[idx[istart]] < zslice) istart++;
+        zslice += dz;
+    }
+    o[c->nsl] = istart;
+    return true;
+}
+
+// orientations [a0, a0 + B) into A (B x natom) and off (B x (nsl + 1)), on
+// up to 16 host threads
+int ms_prepare(const msct_ctx* c, int a0, int B, const double* angZ, const double* angY,
                std::vector<utr::MsAtom>& A, std::vector<int>& off) {
     const int n = c->natom, nsl1 = c->nsl + 1;
     A.resize((size_t)B * n);
     off.assign((size_t)B * nsl1, 0);
-    const float ctb = c->ctbf;
-    const float xc = (float)(c->p.ctblength / 2.0 + c->p.dxc), yc = (float)(c->p.ctblength / 2.0 + c->p.dyc),
-                zc = (float)(c->p.ctblength / 2.0 + c->p.dzc);
-    std::vector<float> X(n), Y(n), Z(n);
-    std::vector<int> idx(n);
-    const double dz = (double)(float)c->p.deltaz;
-    for (int b = 0; b < B; b++) {
-        const double aZ = angZ[a0 + b], aY = angY[a0 + b];
-        const float sinZ = (float)std::sin(aZ), cosZ = (float)std::cos(aZ), sinY = (float)std::sin(aY),
-                    cosY = (float)std::cos(aY);
-        for (int k = 0; k < n; k++) {
-            float xx = xc + (c->x[k] - xc) * cosZ + (-c->y[k] + yc) * sinZ;
-            float yy = yc + (c->x[k] - xc) * sinZ + (c->y[k] - yc) * cosZ;
-            float zz = c->z[k];
-            const float x1 = xc + (xx - xc) * cosY + (zz - zc) * sinY;
-            zz = zc + (-xx + xc) * sinY + (zz - zc) * cosY;
-            X[k] = x1;
-            Y[k] = yy;
-            Z[k] = zz;
-            if (!(x1 >= 0.f && yy >= 0.f && zz >= 0.f && x1 <= ctb && yy <= ctb && zz <= ctb))
-                return set_err(UTR_E_INVALID,
-                               "msct: an atom leaves the cube [0, ctblength]^3 at orientation " +
-                                   std::to_string(a0 + b) + " (the reference stops with the same error)");
-        }
-        for (int k = 0; k < n; k++) idx[k] = k;
-        std::stable_sort(idx.begin(), idx.end(), [&](int i, int j) { return Z[i] < Z[j]; });
-        utr::MsAtom* dst = A.data() + (size_t)b * n;
-        for (int k = 0; k < n; k++) {
-            const int i = idx[k];
-            utr::MsAtom& m = dst[k];
-            m.x = X[i];
-            m.y = Y[i];
-            m.occ = c->occ[i];
-            m.zi = c->zi[i];
-            m.ixo = (int)((double)X[i] / c->g.sx);
-            m.iyo = (int)((double)Y[i] / c->g.sy);
-            m.pad0 = m.pad1 = 0;
-        }
-        int* o = off.data() + (size_t)b * nsl1;
-        int istart = 0;
-        double zslice = dz;
-        for (int s = 0; s < c->nsl; s++) {
-            o[s] = istart;
-            while (istart < n && (double)Z[idx[istart]] < zslice) istart++;
-            zslice += dz;
-        }
-        o[c->nsl] = istart;
-    }
+    const int nt = std::max(1, std::min<int>({B, 16, (int)std::max(1u, std::thread::hardware_concurrency())}));
+    std::vector<int> bad(nt, -1);
+    auto work = [&](int t) {
+        std::vector<float> X, Y, Z;
+        std::vector<int> idx;
+        for (int b = t; b < B; b += nt)
+            if (!ms_prepare_one(c, angZ[a0 + b], angY[a0 + b], A.data() + (size_t)b * n, off.data() + (size_t)b * nsl1,
+                                X, Y, Z, idx)) {
+                bad[t] = b;
+                return;
+            }
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; t++) th.emplace_back(work, t);
+    work(0);
+    for (auto& t : th) t.join();
+    int first = -1;
+    for (int t = 0; t < nt; t++)
+        if (bad[t] >= 0 && (first < 0 || bad[t] < first)) first = bad[t];
+    if (first >= 0)
+        return set_err(UTR_E_INVALID, "msct: an atom leaves the cube [0, ctblength]^3 at orientation " +
+                                          std::to_string(a0 + first) + " (the reference stops with the same error)");
     return UTR_OK;
 }
 
@@ -606,7 +653,7 @@ int ms_images(msct_ctx* c, int nang, const double* angZ, const double* angY, int
                 } else if (!proj) {
                     if ((rc = ms_run_lines(c, 1, nb, s, 0, 0, nullptr, st))) return rc;
                 }
-                if (!proj && (rc = ms_run_cols(1, c, c->W.p, c->W.p, nb, inv, s, nullptr, st))) return rc;
+                if (!proj && (rc = ms_run_cols(1, c, c->W.p, c->W.p, nb, inv, s, c->prop.p, st))) return rc;
             }
             // exit wave -> spectrum along x: already so (multislice, normalised)
             // or one x FFT of the real-space wave (projection, unnormalised)
@@ -708,6 +755,15 @@ int msct_create(const msct_params* p, const double* fe_params, int natom, const 
         const float t = pscale * (ky2[i] * c->wavlen);
         py[i] = {(float)std::cos((double)t), (float)-std::sin((double)t)};
     }
+    // propagate() (slicelib.cpp:822-845): w * py, then * px, inside the limit
+    std::vector<cplx<float>> prop((size_t)nx * ny);
+    for (int j = 0; j < ny; j++)
+        for (int i = 0; i < nx; i++) {
+            cplx<float> f{0.f, 0.f};
+            if (kx2[i] < c->k2max && kx2[i] + ky2[j] < c->k2max)
+                f = {py[j].x * px[i].x - py[j].y * px[i].y, py[j].x * px[i].y + py[j].y * px[i].x};
+            prop[(size_t)j * nx + i] = f;
+        }
     // element tables: vzatomLUT knots and Akima coefficients (slicelib.cpp:2831-2866)
     int zrow[104];
     for (int i = 0; i < 104; i++) zrow[i] = -1;
@@ -750,7 +806,7 @@ int msct_create(const msct_params* p, const double* fe_params, int natom, const 
     hipError_t e = hipSuccess;
     if ((e = up(c->knots, knots)) != hipSuccess || (e = up(c->coef, coef)) != hipSuccess ||
         (e = up(c->kx2, kx2)) != hipSuccess || (e = up(c->ky2, ky2)) != hipSuccess ||
-        (e = up(c->px, px)) != hipSuccess || (e = up(c->py, py)) != hipSuccess ||
+        (e = up(c->prop, prop)) != hipSuccess ||
         (e = up(c->twx, make_twiddles<float>(nx))) != hipSuccess ||
         (e = up(c->twy, make_twiddles<float>(ny))) != hipSuccess)
         return fail(set_err(e == hipErrorOutOfMemory ? UTR_E_NOMEM : UTR_E_DEVICE,
@@ -780,7 +836,7 @@ int msct_slices(const msct_ctx* ctx) { return ctx ? ctx->nsl : -1; }
 int msct_destroy(msct_ctx* c) {
     if (!c) return UTR_OK;
     c->knots.release(); c->coef.release(); c->kx2.release(); c->ky2.release();
-    c->px.release(); c->py.release(); c->twx.release(); c->twy.release(); c->F.release();
+    c->prop.release(); c->twx.release(); c->twy.release(); c->F.release();
     c->atoms.release(); c->off.release(); c->W.release(); c->T.release(); c->E.release();
     delete c;
     return UTR_OK;

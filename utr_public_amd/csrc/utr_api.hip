@@ -791,6 +791,7 @@ int utr_destroy(utr_ctx* c) {
 
 #include <map>
 #include <mutex>
+#include <thread>
 #include <tuple>
 
 struct utr_batch {
