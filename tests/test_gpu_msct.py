@@ -157,3 +157,19 @@ def test_multislice_then_utr_recovers_atom_positions():
     up, rows = utr_setup()
     vol = U.reconstruct(up, rows, np.ascontiguousarray(imgs), precision=U.UTR_FP32)
     check_positions(np.asarray(vol, np.float64), pts)
+
+
+@pytest.mark.parametrize("n,ctb", [(64, 51.2), (96, 57.6)])
+def test_fused_rows_and_tiles_agree(n, ctb):
+    """The potential summed per row group with its x FFT in LDS (default)
+    and per 16 x 16 tile followed by a separate x FFT (UTR_MSCT_FUSED=0)
+    give the same images bit for bit."""
+    p = U.default_msct_params(nx=n, ny=n, ctblength=ctb, deltaz=1.0, output=1)
+    at = atoms_in_cube(8, 120, ctb, 20.0)
+    a = U.multislice_images(p, at, [0.0, 70.0, 190.0], [20.0, 95.0, 300.0], [0.0, 250.0])
+    os.environ["UTR_MSCT_FUSED"] = "0"
+    try:
+        b = U.multislice_images(p, at, [0.0, 70.0, 190.0], [20.0, 95.0, 300.0], [0.0, 250.0])
+    finally:
+        del os.environ["UTR_MSCT_FUSED"]
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))

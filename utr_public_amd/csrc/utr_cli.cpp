@@ -43,10 +43,6 @@ void check(int rc) {
     if (rc != UTR_OK) throw std::runtime_error(utr_last_error());
 }
 
-[[noreturn]] void unsupported(const std::string& what) {
-    throw std::runtime_error("Error: " + what + " is not supported by this build.");
-}
-
 bool ask_yes(const char* question) {
     if (!isatty(0)) {
         printf("%s (stdin is not a terminal: no)\n", question);
