@@ -340,7 +340,9 @@ typedef struct msct_params {
     int nx, ny;             /* pixels (autosliccmd line 12 "nx ny")                 */
     int mode;               /* 0 multislice, 1 projection, 2 first Born (the slice
                                loop of both skips propagation, autoslic.cpp:772)   */
-    int output;             /* 0 intensity |psi|^2, 1 complex psi (re, im pairs)   */
+    int output;             /* autosliccmd noutput: 0 intensity |psi|^2, 1 phase,
+                               2 complex psi (re, im pairs), 3 the 3D potential on
+                               the plane z = ctblength + defocus (no multislice)    */
     double ctblength;       /* cube side in Angstrom (the largest cell edge of the
                                XYZ file, autosliccmd.cpp:629-633)                   */
     double deltaz;          /* slice thickness, Angstrom (line 14)                 */
@@ -365,7 +367,7 @@ int msct_create(const msct_params* p, const double* fe_params, int natom, const 
 /* Images of nang orientations (angZ, angY in radians, the autosliccmd line
  * 25 pair: rotation about Z, then about Y', autosliccmd.cpp:733-748) at ndef
  * defocus distances (Angstrom, autosliccmd.cpp:1065): out holds
- * nang*ndef*ny*nx floats (x fastest), twice that for complex output,
+ * nang*ndef*ny*nx floats (x fastest), twice that for output 2,
  * orientation-major.  An atom leaving the cube after rotation is an error,
  * as in autosliccmd.cpp:857-860.  Host / device output. */
 int msct_images(msct_ctx* ctx, int nang, const double* angZ_rad, const double* angY_rad, int ndef,

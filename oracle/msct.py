@@ -335,6 +335,39 @@ class Model:
         return np.fft.ifft2(F)
 
 
+def vatom(Z, r):
+    """3D potential (V) of element Z at radius r (slicelib.cpp:2699-2737):
+    al/r sum a_i exp(-2 pi r sqrt(b_i)) + ag sum c_i exp(-(pi r)^2/d_i)/d_i^1.5
+    with al = 2 pi^2 a0 e, ag = 2 pi^(5/2) a0 e and the 10-digit pi."""
+    if Z < 1 or Z > 103:
+        return 0.0
+    fp = fe_params()[Z]
+    al, ag, pi = 150.4121417, 266.5985798, 3.141592654
+    r = np.maximum(np.abs(np.asarray(r, np.float64)), 1.0e-10)
+    x = 2.0 * pi * r
+    suml = sum(fp[i] * np.exp(-x * math.sqrt(fp[i + 1])) for i in (0, 2, 4))
+    x = (pi * r) ** 2
+    sumg = sum(fp[i] * np.exp(-x / fp[i + 1]) / math.sqrt(fp[i + 1]) ** 3 for i in (6, 8, 10))
+    return al * suml / r + ag * sumg
+
+
+def potential_plane(p, x, y, z, Zn, zplane):
+    """The noutput 3 image of autosliccmd.cpp:1004-1044, 1146-1151: the sum
+    over all atoms (file order) of vatom at the pixel (xst ix, yst iy) of the
+    plane z = zplane, with float squared distances; xst = ctblength / nx."""
+    nx, ny = p["nx"], p["ny"]
+    ax = f32(p["ctb"])
+    xs = (f32(0) + (ax / f32(nx)) * np.arange(nx, dtype=f32)).astype(f32)
+    ys = (f32(0) + (ax / f32(ny)) * np.arange(ny, dtype=f32)).astype(f32)
+    out = np.zeros((ny, nx))
+    for k in range(len(x)):
+        dx2 = ((f32(x[k]) - xs) ** 2).astype(f32).astype(np.float64)
+        dy2 = ((f32(y[k]) - ys) ** 2).astype(f32).astype(np.float64)
+        dz2 = float(f32(f32(z[k]) - f32(zplane)) ** 2)
+        out += vatom(int(Zn[k]), np.sqrt(dx2[None, :] + dy2[:, None] + dz2))
+    return out
+
+
 def images(p, atoms, angZ, angY, defocus, output="intensity"):
     """Images [orientation][defocus][ny][nx] of an atom set (autosliccmd.cpp:
     709-748 rotation, autoslic::calculate, 1060-1087 propagation, 1196-1200
@@ -348,10 +381,16 @@ def images(p, atoms, angZ, angY, defocus, output="intensity"):
     for a in range(len(angZ)):
         x, y, z = rotate_atoms(atoms["x"], atoms["y"], atoms["z"], angZ[a], angY[a], p["ctb"],
                                p.get("dxc", 0.0), p.get("dyc", 0.0), p.get("dzc", 0.0))
+        if output == "potential":
+            if len(z) and (min(x.min(), y.min(), z.min()) < 0 or max(x.max(), y.max(), z.max()) > f32(p["ctb"])):
+                raise ValueError("atoms outside the cube")
+            for j, d in enumerate(defocus):
+                out[a, j] = potential_plane(p, x, y, z, atoms["Z"], f32(p["ctb"]) + f32(d))
+            continue
         w = m.exit_wave(x, y, z, atoms["occ"], atoms["Z"])
         for j, d in enumerate(defocus):
             R = d - 0.5 * p["ctb"]
             c = m.fresnel(w, R, aobj, p.get("cs3_mm", 0.0), p.get("cs5_mm", 0.0), p.get("astig_dz", 0.0),
                           math.radians(p.get("astig_angle", 0.0)))
-            out[a, j] = c if output == "complex" else np.abs(c) ** 2
+            out[a, j] = c if output == "complex" else (np.angle(c) if output == "phase" else np.abs(c) ** 2)
     return out

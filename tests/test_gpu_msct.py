@@ -46,7 +46,7 @@ def oracle_params(p):
 def check(p, at, angZ, angY, defocus):
     print("msct case:", {k: p[k] for k in ("nx", "ny", "mode", "ctblength", "deltaz", "kev", "aobj_mrad")},
           len(at["Z"]), "atoms", len(angZ), "orientations", flush=True)
-    pc = dict(p, output=1)
+    pc = dict(p, output=2)
     psi = U.multislice_images(pc, at, angZ, angY, defocus)
     ref = M.images(oracle_params(p), at, np.radians(angZ), np.radians(angY), defocus, output="complex")
     e = rel(psi, ref)
@@ -109,7 +109,7 @@ def test_one_slice_multislice_is_propagated_projection():
     at = dict(Z=np.array([6, 8, 16, 7] * 3), x=rng.uniform(10, 40, n).astype(np.float32),
               y=rng.uniform(10, 40, n).astype(np.float32), z=rng.uniform(20.2, 20.8, n).astype(np.float32),
               occ=np.ones(n, np.float32))
-    p = U.default_msct_params(nx=64, ny=64, ctblength=51.2, deltaz=1.0, output=1)
+    p = U.default_msct_params(nx=64, ny=64, ctblength=51.2, deltaz=1.0, output=2)
     m = U.Multislice(p, at)
     assert m.slices == 52
     a = m.images([0.0], [0.0], [100.0])
@@ -164,7 +164,7 @@ def test_fused_rows_and_tiles_agree(n, ctb):
     """The potential summed per row group with its x FFT in LDS (default)
     and per 16 x 16 tile followed by a separate x FFT (UTR_MSCT_FUSED=0)
     give the same images bit for bit."""
-    p = U.default_msct_params(nx=n, ny=n, ctblength=ctb, deltaz=1.0, output=1)
+    p = U.default_msct_params(nx=n, ny=n, ctblength=ctb, deltaz=1.0, output=2)
     at = atoms_in_cube(8, 120, ctb, 20.0)
     a = U.multislice_images(p, at, [0.0, 70.0, 190.0], [20.0, 95.0, 300.0], [0.0, 250.0])
     os.environ["UTR_MSCT_FUSED"] = "0"
@@ -173,3 +173,23 @@ def test_fused_rows_and_tiles_agree(n, ctb):
     finally:
         del os.environ["UTR_MSCT_FUSED"]
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_phase_and_potential_outputs():
+    """autosliccmd's noutput 1 (phase) and 3 (the 3D potential of all atoms
+    on the plane z = ctblength + defocus, no multislice)."""
+    p = U.default_msct_params(nx=64, ny=64, ctblength=51.2, deltaz=1.0, kev=200.0)
+    at = atoms_in_cube(9, 60, 51.2, 18.0)
+    angZ, angY = [0.0, 35.0], [10.0, 120.0]
+    ph = U.multislice_images(dict(p, output=1), at, angZ, angY, [0.0, 150.0])
+    ref = M.images(oracle_params(p), at, np.radians(angZ), np.radians(angY), [0.0, 150.0], output="phase")
+    print("phase max abs diff %.2e" % np.abs(ph - ref).max(), flush=True)
+    assert np.abs(ph - ref).max() < 1e-5
+    # planes through the specimen: defocus -25.6 is the cube centre
+    zs = [-40.0, -25.6, -12.0]
+    pot = U.multislice_images(dict(p, output=3), at, angZ, angY, zs)
+    ref = M.images(oracle_params(p), at, np.radians(angZ), np.radians(angY), zs, output="potential")
+    e = rel(pot, ref)
+    print("potential rel-L2 %.2e" % e, flush=True)
+    assert e < 1e-5
+

@@ -38,7 +38,7 @@ def ref():
     for n in ("_Z6bessk0d", "_Z6bessi0d", "_Z10wavelengthd", "_Z5sigmad"):
         getattr(L, n).restype = d
         getattr(L, n).argtypes = [d]
-    for n in ("_Z6vzatomid", "_Z9vzatomLUTid"):
+    for n in ("_Z6vzatomid", "_Z9vzatomLUTid", "_Z5vatomid"):
         getattr(L, n).restype = d
         getattr(L, n).argtypes = [i, d]
     dp = ctypes.POINTER(d)
@@ -77,6 +77,24 @@ def test_potential_and_lut_match_reference(ref):
         for r2 in (1e-6, 1e-4, 0.0025, 0.3, 1.0, 4.0, 8.99, 9.0, 24.0, 30.0):
             a, b = ref._Z9vzatomLUTid(Z, r2), float(lut(Z, r2))
             assert abs(a - b) <= 1e-13 * abs(a), (Z, r2)
+
+
+def test_3d_potential_matches_reference(ref):
+    for Z in (1, 6, 8, 16, 79):
+        for r in (1e-3, 0.05, 0.3, 1.0, 2.9, 7.5):
+            a, b = ref._Z5vatomid(Z, r), float(M.vatom(Z, r))
+            assert abs(a - b) <= 1e-14 * abs(a), (Z, r)
+
+
+@pytest.mark.parametrize("Z", [6, 16])
+def test_3d_potential_projects_to_projected_potential(Z):
+    """int vatom(sqrt(r^2 + z^2)) dz over the line = vzatom(r): the 3D and
+    the projected potential are one parameterisation (fixes vatom's
+    constants against vzatom's, which the fe transform pins)."""
+    for r in (0.3, 1.0, 2.0):
+        got, _ = scipy.integrate.quad(lambda z: float(M.vatom(Z, math.sqrt(r * r + z * z))), 0.0, 40.0,
+                                      limit=400, points=[0.5, 2.0])
+        assert abs(2 * got - M.vzatom(Z, r)) <= 1e-6 * M.vzatom(Z, r), (r, 2 * got, M.vzatom(Z, r))
 
 
 def test_spline_and_freqn_match_reference(ref):

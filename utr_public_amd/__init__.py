@@ -585,8 +585,10 @@ class Multislice:
         return self._L.msct_slices(self._ctx)
 
     def images(self, angZ_deg, angY_deg, defocus, out=None):
-        """[orientation][defocus][ny][nx] float32 intensities (complex64 for
-        output 1); angles in degrees about Z then Y', defocus in Angstrom.
+        """[orientation][defocus][ny][nx] float32 images: intensity (output 0),
+        phase (1), complex64 amplitude (2) or the 3D potential on the plane
+        z = ctblength + defocus (3), as autosliccmd's noutput; angles in
+        degrees about Z then Y', defocus in Angstrom.
         out: optional torch CUDA tensor of that shape (float32, complex64
         viewed as float32 pairs) filled on the device."""
         aZ = np.radians(np.ascontiguousarray(angZ_deg, dtype=np.float64))
@@ -594,7 +596,7 @@ class Multislice:
         df = np.ascontiguousarray(np.atleast_1d(defocus), dtype=np.float64)
         if len(aZ) != len(aY):
             raise ValueError("angZ and angY differ in length")
-        nx, ny, cplx_out = self.params["nx"], self.params["ny"], self.params["output"] == 1
+        nx, ny, cplx_out = self.params["nx"], self.params["ny"], self.params["output"] == 2
         if out is not None:
             _check(self._L.msct_images_device(self._ctx, len(aZ), _dp(aZ), _dp(aY), len(df), _dp(df),
                                               ctypes.c_void_p(out.data_ptr())))

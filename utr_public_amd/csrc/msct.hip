@@ -39,7 +39,8 @@ struct alignas(16) MsAtom {
     float x, y, occ;
     int zi;        // row of the element in the spline tables
     int ixo, iyo;  // (int)(x / dx), (int)(y / dy) (autoslic.cpp:950-951)
-    int pad0, pad1;
+    float z;
+    int pad;
 };
 
 struct MsGeom {
@@ -226,6 +227,52 @@ __global__ void __launch_bounds__(256) k_ms_potential_rows(const MsAtom* __restr
     for (int p = threadIdx.x; p < nb * N; p += blockDim.x) out[p] = a[lx(p)];
 }
 
+// 3D electrostatic potential of the atoms on a plane at height zp, per
+// pixel (xlo + xst ix, ylo + yst iy), summed over every atom in list order
+// (the noutput 3 branch of autosliccmd.cpp:1004-1044, 1146-1151): float
+// squared distances, vatom() of slicelib.cpp:2699-2737 in FP64, float sum.
+// grid (tiles of 256 pixels, orientation * ndef + d).
+__global__ void __launch_bounds__(256) k_ms_vatom(const MsAtom* __restrict__ atoms, int natom,
+                                                  const double* __restrict__ fe, float xst, float yst,
+                                                  const float* __restrict__ zp, int ndef, int nx, int ny,
+                                                  float* __restrict__ out) {
+    const int od = blockIdx.y, o = od / ndef, d = od - o * ndef;
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    const bool on = p < nx * ny;
+    const int ix = on ? p % nx : 0, iy = on ? p / nx : 0;
+    const float xx = 0.f + xst * (float)ix, yy = 0.f + yst * (float)iy, zz = zp[d];
+    __shared__ MsAtom sa[256];
+    const MsAtom* A = atoms + (size_t)o * natom;
+    const double al = 150.4121417, ag = 266.5985798, pi = 3.141592654;
+    float acc = 0.f;
+    for (int c0 = 0; c0 < natom; c0 += 256) {
+        const int n = min(256, natom - c0);
+        __syncthreads();
+        if ((int)threadIdx.x < n) sa[threadIdx.x] = A[c0 + threadIdx.x];
+        __syncthreads();
+        if (!on) continue;
+        for (int k = 0; k < n; k++) {
+            const MsAtom t = sa[k];
+            const float dx = t.x - xx, dy = t.y - yy, dz = t.z - zz;
+            const float x2 = dx * dx, y2 = dy * dy, z2 = dz * dz;
+            double r = sqrt((double)x2 + (double)y2 + (double)z2);
+            if (r < 1.0e-10) r = 1.0e-10;
+            const double* f = fe + (size_t)t.zi * 12;
+            const double xl = 2.0 * pi * r;
+            double suml = 0.0, sumg = 0.0;
+            for (int i = 0; i < 6; i += 2) suml += f[i] * exp(-xl * sqrt(f[i + 1]));
+            const double xg = (pi * r) * (pi * r);
+            for (int i = 6; i < 12; i += 2) {
+                double tt = sqrt(f[i + 1]);
+                tt = tt * tt * tt;
+                sumg += f[i] * exp(-xg / f[i + 1]) / tt;
+            }
+            acc += (float)(al * suml / r + ag * sumg);
+        }
+    }
+    if (on) out[((size_t)od * ny + iy) * nx + ix] = acc;
+}
+
 // Column pass over [B][ny][nx] (tile = NB adjacent kx columns of one
 // orientation): y FFT (exponent +1), multiply, y inverse FFT (exponent -1),
 // store into dst.  MODE 0: bandwidth limit, factor = scale where
@@ -326,10 +373,11 @@ __global__ void __launch_bounds__(256) k_ms_mul(cplx<float>* __restrict__ W, con
 }
 
 // Lines of E [B][ny][nx] (x-transformed, normalised): inverse x FFT, then
-// |psi|^2 (complex_out false) or psi into out[(o * ndef + d) * ny + y].
+// |psi|^2 (what 0), arg psi (what 1, CArg) or psi (what 2) into
+// out[(o * ndef + d) * ny + y] (autosliccmd.cpp:1196-1239).
 template <int N>
 __global__ void __launch_bounds__(256) k_ms_out(const cplx<float>* __restrict__ E, float* __restrict__ out,
-                                                int ny, int nlines, int ndef, int d, int complex_out,
+                                                int ny, int nlines, int ndef, int d, int what,
                                                 const cplx<float>* __restrict__ tw) {
     constexpr int NB = lds_batch(N, sizeof(cplx<float>), LDS_POST);
     __shared__ cplx<float> sm[2 * lds_len(NB * N)];
@@ -345,9 +393,11 @@ __global__ void __launch_bounds__(256) k_ms_out(const cplx<float>* __restrict__ 
         const int o = l / ny, y = l - o * ny;
         const size_t row = ((size_t)o * ndef + d) * ny + y;
         const cplx<float> v = a[lx(t)];
-        if (complex_out) {
+        if (what == 2) {
             out[2 * (row * N + x)] = v.x;
             out[2 * (row * N + x) + 1] = v.y;
+        } else if (what == 1) {
+            out[row * N + x] = atan2f(v.y, v.x);
         } else {
             out[row * N + x] = v.x * v.x + v.y * v.y;
         }
@@ -474,9 +524,9 @@ struct msct_ctx {
     float ctbf = 0.f, k2max = 0.f, wavlen = 0.f, aobj = 0.f;
     utr::MsGeom g{};
     double scale = 0.0;   // sigma(kev) / 1000
-    DevBuf<double> knots;
+    DevBuf<double> knots, fe_el;  // fe_el: the 12 fe parameters per element row
     DevBuf<double4> coef;
-    DevBuf<float> kx2, ky2;
+    DevBuf<float> kx2, ky2, zp;  // zp: potential planes, ctblength + defocus (float)
     DevBuf<cplx<float>> px, py, twx, twy, F;
     DevBuf<utr::MsAtom> atoms;
     DevBuf<int> off;
@@ -626,7 +676,9 @@ bool ms_prepare_one(const msct_ctx* c, double aZ, double aY, utr::MsAtom* dst, i
         Z[k] = z1;
         idx[k] = k;
     }
-    std::stable_sort(idx.begin(), idx.end(), [&](int i, int j) { return Z[i] < Z[j]; });
+    // the multislice takes the atoms in depth order (sortByZ, autoslic.cpp:720);
+    // the potential output keeps the file order (no multislice runs there)
+    if (c->p.output != 3) std::stable_sort(idx.begin(), idx.end(), [&](int i, int j) { return Z[i] < Z[j]; });
     for (int k = 0; k < n; k++) {
         const int i = idx[k];
         utr::MsAtom& m = dst[k];
@@ -636,7 +688,8 @@ bool ms_prepare_one(const msct_ctx* c, double aZ, double aY, utr::MsAtom* dst, i
         m.zi = c->zi[i];
         m.ixo = (int)((double)X[i] / c->g.sx);
         m.iyo = (int)((double)Y[i] / c->g.sy);
-        m.pad0 = m.pad1 = 0;
+        m.z = Z[i];
+        m.pad = 0;
     }
     const double dz = (double)(float)c->p.deltaz;
     int istart = 0;
@@ -693,7 +746,7 @@ int ms_images(msct_ctx* c, int nang, const double* angZ, const double* angY, int
         if (!std::isfinite(defocus[d])) return ms_fail_invalid("msct: defocus distances must be finite");
     const int nx = c->g.nx, ny = c->g.ny;
     const size_t plane = (size_t)nx * ny;
-    const size_t per_img = plane * (c->p.output ? 2 : 1);
+    const size_t per_img = plane * (c->p.output == 2 ? 2 : 1);
     const int B = std::min(nang, c->batch);
     UTR_HIP_TRY(c->W.alloc(plane * B));
     UTR_HIP_TRY(c->T.alloc(plane * B));
@@ -701,6 +754,7 @@ int ms_images(msct_ctx* c, int nang, const double* angZ, const double* angY, int
     UTR_HIP_TRY(c->atoms.alloc((size_t)B * std::max(c->natom, 1)));
     UTR_HIP_TRY(c->off.alloc((size_t)B * (c->nsl + 1)));
     UTR_HIP_TRY(c->F.alloc(plane * ndef));
+    UTR_HIP_TRY(c->zp.alloc(ndef));
     LocalBuf<float> dout;
     if (!device_out) UTR_HIP_TRY(dout.alloc(per_img * ndef * B));
     hipStream_t st = nullptr;
@@ -718,6 +772,9 @@ int ms_images(msct_ctx* c, int nang, const double* angZ, const double* angY, int
                 std::cos(2.0 * aa), std::sin(2.0 * aa), k2maxo);
             UTR_HIP_TRY(hipGetLastError());
         }
+        std::vector<float> zp(ndef);
+        for (int d = 0; d < ndef; d++) zp[d] = c->ctbf + (float)defocus[d];
+        UTR_HIP_TRY(hipMemcpyAsync(c->zp.p, zp.data(), sizeof(float) * ndef, hipMemcpyHostToDevice, st));
         std::vector<utr::MsAtom> A;
         std::vector<int> off;
         const bool proj = c->p.mode != 0;
@@ -736,6 +793,22 @@ int ms_images(msct_ctx* c, int nang, const double* angZ, const double* angY, int
                 UTR_HIP_TRY(hipMemcpyAsync(c->atoms.p, A.data(), sizeof(utr::MsAtom) * A.size(),
                                            hipMemcpyHostToDevice, st));
             UTR_HIP_TRY(hipMemcpyAsync(c->off.p, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice, st));
+            if (c->p.output == 3) {
+                // 3D potential on the plane z = zmax + defocus (autosliccmd.cpp:1034-1038)
+                float* dst = device_out ? out + plane * ndef * (size_t)a0 : dout.p;
+                if (c->natom > 0) {
+                    utr::k_ms_vatom<<<dim3((unsigned)((plane + 255) / 256), nb * ndef), 256, 0, st>>>(
+                        c->atoms.p, c->natom, c->fe_el.p, c->ctbf / (float)nx, c->ctbf / (float)ny, c->zp.p, ndef,
+                        nx, ny, dst);
+                    UTR_HIP_TRY(hipGetLastError());
+                } else {
+                    UTR_HIP_TRY(hipMemsetAsync(dst, 0, sizeof(float) * plane * ndef * nb, st));
+                }
+                if (!device_out)
+                    UTR_HIP_TRY(hipMemcpyAsync(out + plane * ndef * (size_t)a0, dout.p, sizeof(float) * plane * ndef * nb,
+                                               hipMemcpyDeviceToHost, st));
+                continue;
+            }
             utr::k_ms_init<<<(unsigned)((plane * nb + 255) / 256), 256, 0, st>>>(c->W.p, nx, plane * nb,
                                                                                  proj ? 1 : 0);
             UTR_HIP_TRY(hipGetLastError());
@@ -798,7 +871,8 @@ int msct_create(const msct_params* p, const double* fe_params, int natom, const 
     if (!(p->ctblength > 0) || !(p->deltaz > 0) || !(p->kev > 0))
         return ms_fail_invalid("msct: cube side, slice thickness and energy must be positive");
     if (p->mode < 0 || p->mode > 2) return ms_fail_invalid("msct: mode must be 0 (multislice), 1 or 2 (projection)");
-    if (p->output < 0 || p->output > 1) return ms_fail_invalid("msct: output must be 0 (intensity) or 1 (complex)");
+    if (p->output < 0 || p->output > 3)
+        return ms_fail_invalid("msct: output must be 0 (intensity), 1 (phase), 2 (complex) or 3 (3D potential)");
     if (natom < 0 || (natom > 0 && (!Z || !x || !y || !z || !occ))) return ms_fail_invalid("msct: bad atom arrays");
     for (int k = 0; k < natom; k++) {
         if (Z[k] < 1 || Z[k] > 103) return ms_fail_invalid("msct: atomic numbers must be in 1..103");
@@ -897,7 +971,11 @@ int msct_create(const msct_params* p, const double* fe_params, int natom, const 
         return hipMemcpy(buf.p, v.data(), sizeof(v[0]) * v.size(), hipMemcpyHostToDevice);
     };
     hipError_t e = hipSuccess;
+    std::vector<double> fe_el(std::max<size_t>(1, elems.size() * 12));
+    for (size_t e2 = 0; e2 < elems.size(); e2++)
+        for (int i = 0; i < 12; i++) fe_el[e2 * 12 + i] = fe_params[(size_t)(elems[e2] - 1) * 12 + i];
     if ((e = up(c->knots, knots)) != hipSuccess || (e = up(c->coef, coef)) != hipSuccess ||
+        (e = up(c->fe_el, fe_el)) != hipSuccess ||
         (e = up(c->kx2, kx2)) != hipSuccess || (e = up(c->ky2, ky2)) != hipSuccess ||
         (e = up(c->px, px)) != hipSuccess || (e = up(c->py, py)) != hipSuccess ||
         (e = up(c->twx, make_twiddles<float>(nx))) != hipSuccess ||
@@ -928,7 +1006,7 @@ int msct_slices(const msct_ctx* ctx) { return ctx ? ctx->nsl : -1; }
 
 int msct_destroy(msct_ctx* c) {
     if (!c) return UTR_OK;
-    c->knots.release(); c->coef.release(); c->kx2.release(); c->ky2.release();
+    c->knots.release(); c->fe_el.release(); c->coef.release(); c->kx2.release(); c->ky2.release(); c->zp.release();
     c->px.release(); c->py.release(); c->twx.release(); c->twy.release(); c->F.release();
     c->atoms.release(); c->off.release(); c->W.release(); c->T.release(); c->E.release();
     delete c;
