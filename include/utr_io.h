@@ -49,6 +49,23 @@ int utr_io_file_name(const char* tmpl, long count, long index, char* out, long c
 int utr_io_find_peaks(float* vol, long nx, long ny, long nz, const double* extent6, double atom_xy,
                       double atom_z, int* out_ijk, float* out_height, long cap, long* npeaks);
 
+/*
+ * Atom matching of pdb-compare (MultisliceCppTest/pdb-compare/
+ * pdb-compare.cpp:255-433): test atoms (e.g. peaks of a reconstruction) are
+ * matched to reference atoms (the specimen), both taken in descending
+ * atomic number.  hungarian != 0: the one-to-one matching of least total
+ * squared distance (the reference's HungarianAlgorithm::Solve, no distance
+ * limit); else each reference atom in turn takes the closest unmatched test
+ * atom nearer than max_dist.  match[i] / dist[i] (nref each) receive the
+ * test index and distance for reference atom i, or -1; stats (6) the
+ * number matched, false negatives, false positives, and the mean, maximum
+ * and standard deviation (n - 1) of the matched distances.  Hydrogen in the
+ * reference set is an error, as in the reference.
+ */
+int utr_io_match_atoms(int ntest, const double* test_xyz, const int* test_z, int nref, const double* ref_xyz,
+                       const int* ref_z, double max_dist, int hungarian, int* match, double* dist,
+                       double* stats);
+
 const char* utr_io_last_error(void);
 
 #ifdef __cplusplus

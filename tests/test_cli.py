@@ -101,8 +101,13 @@ def check_json(cwd, pfile):
 def test_exports_and_exe_exist():
     assert os.access(EXE, os.X_OK)
     lib = ctypes.CDLL(IOLIB)
-    for sym in ("utr_io_read_image", "utr_io_write_slice", "utr_io_file_name", "utr_io_last_error"):
-        assert hasattr(lib, sym)
+    import re
+    hdr = open(os.path.join(os.path.dirname(IOLIB), "..", "..", "include", "utr_io.h")).read()
+    syms = set(re.findall(r"\b(utr_io_[a-z0-9_]+)\s*\(", re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)))
+    assert {"utr_io_read_image", "utr_io_write_slice", "utr_io_file_name", "utr_io_find_peaks",
+            "utr_io_match_atoms", "utr_io_last_error"} <= syms
+    for sym in syms:
+        assert hasattr(lib, sym), sym
 
 
 def test_txt_parameter_file_derived_geometry(work):

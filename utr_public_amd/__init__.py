@@ -622,3 +622,48 @@ def multislice_images(params, atoms, angZ_deg, angY_deg, defocus):
         return m.images(angZ_deg, angY_deg, defocus)
     finally:
         m.close()
+
+
+# --------------------------------------------------------------------------
+# Evaluation against a known atom set (libutr_io: pdb-compare's matching)
+
+IO_LIB_PATH = os.path.join(_PKG, "_lib", "libutr_io.so")
+_io = None
+
+
+def _io_lib():
+    global _io
+    if _io is None:
+        if not os.path.exists(IO_LIB_PATH):
+            raise RuntimeError("libutr_io.so is not built (%s); run __graft_entry__.build()" % IO_LIB_PATH)
+        L = ctypes.CDLL(IO_LIB_PATH)
+        vp, i, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+        L.utr_io_match_atoms.argtypes = [i, vp, vp, i, vp, vp, d, i, vp, vp, vp]
+        L.utr_io_last_error.restype = ctypes.c_char_p
+        _io = L
+    return _io
+
+
+def match_atoms(test_xyz, test_Z, ref_xyz, ref_Z, max_dist=1.0, hungarian=True):
+    """pdb-compare's matching (pdb-compare.cpp:255-433) of test atoms (e.g.
+    peaks of a reconstruction) to reference atoms.  Returns (match, dist,
+    stats): per reference atom the matched test index and distance (-1 when
+    unmatched), and a dict with matched, false_negatives, false_positives,
+    mean, max and std of the matched distances."""
+    L = _io_lib()
+    t = np.ascontiguousarray(test_xyz, dtype=np.float64).reshape(-1, 3)
+    r = np.ascontiguousarray(ref_xyz, dtype=np.float64).reshape(-1, 3)
+    tz = np.ascontiguousarray(test_Z, dtype=np.int32)
+    rz = np.ascontiguousarray(ref_Z, dtype=np.int32)
+    m = np.empty(len(r), np.int32)
+    dist = np.empty(len(r))
+    st = np.empty(6)
+    rc = L.utr_io_match_atoms(len(t), t.ctypes.data, tz.ctypes.data, len(r), r.ctypes.data, rz.ctypes.data,
+                              float(max_dist), 1 if hungarian else 0, m.ctypes.data, dist.ctypes.data,
+                              st.ctypes.data)
+    if rc:
+        raise UTRError(rc, L.utr_io_last_error().decode())
+    keys = ("matched", "false_negatives", "false_positives", "mean", "max", "std")
+    stats = {k: (int(v) if i < 3 else float(v)) for i, (k, v) in enumerate(zip(keys, st))}
+    return m, dist, stats
+

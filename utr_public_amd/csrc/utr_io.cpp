@@ -14,6 +14,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <limits>
 #include <sstream>
 
 namespace utrio {
@@ -1110,6 +1111,156 @@ extern "C" int utr_io_find_peaks(float* vol, long nx, long ny, long nz, const do
             }
             if (out_height) out_height[n] = peaks[n].height;
         }
+        return 0;
+    } catch (std::exception& ex) {
+        return io_fail(ex);
+    }
+}
+
+// ------------------------------------------------------- atom matching
+
+namespace {
+
+// Minimum-cost assignment of the rows of c (n x m, n <= m) to distinct
+// columns: shortest augmenting paths with row and column potentials (Kuhn
+// and Munkres' Hungarian method in its O(n^2 m) form).  Returns col[i].
+std::vector<int> min_cost_assignment(const std::vector<double>& c, int n, int m) {
+    const double INF = std::numeric_limits<double>::infinity();
+    std::vector<double> u(n + 1, 0.0), v(m + 1, 0.0);
+    std::vector<int> p(m + 1, 0), way(m + 1, 0);
+    for (int i = 1; i <= n; i++) {
+        p[0] = i;
+        int j0 = 0;
+        std::vector<double> minv(m + 1, INF);
+        std::vector<char> used(m + 1, 0);
+        do {
+            used[j0] = 1;
+            const int i0 = p[j0];
+            double delta = INF;
+            int j1 = 0;
+            for (int j = 1; j <= m; j++) {
+                if (used[j]) continue;
+                const double cur = c[(size_t)(i0 - 1) * m + (j - 1)] - u[i0] - v[j];
+                if (cur < minv[j]) {
+                    minv[j] = cur;
+                    way[j] = j0;
+                }
+                if (minv[j] < delta) {
+                    delta = minv[j];
+                    j1 = j;
+                }
+            }
+            for (int j = 0; j <= m; j++) {
+                if (used[j]) {
+                    u[p[j]] += delta;
+                    v[j] -= delta;
+                } else {
+                    minv[j] -= delta;
+                }
+            }
+            j0 = j1;
+        } while (p[j0] != 0);
+        do {
+            const int j1 = way[j0];
+            p[j0] = p[j1];
+            j0 = j1;
+        } while (j0);
+    }
+    std::vector<int> col(n, -1);
+    for (int j = 1; j <= m; j++)
+        if (p[j]) col[p[j] - 1] = j - 1;
+    return col;
+}
+
+}  // namespace
+
+extern "C" int utr_io_match_atoms(int ntest, const double* test_xyz, const int* test_z, int nref,
+                                  const double* ref_xyz, const int* ref_z, double max_dist, int hungarian,
+                                  int* match, double* dist, double* stats) {
+    try {
+        if (ntest < 2 || nref < 2 || !test_xyz || !test_z || !ref_xyz || !ref_z || !match || !dist || !stats)
+            throw std::runtime_error("Error: fewer than 2 atoms or null arrays in utr_io_match_atoms.");
+        if (!(max_dist > 0)) throw std::runtime_error("Error: maximum match distance must be positive.");
+        for (int i = 0; i < nref; i++)
+            if (ref_z[i] == 1)
+                throw std::runtime_error("Error: hydrogen atom found in the reference data - matching of "
+                                         "hydrogen atoms in not supported.");
+        // both lists in descending atomic number, ties in input order
+        // (pdb-compare.cpp:262-264)
+        std::vector<int> ot(ntest), orf(nref);
+        for (int i = 0; i < ntest; i++) ot[i] = i;
+        for (int i = 0; i < nref; i++) orf[i] = i;
+        std::stable_sort(ot.begin(), ot.end(), [&](int a, int b) { return test_z[a] > test_z[b]; });
+        std::stable_sort(orf.begin(), orf.end(), [&](int a, int b) { return ref_z[a] > ref_z[b]; });
+        // squared distances reference x test (pdb-compare.cpp:363-372)
+        std::vector<double> r2((size_t)nref * ntest);
+        for (int i = 0; i < nref; i++)
+            for (int j = 0; j < ntest; j++) {
+                const double* a = ref_xyz + 3 * (size_t)orf[i];
+                const double* b = test_xyz + 3 * (size_t)ot[j];
+                const double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+                r2[(size_t)i * ntest + j] = dx * dx + dy * dy + dz * dz;
+            }
+        std::vector<int> m(nref, -1);
+        if (hungarian) {
+            // optimal one-to-one matching of the squared distances; every
+            // reference atom is matched when there are enough test atoms
+            // (pdb-compare.cpp:373-383: no distance limit in this mode)
+            if (nref <= ntest) {
+                m = min_cost_assignment(r2, nref, ntest);
+            } else {
+                std::vector<double> t((size_t)ntest * nref);
+                for (int i = 0; i < nref; i++)
+                    for (int j = 0; j < ntest; j++) t[(size_t)j * nref + i] = r2[(size_t)i * ntest + j];
+                const std::vector<int> mt = min_cost_assignment(t, ntest, nref);
+                for (int j = 0; j < ntest; j++)
+                    if (mt[j] >= 0) m[mt[j]] = j;
+            }
+        } else {
+            // each reference atom in turn takes the closest unmatched test
+            // atom if it is nearer than max_dist (pdb-compare.cpp:384-404)
+            std::vector<char> taken(ntest, 0);
+            const double d2max = max_dist * max_dist;
+            for (int i = 0; i < nref; i++) {
+                int jmin = -1;
+                double best = std::numeric_limits<double>::max();
+                for (int j = 0; j < ntest; j++)
+                    if (!taken[j] && r2[(size_t)i * ntest + j] < best) {
+                        best = r2[(size_t)i * ntest + j];
+                        jmin = j;
+                    }
+                if (jmin >= 0 && best < d2max) {
+                    m[i] = jmin;
+                    taken[jmin] = 1;
+                }
+            }
+        }
+        // statistics (pdb-compare.cpp:405-433)
+        int nmatch = 0;
+        double mean = 0.0, dmax = 0.0;
+        for (int i = 0; i < nref; i++) {
+            match[orf[i]] = m[i] >= 0 ? ot[m[i]] : -1;
+            dist[orf[i]] = m[i] >= 0 ? std::sqrt(r2[(size_t)i * ntest + m[i]]) : -1.0;
+            if (m[i] < 0) continue;
+            const double d = std::sqrt(r2[(size_t)i * ntest + m[i]]);
+            nmatch++;
+            mean += d;
+            dmax = std::max(dmax, d);
+        }
+        mean = nmatch ? mean / nmatch : 0.0;
+        double sd = 0.0;
+        for (int i = 0; i < nref; i++)
+            if (m[i] >= 0) {
+                const double d = std::sqrt(r2[(size_t)i * ntest + m[i]]) - mean;
+                sd += d * d;
+            }
+        sd = nmatch <= 1 ? 0.0 : std::sqrt(sd / (nmatch - 1));
+        stats[0] = nmatch;
+        stats[1] = nref - nmatch;   // false negatives
+        stats[2] = ntest - nmatch;  // false positives
+        stats[3] = mean;
+        stats[4] = dmax;
+        stats[5] = sd;
         return 0;
     } catch (std::exception& ex) {
         return io_fail(ex);
