@@ -130,6 +130,8 @@ def test_empty_specimen_and_device_output():
     dev = torch.empty((2, 1, 64, 64), dtype=torch.float32, device="cuda")
     m.images([20.0, 50.0], [30.0, 60.0], [80.0], out=dev)
     assert np.array_equal(dev.cpu().numpy(), host)
+    with pytest.raises(ValueError):  # too small: refused before any launch
+        m.images([20.0, 50.0], [30.0, 60.0], [80.0], out=dev[:1])
 
 
 def test_errors():

@@ -598,6 +598,11 @@ class Multislice:
             raise ValueError("angZ and angY differ in length")
         nx, ny, cplx_out = self.params["nx"], self.params["ny"], self.params["output"] == 2
         if out is not None:
+            need = len(aZ) * len(df) * ny * nx * (2 if cplx_out else 1)
+            if not (out.is_cuda and out.is_contiguous() and out.element_size() * out.numel() >= 4 * need
+                    and str(out.dtype) in ("torch.float32", "torch.complex64")):
+                raise ValueError("out must be a contiguous CUDA float32/complex64 tensor of at least %d floats"
+                                 % need)
             _check(self._L.msct_images_device(self._ctx, len(aZ), _dp(aZ), _dp(aY), len(df), _dp(df),
                                               ctypes.c_void_p(out.data_ptr())))
             return out
