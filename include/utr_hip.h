@@ -352,17 +352,23 @@ typedef struct msct_params {
     double astig_dz;        /* two-fold astigmatism amplitude, Angstrom            */
     double astig_angle_deg; /* and its angle, degrees (autosliccmd's astigm pair)  */
     double dxc, dyc, dzc;   /* rotation centre minus the cube centre, A (line 13)  */
+    int nwobble;            /* thermal configurations averaged (MsctKirkland lines
+                               12 and 14); 0 or 1 = none                           */
+    int reserved;
+    double temperature;     /* K (line 13): displacements scale by sqrt(T / 300)   */
+    unsigned long long seed; /* of the configurations' displacements             */
 } msct_params;
 
 typedef struct msct_ctx msct_ctx;
 
-/* One atom set (ReadXYZcoord's Z, x, y, z, occ; natom may be 0) and the
+/* One atom set (ReadXYZcoord's Z, x, y, z, occ and the rms thermal
+ * displacement at 300 K, wobble, which may be NULL; natom may be 0) and the
  * electron scattering-factor parameters fe_params[(Z-1)*12 + i] of Kirkland
  * (2010) App. C for Z = 1..103 (slicelib.cpp:958-2209, shipped as
  * utr_public_amd/data/fe_params.txt).  Replaces the set-up half of
  * autosliccmd() (autosliccmd.cpp:600-690, 870-910). */
 int msct_create(const msct_params* p, const double* fe_params, int natom, const int* Z, const float* x,
-                const float* y, const float* z, const float* occ, msct_ctx** out);
+                const float* y, const float* z, const float* occ, const float* wobble, msct_ctx** out);
 
 /* Images of nang orientations (angZ, angY in radians, the autosliccmd line
  * 25 pair: rotation about Z, then about Y', autosliccmd.cpp:733-748) at ndef

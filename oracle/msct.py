@@ -368,6 +368,38 @@ def potential_plane(p, x, y, z, Zn, zplane):
     return out
 
 
+_M64 = (1 << 64) - 1
+
+
+def _splitmix(x):
+    x = (x + 0x9E3779B97F4A7C15) & _M64
+    x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+    x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & _M64
+    return ((x ^ (x >> 31)) >> 11) * (1.0 / 9007199254740992.0)
+
+
+def normal(seed, a, w, k, axis):
+    """The seeded standard normal of orientation a, configuration w, atom k,
+    axis (0 x, 1 y, 2 z): SplitMix64 of the key, Box-Muller.  (The reference
+    draws gasdev() numbers from a clock seed, autosliccmd.cpp:531, so it has
+    no reproducible configurations to compare with.)"""
+    key = ((a << 40) ^ (w << 24) ^ (k << 2) ^ axis) & _M64
+    u1 = _splitmix((seed + 2 * key) & _M64) + 1.0 / 9007199254740992.0
+    u2 = _splitmix((seed + 2 * key + 1) & _M64)
+    return math.sqrt(-2.0 * math.log(u1)) * math.cos(2.0 * math.pi * u2)
+
+
+def displace(x, y, z, wobble, temperature, seed, a, w):
+    """Thermal configuration w of orientation a: x + wobble * g * sqrt(T/300)
+    per axis on the rotated coordinates (autoslic.cpp:700-711)."""
+    sc = float(f32(math.sqrt(temperature / 300.0)))
+    out = []
+    for axis, v in enumerate((x, y, z)):
+        g = np.array([normal(seed, a, w, k, axis) for k in range(len(v))])
+        out.append((v + (np.asarray(wobble, np.float64) * g * sc).astype(f32)).astype(f32))
+    return out
+
+
 def images(p, atoms, angZ, angY, defocus, output="intensity"):
     """Images [orientation][defocus][ny][nx] of an atom set (autosliccmd.cpp:
     709-748 rotation, autoslic::calculate, 1060-1087 propagation, 1196-1200
@@ -379,8 +411,32 @@ def images(p, atoms, angZ, angY, defocus, output="intensity"):
     out = np.zeros((len(angZ), len(defocus), p["ny"], p["nx"]),
                    np.complex128 if output == "complex" else np.float64)
     for a in range(len(angZ)):
-        x, y, z = rotate_atoms(atoms["x"], atoms["y"], atoms["z"], angZ[a], angY[a], p["ctb"],
-                               p.get("dxc", 0.0), p.get("dyc", 0.0), p.get("dzc", 0.0))
+        x0, y0, z0 = rotate_atoms(atoms["x"], atoms["y"], atoms["z"], angZ[a], angY[a], p["ctb"],
+                                  p.get("dxc", 0.0), p.get("dyc", 0.0), p.get("dzc", 0.0))
+        nw = max(1, p.get("nwobble", 1))
+        if nw > 1:
+            if len(z0) and (min(x0.min(), y0.min(), z0.min()) < 0 or max(x0.max(), y0.max(), z0.max()) > f32(p["ctb"])):
+                raise ValueError("atoms outside the cube")
+            # intensities (or potentials) summed in float over the
+            # configurations, then divided by their number (autosliccmd.cpp:1200-1208)
+            acc = None
+            for w in range(nw):
+                x, y, z = displace(x0, y0, z0, atoms.get("wobble", np.zeros(len(z0))), p.get("temperature", 300.0),
+                                   p.get("seed", 1), a, w)
+                one = np.zeros((len(defocus), p["ny"], p["nx"]))
+                for j, d in enumerate(defocus):
+                    if output == "potential":
+                        one[j] = potential_plane(p, x, y, z, atoms["Z"], f32(p["ctb"]) + f32(d))
+                    else:
+                        wv = m.exit_wave(x, y, z, atoms["occ"], atoms["Z"]) if j == 0 else wv
+                        R = d - 0.5 * p["ctb"]
+                        c = m.fresnel(wv, R, aobj, p.get("cs3_mm", 0.0), p.get("cs5_mm", 0.0), p.get("astig_dz", 0.0),
+                                      math.radians(p.get("astig_angle", 0.0)))
+                        one[j] = np.abs(c) ** 2
+                acc = one if acc is None else acc + one
+            out[a] = acc / nw
+            continue
+        x, y, z = x0, y0, z0
         if output == "potential":
             if len(z) and (min(x.min(), y.min(), z.min()) < 0 or max(x.max(), y.max(), z.max()) > f32(p["ctb"])):
                 raise ValueError("atoms outside the cube")

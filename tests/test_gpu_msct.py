@@ -195,3 +195,23 @@ def test_phase_and_potential_outputs():
     print("potential rel-L2 %.2e" % e, flush=True)
     assert e < 1e-5
 
+
+
+@pytest.mark.parametrize("output", [0, 3])
+def test_thermal_configurations(output):
+    """nwobble seeded configurations (rms displacement at 300 K scaled by
+    sqrt(T/300)), intensities or potentials averaged over them."""
+    p = U.default_msct_params(nx=64, ny=64, ctblength=51.2, deltaz=1.0, kev=200.0, nwobble=3, temperature=250.0,
+                              seed=11, output=output)
+    at = atoms_in_cube(12, 60, 51.2, 18.0)
+    at["wobble"] = np.full(len(at["Z"]), 0.12, np.float32)
+    defocus = [100.0] if output == 0 else [-25.6, -20.0]
+    got = U.multislice_images(p, at, [10.0, 80.0], [30.0, 140.0], defocus)
+    op = dict(oracle_params(p), nwobble=3, temperature=250.0, seed=11)
+    ref = M.images(op, at, np.radians([10.0, 80.0]), np.radians([30.0, 140.0]), defocus,
+                   output="intensity" if output == 0 else "potential")
+    e = rel(got - 1, ref - 1) if output == 0 else rel(got, ref)
+    print("thermal output %d rel-L2 %.2e" % (output, e), flush=True)
+    assert e < (TOL_CONTRAST if output == 0 else 1e-5)
+    plain = U.multislice_images(dict(p, nwobble=1), at, [10.0, 80.0], [30.0, 140.0], defocus)
+    assert rel(got, plain) > 1e-4  # the displacements change the images

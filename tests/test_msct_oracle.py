@@ -231,3 +231,16 @@ def test_atoms_leaving_the_cube_are_an_error():
               z=np.array([25.0], np.float32), occ=np.ones(1, np.float32))
     with pytest.raises(ValueError):
         M.images(P0, at, [math.pi / 4], [0.0], [0.0])
+
+
+def test_thermal_normals_are_standard_and_reproducible():
+    g = np.array([M.normal(7, a, w, k, ax) for a in range(4) for w in range(5) for k in range(300) for ax in range(3)])
+    assert abs(g.mean()) < 0.03 and abs(g.std() - 1.0) < 0.03
+    assert M.normal(7, 1, 2, 3, 0) == M.normal(7, 1, 2, 3, 0) != M.normal(8, 1, 2, 3, 0)
+
+
+def test_thermal_average_without_displacement_is_the_single_image():
+    at = layer(seed=3)
+    one = M.images(P0, at, [0.2], [0.5], [80.0])
+    avg = M.images(dict(P0, nwobble=3), dict(at, wobble=np.zeros(12, np.float32)), [0.2], [0.5], [80.0])
+    assert np.allclose(avg, one, rtol=1e-12, atol=1e-12)

@@ -63,6 +63,8 @@ MSCT_FIELDS = [
     ("aobj_mrad", ctypes.c_double), ("cs3_mm", ctypes.c_double), ("cs5_mm", ctypes.c_double),
     ("astig_dz", ctypes.c_double), ("astig_angle_deg", ctypes.c_double),
     ("dxc", ctypes.c_double), ("dyc", ctypes.c_double), ("dzc", ctypes.c_double),
+    ("nwobble", ctypes.c_int), ("reserved", ctypes.c_int), ("temperature", ctypes.c_double),
+    ("seed", ctypes.c_ulonglong),
 ]
 
 
@@ -196,7 +198,7 @@ def load():
         getattr(L, f).argtypes = [vp, i, i, i, i, dp, dp, dp]
     fp = ctypes.POINTER(ctypes.c_float)
     L.msct_create.argtypes = [ctypes.POINTER(MsctParams), dp, i, ctypes.POINTER(ctypes.c_int), fp, fp, fp,
-                              fp, ctypes.POINTER(vp)]
+                              fp, fp, ctypes.POINTER(vp)]
     L.msct_images.argtypes = [vp, i, dp, dp, i, dp, vp]
     L.msct_images_device.argtypes = [vp, i, dp, dp, i, dp, vp]
     L.msct_slices.argtypes = [vp]
@@ -553,14 +555,17 @@ def default_msct_params(**kw):
     """Multislice set-up: 128^2 pixels over a 51.2 A cube, 1 A slices, 200 keV,
     no aperture or aberrations (the autosliccmd parameter lines)."""
     d = dict(nx=128, ny=128, mode=0, output=0, ctblength=51.2, deltaz=1.0, kev=200.0, aobj_mrad=0.0,
-             cs3_mm=0.0, cs5_mm=0.0, astig_dz=0.0, astig_angle_deg=0.0, dxc=0.0, dyc=0.0, dzc=0.0)
+             cs3_mm=0.0, cs5_mm=0.0, astig_dz=0.0, astig_angle_deg=0.0, dxc=0.0, dyc=0.0, dzc=0.0,
+             nwobble=1, reserved=0, temperature=300.0, seed=1)
     d.update(kw)
     return d
 
 
 class Multislice:
     """One atom set for the multislice forward model (autosliccmd()):
-    atoms = dict(Z, x, y, z[, occ]) in Angstrom inside [0, ctblength]^3."""
+    atoms = dict(Z, x, y, z[, occ][, wobble]) in Angstrom inside
+    [0, ctblength]^3; wobble is the rms thermal displacement at 300 K, used
+    when params nwobble > 1."""
 
     def __init__(self, params, atoms):
         self._L = load()
@@ -572,12 +577,15 @@ class Multislice:
         n = len(Z)
         xs = [np.ascontiguousarray(atoms[k], dtype=np.float32) for k in ("x", "y", "z")]
         occ = np.ascontiguousarray(atoms.get("occ", np.ones(n)), dtype=np.float32)
+        wob = atoms.get("wobble")
+        wob = None if wob is None else np.ascontiguousarray(wob, dtype=np.float32)
         fe = np.ascontiguousarray(fe_params())
         fpp = ctypes.POINTER(ctypes.c_float)
         ctx = ctypes.c_void_p()
         _check(self._L.msct_create(ctypes.byref(p), _dp(fe), n, Z.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                                    xs[0].ctypes.data_as(fpp), xs[1].ctypes.data_as(fpp),
-                                   xs[2].ctypes.data_as(fpp), occ.ctypes.data_as(fpp), ctypes.byref(ctx)))
+                                   xs[2].ctypes.data_as(fpp), occ.ctypes.data_as(fpp),
+                                   None if wob is None else wob.ctypes.data_as(fpp), ctypes.byref(ctx)))
         self._ctx = ctx
 
     @property

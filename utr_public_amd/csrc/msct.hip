@@ -235,7 +235,7 @@ __global__ void __launch_bounds__(256) k_ms_potential_rows(const MsAtom* __restr
 __global__ void __launch_bounds__(256) k_ms_vatom(const MsAtom* __restrict__ atoms, int natom,
                                                   const double* __restrict__ fe, float xst, float yst,
                                                   const float* __restrict__ zp, int ndef, int nx, int ny,
-                                                  float* __restrict__ out) {
+                                                  float* __restrict__ out, int accum, float div) {
     const int od = blockIdx.y, o = od / ndef, d = od - o * ndef;
     const int p = blockIdx.x * 256 + threadIdx.x;
     const bool on = p < nx * ny;
@@ -270,7 +270,10 @@ __global__ void __launch_bounds__(256) k_ms_vatom(const MsAtom* __restrict__ ato
             acc += (float)(al * suml / r + ag * sumg);
         }
     }
-    if (on) out[((size_t)od * ny + iy) * nx + ix] = acc;
+    if (on) {
+        float* q = out + ((size_t)od * ny + iy) * nx + ix;
+        *q = (accum ? *q + acc : acc) / div;  // thermal average (autosliccmd.cpp:1242-1252)
+    }
 }
 
 // Column pass over [B][ny][nx] (tile = NB adjacent kx columns of one
@@ -378,7 +381,7 @@ __global__ void __launch_bounds__(256) k_ms_mul(cplx<float>* __restrict__ W, con
 template <int N>
 __global__ void __launch_bounds__(256) k_ms_out(const cplx<float>* __restrict__ E, float* __restrict__ out,
                                                 int ny, int nlines, int ndef, int d, int what,
-                                                const cplx<float>* __restrict__ tw) {
+                                                const cplx<float>* __restrict__ tw, int accum, float div) {
     constexpr int NB = lds_batch(N, sizeof(cplx<float>), LDS_POST);
     __shared__ cplx<float> sm[2 * lds_len(NB * N)];
     cplx<float>* a = sm;
@@ -399,7 +402,10 @@ __global__ void __launch_bounds__(256) k_ms_out(const cplx<float>* __restrict__ 
         } else if (what == 1) {
             out[row * N + x] = atan2f(v.y, v.x);
         } else {
-            out[row * N + x] = v.x * v.x + v.y * v.y;
+            // thermal configurations: the running sum, divided by their
+            // number after the last one (autosliccmd.cpp:1200-1208)
+            const float i2 = v.x * v.x + v.y * v.y;
+            out[row * N + x] = (accum ? out[row * N + x] + i2 : i2) / div;
         }
     }
 }
@@ -519,7 +525,7 @@ struct msct_ctx {
     msct_params p{};
     int natom = 0;
     std::vector<int> zi;  // element row per atom
-    std::vector<float> x, y, z, occ;
+    std::vector<float> x, y, z, occ, wob;  // wob: rms thermal displacement at 300 K
     int nsl = 0;          // slices per orientation
     float ctbf = 0.f, k2max = 0.f, wavlen = 0.f, aobj = 0.f;
     utr::MsGeom g{};
@@ -610,7 +616,8 @@ int ms_run_cols(int mode, msct_ctx* c, const cplx<float>* src, cplx<float>* dst,
     return UTR_OK;
 }
 
-int ms_run_lines(msct_ctx* c, int what, int B, int s, int ndef, int d, float* out, hipStream_t st) {
+int ms_run_lines(msct_ctx* c, int what, int B, int s, int ndef, int d, float* out, hipStream_t st,
+                 int accum = 0, float div = 1.f) {
     const int nx = c->g.nx, ny = c->g.ny, nlines = B * ny;
     const bool proj = c->p.mode != 0;
     bool ok = false;
@@ -637,7 +644,7 @@ int ms_run_lines(msct_ctx* c, int what, int B, int s, int ndef, int d, float* ou
                                                                                c->twx.p);           \
         } else {                                                                                    \
             utr::k_ms_out<N><<<(nlines + NB - 1) / NB, 256, 0, st>>>(c->E.p, out, ny, nlines, ndef, \
-                                                                    d, c->p.output, c->twx.p);      \
+                                                                    d, c->p.output, c->twx.p, accum, div); \
         }                                                                                           \
         ok = true;                                                                                  \
     }
@@ -653,8 +660,26 @@ int ms_run_lines(msct_ctx* c, int what, int B, int s, int ndef, int d, float* ou
 // bounds test of autosliccmd.cpp:857-860 and the depth sort and slice
 // membership of autoslic.cpp:720-744.  Returns false when an atom leaves
 // the cube.
-bool ms_prepare_one(const msct_ctx* c, double aZ, double aY, utr::MsAtom* dst, int* o, std::vector<float>& X,
-                    std::vector<float>& Y, std::vector<float>& Z, std::vector<int>& idx) {
+// Standard normal number for (orientation, configuration, atom, axis):
+// counter-based (SplitMix64 of the key, then Box-Muller), so batches and
+// host threads do not change the configurations.  The reference draws them
+// from Numerical Recipes' gasdev seeded with the clock (autosliccmd.cpp:531),
+// so its configurations are not reproducible; these are, for a given seed.
+double ms_splitmix(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return (double)((x ^ (x >> 31)) >> 11) * (1.0 / 9007199254740992.0);
+}
+double ms_normal(uint64_t seed, int a, int w, int k, int axis) {
+    const uint64_t key = ((uint64_t)a << 40) ^ ((uint64_t)w << 24) ^ ((uint64_t)k << 2) ^ (uint64_t)axis;
+    const double u1 = ms_splitmix(seed + 2 * key) + 1.0 / 9007199254740992.0;  // (0, 1]
+    const double u2 = ms_splitmix(seed + 2 * key + 1);
+    return std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * kMsPi * u2);
+}
+
+bool ms_prepare_one(const msct_ctx* c, int a, int w, double aZ, double aY, utr::MsAtom* dst, int* o,
+                    std::vector<float>& X, std::vector<float>& Y, std::vector<float>& Z, std::vector<int>& idx) {
     const int n = c->natom;
     X.resize(n);
     Y.resize(n);
@@ -675,6 +700,20 @@ bool ms_prepare_one(const msct_ctx* c, double aZ, double aY, utr::MsAtom* dst, i
         Y[k] = yy;
         Z[k] = z1;
         idx[k] = k;
+    }
+    if (c->p.nwobble > 1) {
+        // thermal configuration w: rms displacement wobble (at 300 K) scaled
+        // by sqrt(T / 300) on the rotated coordinates (autoslic.cpp:700-711)
+        const double sc = (double)(float)std::sqrt(c->p.temperature / 300.0);
+        for (int k = 0; k < n; k++) {
+            const double wb = c->wob[k];
+            X[k] = X[k] + (float)(wb * ms_normal(c->p.seed, a, w, k, 0) * sc);
+            Y[k] = Y[k] + (float)(wb * ms_normal(c->p.seed, a, w, k, 1) * sc);
+            Z[k] = Z[k] + (float)(wb * ms_normal(c->p.seed, a, w, k, 2) * sc);
+            // the depth range is checked again after the displacement
+            // (autoslic.cpp:733-738)
+            if (!(Z[k] >= 0.f && Z[k] <= ctb)) return false;
+        }
     }
     // the multislice takes the atoms in depth order (sortByZ, autoslic.cpp:720);
     // the potential output keeps the file order (no multislice runs there)
@@ -705,7 +744,7 @@ bool ms_prepare_one(const msct_ctx* c, double aZ, double aY, utr::MsAtom* dst, i
 
 // orientations [a0, a0 + B) into A (B x natom) and off (B x (nsl + 1)), on
 // up to 16 host threads
-int ms_prepare(const msct_ctx* c, int a0, int B, const double* angZ, const double* angY,
+int ms_prepare(const msct_ctx* c, int a0, int B, int w, const double* angZ, const double* angY,
                std::vector<utr::MsAtom>& A, std::vector<int>& off) {
     const int n = c->natom, nsl1 = c->nsl + 1;
     A.resize((size_t)B * n);
@@ -716,8 +755,8 @@ int ms_prepare(const msct_ctx* c, int a0, int B, const double* angZ, const doubl
         std::vector<float> X, Y, Z;
         std::vector<int> idx;
         for (int b = t; b < B; b += nt)
-            if (!ms_prepare_one(c, angZ[a0 + b], angY[a0 + b], A.data() + (size_t)b * n, off.data() + (size_t)b * nsl1,
-                                X, Y, Z, idx)) {
+            if (!ms_prepare_one(c, a0 + b, w, angZ[a0 + b], angY[a0 + b], A.data() + (size_t)b * n,
+                                off.data() + (size_t)b * nsl1, X, Y, Z, idx)) {
                 bad[t] = b;
                 return;
             }
@@ -784,9 +823,13 @@ int ms_images(msct_ctx* c, int nang, const double* angZ, const double* angY, int
         const bool fused = !(fenv && std::atoi(fenv) == 0);
         const int tiles_x = (nx + 15) / 16, tiles = tiles_x * ((ny + 15) / 16);
         const float inv = 1.f / (float)plane;
-        for (int a0 = 0; a0 < nang; a0 += B) {
+        const int nw = std::max(1, c->p.nwobble);
+        for (int bw = 0; bw < ((nang + B - 1) / B) * nw; bw++) {
+            const int a0 = (bw / nw) * B, w = bw % nw;
+            const int accum = w > 0 ? 1 : 0;
+            const float div = w == nw - 1 ? (float)nw : 1.f;
             const int nb = std::min(B, nang - a0);
-            int rc = ms_prepare(c, a0, nb, angZ, angY, A, off);
+            int rc = ms_prepare(c, a0, nb, w, angZ, angY, A, off);
             if (rc) return rc;
             UTR_HIP_TRY(hipStreamSynchronize(st));  // the previous batch no longer reads atoms/off
             if (c->natom > 0)
@@ -799,12 +842,12 @@ int ms_images(msct_ctx* c, int nang, const double* angZ, const double* angY, int
                 if (c->natom > 0) {
                     utr::k_ms_vatom<<<dim3((unsigned)((plane + 255) / 256), nb * ndef), 256, 0, st>>>(
                         c->atoms.p, c->natom, c->fe_el.p, c->ctbf / (float)nx, c->ctbf / (float)ny, c->zp.p, ndef,
-                        nx, ny, dst);
+                        nx, ny, dst, accum, div);
                     UTR_HIP_TRY(hipGetLastError());
                 } else {
                     UTR_HIP_TRY(hipMemsetAsync(dst, 0, sizeof(float) * plane * ndef * nb, st));
                 }
-                if (!device_out)
+                if (!device_out && w == nw - 1)
                     UTR_HIP_TRY(hipMemcpyAsync(out + plane * ndef * (size_t)a0, dout.p, sizeof(float) * plane * ndef * nb,
                                                hipMemcpyDeviceToHost, st));
                 continue;
@@ -837,9 +880,9 @@ int ms_images(msct_ctx* c, int nang, const double* angZ, const double* angY, int
             float* dst = device_out ? out + per_img * ndef * (size_t)a0 : dout.p;
             for (int d = 0; d < ndef; d++) {
                 if ((rc = ms_run_cols(2, c, c->W.p, c->E.p, nb, fs, 0, c->F.p + plane * d, st))) return rc;
-                if ((rc = ms_run_lines(c, 3, nb, 0, ndef, d, dst, st))) return rc;
+                if ((rc = ms_run_lines(c, 3, nb, 0, ndef, d, dst, st, accum, div))) return rc;
             }
-            if (!device_out)
+            if (!device_out && w == nw - 1)
                 UTR_HIP_TRY(hipMemcpyAsync(out + per_img * ndef * (size_t)a0, dout.p,
                                            sizeof(float) * per_img * ndef * nb, hipMemcpyDeviceToHost, st));
         }
@@ -857,7 +900,7 @@ int ms_images(msct_ctx* c, int nang, const double* angZ, const double* angY, int
 extern "C" {
 
 int msct_create(const msct_params* p, const double* fe_params, int natom, const int* Z, const float* x,
-                const float* y, const float* z, const float* occ, msct_ctx** out) {
+                const float* y, const float* z, const float* occ, const float* wobble, msct_ctx** out) {
     if (!out) return ms_fail_invalid("null output handle");
     *out = nullptr;
     if (!p || !fe_params) return ms_fail_invalid("msct_create: null parameters or fe table");
@@ -874,6 +917,12 @@ int msct_create(const msct_params* p, const double* fe_params, int natom, const 
     if (p->output < 0 || p->output > 3)
         return ms_fail_invalid("msct: output must be 0 (intensity), 1 (phase), 2 (complex) or 3 (3D potential)");
     if (natom < 0 || (natom > 0 && (!Z || !x || !y || !z || !occ))) return ms_fail_invalid("msct: bad atom arrays");
+    if (p->nwobble < 0 || !std::isfinite(p->temperature) || p->temperature < 0)
+        return ms_fail_invalid("msct: nwobble must be >= 0 and the temperature finite and >= 0");
+    if (p->nwobble > 1 && (p->output == 1 || p->output == 2))
+        return ms_fail_invalid("msct: thermal configurations average intensities or potentials (output 0 or 3)");
+    for (int k = 0; wobble && k < natom; k++)
+        if (!std::isfinite(wobble[k])) return ms_fail_invalid("msct: thermal displacements must be finite");
     for (int k = 0; k < natom; k++) {
         if (Z[k] < 1 || Z[k] > 103) return ms_fail_invalid("msct: atomic numbers must be in 1..103");
         if (!std::isfinite(x[k]) || !std::isfinite(y[k]) || !std::isfinite(z[k]) || !std::isfinite(occ[k]))
@@ -965,6 +1014,8 @@ int msct_create(const msct_params* p, const double* fe_params, int natom, const 
     c->y.assign(y, y + natom);
     c->z.assign(z, z + natom);
     c->occ.assign(occ, occ + natom);
+    if (wobble) c->wob.assign(wobble, wobble + natom);
+    else c->wob.assign(natom, 0.f);
     auto up = [](auto& buf, const auto& v) -> hipError_t {
         hipError_t e = buf.alloc(v.size());
         if (e != hipSuccess) return e;
