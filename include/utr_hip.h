@@ -352,8 +352,9 @@ typedef struct msct_params {
     double astig_dz;        /* two-fold astigmatism amplitude, Angstrom            */
     double astig_angle_deg; /* and its angle, degrees (autosliccmd's astigm pair)  */
     double dxc, dyc, dzc;   /* rotation centre minus the cube centre, A (line 13)  */
-    int nwobble;            /* thermal configurations averaged (MsctKirkland lines
-                               12 and 14); 0 or 1 = none                           */
+    int nwobble;            /* thermal vibrations (MsctKirkland lines 12-14): 0 none,
+                               1 Debye-Waller smoothing of every slice's potential
+                               (trlayerDW), > 1 configurations averaged            */
     int reserved;
     double temperature;     /* K (line 13): displacements scale by sqrt(T / 300)   */
     unsigned long long seed; /* of the configurations' displacements             */

@@ -265,8 +265,13 @@ class Model:
         k2 = (self.ky2[:, None] + self.kx2[None, :]).astype(f32)
         self.keep = k2 < self.k2max
 
-    def trlayer(self, x, y, occ, Zn):
-        """Transmission function of one slice (autoslic.cpp:917-1013)."""
+    def trlayer(self, x, y, occ, Zn, dw=None):
+        """Transmission function of one slice (autoslic.cpp:917-1013).  dw:
+        (wobble, temperature) for trlayerDW (autoslic.cpp:1040-1104): the
+        potential is multiplied in Fourier space by exp(-B k^2 / 4), B = 8 pi^2
+        times the mean of wobble^2 T/300 over the first na entries of the
+        file-order wobble list (the reference passes the unsorted list
+        with the slice's atom count), before the phase is formed."""
         nx, ny, ax = self.nx, self.ny, float(f32(self.ctb))
         scale = sigma(self.kevf) / 1000.0
         sx, sy = float(f32(ax) / f32(nx)), float(f32(ax) / f32(ny))  # float steps, double radii
@@ -287,12 +292,20 @@ class Model:
             vz = float(occ[i]) * self.lut(int(Zn[i]), rsq)
             np.add.at(v, (np.broadcast_to((iy % ny)[:, None], m.shape)[m],
                           np.broadcast_to((ix % nx)[None, :], m.shape)[m]), vz[m])
+        if dw is not None:
+            wob, temp = dw
+            na = len(x)
+            w2 = (np.asarray(wob[:na], f32) * np.asarray(wob[:na], f32)).astype(f32).astype(np.float64)
+            bdw = float(np.sum(w2 * (temp / 300.0))) / na * 2.0 * (2.0 * math.pi) ** 2
+            k2 = (self.ky2[:, None] + self.kx2[None, :]).astype(f32).astype(np.float64)
+            emM = np.exp(-0.25 * bdw * k2).astype(f32).astype(np.float64)
+            v = np.real(cf_ifft(cf_fft(v.astype(np.complex128)) * emM))
         t = np.exp(1j * scale * v)
         T = cf_fft(t)
         T[~self.keep] = 0
         return cf_ifft(T)
 
-    def exit_wave(self, x, y, z, occ, Zn):
+    def exit_wave(self, x, y, z, occ, Zn, dw=None):
         """Coherent multislice of one orientation (autoslic.cpp:666-800):
         plane wave in, transmit slice by slice, propagate after each slice
         (multislice) or not at all (projection)."""
@@ -304,7 +317,7 @@ class Model:
         s = 0
         for na in slice_counts(z, self.deltaz, self.ctb):
             if na > 0:
-                wave = wave * self.trlayer(x[s:s + na], y[s:s + na], occ[s:s + na], Zn[s:s + na])
+                wave = wave * self.trlayer(x[s:s + na], y[s:s + na], occ[s:s + na], Zn[s:s + na], dw)
             if self.mode == 0:
                 W = cf_fft(wave)
                 W = W * self.propy[:, None] * self.propx[None, :]
@@ -437,13 +450,16 @@ def images(p, atoms, angZ, angY, defocus, output="intensity"):
             out[a] = acc / nw
             continue
         x, y, z = x0, y0, z0
+        # one thermal configuration: Debye-Waller smoothing of each slice
+        dw = (atoms.get("wobble", np.zeros(len(z))), p.get("temperature", 300.0)) \
+            if p.get("nwobble", 0) == 1 and output != "potential" else None
         if output == "potential":
             if len(z) and (min(x.min(), y.min(), z.min()) < 0 or max(x.max(), y.max(), z.max()) > f32(p["ctb"])):
                 raise ValueError("atoms outside the cube")
             for j, d in enumerate(defocus):
                 out[a, j] = potential_plane(p, x, y, z, atoms["Z"], f32(p["ctb"]) + f32(d))
             continue
-        w = m.exit_wave(x, y, z, atoms["occ"], atoms["Z"])
+        w = m.exit_wave(x, y, z, atoms["occ"], atoms["Z"], dw)
         for j, d in enumerate(defocus):
             R = d - 0.5 * p["ctb"]
             c = m.fresnel(w, R, aobj, p.get("cs3_mm", 0.0), p.get("cs5_mm", 0.0), p.get("astig_dz", 0.0),

@@ -213,5 +213,24 @@ def test_thermal_configurations(output):
     e = rel(got - 1, ref - 1) if output == 0 else rel(got, ref)
     print("thermal output %d rel-L2 %.2e" % (output, e), flush=True)
     assert e < (TOL_CONTRAST if output == 0 else 1e-5)
-    plain = U.multislice_images(dict(p, nwobble=1), at, [10.0, 80.0], [30.0, 140.0], defocus)
+    plain = U.multislice_images(dict(p, nwobble=0), at, [10.0, 80.0], [30.0, 140.0], defocus)
     assert rel(got, plain) > 1e-4  # the displacements change the images
+
+
+def test_debye_waller_single_configuration():
+    """nwobble = 1: each slice's potential filtered by exp(-B k^2 / 4)
+    before the phase (trlayerDW), B from the first na file-order wobbles."""
+    p = U.default_msct_params(nx=64, ny=64, ctblength=51.2, deltaz=1.0, kev=200.0, nwobble=1, temperature=300.0)
+    at = atoms_in_cube(13, 80, 51.2, 18.0)
+    at["wobble"] = np.linspace(0.05, 0.2, len(at["Z"])).astype(np.float32)
+    got = U.multislice_images(p, at, [10.0, 80.0], [30.0, 140.0], [50.0, 200.0])
+    ref = M.images(dict(oracle_params(p), nwobble=1, temperature=300.0), at, np.radians([10.0, 80.0]),
+                   np.radians([30.0, 140.0]), [50.0, 200.0])
+    e = rel(got - 1, ref - 1)
+    print("debye-waller contrast rel-L2 %.2e" % e, flush=True)
+    assert e < TOL_CONTRAST
+    plain = U.multislice_images(dict(p, nwobble=0), at, [10.0, 80.0], [30.0, 140.0], [50.0, 200.0])
+    assert np.std(got) < np.std(plain)  # the smoothing lowers the contrast
+    with pytest.raises(U.UTRError, match="not built"):
+        U.Multislice(dict(p, output=3), at)
+

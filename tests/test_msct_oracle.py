@@ -244,3 +244,17 @@ def test_thermal_average_without_displacement_is_the_single_image():
     one = M.images(P0, at, [0.2], [0.5], [80.0])
     avg = M.images(dict(P0, nwobble=3), dict(at, wobble=np.zeros(12, np.float32)), [0.2], [0.5], [80.0])
     assert np.allclose(avg, one, rtol=1e-12, atol=1e-12)
+
+
+def test_debye_waller_branch():
+    """trlayerDW: zero displacements change nothing; larger ones smooth the
+    potential, so the image contrast falls monotonically."""
+    at = layer(seed=5)
+    base = M.images(P0, at, [0.0], [0.0], [60.0])
+    c = []
+    for w in (0.0, 0.1, 0.3):
+        im = M.images(dict(P0, nwobble=1), dict(at, wobble=np.full(12, w, np.float32)), [0.0], [0.0], [60.0])
+        c.append(float(np.std(im)))
+        if w == 0.0:
+            assert np.allclose(im, base, rtol=1e-12, atol=1e-12)
+    assert c[0] > c[1] > c[2]

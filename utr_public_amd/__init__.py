@@ -556,7 +556,7 @@ def default_msct_params(**kw):
     no aperture or aberrations (the autosliccmd parameter lines)."""
     d = dict(nx=128, ny=128, mode=0, output=0, ctblength=51.2, deltaz=1.0, kev=200.0, aobj_mrad=0.0,
              cs3_mm=0.0, cs5_mm=0.0, astig_dz=0.0, astig_angle_deg=0.0, dxc=0.0, dyc=0.0, dzc=0.0,
-             nwobble=1, reserved=0, temperature=300.0, seed=1)
+             nwobble=0, reserved=0, temperature=300.0, seed=1)
     d.update(kw)
     return d
 
@@ -565,7 +565,7 @@ class Multislice:
     """One atom set for the multislice forward model (autosliccmd()):
     atoms = dict(Z, x, y, z[, occ][, wobble]) in Angstrom inside
     [0, ctblength]^3; wobble is the rms thermal displacement at 300 K, used
-    when params nwobble > 1."""
+    when params nwobble >= 1 (1: Debye-Waller factor, > 1: configurations)."""
 
     def __init__(self, params, atoms):
         self._L = load()

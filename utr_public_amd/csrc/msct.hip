@@ -152,7 +152,7 @@ __global__ void __launch_bounds__(256) k_ms_potential_rows(const MsAtom* __restr
                                                            MsGeom g, const double* __restrict__ knots,
                                                            const double4* __restrict__ coef, double scale,
                                                            cplx<float>* __restrict__ trans,
-                                                           const cplx<float>* __restrict__ tw) {
+                                                           const cplx<float>* __restrict__ tw, int raw) {
     constexpr int NB = lds_batch(N, sizeof(cplx<float>), LDS_POST);
     constexpr int PER = (NB * N + 255) / 256;
     __shared__ cplx<float> sm[2 * lds_len(NB * N)];
@@ -216,9 +216,13 @@ __global__ void __launch_bounds__(256) k_ms_potential_rows(const MsAtom* __restr
     for (int q = 0; q < PER; q++) {
         const int p = (int)threadIdx.x + q * 256;
         if (p < nb * N) {
-            float sn, cs;
-            sincosf((float)(scale * (double)acc[q]), &sn, &cs);
-            a[lx(p)] = {cs, sn};
+            if (raw) {  // the potential itself, for the Debye-Waller filter
+                a[lx(p)] = {acc[q], 0.f};
+            } else {
+                float sn, cs;
+                sincosf((float)(scale * (double)acc[q]), &sn, &cs);
+                a[lx(p)] = {cs, sn};
+            }
         }
     }
     __syncthreads();
@@ -291,14 +295,15 @@ __global__ void __launch_bounds__(256) k_ms_cols(const cplx<float>* __restrict__
                                                  float k2max, const cplx<float>* __restrict__ px,
                                                  const cplx<float>* __restrict__ py,
                                                  const cplx<float>* __restrict__ F, float scale,
-                                                 const int* __restrict__ off, int nsl1, int s) {
+                                                 const int* __restrict__ off, int nsl1, int s,
+                                                 const float* __restrict__ dwm) {
     constexpr int NB = lds_batch(N, sizeof(cplx<float>), MS_LDS);
     __shared__ cplx<float> sm[2 * lds_len(NB * N)];
     cplx<float>* a = sm;
     cplx<float>* b = sm + lds_len(NB * N);
     int tx, o;
     if (!xcd_tile(gx, gy, tx, o)) return;
-    if (MODE == 0 && off[o * nsl1 + s] == off[o * nsl1 + s + 1]) return;
+    if ((MODE == 0 || MODE == 3) && off[o * nsl1 + s] == off[o * nsl1 + s + 1]) return;
     const int c0 = tx * NB;
     const int nb = min(NB, nx - c0);
     const size_t base = (size_t)o * N * nx + c0;
@@ -312,7 +317,11 @@ __global__ void __launch_bounds__(256) k_ms_cols(const cplx<float>* __restrict__
         const int c = t / N, y = t - c * N;
         const int kx = c0 + c;
         cplx<float> v = a[lx(t)];
-        if (MODE == 2) {
+        if (MODE == 3) {
+            // exp(-M) of trlayerDW (autoslic.cpp:1080-1088): every frequency
+            const float e = (float)exp((double)dwm[o * nsl1 + s] * (double)(ky2[y] + kx2[kx]));
+            v = {v.x * e * scale, v.y * e * scale};
+        } else if (MODE == 2) {
             v = cmul(v, F[(size_t)y * nx + kx]);
             v = {v.x * scale, v.y * scale};
         } else if (kx2[kx] + ky2[y] < k2max) {
@@ -373,6 +382,32 @@ __global__ void __launch_bounds__(256) k_ms_mul(cplx<float>* __restrict__ W, con
     __syncthreads();
     fft_lds<float, N>(a, b, nb, tw, +1, threadIdx.x, blockDim.x);
     for (int t = threadIdx.x; t < nb * N; t += blockDim.x) W[base + t] = a[lx(t)];
+}
+
+// Debye-Waller branch, after the filter: lines of T (the filtered potential,
+// x-transformed and normalised): inverse x FFT, the transmission
+// exp(i sigma Re V) (autoslic.cpp:1091-1097), x FFT back into T.
+template <int N>
+__global__ void __launch_bounds__(256) k_ms_dwexp(cplx<float>* __restrict__ T, int nlines, double scale,
+                                                  const cplx<float>* __restrict__ tw) {
+    constexpr int NB = lds_batch(N, sizeof(cplx<float>), LDS_POST);
+    __shared__ cplx<float> sm[2 * lds_len(NB * N)];
+    cplx<float>* a = sm;
+    cplx<float>* b = sm + lds_len(NB * N);
+    const int l0 = blockIdx.x * NB;
+    const int nb = min(NB, nlines - l0);
+    cplx<float>* base = T + (size_t)l0 * N;
+    for (int t = threadIdx.x; t < nb * N; t += blockDim.x) a[lx(t)] = base[t];
+    __syncthreads();
+    fft_lds<float, N>(a, b, nb, tw, -1, threadIdx.x, blockDim.x);
+    for (int t = threadIdx.x; t < nb * N; t += blockDim.x) {
+        float sn, cs;
+        sincosf((float)(scale * (double)a[lx(t)].x), &sn, &cs);
+        a[lx(t)] = {cs, sn};
+    }
+    __syncthreads();
+    fft_lds<float, N>(a, b, nb, tw, +1, threadIdx.x, blockDim.x);
+    for (int t = threadIdx.x; t < nb * N; t += blockDim.x) base[t] = a[lx(t)];
 }
 
 // Lines of E [B][ny][nx] (x-transformed, normalised): inverse x FFT, then
@@ -526,6 +561,9 @@ struct msct_ctx {
     int natom = 0;
     std::vector<int> zi;  // element row per atom
     std::vector<float> x, y, z, occ, wob;  // wob: rms thermal displacement at 300 K
+    std::vector<double> dwp;               // prefix sums of wob^2 T/300 (file order)
+    DevBuf<float> dwm;                     // -B/4 of trlayerDW per (orientation, slice)
+    std::vector<float> dwm_h;
     int nsl = 0;          // slices per orientation
     float ctbf = 0.f, k2max = 0.f, wavlen = 0.f, aobj = 0.f;
     utr::MsGeom g{};
@@ -598,15 +636,19 @@ int ms_run_cols(int mode, msct_ctx* c, const cplx<float>* src, cplx<float>* dst,
         if (mode == 0)                                                                              \
             utr::k_ms_cols<N, 0><<<grid, 256, 0, st>>>(src, dst, nx, gx, B, c->twy.p, c->kx2.p,     \
                                                        c->ky2.p, c->k2max, c->px.p, c->py.p,        \
-                                                       F, scale, c->off.p, c->g.nsl1, s);      \
+                                                       F, scale, c->off.p, c->g.nsl1, s, c->dwm.p); \
         else if (mode == 1)                                                                         \
             utr::k_ms_cols<N, 1><<<grid, 256, 0, st>>>(src, dst, nx, gx, B, c->twy.p, c->kx2.p,     \
                                                        c->ky2.p, c->k2max, c->px.p, c->py.p,        \
-                                                       F, scale, c->off.p, c->g.nsl1, s);      \
-        else                                                                                        \
+                                                       F, scale, c->off.p, c->g.nsl1, s, c->dwm.p); \
+        else if (mode == 2)                                                                         \
             utr::k_ms_cols<N, 2><<<grid, 256, 0, st>>>(src, dst, nx, gx, B, c->twy.p, c->kx2.p,     \
                                                        c->ky2.p, c->k2max, c->px.p, c->py.p,        \
-                                                       F, scale, c->off.p, c->g.nsl1, s);      \
+                                                       F, scale, c->off.p, c->g.nsl1, s, c->dwm.p); \
+        else                                                                                        \
+            utr::k_ms_cols<N, 3><<<grid, 256, 0, st>>>(src, dst, nx, gx, B, c->twy.p, c->kx2.p,     \
+                                                       c->ky2.p, c->k2max, c->px.p, c->py.p,        \
+                                                       F, scale, c->off.p, c->g.nsl1, s, c->dwm.p); \
         ok = true;                                                                                  \
     }
     UTR_DISPATCH_N(ny, MS_COLS)
@@ -638,7 +680,10 @@ int ms_run_lines(msct_ctx* c, int what, int B, int s, int ndef, int d, float* ou
         } else if (what == 4) {                                                                     \
             utr::k_ms_potential_rows<N><<<dim3((ny + NB - 1) / NB, B), 256, 0, st>>>(                  \
                 c->atoms.p, c->off.p, c->natom, s, c->g, c->knots.p, c->coef.p, c->scale, c->T.p,    \
-                c->twx.p);                                                                          \
+                c->twx.p, c->p.nwobble == 1 ? 1 : 0);                                               \
+        } else if (what == 5) {                                                                     \
+            utr::k_ms_dwexp<N><<<(nlines + NB - 1) / NB, 256, 0, st>>>(c->T.p, nlines, c->scale,       \
+                                                                      c->twx.p);                    \
         } else if (what == 2) {                                                                     \
             utr::k_post_lines<float, N><<<(nlines + NB - 1) / NB, 256, 0, st>>>(c->W.p, nlines, +1, \
                                                                                c->twx.p);           \
@@ -678,7 +723,7 @@ double ms_normal(uint64_t seed, int a, int w, int k, int axis) {
     return std::sqrt(-2.0 * std::log(u1)) * std::cos(2.0 * kMsPi * u2);
 }
 
-bool ms_prepare_one(const msct_ctx* c, int a, int w, double aZ, double aY, utr::MsAtom* dst, int* o,
+bool ms_prepare_one(const msct_ctx* c, int a, int w, double aZ, double aY, utr::MsAtom* dst, int* o, float* dwm,
                     std::vector<float>& X, std::vector<float>& Y, std::vector<float>& Z, std::vector<int>& idx) {
     const int n = c->natom;
     X.resize(n);
@@ -739,16 +784,26 @@ bool ms_prepare_one(const msct_ctx* c, int a, int w, double aZ, double aY, utr::
         zslice += dz;
     }
     o[c->nsl] = istart;
+    if (dwm) {
+        // trlayerDW (autoslic.cpp:1047-1056, 1077): B = 8 pi^2 times the mean of
+        // wobble^2 T/300 over the first na entries of the file-order list
+        const double tp2 = 2.0 * (2.0 * kMsPi) * (2.0 * kMsPi);
+        for (int s = 0; s < c->nsl; s++) {
+            const int na = o[s + 1] - o[s];
+            dwm[s] = na > 0 ? (float)(-0.25 * (c->dwp[na] / na) * tp2) : 0.f;
+        }
+    }
     return true;
 }
 
 // orientations [a0, a0 + B) into A (B x natom) and off (B x (nsl + 1)), on
 // up to 16 host threads
 int ms_prepare(const msct_ctx* c, int a0, int B, int w, const double* angZ, const double* angY,
-               std::vector<utr::MsAtom>& A, std::vector<int>& off) {
+               std::vector<utr::MsAtom>& A, std::vector<int>& off, std::vector<float>& dwm) {
     const int n = c->natom, nsl1 = c->nsl + 1;
     A.resize((size_t)B * n);
     off.assign((size_t)B * nsl1, 0);
+    dwm.assign(c->p.nwobble == 1 ? (size_t)B * nsl1 : 0, 0.f);
     const int nt = std::max(1, std::min<int>({B, 16, (int)std::max(1u, std::thread::hardware_concurrency())}));
     std::vector<int> bad(nt, -1);
     auto work = [&](int t) {
@@ -756,7 +811,8 @@ int ms_prepare(const msct_ctx* c, int a0, int B, int w, const double* angZ, cons
         std::vector<int> idx;
         for (int b = t; b < B; b += nt)
             if (!ms_prepare_one(c, a0 + b, w, angZ[a0 + b], angY[a0 + b], A.data() + (size_t)b * n,
-                                off.data() + (size_t)b * nsl1, X, Y, Z, idx)) {
+                                off.data() + (size_t)b * nsl1,
+                                c->p.nwobble == 1 ? dwm.data() + (size_t)b * nsl1 : nullptr, X, Y, Z, idx)) {
                 bad[t] = b;
                 return;
             }
@@ -792,6 +848,7 @@ int ms_images(msct_ctx* c, int nang, const double* angZ, const double* angY, int
     UTR_HIP_TRY(c->E.alloc(plane * B));
     UTR_HIP_TRY(c->atoms.alloc((size_t)B * std::max(c->natom, 1)));
     UTR_HIP_TRY(c->off.alloc((size_t)B * (c->nsl + 1)));
+    UTR_HIP_TRY(c->dwm.alloc((size_t)B * (c->nsl + 1)));
     UTR_HIP_TRY(c->F.alloc(plane * ndef));
     UTR_HIP_TRY(c->zp.alloc(ndef));
     LocalBuf<float> dout;
@@ -829,13 +886,16 @@ int ms_images(msct_ctx* c, int nang, const double* angZ, const double* angY, int
             const int accum = w > 0 ? 1 : 0;
             const float div = w == nw - 1 ? (float)nw : 1.f;
             const int nb = std::min(B, nang - a0);
-            int rc = ms_prepare(c, a0, nb, w, angZ, angY, A, off);
+            int rc = ms_prepare(c, a0, nb, w, angZ, angY, A, off, c->dwm_h);
             if (rc) return rc;
             UTR_HIP_TRY(hipStreamSynchronize(st));  // the previous batch no longer reads atoms/off
             if (c->natom > 0)
                 UTR_HIP_TRY(hipMemcpyAsync(c->atoms.p, A.data(), sizeof(utr::MsAtom) * A.size(),
                                            hipMemcpyHostToDevice, st));
             UTR_HIP_TRY(hipMemcpyAsync(c->off.p, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice, st));
+            if (!c->dwm_h.empty())
+                UTR_HIP_TRY(hipMemcpyAsync(c->dwm.p, c->dwm_h.data(), sizeof(float) * c->dwm_h.size(),
+                                           hipMemcpyHostToDevice, st));
             if (c->p.output == 3) {
                 // 3D potential on the plane z = zmax + defocus (autosliccmd.cpp:1034-1038)
                 float* dst = device_out ? out + plane * ndef * (size_t)a0 : dout.p;
@@ -857,7 +917,12 @@ int ms_images(msct_ctx* c, int nang, const double* angZ, const double* angY, int
             UTR_HIP_TRY(hipGetLastError());
             for (int s = 0; s < c->nsl; s++) {
                 if (c->natom > 0) {
-                    if (fused) {
+                    if (c->p.nwobble == 1) {
+                        // Debye-Waller: raw potential, its filter, then the phase
+                        if ((rc = ms_run_lines(c, 4, nb, s, 0, 0, nullptr, st))) return rc;
+                        if ((rc = ms_run_cols(3, c, c->T.p, c->T.p, nb, inv, s, nullptr, st))) return rc;
+                        if ((rc = ms_run_lines(c, 5, nb, s, 0, 0, nullptr, st))) return rc;
+                    } else if (fused) {
                         if ((rc = ms_run_lines(c, 4, nb, s, 0, 0, nullptr, st))) return rc;
                     } else {
                         utr::k_ms_potential<<<dim3(tiles, nb), 256, 0, st>>>(c->atoms.p, c->off.p, c->natom, s,
@@ -919,6 +984,8 @@ int msct_create(const msct_params* p, const double* fe_params, int natom, const 
     if (natom < 0 || (natom > 0 && (!Z || !x || !y || !z || !occ))) return ms_fail_invalid("msct: bad atom arrays");
     if (p->nwobble < 0 || !std::isfinite(p->temperature) || p->temperature < 0)
         return ms_fail_invalid("msct: nwobble must be >= 0 and the temperature finite and >= 0");
+    if (p->nwobble == 1 && p->output == 3)
+        return set_err(UTR_E_UNSUPPORTED, "msct: the Debye-Waller factor of the potential output is not built");
     if (p->nwobble > 1 && (p->output == 1 || p->output == 2))
         return ms_fail_invalid("msct: thermal configurations average intensities or potentials (output 0 or 3)");
     for (int k = 0; wobble && k < natom; k++)
@@ -1016,6 +1083,9 @@ int msct_create(const msct_params* p, const double* fe_params, int natom, const 
     c->occ.assign(occ, occ + natom);
     if (wobble) c->wob.assign(wobble, wobble + natom);
     else c->wob.assign(natom, 0.f);
+    c->dwp.assign(natom + 1, 0.0);
+    for (int k = 0; k < natom; k++)
+        c->dwp[k + 1] = c->dwp[k] + (double)(c->wob[k] * c->wob[k]) * (p->temperature / 300.0);
     auto up = [](auto& buf, const auto& v) -> hipError_t {
         hipError_t e = buf.alloc(v.size());
         if (e != hipSuccess) return e;
@@ -1059,7 +1129,7 @@ int msct_destroy(msct_ctx* c) {
     if (!c) return UTR_OK;
     c->knots.release(); c->fe_el.release(); c->coef.release(); c->kx2.release(); c->ky2.release(); c->zp.release();
     c->px.release(); c->py.release(); c->twx.release(); c->twy.release(); c->F.release();
-    c->atoms.release(); c->off.release(); c->W.release(); c->T.release(); c->E.release();
+    c->atoms.release(); c->off.release(); c->dwm.release(); c->W.release(); c->T.release(); c->E.release();
     delete c;
     return UTR_OK;
 }
