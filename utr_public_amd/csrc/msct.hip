@@ -171,11 +171,16 @@ __global__ void __launch_bounds__(256) k_ms_potential_rows(const MsAtom* __restr
     for (int c0 = a0; c0 < a1; c0 += 256) {
         const int n = min(256, a1 - c0);
         bool hit = false;
-        MsAtom at{};
+        // the record travels as two 16-byte vectors (a local MsAtom
+        // aggregate was given a scratch slot by the compiler)
+        float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0;
         if ((int)threadIdx.x < n) {
-            at = A[c0 + threadIdx.x];
+            const float4* src = reinterpret_cast<const float4*>(A + c0 + threadIdx.x);
+            r0 = src[0];
+            r1 = src[1];
             // rows y0 .. y0+nb-1 meet some image of the atom's y window
-            const int dy = ms_pmod(y0 - (at.iyo - g.idy), g.ny);
+            const int iyo = __float_as_int(r1.y);
+            const int dy = ms_pmod(y0 - (iyo - g.idy), g.ny);
             hit = dy <= 2 * g.idy || g.ny - dy <= nb - 1;
         }
         const unsigned long long m = __ballot(hit);
@@ -186,7 +191,11 @@ __global__ void __launch_bounds__(256) k_ms_potential_rows(const MsAtom* __restr
         int base = 0;
         for (int k = 0; k < wv; k++) base += wcount[k];
         const int total = wcount[0] + wcount[1] + wcount[2] + wcount[3];
-        if (hit) sa[base + pre] = at;
+        if (hit) {
+            float4* dstv = reinterpret_cast<float4*>(sa + base + pre);
+            dstv[0] = r0;
+            dstv[1] = r1;
+        }
         __syncthreads();
         for (int k = 0; k < total; k++) {
             const MsAtom t = sa[k];
