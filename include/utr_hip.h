@@ -331,9 +331,13 @@ int dht_destroy(dht_ctx* ctx);
 /* ---------------------------------------------------------------------------
  * TEM multislice forward model: the image simulator of TemsimLibrary
  * (autosliccmd(), autosliccmd.cpp:177-1300, with autoslic::calculate,
- * autoslic.cpp:209-869), coherent branch only: plane incident wave, one
- * atom configuration (no thermal wobble), no ice, no crystal tilt, Z'' = 0
- * and no XY shifts of the images.  One context holds one atom set; each call
+ * autoslic.cpp:209-869), coherent branch: plane incident wave, no ice, no
+ * crystal tilt, Z'' = 0 and no XY shifts of the images.  Thermal vibration
+ * follows msct_params.nwobble: 0 one static configuration, 1 the Debye-Waller
+ * smoothing of every slice's potential (trlayerDW, autoslic.cpp:1040-1104;
+ * its 2D variant for output 3 is not built), > 1 the average over nwobble
+ * displaced configurations (autoslic.cpp:700-711; the displacements come
+ * from a seeded generator).  One context holds one atom set; each call
  * simulates any number of orientations and defocus distances.
  * ------------------------------------------------------------------------- */
 typedef struct msct_params {
@@ -385,6 +389,12 @@ int msct_images_device(msct_ctx* ctx, int nang, const double* angZ_rad, const do
 /* Slices per orientation: k * deltaz for k = 1, 2, ... while below the cube
  * side + deltaz (autoslic.cpp:727-744). */
 int msct_slices(const msct_ctx* ctx);
+/* Orientations simulated together per batch: as many as fit the
+ * UTR_MSCT_BATCH_BYTES budget (default 6 GiB) of device memory held per
+ * orientation (three complex planes, the rotated atom records and the slice
+ * table).  No counterpart in the reference (it runs one orientation at a
+ * time per thread, MultisliceT.cpp:398-450). */
+int msct_batch(const msct_ctx* ctx);
 int msct_destroy(msct_ctx* ctx);
 
 #ifdef __cplusplus

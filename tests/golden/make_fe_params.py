@@ -14,7 +14,11 @@ import sys
 SRC = "/root/reference/TemsimLibrary/slicelib.cpp"
 
 
-def main(src=SRC, out=os.path.join(os.path.dirname(os.path.abspath(__file__)), "fe_params.txt")):
+OUT = os.path.join(os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)))),
+                   "utr_public_amd", "data", "fe_params.txt")
+
+
+def main(src=SRC, out=OUT):
     text = open(src).read()
     vals = {}
     for z, i, v in re.findall(r"fparams\[(\d+)\]\[(\d+)\]\s*=\s*([-+0-9.eE]+)\s*;", text):

@@ -7,6 +7,8 @@ orientation / subseries files, FileNames numbering (ReadmeUTR.txt:241-255,
 import ctypes
 import json
 import os
+import re
+import shutil
 import struct
 import subprocess
 
@@ -372,3 +374,112 @@ def test_reprocess_rejects_other_extensions(work):
     write_params(work / "UTR.txt", override={33: "1"}, out="out/vol.png")
     rc, out = run_cli(work, "UTR.txt")
     assert rc == 1
+
+
+@pytest.mark.parametrize("nx,ny,header,element,msg", [
+    (4, 4, 0, 2, "ElementLength"),
+    (4, 4, -1, 4, "header"),
+    (0, 4, 0, 4, "sizes"),
+    (4, -2, 0, 8, "sizes"),
+])
+def test_read_raw_rejects_bad_layout(tmp_path, io, nx, ny, header, element, msg):
+    """The C ABI RAW reader checks the layout itself (element 4 or 8 bytes,
+    header >= 0, positive sizes) instead of reading past the file's bytes."""
+    p = tmp_path / "img.raw"
+    p.write_bytes(np.zeros(64, np.float32).tobytes())
+    w, h = ctypes.c_long(), ctypes.c_long()
+    rc = io.utr_io_read_image(str(p).encode(), nx, ny, header, element, 0, None, 0, ctypes.byref(w), ctypes.byref(h))
+    assert rc != 0
+    assert msg in io.utr_io_last_error().decode()
+
+
+# ---------------------------------------------------------------------------
+# The reference's own parameter files (PhaseRetrieval/UTR.txt, UTR.xri and
+# the orientation table DefocusRand108_2_NEW2.txt, copied unchanged under
+# tests/golden/reference_params/), with their Windows paths rewritten to a
+# scratch folder.  Expected outcomes are those UTR.cpp itself produces on
+# them.
+
+REFP = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_params")
+
+
+def rewrite_params(src, dst, repl):
+    """Copy a parameter file, replacing the value part (everything after the
+    first whitespace-delimited title token, as UTR.cpp's sscanf reads it) of
+    the numbered lines in repl {number: text}."""
+    out = []
+    for line in open(os.path.join(REFP, src)).read().splitlines():
+        m = re.match(r"^(\d+)\*?\.\S+", line)
+        if m and int(m.group(1)) in repl:
+            line = m.group(0) + " " + repl[int(m.group(1))]
+        out.append(line)
+    with open(dst, "w") as f:
+        f.write("\n".join(out) + "\n")
+
+
+def one_defocus_table(dst, rows):
+    with open(dst, "w") as f:
+        f.write("//Z_rotation\tY'_rotation\tZ''_rotation\tDefocus\n")
+        for r in rows:
+            f.write("%.10g\t%.10g\t%.10g\t%.10g\n" % tuple(r))
+
+
+def test_reference_utr_txt_with_two_defocus_table_is_rejected(tmp_path):
+    """UTR.txt with the shipped two-defocus table: UTR.cpp:223-224 stops."""
+    shutil.copy(os.path.join(REFP, "DefocusRand108_2_NEW2.txt"), tmp_path)
+    rewrite_params("UTR.txt", tmp_path / "UTR.txt",
+                   {3: "DefocusRand108_2_NEW2.txt", 5: str(tmp_path / "au.grd"),
+                    26: str(tmp_path / "out.grd"), 34: str(tmp_path)})
+    rc, out = run_cli(tmp_path, "UTR.txt", "--check")
+    assert rc != 0
+    assert ("implemented only for defocus parameter files with one defocus distance per orientation"
+            in out), out
+
+
+def test_reference_utr_txt_tie_hom_in_tem_mode_is_rejected(tmp_path):
+    """UTR.txt as shipped asks for TIE-Hom-2D (parameter 15 = 1) with
+    beta/delta = 1e-4 in TEM mode: UTR.cpp:455-456 stops."""
+    t = np.loadtxt(os.path.join(REFP, "DefocusRand108_2_NEW2.txt"), comments="//")
+    assert t.shape == (108, 6)
+    one_defocus_table(tmp_path / "rows.txt", t[:, :4])
+    rewrite_params("UTR.txt", tmp_path / "UTR.txt",
+                   {3: "rows.txt", 5: str(tmp_path / "au.grd"), 26: str(tmp_path / "out.grd"),
+                    34: str(tmp_path)})
+    rc, out = run_cli(tmp_path, "UTR.txt", "--check")
+    assert rc != 0
+    assert "beta/delta must be negative in TIE-Hom phase retrieval" in out, out
+
+
+def test_reference_utr_txt_geometry_with_ctf3d(tmp_path):
+    """UTR.txt with parameter 15 switched to CTF-3D (4) and a TIFF output
+    name (its 16-bit output settings cannot go to GRD files, UTR.cpp:652)
+    passes validation; the derived geometry follows UTR.cpp:284-330 for its
+    2048 x 1024 images at 0.457 A and the +-116 A defocus range."""
+    t = np.loadtxt(os.path.join(REFP, "DefocusRand108_2_NEW2.txt"), comments="//")
+    rows = t[:, :4].copy()
+    rows[:, 3] = 10.0
+    one_defocus_table(tmp_path / "rows.txt", rows)
+    rewrite_params("UTR.txt", tmp_path / "UTR.txt",
+                   {3: "rows.txt", 5: str(tmp_path / "au.grd"), 15: "4", 26: str(tmp_path / "out.tif"),
+                    34: str(tmp_path)})
+    j = check_json(tmp_path, "UTR.txt")
+    assert (j["nx"], j["ny"]) == (2048, 1024)
+    assert j["nz"] == int(232.0 / 0.457 + 0.5)
+    assert j["rows"] == 108 and j["ctf_mode"] == 4 and j["modality"] == 0
+    assert j["angZ0"] == rows[0, 0] and j["angY0"] == rows[0, 1] and j["defocus0"] == 10.0
+    assert j["threads"] == 24 and j["aobj_mrad"] == 40.0 and j["nsr"] == 0.03
+
+
+def test_reference_utr_xri(tmp_path):
+    """UTR.xri (27 lines, X-ray modality): TIE-Hom-3D with delta/beta = 275
+    passes; zlo/zhi span the image's x extent and asigma = -1/275
+    (UTR.cpp:308-310, 446-451)."""
+    one_defocus_table(tmp_path / "rows.txt", [(0.0, k * 0.375, 0.0, 0.0) for k in range(480)])
+    rewrite_params("UTR.xri", tmp_path / "UTR.xri",
+                   {2: "rows.txt", 4: str(tmp_path / "p.grd"), 18: str(tmp_path / "out.tif"),
+                    26: str(tmp_path)})
+    j = check_json(tmp_path, "UTR.xri")
+    assert j["modality"] == 1 and (j["nx"], j["ny"]) == (1256, 624)
+    assert j["xst"] == 100.0 and j["zlo"] == -62800.0 and j["zhi"] == 62800.0 and j["nz"] == 1256
+    assert j["asigma"] == -1.0 / 275.0 and j["ctf_mode"] == 2 and j["rows"] == 480
+    assert j["dxc"] == 400.0 and j["epsilon"] == 1e-6

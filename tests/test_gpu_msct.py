@@ -91,9 +91,14 @@ def test_batches_and_determinism():
     at = atoms_in_cube(5, 80, 51.2, 20.0)
     angZ, angY = np.linspace(0, 300, 7), np.linspace(0, 180, 7)
     a = U.multislice_images(p, at, angZ, angY, [0.0, 200.0])
-    os.environ["UTR_MSCT_BATCH_BYTES"] = str(2 * 3 * 64 * 64 * 8)  # two orientations per batch
+    # two orientations per batch: three complex planes, 80 atom records and
+    # the slice table each
+    os.environ["UTR_MSCT_BATCH_BYTES"] = str(2 * (3 * 64 * 64 * 8 + 80 * 32 + 60 * 8))
     try:
-        b = U.multislice_images(p, at, angZ, angY, [0.0, 200.0])
+        m = U.Multislice(p, at)
+        assert m.batch == 2
+        b = m.images(angZ, angY, [0.0, 200.0])
+        m.close()
     finally:
         del os.environ["UTR_MSCT_BATCH_BYTES"]
     assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
@@ -234,3 +239,70 @@ def test_debye_waller_single_configuration():
     with pytest.raises(U.UTRError, match="not built"):
         U.Multislice(dict(p, output=3), at)
 
+
+
+@pytest.mark.parametrize("output", [0, 3])
+def test_thermal_device_output_over_several_batches(output, monkeypatch):
+    """Averaged thermal configurations written to a device tensor, with a
+    batch budget small enough for three batches (the device staging buffer
+    reused and each batch's accumulation reading the caller's tensor), equal
+    bit for bit to the host-output run in one batch."""
+    import torch
+    p = U.default_msct_params(nx=64, ny=64, ctblength=51.2, deltaz=1.0, kev=200.0, nwobble=3, temperature=250.0,
+                              seed=5, output=output)
+    at = atoms_in_cube(21, 50, 51.2, 18.0)
+    at["wobble"] = np.full(len(at["Z"]), 0.1, np.float32)
+    angZ, angY = [0.0, 10.0, 20.0, 30.0, 40.0], [5.0, 50.0, 95.0, 140.0, 185.0]
+    defocus = [0.0, 120.0] if output == 0 else [-25.6, -10.0]
+    host = U.multislice_images(p, at, angZ, angY, defocus)
+    per = 3 * 64 * 64 * 8 + len(at["Z"]) * 32
+    monkeypatch.setenv("UTR_MSCT_BATCH_BYTES", str(2 * per + 100))
+    m = U.Multislice(p, at)
+    try:
+        assert m.batch == 2
+        out = torch.empty((5, 2, 64, 64), dtype=torch.float32, device="cuda")
+        m.images(angZ, angY, defocus, out=out)
+        torch.cuda.synchronize()
+        dev = out.cpu().numpy()
+    finally:
+        m.close()
+    print("thermal device output %d: max |diff| %.3e" % (output, np.abs(dev - host).max()), flush=True)
+    assert np.array_equal(dev, host)
+
+
+def test_batch_counts_atom_records(monkeypatch):
+    """The per-orientation batch cost includes the rotated atom records and
+    the slice table, so many atoms shrink the batch."""
+    nx = ny = 64
+    rng = np.random.default_rng(3)
+    n = 20000
+    at = dict(Z=np.full(n, 6, np.int32), x=rng.uniform(10, 40, n).astype(np.float32),
+              y=rng.uniform(10, 40, n).astype(np.float32), z=rng.uniform(10, 40, n).astype(np.float32),
+              occ=np.ones(n, np.float32))
+    budget = 64 << 20
+    monkeypatch.setenv("UTR_MSCT_BATCH_BYTES", str(budget))
+    p = U.default_msct_params(nx=nx, ny=ny, ctblength=51.2, deltaz=1.0, kev=200.0)
+    m = U.Multislice(p, at)
+    try:
+        per = 3 * nx * ny * 8 + n * 32 + (m.slices + 1) * 8
+        print("batch", m.batch, "expected", budget // per, flush=True)
+        assert m.batch == budget // per
+        assert m.batch < budget // (3 * nx * ny * 8)
+    finally:
+        m.close()
+
+
+def test_potential_output_keeps_displaced_atoms_outside_the_depth_range():
+    """Thermal displacements that carry atoms out of [0, ctblength] in depth
+    stop the multislice (autoslic.cpp:733-738) but not the potential output,
+    which applies them unchecked (autosliccmd.cpp:1001-1009)."""
+    n = 20
+    at = dict(Z=np.full(n, 6, np.int32), x=np.linspace(20, 30, n).astype(np.float32),
+              y=np.full(n, 25.6, np.float32), z=np.full(n, 0.01, np.float32), occ=np.ones(n, np.float32),
+              wobble=np.full(n, 1.0, np.float32))
+    p = U.default_msct_params(nx=64, ny=64, ctblength=51.2, deltaz=1.0, kev=200.0, nwobble=3, temperature=300.0,
+                              seed=2)
+    with pytest.raises(U.UTRError):
+        U.multislice_images(dict(p, output=0), at, [0.0], [0.0], [0.0])
+    pot = U.multislice_images(dict(p, output=3), at, [0.0], [0.0], [-51.0])
+    assert np.isfinite(pot).all() and pot.max() > 0

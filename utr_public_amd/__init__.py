@@ -42,7 +42,7 @@ EXPORTS = [
     "dht_reset", "dht_dims", "dht_get_timings", "dht_destroy", "dht_set_image_correction",
     "utr_highpass3d", "utr_highpass3d_device", "utr_rotate3d", "utr_rotate3d_device",
     "utr_symmetrize3d", "utr_symmetrize3d_device", "utr_invlaplace3d", "utr_invlaplace3d_device",
-    "msct_create", "msct_images", "msct_images_device", "msct_slices", "msct_destroy",
+    "msct_create", "msct_images", "msct_images_device", "msct_slices", "msct_batch", "msct_destroy",
 ]
 
 DHT_FIELDS = [
@@ -202,6 +202,7 @@ def load():
     L.msct_images.argtypes = [vp, i, dp, dp, i, dp, vp]
     L.msct_images_device.argtypes = [vp, i, dp, dp, i, dp, vp]
     L.msct_slices.argtypes = [vp]
+    L.msct_batch.argtypes = [vp]
     L.msct_destroy.argtypes = [vp]
     _lib = L
     return L
@@ -591,6 +592,11 @@ class Multislice:
     @property
     def slices(self):
         return self._L.msct_slices(self._ctx)
+
+    @property
+    def batch(self):
+        """Orientations simulated together (the UTR_MSCT_BATCH_BYTES budget)."""
+        return self._L.msct_batch(self._ctx)
 
     def images(self, angZ_deg, angY_deg, defocus, out=None):
         """[orientation][defocus][ny][nx] float32 images: intensity (output 0),

@@ -1,6 +1,8 @@
 // msct.hip -- TEM multislice forward model on gfx950: the coherent branch of
-// TemsimLibrary/autoslic.cpp (autoslic::calculate, lpartl = 0, no thermal
-// configurations), its projected atomic potentials (slicelib.cpp vzatom /
+// TemsimLibrary/autoslic.cpp (autoslic::calculate, lpartl = 0; nwobble 0 = one
+// static configuration, 1 = Debye-Waller smoothing of each slice's potential
+// (trlayerDW), > 1 = averaged thermal configurations), its projected atomic
+// potentials (slicelib.cpp vzatom /
 // vzatomLUT), the per-orientation atom rotation and the free-space
 // propagation of the exit wave to each defocus distance (autosliccmd.cpp),
 // behind the msct_* C ABI of include/utr_hip.h.  Included by utr_api.hip
@@ -764,9 +766,10 @@ bool ms_prepare_one(const msct_ctx* c, int a, int w, double aZ, double aY, utr::
             X[k] = X[k] + (float)(wb * ms_normal(c->p.seed, a, w, k, 0) * sc);
             Y[k] = Y[k] + (float)(wb * ms_normal(c->p.seed, a, w, k, 1) * sc);
             Z[k] = Z[k] + (float)(wb * ms_normal(c->p.seed, a, w, k, 2) * sc);
-            // the depth range is checked again after the displacement
-            // (autoslic.cpp:733-738)
-            if (!(Z[k] >= 0.f && Z[k] <= ctb)) return false;
+            // the multislice checks the depth range again after the
+            // displacement (autoslic.cpp:733-738); the potential output
+            // applies it unchecked (autosliccmd.cpp:1001-1009)
+            if (c->p.output != 3 && !(Z[k] >= 0.f && Z[k] <= ctb)) return false;
         }
     }
     // the multislice takes the atoms in depth order (sortByZ, autoslic.cpp:720);
@@ -837,6 +840,19 @@ int ms_prepare(const msct_ctx* c, int a0, int B, int w, const double* angZ, cons
         return set_err(UTR_E_INVALID, "msct: an atom leaves the cube [0, ctblength]^3 at orientation " +
                                           std::to_string(a0 + first) + " (the reference stops with the same error)");
     return UTR_OK;
+}
+
+// Orientations per batch within `budget` bytes of device memory.  Each
+// orientation of a batch holds three complex planes (wave, transmission,
+// propagated exit wave), its rotated atom records and its slice table (start
+// offset and Debye-Waller factor per slice).  The host-output staging of
+// the images (at most 2 floats per pixel and defocus distance) is allocated
+// per call and not counted here.
+int ms_batch_size(int nx, int ny, int natom, int nsl, size_t budget) {
+    const size_t per = (size_t)3 * nx * ny * sizeof(cplx<float>) +
+                       (size_t)std::max(natom, 1) * sizeof(utr::MsAtom) +
+                       (size_t)(nsl + 1) * (sizeof(int) + sizeof(float));
+    return (int)std::max<size_t>(1, std::min<size_t>(budget / per, 1 << 16));
 }
 
 int ms_images(msct_ctx* c, int nang, const double* angZ, const double* angY, int ndef, const double* defocus,
@@ -1112,12 +1128,11 @@ int msct_create(const msct_params* p, const double* fe_params, int natom, const 
         (e = up(c->twy, make_twiddles<float>(ny))) != hipSuccess)
         return fail(set_err(e == hipErrorOutOfMemory ? UTR_E_NOMEM : UTR_E_DEVICE,
                             std::string("msct_create: upload failed: ") + hipGetErrorString(e)));
-    // orientations per batch: three complex planes each (wave, transmission,
-    // propagated exit wave); UTR_MSCT_BATCH_BYTES caps the total
+    // orientations per batch: UTR_MSCT_BATCH_BYTES caps the device memory of
+    // everything held per orientation of a batch
     size_t budget = (size_t)6 << 30;
     if (const char* s = std::getenv("UTR_MSCT_BATCH_BYTES")) budget = std::max<size_t>(1, std::strtoull(s, nullptr, 10));
-    const size_t per = (size_t)3 * nx * ny * sizeof(cplx<float>);
-    c->batch = (int)std::max<size_t>(1, std::min<size_t>(budget / per, 1 << 16));
+    c->batch = ms_batch_size(nx, ny, natom, c->nsl, budget);
     *out = c;
     return UTR_OK;
 }
@@ -1133,6 +1148,8 @@ int msct_images_device(msct_ctx* ctx, int nang, const double* angZ_rad, const do
 }
 
 int msct_slices(const msct_ctx* ctx) { return ctx ? ctx->nsl : -1; }
+
+int msct_batch(const msct_ctx* ctx) { return ctx ? ctx->batch : -1; }
 
 int msct_destroy(msct_ctx* c) {
     if (!c) return UTR_OK;

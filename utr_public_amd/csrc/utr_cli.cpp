@@ -130,48 +130,52 @@ int run(int argc, char** argv) {
     long t_read = 0, t_write = 0, t_peaks = 0;
 
     printf("\nReading input parameter file %s ...", pfile.c_str());
-    Config c = read_config(pfile);
+    // orientation table and subseries selection (UTR.cpp:113-230), read and
+    // checked where the reference does: after parameter 4
+    std::vector<Orientation> rows;
+    std::vector<float> weights;
+    std::vector<int> sel;
+    long nfull = 0;
+    bool single_defocus = true;
+    auto tables = [&](const Config& c4) {
+        printf("\nReading defocus parameter file %s ...", c4.defocus_file.c_str());
+        rows = read_orientations(c4.defocus_file);
+        nfull = (long)rows.size();
+        if (c4.subseries_file != "NONE") {
+            read_subseries(c4.subseries_file, sel, weights);
+            if (sel.empty()) throw std::runtime_error("Error: the number of entries in the subseries selection file is equal to zero.");
+            if ((long)sel.size() > nfull)
+                throw std::runtime_error("Error: the number of entries in the subseries selection file is larger than the number of entries in the defocus file.");
+            double total = 0;
+            for (size_t i = 0; i < sel.size(); i++) {
+                if (sel[i] > nfull - 1)
+                    throw std::runtime_error("Error: some frame indexes in the subseries selection file are larger than the number of entries - 1 in the defocus file.");
+                if (sel[i] < 0) throw std::runtime_error("Error: some frame indexes in the subseries selection file are negative.");
+                total += weights[i];
+            }
+            if (total == 0)
+                throw std::runtime_error("Error: the sum of weights in the second column of the subseries selection file is equal to zero.");
+            std::vector<Orientation> r2;
+            for (int i : sel) r2.push_back(rows[i]);
+            rows.swap(r2);
+            printf("\nThere are %zd entries in the subseries selection file.", rows.size());
+        }
+        size_t ndefmax = 0;
+        for (const auto& o : rows) {
+            ndefmax = std::max(ndefmax, o.defocus.size());
+            if (o.defocus[0].second != rows[0].defocus[0].second) single_defocus = false;
+        }
+        if (ndefmax > 1)
+            throw std::runtime_error("Error: this program is currently implemented only for defocus parameter files with one defocus distance per orientation.");
+    };
+    Config c = read_config(pfile, tables);
+    const long nrows = (long)rows.size();
     printf("\nUnits of length (UL) are set to %s",
            c.modality == 0 ? "Angstroms (TEM imaging mode will be used)" : "microns (X-ray CT imaging mode will be used)");
     if (!std::filesystem::exists(path_folder(c.output_template)))
         throw std::runtime_error("Error: the specified file folder for 3D potential or beta does not seem to exist.");
     if (!std::filesystem::exists(c.aux_folder))
         throw std::runtime_error("Error: the specified auxiliary file folder does not seem to exist.");
-
-    // orientation table and subseries selection (UTR.cpp:105-230)
-    printf("\nReading defocus parameter file %s ...", c.defocus_file.c_str());
-    std::vector<Orientation> rows = read_orientations(c.defocus_file);
-    const long nfull = (long)rows.size();
-    std::vector<int> sel;
-    std::vector<float> weights;
-    if (c.subseries_file != "NONE") {
-        read_subseries(c.subseries_file, sel, weights);
-        if (sel.empty()) throw std::runtime_error("Error: the number of entries in the subseries selection file is equal to zero.");
-        if ((long)sel.size() > nfull)
-            throw std::runtime_error("Error: the number of entries in the subseries selection file is larger than the number of entries in the defocus file.");
-        double total = 0;
-        for (size_t i = 0; i < sel.size(); i++) {
-            if (sel[i] > nfull - 1)
-                throw std::runtime_error("Error: some frame indexes in the subseries selection file are larger than the number of entries - 1 in the defocus file.");
-            if (sel[i] < 0) throw std::runtime_error("Error: some frame indexes in the subseries selection file are negative.");
-            total += weights[i];
-        }
-        if (total == 0)
-            throw std::runtime_error("Error: the sum of weights in the second column of the subseries selection file is equal to zero.");
-        std::vector<Orientation> r2;
-        for (int i : sel) r2.push_back(rows[i]);
-        rows.swap(r2);
-        printf("\nThere are %zd entries in the subseries selection file.", rows.size());
-    }
-    const long nrows = (long)rows.size();
-    size_t ndefmax = 0;
-    bool single_defocus = true;
-    for (const auto& o : rows) {
-        ndefmax = std::max(ndefmax, o.defocus.size());
-        if (o.defocus[0].second != rows[0].defocus[0].second) single_defocus = false;
-    }
-    if (ndefmax > 1)
-        throw std::runtime_error("Error: this program is currently implemented only for defocus parameter files with one defocus distance per orientation.");
     bool astig = false;
     for (const auto& o : rows) astig |= o.astig_dz != 0 || o.astig_angle != 0;
     if (check_only) {

@@ -118,7 +118,7 @@ std::string path_folder(const std::string& path) {
 
 // ------------------------------------------------------- parameter file
 
-Config read_config(const std::string& path) {
+Config read_config(const std::string& path, const std::function<void(const Config&)>& tables) {
     Config c;
     const size_t dot = path.find_last_of('.');
     if (dot == std::string::npos)
@@ -150,6 +150,7 @@ Config read_config(const std::string& path) {
     if (c.subseries_file != "NONE" && file_extension(c.subseries_file) != ".TXT")
         fail("Error: unrecognised filename extension in the file with indexes and weights for "
              "subseries selection in input parameter file.");
+    if (tables) tables(c);
     c.input_template = R.next(2, "Error reading defocus series file name base from input parameter file.")[1];  // 5
     {
         const std::string e = file_extension(c.input_template);
@@ -521,6 +522,13 @@ void write_grd(const std::string& path, long nx, long ny, const float* v, double
 }
 
 Image read_raw(const std::string& path, long nx, long ny, long header, long element, bool big_endian) {
+    // the parameter-file reader checks the element size (UTR.cpp:267-268);
+    // the C ABI entry point reaches here directly
+    if (element != 4 && element != 8)
+        fail("Unrecognised ElementLength in input parameter file (only sizeof(float) or "
+             "sizeof(double) are allowed).");
+    if (header < 0) fail("Error: RAW header length must not be negative.");
+    if (nx <= 0 || ny <= 0) fail("Error: RAW image sizes must be positive.");
     auto b = read_all(path);
     const size_t n = (size_t)nx * ny;
     if (b.size() < (size_t)header + n * element) fail("Error: RAW file " + path + " is too short.");

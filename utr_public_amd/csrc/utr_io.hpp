@@ -6,6 +6,7 @@
 #pragma once
 
 #include <cstdint>
+#include <functional>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -78,7 +79,10 @@ std::string path_folder(const std::string& path);
 
 // Reads and validates a parameter file; throws std::runtime_error with the
 // reference's messages.  Folder existence checks are left to the caller.
-Config read_config(const std::string& path);
+// tables (may be empty) runs right after parameter 4 is read, where
+// UTR.cpp:113-230 reads the orientation and subseries tables and validates
+// them, so that their errors come before those of parameters 5 and later.
+Config read_config(const std::string& path, const std::function<void(const Config&)>& tables = {});
 
 // Orientation files: ".txt" (read as written) and ".relionnew" (all values
 // except the astigmatism angle negated, ReadmeUTR.txt:215-220).
