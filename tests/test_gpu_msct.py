@@ -255,8 +255,8 @@ def test_thermal_device_output_over_several_batches(output, monkeypatch):
     angZ, angY = [0.0, 10.0, 20.0, 30.0, 40.0], [5.0, 50.0, 95.0, 140.0, 185.0]
     defocus = [0.0, 120.0] if output == 0 else [-25.6, -10.0]
     host = U.multislice_images(p, at, angZ, angY, defocus)
-    per = 3 * 64 * 64 * 8 + len(at["Z"]) * 32
-    monkeypatch.setenv("UTR_MSCT_BATCH_BYTES", str(2 * per + 100))
+    per = 3 * 64 * 64 * 8 + len(at["Z"]) * 32 + 60 * 8  # planes, atom records, slice table
+    monkeypatch.setenv("UTR_MSCT_BATCH_BYTES", str(2 * per))
     m = U.Multislice(p, at)
     try:
         assert m.batch == 2
