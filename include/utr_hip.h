@@ -193,6 +193,19 @@ int utr_highpass3d_device(float* d_vol, int nx, int ny, int nz, double xst, doub
                           double width);
 int utr_highpass3d(float* vol, int nx, int ny, int nz, double xst, double yst, double zst, double width);
 
+/* Allocation-free form of utr_highpass3d_device.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and result as utr_highpass3d_device; it calls
+ * no hipMalloc/hipFree and creates no stream, using instead a fixed work area
+ * that the library holds on every GPU it runs on for the life of the process
+ * (16 MiB + 24 KiB: nx*ny*nz at most 2^21 voxels, e.g. 128^3; larger volumes
+ * are rejected with UTR_E_INVALID).  Runs on the null stream and returns
+ * after the result is in d_vol.  All calls on one GPU share that work area:
+ * calls that overlap on one GPU (from several host threads) race. */
+int utr_highpass3d_device_noalloc(float* d_vol, int nx, int ny, int nz, double xst, double yst,
+                                  double zst, double width);
+
 /* Regularised inverse (-Laplacian) of a real volume, replacing
  * fft3f.InverseMLaplacian(K3out, epsilon, epsilon) of UTR.cpp:1363-1367
  * (parameter 22 with parameter 33): the 3D FFT times

@@ -40,7 +40,7 @@ EXPORTS = [
     "utr_batch_create", "utr_batch_run", "utr_batch_job_form", "utr_batch_destroy",
     "dht_create", "dht_add_images", "dht_add_images_device", "dht_finalize", "dht_finalize_device",
     "dht_reset", "dht_dims", "dht_get_timings", "dht_destroy", "dht_set_image_correction",
-    "utr_highpass3d", "utr_highpass3d_device", "utr_rotate3d", "utr_rotate3d_device",
+    "utr_highpass3d", "utr_highpass3d_device", "utr_highpass3d_device_noalloc", "utr_rotate3d", "utr_rotate3d_device",
     "utr_symmetrize3d", "utr_symmetrize3d_device", "utr_invlaplace3d", "utr_invlaplace3d_device",
     "msct_create", "msct_images", "msct_images_device", "msct_slices", "msct_batch", "msct_destroy",
 ]
@@ -188,7 +188,7 @@ def load():
     L.dht_set_image_correction.argtypes = [vp, dp, dp, dp]
     d = ctypes.c_double
     i = ctypes.c_int
-    for f in ("utr_highpass3d", "utr_highpass3d_device"):
+    for f in ("utr_highpass3d", "utr_highpass3d_device", "utr_highpass3d_device_noalloc"):
         getattr(L, f).argtypes = [vp, i, i, i, d, d, d, d]
     for f in ("utr_invlaplace3d", "utr_invlaplace3d_device"):
         getattr(L, f).argtypes = [vp, i, i, i, d, d, d, d, d]

@@ -326,17 +326,48 @@ int utr_symmetrize3d(float* vol, int nx, int ny, int nz, int nsym, const double*
 
 namespace {
 
-// d_vol (real, nz*ny*nx) -> 3D FFT -> filter(scratch, blocks, stream) ->
-// inverse FFT -> d_vol -= Re (subtract) or d_vol = Re.
+// d_vol (real, nz*ny*nx) -> 3D FFT in work -> filter(work, blocks, stream) ->
+// inverse FFT -> d_vol -= Re (subtract) or d_vol = Re, launched on stream s
+// (not synchronised).  tw[0..2]: twiddle tables of nx, ny, nz on the device.
 template <typename F>
-int post_fourier(float* d_vol, int nx, int ny, int nz, F&& filter, bool subtract) {
+int post_fourier_launch(float* d_vol, int nx, int ny, int nz, utr::cplx<float>* work,
+                        utr::cplx<float>* const tw[3], hipStream_t s, F&& filter, bool subtract) {
+    const size_t n = (size_t)nx * ny * nz;
+    const unsigned blocks = (unsigned)((n + 255) / 256);
+    utr::k_post_load<<<blocks, 256, 0, s>>>(d_vol, work, n);
+    UTR_HIP_TRY(hipGetLastError());
+    for (int ax = 0; ax < 3; ax++) {
+        const int rc = post_fft_axis(work, nx, ny, nz, ax, -1, tw[ax], s);
+        if (rc) return rc;
+    }
+    filter(work, blocks, s);
+    UTR_HIP_TRY(hipGetLastError());
+    for (int ax = 2; ax >= 0; ax--) {
+        const int rc = post_fft_axis(work, nx, ny, nz, ax, +1, tw[ax], s);
+        if (rc) return rc;
+    }
+    if (subtract) utr::k_post_subtract<<<blocks, 256, 0, s>>>(d_vol, work, n);
+    else utr::k_post_real<<<blocks, 256, 0, s>>>(d_vol, work, n);
+    UTR_HIP_TRY(hipGetLastError());
+    return UTR_OK;
+}
+
+int post_check_edges(int nx, int ny, int nz) {
     if (!fft_size_supported(nx) || !fft_size_supported(ny) || !fft_size_supported(nz))
         return set_err(UTR_E_UNSUPPORTED,
                        "volume edge not supported by the FFT kernels (supported: 16 32 48 64 96 128 "
                        "160 192 256 320 384 512)");
+    return UTR_OK;
+}
+
+// The same with a complex work volume and twiddle tables allocated here, on a
+// stream of its own, synchronised before returning.
+template <typename F>
+int post_fourier(float* d_vol, int nx, int ny, int nz, F&& filter, bool subtract) {
+    if (const int rc = post_check_edges(nx, ny, nz)) return rc;
     const size_t n = (size_t)nx * ny * nz;
-    LocalBuf<utr::cplx<float>> scratch, tx, ty, tz;
-    UTR_HIP_TRY(scratch.alloc(n));
+    LocalBuf<utr::cplx<float>> work, tx, ty, tz;
+    UTR_HIP_TRY(work.alloc(n));
     const int ns[3] = {nx, ny, nz};
     DevBuf<utr::cplx<float>>* tws[3] = {&tx, &ty, &tz};
     for (int d = 0; d < 3; d++) {
@@ -347,51 +378,134 @@ int post_fourier(float* d_vol, int nx, int ny, int nz, F&& filter, bool subtract
     }
     hipStream_t s = nullptr;
     UTR_HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    auto run = [&]() -> int {
-        const unsigned blocks = (unsigned)((n + 255) / 256);
-        utr::k_post_load<<<blocks, 256, 0, s>>>(d_vol, scratch.p, n);
-        UTR_HIP_TRY(hipGetLastError());
-        for (int ax = 0; ax < 3; ax++) {
-            const int rc = post_fft_axis(scratch.p, nx, ny, nz, ax, -1, tws[ax]->p, s);
-            if (rc) return rc;
-        }
-        filter(scratch.p, blocks, s);
-        UTR_HIP_TRY(hipGetLastError());
-        for (int ax = 2; ax >= 0; ax--) {
-            const int rc = post_fft_axis(scratch.p, nx, ny, nz, ax, +1, tws[ax]->p, s);
-            if (rc) return rc;
-        }
-        if (subtract) utr::k_post_subtract<<<blocks, 256, 0, s>>>(d_vol, scratch.p, n);
-        else utr::k_post_real<<<blocks, 256, 0, s>>>(d_vol, scratch.p, n);
-        UTR_HIP_TRY(hipGetLastError());
-        UTR_HIP_TRY(hipStreamSynchronize(s));
-        return UTR_OK;
-    };
-    const int rc = run();
+    utr::cplx<float>* const twp[3] = {tx.p, ty.p, tz.p};
+    int rc = post_fourier_launch(d_vol, nx, ny, nz, work.p, twp, s, filter, subtract);
+    if (!rc) {
+        const hipError_t e = hipStreamSynchronize(s);
+        if (e != hipSuccess) rc = set_err(UTR_E_DEVICE, std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
+    }
     (void)hipStreamSynchronize(s);
     (void)hipStreamDestroy(s);
     return rc;
 }
 
+// Gaussian high-pass filter of parameter 23 (UTR.cpp:1357-1378) as a filter
+// callback for post_fourier / post_fourier_launch.
+struct HighpassFilter {
+    int nx, ny, nz;
+    double cx, cy, cz, invN;
+    HighpassFilter(int nx_, int ny_, int nz_, double xst, double yst, double zst, double width)
+        : nx(nx_), ny(ny_), nz(nz_) {
+        const double pi = 3.14159265358979323846;
+        const double sigma = width / 2.355;
+        const double k2 = 2.0 * pi * pi * sigma * sigma;
+        cx = k2 / ((nx * xst) * (nx * xst));
+        cy = k2 / ((ny * yst) * (ny * yst));
+        cz = k2 / ((nz * zst) * (nz * zst));
+        invN = 1.0 / ((double)nx * ny * nz);
+    }
+    void operator()(utr::cplx<float>* d, unsigned blocks, hipStream_t s) const {
+        utr::k_post_gauss<<<blocks, 256, 0, s>>>(d, nx, ny, nz, cx, cy, cz, invN);
+    }
+};
+
+int highpass_check(const float* d_vol, int nx, int ny, int nz, double xst, double yst, double zst,
+                   double width) {
+    if (!d_vol) return set_err(UTR_E_INVALID, "null volume pointer");
+    if (nx <= 0 || ny <= 0 || nz <= 0 || !(xst > 0) || !(yst > 0) || !(zst > 0))
+        return set_err(UTR_E_INVALID, "utr_highpass3d: sizes and steps must be positive");
+    if (!(width > 0)) return set_err(UTR_E_INVALID, "utr_highpass3d: the filter width must be positive");
+    return UTR_OK;
+}
+
+// ------------------------------------------- allocation-free high-pass filter
+// Work memory of utr_highpass3d_device_noalloc: three twiddle tables of
+// kHpTwiddles entries, then the complex work volume of at most kHpMaxVoxels
+// voxels (16 MiB; 128^3 or any other nx*ny*nz up to 2^21).  Every GPU the
+// library runs on holds its own copy for the life of the process.  Host code
+// never uses the array's name as an address: hp_scratch() fetches each
+// device's address once with hipGetSymbolAddress and that pointer is what
+// the kernels receive.
+constexpr size_t kHpTwiddles = 1024;           // >= FacList::ntw for every supported edge
+constexpr size_t kHpMaxVoxels = (size_t)1 << 21;
+static __device__ __align__(256) utr::cplx<float> scratch[3 * kHpTwiddles + kHpMaxVoxels];
+
+constexpr int kHpMaxDevices = 64;
+std::mutex g_hp_mu;
+utr::cplx<float>* g_hp_ptr[kHpMaxDevices] = {};
+
+// Address of this device's copy of scratch, looked up on the first call on
+// that device (with that device current) and kept.
+int hp_scratch(utr::cplx<float>** out) {
+    int dev = -1;
+    UTR_HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kHpMaxDevices) return set_err(UTR_E_DEVICE, "device ordinal out of range");
+    std::lock_guard<std::mutex> lk(g_hp_mu);
+    if (!g_hp_ptr[dev]) {
+        void* p = nullptr;
+        UTR_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return set_err(UTR_E_DEVICE, "hipGetSymbolAddress returned a null address");
+        g_hp_ptr[dev] = static_cast<utr::cplx<float>*>(p);
+    }
+    *out = g_hp_ptr[dev];
+    return UTR_OK;
+}
+
 }  // namespace
+
+namespace utr {
+
+// Twiddle table of length n in the FacList layout (the table make_twiddles
+// builds on the host), evaluated on the device in FP64.
+__global__ void k_post_twiddles(cplx<float>* __restrict__ tw, int n) {
+    const FacList F = factorize(n);
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= F.ntw) return;
+    for (int s = 0; s < F.nf; s++) {
+        const int p = F.p[s], R = F.f[s];
+        const int loc = t - F.off[s];
+        if (loc < 0 || loc >= p * (R - 1)) continue;
+        const int k = loc / (R - 1), r = loc - k * (R - 1) + 1;
+        const double ang = -2.0 * kPi * (double)r * k / ((double)p * R);
+        tw[t] = {(float)cos(ang), (float)sin(ang)};
+    }
+}
+
+}  // namespace utr
 
 extern "C" {
 
 int utr_highpass3d_device(float* d_vol, int nx, int ny, int nz, double xst, double yst, double zst,
                           double width) {
-    if (!d_vol) return set_err(UTR_E_INVALID, "null volume pointer");
-    if (nx <= 0 || ny <= 0 || nz <= 0 || !(xst > 0) || !(yst > 0) || !(zst > 0))
-        return set_err(UTR_E_INVALID, "utr_highpass3d: sizes and steps must be positive");
-    if (!(width > 0)) return set_err(UTR_E_INVALID, "utr_highpass3d: the filter width must be positive");
+    if (const int rc = highpass_check(d_vol, nx, ny, nz, xst, yst, zst, width)) return rc;
+    return post_fourier(d_vol, nx, ny, nz, HighpassFilter(nx, ny, nz, xst, yst, zst, width), true);
+}
+
+int utr_highpass3d_device_noalloc(float* d_vol, int nx, int ny, int nz, double xst, double yst,
+                                  double zst, double width) {
+    if (const int rc = highpass_check(d_vol, nx, ny, nz, xst, yst, zst, width)) return rc;
+    if (const int rc = post_check_edges(nx, ny, nz)) return rc;
     const size_t n = (size_t)nx * ny * nz;
-    const double pi = 3.14159265358979323846;
-    const double sigma = width / 2.355;
-    const double k2 = 2.0 * pi * pi * sigma * sigma;
-    return post_fourier(d_vol, nx, ny, nz, [&](utr::cplx<float>* d, unsigned blocks, hipStream_t s) {
-        utr::k_post_gauss<<<blocks, 256, 0, s>>>(d, nx, ny, nz, k2 / ((nx * xst) * (nx * xst)),
-                                                 k2 / ((ny * yst) * (ny * yst)),
-                                                 k2 / ((nz * zst) * (nz * zst)), 1.0 / (double)n);
-    }, true);
+    if (n > kHpMaxVoxels)
+        return set_err(UTR_E_INVALID, "utr_highpass3d_device_noalloc: nx*ny*nz = " + std::to_string(n) +
+                                          " exceeds the fixed work area of " +
+                                          std::to_string(kHpMaxVoxels) + " voxels");
+    utr::cplx<float>* base = nullptr;
+    if (const int rc = hp_scratch(&base)) return rc;
+    // Nothing needs clearing: the twiddle kernels write every table entry the
+    // FFTs read and k_post_load writes every work voxel before it is read.
+    utr::cplx<float>* const tw[3] = {base, base + kHpTwiddles, base + 2 * kHpTwiddles};
+    const int ns[3] = {nx, ny, nz};
+    hipStream_t s = nullptr;  // the null stream: no stream is created
+    for (int d = 0; d < 3; d++) {
+        utr::k_post_twiddles<<<(unsigned)(kHpTwiddles / 256), 256, 0, s>>>(tw[d], ns[d]);
+        UTR_HIP_TRY(hipGetLastError());
+    }
+    const int rc = post_fourier_launch(d_vol, nx, ny, nz, base + 3 * kHpTwiddles, tw, s,
+                                       HighpassFilter(nx, ny, nz, xst, yst, zst, width), true);
+    if (rc) return rc;
+    UTR_HIP_TRY(hipStreamSynchronize(s));
+    return UTR_OK;
 }
 
 int utr_invlaplace3d_device(float* d_vol, int nx, int ny, int nz, double xst, double yst, double zst,
