@@ -410,6 +410,49 @@ int msct_slices(const msct_ctx* ctx);
 int msct_batch(const msct_ctx* ctx);
 int msct_destroy(msct_ctx* ctx);
 
+/* ---------------------------------------------------------------------------
+ * Learned residual corrector.  No counterpart in the reference (which has no
+ * learned component): a 3-level 3D U-Net (3x3x3 convolutions with 16/32/64
+ * channels, ReLU, 2x average pooling, nearest upsampling, skip connections by
+ * channel concatenation; head: alpha * input + a 1x1x1 convolution of the last
+ * 16 channels) mapping a UTR volume to a corrected one.  BF16 weights and
+ * activations on the matrix cores with FP32 accumulation; trained on the GPU
+ * by the library's own forward, backward and Adam kernels.  Values enter the
+ * network multiplied by `scale` (utr_corr_set_params) and leave divided by it.
+ * Crops are cubes of T = tile + 2 halo voxels (FP32, x fastest); the loss is
+ * the mean squared error over the central tile^3 of each crop.
+ * ------------------------------------------------------------------------- */
+typedef struct utr_corr utr_corr;
+
+/* Network with seeded He-initialised weights (the head starts as the
+ * identity: alpha = 1), for crops of tile + 2 halo voxels (a multiple of 4,
+ * at most 512) and up to `batch` crops per call. */
+int utr_corr_create(int tile, int halo, int batch, unsigned long long seed, utr_corr** out);
+/* Number of FP32 parameters: per layer W[27][cin][cout] then b[cout] for the
+ * ten convolutions (cin, cout) = (16,16) (16,16) (16,32) (32,32) (32,64)
+ * (64,64) (96,32) (32,32) (48,16) (16,16) -- the first layer's input is the
+ * volume in channel 0 of 16 -- then the head w1[16], b1, alpha. */
+int utr_corr_nparams(const utr_corr* c);
+int utr_corr_tile(const utr_corr* c, int* tile, int* halo, int* batch);
+/* Copies the parameters (nparams floats, may be NULL) and the input scale. */
+int utr_corr_get_params(utr_corr* c, float* params, float* scale);
+/* Sets the input scale and (params may be NULL) the parameters; resets the
+ * Adam moments. */
+int utr_corr_set_params(utr_corr* c, const float* params, float scale);
+/* Network output for nb <= batch crops in device memory (nb * T^3 floats). */
+int utr_corr_forward(utr_corr* c, const float* d_crops, int nb, float* d_out);
+/* Loss and gradient (nparams floats to host memory, may be NULL) of nb crops
+ * against targets of the same shape; no update. */
+int utr_corr_gradient(utr_corr* c, const float* d_crops, const float* d_targets, int nb, float* loss,
+                      float* grad);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) with learning rate lr;
+ * loss (may be NULL) is the batch loss before the step. */
+int utr_corr_train_step(utr_corr* c, const float* d_crops, const float* d_targets, int nb, float lr, float* loss);
+/* Corrected volume (nz*ny*nx floats in device memory) from tiles of the
+ * input volume with their halos (zeros outside the volume). */
+int utr_corr_apply(utr_corr* c, const float* d_vol, float* d_out, int nx, int ny, int nz);
+int utr_corr_destroy(utr_corr* c);
+
 #ifdef __cplusplus
 }
 #endif

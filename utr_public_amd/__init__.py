@@ -43,6 +43,8 @@ EXPORTS = [
     "utr_highpass3d", "utr_highpass3d_device", "utr_highpass3d_device_noalloc", "utr_rotate3d", "utr_rotate3d_device",
     "utr_symmetrize3d", "utr_symmetrize3d_device", "utr_invlaplace3d", "utr_invlaplace3d_device",
     "msct_create", "msct_images", "msct_images_device", "msct_slices", "msct_batch", "msct_destroy",
+    "utr_corr_create", "utr_corr_nparams", "utr_corr_tile", "utr_corr_get_params", "utr_corr_set_params",
+    "utr_corr_forward", "utr_corr_gradient", "utr_corr_train_step", "utr_corr_apply", "utr_corr_destroy",
 ]
 
 DHT_FIELDS = [
@@ -204,6 +206,18 @@ def load():
     L.msct_slices.argtypes = [vp]
     L.msct_batch.argtypes = [vp]
     L.msct_destroy.argtypes = [vp]
+    fp_ = ctypes.POINTER(ctypes.c_float)
+    ip_ = ctypes.POINTER(ctypes.c_int)
+    L.utr_corr_create.argtypes = [i, i, i, ctypes.c_ulonglong, ctypes.POINTER(vp)]
+    L.utr_corr_nparams.argtypes = [vp]
+    L.utr_corr_tile.argtypes = [vp, ip_, ip_, ip_]
+    L.utr_corr_get_params.argtypes = [vp, fp_, fp_]
+    L.utr_corr_set_params.argtypes = [vp, fp_, ctypes.c_float]
+    L.utr_corr_forward.argtypes = [vp, vp, i, vp]
+    L.utr_corr_gradient.argtypes = [vp, vp, vp, i, fp_, fp_]
+    L.utr_corr_train_step.argtypes = [vp, vp, vp, i, ctypes.c_float, fp_]
+    L.utr_corr_apply.argtypes = [vp, vp, vp, i, i, i]
+    L.utr_corr_destroy.argtypes = [vp]
     _lib = L
     return L
 
@@ -686,3 +700,85 @@ def match_atoms(test_xyz, test_Z, ref_xyz, ref_Z, max_dist=1.0, hungarian=True):
     stats = {k: (int(v) if i < 3 else float(v)) for i, (k, v) in enumerate(zip(keys, st))}
     return m, dist, stats
 
+
+
+# --------------------------------------------------------------------------
+# Learned residual corrector (utr_corr_* C ABI)
+
+class Corrector:
+    """The learned 3D residual corrector: a 3-level U-Net (BF16 on the
+    matrix cores, FP32 accumulation) trained and applied on the GPU by the
+    library's own kernels.  Crops are (nb, T, T, T) float32 CUDA tensors with
+    T = tile + 2 * halo; volumes are (nz, ny, nx) float32 CUDA tensors."""
+
+    def __init__(self, tile=64, halo=8, batch=4, seed=1):
+        self._L = load()
+        ctx = ctypes.c_void_p()
+        _check(self._L.utr_corr_create(tile, halo, batch, seed, ctypes.byref(ctx)))
+        self._ctx = ctx
+        self.tile, self.halo, self.batch = tile, halo, batch
+        self.T = tile + 2 * halo
+        self.nparams = self._L.utr_corr_nparams(ctx)
+
+    @staticmethod
+    def _ptr(t, n=None):
+        if not (t.is_cuda and t.is_contiguous() and str(t.dtype) == "torch.float32"):
+            raise ValueError("expected a contiguous float32 CUDA tensor")
+        if n is not None and t.numel() < n:
+            raise ValueError("tensor holds %d floats, %d needed" % (t.numel(), n))
+        return ctypes.c_void_p(t.data_ptr())
+
+    def params(self):
+        """(parameter vector float32[nparams], input scale)."""
+        w = np.empty(self.nparams, np.float32)
+        s = ctypes.c_float()
+        _check(self._L.utr_corr_get_params(self._ctx, w.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                           ctypes.byref(s)))
+        return w, s.value
+
+    def set_params(self, params=None, scale=1.0):
+        p = None
+        if params is not None:
+            params = np.ascontiguousarray(params, np.float32)
+            if params.size != self.nparams:
+                raise ValueError("expected %d parameters" % self.nparams)
+            p = params.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        _check(self._L.utr_corr_set_params(self._ctx, p, float(scale)))
+
+    def forward(self, crops, out):
+        nb = crops.shape[0]
+        n = nb * self.T ** 3
+        _check(self._L.utr_corr_forward(self._ctx, self._ptr(crops, n), nb, self._ptr(out, n)))
+        return out
+
+    def gradient(self, crops, targets):
+        """(loss, gradient float32[nparams]) without an update."""
+        nb = crops.shape[0]
+        n = nb * self.T ** 3
+        g = np.empty(self.nparams, np.float32)
+        loss = ctypes.c_float()
+        _check(self._L.utr_corr_gradient(self._ctx, self._ptr(crops, n), self._ptr(targets, n), nb,
+                                         ctypes.byref(loss), g.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return loss.value, g
+
+    def train_step(self, crops, targets, lr):
+        nb = crops.shape[0]
+        n = nb * self.T ** 3
+        loss = ctypes.c_float()
+        _check(self._L.utr_corr_train_step(self._ctx, self._ptr(crops, n), self._ptr(targets, n), nb, float(lr),
+                                           ctypes.byref(loss)))
+        return loss.value
+
+    def apply(self, vol, out):
+        nz, ny, nx = vol.shape
+        _check(self._L.utr_corr_apply(self._ctx, self._ptr(vol, vol.numel()), self._ptr(out, vol.numel()),
+                                      nx, ny, nz))
+        return out
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            self._L.utr_corr_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        self.close()

@@ -1015,3 +1015,5 @@ int utr_batch_job_form(const utr_batch* b, int j) {
 #include "post.hip"
 
 #include "msct.hip"
+
+#include "corrector.hip"
