@@ -411,6 +411,38 @@ int msct_batch(const msct_ctx* ctx);
 int msct_destroy(msct_ctx* ctx);
 
 /* ---------------------------------------------------------------------------
+ * Evaluation against the forward model.
+ * ------------------------------------------------------------------------- */
+
+/* Ground-truth potential of an atom set on an nz*ny*nx grid (device memory,
+ * x fastest): voxel (i, j, k) at cube position (i, j, k) * xst, atoms (Z, x,
+ * y, z in Angstrom, cube coordinates) on the periodic cube of sides n * xst.
+ * It is the potential whose point samples autosliccmd's noutput 3 sums
+ * (vatom, slicelib.cpp:2699-2737; autosliccmd.cpp:1146-1151), restricted to
+ * spatial frequencies |q| < qmax (1/A; qmax <= 0: all up to Nyquist) and
+ * evaluated exactly by Fourier synthesis from its transform
+ * (al/pi) sum a_i/(q^2+b_i) + ag pi^-1.5 sum c_i exp(-d_i q^2), times scale.
+ * fe_params as msct_create.  Edges must be supported FFT lengths. */
+int utr_ground_truth_device(int nx, int ny, int nz, double xst, double qmax, double scale, const double* fe_params,
+                            int natom, const int* Z, const float* x, const float* y, const float* z, float* d_vol);
+
+/* Multislice reprojection of a reconstructed volume (device memory, UTR
+ * units, nz*ny*nx with nx = ny = nz), replacing the imodeReproj block of
+ * DHT.cpp:1509-1731: the volume times pi/(lambda E) zst (DHT.cpp:1514) is
+ * turned to each row's orientation (about Z, then Y', as the forward model
+ * turns atoms; trilinear sampling, the absent Multislice_eV restated),
+ * propagated plane by plane with the multislice kernels (bandwidth limit,
+ * propagator of thickness xst), then to the row's defocus distance from the
+ * cube centre with the objective aperture and Cs of p.  d_out (may be NULL)
+ * receives the nrows*ny*nx intensities; err (may be NULL, then d_images may
+ * be NULL) the error of DHT.cpp:1686 per row,
+ * || |psi| - sqrt(I0) ||^2 / || sqrt(I0) ||^2, against d_images renormalised
+ * with p's f1, f2 and clamped at 0 (DHT.cpp:1641-1647).  TEM only. */
+int utr_reproject_device(const utr_params* p, const float* d_vol, int nrows, const double* angZ_deg,
+                         const double* angY_deg, const double* defocus, const float* d_images, float* d_out,
+                         double* err);
+
+/* ---------------------------------------------------------------------------
  * Learned residual corrector.  No counterpart in the reference (which has no
  * learned component): a 3-level 3D U-Net (3x3x3 convolutions with 16/32/64
  * channels, ReLU, 2x average pooling, nearest upsampling, skip connections by
@@ -446,7 +478,8 @@ int utr_corr_forward(utr_corr* c, const float* d_crops, int nb, float* d_out);
 int utr_corr_gradient(utr_corr* c, const float* d_crops, const float* d_targets, int nb, float* loss,
                       float* grad);
 /* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) with learning rate lr;
- * loss (may be NULL) is the batch loss before the step. */
+ * loss (may be NULL) is the batch loss before the step.  Like every
+ * utr_corr_* call, it returns after the device work is done. */
 int utr_corr_train_step(utr_corr* c, const float* d_crops, const float* d_targets, int nb, float lr, float* loss);
 /* Corrected volume (nz*ny*nx floats in device memory) from tiles of the
  * input volume with their halos (zeros outside the volume). */

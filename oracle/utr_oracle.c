@@ -351,7 +351,10 @@ int ora_spectrum_astig(const ora_geom* g, const float* img, double zout, double 
     const double xap = g->xhi - g->xlo, yap = g->yhi - g->ylo;
     if (g->ctf_mode == 1 && zout != 0) {
         /* 2D TIE-Hom (UTR.cpp:1001-1006) restated from InverseMLaplacian3D
-         * (UTR.h:159-181) in 2D.  PARITY UNPINNED (XArrayLibrary absent). */
+         * (UTR.h:159-181) in 2D (the function is in the absent XArrayLibrary).
+         * Pinned in tests/test_filter_physics.py: it returns the thickness of
+         * Paganin's homogeneous-object forward model to 0.5 % and equals
+         * InverseMLaplacian3D on the q_z = 0 plane. */
         double alpha = -4.0 * ORA_PI * g->asigma / (g->zout_aver * g->wl);
         double norm = -alpha * sqrt(1.0 + g->asigma * g->asigma) / g->asigma;
         double fact = 4.0 * ORA_PI * ORA_PI;
@@ -365,7 +368,10 @@ int ora_spectrum_astig(const ora_geom* g, const float* img, double zout, double 
             }
     } else if (g->ctf_mode == 3) {
         /* 2D CTF inversion (UTR.cpp:1011-1016) restated from CTFcorrection3D
-         * (UTR.h:70-128) in 2D.  PARITY UNPINNED (XArrayLibrary absent). */
+         * (UTR.h:70-128) in 2D (the function is in the absent XArrayLibrary).
+         * Pinned in tests/test_filter_physics.py: it recovers the multislice
+         * exit-wave phase of a thin specimen where the CTF is strong and
+         * equals CTFcorrection3D on the q_z = 0 plane. */
         double omega = atan(g->asigma);
         double fac2 = ORA_PI * g->wl * zout;
         double dk2 = 1.0 / (xap * xap), de2 = 1.0 / (yap * yap);

@@ -14,7 +14,10 @@
  *                      a brute-force centred DFT and a closed-form Gaussian);
  *                      modes 1 and 3 restate the absent XArrayLibrary 2D
  *                      filters (InverseMLaplacian1, InvertCTF_DT2) from the
- *                      3D analogues in UTR.h -- PARITY UNPINNED.
+ *                      3D analogues in UTR.h; pinned by physics and by the 3D
+ *                      filters (tests/test_filter_physics.py: Paganin's
+ *                      homogeneous-object thickness; the multislice phase of
+ *                      a thin specimen; equality on the q_z = 0 plane).
  *   ora_grid           pinned: brute-force direct splat, weight conservation,
  *                      axis-aligned exact mapping, linearity.
  *   ora_normalize_filter, ora_hermitian, ora_mask_scale
@@ -159,10 +162,12 @@ void ora_last_timings(double* t4);
  *                      pinned against closed forms (free-space unitarity,
  *                      propagation of a Gaussian, zero-distance identity).
  *                      method 4: weak-object CTF inversion (absent
- *                      XA_IWFR::InvertPhaseCTF) -- PARITY UNPINNED; pinned by
- *                      the forward weak-object model it inverts.
- *   ora_dht_reconstruct method 5 reuses ora_spectrum (mode 3 restatement of the
- *                      absent InvertCTF_DT1 -- PARITY UNPINNED) and ora_grid,
+ *                      XA_IWFR::InvertPhaseCTF), pinned by the forward
+ *                      weak-object model it inverts and by the multislice
+ *                      phase of a thin specimen it recovers in the plane
+ *                      through the specimen (tests/test_filter_physics.py).
+ *   ora_dht_reconstruct method 5 reuses ora_spectrum (the mode 3 restatement,
+ *                      pinned as above, for the absent InvertCTF_DT1) and ora_grid,
  *                      with the Filt3 filter, Hermitian fix, c2r and scale of
  *                      DHT.cpp:1341-1416 restated line by line.
  */

@@ -43,6 +43,7 @@ EXPORTS = [
     "utr_highpass3d", "utr_highpass3d_device", "utr_highpass3d_device_noalloc", "utr_rotate3d", "utr_rotate3d_device",
     "utr_symmetrize3d", "utr_symmetrize3d_device", "utr_invlaplace3d", "utr_invlaplace3d_device",
     "msct_create", "msct_images", "msct_images_device", "msct_slices", "msct_batch", "msct_destroy",
+    "utr_ground_truth_device", "utr_reproject_device",
     "utr_corr_create", "utr_corr_nparams", "utr_corr_tile", "utr_corr_get_params", "utr_corr_set_params",
     "utr_corr_forward", "utr_corr_gradient", "utr_corr_train_step", "utr_corr_apply", "utr_corr_destroy",
 ]
@@ -208,6 +209,8 @@ def load():
     L.msct_destroy.argtypes = [vp]
     fp_ = ctypes.POINTER(ctypes.c_float)
     ip_ = ctypes.POINTER(ctypes.c_int)
+    L.utr_ground_truth_device.argtypes = [i, i, i, d, d, d, dp, i, ip_, fp_, fp_, fp_, vp]
+    L.utr_reproject_device.argtypes = [ctypes.POINTER(UtrParams), vp, i, dp, dp, dp, vp, vp, dp]
     L.utr_corr_create.argtypes = [i, i, i, ctypes.c_ulonglong, ctypes.POINTER(vp)]
     L.utr_corr_nparams.argtypes = [vp]
     L.utr_corr_tile.argtypes = [vp, ip_, ip_, ip_]
@@ -782,3 +785,48 @@ class Corrector:
 
     def __del__(self):
         self.close()
+
+
+# --------------------------------------------------------------------------
+# Evaluation against the forward model
+
+def ground_truth(n, xst, atoms, qmax, scale=1.0, out=None):
+    """Band-limited ground-truth potential of an atom set on an n^3 grid of
+    step xst (utr_ground_truth_device): a float32 CUDA tensor (n, n, n),
+    voxel (i, j, k) at cube position (i, j, k) * xst; atoms: dict with Z, x,
+    y, z in cube coordinates (A); potential in V times scale, frequencies
+    |q| < qmax (1/A)."""
+    import torch
+    L = load()
+    Z = np.ascontiguousarray(atoms["Z"], np.int32)
+    xs = [np.ascontiguousarray(atoms[k], np.float32) for k in ("x", "y", "z")]
+    if out is None:
+        out = torch.empty((n, n, n), dtype=torch.float32, device="cuda")
+    fe = np.ascontiguousarray(fe_params())
+    fpp = ctypes.POINTER(ctypes.c_float)
+    _check(L.utr_ground_truth_device(n, n, n, float(xst), float(qmax), float(scale), _dp(fe), len(Z),
+                                     Z.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), xs[0].ctypes.data_as(fpp),
+                                     xs[1].ctypes.data_as(fpp), xs[2].ctypes.data_as(fpp),
+                                     ctypes.c_void_p(out.data_ptr())))
+    return out
+
+
+def reproject(params, vol, rows, images=None, out=None):
+    """Multislice reprojection of a CUDA volume (nz, ny, nx) in UTR units at
+    the orientation rows (dict angZ, angY, defocus; degrees, A) with the
+    error of DHT.cpp:1686 against images (CUDA tensor (rows, ny, nx) or
+    None).  Returns (out or None, errors float64[rows] or None)."""
+    L = load()
+    p = make_params(params)
+    aZ = np.ascontiguousarray(rows["angZ"], np.float64)
+    aY = np.ascontiguousarray(rows["angY"], np.float64)
+    df = np.ascontiguousarray(rows["defocus"], np.float64)
+    err = np.zeros(len(aZ)) if images is not None else None
+    for t in (vol, images, out):
+        if t is not None and not (t.is_cuda and t.is_contiguous() and str(t.dtype) == "torch.float32"):
+            raise ValueError("volumes and images must be contiguous float32 CUDA tensors")
+    _check(L.utr_reproject_device(ctypes.byref(p), ctypes.c_void_p(vol.data_ptr()), len(aZ), _dp(aZ), _dp(aY),
+                                  _dp(df), None if images is None else ctypes.c_void_p(images.data_ptr()),
+                                  None if out is None else ctypes.c_void_p(out.data_ptr()),
+                                  None if err is None else _dp(err)))
+    return out, err

@@ -900,12 +900,11 @@ int utr_corr_train_step(utr_corr* c, const float* d_in, const float* d_tgt, int 
                                                         (float)b2, 1e-8f);
     UTR_HIP_TRY(hipGetLastError());
     if (int rc = cr_repack(c)) return rc;
-    if (loss) {
-        float res[20];
-        UTR_HIP_TRY(hipMemcpyAsync(res, c->res.p, sizeof(res), hipMemcpyDeviceToHost, c->st));
-        UTR_HIP_TRY(hipStreamSynchronize(c->st));
-        *loss = res[18];
-    }
+    // returns with the step done, so the caller may reuse the crop buffers
+    float res[20];
+    UTR_HIP_TRY(hipMemcpyAsync(res, c->res.p, sizeof(res), hipMemcpyDeviceToHost, c->st));
+    UTR_HIP_TRY(hipStreamSynchronize(c->st));
+    if (loss) *loss = res[18];
     return UTR_OK;
 }
 

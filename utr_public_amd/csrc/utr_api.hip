@@ -21,6 +21,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <limits>
 #include <vector>
 
 #include "../../include/utr_hip.h"
@@ -1015,5 +1016,7 @@ int utr_batch_job_form(const utr_batch* b, int j) {
 #include "post.hip"
 
 #include "msct.hip"
+
+#include "evaluate.hip"
 
 #include "corrector.hip"
