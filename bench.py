@@ -57,6 +57,8 @@ def main():
                     help="skip the DHT variants (reported under 'dht', not in value)")
     ap.add_argument("--no-msct", action="store_true",
                     help="skip the multislice forward model (reported under 'multislice')")
+    ap.add_argument("--no-corrector", action="store_true",
+                    help="skip the learned corrector workload (reported under 'corrector')")
     ap.add_argument("--calib-copy", action="store_true",
                     help="first run a plain 256 MiB device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -292,6 +294,34 @@ def main():
             "slice_pixels_per_s": 360 * nsl * 256 * 256 / tm}
         log("multislice: %.1f ms for 360 orientations x 2 defocus (%d slices)" % (tm * 1e3, nsl))
         ms_case = (mp, mat, mdef)
+
+    if not args.no_corrector:
+        # Learned corrector: 200 seeded pairs (multislice -> 90 noisy images at
+        # 100 e/A^2 -> plain UTR 256^3; band-limited ground truth), training on
+        # the GPU, tiled inference, reprojection gate, error against the truth
+        from utr_public_amd import corrector as C
+        cr = C.run("c5", log=log)
+        ho = cr["heldout"]
+        extra["corrector"] = {
+            "workload": "C:N:O:S at the density of 3000 atoms in an 8 nm sphere (%d atoms in a 72 A sphere), "
+                        "90 orientations about y, 2000 A defocus, 100 e/A^2 Poisson noise, UTR mode 0 + ESCC "
+                        "256^3, 3-level U-Net 16/32/64, 64^3 tiles + 8-voxel halo" % cr["natoms"],
+            "pairs": C.CONFIGS["c5"]["pairs"], "pairs_s": round(cr["pairs_s"], 2),
+            "train_steps": C.CONFIGS["c5"]["steps"], "train_s": round(cr["train_s"], 2),
+            "train_mfma_TFLOPs": round(cr["train_tflops"], 1), "infer_ms": round(cr["infer_s"] * 1e3, 2),
+            "infer_mfma_TFLOPs": round(cr["infer_tflops"], 1), "mfma_peak_TFLOPs": 2500.0,
+            "loss_first": cr["loss_first"], "loss_last": cr["loss_last"], "alpha0": cr["alpha0"],
+            "error_plain": [round(t["plain"], 4) for t in ho],
+            "error_plain_best_scale": [round(t["plain_best_scale"], 4) for t in ho],
+            "error_corrected": [round(t["corrected"], 4) for t in ho],
+            "accepted": [t["accepted"] for t in ho],
+            "reprojection_held_out_plain": [t["held_out_plain"] for t in ho],
+            "reprojection_held_out_corrected": [t["held_out_corrected"] for t in ho],
+            "reprojection_in_sample_plain": [t["in_sample_plain"] for t in ho],
+            "reprojection_in_sample_corrected": [t["in_sample_corrected"] for t in ho],
+            "gate_ms": round(float(np.mean([t["gate_s"] for t in ho])) * 1e3, 1)}
+        log("corrector: errors plain %s corrected %s" % (extra["corrector"]["error_plain"],
+                                                         extra["corrector"]["error_corrected"]))
 
     cpu = None
     if not args.no_cpu_baseline:
