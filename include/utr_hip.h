@@ -193,6 +193,34 @@ int utr_highpass3d_device(float* d_vol, int nx, int ny, int nz, double xst, doub
                           double width);
 int utr_highpass3d(float* vol, int nx, int ny, int nz, double xst, double yst, double zst, double width);
 
+/* ThresholdLow of parameter 23 (UTR.cpp:1383): v = v - background, then
+ * values below threshold set to threshold; n floats in place. */
+int utr_threshold_device(float* d_vol, long long n, float background, float threshold);
+int utr_threshold(float* vol, long long n, float background, float threshold);
+
+/* Reslice and axis flips of parameters 29-30 (UTR.cpp:1449-1453): output
+ * slices in XY (mode 0), XZ (1), YZ (2), YX (3), ZX (4) or ZY (5) planes --
+ * the first letter along a slice's rows, the second down the slice, slices
+ * stacked along the remaining axis -- after reversing the flipped input
+ * axes.  out holds nx*ny*nz floats, distinct from in; the output edges are
+ * the input edges in the mode's order. */
+int utr_reslice_device(const float* d_in, float* d_out, int nx, int ny, int nz, int mode, int flipx, int flipy,
+                       int flipz);
+int utr_reslice(const float* in, float* out, int nx, int ny, int nz, int mode, int flipx, int flipy, int flipz);
+
+/* Thick slices of parameter 31 (UTR.cpp:1456-1490): nz / step output slices,
+ * each the MIP (mode 1) or the average (mode 2) of step consecutive input
+ * slices; out holds nx*ny*(nz/step) floats. */
+int utr_thick_slices_device(const float* d_in, float* d_out, int nx, int ny, int nz, int mode, int step);
+int utr_thick_slices(const float* in, float* out, int nx, int ny, int nz, int mode, int step);
+
+/* 16-bit output of parameter 27 (UTR.cpp:1494-1498; Convert is in the absent
+ * XArrayLibrary, restated): values clamped to [minin, maxin] and mapped
+ * linearly onto 0 .. maxout, rounded half away from zero. */
+int utr_convert16_device(const float* d_in, unsigned short* d_out, long long n, float minin, float maxin,
+                         long long maxout);
+int utr_convert16(const float* in, unsigned short* out, long long n, float minin, float maxin, long long maxout);
+
 /* Allocation-free form of utr_highpass3d_device.  No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid

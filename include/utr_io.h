@@ -66,6 +66,19 @@ int utr_io_match_atoms(int ntest, const double* test_xyz, const int* test_z, int
                        const int* ref_z, double max_dist, int hungarian, int* match, double* dist,
                        double* stats);
 
+/*
+ * Host forms of the output post-processing of UTR.cpp:1383 and 1449-1498 (the
+ * command-line program runs the GPU forms of include/utr_hip.h; these are
+ * their host restatements, used to check them): ThresholdLow in place;
+ * Reslice with flips (modes 0-5 as utr_reslice_device); thick MIP (1) or
+ * average (2) slices over step input slices; 16-bit conversion.
+ */
+int utr_io_threshold(float* vol, long n, float background, float threshold);
+int utr_io_reslice(const float* in, float* out, long nx, long ny, long nz, int mode, int flipx, int flipy,
+                   int flipz);
+int utr_io_thick_slices(const float* in, float* out, long nx, long ny, long nz, int mode, int step);
+int utr_io_convert16(const float* in, unsigned short* out, long n, float minin, float maxin, long maxout);
+
 const char* utr_io_last_error(void);
 
 #ifdef __cplusplus

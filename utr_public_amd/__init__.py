@@ -44,6 +44,8 @@ EXPORTS = [
     "utr_symmetrize3d", "utr_symmetrize3d_device", "utr_invlaplace3d", "utr_invlaplace3d_device",
     "msct_create", "msct_images", "msct_images_device", "msct_slices", "msct_batch", "msct_destroy",
     "utr_ground_truth_device", "utr_reproject_device",
+    "utr_threshold", "utr_threshold_device", "utr_reslice", "utr_reslice_device", "utr_thick_slices",
+    "utr_thick_slices_device", "utr_convert16", "utr_convert16_device",
     "utr_corr_create", "utr_corr_nparams", "utr_corr_tile", "utr_corr_get_params", "utr_corr_set_params",
     "utr_corr_forward", "utr_corr_gradient", "utr_corr_train_step", "utr_corr_apply", "utr_corr_destroy",
 ]
@@ -209,6 +211,15 @@ def load():
     L.msct_destroy.argtypes = [vp]
     fp_ = ctypes.POINTER(ctypes.c_float)
     ip_ = ctypes.POINTER(ctypes.c_int)
+    ll = ctypes.c_longlong
+    for f in ("utr_threshold", "utr_threshold_device"):
+        getattr(L, f).argtypes = [vp, ll, ctypes.c_float, ctypes.c_float]
+    for f in ("utr_reslice", "utr_reslice_device"):
+        getattr(L, f).argtypes = [vp, vp, i, i, i, i, i, i, i]
+    for f in ("utr_thick_slices", "utr_thick_slices_device"):
+        getattr(L, f).argtypes = [vp, vp, i, i, i, i, i]
+    for f in ("utr_convert16", "utr_convert16_device"):
+        getattr(L, f).argtypes = [vp, vp, ll, ctypes.c_float, ctypes.c_float, ll]
     L.utr_ground_truth_device.argtypes = [i, i, i, d, d, d, dp, i, ip_, fp_, fp_, fp_, vp]
     L.utr_reproject_device.argtypes = [ctypes.POINTER(UtrParams), vp, i, dp, dp, dp, vp, vp, dp]
     L.utr_corr_create.argtypes = [i, i, i, ctypes.c_ulonglong, ctypes.POINTER(vp)]
@@ -513,6 +524,49 @@ def highpass3d(vol, xst, yst, zst, width):
     out = np.ascontiguousarray(vol, dtype=np.float32).copy()
     nz, ny, nx = out.shape
     _check(load().utr_highpass3d(out.ctypes.data, nx, ny, nz, xst, yst, zst, width))
+    return out
+
+
+def threshold(vol, background, threshold):
+    """ThresholdLow of parameter 23 on the GPU (UTR.cpp:1383): a new array
+    with vol - background, values below threshold set to it."""
+    out = np.ascontiguousarray(vol, dtype=np.float32).copy()
+    _check(load().utr_threshold(out.ctypes.data, out.size, float(background), float(threshold)))
+    return out
+
+
+_RESLICE_AXES = [(0, 1, 2), (0, 2, 1), (1, 2, 0), (1, 0, 2), (2, 0, 1), (2, 1, 0)]
+
+
+def reslice(vol, mode, flipx=False, flipy=False, flipz=False):
+    """Reslice with flips on the GPU (parameters 29-30, UTR.cpp:1449-1453) of
+    a (nz, ny, nx) volume; mode 0 XY .. 5 ZY."""
+    vol = np.ascontiguousarray(vol, dtype=np.float32)
+    nz, ny, nx = vol.shape
+    n = (nx, ny, nz)
+    a = _RESLICE_AXES[mode]
+    out = np.empty((n[a[2]], n[a[1]], n[a[0]]), np.float32)
+    _check(load().utr_reslice(vol.ctypes.data, out.ctypes.data, nx, ny, nz, mode, int(flipx), int(flipy),
+                              int(flipz)))
+    return out
+
+
+def thick_slices(vol, mode, step):
+    """Thick MIP (1) or average (2) slices over step slices on the GPU
+    (parameter 31, UTR.cpp:1456-1490)."""
+    vol = np.ascontiguousarray(vol, dtype=np.float32)
+    nz, ny, nx = vol.shape
+    out = np.empty((max(nz // max(step, 1), 0), ny, nx), np.float32)
+    _check(load().utr_thick_slices(vol.ctypes.data, out.ctypes.data, nx, ny, nz, mode, step))
+    return out
+
+
+def convert16(vol, minin, maxin, maxout):
+    """16-bit output on the GPU (parameter 27, UTR.cpp:1494-1498)."""
+    vol = np.ascontiguousarray(vol, dtype=np.float32)
+    out = np.empty(vol.shape, np.uint16)
+    _check(load().utr_convert16(vol.ctypes.data, out.ctypes.data, vol.size, float(minin), float(maxin),
+                                int(maxout)))
     return out
 
 

@@ -182,6 +182,73 @@ __global__ void k_post_scale(float* __restrict__ v, float w, size_t n) {
     if (t < n) v[t] *= w;
 }
 
+// ThresholdLow(background, threshold) of UTR.cpp:1383: v - background, then
+// values below threshold set to it (the comparison leaves NaN as NaN).
+__global__ void k_post_threshold(float* __restrict__ v, size_t n, float a, float b) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    float x = v[t] - a;
+    if (x < b) x = b;
+    v[t] = x;
+}
+
+// Reslice with axis flips (UTR.cpp:1449-1453): output slices mode 0 XY, 1 XZ,
+// 2 YZ, 3 YX, 4 ZX, 5 ZY (first letter along a slice's rows, second down the
+// slice, slices stacked along the remaining axis); a flip reverses an input
+// axis first.  One thread per output voxel; reads gather, writes coalesce.
+__global__ void k_post_reslice(const float* __restrict__ in, float* __restrict__ out, int nx, int ny, int nz,
+                               int a0, int a1, int a2, int fx, int fy, int fz) {
+    const int n[3] = {nx, ny, nz};
+    const int ox = n[a0], oy = n[a1];
+    const size_t total = (size_t)nx * ny * nz;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const int i = (int)(t % ox), j = (int)((t / ox) % oy), k = (int)(t / ((size_t)ox * oy));
+    int src[3];
+    src[a0] = i;
+    src[a1] = j;
+    src[a2] = k;
+    if (fx) src[0] = nx - 1 - src[0];
+    if (fy) src[1] = ny - 1 - src[1];
+    if (fz) src[2] = nz - 1 - src[2];
+    out[t] = in[((size_t)src[2] * ny + src[1]) * nx + src[0]];
+}
+
+// Thick slices (UTR.cpp:1456-1490): slice k1 of the nz / step output is the
+// MIP (mode 1) or the average (mode 2, float sum in slice order, then the
+// division) of input slices k1 step .. k1 step + step - 1.
+__global__ void k_post_thick(const float* __restrict__ in, float* __restrict__ out, int nx, int ny, int nzt,
+                             int mode, int step) {
+    const size_t plane = (size_t)nx * ny;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= plane * nzt) return;
+    const size_t p = t % plane;
+    const size_t k1 = t / plane;
+    const float* s = in + k1 * step * plane + p;
+    float v = s[0];
+    if (mode == 1) {
+        for (int m = 1; m < step; m++) {
+            const float w = s[(size_t)m * plane];
+            if (w > v) v = w;
+        }
+    } else {
+        for (int m = 1; m < step; m++) v += s[(size_t)m * plane];
+        v /= (float)step;
+    }
+    out[t] = v;
+}
+
+// 16-bit output (UTR.cpp:1494-1498, the absent Convert restated): values
+// clamped to [minin, maxin], mapped linearly onto [0, maxout], rounded half
+// away from zero (FP64).
+__global__ void k_post_u16(const float* __restrict__ in, unsigned short* __restrict__ out, size_t n, double minin,
+                           double maxin, double scale) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const double x = fmin(fmax((double)in[t], minin), maxin);
+    out[t] = (unsigned short)round((x - minin) * scale);
+}
+
 }  // namespace utr
 
 namespace {
@@ -479,6 +546,112 @@ int utr_highpass3d_device(float* d_vol, int nx, int ny, int nz, double xst, doub
                           double width) {
     if (const int rc = highpass_check(d_vol, nx, ny, nz, xst, yst, zst, width)) return rc;
     return post_fourier(d_vol, nx, ny, nz, HighpassFilter(nx, ny, nz, xst, yst, zst, width), true);
+}
+
+int utr_threshold_device(float* d_vol, long long n, float background, float threshold) {
+    if (!d_vol || n < 0) return set_err(UTR_E_INVALID, "utr_threshold_device: null volume or negative size");
+    if (n == 0) return UTR_OK;
+    utr::k_post_threshold<<<(unsigned)(((size_t)n + 255) / 256), 256>>>(d_vol, (size_t)n, background, threshold);
+    UTR_HIP_TRY(hipGetLastError());
+    UTR_HIP_TRY(hipDeviceSynchronize());
+    return UTR_OK;
+}
+
+int utr_reslice_device(const float* d_in, float* d_out, int nx, int ny, int nz, int mode, int flipx, int flipy,
+                       int flipz) {
+    if (int rc = check_vol(d_in, nx, ny, nz)) return rc;
+    if (!d_out || d_out == d_in) return set_err(UTR_E_INVALID, "utr_reslice_device: distinct output needed");
+    if (mode < 0 || mode > 5) return set_err(UTR_E_INVALID, "utr_reslice_device: mode must be 0..5");
+    static const int axes[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 2, 0}, {1, 0, 2}, {2, 0, 1}, {2, 1, 0}};
+    const int* a = axes[mode];
+    const size_t n = (size_t)nx * ny * nz;
+    utr::k_post_reslice<<<(unsigned)((n + 255) / 256), 256>>>(d_in, d_out, nx, ny, nz, a[0], a[1], a[2], flipx != 0,
+                                                             flipy != 0, flipz != 0);
+    UTR_HIP_TRY(hipGetLastError());
+    UTR_HIP_TRY(hipDeviceSynchronize());
+    return UTR_OK;
+}
+
+int utr_thick_slices_device(const float* d_in, float* d_out, int nx, int ny, int nz, int mode, int step) {
+    if (int rc = check_vol(d_in, nx, ny, nz)) return rc;
+    if (!d_out || d_out == d_in) return set_err(UTR_E_INVALID, "utr_thick_slices_device: distinct output needed");
+    if (mode != 1 && mode != 2) return set_err(UTR_E_INVALID, "utr_thick_slices_device: mode must be 1 (MIP) or 2 (average)");
+    if (step < 1) return set_err(UTR_E_INVALID, "utr_thick_slices_device: the slice thickness must be at least one voxel");
+    if (step > nz) return set_err(UTR_E_INVALID, "Slice thickness is larger than the sample thickness.");  // UTR.cpp:1460
+    const int nzt = nz / step;
+    const size_t n = (size_t)nx * ny * nzt;
+    utr::k_post_thick<<<(unsigned)((n + 255) / 256), 256>>>(d_in, d_out, nx, ny, nzt, mode, step);
+    UTR_HIP_TRY(hipGetLastError());
+    UTR_HIP_TRY(hipDeviceSynchronize());
+    return UTR_OK;
+}
+
+int utr_convert16_device(const float* d_in, unsigned short* d_out, long long n, float minin, float maxin,
+                         long long maxout) {
+    if (!d_in || !d_out || n < 0) return set_err(UTR_E_INVALID, "utr_convert16_device: null buffers or negative size");
+    if (!(maxin > minin) || maxout < 1 || maxout > 65535)
+        return set_err(UTR_E_INVALID, "utr_convert16_device: need min-in < max-in and 1 <= max-out <= 65535");
+    if (n == 0) return UTR_OK;
+    const double scale = (double)maxout / ((double)maxin - (double)minin);
+    utr::k_post_u16<<<(unsigned)(((size_t)n + 255) / 256), 256>>>(d_in, d_out, (size_t)n, (double)minin,
+                                                                  (double)maxin, scale);
+    UTR_HIP_TRY(hipGetLastError());
+    UTR_HIP_TRY(hipDeviceSynchronize());
+    return UTR_OK;
+}
+
+// Host-memory forms: one upload, the kernel, one download.
+int utr_threshold(float* vol, long long n, float background, float threshold) {
+    if (!vol || n < 0) return set_err(UTR_E_INVALID, "utr_threshold: null volume or negative size");
+    if (n == 0) return UTR_OK;
+    LocalBuf<float> d;
+    UTR_HIP_TRY(d.alloc((size_t)n));
+    UTR_HIP_TRY(hipMemcpy(d.p, vol, sizeof(float) * n, hipMemcpyHostToDevice));
+    if (int rc = utr_threshold_device(d.p, n, background, threshold)) return rc;
+    UTR_HIP_TRY(hipMemcpy(vol, d.p, sizeof(float) * n, hipMemcpyDeviceToHost));
+    return UTR_OK;
+}
+
+int utr_reslice(const float* in, float* out, int nx, int ny, int nz, int mode, int flipx, int flipy, int flipz) {
+    if (int rc = check_vol(in, nx, ny, nz)) return rc;
+    if (!out) return set_err(UTR_E_INVALID, "utr_reslice: null output");
+    const size_t n = (size_t)nx * ny * nz;
+    LocalBuf<float> a, b;
+    UTR_HIP_TRY(a.alloc(n));
+    UTR_HIP_TRY(b.alloc(n));
+    UTR_HIP_TRY(hipMemcpy(a.p, in, sizeof(float) * n, hipMemcpyHostToDevice));
+    if (int rc = utr_reslice_device(a.p, b.p, nx, ny, nz, mode, flipx, flipy, flipz)) return rc;
+    UTR_HIP_TRY(hipMemcpy(out, b.p, sizeof(float) * n, hipMemcpyDeviceToHost));
+    return UTR_OK;
+}
+
+int utr_thick_slices(const float* in, float* out, int nx, int ny, int nz, int mode, int step) {
+    if (int rc = check_vol(in, nx, ny, nz)) return rc;
+    if (!out) return set_err(UTR_E_INVALID, "utr_thick_slices: null output");
+    if (mode != 1 && mode != 2) return set_err(UTR_E_INVALID, "utr_thick_slices: mode must be 1 (MIP) or 2 (average)");
+    if (step < 1) return set_err(UTR_E_INVALID, "utr_thick_slices: the slice thickness must be at least one voxel");
+    if (step > nz) return set_err(UTR_E_INVALID, "Slice thickness is larger than the sample thickness.");  // UTR.cpp:1460
+    const size_t n = (size_t)nx * ny * nz, no = (size_t)nx * ny * (nz / step);
+    LocalBuf<float> a, b;
+    UTR_HIP_TRY(a.alloc(n));
+    UTR_HIP_TRY(b.alloc(no));
+    UTR_HIP_TRY(hipMemcpy(a.p, in, sizeof(float) * n, hipMemcpyHostToDevice));
+    if (int rc = utr_thick_slices_device(a.p, b.p, nx, ny, nz, mode, step)) return rc;
+    UTR_HIP_TRY(hipMemcpy(out, b.p, sizeof(float) * no, hipMemcpyDeviceToHost));
+    return UTR_OK;
+}
+
+int utr_convert16(const float* in, unsigned short* out, long long n, float minin, float maxin, long long maxout) {
+    if (!in || !out || n < 0) return set_err(UTR_E_INVALID, "utr_convert16: null buffers or negative size");
+    if (n == 0) return UTR_OK;
+    LocalBuf<float> a;
+    LocalBuf<unsigned short> b;
+    UTR_HIP_TRY(a.alloc((size_t)n));
+    UTR_HIP_TRY(b.alloc((size_t)n));
+    UTR_HIP_TRY(hipMemcpy(a.p, in, sizeof(float) * n, hipMemcpyHostToDevice));
+    if (int rc = utr_convert16_device(a.p, b.p, n, minin, maxin, maxout)) return rc;
+    UTR_HIP_TRY(hipMemcpy(out, b.p, sizeof(unsigned short) * n, hipMemcpyDeviceToHost));
+    return UTR_OK;
 }
 
 int utr_highpass3d_device_noalloc(float* d_vol, int nx, int ny, int nz, double xst, double yst,

@@ -89,6 +89,13 @@ void write_stack(const Config& c, const Volume& v, const std::vector<std::string
         if (!found) printf("\nWARNING: unable to open UTR_DICOM.txt file; using default values for DICOM tags!");
     }
     const double spacing_mm = c.modality == 0 ? c.xst * 1e-7 : c.xst * 1e-3;
+    // 16-bit output (UTR.cpp:1494-1498), converted on the GPU
+    std::vector<uint16_t> v16;
+    if (e != ".GRD" && c.out16()) {
+        v16.resize(v.v.size());
+        check(utr_convert16(v.v.data(), v16.data(), (long long)v.v.size(), (float)c.b16_minin, (float)c.b16_maxin,
+                            (long long)c.b16_maxout));
+    }
     std::string err;
 #pragma omp parallel for schedule(dynamic)
     for (long k = 0; k < v.nz; k++) {
@@ -98,10 +105,9 @@ void write_stack(const Config& c, const Volume& v, const std::vector<std::string
                 write_grd(names[k], v.nx, v.ny, s, ext[axes[0]][0], ext[axes[0]][1], ext[axes[1]][0],
                           ext[axes[1]][1]);
             } else if (c.out16()) {
-                std::vector<uint16_t> u(n2);
-                to_u16(s, n2, c.b16_minin, c.b16_maxin, c.b16_maxout, u.data());
-                if (e == ".DCM") write_dicom_u16(names[k], v.nx, v.ny, u.data(), tags, k, spacing_mm);
-                else write_tiff_u16(names[k], v.nx, v.ny, u.data());
+                const uint16_t* u = v16.data() + k * n2;
+                if (e == ".DCM") write_dicom_u16(names[k], v.nx, v.ny, u, tags, k, spacing_mm);
+                else write_tiff_u16(names[k], v.nx, v.ny, u);
             } else {
                 write_tiff_f32(names[k], v.nx, v.ny, s);
             }
@@ -430,7 +436,7 @@ int run(int argc, char** argv) {
     float vmin = vol.v.empty() ? 0.f : *std::min_element(vol.v.begin(), vol.v.end());
     if (c.background != 0 || vmin < c.threshold) {
         printf("\nThresholding the reconstructed 3D object ...");
-        threshold_low(vol, (float)c.background, (float)c.threshold);
+        check(utr_threshold(vol.v.data(), (long long)vol.v.size(), (float)c.background, (float)c.threshold));
     }
     if (c.rot_out[0] != 0 || c.rot_out[1] != 0 || c.rot_out[2] != 0) {
         // output rotation (UTR.cpp:656-661, 1387-1394): the angles as written, negated
@@ -514,11 +520,28 @@ int run(int argc, char** argv) {
     static const int axes_of[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 2, 0}, {1, 0, 2}, {2, 0, 1}, {2, 1, 0}};
     if (c.reslice || c.flip[0] || c.flip[1] || c.flip[2]) {
         printf("\nReslicing the output 3D volume ...");
-        vol = reslice(vol, c.reslice, c.flip[0], c.flip[1], c.flip[2]);
+        const long n[3] = {vol.nx, vol.ny, vol.nz};
+        const int* a = axes_of[c.reslice];
+        Volume r;
+        r.nx = n[a[0]];
+        r.ny = n[a[1]];
+        r.nz = n[a[2]];
+        r.v.resize(vol.v.size());
+        check(utr_reslice(vol.v.data(), r.v.data(), (int)vol.nx, (int)vol.ny, (int)vol.nz, c.reslice, c.flip[0],
+                          c.flip[1], c.flip[2]));
+        vol = std::move(r);
     }
     if (c.thick_mode > 0) {
         printf("\nCreating thick output slices ...");
-        vol = thick_slices(vol, c.thick_mode, (int)(c.thick_UL / c.xst + 0.5));
+        const int step = (int)(c.thick_UL / c.xst + 0.5);
+        if (step > vol.nz) throw std::runtime_error("Slice thickness is larger than the sample thickness.");
+        Volume t;
+        t.nx = vol.nx;
+        t.ny = vol.ny;
+        t.nz = step > 0 ? vol.nz / step : 0;
+        t.v.resize((size_t)t.nx * t.ny * t.nz);
+        check(utr_thick_slices(vol.v.data(), t.v.data(), (int)vol.nx, (int)vol.ny, (int)vol.nz, c.thick_mode, step));
+        vol = std::move(t);
     }
     const double ext[3][2] = {{xlo, xhi}, {ylo, yhi}, {zlo, zhi}};
     auto outnames = file_names(out_template, vol.nz);

@@ -1082,6 +1082,67 @@ extern "C" int utr_io_write_slice(const char* path, long nx, long ny, const floa
     }
 }
 
+extern "C" int utr_io_threshold(float* vol, long n, float background, float threshold) {
+    try {
+        if (!vol || n < 0) throw std::runtime_error("utr_io_threshold: null volume or negative size");
+        utrio::Volume v;
+        v.nx = n;
+        v.ny = v.nz = 1;
+        v.v.assign(vol, vol + n);
+        utrio::threshold_low(v, background, threshold);
+        std::copy(v.v.begin(), v.v.end(), vol);
+        return 0;
+    } catch (std::exception& ex) {
+        return io_fail(ex);
+    }
+}
+
+extern "C" int utr_io_reslice(const float* in, float* out, long nx, long ny, long nz, int mode, int flipx,
+                              int flipy, int flipz) {
+    try {
+        if (!in || !out || nx <= 0 || ny <= 0 || nz <= 0 || mode < 0 || mode > 5)
+            throw std::runtime_error("utr_io_reslice: bad arguments");
+        utrio::Volume v;
+        v.nx = nx;
+        v.ny = ny;
+        v.nz = nz;
+        v.v.assign(in, in + (size_t)nx * ny * nz);
+        const utrio::Volume r = utrio::reslice(v, mode, flipx != 0, flipy != 0, flipz != 0);
+        std::copy(r.v.begin(), r.v.end(), out);
+        return 0;
+    } catch (std::exception& ex) {
+        return io_fail(ex);
+    }
+}
+
+extern "C" int utr_io_thick_slices(const float* in, float* out, long nx, long ny, long nz, int mode, int step) {
+    try {
+        if (!in || !out || nx <= 0 || ny <= 0 || nz <= 0 || (mode != 1 && mode != 2) || step < 1)
+            throw std::runtime_error("utr_io_thick_slices: bad arguments");
+        utrio::Volume v;
+        v.nx = nx;
+        v.ny = ny;
+        v.nz = nz;
+        v.v.assign(in, in + (size_t)nx * ny * nz);
+        const utrio::Volume r = utrio::thick_slices(v, mode, step);
+        std::copy(r.v.begin(), r.v.end(), out);
+        return 0;
+    } catch (std::exception& ex) {
+        return io_fail(ex);
+    }
+}
+
+extern "C" int utr_io_convert16(const float* in, unsigned short* out, long n, float minin, float maxin,
+                                long maxout) {
+    try {
+        if (!in || !out || n < 0) throw std::runtime_error("utr_io_convert16: bad arguments");
+        utrio::to_u16(in, (size_t)n, minin, maxin, maxout, out);
+        return 0;
+    } catch (std::exception& ex) {
+        return io_fail(ex);
+    }
+}
+
 extern "C" int utr_io_file_name(const char* tmpl, long count, long index, char* out, long cap) {
     try {
         if (index < 0 || index >= count) throw std::runtime_error("Error: file index out of range.");
