@@ -314,8 +314,9 @@ int utr_batch_destroy(utr_batch* b);
  * DHT.cpp:1010-1055), both followed by the real-space rotate-and-splat of
  * DHT.cpp:1081-1164 and the scale of DHT.cpp:1325-1340; method 5 (sCTF on the
  * Ewald sphere with the Filt3 filter, DHT.cpp:1056-1067, 1195-1416).
- * Methods 1 and 2 (IWFR, CTFL2: several defocus distances per orientation)
- * are UTR_E_UNSUPPORTED.
+ * Methods 1 and 2 (IWFR and CTFL2, DHT.cpp:889-917) retrieve the complex
+ * amplitude of each orientation from its defocus series and then follow
+ * method 3's back-propagation: create them with dht_create_series.
  */
 typedef struct dht_params {
     int nx, ny;              /* 6: image width/height (powers of 2; even for method 5) */
@@ -327,13 +328,15 @@ typedef struct dht_params {
     double aobj_mrad;        /* 11: objective aperture                       */
     double cs3_mm, cs5_mm;   /* 12: spherical aberrations                    */
     double asigma;           /* 13: beta/delta (methods 4, 5)                */
-    int method;              /* 14: 3, 4 or 5                                */
+    int method;              /* 14: 1 to 5                                   */
     int escc;                /* 15: Ewald sphere curvature (per-plane back-propagation) */
     double critical_snr;     /* 17: method 5                                 */
-    double epsilon;          /* 18: Fact (method 3) or Tikhonov parameter    */
+    double epsilon;          /* 18: IWFR stopping error (method 1), Fact
+                                (method 3) or Tikhonov parameter (2, 4, 5)   */
     int inv_laplace;         /* 23: inverse 3D Laplacian (method 5)          */
     double alpha;            /* 24: its regularisation parameter             */
     double norm_f1, norm_f2; /* 9: input -> input/f1 + f2                    */
+    int itermax;             /* 17: method 1, maximal number of iterations   */
 } dht_params;
 
 typedef struct dht_ctx dht_ctx;
@@ -345,6 +348,31 @@ typedef struct dht_ctx dht_ctx;
 int dht_create(const dht_params* params, int nrows, const double* angZ_deg,
                const double* angY_deg, const double* defocus, const float* weights,
                dht_ctx** out);
+
+/* Methods 1 and 2: nangles orientations with counts[a] >= 2 images each
+ * (the rows of the defocus/orientations file with several defocus entries,
+ * DHT.cpp:142-152, 272-283).  defocus holds one distance per image, in
+ * orientation order (sum of counts entries); after this call the row index
+ * of dht_set_image_correction and dht_add_images[_device] is the image
+ * index in that order, so each image has its own Z'' angle (repeat an
+ * orientation's RELION shift for its images).  IWFR retrieves the wave at
+ * the mean distance of the series (DHT.cpp:895-897), CTFL2 at its first
+ * distance (DHT.cpp:908).  Methods 3-5 with every count 1 behave as
+ * dht_create. */
+int dht_create_series(const dht_params* params, int nangles, const double* angZ_deg,
+                      const double* angY_deg, const int* counts, const double* defocus,
+                      const float* weights, dht_ctx** out);
+
+/* Method 1: the IWFR iterations each orientation took in the last
+ * dht_finalize (nangles ints; 0 for method 2).  No counterpart in the
+ * reference, which prints them when verbose. */
+int dht_get_iterations(const dht_ctx* ctx, int* iters);
+
+/* Methods 1 and 2, after dht_finalize: the retrieved complex amplitudes of
+ * orientations [first, first+count) at their zout, count*ny*nx (re, im)
+ * float pairs -- the data of the GRC files DHT.cpp:902, 916 write when
+ * parameter 16 is 1. */
+int dht_get_waves(dht_ctx* ctx, int first, int count, float* psi);
 
 /* Z'' angles (degrees) and RELION image shifts (Angstrom) per row, undone on
  * each renormalised input image before its 2D stage (the Move / Rotate block

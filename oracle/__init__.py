@@ -385,7 +385,7 @@ DHT_FIELDS = [
     ("cs5_mm", ctypes.c_double), ("asigma", ctypes.c_double), ("method", ctypes.c_int),
     ("escc", ctypes.c_int), ("critical_snr", ctypes.c_double), ("epsilon", ctypes.c_double),
     ("inv_laplace", ctypes.c_int), ("alpha", ctypes.c_double),
-    ("norm_f1", ctypes.c_double), ("norm_f2", ctypes.c_double),
+    ("norm_f1", ctypes.c_double), ("norm_f2", ctypes.c_double), ("itermax", ctypes.c_int),
 ]
 
 
@@ -404,6 +404,10 @@ def _dht_lib():
                                      ctypes.POINTER(ctypes.c_int)]
         L.ora_dht_splat.argtypes = [P, dp, ctypes.c_int, ctypes.c_double, ctypes.c_double, dp]
         L.ora_dht_reconstruct.argtypes = [P, ctypes.c_int, dp, dp, dp, fp, fp, dp, ctypes.c_int]
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.ora_dht_retrieve.argtypes = [P, ctypes.c_int, fp, dp, dp, dp, ip]
+        L.ora_dht_planes_wave.argtypes = [P, dp, ctypes.c_double, ctypes.c_double, dp, ip]
+        L.ora_dht_reconstruct_series.argtypes = [P, ctypes.c_int, dp, dp, ip, dp, fp, fp, dp, ctypes.c_int]
         L._dht_ready = True
     return L
 
@@ -411,7 +415,7 @@ def _dht_lib():
 def make_dht_params(d):
     p = OraDhtParams()
     for name, _ in DHT_FIELDS:
-        setattr(p, name, d[name])
+        setattr(p, name, d.get(name, 0) if name == "itermax" else d[name])
     return p
 
 
@@ -447,21 +451,61 @@ def dht_splat(params, planes, angZ_deg, angY_deg, K3out=None):
     return K3out
 
 
+def dht_retrieve(params, images, defocus):
+    """Methods 1, 2: (psi complex (ny, nx) at zout, zout, IWFR iterations) of
+    one orientation's defocus series images (cnt, ny, nx)."""
+    p = make_dht_params(params)
+    images = np.ascontiguousarray(images, dtype=np.float32)
+    z = np.ascontiguousarray(defocus, dtype=np.float64)
+    assert images.ndim == 3 and images.shape[0] == len(z)
+    psi = np.zeros(images.shape[1:] + (2,))
+    zout, it = ctypes.c_double(), ctypes.c_int()
+    if _dht_lib().ora_dht_retrieve(ctypes.byref(p), len(z), _fp(images), _dp(z), _dp(psi),
+                                   ctypes.byref(zout), ctypes.byref(it)):
+        raise OracleError(lib().ora_last_error().decode())
+    return psi[..., 0] + 1j * psi[..., 1], zout.value, it.value
+
+
+def dht_planes_wave(params, psi, zout, weight=1.0):
+    """Plane stack of one orientation from its complex amplitude at zout
+    (methods 1-3)."""
+    p = make_dht_params(params)
+    nz = dht_nz(params)
+    psi = np.asarray(psi)
+    w = np.ascontiguousarray(np.stack([psi.real, psi.imag], -1), dtype=np.float64)
+    out = np.zeros((nz,) + psi.shape)
+    n = ctypes.c_int()
+    if _dht_lib().ora_dht_planes_wave(ctypes.byref(p), _dp(w), zout, weight, _dp(out), ctypes.byref(n)):
+        raise OracleError(lib().ora_last_error().decode())
+    return out[:n.value].copy()
+
+
 def dht_reconstruct(params, rows, images, weights=None, nthreads=1):
+    """rows: angZ, angY per orientation; defocus (and angZ2, shift_x,
+    shift_y) per image; counts (optional): images per orientation."""
     p = make_dht_params(params)
     angZ = np.ascontiguousarray(rows["angZ"], dtype=np.float64)
     angY = np.ascontiguousarray(rows["angY"], dtype=np.float64)
     defoc = np.ascontiguousarray(rows["defocus"], dtype=np.float64)
     images = np.ascontiguousarray(images, dtype=np.float32)
     if any(np.any(np.asarray(rows.get(k, 0.0)) != 0) for k in ("angZ2", "shift_x", "shift_y")):
-        # DHT.cpp:862-880: the same correction as UTR.cpp:960-977
-        images = np.ascontiguousarray(prep_images(params, rows, images))
+        # DHT.cpp:862-880: the same correction as UTR.cpp:960-977, per image
+        z = np.zeros(len(defoc))
+        per_image = {"angZ": z, "angY": z, "defocus": defoc}
+        for k in ("angZ2", "shift_x", "shift_y"):
+            if rows.get(k) is not None:
+                per_image[k] = rows[k]
+        images = np.ascontiguousarray(prep_images(params, per_image, images))
     nz = dht_nz(params)
     vol = np.zeros((nz,) + images.shape[1:])
     w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float32)
-    if _dht_lib().ora_dht_reconstruct(ctypes.byref(p), len(angZ), _dp(angZ), _dp(angY), _dp(defoc),
-                                      _fp(w) if w is not None else None, _fp(images), _dp(vol),
-                                      nthreads):
+    counts = rows.get("counts")
+    c = None if counts is None else np.ascontiguousarray(counts, dtype=np.int32)
+    if _dht_lib().ora_dht_reconstruct_series(ctypes.byref(p), len(angZ), _dp(angZ), _dp(angY),
+                                             c.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+                                             if c is not None else None, _dp(defoc),
+                                             _fp(w) if w is not None else None, _fp(images), _dp(vol),
+                                             nthreads):
         raise OracleError(lib().ora_last_error().decode())
     return vol
 

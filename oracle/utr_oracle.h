@@ -166,6 +166,12 @@ void ora_last_timings(double* t4);
  *                      weak-object model it inverts and by the multislice
  *                      phase of a thin specimen it recovers in the plane
  *                      through the specimen (tests/test_filter_physics.py).
+ *   ora_dht_retrieve   methods 1 and 2: IWFR and the multi-distance CTF
+ *                      least squares, restated from their publications (the
+ *                      absent XA_IWFR::Iwfr and ::CTFL2) -- PARITY UNPINNED,
+ *                      pinned by the physics: both recover the exit wave of
+ *                      a specimen simulated by the pinned multislice at
+ *                      several defocus distances (tests/test_series_oracle.py).
  *   ora_dht_reconstruct method 5 reuses ora_spectrum (the mode 3 restatement,
  *                      pinned as above, for the absent InvertCTF_DT1) and ora_grid,
  *                      with the Filt3 filter, Hermitian fix, c2r and scale of
@@ -181,13 +187,14 @@ typedef struct ora_dht_params {
     double aobj_mrad;      /* param 11 */
     double cs3_mm, cs5_mm; /* param 12 */
     double asigma;         /* param 13 */
-    int method;            /* param 14: 3, 4 or 5 */
+    int method;            /* param 14: 1 to 5 */
     int escc;              /* param 15 */
     double critical_snr;   /* param 17 (method 5) */
     double epsilon;        /* param 18 */
     int inv_laplace;       /* param 23 (method 5) */
     double alpha;          /* param 24 */
     double norm_f1, norm_f2; /* param 9 */
+    int itermax;           /* param 17 (method 1) */
 } ora_dht_params;
 
 /* Output plane stack of one orientation (DHT.cpp:957-1055): method 3 or 4,
@@ -207,6 +214,24 @@ int ora_dht_splat(const ora_dht_params* p, const double* planes, int nplanes,
 int ora_dht_reconstruct(const ora_dht_params* p, int nrows, const double* angZ_deg,
                         const double* angY_deg, const double* defocus, const float* weights,
                         const float* images, double* vol, int nthreads);
+
+/* Methods 1 and 2 (DHT.cpp:889-917): the complex amplitude psi (2*ny*nx
+ * doubles, re/im) at zout retrieved from the cnt images of one orientation
+ * at distances defocus[0..cnt): IWFR at zout = the mean distance (iters: the
+ * iterations done), CTFL2 at zout = defocus[0]. */
+int ora_dht_retrieve(const ora_dht_params* p, int cnt, const float* imgs, const double* defocus,
+                     double* psi, double* zout, int* iters);
+
+/* Methods 1-3: output planes of one orientation from a complex amplitude
+ * psi at zout (the back-propagation of DHT.cpp:957-1006). */
+int ora_dht_planes_wave(const ora_dht_params* p, const double* psi, double zout, double weight,
+                        double* planes, int* nplanes);
+
+/* Whole reconstruction with counts[a] images per orientation (NULL: one);
+ * defocus and images run over all images in orientation order. */
+int ora_dht_reconstruct_series(const ora_dht_params* p, int nangles, const double* angZ_deg,
+                               const double* angY_deg, const int* counts, const double* defocus,
+                               const float* weights, const float* images, double* vol, int nthreads);
 
 /* Volume depth of a DHT reconstruction (DHT.cpp:379). */
 int ora_dht_nz(const ora_dht_params* p);

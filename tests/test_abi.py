@@ -71,7 +71,8 @@ def test_dht_param_struct_layout_matches_oracle():
 
 
 @pytest.mark.parametrize("bad,msg", [
-    (dict(method=1), "IWFR and CTFL2"),
+    (dict(method=1), "incompatibe with a single defocus distance"),
+    (dict(method=2), "incompatibe with a single defocus distance"),
     (dict(method=7), "unknown phase retrieval method"),
     (dict(nx=48, ny=48), "integer powers of 2"),
     (dict(method=4, dzextra=0.0, epsilon=0.0), "epsilon = 0"),
@@ -85,3 +86,24 @@ def test_dht_invalid_parameters_rejected(bad, msg):
     rows = dict(angZ=np.zeros(2), angY=np.array([0.0, 30.0]), defocus=np.full(2, 100.0))
     with pytest.raises(U.UTRError, match=msg):
         U.DhtReconstructor(p, rows)
+
+
+@pytest.mark.parametrize("bad,counts,msg", [
+    (dict(method=1, itermax=0), [2, 2], "maximal number of iterations"),
+    (dict(method=2, epsilon=0.0), [2, 2], "epsilon = 0"),
+    (dict(method=1, dzextra=0.0), [2, 2], "extra defocus = 0"),
+    (dict(method=1, asigma=0.1), [2, 2], "non-zero absorption coefficient"),
+    (dict(method=1), [1, 3], "incompatibe with a single defocus distance"),
+    (dict(method=2), [3, 1], "incompatibe with a single defocus distance"),
+    (dict(method=3), [2, 2], "incompatibe with multiple defocus distances"),
+    (dict(method=4, dzextra=0.0, epsilon=0.01), [1, 2], "incompatibe with multiple defocus distances"),
+])
+def test_dht_series_invalid_parameters_rejected(bad, counts, msg):
+    """DHT.cpp:272-283, 333-339: the defocus-series checks of methods 1, 2
+    and the single-image methods (no GPU call is reached)."""
+    p = U.default_dht_params(**bad)
+    rows = dict(angZ=np.zeros(2), angY=np.array([0.0, 30.0]), counts=counts,
+                defocus=np.full(sum(counts), 100.0))
+    with pytest.raises(U.UTRError, match=msg) as e:
+        U.DhtReconstructor(p, rows)
+    assert e.value.code == -1

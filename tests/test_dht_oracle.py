@@ -216,5 +216,8 @@ def test_sctf_is_the_ewald_gridding_with_the_filt3_filter():
 def test_invalid_parameters():
     with pytest.raises(oracle.OracleError, match="extra defocus = 0"):
         oracle.dht_nz(params(dzextra=0.0))
-    with pytest.raises(oracle.OracleError, match="IWFR"):
-        oracle.dht_nz(params(method=2))
+    imgs = np.ones((2, 16, 16), np.float32)
+    with pytest.raises(oracle.OracleError, match="single defocus distance"):
+        oracle.dht_reconstruct(params(method=2), _rows(2, 1), imgs)
+    with pytest.raises(oracle.OracleError, match="multiple defocus distances"):
+        oracle.dht_reconstruct(params(method=3), dict(_rows(1, 1), counts=[2], defocus=[50.0, 60.0]), imgs)

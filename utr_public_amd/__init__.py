@@ -40,6 +40,7 @@ EXPORTS = [
     "utr_batch_create", "utr_batch_run", "utr_batch_job_form", "utr_batch_destroy",
     "dht_create", "dht_add_images", "dht_add_images_device", "dht_finalize", "dht_finalize_device",
     "dht_reset", "dht_dims", "dht_get_timings", "dht_destroy", "dht_set_image_correction",
+    "dht_create_series", "dht_get_iterations", "dht_get_waves",
     "utr_highpass3d", "utr_highpass3d_device", "utr_highpass3d_device_noalloc", "utr_rotate3d", "utr_rotate3d_device",
     "utr_symmetrize3d", "utr_symmetrize3d_device", "utr_invlaplace3d", "utr_invlaplace3d_device",
     "msct_create", "msct_images", "msct_images_device", "msct_slices", "msct_batch", "msct_destroy",
@@ -58,7 +59,7 @@ DHT_FIELDS = [
     ("cs5_mm", ctypes.c_double), ("asigma", ctypes.c_double), ("method", ctypes.c_int),
     ("escc", ctypes.c_int), ("critical_snr", ctypes.c_double), ("epsilon", ctypes.c_double),
     ("inv_laplace", ctypes.c_int), ("alpha", ctypes.c_double),
-    ("norm_f1", ctypes.c_double), ("norm_f2", ctypes.c_double),
+    ("norm_f1", ctypes.c_double), ("norm_f2", ctypes.c_double), ("itermax", ctypes.c_int),
 ]
 
 
@@ -121,7 +122,7 @@ def default_dht_params(**kw):
     d = dict(nx=64, ny=64, xst=0.4, zlo=-12.8, zhi=12.8, dzextra=100.0, dxc=0.0, dyc=0.0,
              dzc=0.0, wl=0.02507934, aobj_mrad=0.0, cs3_mm=0.0, cs5_mm=0.0, asigma=0.0,
              method=3, escc=1, critical_snr=0.5, epsilon=1.0, inv_laplace=0, alpha=0.0,
-             norm_f1=1.0, norm_f2=0.0)
+             norm_f1=1.0, norm_f2=0.0, itermax=200)
     d.update(kw)
     return d
 
@@ -191,6 +192,10 @@ def load():
     L.dht_get_timings.argtypes = [vp, ctypes.POINTER(UtrTimings)]
     L.dht_destroy.argtypes = [vp]
     L.dht_set_image_correction.argtypes = [vp, dp, dp, dp]
+    L.dht_create_series.argtypes = [ctypes.POINTER(DhtParams), ctypes.c_int, dp, dp,
+                                    ctypes.POINTER(ctypes.c_int), dp, vp, ctypes.POINTER(vp)]
+    L.dht_get_iterations.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
+    L.dht_get_waves.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
     d = ctypes.c_double
     i = ctypes.c_int
     for f in ("utr_highpass3d", "utr_highpass3d_device", "utr_highpass3d_device_noalloc"):
@@ -441,32 +446,43 @@ class Batch:
 
 
 class DhtReconstructor:
-    """One reconstruction of the DHT program's single-defocus methods (3, 4,
-    5; MultisliceCppTest/PhaseRetrieval/DHT.cpp) on the current HIP device.
-    rows: angZ, angY (degrees), defocus (Angstrom), optional weights, angZ2
-    (Z'' in degrees) and shift_x, shift_y (RELION image shifts, Angstrom)."""
+    """One reconstruction of the DHT program (MultisliceCppTest/
+    PhaseRetrieval/DHT.cpp) on the current HIP device.  rows: angZ, angY
+    (degrees) and optional weights per orientation; defocus (Angstrom),
+    angZ2 (Z'' in degrees) and shift_x, shift_y (RELION image shifts,
+    Angstrom) per input image; counts (optional): images per orientation,
+    the defocus series of methods 1 and 2 (dht_create_series)."""
 
     def __init__(self, params, rows):
         L = load()
         self._L = L
         self.params = dict(params)
+        self.params.setdefault("itermax", 0)
         self.angZ = np.ascontiguousarray(rows["angZ"], dtype=np.float64)
         self.angY = np.ascontiguousarray(rows["angY"], dtype=np.float64)
         self.defocus = np.ascontiguousarray(rows["defocus"], dtype=np.float64)
         self.weights = None
         if rows.get("weights") is not None:
             self.weights = np.ascontiguousarray(rows["weights"], dtype=np.float32)
+        self.counts = None
+        if rows.get("counts") is not None:
+            self.counts = np.ascontiguousarray(rows["counts"], dtype=np.int32)
+            assert self.counts.shape == self.angZ.shape and self.counts.sum() == len(self.defocus)
         p = DhtParams()
         for name, _ in DHT_FIELDS:
             setattr(p, name, self.params[name])
         ctx = ctypes.c_void_p()
-        _check(L.dht_create(ctypes.byref(p), len(self.angZ), _dp(self.angZ), _dp(self.angY),
-                            _dp(self.defocus),
-                            self.weights.ctypes.data if self.weights is not None else None,
-                            ctypes.byref(ctx)))
+        w = self.weights.ctypes.data if self.weights is not None else None
+        if self.counts is None:
+            _check(L.dht_create(ctypes.byref(p), len(self.angZ), _dp(self.angZ), _dp(self.angY),
+                                _dp(self.defocus), w, ctypes.byref(ctx)))
+        else:
+            _check(L.dht_create_series(ctypes.byref(p), len(self.angZ), _dp(self.angZ), _dp(self.angY),
+                                       self.counts.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                       _dp(self.defocus), w, ctypes.byref(ctx)))
         self._ctx = ctx
         if any(rows.get(k) is not None for k in ("angZ2", "shift_x", "shift_y")):
-            z = np.zeros(len(self.angZ))
+            z = np.zeros(len(self.defocus))
             cols = [np.ascontiguousarray(rows.get(k) if rows.get(k) is not None else z, dtype=np.float64)
                     for k in ("angZ2", "shift_x", "shift_y")]
             assert all(c.shape == z.shape for c in cols)
@@ -498,6 +514,20 @@ class DhtReconstructor:
         t = UtrTimings()
         _check(self._L.dht_get_timings(self._ctx, ctypes.byref(t)))
         return {k: getattr(t, k) for k, _ in t._fields_}
+
+    def iterations(self):
+        """Methods 1, 2: IWFR iterations per orientation of the last finalize."""
+        it = np.zeros(len(self.angZ), np.int32)
+        _check(self._L.dht_get_iterations(self._ctx, it.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        return it
+
+    def waves(self, first=0, count=None):
+        """Methods 1, 2, after finalize: the retrieved complex amplitudes
+        (count, ny, nx) at each orientation's zout."""
+        count = len(self.angZ) - first if count is None else count
+        out = np.empty((count,) + self.shape[1:] + (2,), dtype=np.float32)
+        _check(self._L.dht_get_waves(self._ctx, first, count, out.ctypes.data))
+        return out[..., 0] + 1j * out[..., 1]
 
     def close(self):
         if getattr(self, "_ctx", None):

@@ -1,6 +1,8 @@
-// dht.hip -- the single-defocus reconstructions of the DHT program on gfx950
+// dht.hip -- the reconstructions of the DHT program on gfx950
 // (MultisliceCppTest/PhaseRetrieval/DHT.cpp), behind the dht_* C ABI of
 // include/utr_hip.h.  Included by utr_api.hip (one translation unit).
+// Methods 1 and 2 retrieve each orientation's wave from a defocus series in
+// dht_series.hip and then take method 3's back-propagation path below.
 //
 // Methods 3 and 4 (real-space path):
 //   k_dht_rows / k_dht_cols    phase-retrieval input and forward 2D FFT of each
@@ -46,6 +48,18 @@ struct alignas(16) DhtRow {
     float un[3];
     float inv[9];  // inverse of the 3x3 map [ui uj un]
     int pad;
+};
+
+// Methods 1 and 2: one input image of an orientation's defocus series
+// (d = its distance minus the orientation's zout) and one orientation's
+// images first .. first+count-1.
+struct alignas(16) SerImg {
+    double d;
+    int angle;
+    int pad;
+};
+struct SerAng {
+    int first, count;
 };
 
 constexpr int DHT_BX = 16, DHT_BY = 8, DHT_BZ = 8;
@@ -169,7 +183,7 @@ __global__ void __launch_bounds__(256) k_dht_prop_rows(const cplx<float>* __rest
         const int m = k >= N + nb ? 1 : 0, kk = k - m * (N + nb);
         const int n = g.escc ? p0 + m : g.nz / 2;
         const double zn = g.zlodz + g.xst * n;  // voutdefocus[n], DHT.cpp:388
-        const double d = g.method == 3 ? zn - R.zout : R.zout - zn;
+        const double d = g.method != 4 ? zn - R.zout : R.zout - zn;  // 1-3: back-propagation
         const double f2 = pi * g.wl * d;
         const double f = kk < N ? dht_freq(kk, N, g.xst) : dht_freq(k0 + kk - N, ny, g.xst);
         double chi = f2 * (f * f);
@@ -715,7 +729,8 @@ struct dht_ctx {
     utr_ctx* sctf = nullptr;  // method 5
     // methods 3 and 4
     utr::DhtGeom g{};
-    int nrows = 0;
+    int nrows = 0;  // input images (one per orientation, or a defocus series each)
+    int nang = 0;   // orientations
     int batch = 1;
     double dnorm = 0;
     int device = 0;
@@ -736,7 +751,26 @@ struct dht_ctx {
     std::vector<utr::PrepRow> prep;
     DevBuf<utr::PrepRow> prep_d;
     DevBuf<float> img_prep, prep_aver;
+    // methods 1 and 2: the defocus series of every orientation
+    int series = 0;
+    int itermax = 0;
+    int ser_batch = 1;  // orientations per retrieval pass
+    std::vector<int> first, count;  // images of orientation a: first[a] .. first[a]+count[a]-1
+    std::vector<int> iters;         // IWFR iterations of each orientation in the last finalize
+    DevBuf<float> simg;             // the renormalised-on-read input images
+    DevBuf<cplx<float>> B, E;       // retrieval planes; per-image propagator factors
+    DevBuf<utr::SerImg> ser_img;
+    DevBuf<utr::SerAng> ser_ang;
+    DevBuf<double> ser_part;
+    DevBuf<int> ser_done, ser_iters, ser_active;
+    int waves_ready = 0;  // a finalize has retrieved the waves
+    float ret_ms = 0;     // retrieval time of the last finalize
 };
+
+namespace {
+int dht_series_add(dht_ctx* c, int first, int count, const float* d_img);
+int dht_series_retrieve(dht_ctx* c);
+}  // namespace
 
 namespace {
 
@@ -763,9 +797,7 @@ bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 
 // Validation of DHT.cpp:36-720 for the numeric path; "" if valid.
 std::string dht_validate(const dht_params* p) {
-    if (p->method == 1 || p->method == 2)
-        return "IWFR and CTFL2 (methods 1, 2) need several defocus distances per orientation; not supported";
-    if (p->method < 3 || p->method > 5) return "Error: unknown phase retrieval method in the input parameter file.";
+    if (p->method < 1 || p->method > 5) return "Error: unknown phase retrieval method in the input parameter file.";
     if (p->asigma != 0 && !(p->method == 4 || p->method == 5))
         return "Error: selected phase retrieval method is incompatibe non-zero absorption coefficient in the input parameter file.";
     if (p->method != 5) {
@@ -780,14 +812,15 @@ std::string dht_validate(const dht_params* p) {
     if (p->escc != 0 && p->escc != 1)
         return "Error: unknown value for Ewald sphere curvature correction switch in input parameter file.";
     if (p->norm_f1 == 0) return "The first normalization factor cannot be zero (leads to division by zero).";
+    if (p->method == 1 && p->itermax < 1) return "Error: the maximal number of iterations must be >= 1.";
     if (p->method == 5 && p->critical_snr < 0) return "Error: the critical SNR value must be non-negative.";
     if (p->epsilon < 0 && p->method != 3)
         return "Error: minimal phase error / regulalization parameter must be non-negative, unless -0.5LogAmp phase retrieval method is selected.";
-    if (p->method == 4 && p->epsilon == 0)
+    if ((p->method == 2 || p->method == 4) && p->epsilon == 0)
         return "Error: the estimated regularization parameter (epsilon = 0) is not supported; give epsilon > 0.";
     if (p->inv_laplace && p->alpha < 0)
         return "Error: regularization parameter for 3D Laplacian filter must be non-negative.";
-    if (p->dzextra == 0 && p->method == 3)
+    if (p->dzextra == 0 && p->method <= 3)
         return "Error: extra defocus = 0, which is unexpected for the selected reconstruction (phase retrieval) method.";
     if (p->dzextra != 0 && (p->method == 4 || p->method == 5))
         return "ERROR: extra defocus != 0, which is unexpected for the phase retrieval method selected above.";
@@ -880,6 +913,7 @@ int dht_create_sctf(const dht_params* p, int nrows, const double* angZ, const do
 }
 
 int dht_add_plain(dht_ctx* c, int first_row, int count, const float* d_img) {
+    if (c->series) return dht_series_add(c, first_row, count, d_img);
     const auto& g = c->g;
     UTR_HIP_TRY(hipEventRecord(c->ev[0], c->stream));
     bool ok = false;
@@ -956,8 +990,12 @@ int dht_run(dht_ctx* c, float* d_vol) {
     const auto& g = c->g;
     if (c->n_added != c->nrows)
         return set_err(UTR_E_STATE, "dht_finalize: " + std::to_string(c->nrows - c->n_added) +
-                                        " orientation rows have no image");
+                                        (c->series ? " input images are missing" : " orientation rows have no image"));
     const size_t N = (size_t)g.nx * g.ny * g.nz;
+    if (c->series) {
+        const int rc = dht_series_retrieve(c);
+        if (rc) return rc;
+    }
     UTR_HIP_TRY(hipEventRecord(c->ev[2], c->stream));
     utr::k_dht_fill<<<(unsigned)((N + 255) / 256), 256, 0, c->stream>>>(c->acc3.p, N);
     const long nbricks = (long)((g.nx + utr::DHT_BX - 1) / utr::DHT_BX) *
@@ -967,8 +1005,8 @@ int dht_run(dht_ctx* c, float* d_vol) {
     const char* form = std::getenv("UTR_GRID_FORM");
     const bool plane = c->all_z0 && !(form && std::strcmp(form, "brick") == 0);
     const long ntiles = (long)((g.nx + utr::DP_X - 1) / utr::DP_X) * ((g.nz + utr::DP_Z - 1) / utr::DP_Z);
-    for (int a0 = 0; a0 < c->nrows; a0 += c->batch) {
-        const int na = std::min(c->batch, c->nrows - a0);
+    for (int a0 = 0; a0 < c->nang; a0 += c->batch) {
+        const int na = std::min(c->batch, c->nang - a0);
         utr::k_dht_zero_u32<<<1, 64, 0, c->stream>>>(c->pmax.p);
         // per orientation: its plane stack's row pass into T, then the column
         // pass into its slot of P; T (one orientation) stays in the
@@ -1022,42 +1060,23 @@ int dht_run(dht_ctx* c, float* d_vol) {
     UTR_HIP_TRY(hipStreamSynchronize(c->stream));
     float ms = 0;
     UTR_HIP_TRY(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
-    c->tim.fft2d_ctf_ms = c->add_ms;
+    c->tim.fft2d_ctf_ms = c->add_ms + c->ret_ms;
     c->tim.backprop_ms = ms;
     c->tim.fft3d_ms = 0;
-    c->tim.total_ms = c->add_ms + ms;
+    c->tim.total_ms = c->add_ms + c->ret_ms + ms;
     return UTR_OK;
 }
 
 }  // namespace
 
-extern "C" {
+namespace {
 
-int dht_create(const dht_params* p, int nrows, const double* angZ_deg, const double* angY_deg,
-               const double* defocus, const float* weights, dht_ctx** out) {
-    if (!out) return set_err(UTR_E_INVALID, "null output pointer");
-    *out = nullptr;
-    if (!p) return set_err(UTR_E_INVALID, "null parameters");
-    if (nrows < 1 || !angZ_deg || !angY_deg || !defocus)
-        return set_err(UTR_E_INVALID, "Error: the orientation file contains no rows.");
-    std::string err = dht_validate(p);
-    if (!err.empty())
-        return set_err(p->method == 1 || p->method == 2 ? UTR_E_UNSUPPORTED : UTR_E_INVALID, err);
-    dht_ctx* c = new dht_ctx();
-    c->method = p->method;
-    if (p->method == 5) {
-        int rc = dht_create_sctf(p, nrows, angZ_deg, angY_deg, defocus, weights, c);
-        if (rc) {
-            delete c;
-            return rc;
-        }
-        *out = c;
-        return UTR_OK;
-    }
-    auto fail = [&](int rc) {
-        dht_destroy(c);
-        return rc;
-    };
+// Geometry, per-orientation tables and buffers of the real-space methods
+// (1 to 4) for nang orientations whose fields are defined at zout[a].  On
+// failure the caller destroys c.
+int dht_setup(dht_ctx* c, const dht_params* p, int nang, const double* angZ_deg, const double* angY_deg,
+              const double* zout, const float* weights) {
+    c->nang = nang;
     utr::DhtGeom& g = c->g;
     g.nx = p->nx; g.ny = p->ny; g.method = p->method; g.escc = p->escc;
     g.use_f1 = p->norm_f1 != 1.0; g.use_f2 = p->norm_f2 != 0.0;
@@ -1065,8 +1084,8 @@ int dht_create(const dht_params* p, int nrows, const double* angZ_deg, const dou
     g.xst = p->xst; g.wl = p->wl;
     g.k2maxo = std::pow(p->aobj_mrad * (double)0.001f / p->wl, 2.0);  // DHT.cpp:250
     const double cs3 = p->cs3_mm * 1.e+7, cs5 = p->cs5_mm * 1.e+7;
-    g.c3 = p->method == 3 ? -cs3 : cs3;  // iSign = -1 reverts the aberrations (DHT.cpp:962)
-    g.c5 = p->method == 3 ? -cs5 : cs5;
+    g.c3 = p->method != 4 ? -cs3 : cs3;  // iSign = -1 reverts the aberrations (DHT.cpp:962)
+    g.c5 = p->method != 4 ? -cs5 : cs5;
     g.asigma = p->asigma; g.epsilon = p->epsilon;
     g.xlo = -0.5 * p->xst * p->nx; g.ylo = -0.5 * p->xst * p->ny;
     const double xhi = g.xlo + p->xst * p->nx, yhi = g.ylo + p->xst * p->ny;
@@ -1075,7 +1094,7 @@ int dht_create(const dht_params* p, int nrows, const double* angZ_deg, const dou
     g.zlodz = zlo + p->dzextra;
     const double zhidz = zhi + p->dzextra;
     const long nz = (long)((zhidz - g.zlodz) / p->xst + 0.5);
-    if (nz <= 0) return fail(set_err(UTR_E_INVALID, "Error: number of output defocus planes must be positive."));
+    if (nz <= 0) return (set_err(UTR_E_INVALID, "Error: number of output defocus planes must be positive."));
     g.nz = (int)nz;
     g.nplanes = p->escc ? g.nz : 1;
     g.xc = (xhi + g.xlo) / 2.0 + p->dxc;
@@ -1083,50 +1102,48 @@ int dht_create(const dht_params* p, int nrows, const double* angZ_deg, const dou
     g.zc = (zhidz + g.zlodz) / 2.0 + p->dzc;
     g.nx2 = g.nx - 2; g.ny2 = g.ny - 2; g.nz2 = g.nz - 2;
     if (!fft_size_supported(g.nx) || !fft_size_supported(g.ny))
-        return fail(set_err(UTR_E_UNSUPPORTED, "image size not supported by the FFT kernels"));
-    c->nrows = nrows;
-    c->added.assign(nrows, 0);
+        return (set_err(UTR_E_UNSUPPORTED, "image size not supported by the FFT kernels"));
     // DHT.cpp:1325-1340
     double total = 0;
-    if (weights) for (int r = 0; r < nrows; r++) total += weights[r];
-    else total = nrows;
+    if (weights) for (int r = 0; r < nang; r++) total += weights[r];
+    else total = nang;
     const double EE = tem_energy(p->wl), dAtomWidth = 1.0, dGamma = 0.1;
-    if (p->method == 3) c->dnorm = EE * dGamma / dAtomWidth * (-p->wl / utr::kPi) / total;
+    if (p->method != 4) c->dnorm = EE * dGamma / dAtomWidth * (-p->wl / utr::kPi) / total;
     else c->dnorm = p->wl * EE / utr::kPi / (dAtomWidth * 8.0) / total;
     // per-orientation rows and tables
-    std::vector<utr::DhtRow> rows(nrows);
-    std::vector<double> xs((size_t)nrows * 2 * g.nx), ys((size_t)nrows * 2 * g.ny), zs((size_t)nrows * 2 * g.nz);
+    std::vector<utr::DhtRow> rows(nang);
+    std::vector<double> xs((size_t)nang * 2 * g.nx), ys((size_t)nang * 2 * g.ny), zs((size_t)nang * 2 * g.nz);
     c->all_z0 = 1;
-    for (int r = 0; r < nrows; r++) {
+    for (int r = 0; r < nang; r++) {
         if (angZ_deg[r] != 0.0 || g.nx > utr::DP_NXMAX) c->all_z0 = 0;  // x table staged in LDS
         std::memset(&rows[r], 0, sizeof rows[r]);
         dht_tables(g, angZ_deg[r], angY_deg[r], xs.data() + (size_t)r * 2 * g.nx,
                    ys.data() + (size_t)r * 2 * g.ny, zs.data() + (size_t)r * 2 * g.nz, rows[r]);
-        rows[r].zout = defocus[r];
+        rows[r].zout = zout[r];
         rows[r].weight = weights ? weights[r] : 1.0f;
     }
     const size_t n2 = (size_t)g.nx * g.ny, N = n2 * g.nz;
     const size_t per_angle = (size_t)g.nplanes * n2 * sizeof(float);  // its plane stack in P
-    c->batch = (int)std::max<size_t>(1, std::min<size_t>(std::min<size_t>((size_t)nrows, 65535),
+    c->batch = (int)std::max<size_t>(1, std::min<size_t>(std::min<size_t>((size_t)nang, 65535),
                                                          dht_batch_bytes() / per_angle));
-    if (g.nplanes > 65535) return fail(set_err(UTR_E_UNSUPPORTED, "too many output planes"));
-    if (hipGetDevice(&c->device) != hipSuccess) return fail(set_err(UTR_E_DEVICE, "no HIP device"));
+    if (g.nplanes > 65535) return (set_err(UTR_E_UNSUPPORTED, "too many output planes"));
+    if (hipGetDevice(&c->device) != hipSuccess) return (set_err(UTR_E_DEVICE, "no HIP device"));
     if (hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess ||
         c->ncu <= 0)
         c->ncu = 256;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
-        return fail(set_err(UTR_E_DEVICE, "hipStreamCreate failed"));
+        return (set_err(UTR_E_DEVICE, "hipStreamCreate failed"));
     for (auto& e : c->ev)
-        if (hipEventCreate(&e) != hipSuccess) return fail(set_err(UTR_E_DEVICE, "hipEventCreate failed"));
-    if (c->rows.alloc(nrows) || c->xs.alloc(xs.size()) || c->ys.alloc(ys.size()) || c->zs.alloc(zs.size()) ||
-        c->acc3.alloc(N) || c->spec.alloc((size_t)nrows * n2) || c->pmax.alloc(1) ||
+        if (hipEventCreate(&e) != hipSuccess) return (set_err(UTR_E_DEVICE, "hipEventCreate failed"));
+    if (c->rows.alloc(nang) || c->xs.alloc(xs.size()) || c->ys.alloc(ys.size()) || c->zs.alloc(zs.size()) ||
+        c->acc3.alloc(N) || c->spec.alloc((size_t)nang * n2) || c->pmax.alloc(1) ||
         c->T.alloc((size_t)g.nplanes * n2) || c->P.alloc((size_t)c->batch * g.nplanes * n2))
-        return fail(set_err(UTR_E_NOMEM, "device allocation of the DHT buffers failed"));
-    if (hipMemcpy(c->rows.p, rows.data(), sizeof(utr::DhtRow) * nrows, hipMemcpyHostToDevice) ||
+        return (set_err(UTR_E_NOMEM, "device allocation of the DHT buffers failed"));
+    if (hipMemcpy(c->rows.p, rows.data(), sizeof(utr::DhtRow) * nang, hipMemcpyHostToDevice) ||
         hipMemcpy(c->xs.p, xs.data(), sizeof(double) * xs.size(), hipMemcpyHostToDevice) ||
         hipMemcpy(c->ys.p, ys.data(), sizeof(double) * ys.size(), hipMemcpyHostToDevice) ||
         hipMemcpy(c->zs.p, zs.data(), sizeof(double) * zs.size(), hipMemcpyHostToDevice))
-        return fail(set_err(UTR_E_DEVICE, "upload of the orientation tables failed"));
+        return (set_err(UTR_E_DEVICE, "upload of the orientation tables failed"));
     {
         // aperture and Cs phase table of the propagators (DHT.cpp:962-1000):
         // ap = exp(-i (f4 q^4 + f6 q^6)) inside q^2 < k2maxo, 0 outside
@@ -1150,13 +1167,52 @@ int dht_create(const dht_params* p, int nrows, const double* angZ_deg, const dou
                 }
         if (c->ap.alloc(ap.size()) ||
             hipMemcpy(c->ap.p, ap.data(), sizeof(cplx<float>) * ap.size(), hipMemcpyHostToDevice))
-            return fail(set_err(UTR_E_DEVICE, "upload of the aperture table failed"));
+            return (set_err(UTR_E_DEVICE, "upload of the aperture table failed"));
     }
     auto twx = make_twiddles<float>(g.nx), twy = make_twiddles<float>(g.ny);
     if (c->twx.alloc(twx.size()) || c->twy.alloc(twy.size()) ||
         hipMemcpy(c->twx.p, twx.data(), sizeof(cplx<float>) * twx.size(), hipMemcpyHostToDevice) ||
         hipMemcpy(c->twy.p, twy.data(), sizeof(cplx<float>) * twy.size(), hipMemcpyHostToDevice))
-        return fail(set_err(UTR_E_DEVICE, "twiddle upload failed"));
+        return (set_err(UTR_E_DEVICE, "twiddle upload failed"));
+    return UTR_OK;
+}
+
+
+}  // namespace
+
+extern "C" {
+
+int dht_create(const dht_params* p, int nrows, const double* angZ_deg, const double* angY_deg,
+               const double* defocus, const float* weights, dht_ctx** out) {
+    if (!out) return set_err(UTR_E_INVALID, "null output pointer");
+    *out = nullptr;
+    if (!p) return set_err(UTR_E_INVALID, "null parameters");
+    if (nrows < 1 || !angZ_deg || !angY_deg || !defocus)
+        return set_err(UTR_E_INVALID, "Error: the orientation file contains no rows.");
+    std::string err = dht_validate(p);
+    if (!err.empty()) return set_err(UTR_E_INVALID, err);
+    if (p->method == 1 || p->method == 2)  // DHT.cpp:272-276
+        return set_err(UTR_E_INVALID,
+                       "Error: selected phase retrieval method is incompatibe with a single defocus distance "
+                       "appearing in some lines of the defocus/orientations parameter file.");
+    dht_ctx* c = new dht_ctx();
+    c->method = p->method;
+    if (p->method == 5) {
+        int rc = dht_create_sctf(p, nrows, angZ_deg, angY_deg, defocus, weights, c);
+        if (rc) {
+            delete c;
+            return rc;
+        }
+        *out = c;
+        return UTR_OK;
+    }
+    c->nrows = nrows;
+    c->added.assign(nrows, 0);
+    const int rc = dht_setup(c, p, nrows, angZ_deg, angY_deg, defocus, weights);
+    if (rc) {
+        dht_destroy(c);
+        return rc;
+    }
     *out = c;
     return UTR_OK;
 }
@@ -1279,6 +1335,8 @@ int dht_destroy(dht_ctx* c) {
     c->spec.release(); c->T.release(); c->twx.release(); c->twy.release(); c->ap.release();
     c->prep_d.release(); c->img_prep.release(); c->prep_aver.release();
     c->P.release(); c->stage.release(); c->vol.release(); c->pmax.release();
+    c->simg.release(); c->B.release(); c->E.release(); c->ser_img.release(); c->ser_ang.release();
+    c->ser_part.release(); c->ser_done.release(); c->ser_iters.release(); c->ser_active.release();
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -1010,8 +1010,9 @@ int utr_batch_job_form(const utr_batch* b, int j) {
 
 }  // extern "C"
 
-// DHT variants (methods 3-5) share the helpers above
+// DHT variants (methods 1-5) share the helpers above
 #include "dht.hip"
+#include "dht_series.hip"
 // post-processing of the reconstructed volume
 #include "post.hip"
 
