@@ -198,19 +198,30 @@ def main():
         # DHT variants on the same images (methods 3, 4: per-plane propagation
         # and real-space splat, O(A N^3); method 5: Ewald-sphere sCTF)
         dht = {}
-        for m, kw in ((3, dict(dzextra=100.0, epsilon=1.0)),
+        # methods 1, 2: a series of three distances per orientation (defocus
+        # -100, +0, +100 A; the orientation's image and its two neighbours)
+        d_ser = torch.stack([d_img, d_img.roll(1, 0), d_img.roll(2, 0)], 1).reshape(3 * A, ny, nx).contiguous()
+        srows = dict(angZ=rows["angZ"], angY=rows["angY"], counts=np.full(A, 3),
+                     defocus=(np.asarray(rows["defocus"], np.float64)[:, None]
+                              + np.array([-100.0, 0.0, 100.0])[None]).reshape(-1))
+        iw_iters = 50
+        for m, kw in ((1, dict(dzextra=100.0, epsilon=0.0, itermax=iw_iters)),
+                      (2, dict(dzextra=100.0, epsilon=1e-8)),
+                      (3, dict(dzextra=100.0, epsilon=1.0)),
                       (4, dict(dzextra=0.0, asigma=0.05, epsilon=0.02)),
                       (5, dict(dzextra=0.0, asigma=0.05, epsilon=0.02, critical_snr=0.5))):
             dp = U.default_dht_params(nx=nx, ny=ny, xst=p["xst"], zlo=p["zlo"], zhi=p["zhi"],
                                       wl=p["wl"], method=m, escc=1, **kw)
-            r = U.DhtReconstructor(dp, rows)
+            series = m <= 2
+            r = U.DhtReconstructor(dp, srows if series else rows)
+            src, cnt = (d_ser, 3 * A) if series else (d_img, A)
             dvol = torch.empty((nz, ny, nx), dtype=torch.float32, device="cuda")
             ts = []
             for it in range(3):
                 torch.cuda.synchronize()
                 t0 = time.perf_counter()
                 r.reset()
-                r.add_images_device(d_img.data_ptr(), A)
+                r.add_images_device(src.data_ptr(), cnt)
                 r.finalize_device(dvol.data_ptr())
                 torch.cuda.synchronize()
                 ts.append(time.perf_counter() - t0)
@@ -219,7 +230,17 @@ def main():
             t = float(np.mean(ts[1:]))
             dht["method%d" % m] = {"ms_per_step": round(t * 1e3, 2), "value": work / t,
                                    "phase_ms": {k: round(v, 2) for k, v in tm.items()}}
+            if m == 1:
+                # one IWFR iteration moves every image's field through two FFT
+                # passes (read + write 8 B each), reads the image (4 B) and
+                # reads + writes the orientation's estimate (16 B)
+                it_ms = tm["fft2d_ctf_ms"] / iw_iters
+                it_bytes = 36.0 * 3 * A * nx * ny + 16.0 * A * nx * ny
+                dht["method1"]["iteration"] = {"iterations": iw_iters, "ms": round(it_ms, 4),
+                                               "algorithmic_bytes": it_bytes,
+                                               "GBs": round(it_bytes / (it_ms * 1e-3) / 1e9, 1)}
             log("dht method %d: %.1f ms" % (m, t * 1e3))
+        del d_ser
         extra["dht"] = dht
 
     if not args.no_batch:

@@ -758,6 +758,7 @@ struct dht_ctx {
     std::vector<int> first, count;  // images of orientation a: first[a] .. first[a]+count[a]-1
     std::vector<int> iters;         // IWFR iterations of each orientation in the last finalize
     DevBuf<float> simg;             // the renormalised-on-read input images
+    DevBuf<float> A1;               // sqrt(I) - 1 of one pass's images
     DevBuf<cplx<float>> B, E;       // retrieval planes; per-image propagator factors
     DevBuf<utr::SerImg> ser_img;
     DevBuf<utr::SerAng> ser_ang;
@@ -1335,7 +1336,7 @@ int dht_destroy(dht_ctx* c) {
     c->spec.release(); c->T.release(); c->twx.release(); c->twy.release(); c->ap.release();
     c->prep_d.release(); c->img_prep.release(); c->prep_aver.release();
     c->P.release(); c->stage.release(); c->vol.release(); c->pmax.release();
-    c->simg.release(); c->B.release(); c->E.release(); c->ser_img.release(); c->ser_ang.release();
+    c->simg.release(); c->B.release(); c->A1.release(); c->E.release(); c->ser_img.release(); c->ser_ang.release();
     c->ser_part.release(); c->ser_done.release(); c->ser_iters.release(); c->ser_active.release();
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);

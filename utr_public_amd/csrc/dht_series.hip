@@ -69,10 +69,13 @@ __device__ __forceinline__ bool ser_in_aperture(int kx, int ky, const DhtGeom& g
 
 // Rows of images p0 + blockIdx.y (Bp, img: that image's planes at index
 // blockIdx.y); grid (ceil(ny / NB), images).
+// A1: sqrt(I) - 1 of the same images, written by the start (MODE 0) and
+// read by the iterations (MODE 1).
 template <int N, int MODE>
 __global__ void __launch_bounds__(256) k_ser_rows(cplx<float>* __restrict__ Bp, const float* __restrict__ img,
-                                                  int p0, DhtGeom g, const SerImg* __restrict__ im,
-                                                  const int* __restrict__ done, const cplx<float>* __restrict__ tw) {
+                                                  float* __restrict__ A1, int p0, DhtGeom g,
+                                                  const SerImg* __restrict__ im, const int* __restrict__ done,
+                                                  const cplx<float>* __restrict__ tw) {
     constexpr int NB = lds_batch(N, sizeof(cplx<float>), LDS_DHT);
     __shared__ cplx<float> sm[2 * lds_len(NB * N)];
     cplx<float>* a = sm;
@@ -80,10 +83,14 @@ __global__ void __launch_bounds__(256) k_ser_rows(cplx<float>* __restrict__ Bp, 
     const int pr = blockIdx.y, tid = threadIdx.x, ny = g.ny;
     if (MODE == 1 && done[im[p0 + pr].angle]) return;  // uniform over the workgroup
     const int j0 = blockIdx.x * NB, nb = min(NB, ny - j0);
-    const float* src = img + ((size_t)pr * ny + j0) * N;
-    cplx<float>* row = Bp + ((size_t)pr * ny + j0) * N;
+    const size_t off = ((size_t)pr * ny + j0) * N;
+    cplx<float>* row = Bp + off;
     if (MODE == 0) {
-        for (int t = tid; t < nb * N; t += 256) a[lx(t)] = {ser_amp_m1(src[t], g), 0.f};
+        for (int t = tid; t < nb * N; t += 256) {
+            const float am1 = ser_amp_m1(img[off + t], g);
+            A1[off + t] = am1;
+            a[lx(t)] = {am1, 0.f};
+        }
     } else {
         for (int t = tid; t < nb * N; t += 256) a[lx(t)] = row[t];
         __syncthreads();
@@ -91,7 +98,7 @@ __global__ void __launch_bounds__(256) k_ser_rows(cplx<float>* __restrict__ Bp, 
         const float inv = (float)(1.0 / ((double)N * ny));
         for (int t = tid; t < nb * N; t += 256) {
             const cplx<float> v = a[lx(t)];
-            a[lx(t)] = ser_replace({v.x * inv, v.y * inv}, ser_amp_m1(src[t], g));
+            a[lx(t)] = ser_replace({v.x * inv, v.y * inv}, A1[off + t]);
         }
     }
     __syncthreads();
@@ -357,6 +364,15 @@ __global__ void __launch_bounds__(256) k_ser_wave(const cplx<float>* __restrict_
 
 namespace {
 
+size_t dht_series_bytes() {
+    const char* e = std::getenv("UTR_DHT_SERIES_BYTES");
+    if (e && *e) {
+        const long long v = std::atoll(e);
+        if (v > 0) return (size_t)v;
+    }
+    return size_t(160) << 20;
+}
+
 int dht_cols_nb(int ny) { return utr::lds_batch(ny, (int)sizeof(cplx<float>), utr::LDS_DHT); }
 int dht_rows_nb(int nx) { return utr::lds_batch(nx, (int)sizeof(cplx<float>), utr::LDS_DHT); }
 
@@ -377,7 +393,7 @@ int dht_series_add(dht_ctx* c, int first, int count, const float* d_img) {
 // Orientations [a0, a1) of one retrieval pass and their images [p0, p1).
 void ser_pass(const dht_ctx* c, int a0, int& a1, int& p0, int& p1) {
     const size_t n2 = (size_t)c->g.nx * c->g.ny;
-    const size_t cap = std::max<size_t>(1, c->B.n / n2);
+    const size_t cap = std::max<size_t>(1, c->A1.n / n2);
     p0 = c->first[a0];
     a1 = a0 + 1;
     while (a1 < c->nang && a1 - a0 < c->ser_batch &&
@@ -402,8 +418,9 @@ int dht_series_retrieve(dht_ctx* c) {
         if (c->method == 1) {
 #define SER_ROWS(N, M)                                                                                     \
     {                                                                                                      \
-        utr::k_ser_rows<N, M><<<dim3(gy, np), 256, 0, c->stream>>>(c->B.p, c->simg.p + (size_t)p0 * n2, p0, g, \
-                                                                   c->ser_img.p, c->ser_done.p, c->twx.p);  \
+        utr::k_ser_rows<N, M><<<dim3(gy, np), 256, 0, c->stream>>>(c->B.p, c->simg.p + (size_t)p0 * n2,        \
+                                                                   c->A1.p, p0, g, c->ser_img.p,            \
+                                                                   c->ser_done.p, c->twx.p);                \
         ok = true;                                                                                         \
     }
 #define SER_ROWS0(N) SER_ROWS(N, 0)
@@ -576,16 +593,20 @@ int dht_create_series(const dht_params* p, int nangles, const double* angZ_deg, 
             s.angle = a;
         }
     }
-    // one pass holds at most the batch budget of retrieval planes, and at
-    // least one orientation's series
-    const size_t budget = dht_batch_bytes() / 2;
-    size_t cap = std::max<size_t>((size_t)cmax, budget / (n2 * sizeof(cplx<float>)));
+    // A retrieval pass iterates on the series of a group of orientations
+    // whose planes (12 B per image pixel: field and amplitude) fit a budget
+    // small enough to stay in the memory-side cache between the passes of
+    // an iteration (UTR_DHT_SERIES_BYTES, default 160 MB; at most half the
+    // DHT batch budget), and at least one orientation's series.
+    size_t budget = std::min<size_t>(dht_series_bytes(), dht_batch_bytes() / 2);
+    size_t cap = std::max<size_t>((size_t)cmax, budget / (n2 * (sizeof(cplx<float>) + sizeof(float))));
     cap = std::min<size_t>(cap, (size_t)total);
     cap = std::min<size_t>(cap, 65535);
     if ((size_t)cmax > cap) return fail(set_err(UTR_E_UNSUPPORTED, "too many images in one orientation"));
     c->ser_batch = 65535;
     const int gx = (g.nx + dht_cols_nb(g.ny) - 1) / dht_cols_nb(g.ny);
-    if (c->simg.alloc((size_t)total * n2) || c->B.alloc(cap * n2) || c->ser_img.alloc((size_t)total) ||
+    if (c->simg.alloc((size_t)total * n2) || c->B.alloc(cap * n2) || c->A1.alloc(cap * n2) ||
+        c->ser_img.alloc((size_t)total) ||
         c->ser_ang.alloc(nangles) || c->ser_part.alloc((size_t)std::min<size_t>(cap, 65535) * gx * 2) ||
         c->ser_done.alloc(nangles) || c->ser_iters.alloc(nangles) || c->ser_active.alloc(1))
         return fail(set_err(UTR_E_NOMEM, "device allocation of the series buffers failed"));
