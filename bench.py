@@ -44,6 +44,17 @@ def measured_traffic(kernel):
     return None
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -169,7 +180,26 @@ def main():
         rec.finalize(host_vol)
     th = (time.perf_counter() - ts) / reps
     extra["host_io"] = {"value": work / th, "ms_per_step": round(th * 1e3, 3),
-                        "note": "images in host memory, volume to host memory"}
+                        "note": "images in pageable host memory, volume to pageable host memory "
+                                "(pinned staging slots filled and drained by host threads)"}
+    # the same with the caller's buffers in pinned memory (utr_host_alloc):
+    # the copies are DMA only
+    p_img = U.host_empty(imgs.shape)
+    p_img[...] = imgs
+    p_vol = U.host_empty((nz, ny, nx))
+    rec.reset()
+    rec.add_images(p_img)
+    rec.finalize(p_vol)
+    ts = time.perf_counter()
+    for _ in range(reps):
+        rec.reset()
+        rec.add_images(p_img)
+        rec.finalize(p_vol)
+    tp = (time.perf_counter() - ts) / reps
+    extra["host_io_pinned"] = {"value": work / tp, "ms_per_step": round(tp * 1e3, 3),
+                               "note": "images and volume in pinned host memory (utr_host_alloc)",
+                               "matches_device_result": bool(np.array_equal(p_vol, vol32))}
+    del p_img, p_vol
 
     if not args.no_fp64:
         rec64 = U.Reconstructor(p, rows, U.UTR_FP64)
@@ -358,13 +388,31 @@ def main():
                 "sample": "12 of the 360 orientations, both defocus distances, FP64 numpy restatement",
                 "seconds": round(tc, 2)}
         import oracle
-        ncores = min(16, os.cpu_count() or 1)
+        # the baseline: the reference's CPU structure in FP32 (OpenMP over the
+        # rows with per-thread accumulators, threaded 3D c2r) on the host's
+        # OpenMP threads, full workload
         ts = time.perf_counter()
-        oracle.reconstruct(p, rows, imgs, nthreads=ncores)
+        v_cpu, t_cpu = oracle.cpu32_reconstruct(p, rows, imgs, nthreads=0)
         tc = time.perf_counter() - ts
-        cpu = {"value": work / tc, "unit": "angle*voxels/s", "cores": ncores, "kind": "port",
-               "sample": "full workload (360 rows, 256^3), FP64 CPU restatement, OpenMP over rows",
-               "seconds": round(tc, 2)}
+        nthr = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+        cpu = {"value": work / tc, "unit": "angle*voxels/s", "cores": nthr, "kind": "port",
+               "sample": "full workload (360 rows, 256^3): FP32 C++ restatement of UTR.cpp's CPU path "
+                         "(OpenMP over rows, per-thread accumulators, threaded 3D c2r)",
+               "seconds": round(tc, 3), "phase_ms": {k: round(float(v), 1) for k, v in t_cpu.items()},
+               "fft": oracle.cpu32_fft_name(), "cpu_model": cpu_model(),
+               "host_cpus_visible": os.cpu_count(),
+               "rel_l2_vs_gpu_fp32": float(np.linalg.norm(v_cpu.astype(np.float64) - vol32)
+                                           / np.linalg.norm(vol32))}
+        del v_cpu
+        # the FP64 oracle on the same full workload: parity of the GPU line
+        ts = time.perf_counter()
+        ref = oracle.reconstruct(p, rows, imgs, nthreads=nthr)
+        to = time.perf_counter() - ts
+        extra["parity_vs_fp64_oracle"] = {
+            "rel_l2": float(np.linalg.norm(vol32 - ref) / np.linalg.norm(ref)),
+            "max_abs": float(np.abs(vol32 - ref).max()), "max_abs_ref": float(np.abs(ref).max()),
+            "oracle_seconds": round(to, 2), "oracle_threads": nthr}
+        del ref
 
     out = {
         "metric": "angle*voxels/s, UTR 256^3 x 360 angles, FP32 (rel-L2 vs FP64 reported)",

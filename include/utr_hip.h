@@ -89,6 +89,18 @@ int utr_create(const utr_params* params, int nrows, const double* angZ_deg,
                const float* weights, int precision, utr_ctx** out);
 
 /*
+ * Pinned host memory.  utr_add_images, utr_finalize and the other host-
+ * memory entry points move data between memory from utr_host_alloc and the
+ * device by DMA directly; pageable memory goes through the context's pinned
+ * staging slots, filled and drained by a few host threads (UTR_HOST_THREADS,
+ * default min(8, hardware threads)).  No counterpart in the reference, which
+ * keeps its images in host memory; for callers that read their inputs
+ * straight into pinned buffers.
+ */
+int utr_host_alloc(size_t bytes, void** out);
+int utr_host_free(void* p);
+
+/*
  * Supplies the flat-field corrected images of rows [first_row,
  * first_row+count) as count*ny*nx floats (x fastest) in host memory.
  * Replaces the per-row body of the OpenMP loop UTR.cpp:877-1040 up to and

@@ -334,6 +334,49 @@ def prep_images(params, rows, images):
     return out
 
 
+_CPU32_PATH = os.path.join(_HERE, "libutr_cpu32.so")
+_cpu32 = None
+
+
+def cpu32_lib():
+    """The FP32 CPU baseline (utr_cpu32.cpp): the reference's CPU structure
+    in float, for timing beside the GPU."""
+    global _cpu32
+    if _cpu32 is None:
+        if not os.path.exists(_CPU32_PATH):
+            build()
+        L = ctypes.CDLL(_CPU32_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        fp = ctypes.POINTER(ctypes.c_float)
+        L.cpu32_last_error.restype = ctypes.c_char_p
+        L.cpu32_fft_name.restype = ctypes.c_char_p
+        L.cpu32_reconstruct.argtypes = [ctypes.POINTER(OraParams), ctypes.c_int, dp, dp, dp, fp, fp, fp,
+                                        ctypes.c_int, dp]
+        _cpu32 = L
+    return _cpu32
+
+
+def cpu32_reconstruct(params, rows, images, weights=None, nthreads=0):
+    """(volume float32 (nz, ny, nx), timings dict) of the FP32 CPU baseline,
+    CTF mode 0; nthreads 0: OpenMP's default."""
+    L = cpu32_lib()
+    angZ, angY, _, defoc = _rows(rows)
+    p = make_params(params)
+    g = derive(params, rows)
+    images = np.ascontiguousarray(images, dtype=np.float32)
+    vol = np.zeros((g.nz, g.ny, g.nx), np.float32)
+    w = None if weights is None else np.ascontiguousarray(weights, dtype=np.float32)
+    t = np.zeros(4)
+    if L.cpu32_reconstruct(ctypes.byref(p), len(angZ), _dp(angZ), _dp(angY), _dp(defoc),
+                           _fp(w) if w is not None else None, _fp(images), _fp(vol), nthreads, _dp(t)):
+        raise OracleError(L.cpu32_last_error().decode())
+    return vol, {"rows_ms": t[0], "sum_normalise_ms": t[1], "fft3d_ms": t[2], "total_ms": t[3]}
+
+
+def cpu32_fft_name():
+    return cpu32_lib().cpu32_fft_name().decode()
+
+
 def reconstruct(params, rows, images, weights=None, nthreads=1, samp=False):
     """Full FP64 reconstruction.  Returns the volume (nz, ny, nx) float64 and,
     if samp is True, the sampling matrix.  Rows with a Z'' angle or a RELION

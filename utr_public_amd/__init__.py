@@ -40,7 +40,7 @@ EXPORTS = [
     "utr_batch_create", "utr_batch_run", "utr_batch_job_form", "utr_batch_destroy",
     "dht_create", "dht_add_images", "dht_add_images_device", "dht_finalize", "dht_finalize_device",
     "dht_reset", "dht_dims", "dht_get_timings", "dht_destroy", "dht_set_image_correction",
-    "dht_create_series", "dht_get_iterations", "dht_get_waves",
+    "dht_create_series", "dht_get_iterations", "dht_get_waves", "utr_host_alloc", "utr_host_free",
     "utr_highpass3d", "utr_highpass3d_device", "utr_highpass3d_device_noalloc", "utr_rotate3d", "utr_rotate3d_device",
     "utr_symmetrize3d", "utr_symmetrize3d_device", "utr_invlaplace3d", "utr_invlaplace3d_device",
     "msct_create", "msct_images", "msct_images_device", "msct_slices", "msct_batch", "msct_destroy",
@@ -192,6 +192,8 @@ def load():
     L.dht_get_timings.argtypes = [vp, ctypes.POINTER(UtrTimings)]
     L.dht_destroy.argtypes = [vp]
     L.dht_set_image_correction.argtypes = [vp, dp, dp, dp]
+    L.utr_host_alloc.argtypes = [ctypes.c_size_t, ctypes.POINTER(vp)]
+    L.utr_host_free.argtypes = [vp]
     L.dht_create_series.argtypes = [ctypes.POINTER(DhtParams), ctypes.c_int, dp, dp,
                                     ctypes.POINTER(ctypes.c_int), dp, vp, ctypes.POINTER(vp)]
     L.dht_get_iterations.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
@@ -629,6 +631,31 @@ def symmetrize3d(vol, angZ, angY, angZ2):
     assert a[0].shape == a[1].shape == a[2].shape and a[0].ndim == 1
     _check(load().utr_symmetrize3d(out.ctypes.data, nx, ny, nz, len(a[0]), *(_dp(x) for x in a)))
     return out
+
+
+class _PinnedBlock:
+    def __init__(self, nbytes):
+        self._L = load()
+        ptr = ctypes.c_void_p()
+        _check(self._L.utr_host_alloc(nbytes, ctypes.byref(ptr)))
+        self.ptr = ptr.value
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            self._L.utr_host_free(ctypes.c_void_p(self.ptr))
+            self.ptr = None
+
+
+def host_empty(shape, dtype=np.float32):
+    """A numpy array in pinned host memory (utr_host_alloc): the library's
+    host-memory entry points copy it by DMA without staging.  Freed with the
+    array."""
+    dtype = np.dtype(dtype)
+    nbytes = int(np.prod(shape)) * dtype.itemsize
+    owner = _PinnedBlock(nbytes)
+    buf = (ctypes.c_char * max(nbytes, 1)).from_address(owner.ptr)
+    buf._owner = owner
+    return np.frombuffer(buf, dtype=dtype, count=int(np.prod(shape))).reshape(shape)
 
 
 def dht_reconstruct(params, rows, images):

@@ -23,6 +23,11 @@
 #include <string>
 #include <limits>
 #include <vector>
+#include <atomic>
+#include <condition_variable>
+#include <map>
+#include <mutex>
+#include <thread>
 
 #include "../../include/utr_hip.h"
 #include "utr_fft.hpp"
@@ -124,6 +129,140 @@ struct LocalBuf : DevBuf<T> {
 constexpr int kStageRows = 32;  // rows per host->device staging batch
 constexpr int kRowsWgPerCu = 8; // resident workgroups of the persistent row FFT
 
+// ------------------------------------------------ host memory, host copies
+//
+// Pinned host memory handed out by utr_host_alloc, by address range: the
+// host-memory entry points move data between such memory and the device by
+// DMA directly.  Other (pageable) host memory goes through pinned staging
+// slots owned by the context, filled and drained by HostCopier.
+
+std::mutex g_pinned_mu;
+std::map<uintptr_t, size_t> g_pinned;  // base -> bytes
+
+bool host_pinned(const void* p, size_t bytes) {
+    const uintptr_t a = (uintptr_t)p;
+    std::lock_guard<std::mutex> lk(g_pinned_mu);
+    auto it = g_pinned.upper_bound(a);
+    if (it == g_pinned.begin()) return false;
+    --it;
+    return a >= it->first && a + bytes <= it->first + it->second;
+}
+
+// A few host threads copying one buffer together: one thread's memcpy from
+// pageable memory reaches a few GB/s, well below the DMA rate of the pinned
+// slots it feeds.  The calling thread takes part; copy() returns when the
+// whole buffer is copied.  One process-wide instance (calls serialise).
+class HostCopier {
+public:
+    static HostCopier& get() {
+        static HostCopier inst(threads());
+        return inst;
+    }
+    void copy(void* dst, const void* src, size_t bytes) {
+        std::lock_guard<std::mutex> one(call_mu_);
+        if (bytes < (kPiece << 1) || th_.empty()) {
+            std::memcpy(dst, src, bytes);
+            return;
+        }
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            dst_ = (char*)dst;
+            src_ = (const char*)src;
+            bytes_ = bytes;
+            next_.store(0);
+            busy_ = (int)th_.size();
+            gen_++;
+        }
+        cv_.notify_all();
+        work();
+        std::unique_lock<std::mutex> lk(m_);
+        done_.wait(lk, [&] { return busy_ == 0; });
+    }
+    ~HostCopier() {
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto& t : th_) t.join();
+    }
+
+private:
+    static constexpr size_t kPiece = size_t(1) << 20;
+    static int threads() {
+        // UTR_HOST_THREADS overrides; the caller is one of them
+        const char* e = std::getenv("UTR_HOST_THREADS");
+        int n = e && *e ? std::atoi(e) : (int)std::min(8u, std::max(1u, std::thread::hardware_concurrency()));
+        return std::max(1, std::min(n, 64));
+    }
+    explicit HostCopier(int n) {
+        for (int i = 1; i < n; i++) th_.emplace_back([this] { run(); });
+    }
+    void work() {
+        for (;;) {
+            const size_t o = next_.fetch_add(kPiece);
+            if (o >= bytes_) break;
+            std::memcpy(dst_ + o, src_ + o, std::min(kPiece, bytes_ - o));
+        }
+    }
+    void run() {
+        unsigned seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> lk(m_);
+                cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+                if (stop_) return;
+                seen = gen_;
+            }
+            work();
+            std::lock_guard<std::mutex> lk(m_);
+            if (--busy_ == 0) done_.notify_one();
+        }
+    }
+    std::vector<std::thread> th_;
+    std::mutex m_, call_mu_;
+    std::condition_variable cv_, done_;
+    char* dst_ = nullptr;
+    const char* src_ = nullptr;
+    size_t bytes_ = 0;
+    std::atomic<size_t> next_{0};
+    unsigned gen_ = 0;
+    int busy_ = 0;
+    bool stop_ = false;
+};
+
+// Three pinned staging slots of `slot` floats with an event per slot (the
+// slot is free once the copy recorded after its last use has completed).
+struct PinnedStage {
+    float* p = nullptr;
+    size_t slot = 0;
+    hipEvent_t ev[3] = {};
+    hipError_t ensure(size_t floats, hipStream_t s) {
+        if (p && slot >= floats) return hipSuccess;
+        release();
+        hipError_t e = hipHostMalloc((void**)&p, sizeof(float) * floats * 3, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return e;
+        }
+        slot = floats;
+        for (auto& v : ev) {
+            if ((e = hipEventCreateWithFlags(&v, hipEventDisableTiming)) != hipSuccess) return e;
+            if ((e = hipEventRecord(v, s)) != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
+    float* at(int i) const { return p + (size_t)i * slot; }
+    void release() {
+        for (auto& v : ev)
+            if (v) (void)hipEventDestroy(v);
+        for (auto& v : ev) v = nullptr;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        slot = 0;
+    }
+};
+
 }  // namespace
 
 struct utr_ctx {
@@ -142,6 +281,7 @@ struct utr_ctx {
     // shared
     DevBuf<utr::RowParams> rows;
     DevBuf<float> img_stage;
+    PinnedStage hstage;  // pinned slots for pageable host images and volumes
     // input image correction (Z'' rotation, RELION shift) of the rows that need it
     DevBuf<utr::PrepRow> prep;
     DevBuf<float> img_prep, prep_aver;
@@ -371,6 +511,35 @@ int add_rows(utr_ctx* c, int first_row, int count, const float* d_img) {
     return finish_rows(c, e0, first_row, count);
 }
 
+// Device -> host copy of n floats into the caller's memory: one DMA into
+// pinned caller memory; otherwise in pieces through the pinned slots, each
+// piece drained by the host threads while the DMA of the next two runs.
+int copy_to_host(utr_ctx* c, float* dst, const float* d_src, size_t n) {
+    if (host_pinned(dst, sizeof(float) * n)) {
+        UTR_HIP_TRY(hipMemcpyAsync(dst, d_src, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+        UTR_HIP_TRY(hipStreamSynchronize(c->stream));
+        return UTR_OK;
+    }
+    const size_t piece = size_t(2) << 20;  // floats (8 MB)
+    UTR_HIP_TRY(c->hstage.ensure(std::max<size_t>(piece, c->hstage.slot), c->stream));
+    const size_t np = (n + piece - 1) / piece;
+    auto issue = [&](size_t i) -> hipError_t {
+        const size_t off = i * piece, m = std::min(piece, n - off);
+        hipError_t e = hipMemcpyAsync(c->hstage.at((int)(i % 3)), d_src + off, sizeof(float) * m,
+                                      hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipEventRecord(c->hstage.ev[i % 3], c->stream);
+        return e;
+    };
+    for (size_t i = 0; i < np && i < 2; i++) UTR_HIP_TRY(issue(i));
+    for (size_t i = 0; i < np; i++) {
+        if (i + 2 < np) UTR_HIP_TRY(issue(i + 2));
+        UTR_HIP_TRY(hipEventSynchronize(c->hstage.ev[i % 3]));
+        const size_t off = i * piece, m = std::min(piece, n - off);
+        HostCopier::get().copy(dst + off, c->hstage.at((int)(i % 3)), sizeof(float) * m);
+    }
+    return UTR_OK;
+}
+
 int check_rows(utr_ctx* c, int first_row, int count) {
     if (!c) return set_err(UTR_E_INVALID, "null context");
     if (first_row < 0 || count < 0 || first_row + count > c->plan.nrows)
@@ -564,16 +733,27 @@ int utr_add_images(utr_ctx* c, int first_row, int count, const float* images) {
     if (!images) return set_err(UTR_E_INVALID, "null image pointer");
     const auto& g = c->plan.g;
     const size_t n2 = (size_t)g.nx * g.ny, slot = n2 * kStageRows;
-    // two device staging slots: the runtime's copy of batch b+1 from the
-    // caller's (pageable) memory overlaps the 2D stage of batch b.  (A pinned
-    // bounce buffer filled by one host thread measured slower.)
+    // Two device staging slots: the copy of batch b+1 overlaps the 2D stage
+    // of batch b.  Pinned caller memory (utr_host_alloc) is copied by DMA
+    // directly; pageable memory is first copied by several host threads into
+    // one of three pinned slots, while the DMA and kernels of the previous
+    // batches run.
+    const bool pinned = host_pinned(images, sizeof(float) * n2 * count);
     UTR_HIP_TRY(c->img_stage.alloc(2 * slot));
+    if (!pinned) UTR_HIP_TRY(c->hstage.ensure(slot, c->stream));
     hipEvent_t e0 = c->ev[6];
     UTR_HIP_TRY(hipEventRecord(e0, c->stream));
     for (int r = 0, b = 0; r < count; r += kStageRows, b++) {
-        const int k = std::min(kStageRows, count - r), s = b & 1;
-        UTR_HIP_TRY(hipMemcpyAsync(c->img_stage.p + s * slot, images + (size_t)r * n2, sizeof(float) * n2 * k,
-                                   hipMemcpyHostToDevice, c->stream));
+        const int k = std::min(kStageRows, count - r), s = b & 1, hs = b % 3;
+        const size_t bytes = sizeof(float) * n2 * k;
+        const float* src = images + (size_t)r * n2;
+        if (!pinned) {
+            UTR_HIP_TRY(hipEventSynchronize(c->hstage.ev[hs]));  // the slot's last DMA is done
+            HostCopier::get().copy(c->hstage.at(hs), src, bytes);
+            src = c->hstage.at(hs);
+        }
+        UTR_HIP_TRY(hipMemcpyAsync(c->img_stage.p + s * slot, src, bytes, hipMemcpyHostToDevice, c->stream));
+        if (!pinned) UTR_HIP_TRY(hipEventRecord(c->hstage.ev[hs], c->stream));
         rc = enqueue_rows(c, first_row + r, k, c->img_stage.p + s * slot);
         if (rc) {
             (void)hipStreamSynchronize(c->stream);
@@ -581,6 +761,31 @@ int utr_add_images(utr_ctx* c, int first_row, int count, const float* images) {
         }
     }
     return finish_rows(c, e0, first_row, count);
+}
+
+int utr_host_alloc(size_t bytes, void** out) {
+    if (!out) return set_err(UTR_E_INVALID, "null output pointer");
+    *out = nullptr;
+    void* p = nullptr;
+    UTR_HIP_TRY(hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault));
+    {
+        std::lock_guard<std::mutex> lk(g_pinned_mu);
+        g_pinned[(uintptr_t)p] = bytes ? bytes : 1;
+    }
+    *out = p;
+    return UTR_OK;
+}
+
+int utr_host_free(void* p) {
+    if (!p) return UTR_OK;
+    {
+        std::lock_guard<std::mutex> lk(g_pinned_mu);
+        auto it = g_pinned.find((uintptr_t)p);
+        if (it == g_pinned.end()) return set_err(UTR_E_INVALID, "utr_host_free: not memory from utr_host_alloc");
+        g_pinned.erase(it);
+    }
+    UTR_HIP_TRY(hipHostFree(p));
+    return UTR_OK;
 }
 
 int utr_finalize_device(utr_ctx* c, float* d_vol) {
@@ -595,8 +800,7 @@ int utr_finalize(utr_ctx* c, float* vol) {
     UTR_HIP_TRY(c->vol32.alloc(vox(c)));
     int rc = do_finalize(c, c->vol32.p, nullptr);
     if (rc) return rc;
-    UTR_HIP_TRY(hipMemcpy(vol, c->vol32.p, sizeof(float) * vox(c), hipMemcpyDeviceToHost));
-    return UTR_OK;
+    return copy_to_host(c, vol, c->vol32.p, vox(c));
 }
 
 int utr_finalize_f64(utr_ctx* c, double* vol) {
@@ -772,6 +976,7 @@ int utr_destroy(utr_ctx* c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     c->rows.release();
     c->img_stage.release();
+    c->hstage.release();
     c->prep.release();
     c->img_prep.release();
     c->prep_aver.release();
@@ -790,9 +995,7 @@ int utr_destroy(utr_ctx* c) {
 
 // ---------------------------------------------------------- batched form
 
-#include <map>
-#include <mutex>
-#include <thread>
+
 #include <tuple>
 
 struct utr_batch {
