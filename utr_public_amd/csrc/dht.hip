@@ -910,6 +910,7 @@ int dht_create_sctf(const dht_params* p, int nrows, const double* angZ, const do
     g.do_invlap = p->inv_laplace;
     g.invlap_eps = p->alpha;
     g.fnorm = p->wl * tem_energy(p->wl) / (2.0 * utr::kPi * std::sqrt(1.0 + p->asigma * p->asigma));
+    utr::derive_epilogue(g);
     return UTR_OK;
 }
 

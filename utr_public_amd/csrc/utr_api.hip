@@ -290,6 +290,8 @@ struct utr_ctx {
     DevBuf<float> S32, vol32;
     DevBuf<unsigned> rowmax;  // per-row max |K| as float bits (FP32 forms)
     DevBuf<int> scales;       // fixed-point scales of the plane gridding
+    int grid_planes = 0;      // y planes per workgroup of the plane gridding (0: default)
+    DevBuf<int> tile_order;   // plane-gridding tiles by decreasing expected work
     // FP64 form
     DevBuf<cplx<double>> spec64, V64, H64, twx64, twy64, twz64;
     DevBuf<double> S64, vol64;
@@ -540,6 +542,28 @@ int copy_to_host(utr_ctx* c, float* dst, const float* d_src, size_t n) {
     return UTR_OK;
 }
 
+// The plane gridding's tiles by increasing distance of their centre from
+// the plane's rotation centre (nx/2, nz/2): the point density of the lines
+// falls off as 1/r, so the nearest tiles carry the most points and start
+// first.  Uploaded once at creation (finalize may run under graph capture).
+int upload_tile_order(utr_ctx* c) {
+    const auto& g = c->plan.g;
+    const int ntx = (g.nx + utr::PX - 1) / utr::PX, ntz = (g.nz + utr::PZ - 1) / utr::PZ;
+    const int ntiles = ntx * ntz;
+    std::vector<int> ord(ntiles);
+    std::vector<double> d2(ntiles);
+    for (int t = 0; t < ntiles; t++) {
+        ord[t] = t;
+        const double cx = (t % ntx) * utr::PX + 0.5 * utr::PX - g.nxd2;
+        const double cz = (t / ntx) * utr::PZ + 0.5 * utr::PZ - g.nzd2;
+        d2[t] = cx * cx + cz * cz;
+    }
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return d2[a] < d2[b]; });
+    UTR_HIP_TRY(c->tile_order.alloc(ntiles));
+    UTR_HIP_TRY(hipMemcpy(c->tile_order.p, ord.data(), sizeof(int) * ntiles, hipMemcpyHostToDevice));
+    return UTR_OK;
+}
+
 int check_rows(utr_ctx* c, int first_row, int count) {
     if (!c) return set_err(UTR_E_INVALID, "null context");
     if (first_row < 0 || count < 0 || first_row + count > c->plan.nrows)
@@ -556,15 +580,25 @@ int launch_grid32(utr_ctx* c, float* d_S) {
     const bool brick = form && std::strcmp(form, "brick") == 0;
     if (c->plan.all_y_exact && !brick) {
         const long ntiles = (long)((g.nx + utr::PX - 1) / utr::PX) * ((g.nz + utr::PZ - 1) / utr::PZ);
-        const long nwg = ntiles * g.ny;
+        int pg = c->grid_planes;
+        if (pg <= 0) {
+            const char* e = std::getenv("UTR_GRID_PLANES");
+            pg = e ? std::atoi(e) : 0;
+        }
+        if (pg <= 0) pg = 4;
+        const long ngroups = (g.ny + pg - 1) / pg;
+        const long nwg = 8L * ((ngroups + 7) / 8) * ntiles;
         if (nwg <= 0 || nwg > 0x7ffffff0L) return set_err(UTR_E_INVALID, "volume too large");
+        if (c->tile_order.n != (size_t)ntiles) return set_err(UTR_E_STATE, "plane-gridding tile order missing");
         utr::k_fx_scales<<<1, 256, 0, c->stream>>>(c->rowmax.p, c->plan.nrows, c->scales.p);
         if (g.rot_centre_shift)
-            utr::k_grid_yp<true><<<utr::xcd_grid((unsigned)nwg), 256, 0, c->stream>>>(
-                c->spec32.p, c->rows.p, c->plan.nrows, c->scales.p, g, c->V32.p, d_S, (int)ntiles);
+            utr::k_grid_yp<true><<<utr::yp_grid((int)ntiles, (int)ngroups), 256, 0, c->stream>>>(
+                c->spec32.p, c->rows.p, c->plan.nrows, c->scales.p, g, c->V32.p, d_S, (int)ntiles,
+                (int)ngroups, pg, c->tile_order.p);
         else
-            utr::k_grid_yp<false><<<utr::xcd_grid((unsigned)nwg), 256, 0, c->stream>>>(
-                c->spec32.p, c->rows.p, c->plan.nrows, c->scales.p, g, c->V32.p, d_S, (int)ntiles);
+            utr::k_grid_yp<false><<<utr::yp_grid((int)ntiles, (int)ngroups), 256, 0, c->stream>>>(
+                c->spec32.p, c->rows.p, c->plan.nrows, c->scales.p, g, c->V32.p, d_S, (int)ntiles,
+                (int)ngroups, pg, c->tile_order.p);
     } else {
         const long nblocks = (long)((g.nx + utr::SBX - 1) / utr::SBX) *
                              ((g.ny + utr::SBY - 1) / utr::SBY) * ((g.nz + utr::SBZ - 1) / utr::SBZ);
@@ -699,6 +733,10 @@ int utr_create(const utr_params* params, int nrows, const double* angZ_deg, cons
             c->H32.alloc(nh) != hipSuccess || c->rowmax.alloc(nrows) != hipSuccess ||
             c->scales.alloc(2) != hipSuccess)
             return fail(set_err(UTR_E_NOMEM, "device allocation of the FP32 buffers failed"));
+        if (c->plan.all_y_exact) {
+            const int rc = upload_tile_order(c);
+            if (rc) return fail(rc);
+        }
         // on the context's stream: it is non-blocking, so a null-stream memset
         // would not be ordered before the kernels launched on it
         if (hipMemsetAsync(c->rowmax.p, 0, sizeof(unsigned) * nrows, c->stream) != hipSuccess)
@@ -982,6 +1020,7 @@ int utr_destroy(utr_ctx* c) {
     c->prep_aver.release();
     c->spec32.release(); c->V32.release(); c->H32.release(); c->S32.release(); c->vol32.release();
     c->twx32.release(); c->twy32.release(); c->twz32.release(); c->rowmax.release(); c->scales.release();
+    c->tile_order.release();
     c->spec64.release(); c->V64.release(); c->H64.release(); c->S64.release(); c->vol64.release();
     c->twx64.release(); c->twy64.release(); c->twz64.release();
     for (auto& e : c->ev)

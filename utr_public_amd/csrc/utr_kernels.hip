@@ -438,12 +438,12 @@ __device__ __forceinline__ GridPos grid_pos(const GeomDev& g, const double* m, i
 // --------------------------------------------------- voxel epilogue
 
 // Sampling normalisation + noise filter (UTR.cpp:1050-1128) and the 3D
-// filters (UTR.cpp:1177-1201; UTR.h:15-185) for voxel (i, j, k).
+// filters (UTR.cpp:1177-1201; UTR.h:15-185) for voxel (i, j, k).  Factors
+// that depend on the geometry only come precomputed in g.ep_* (FP64).
 template <typename T>
 __device__ __forceinline__ cplx<T> voxel_epilogue(const GeomDev& g, int i, int j, int k,
                                                   cplx<T> v, T x) {
-    const double pi = 3.14159265358979323846;
-    const T pif = T(pi);
+    const T inv_pi = T(0.318309886183790671538);
     const T target = T(kTargetSNR);
     if (g.dht_sctf) {
         // DHT.cpp:1350-1365: x -> (1/x) exp((x - SNRc)/x^4) where sampled
@@ -464,28 +464,21 @@ __device__ __forceinline__ cplx<T> voxel_epilogue(const GeomDev& g, int i, int j
         if (g.ctf_mode != 1) {
             fnsr = T(g.nsr2 * (double)sqrt(x));
         } else {
-            const double norm = sqrt(1.0 + g.asigma * g.asigma) * g.nsr2;
-            const double fac2 = pi * g.wl * g.zout_aver;
-            const double dksi2 = fac2 / ((g.xhi - g.xlo) * (g.xhi - g.xlo));
-            const double deta2 = fac2 / ((g.yhi - g.ylo) * (g.yhi - g.ylo));
-            const double dzeta2 = fac2 / ((g.zhi - g.zlo) * (g.zhi - g.zlo));
             const int k1 = k - g.nzd2, j1 = j - g.nyd2, i1 = i - g.nxd2;
-            const double zeta2 = k1 * k1 * dzeta2 - g.asigma;
-            const double eta2 = j1 * j1 * deta2 + zeta2;
-            fnsr = T(norm * (double)sqrt(x) / (dksi2 * i1 * i1 + eta2 + g.epsilon));
+            const double zeta2 = k1 * k1 * g.ep_dzeta2 - g.asigma;
+            const double eta2 = j1 * j1 * g.ep_deta2 + zeta2;
+            fnsr = T(g.ep_nsr1 * (double)sqrt(x) / (g.ep_dksi2 * i1 * i1 + eta2 + g.epsilon));
         }
         const T y = sqrt(v.x * v.x + v.y * v.y) / fnsr;
-        const T f = (T(1) / x) * (T(0.5) + atan(T(100) * (y - target)) / pif);
+        const T f = (T(1) / x) * (T(0.5) + atan(T(100) * (y - target)) * inv_pi);
         v.x *= f;
         v.y *= f;
     }
     const int k1 = k - g.nzd2, j1 = j - g.nyd2, i1 = i - g.nxd2;
     if (g.do_tie3d || g.do_invlap) {
         // InverseMLaplacian3D(V3, zhi-zlo, yhi-ylo, xhi-ylo, alpha, norm)
-        const double fact = 4.0 * pi * pi;
-        const double xa = g.xhi - g.ylo, ya = g.yhi - g.ylo, za = g.zhi - g.zlo;
-        const double base = fact * (double)i1 * i1 / (xa * xa) + fact * (double)j1 * j1 / (ya * ya) +
-                            fact * (double)k1 * k1 / (za * za);
+        const double base = g.ep_lapx * ((double)i1 * i1) + g.ep_lapy * ((double)j1 * j1) +
+                            g.ep_lapz * ((double)k1 * k1);
         if (g.do_tie3d) {
             const T f = T(g.tie_norm / (base + g.tie_alpha));
             v.x *= f;
@@ -499,23 +492,18 @@ __device__ __forceinline__ cplx<T> voxel_epilogue(const GeomDev& g, int i, int j
     }
     if (g.do_ctf3d) {
         // CTFcorrection3D (UTR.h:86-131); phase argument in FP64
-        const double xap2 = (g.xhi - g.xlo) * (g.xhi - g.xlo);
-        const double yap2 = (g.yhi - g.ylo) * (g.yhi - g.ylo);
-        const double zap2 = (g.zhi - g.zlo) * (g.zhi - g.zlo);
-        const double fac2 = pi * g.wl * g.zout_aver;
-        const double zeta2 = (1.0 / zap2) * k1 * k1;
-        const double zeta2fac2 = fac2 * zeta2 - atan(g.asigma);
-        double eta2 = (1.0 / yap2) * j1 * j1;
-        const double eta2fac2 = fac2 * eta2 + zeta2fac2;
+        const double zeta2 = g.ep_iz2 * k1 * k1;
+        const double zeta2fac2 = g.ep_fac2 * zeta2 - g.ep_atans;
+        double eta2 = g.ep_iy2 * j1 * j1;
+        const double eta2fac2 = g.ep_fac2 * eta2 + zeta2fac2;
         eta2 += zeta2;
-        double ksi2 = (1.0 / xap2) * i1 * i1;
-        const double ksi2fac2 = fac2 * ksi2 + eta2fac2;
+        double ksi2 = g.ep_ix2 * i1 * i1;
+        const double ksi2fac2 = g.ep_fac2 * ksi2 + eta2fac2;
         ksi2 += eta2;
         if (!(g.k2maxo > 0) || ksi2 < g.k2maxo) {
             double dt = ksi2fac2;
             if (g.cs3 != 0 || g.cs5 != 0)
-                dt += pi * g.wl * g.wl * g.wl / 2.0 * g.cs3 * ksi2 * ksi2 +
-                      pi * g.wl * g.wl * g.wl * g.wl * g.wl / 3.0 * g.cs5 * ksi2 * ksi2 * ksi2;
+                dt += g.ep_c3 * ksi2 * ksi2 + g.ep_c5 * ksi2 * ksi2 * ksi2;
             double sn, cs;
             sincos(dt, &sn, &cs);
             const T sr = T(sn / (sn * sn + g.ctf_eps));
@@ -1095,16 +1083,19 @@ __global__ void __launch_bounds__(256, 3) k_grid_fx(const cplx<float>* __restric
 //
 // When no row rotates about Z, a detector line b reaches exactly one y plane
 // of the volume (j = ny/2 + b, weight 1) and its points move in that plane's
-// (x, z) coordinates.  Each workgroup then owns a PX x PZ tile of one y plane
-// and takes, from every row, the one line that lands in the plane: the line
-// is clipped to the tile once, the clipped lines are listed in row order, and
-// each thread takes an equal run of the listed points (crossing from one line
-// to the next, with the next line's coefficients loaded one line ahead), so
-// that all threads get the same work.  Point positions, corner weights and
-// the fixed-point sums are those of k_grid_fx (the same expressions in the
-// same order), so both kernels produce the same V3 and Samp3 sums.  Tiles of one plane are numbered
-// contiguously per XCD, so the plane's spectrum lines are fetched into that
-// XCD's L2 once.
+// (x, z) coordinates.  Each workgroup owns a PX x PZ tile of a group of
+// consecutive y planes and takes, from every row, the one line that lands in
+// each plane.  The lines of neighbouring planes differ only through the
+// Ewald term of b, so one clip of every row's line to the tile, conservative
+// over the group's b range, serves all planes of the group: the clipped
+// lines are listed in row order, each thread takes an equal run of the
+// listed points (crossing from one line to the next, with the next line's
+// coefficients loaded one line ahead), and the planes are then gridded one
+// after another with the same list and the same runs.  Point positions,
+// corner weights and the fixed-point sums are those of k_grid_fx (the same
+// expressions in the same order), so both kernels produce the same V3 and
+// Samp3 sums.  Groups are numbered contiguously per XCD, so a plane's
+// spectrum lines are fetched into one XCD's L2 once.
 
 constexpr int PX = 32, PZ = 32;
 constexpr int PVOX = PX * PZ;
@@ -1112,7 +1103,8 @@ constexpr int POWN = PVOX / 256;
 static_assert(PVOX % 256 == 0, "tile must be a multiple of the workgroup");
 constexpr int PSX = PX + 1;          // LDS stride along z (odd: lanes walking along z spread over banks)
 constexpr int PPAD = PSX * PZ;       // LDS entries per component
-constexpr int P_LIST = 256;          // rows tested per chunk
+constexpr int P_LIST = 384;          // rows clipped per chunk (two per thread)
+static_assert(P_LIST <= 512, "two rows per thread");
 
 // Fixed-point scales 2^sv (V3) and 2^sw (Samp3) from the largest spectrum
 // component and the row count.  Per row a voxel receives a total weight
@@ -1146,128 +1138,178 @@ __global__ void __launch_bounds__(256) k_fx_scales(const unsigned* __restrict__ 
     }
 }
 
+// Clip one row's detector line b in [fb_lo, fb_hi] to the x range
+// [bmin0, bmax0] and z range [bmin1, bmax1] of a tile: the a interval
+// [a0, a1] outside which no point of any of these lines can put a corner
+// in the tile.  x(a) and z(a) are linear in a up to the Ewald term
+// m2*zz(a), whose deviation from its tangent at the interval centre is at
+// most wl2*fx2*W^2; two passes, the second about the first's interval.  The
+// spread of the lines over b enters through m1*b and through fy2*b^2 in zz.
+// Returns the number of points (0: the line misses the tile).
+__device__ __forceinline__ int clip_line(const GeomDev& g, const RowParams& R, float fbm, float fbh,
+                                         float fb2m, float fb2h, const float bmin[2],
+                                         const float bmax[2], float fx2, float fy2, float wl2f,
+                                         int& a0) {
+    const float cgrid[2] = {(float)g.nxd2, (float)g.nzd2};
+    float alo = (float)(-g.nxd2), ahi = (float)(g.nx - 1 - g.nxd2);
+    for (int it = 0; it < 2 && alo <= ahi; it++) {
+        const float ac = 0.5f * (alo + ahi), W = 0.5f * (ahi - alo);
+        const float zz0 = g.escc ? -wl2f * (fx2 * ac * ac + fy2 * fb2m) : 0.f;
+        const float dzz = g.escc ? -2.f * wl2f * fx2 * ac : 0.f;
+        const float curv = g.escc ? wl2f * fx2 * W * W : 0.f;
+        const float bspr = g.escc ? wl2f * fy2 * fb2h : 0.f;
+#pragma unroll
+        for (int d = 0; d < 2; d++) {
+            const int rd = d ? 6 : 0;
+            const float m0 = (float)R.m[rd], m1 = (float)R.m[rd + 1], m2 = (float)R.m[rd + 2];
+            const float Pc = cgrid[d] + m0 * ac + (d ? 0.f : m1 * fbm) + m2 * zz0;
+            const float Pd = m0 + m2 * dzz;
+            const float mg = fabsf(m2) * (curv * 1.01f + bspr * 1.01f) + (d ? 0.f : fabsf(m1) * fbh) +
+                             2e-4f * (1.f + fabsf(Pc)) + 0.01f;
+            const float lo_d = bmin[d] - 1.f - mg - Pc, hi_d = bmax[d] + 1.f + mg - Pc;
+            if (fabsf(Pd) < 1e-6f) {
+                if (lo_d > 0.f || hi_d < 0.f) alo = ahi + 1.f;
+            } else {
+                // the margins above exceed the 1-ulp error of the reciprocal
+                const float rp = __builtin_amdgcn_rcpf(Pd);
+                float t0 = lo_d * rp, t1 = hi_d * rp;
+                if (Pd < 0.f) { const float tt = t0; t0 = t1; t1 = tt; }
+                alo = fmaxf(alo, ac + t0);
+                ahi = fminf(ahi, ac + t1);
+            }
+        }
+    }
+    if (!(alo <= ahi)) return 0;
+    a0 = max((int)ceilf(alo - 1e-3f), -g.nxd2);
+    const int a1 = min((int)floorf(ahi + 1e-3f), g.nx - 1 - g.nxd2);
+    return a0 <= a1 ? a1 - a0 + 1 : 0;
+}
+
+// Launch size of k_grid_yp: each XCD takes ceil(ngroups / 8) consecutive
+// plane groups, all tiles of them.
+__host__ __device__ constexpr unsigned yp_grid(int ntiles, int ngroups) {
+    return 8u * (unsigned)((ngroups + 7) / 8) * (unsigned)ntiles;
+}
+
 // ROT: rows carry the centre-shift phase (compiled separately: its FP64
 // sincos would otherwise set the register budget of the common case, which
-// fits 4 workgroups per CU).
+// fits 4 workgroups per CU).  Workgroup (tile, group) grids planes
+// pg*group .. pg*group + pg - 1; ntiles tiles per plane, ngroups groups.
+// Workgroups are dealt to the XCDs round robin (workgroup L on XCD L % 8)
+// and start in launch order, so each XCD's workgroups are ordered by tile
+// rank first: tile_order lists the tiles by decreasing expected point count
+// (the lines of all rows cross near the plane's centre), and the heaviest
+// tiles start first.
 template <bool ROT>
 __global__ void __launch_bounds__(256, ROT ? 1 : 4) k_grid_yp(const cplx<float>* __restrict__ spec,
                                                  const RowParams* __restrict__ rows, int nrows,
                                                  const int* __restrict__ scales, GeomDev g,
                                                  cplx<float>* __restrict__ Vout,
-                                                 float* __restrict__ Sout, int ntiles) {
+                                                 float* __restrict__ Sout, int ntiles, int ngroups,
+                                                 int pg, const int* __restrict__ tile_order) {
     __shared__ unsigned long long acc[3 * PPAD];
     __shared__ float tiny[3 * PVOX];
     __shared__ unsigned short s_row[P_LIST];  // rows whose line reaches the tile
     __shared__ short s_a0[P_LIST];                // first a of its clipped line
-    __shared__ int s_pre[P_LIST + 1];             // point counts, then their prefix
-    __shared__ int s_cnt[4], s_wsum[4];
+    __shared__ int s_pre[P_LIST + 1];             // exclusive prefix of the point counts
+    __shared__ int s_wsum[4];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    int tile, j;
-    if (!xcd_tile(ntiles, g.ny, tile, j)) return;
+    const int ngx = (ngroups + 7) >> 3;  // groups per XCD
+    const int lidx = (int)(blockIdx.x >> 3), rank = lidx / ngx;
+    const int grp = (int)(blockIdx.x & 7u) * ngx + (lidx - rank * ngx);
+    if (rank >= ntiles || grp >= ngroups) return;
+    const int tile = tile_order[rank];
     const int ntx = (g.nx + PX - 1) / PX;
     const int x0 = (tile % ntx) * PX, z0 = (tile / ntx) * PZ;
-    for (int t = tid; t < 3 * PPAD; t += 256) acc[t] = 0ull;
-    for (int t = tid; t < 3 * PVOX; t += 256) tiny[t] = 0.f;
+    const int jfirst = grp * pg, jend = min(jfirst + pg, g.ny);
+    // planes that receive weight: j <= ny - 2 (the last plane's cells would
+    // need plane ny)
+    const int jlast_w = min(jend, g.ny - 1) - 1;
     const int sv = scales[0], sw = scales[1];
     const double vscale = ldexp(1.0, sv), wscale = ldexp(1.0, sw);
-    __syncthreads();  // LDS zeroed
+    const double vinv = ldexp(1.0, -sv), winv = ldexp(1.0, -sw);
 
     const float fx2 = (float)(g.fxst * g.fxst), fy2 = (float)(g.fyst * g.fyst);
     const float wl2f = (float)g.wl2;
-    // the plane's line: b = j - ny/2; the last plane gets no weight (its
-    // cells would need plane ny)
-    const int b = j - g.nyd2;
-    const bool plane_ok = j <= g.ny - 2;
-    const float fb = (float)b;
     const float bmin[2] = {(float)x0, (float)z0};
     const float bmax[2] = {(float)(x0 + PX - 1), (float)(z0 + PZ - 1)};
-    const float cgrid[2] = {(float)g.nxd2, (float)g.nzd2};
-    const double bd = (double)b;
-    const double qy = __dmul_rn(g.fyst, bd);
-    const double qy2 = __dmul_rn(qy, qy);
+    // the group's detector lines b in [blo, bhi]
+    const int blo = jfirst - g.nyd2, bhi = jlast_w - g.nyd2;
+    const float fbm = 0.5f * (float)(blo + bhi), fbh = 0.5f * (float)(bhi - blo);
+    const float b2lo = (blo <= 0 && bhi >= 0) ? 0.f : fminf((float)blo * blo, (float)bhi * bhi);
+    const float b2hi = fmaxf((float)blo * blo, (float)bhi * bhi);
+    const float fb2m = 0.5f * (b2lo + b2hi), fb2h = 0.5f * (b2hi - b2lo);
+    const unsigned rstride = (unsigned)(g.ny * g.nx) & 0xffffffu;  // ny nx <= 2^18
 
-    for (int base = 0; base < nrows && plane_ok; base += P_LIST) {
-        const int nchunk = min(P_LIST, nrows - base);
-        // clip each row's line to the tile: x(a) and z(a) are linear in a up
-        // to the Ewald term m2*zz(a), whose deviation from its tangent at the
-        // interval centre is at most wl2*fx2*W^2; two passes, the second
-        // about the first's interval
-        bool pass = false;
-        int a0 = 0, a1 = -1;
-        if (tid < nchunk) {
-            const RowParams& R = rows[base + tid];
-            float alo = (float)(-g.nxd2), ahi = (float)(g.nx - 1 - g.nxd2);
-            for (int it = 0; it < 2 && alo <= ahi; it++) {
-                const float ac = 0.5f * (alo + ahi), W = 0.5f * (ahi - alo);
-                const float zz0 = g.escc ? -wl2f * (fx2 * ac * ac + fy2 * fb * fb) : 0.f;
-                const float dzz = g.escc ? -2.f * wl2f * fx2 * ac : 0.f;
-                const float curv = g.escc ? wl2f * fx2 * W * W : 0.f;
+    const int nchunks = (nrows + P_LIST - 1) / P_LIST;
+    int n = 0, total = 0, q0 = 0, p0 = 0, p1 = 0;
+    for (int j = jfirst; j < jend; j++) {
+        for (int t = tid; t < 3 * PPAD; t += 256) acc[t] = 0ull;
+        for (int t = tid; t < 3 * PVOX; t += 256) tiny[t] = 0.f;
+        __syncthreads();  // LDS zeroed (and the previous plane's epilogue done)
+        const int b = j - g.nyd2;
+        const double bd = (double)b;
+        const double qy = __dmul_rn(g.fyst, bd);
+        const double qy2 = __dmul_rn(qy, qy);
+        const cplx<float>* spec_j = spec + (size_t)j * g.nx + g.nxd2;
+        for (int base = 0; base < nrows && j <= jlast_w; base += P_LIST) {
+            if (nchunks > 1 || j == jfirst) {
+                // list the chunk's rows whose lines reach the tile: two rows
+                // per thread (2 tid, 2 tid + 1), one block scan of the
+                // packed (line count << 20 | point count)
+                const int nchunk = min(P_LIST, nrows - base);
+                int npt[2] = {0, 0}, a0s[2] = {0, 0};
 #pragma unroll
-                for (int d = 0; d < 2; d++) {
-                    const int rd = d ? 6 : 0;
-                    const float m0 = (float)R.m[rd], m1 = (float)R.m[rd + 1], m2 = (float)R.m[rd + 2];
-                    const float Pc = cgrid[d] + m0 * ac + (d ? 0.f : m1 * fb) + m2 * zz0;
-                    const float Pd = m0 + m2 * dzz;
-                    const float mg = fabsf(m2) * curv * 1.01f + 2e-4f * (1.f + fabsf(Pc)) + 0.01f;
-                    const float lo_d = bmin[d] - 1.f - mg - Pc, hi_d = bmax[d] + 1.f + mg - Pc;
-                    if (fabsf(Pd) < 1e-6f) {
-                        if (lo_d > 0.f || hi_d < 0.f) alo = ahi + 1.f;
-                    } else {
-                        // the margins above exceed the 1-ulp error of the reciprocal
-                        const float rp = __builtin_amdgcn_rcpf(Pd);
-                        float t0 = lo_d * rp, t1 = hi_d * rp;
-                        if (Pd < 0.f) { const float tt = t0; t0 = t1; t1 = tt; }
-                        alo = fmaxf(alo, ac + t0);
-                        ahi = fminf(ahi, ac + t1);
+                for (int h = 0; h < 2; h++) {
+                    const int ri = 2 * tid + h;
+                    if (ri < nchunk)
+                        npt[h] = clip_line(g, rows[base + ri], fbm, fbh, fb2m, fb2h, bmin, bmax, fx2,
+                                           fy2, wl2f, a0s[h]);
+                }
+                const int v = ((int)(npt[0] > 0) + (int)(npt[1] > 0)) << 20 | (npt[0] + npt[1]);
+                int incl = v;
+                for (int o = 1; o < 64; o <<= 1) {
+                    const int u = __shfl_up(incl, o);
+                    if (lane >= o) incl += u;
+                }
+                if (lane == 63) s_wsum[wave] = incl;
+                __syncthreads();
+                int woff = 0;
+                for (int w = 0; w < wave; w++) woff += s_wsum[w];
+                const int tot = s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
+                const int ex = woff + incl - v;
+                int off = ex >> 20, pre = ex & 0xfffff;
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    if (npt[h] > 0) {
+                        s_row[off] = (unsigned short)(base + 2 * tid + h);
+                        s_a0[off] = (short)a0s[h];
+                        s_pre[off] = pre;
+                        off++;
+                        pre += npt[h];
                     }
                 }
-            }
-            if (alo <= ahi) {
-                a0 = max((int)ceilf(alo - 1e-3f), -g.nxd2);
-                a1 = min((int)floorf(ahi + 1e-3f), g.nx - 1 - g.nxd2);
-            }
-            pass = a0 <= a1;
-        }
-        const unsigned long long mask = __ballot(pass);
-        const int pre = __popcll(mask & ((1ull << lane) - 1ull));
-        if (lane == 0) s_cnt[wave] = __popcll(mask);
-        __syncthreads();
-        int off = 0;
-        for (int w = 0; w < wave; w++) off += s_cnt[w];
-        const int npt = pass ? a1 - a0 + 1 : 0;  // points of the line
-        if (pass) {
-            s_row[off + pre] = (unsigned short)(base + tid);
-            s_a0[off + pre] = (short)a0;
-        }
-        // exclusive prefix of the point counts, in list order (the listed
-        // rows keep their thread order, so the prefix follows the ballot)
-        int incl = npt;
-        for (int o = 1; o < 64; o <<= 1) {
-            const int v = __shfl_up(incl, o);
-            if (lane >= o) incl += v;
-        }
-        if (lane == 63) s_wsum[wave] = incl;
-        __syncthreads();
-        int woff = 0;
-        for (int w = 0; w < wave; w++) woff += s_wsum[w];
-        if (pass) s_pre[off + pre] = woff + incl - npt;
-        const int n = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
-        const int total = s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
-        if (tid == 0) s_pre[n] = total;
-        __syncthreads();
-        {
-            // each thread takes an equal run of the listed points, in list order
-            const int p0 = (int)(((long long)tid * total) >> 8);
-            const int p1 = (int)(((long long)(tid + 1) * total) >> 8);
-            if (p0 < p1) {
-                int lo = 0, hi = n - 1;
-                while (lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if (s_pre[mid] <= p0) lo = mid;
-                    else hi = mid - 1;
+                n = tot >> 20;
+                total = tot & 0xfffff;
+                if (tid == 0) s_pre[n] = total;
+                __syncthreads();
+                // each thread takes an equal run of the listed points, in list
+                // order; q0: the listed line holding its first point
+                p0 = (int)(((long long)tid * total) >> 8);
+                p1 = (int)(((long long)(tid + 1) * total) >> 8);
+                if (p0 < p1) {
+                    int lo = 0, hi = n - 1;
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (s_pre[mid] <= p0) lo = mid;
+                        else hi = mid - 1;
+                    }
+                    q0 = lo;
                 }
-                int q = lo;
+            }
+            if (p0 < p1) {
+                int q = q0;
                 // the current row, and the next listed row loaded one row ahead
                 int r = s_row[q];
                 int abase = s_a0[q] - s_pre[q], pend = s_pre[q + 1];
@@ -1278,7 +1320,7 @@ __global__ void __launch_bounds__(256, ROT ? 1 : 4) k_grid_yp(const cplx<float>*
                     ph[1] = rows[r].ph[1];
                     ph[2] = rows[r].ph[2];
                 }
-                const cplx<float>* Kl = spec + ((size_t)r * g.ny + j) * g.nx + g.nxd2 + abase;
+                const cplx<float>* Kl = spec_j + (size_t)((unsigned)r & 0xffffu) * rstride + abase;
                 int abn = abase, pendn = pend;
                 double n0 = m0, n1 = m1, n2 = m2, n6 = m6, n8 = m8, nph[3] = {ph[0], ph[1], ph[2]};
                 const cplx<float>* Kn = Kl;
@@ -1294,7 +1336,7 @@ __global__ void __launch_bounds__(256, ROT ? 1 : 4) k_grid_yp(const cplx<float>*
                             nph[1] = Rn.ph[1];
                             nph[2] = Rn.ph[2];
                         }
-                        Kn = spec + ((size_t)rn * g.ny + j) * g.nx + g.nxd2 + abn;
+                        Kn = spec_j + (size_t)((unsigned)rn & 0xffffu) * rstride + abn;
                     }
                 };
                 load_next();
@@ -1384,23 +1426,28 @@ __global__ void __launch_bounds__(256, ROT ? 1 : 4) k_grid_yp(const cplx<float>*
                     splat(abase + p, kraw);
                 }
             }
+            __syncthreads();  // the sums of the chunk are complete; its list may be replaced
         }
-        __syncthreads();
-    }
-    const double vinv = ldexp(1.0, -sv), winv = ldexp(1.0, -sw);
+        // an opaque copy of the thread index: the voxel terms below are then
+        // evaluated per plane instead of being kept live (and spilled)
+        // across the plane loop
+        int vt = tid;
+        asm volatile("" : "+v"(vt));
 #pragma unroll
-    for (int k = 0; k < POWN; k++) {
-        const int v = tid + 256 * k;
-        const int lx = v % PX, lz = v / PX;
-        const int vx = x0 + lx, vz = z0 + lz;
-        if (vx >= g.nx || vz >= g.nz) continue;
-        const int li = lx + PSX * lz;
-        const float vr = (float)((double)(long long)acc[li] * vinv + (double)tiny[v]);
-        const float vi = (float)((double)(long long)acc[PPAD + li] * vinv + (double)tiny[PVOX + v]);
-        const float S = (float)((double)(long long)acc[2 * PPAD + li] * winv + (double)tiny[2 * PVOX + v]);
-        const size_t vidx = ((size_t)vz * g.ny + j) * g.nx + vx;
-        if (Sout) Sout[vidx] = S;
-        Vout[vidx] = voxel_epilogue<float>(g, vx, j, vz, cplx<float>{vr, vi}, S);
+        for (int k = 0; k < POWN; k++) {
+            const int v = vt + 256 * k;
+            const int lx = v % PX, lz = v / PX;
+            const int vx = x0 + lx, vz = z0 + lz;
+            if (vx >= g.nx || vz >= g.nz) continue;
+            const int li = lx + PSX * lz;
+            const float vr = (float)((double)(long long)acc[li] * vinv + (double)tiny[v]);
+            const float vi = (float)((double)(long long)acc[PPAD + li] * vinv + (double)tiny[PVOX + v]);
+            const float S = (float)((double)(long long)acc[2 * PPAD + li] * winv + (double)tiny[2 * PVOX + v]);
+            const size_t vidx = ((size_t)vz * g.ny + j) * g.nx + vx;
+            if (Sout) Sout[vidx] = S;
+            Vout[vidx] = voxel_epilogue<float>(g, vx, j, vz, cplx<float>{vr, vi}, S);
+        }
+        __syncthreads();  // the epilogue has read the sums before the next plane zeroes them
     }
 }
 

@@ -254,7 +254,31 @@ bool build_plan(const utr_params* p, int nrows, const double* angZ, const double
     P.all_y_exact = nrows > 0 && nrows <= 65535;
     for (int r = 0; r < nrows; r++)
         if (!P.rows[r].y_exact) P.all_y_exact = 0;
+    derive_epilogue(g);
     return true;
+}
+
+void derive_epilogue(GeomDev& g) {
+    const double fac2 = kPi * g.wl * g.zout_aver;
+    const double xa = g.xhi - g.xlo, ya = g.yhi - g.ylo, za = g.zhi - g.zlo;
+    // mode-1 NSR (UTR.cpp:1085-1110)
+    g.ep_nsr1 = std::sqrt(1.0 + g.asigma * g.asigma) * g.nsr2;
+    g.ep_dksi2 = fac2 / (xa * xa);
+    g.ep_deta2 = fac2 / (ya * ya);
+    g.ep_dzeta2 = fac2 / (za * za);
+    // InverseMLaplacian3D(V3, zhi-zlo, yhi-ylo, xhi-ylo, ...) (UTR.cpp:1184-1196)
+    const double fact = 4.0 * kPi * kPi, xl = g.xhi - g.ylo;
+    g.ep_lapx = fact / (xl * xl);
+    g.ep_lapy = fact / (ya * ya);
+    g.ep_lapz = fact / (za * za);
+    // CTFcorrection3D (UTR.h:86-131)
+    g.ep_ix2 = 1.0 / (xa * xa);
+    g.ep_iy2 = 1.0 / (ya * ya);
+    g.ep_iz2 = 1.0 / (za * za);
+    g.ep_fac2 = fac2;
+    g.ep_atans = std::atan(g.asigma);
+    g.ep_c3 = kPi * g.wl * g.wl * g.wl / 2.0 * g.cs3;
+    g.ep_c5 = kPi * g.wl * g.wl * g.wl * g.wl * g.wl / 3.0 * g.cs5;
 }
 
 }  // namespace utr

@@ -43,7 +43,20 @@ struct GeomDev {
     // sphere mask (DHT.cpp:1350-1416)
     int dht_sctf, no_mask;
     double critical_snr;
+    // geometry-only factors of the voxel epilogue, set by derive_epilogue()
+    // from the fields above (FP64; the kernels only multiply by them)
+    double ep_nsr1;                       // mode 1: sqrt(1 + asigma^2) nsr2
+    double ep_dksi2, ep_deta2, ep_dzeta2; // mode 1: pi wl zout_aver / aperture^2 (x, y, z)
+    double ep_lapx, ep_lapy, ep_lapz;     // InverseMLaplacian3D: 4 pi^2 / aperture^2 (x aperture xhi - ylo)
+    double ep_ix2, ep_iy2, ep_iz2;        // CTFcorrection3D: 1 / aperture^2
+    double ep_fac2;                       // pi wl zout_aver
+    double ep_atans;                      // atan(asigma)
+    double ep_c3, ep_c5;                  // pi wl^3 / 2 cs3, pi wl^5 / 3 cs5
 };
+
+// Fills the ep_* factors of g from its other fields; called whenever those
+// change.
+void derive_epilogue(GeomDev& g);
 
 // Per orientation row.  m: grid-unit forward map (rows dx1, dy1, dz1 over
 // a = i - nx/2, b = j - ny/2, zz physical).  minv: its inverse.
