@@ -292,6 +292,7 @@ struct utr_ctx {
     DevBuf<int> scales;       // fixed-point scales of the plane gridding
     int grid_planes = 0;      // y planes per workgroup of the plane gridding (0: default)
     DevBuf<int> tile_order;   // plane-gridding tiles by decreasing expected work
+    DevBuf<double> rowc;      // per row: m0 m1 m2 m6 m8 (padded to 6) for the plane gridding
     // FP64 form
     DevBuf<cplx<double>> spec64, V64, H64, twx64, twy64, twz64;
     DevBuf<double> S64, vol64;
@@ -546,7 +547,7 @@ int copy_to_host(utr_ctx* c, float* dst, const float* d_src, size_t n) {
 // the plane's rotation centre (nx/2, nz/2): the point density of the lines
 // falls off as 1/r, so the nearest tiles carry the most points and start
 // first.  Uploaded once at creation (finalize may run under graph capture).
-int upload_tile_order(utr_ctx* c) {
+int upload_plane_tables(utr_ctx* c) {
     const auto& g = c->plan.g;
     const int ntx = (g.nx + utr::PX - 1) / utr::PX, ntz = (g.nz + utr::PZ - 1) / utr::PZ;
     const int ntiles = ntx * ntz;
@@ -561,6 +562,16 @@ int upload_tile_order(utr_ctx* c) {
     std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return d2[a] < d2[b]; });
     UTR_HIP_TRY(c->tile_order.alloc(ntiles));
     UTR_HIP_TRY(hipMemcpy(c->tile_order.p, ord.data(), sizeof(int) * ntiles, hipMemcpyHostToDevice));
+    // the map coefficients the points read, 48 bytes a row
+    const int nr = c->plan.nrows;
+    std::vector<double> rc(6 * (size_t)nr, 0.0);
+    for (int r = 0; r < nr; r++) {
+        const double* m = c->plan.rows[r].m;
+        rc[6 * r + 0] = m[0]; rc[6 * r + 1] = m[1]; rc[6 * r + 2] = m[2];
+        rc[6 * r + 3] = m[6]; rc[6 * r + 4] = m[8];
+    }
+    UTR_HIP_TRY(c->rowc.alloc(rc.size()));
+    UTR_HIP_TRY(hipMemcpy(c->rowc.p, rc.data(), sizeof(double) * rc.size(), hipMemcpyHostToDevice));
     return UTR_OK;
 }
 
@@ -589,16 +600,22 @@ int launch_grid32(utr_ctx* c, float* d_S) {
         const long ngroups = (g.ny + pg - 1) / pg;
         const long nwg = 8L * ((ngroups + 7) / 8) * ntiles;
         if (nwg <= 0 || nwg > 0x7ffffff0L) return set_err(UTR_E_INVALID, "volume too large");
-        if (c->tile_order.n != (size_t)ntiles) return set_err(UTR_E_STATE, "plane-gridding tile order missing");
+        if (c->tile_order.n != (size_t)ntiles || c->rowc.n != 6 * (size_t)c->plan.nrows)
+            return set_err(UTR_E_STATE, "plane-gridding tables missing");
         utr::k_fx_scales<<<1, 256, 0, c->stream>>>(c->rowmax.p, c->plan.nrows, c->scales.p);
-        if (g.rot_centre_shift)
-            utr::k_grid_yp<true><<<utr::yp_grid((int)ntiles, (int)ngroups), 256, 0, c->stream>>>(
-                c->spec32.p, c->rows.p, c->plan.nrows, c->scales.p, g, c->V32.p, d_S, (int)ntiles,
-                (int)ngroups, pg, c->tile_order.p);
-        else
-            utr::k_grid_yp<false><<<utr::yp_grid((int)ntiles, (int)ngroups), 256, 0, c->stream>>>(
-                c->spec32.p, c->rows.p, c->plan.nrows, c->scales.p, g, c->V32.p, d_S, (int)ntiles,
-                (int)ngroups, pg, c->tile_order.p);
+        // the noise gate alone in the epilogue (the metric's mode 0 with NSR)
+        const bool gate = g.nsr2 != 0 && g.ctf_mode != 1 && !g.dht_sctf && !g.do_tie3d &&
+                          !g.do_invlap && !g.do_ctf3d;
+        const unsigned nb = utr::yp_grid((int)ntiles, (int)ngroups);
+#define UTR_GRID_YP(ROT, GATE)                                                                   \
+    utr::k_grid_yp<ROT, GATE><<<nb, 256, 0, c->stream>>>(c->spec32.p, c->rows.p, c->rowc.p,      \
+                                                         c->plan.nrows, c->scales.p, g, c->V32.p, \
+                                                         d_S, (int)ntiles, (int)ngroups, pg,      \
+                                                         c->tile_order.p)
+        if (g.rot_centre_shift) UTR_GRID_YP(true, false);
+        else if (gate) UTR_GRID_YP(false, true);
+        else UTR_GRID_YP(false, false);
+#undef UTR_GRID_YP
     } else {
         const long nblocks = (long)((g.nx + utr::SBX - 1) / utr::SBX) *
                              ((g.ny + utr::SBY - 1) / utr::SBY) * ((g.nz + utr::SBZ - 1) / utr::SBZ);
@@ -734,7 +751,7 @@ int utr_create(const utr_params* params, int nrows, const double* angZ_deg, cons
             c->scales.alloc(2) != hipSuccess)
             return fail(set_err(UTR_E_NOMEM, "device allocation of the FP32 buffers failed"));
         if (c->plan.all_y_exact) {
-            const int rc = upload_tile_order(c);
+            const int rc = upload_plane_tables(c);
             if (rc) return fail(rc);
         }
         // on the context's stream: it is non-blocking, so a null-stream memset
@@ -1020,7 +1037,7 @@ int utr_destroy(utr_ctx* c) {
     c->prep_aver.release();
     c->spec32.release(); c->V32.release(); c->H32.release(); c->S32.release(); c->vol32.release();
     c->twx32.release(); c->twy32.release(); c->twz32.release(); c->rowmax.release(); c->scales.release();
-    c->tile_order.release();
+    c->tile_order.release(); c->rowc.release();
     c->spec64.release(); c->V64.release(); c->H64.release(); c->S64.release(); c->vol64.release();
     c->twx64.release(); c->twy64.release(); c->twz64.release();
     for (auto& e : c->ev)

@@ -437,6 +437,14 @@ __device__ __forceinline__ GridPos grid_pos(const GeomDev& g, const double* m, i
 
 // --------------------------------------------------- voxel epilogue
 
+// 1/b: in FP32 from the hardware reciprocal and one Newton step (within an
+// ulp of the correctly rounded quotient), in FP64 the IEEE quotient
+__device__ __forceinline__ float recip_nr(float b) {
+    const float r = __builtin_amdgcn_rcpf(b);
+    return fmaf(fmaf(-b, r, 1.f), r, r);
+}
+__device__ __forceinline__ double recip_nr(double b) { return 1.0 / b; }
+
 // Sampling normalisation + noise filter (UTR.cpp:1050-1128) and the 3D
 // filters (UTR.cpp:1177-1201; UTR.h:15-185) for voxel (i, j, k).  Factors
 // that depend on the geometry only come precomputed in g.ep_* (FP64).
@@ -469,8 +477,8 @@ __device__ __forceinline__ cplx<T> voxel_epilogue(const GeomDev& g, int i, int j
             const double eta2 = j1 * j1 * g.ep_deta2 + zeta2;
             fnsr = T(g.ep_nsr1 * (double)sqrt(x) / (g.ep_dksi2 * i1 * i1 + eta2 + g.epsilon));
         }
-        const T y = sqrt(v.x * v.x + v.y * v.y) / fnsr;
-        const T f = (T(1) / x) * (T(0.5) + atan(T(100) * (y - target)) * inv_pi);
+        const T y = sqrt(v.x * v.x + v.y * v.y) * recip_nr(fnsr);
+        const T f = recip_nr(x) * (T(0.5) + atan(T(100) * (y - target)) * inv_pi);
         v.x *= f;
         v.y *= f;
     }
@@ -705,6 +713,7 @@ constexpr int SPZ = SPY * (SBY + 1);   // LDS stride along z
 constexpr int SPAD = SPZ * SBZ;        // LDS entries per component
 constexpr int S_LIST = 256;            // rows tested per chunk
 constexpr double S_TINY_D = 9.5367431640625e-07;  // 2^-20
+constexpr double S_TINY_1D = 9.765625e-04;         // 2^-10
 
 __device__ __forceinline__ int lds_idx(int lx, int ly, int lz) { return lx + SPY * ly + SPZ * lz; }
 
@@ -1200,9 +1209,16 @@ __host__ __device__ constexpr unsigned yp_grid(int ntiles, int ngroups) {
 // rank first: tile_order lists the tiles by decreasing expected point count
 // (the lines of all rows cross near the plane's centre), and the heaviest
 // tiles start first.
-template <bool ROT>
+//
+// rowc: the five map coefficients the points use, per row (m0 m1 m2 m6 m8,
+// padded to 6 doubles): a compact table that stays in the L1 cache.
+// GATE: the voxel epilogue is the noise gate alone (no 3D filter, mode
+// other than 1, nonzero NSR, not the sCTF): a straight-line instantiation
+// of voxel_epilogue's noise-gate branch (the same operations).
+template <bool ROT, bool GATE>
 __global__ void __launch_bounds__(256, ROT ? 1 : 4) k_grid_yp(const cplx<float>* __restrict__ spec,
-                                                 const RowParams* __restrict__ rows, int nrows,
+                                                 const RowParams* __restrict__ rows,
+                                                 const double* __restrict__ rowc, int nrows,
                                                  const int* __restrict__ scales, GeomDev g,
                                                  cplx<float>* __restrict__ Vout,
                                                  float* __restrict__ Sout, int ntiles, int ngroups,
@@ -1229,6 +1245,7 @@ __global__ void __launch_bounds__(256, ROT ? 1 : 4) k_grid_yp(const cplx<float>*
     const int sv = scales[0], sw = scales[1];
     const double vscale = ldexp(1.0, sv), wscale = ldexp(1.0, sw);
     const double vinv = ldexp(1.0, -sv), winv = ldexp(1.0, -sw);
+    const double xmaxd = (double)(g.nx - 2), zmaxd = (double)(g.nz - 2);
 
     const float fx2 = (float)(g.fxst * g.fxst), fy2 = (float)(g.fyst * g.fyst);
     const float wl2f = (float)g.wl2;
@@ -1310,47 +1327,44 @@ __global__ void __launch_bounds__(256, ROT ? 1 : 4) k_grid_yp(const cplx<float>*
             }
             if (p0 < p1) {
                 int q = q0;
-                // the current row, and the next listed row loaded one row ahead
+                // the current line: its row's coefficients, first a and end;
+                // the next listed line's spectrum pointer (for the one-ahead
+                // value load)
                 int r = s_row[q];
                 int abase = s_a0[q] - s_pre[q], pend = s_pre[q + 1];
-                double m0 = rows[r].m[0], m1 = rows[r].m[1], m2 = rows[r].m[2], m6 = rows[r].m[6], m8 = rows[r].m[8];
+                double m0, m1, m2, m6, m8;
                 double ph[3] = {0.0, 0.0, 0.0};
-                if (ROT) {
-                    ph[0] = rows[r].ph[0];
-                    ph[1] = rows[r].ph[1];
-                    ph[2] = rows[r].ph[2];
-                }
-                const cplx<float>* Kl = spec_j + (size_t)((unsigned)r & 0xffffu) * rstride + abase;
-                int abn = abase, pendn = pend;
-                double n0 = m0, n1 = m1, n2 = m2, n6 = m6, n8 = m8, nph[3] = {ph[0], ph[1], ph[2]};
-                const cplx<float>* Kn = Kl;
-                auto load_next = [&]() {
-                    if (q + 1 < n) {
-                        const int rn = s_row[q + 1];
-                        abn = s_a0[q + 1] - s_pre[q + 1];
-                        pendn = s_pre[q + 2];
-                        const RowParams& Rn = rows[rn];
-                        n0 = Rn.m[0]; n1 = Rn.m[1]; n2 = Rn.m[2]; n6 = Rn.m[6]; n8 = Rn.m[8];
-                        if (ROT) {
-                            nph[0] = Rn.ph[0];
-                            nph[1] = Rn.ph[1];
-                            nph[2] = Rn.ph[2];
-                        }
-                        Kn = spec_j + (size_t)((unsigned)rn & 0xffffu) * rstride + abn;
+                auto load_row = [&]() {
+                    const double* c6 = rowc + 6 * r;
+                    m0 = c6[0]; m1 = c6[1]; m2 = c6[2]; m6 = c6[3]; m8 = c6[4];
+                    if (ROT) {
+                        ph[0] = rows[r].ph[0];
+                        ph[1] = rows[r].ph[1];
+                        ph[2] = rows[r].ph[2];
                     }
                 };
-                load_next();
+                load_row();
+                const cplx<float>* Kl = spec_j + (size_t)((unsigned)r & 0xffffu) * rstride + abase;
+                const cplx<float>* Kn = Kl;
+                int rn = r;
+                auto next_line = [&]() {
+                    if (q + 1 < n) {
+                        rn = s_row[q + 1];
+                        Kn = spec_j + (size_t)((unsigned)rn & 0xffffu) * rstride + (s_a0[q + 1] - s_pre[q + 1]);
+                    }
+                };
+                next_line();
                 auto splat = [&](const int a, const cplx<float> kraw) {
                     const double ad = (double)a;
                     const double qx = __dmul_rn(g.fxst, ad);
                     const double zz = g.escc ? __dmul_rn(-g.wl2, __fma_rn(qx, qx, qy2)) : 0.0;
                     const double ox = __fma_rn(ad, m0, __fma_rn(bd, m1, __dmul_rn(zz, m2)));
                     const double oz = __fma_rn(ad, m6, __dmul_rn(zz, m8));
-                    const double dx1 = __dadd_rn((double)g.nxd2, ox);
-                    const double dz1 = __dadd_rn((double)g.nzd2, oz);
+                    const double dx1 = __dadd_rn(g.d_nxd2, ox);
+                    const double dz1 = __dadd_rn(g.d_nzd2, oz);
                     const double fnn = floor(dz1), fii = floor(dx1);
-                    if (fnn < 0.0 || fnn > (double)(g.nz - 2)) return;
-                    if (fii < 0.0 || fii > (double)(g.nx - 2)) return;
+                    if (fnn < 0.0 || fnn > zmaxd) return;
+                    if (fii < 0.0 || fii > xmaxd) return;
                     const int ii = (int)fii - x0, nn = (int)fnn - z0;
                     if (ii < -1 || ii >= PX || nn < -1 || nn >= PZ) return;
                     const double fx = dx1 - fii, fz = dz1 - fnn;
@@ -1367,20 +1381,31 @@ __global__ void __launch_bounds__(256, ROT ? 1 : 4) k_grid_yp(const cplx<float>*
                     const double kxs = (double)kv.x * vscale, kys = (double)kv.y * vscale;
                     long long tr[4], ti[4], tw[4];
                     float wc[4];
-                    bool big[4], tny[4];
+                    bool big[4], tny[4] = {false, false, false, false};
                     bool any_tiny = false;
+                    double wds[4];
 #pragma unroll
                     for (int c = 0; c < 4; c++) {
                         const int cx = c & 1, cz = c >> 1;
-                        const bool inb = (unsigned)(ii + cx) < (unsigned)PX && (unsigned)(nn + cz) < (unsigned)PZ;
+                        big[c] = (unsigned)(ii + cx) < (unsigned)PX && (unsigned)(nn + cz) < (unsigned)PZ;
                         const double wd = wxs[cx] * 1.0 * wzs[cz];
-                        big[c] = inb && wd >= S_TINY_D;
-                        tny[c] = inb && wd != 0.0 && wd < S_TINY_D;
-                        any_tiny |= tny[c];
+                        wds[c] = wd;
                         wc[c] = (float)wd;
                         tr[c] = fx_term(wd, kxs);
                         ti[c] = fx_term(wd, kys);
                         tw[c] = fx_term(wd, wscale);
+                    }
+                    // with both 1D weights of each axis at least 2^-10 every corner
+                    // weight is at least 2^-20: the per-corner tests only run when a
+                    // fraction lies within 2^-10 of a grid plane
+                    if (!(fx >= S_TINY_1D && fx <= 1.0 - S_TINY_1D && fz >= S_TINY_1D && fz <= 1.0 - S_TINY_1D)) {
+#pragma unroll
+                        for (int c = 0; c < 4; c++) {
+                            const bool inb = big[c];
+                            big[c] = inb && wds[c] >= S_TINY_D;
+                            tny[c] = inb && wds[c] != 0.0 && wds[c] < S_TINY_D;
+                            any_tiny |= tny[c];
+                        }
                     }
                     const int li0 = ii + PSX * nn;
 #pragma unroll
@@ -1407,18 +1432,14 @@ __global__ void __launch_bounds__(256, ROT ? 1 : 4) k_grid_yp(const cplx<float>*
                 cplx<float> kv = Kl[p0];
                 for (int p = p0; p < p1; p++) {
                     if (p == pend) {
-                        // enter the next listed row
+                        // enter the next listed line
                         q++;
-                        abase = abn;
-                        pend = pendn;
-                        m0 = n0; m1 = n1; m2 = n2; m6 = n6; m8 = n8;
-                        if (ROT) {
-                            ph[0] = nph[0];
-                            ph[1] = nph[1];
-                            ph[2] = nph[2];
-                        }
+                        r = rn;
                         Kl = Kn;
-                        load_next();
+                        abase = s_a0[q] - s_pre[q];
+                        pend = s_pre[q + 1];
+                        load_row();
+                        next_line();
                     }
                     const cplx<float> kraw = kv;
                     // the next point's value: this row, or the next one
@@ -1445,7 +1466,20 @@ __global__ void __launch_bounds__(256, ROT ? 1 : 4) k_grid_yp(const cplx<float>*
             const float S = (float)((double)(long long)acc[2 * PPAD + li] * winv + (double)tiny[2 * PVOX + v]);
             const size_t vidx = ((size_t)vz * g.ny + j) * g.nx + vx;
             if (Sout) Sout[vidx] = S;
-            Vout[vidx] = voxel_epilogue<float>(g, vx, j, vz, cplx<float>{vr, vi}, S);
+            if (GATE) {
+                // voxel_epilogue's noise-gate branch (UTR.cpp:1050-1128)
+                cplx<float> o = {0.f, 0.f};
+                if (S != 0.f) {
+                    const float fnsr = (float)(g.nsr2 * (double)sqrtf(S));
+                    const float y = sqrtf(vr * vr + vi * vi) * recip_nr(fnsr);
+                    const float f = recip_nr(S) * (0.5f + atanf(100.f * (y - (float)kTargetSNR)) *
+                                                           0.318309886183790671538f);
+                    o = {vr * f, vi * f};
+                }
+                Vout[vidx] = o;
+            } else {
+                Vout[vidx] = voxel_epilogue<float>(g, vx, j, vz, cplx<float>{vr, vi}, S);
+            }
         }
         __syncthreads();  // the epilogue has read the sums before the next plane zeroes them
     }

@@ -279,6 +279,8 @@ void derive_epilogue(GeomDev& g) {
     g.ep_atans = std::atan(g.asigma);
     g.ep_c3 = kPi * g.wl * g.wl * g.wl / 2.0 * g.cs3;
     g.ep_c5 = kPi * g.wl * g.wl * g.wl * g.wl * g.wl / 3.0 * g.cs5;
+    g.d_nxd2 = (double)g.nxd2;
+    g.d_nzd2 = (double)g.nzd2;
 }
 
 }  // namespace utr

@@ -52,6 +52,7 @@ struct GeomDev {
     double ep_fac2;                       // pi wl zout_aver
     double ep_atans;                      // atan(asigma)
     double ep_c3, ep_c5;                  // pi wl^3 / 2 cs3, pi wl^5 / 3 cs5
+    double d_nxd2, d_nzd2;                // nx/2, nz/2 as doubles (grid centre of the scatter)
 };
 
 // Fills the ep_* factors of g from its other fields; called whenever those
