@@ -3,9 +3,11 @@
 
 Workload (the reference metric's configuration): 256x256 images at 0.6 A,
 360 orientation rows about y (1 degree steps), 200 nm defocus, 256^3 volume,
-300 keV, UTR mode 0 with Ewald sphere correction, Poisson noise at 36
-counts/pixel (NSR 1/6) on seeded synthetic projections of 3000 atoms
-(C:N:O:S = 62:17:19:2) in an 8 nm sphere.
+300 keV, UTR mode 0 with Ewald sphere correction; the images are the GPU
+multislice forward model's (msct_*) of 3000 seeded atoms (C:N:O:S =
+62:17:19:2) in an 8 nm sphere (those inside the 75 A that keeps every
+orientation in the 153.6 A cube), with Poisson noise at 100 e/A^2 (36
+counts/pixel, NSR 1/6).
 
 One step = one full reconstruction: the 2D stage of all 360 rows, gridding,
 normalisation and noise filter, Hermitian symmetrisation, 3D inverse FFT,
@@ -55,6 +57,41 @@ def cpu_model():
     return "unknown"
 
 
+def config_errors(nthreads):
+    """Relative L2 and max-abs error of the FP32 form on the named
+    configurations' multislice images: C1 and C2 against the FP64 CPU
+    oracle, C3 (512^3) against the plain FP64 GPU form."""
+    import torch
+    import oracle
+    import utr_public_amd as U
+    from utr_public_amd import synth
+    res = {}
+    for name in ("c1", "c2", "c3"):
+        p, rows, at = synth.config(name)
+        imgs = synth.multislice_images_for(p, rows, at, seed=1)
+        d_img = torch.from_numpy(imgs).cuda()
+        with U.Reconstructor(p, rows, U.UTR_FP32) as r:
+            d_vol = torch.empty(r.shape, dtype=torch.float32, device="cuda")
+            r.add_images_device(d_img.data_ptr(), len(rows["angY"]))
+            r.finalize_device(d_vol.data_ptr())
+            v32 = d_vol.cpu().numpy().astype(np.float64)
+        del d_img
+        ts = time.perf_counter()
+        if name == "c3":
+            ref = U.reconstruct(p, rows, imgs, U.UTR_FP64, f64_out=True)
+            what = "FP64 GPU form"
+        else:
+            ref = oracle.reconstruct(p, rows, imgs, nthreads=nthreads)
+            what = "FP64 CPU oracle"
+        res[name] = {"volume": list(v32.shape), "rows": len(rows["angY"]), "reference": what,
+                     "rel_l2": float(np.linalg.norm(v32 - ref) / np.linalg.norm(ref)),
+                     "max_abs": float(np.abs(v32 - ref).max()), "max_abs_ref": float(np.abs(ref).max()),
+                     "reference_seconds": round(time.perf_counter() - ts, 2)}
+        log("config %s: rel-L2 %.2e vs %s" % (name, res[name]["rel_l2"], what))
+        del ref, v32
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,8 +131,8 @@ def main():
     p, rows, at = synth.config("metric")
     A = len(rows["angY"])
     t0 = time.time()
-    imgs = synth.images_for(p, rows, at, seed=1)
-    log("synthetic inputs: %d x %d x %d in %.1f s" % (imgs.shape + (time.time() - t0,)))
+    imgs = synth.multislice_images_for(p, rows, at, seed=1)
+    log("multislice inputs: %d x %d x %d in %.1f s" % (imgs.shape + (time.time() - t0,)))
     d_img = torch.from_numpy(imgs).cuda()
     rec = U.Reconstructor(p, rows, U.UTR_FP32)
     nz, ny, nx = rec.shape
@@ -413,13 +450,14 @@ def main():
             "max_abs": float(np.abs(vol32 - ref).max()), "max_abs_ref": float(np.abs(ref).max()),
             "oracle_seconds": round(to, 2), "oracle_threads": nthr}
         del ref
+        extra["configs"] = config_errors(nthr)
 
     out = {
         "metric": "angle*voxels/s, UTR 256^3 x 360 angles, FP32 (rel-L2 vs FP64 reported)",
         "value": value, "unit": "angle*voxels/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic (seeded projections of 3000 random atoms, Poisson noise)",
+        "data": "synthetic (GPU multislice images of 3000 seeded random atoms, Poisson noise at 100 e/A^2)",
         "config": {"workload": "UTR 256^3 x 360 rows about y, 256^2 px at 0.6 A, 300 keV, "
                                "mode 0 + ESCC, NSR 1/6",
                    "rows": A, "image": [ny, nx], "volume": [nz, ny, nx]},

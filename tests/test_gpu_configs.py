@@ -1,51 +1,80 @@
-"""The larger configurations, where the CPU oracle would take minutes: the
-FP32 form is checked against the plain FP64 GPU form, which the smaller
-cases pin to the oracle at 1e-10 (tests/test_gpu_parity.py).
-  C2: 256^3 from 720 rows (each angle at defocus 0 and 2000 A, flattened)
-  C3: 512^3 from 720 rows
-Bar: relative L2 <= 1e-5, finite output, and a bit-identical repeat."""
+"""The named configurations on the images they name: the GPU multislice
+forward model's images of the seeded atom sets (synth.multislice_images_for),
+reconstructed by the FP32 form and checked against the FP64 references.
+  C1: 128^3 from 36 rows about y over [0,180), 200 keV, no noise, against
+      the FP64 CPU oracle
+  C2: 256^3 from 720 rows (each angle at defocus 0 and 2000 A), 300 keV,
+      Poisson noise at 100 e/A^2, against the FP64 CPU oracle (16 threads)
+  C3: 512^3 from 720 rows, against the plain FP64 GPU form (the CPU oracle
+      needs minutes and tens of GB at this size; the smaller cases pin the
+      FP64 form to the oracle at 1e-10, tests/test_gpu_parity.py)
+Bar: relative L2 <= 1e-5, finite output, and a bit-identical repeat of the
+FP32 form.  Each case prints its relative L2 and max-abs error."""
+import os
+
 import numpy as np
 import pytest
 import torch
 
+import oracle
 import utr_public_amd as U
 from utr_public_amd import synth
 
 pytestmark = pytest.mark.gpu
 
+TOL = 1e-5
 
-def run_pair(name):
-    p, rows, at = synth.config(name)
-    imgs = synth.images_for(p, rows, at, seed=1)
+
+def fp32_twice(p, rows, imgs):
     A = len(rows["angY"])
-    print("config", name, imgs.shape, flush=True)
     d_img = torch.from_numpy(imgs).cuda()
-    out = {}
-    for form in (U.UTR_FP32, U.UTR_FP64):
-        with U.Reconstructor(p, rows, form) as r:
-            d_vol = torch.empty(r.shape, dtype=torch.float32, device="cuda")
+    with U.Reconstructor(p, rows, U.UTR_FP32) as r:
+        d_vol = torch.empty(r.shape, dtype=torch.float32, device="cuda")
+        outs = []
+        for _ in range(2):
+            r.reset()
             r.add_images_device(d_img.data_ptr(), A)
             r.finalize_device(d_vol.data_ptr())
             torch.cuda.synchronize()
-            out[form] = d_vol.cpu().numpy()
-            if form == U.UTR_FP32:
-                r.reset()
-                r.add_images_device(d_img.data_ptr(), A)
-                r.finalize_device(d_vol.data_ptr())
-                torch.cuda.synchronize()
-                again = d_vol.cpu().numpy()
-                assert np.array_equal(again.view(np.uint32), out[form].view(np.uint32))
-    a = out[U.UTR_FP32].astype(np.float64)
-    b = out[U.UTR_FP64].astype(np.float64)
+            outs.append(d_vol.cpu().numpy())
+    assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
+    return outs[0]
+
+
+def report(name, v32, ref, what):
+    a = v32.astype(np.float64)
+    b = np.asarray(ref, np.float64)
     assert np.isfinite(a).all()
     e = float(np.linalg.norm(a - b) / np.linalg.norm(b))
-    print("  rel-L2 fp32 vs fp64: %.2e  max-abs %.2e" % (e, np.abs(a - b).max()), flush=True)
-    assert e < 1e-5
+    m = float(np.abs(a - b).max())
+    print("%s: FP32 vs %s rel-L2 %.2e, max-abs %.2e (max |ref| %.3g)" % (name, what, e, m, np.abs(b).max()),
+          flush=True)
+    assert e <= TOL
+    return e
 
 
-def test_c2_720_rows_two_defocus():
-    run_pair("c2")
+def config_images(name, seed=1):
+    p, rows, at = synth.config(name)
+    imgs = synth.multislice_images_for(p, rows, at, seed=seed)
+    print("config %s: %s multislice images, %.0f e/A^2, mean %.4f" % (name, imgs.shape, at["dose"],
+                                                                      float(imgs.mean())), flush=True)
+    assert np.isfinite(imgs).all() and 0.5 < float(imgs.mean()) < 1.5
+    return p, rows, imgs
 
 
-def test_c3_512_cube_720_rows():
-    run_pair("c3")
+def test_c1_multislice_against_oracle():
+    p, rows, imgs = config_images("c1")
+    v32 = fp32_twice(p, rows, imgs)
+    report("C1", v32, oracle.reconstruct(p, rows, imgs, nthreads=min(16, os.cpu_count() or 1)), "FP64 oracle")
+
+
+def test_c2_multislice_against_oracle():
+    p, rows, imgs = config_images("c2")
+    v32 = fp32_twice(p, rows, imgs)
+    report("C2", v32, oracle.reconstruct(p, rows, imgs, nthreads=min(16, os.cpu_count() or 1)), "FP64 oracle")
+
+
+def test_c3_multislice_against_fp64_form():
+    p, rows, imgs = config_images("c3")
+    v32 = fp32_twice(p, rows, imgs)
+    report("C3", v32, U.reconstruct(p, rows, imgs, U.UTR_FP64, f64_out=True), "FP64 GPU form")

@@ -1,8 +1,10 @@
 """Seeded synthetic UTR inputs (ctypes over _lib/libutr_synth.so).
 
-Atom sets and projection images of the shape and statistics the UTR
-configurations describe (flat-field corrected intensities around 1, optional
-Poisson noise), standing in for MsctKirkland output.  Atom rotation follows
+Atom sets and images of the shape and statistics the UTR configurations
+describe: analytic projections (project, images_for; flat-field corrected
+intensities around 1, optional Poisson noise) for CPU-side and shape tests,
+and the GPU multislice forward model's images (multislice_images_for), the
+MsctKirkland output the configurations name.  Atom rotation follows
 TemsimLibrary/autosliccmd.cpp:734-748.
 """
 import ctypes
@@ -85,6 +87,8 @@ def config(name, seed=1):
     c1     : 128^3 from 36 rows about y over [0,180), 50 nm defocus.
     c2     : metric geometry with 720 rows (defocus 0 and 2000 A per angle).
     c3     : 512^3 from 720 rows about y over [0,360).
+    The atom spec also carries the config's beam energy (kev) and dose
+    (e/A^2) for multislice_images_for.
     """
     from . import default_params
     if name == "c1":
@@ -92,7 +96,8 @@ def config(name, seed=1):
                            escc=1, nsr=0.0, ctf_mode=0)
         angY = np.arange(36) * 5.0
         rows = dict(angZ=np.zeros(36), angY=angY, defocus=np.full(36, 500.0))
-        at = dict(natoms=400, shape="cube", size=40.0, min_sep=1.4, species=([6], [1.0]))
+        at = dict(natoms=400, shape="cube", size=40.0, min_sep=1.4, species=([6], [1.0]), kev=200.0,
+                  dose=0.0)
     elif name in ("metric", "c2"):
         p = default_params(nx=256, ny=256, xst=0.6, zlo=-76.8, zhi=76.8, wl=0.01968749,
                            escc=1, nsr=1.0 / 6.0, ctf_mode=0)
@@ -103,13 +108,15 @@ def config(name, seed=1):
             angY = np.repeat(np.arange(360) * 1.0, 2)
             df = np.tile([0.0, 2000.0], 360)
             rows = dict(angZ=np.zeros(720), angY=angY, defocus=df)
-        at = dict(natoms=3000, shape="sphere", size=80.0, min_sep=0.0, species=CNOS)
+        at = dict(natoms=3000, shape="sphere", size=80.0, min_sep=0.0, species=CNOS, kev=300.0,
+                  dose=100.0)
     elif name == "c3":
         p = default_params(nx=512, ny=512, xst=0.6, zlo=-153.6, zhi=153.6, wl=0.01968749,
                            escc=1, nsr=1.0 / 6.0, ctf_mode=0)
         angY = np.arange(720) * 0.5
         rows = dict(angZ=np.zeros(720), angY=angY, defocus=np.full(720, 2000.0))
-        at = dict(natoms=12000, shape="sphere", size=160.0, min_sep=0.0, species=CNOS)
+        at = dict(natoms=12000, shape="sphere", size=160.0, min_sep=0.0, species=CNOS, kev=300.0,
+                  dose=100.0)
     else:
         raise KeyError(name)
     return p, rows, at
@@ -140,6 +147,58 @@ def c4_jobs(seed=1):
         at = dict(natoms=natoms, shape="cube", size=n * xst * 0.8, min_sep=0.0, species=CNOS)
         jobs.append((p, rows, at))
     return jobs
+
+
+def multislice_images_for(params, rows, at, seed=1, dose=None, kev=None, deltaz=1.0):
+    """Images of a config from the GPU multislice forward model (msct_*): the
+    config's seeded atom set inside the params' cube (nx * xst per side),
+    imaged at every row's orientation and defocus, then Poisson counts at
+    `dose` electrons per A^2 (default: the config's, none if it has none).
+    Atoms farther than 0.49 of the cube side from the centre are left out,
+    so that every orientation keeps them inside the cube (the multislice
+    stops on an atom outside it, as autosliccmd does).  Returns float32
+    [rows, ny, nx] in row order.  Needs a GPU."""
+    import torch
+    from . import Multislice, default_msct_params
+    nx, ny, xst = params["nx"], params["ny"], params["xst"]
+    if nx != ny:
+        raise ValueError("multislice images need a square image")
+    L = nx * xst
+    if kev is None:
+        kev = at.get("kev")
+    if kev is None:
+        # invert wl = 12.3984244 / sqrt(kev (2 m0c2 + kev)) (slicelib.cpp:2891-2900)
+        m0c2 = 510.99906
+        kev = -m0c2 + (m0c2 * m0c2 + (12.3984244 / params["wl"]) ** 2) ** 0.5
+    if dose is None:
+        dose = at.get("dose", 0.0)
+    xyz, Z = atoms(seed, at["natoms"], at["shape"], at["size"], at["min_sep"], at["species"])
+    keep = np.sqrt((xyz ** 2).sum(1)) <= 0.49 * L
+    xyz, Z = xyz[keep], Z[keep]
+    h = L / 2
+    spec = dict(Z=Z, x=(xyz[:, 0] + h).astype(np.float32), y=(xyz[:, 1] + h).astype(np.float32),
+                z=(xyz[:, 2] + h).astype(np.float32), occ=np.ones(len(Z), np.float32))
+    mp = default_msct_params(nx=nx, ny=ny, ctblength=L, deltaz=deltaz, kev=float(kev))
+    angZ = np.asarray(rows["angZ"], np.float64)
+    angY = np.asarray(rows["angY"], np.float64)
+    df = np.asarray(rows["defocus"], np.float64)
+    out = torch.empty((len(df), ny, nx), dtype=torch.float32, device="cuda")
+    m = Multislice(mp, spec)
+    try:
+        for d in np.unique(df):
+            idx = np.nonzero(df == d)[0]
+            part = torch.empty((len(idx), 1, ny, nx), dtype=torch.float32, device="cuda")
+            m.images(angZ[idx], angY[idx], [float(d)], out=part)
+            out[torch.from_numpy(idx).cuda()] = part[:, 0]
+    finally:
+        m.close()
+    if dose > 0:
+        counts = dose * xst * xst
+        g = torch.Generator(device="cuda")
+        g.manual_seed(int(seed))
+        out = torch.poisson(torch.clamp(out, min=0.0) * counts, generator=g) / counts
+    torch.cuda.synchronize()
+    return out.cpu().numpy()
 
 
 def images_for(params, rows, at, seed=1, dose=None):
