@@ -293,7 +293,6 @@ struct utr_ctx {
     int grid_planes = 0;      // y planes per workgroup of the plane gridding (0: default)
     DevBuf<int> tile_order;   // plane-gridding tiles by decreasing expected work
     DevBuf<double> rowc;      // per row: m0 m1 m2 m6 m8 (padded to 6) for the plane gridding
-    DevBuf<cplx<float>> w256;  // exp(-2 pi i j / 256): the 16 x 16 register FFTs (256-point axes)
     // FP64 form
     DevBuf<cplx<double>> spec64, V64, H64, twx64, twy64, twz64;
     DevBuf<double> S64, vol64;
@@ -391,14 +390,7 @@ int launch_3d(utr_ctx* c, const cplx<T>* d_V, cplx<T>* d_H, TO* d_vol, const cpl
                                                                             g.nz);              \
         ok = true;                                                                              \
     }
-    if constexpr (std::is_same<T, float>::value) {
-        if (g.ny == 256 && c->w256.p) {
-            const int gx = (W + 15) / 16;
-            utr::k_fft_y256<<<utr::xcd_grid(gx * g.nz), 256, 0, c->stream>>>(d_H, g, c->w256.p, gx, g.nz);
-            ok = true;
-        }
-    }
-    if (!ok) UTR_DISPATCH_N(g.ny, UTR_FY)
+    UTR_DISPATCH_N(g.ny, UTR_FY)
 #undef UTR_FY
     if (!ok) return set_err(UTR_E_UNSUPPORTED, "unsupported image height for the FFT");
     ok = false;
@@ -762,14 +754,6 @@ int utr_create(const utr_params* params, int nrows, const double* angZ_deg, cons
             const int rc = upload_plane_tables(c);
             if (rc) return fail(rc);
         }
-        if (c->plan.g.nx == 256 || c->plan.g.ny == 256 || c->plan.g.nz == 256) {
-            std::vector<cplx<float>> w(256);
-            for (int j = 0; j < 256; j++)
-                w[j] = {(float)std::cos(-2.0 * utr::kPi * j / 256.0), (float)std::sin(-2.0 * utr::kPi * j / 256.0)};
-            if (c->w256.alloc(256) != hipSuccess ||
-                hipMemcpy(c->w256.p, w.data(), sizeof(cplx<float>) * 256, hipMemcpyHostToDevice) != hipSuccess)
-                return fail(set_err(UTR_E_NOMEM, "twiddle table upload failed"));
-        }
         // on the context's stream: it is non-blocking, so a null-stream memset
         // would not be ordered before the kernels launched on it
         if (hipMemsetAsync(c->rowmax.p, 0, sizeof(unsigned) * nrows, c->stream) != hipSuccess)
@@ -1053,7 +1037,7 @@ int utr_destroy(utr_ctx* c) {
     c->prep_aver.release();
     c->spec32.release(); c->V32.release(); c->H32.release(); c->S32.release(); c->vol32.release();
     c->twx32.release(); c->twy32.release(); c->twz32.release(); c->rowmax.release(); c->scales.release();
-    c->tile_order.release(); c->rowc.release(); c->w256.release();
+    c->tile_order.release(); c->rowc.release();
     c->spec64.release(); c->V64.release(); c->H64.release(); c->S64.release(); c->vol64.release();
     c->twx64.release(); c->twy64.release(); c->twz64.release();
     for (auto& e : c->ev)

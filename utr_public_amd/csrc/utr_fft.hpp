@@ -213,74 +213,4 @@ __device__ __forceinline__ void fft_lds(cplx<T>*& a, cplx<T>*& b, int nb,
     fft_passes<T, N, 0>(a, b, nb, tw, sign, tid, nthr);
 }
 
-// ------------------------------------------ 256 points as 16 x 16 in registers
-// X[k] = sum_n x[n] W^(n k), W = exp(sign 2 pi i / 256), with n = t + 16 m and
-// k = k2 + 16 k1:
-//   X[k2 + 16 k1] = sum_t W16^(t k1) W^(t k2) sum_m x[t + 16 m] W16^(m k2).
-// Sixteen threads share one sequence; thread t holds x[t + 16 m] in v[m]:
-// a 16-point DFT over m in registers, the twiddles W^(t k2), one transpose
-// through LDS (thread t then holds the k2 = t column), and a 16-point DFT
-// over t leave X[t + 16 k1] in v[k1].  One LDS round trip and one barrier
-// per transform, against a write and a read per radix pass for the Stockham
-// passes above.
-
-// 16-point DFT in registers (n = n1 + 4 n2, k = 4 k1 + k2), natural order in
-// and out.
-template <typename T>
-__device__ __forceinline__ void dft16_reg(cplx<T> (&v)[16], int sign) {
-    // W16^j = (cos 2 pi j / 16, sign sin 2 pi j / 16) for the products n1 k2
-    const T c1 = T(0.92387953251128675613), s1 = T(0.38268343236508977173);
-    const T h = T(0.70710678118654752440);
-    cplx<T> A[4][4];
-#pragma unroll
-    for (int n1 = 0; n1 < 4; n1++) {
-        cplx<T> u[4] = {v[n1], v[n1 + 4], v[n1 + 8], v[n1 + 12]};
-        dft_small<T, 4>(u, sign);
-#pragma unroll
-        for (int k2 = 0; k2 < 4; k2++) A[n1][k2] = u[k2];
-    }
-    const T sg = T(sign);
-    // twiddles W16^(n1 k2): j = 1, 2, 3 (n1 = 1), 2, 4, 6 (n1 = 2), 3, 6, 9 (n1 = 3)
-    const cplx<T> w1 = {c1, sg * s1}, w2 = {h, sg * h}, w3 = {s1, sg * c1};
-    A[1][1] = cmul(A[1][1], w1);
-    A[1][2] = cmul(A[1][2], w2);
-    A[1][3] = cmul(A[1][3], w3);
-    A[2][1] = cmul(A[2][1], w2);
-    A[2][2] = mul_si(A[2][2], sign);
-    A[2][3] = cmul(A[2][3], cplx<T>{-h, sg * h});
-    A[3][1] = cmul(A[3][1], w3);
-    A[3][2] = cmul(A[3][2], cplx<T>{-h, sg * h});
-    A[3][3] = cmul(A[3][3], cplx<T>{-c1, -sg * s1});
-#pragma unroll
-    for (int k2 = 0; k2 < 4; k2++) {
-        cplx<T> u[4] = {A[0][k2], A[1][k2], A[2][k2], A[3][k2]};
-        dft_small<T, 4>(u, sign);
-#pragma unroll
-        for (int k1 = 0; k1 < 4; k1++) v[4 * k1 + k2] = u[k1];
-    }
-}
-
-// The 256-point transform of the sequence whose sixteen threads (index t)
-// hold v as above.  w256[j] = exp(-2 pi i j / 256) (conjugated here for
-// sign +1); lds: the sequence's 16 x 17 complex scratch (row stride 17 keeps
-// the transposed reads on distinct banks).  Contains one __syncthreads():
-// every thread of the workgroup must call it.
-template <typename T>
-__device__ __forceinline__ void fft256_reg(cplx<T> (&v)[16], int t, cplx<T>* lds,
-                                           const cplx<T>* __restrict__ w256, int sign) {
-    dft16_reg(v, sign);
-#pragma unroll
-    for (int k2 = 1; k2 < 16; k2++) {
-        cplx<T> w = w256[(t * k2) & 255];
-        if (sign > 0) w.y = -w.y;
-        v[k2] = cmul(v[k2], w);
-    }
-#pragma unroll
-    for (int k2 = 0; k2 < 16; k2++) lds[t * 17 + k2] = v[k2];
-    __syncthreads();
-#pragma unroll
-    for (int t2 = 0; t2 < 16; t2++) v[t2] = lds[t2 * 17 + t];
-    dft16_reg(v, sign);
-}
-
 }  // namespace utr

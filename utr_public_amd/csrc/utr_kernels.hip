@@ -1630,30 +1630,6 @@ __global__ void __launch_bounds__(256) k_fft_y(cplx<T>* __restrict__ H, GeomDev 
     }
 }
 
-// The IFFT along y for ny = 256 (FP32): sixteen columns of the half spectrum
-// per workgroup, thread (s, t) = (tid % 16, tid / 16) holding rows
-// t + 16 m of column c0 + s (each row's 16 columns are one 128-byte
-// segment), transformed by fft256_reg and written back in place.
-__global__ void __launch_bounds__(256) k_fft_y256(cplx<float>* __restrict__ H, GeomDev g,
-                                                  const cplx<float>* __restrict__ w256, int gx, int gy) {
-    __shared__ cplx<float> sm[16 * 16 * 17];
-    const int W = g.nx / 2 + 1, WP = h_stride(g.nx);
-    int tx, ty;
-    if (!xcd_tile(gx, gy, tx, ty)) return;
-    const int s = threadIdx.x & 15, t = threadIdx.x >> 4;
-    const int col = tx * 16 + s;
-    const bool on = col < W;
-    cplx<float>* base = H + (size_t)ty * 256 * WP + (on ? col : 0);
-    cplx<float> v[16];
-#pragma unroll
-    for (int m = 0; m < 16; m++) v[m] = on ? base[(size_t)(t + 16 * m) * WP] : cplx<float>{0.f, 0.f};
-    fft256_reg(v, t, sm + s * 272, w256, +1);
-    if (on) {
-#pragma unroll
-        for (int k = 0; k < 16; k++) base[(size_t)(t + 16 * k) * WP] = v[k];
-    }
-}
-
 // Complex-to-real inverse FFT along x of two rows at once (z = x1 + i x2),
 // then (-1)^(k+j+(nz+ny)/2)/N, sphere mask, cupping compensation and final
 // scale (UTR.cpp:1235-1279).
