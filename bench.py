@@ -324,8 +324,10 @@ def main():
             B.run()
             gms.append(B.graph_ms)
         forms = [B.form(j) for j in range(len(jobs))]
+        tunings = [B.tuning(j) for j in range(len(jobs))]
         B.close()
         seq = 0.0
+        seq_jobs = []
         for jp, jr, ji in jobs:
             r = U.Reconstructor(jp, jr, U.UTR_FP32)
             dimg = torch.from_numpy(ji).cuda()
@@ -339,15 +341,16 @@ def main():
                 torch.cuda.synchronize()
                 t = time.perf_counter() - t0
             seq += t
+            seq_jobs.append(round(t * 1e3, 3))
             r.close()
             del dimg, dvol
         g_ms = float(np.mean(gms))
         extra["batched"] = {"jobs": len(jobs), "edges": [jp["nx"] for jp, _, _ in jobs],
                             "rows": [len(jr["angY"]) for _, jr, _ in jobs],
                             "graph_ms": round(g_ms, 3), "value": bwork / (g_ms * 1e-3),
-                            "sequential_ms": round(seq * 1e3, 3),
+                            "sequential_ms": round(seq * 1e3, 3), "sequential_job_ms": seq_jobs,
                             "speedup_vs_sequential": round(seq * 1e3 / g_ms, 2),
-                            "forms": forms}
+                            "forms": forms, "tuning": tunings}
         log("batched: graph %.2f ms, sequential %.2f ms" % (g_ms, seq * 1e3))
 
     ms_case = None

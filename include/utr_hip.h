@@ -305,8 +305,13 @@ typedef struct utr_batch utr_batch;
 /* Creates the contexts and captures the graph.  Device pointers in jobs must
  * stay valid while the batch exists; host buffers are read / written at each
  * utr_batch_run.  autotune != 0: for every distinct job shape (nx, ny, nz,
- * nrows) both gridding forms are timed once on that job's images and the
- * faster one is captured (the choice is cached for the process). */
+ * nrows) the launch configuration is tuned once on that job's images, each
+ * candidate timed as a whole reconstruction (best of two runs), by
+ * coordinate descent: y planes per plane-gridding workgroup (1, 2, 4),
+ * resident workgroups per CU of the persistent row FFT (4, 8) and of the
+ * column FFT (2, 4), then the gather form against the scatter; the choice is
+ * cached for the process and captured.  Without autotune each job uses the
+ * defaults (planes by size, row FFT 8 per CU, column FFT as LDS allows). */
 int utr_batch_create(const utr_job* jobs, int njobs, int autotune, utr_batch** out);
 
 /* Runs all jobs: host images in, one graph launch, host volumes out.
@@ -315,6 +320,11 @@ int utr_batch_run(utr_batch* b, double* graph_ms);
 
 /* Form chosen for job j (UTR_FP32 or UTR_FP32_GATHER). */
 int utr_batch_job_form(const utr_batch* b, int j);
+
+/* The launch configuration captured for job j: form, planes per gridding
+ * workgroup and the FFT workgroups per CU (0: the default).  No counterpart
+ * in the reference (its jobs are separate runs of UTR.exe). */
+int utr_batch_job_tuning(const utr_batch* b, int j, int* form, int* planes, int* rows_wpc, int* cols_wpc);
 
 int utr_batch_destroy(utr_batch* b);
 

@@ -33,6 +33,10 @@ def test_c4_jobs_against_fp64():
         out = [o.copy() for o in B.run()]
         forms = [B.form(j) for j in range(len(jobs))]
         print("graph %.2f ms, forms %s" % (B.graph_ms, forms), flush=True)
+        for j in range(len(jobs)):
+            t = B.tuning(j)
+            assert t["form"] == forms[j] and t["planes"] in (0, 1, 2, 4)
+            print("  job %2d tuning %s" % (j, t), flush=True)
     threads = min(16, os.cpu_count() or 1)
     for k, ((p, rows, imgs), o) in enumerate(zip(jobs, out)):
         if p["nx"] <= 128:

@@ -37,7 +37,7 @@ EXPORTS = [
     "utr_set_image_shifts", "utr_set_astigmatism",
     "utr_reset", "utr_dims", "utr_get_timings", "utr_destroy", "utr_last_error",
     "utr_abi_version", "utr_angle_coeffs", "utr_get_spectrum", "utr_get_fourier_volume",
-    "utr_batch_create", "utr_batch_run", "utr_batch_job_form", "utr_batch_destroy",
+    "utr_batch_create", "utr_batch_run", "utr_batch_job_form", "utr_batch_job_tuning", "utr_batch_destroy",
     "dht_create", "dht_add_images", "dht_add_images_device", "dht_finalize", "dht_finalize_device",
     "dht_reset", "dht_dims", "dht_get_timings", "dht_destroy", "dht_set_image_correction",
     "dht_create_series", "dht_get_iterations", "dht_get_waves", "utr_host_alloc", "utr_host_free",
@@ -179,6 +179,8 @@ def load():
                                    ctypes.POINTER(vp)]
     L.utr_batch_run.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
     L.utr_batch_job_form.argtypes = [vp, ctypes.c_int]
+    ip = ctypes.POINTER(ctypes.c_int)
+    L.utr_batch_job_tuning.argtypes = [vp, ctypes.c_int, ip, ip, ip, ip]
     L.utr_batch_destroy.argtypes = [vp]
     L.dht_create.argtypes = [ctypes.POINTER(DhtParams), ctypes.c_int, dp, dp, dp, vp,
                              ctypes.POINTER(vp)]
@@ -428,6 +430,13 @@ class Batch:
 
     def form(self, j):
         return self._L.utr_batch_job_form(self._b, j)
+
+    def tuning(self, j):
+        """Launch configuration captured for job j: form, planes per gridding
+        workgroup, row and column FFT workgroups per CU (0: default)."""
+        v = [ctypes.c_int() for _ in range(4)]
+        _check(self._L.utr_batch_job_tuning(self._b, j, *[ctypes.byref(x) for x in v]))
+        return dict(zip(("form", "planes", "rows_wpc", "cols_wpc"), (x.value for x in v)))
 
     def close(self):
         if getattr(self, "_b", None):

@@ -290,7 +290,9 @@ struct utr_ctx {
     DevBuf<float> S32, vol32;
     DevBuf<unsigned> rowmax;  // per-row max |K| as float bits (FP32 forms)
     DevBuf<int> scales;       // fixed-point scales of the plane gridding
-    int grid_planes = 0;      // y planes per workgroup of the plane gridding (0: default)
+    int grid_planes = 0;      // y planes per workgroup of the plane gridding (0: by size)
+    int rows_wpc = 0;         // resident workgroups per CU of the persistent row FFT (0: default)
+    int cols_wpc = 0;         // same for the column FFT (0: as many as LDS allows)
     DevBuf<int> tile_order;   // plane-gridding tiles by decreasing expected work
     DevBuf<double> rowc;      // per row: m0 m1 m2 m6 m8 (padded to 6) for the plane gridding
     // FP64 form
@@ -336,7 +338,7 @@ int launch_2d(utr_ctx* c, const float* d_img, int row0, int count, cplx<T>* d_sp
     {                                                                                           \
         constexpr int NB = utr::lds_batch(N, sizeof(cplx<T>), utr::LDS_ROWS);                      \
         const int nbr = (g.ny + NB - 1) / NB, nbt = nbr * count;                                \
-        const int grid = std::min(nbt, c->ncu * kRowsWgPerCu);                                  \
+        const int grid = std::min(nbt, c->ncu * (c->rows_wpc > 0 ? c->rows_wpc : kRowsWgPerCu)); \
         utr::k_rows<T, N><<<grid, 256, 0, c->stream>>>(d_img, d_spec, g.ny, g, twx, nbr, nbt);  \
         ok = true;                                                                              \
     }
@@ -350,7 +352,8 @@ int launch_2d(utr_ctx* c, const float* d_img, int row0, int count, cplx<T>* d_sp
         const int ncol = g.nx / 2 + 1; /* the rest by Hermitian symmetry */                     \
         const int gx = (ncol + NB - 1) / NB;                                                    \
         const int lds = 2 * utr::lds_len(NB * N) * (int)sizeof(cplx<T>);                          \
-        const int wpc = std::max(1, std::min(8, 163840 / lds)); /* resident per CU */           \
+        int wpc = std::max(1, std::min(8, 163840 / lds)); /* resident per CU */                 \
+        if (c->cols_wpc > 0) wpc = std::min(wpc, c->cols_wpc);                                  \
         const unsigned grid = std::min(utr::xcd_grid(gx * count), persistent_cap(c, wpc));      \
         utr::k_cols<T, N><<<grid, 256, 0, c->stream>>>(d_spec, g.nx, ncol, g, c->rows.p, row0,  \
                                                        twy, c->rowmax.p, gx, count);            \
@@ -596,7 +599,9 @@ int launch_grid32(utr_ctx* c, float* d_S) {
             const char* e = std::getenv("UTR_GRID_PLANES");
             pg = e ? std::atoi(e) : 0;
         }
-        if (pg <= 0) pg = 4;
+        // default: groups of up to 4 planes while the launch keeps at least
+        // four workgroups per resident slot (4 per CU)
+        if (pg <= 0) pg = (int)std::max(1L, std::min(4L, ntiles * g.ny / (16L * c->ncu)));
         const long ngroups = (g.ny + pg - 1) / pg;
         const long nwg = 8L * ((ngroups + 7) / 8) * ntiles;
         if (nwg <= 0 || nwg > 0x7ffffff0L) return set_err(UTR_E_INVALID, "volume too large");
@@ -1093,8 +1098,14 @@ int enqueue_job(utr_ctx* c, const float* d_img, float* d_vol) {
     return launch_3d<float, float>(c, c->V32.p, c->H32.p, d_vol, c->twx32.p, c->twy32.p, c->twz32.p);
 }
 
+// The batched form's tuning of one job shape: gridding form, planes per
+// plane-gridding workgroup, resident workgroups per CU of the persistent
+// row and column FFTs.
+struct JobTune {
+    int form = UTR_FP32, planes = 0, rows_wpc = 0, cols_wpc = 0;
+};
 std::mutex g_tune_mu;
-std::map<std::tuple<int, int, int, int>, int> g_tune;  // shape -> chosen form
+std::map<std::tuple<int, int, int, int>, JobTune> g_tune;  // (nx, ny, nz, rows) -> tuning
 
 int time_job(utr_ctx* c, const float* d_img, float* d_vol, float* ms) {
     hipEvent_t a = c->ev[4], b = c->ev[5];
@@ -1183,23 +1194,69 @@ int utr_batch_create(const utr_job* jobs, int njobs, int autotune, utr_batch** o
             return fail(set_err(UTR_E_DEVICE, "event creation failed"));
         if (autotune) {
             const auto key = std::make_tuple(g.nx, g.ny, g.nz, J.nrows);
-            int chosen = -1;
+            bool have = false;
+            JobTune T;
             {
                 std::lock_guard<std::mutex> lk(g_tune_mu);
                 auto it = g_tune.find(key);
-                if (it != g_tune.end()) chosen = it->second;
+                if (it != g_tune.end()) {
+                    T = it->second;
+                    have = true;
+                }
             }
-            if (chosen < 0) {
-                float t_fx = 0, t_ga = 0;
-                c->precision = UTR_FP32;
-                if ((rc = time_job(c, b->d_img[j], b->d_vol[j], &t_fx))) return fail(rc);
-                c->precision = UTR_FP32_GATHER;
-                if ((rc = time_job(c, b->d_img[j], b->d_vol[j], &t_ga))) return fail(rc);
-                chosen = t_ga < t_fx ? UTR_FP32_GATHER : UTR_FP32;
+            if (!have) {
+                // coordinate descent from the defaults, each candidate timed
+                // on the job itself (best of two runs): planes per gridding
+                // workgroup, the two FFT launch shapes, then the gather form
+                auto apply = [&](const JobTune& t) {
+                    c->precision = t.form;
+                    c->grid_planes = t.planes;
+                    c->rows_wpc = t.rows_wpc;
+                    c->cols_wpc = t.cols_wpc;
+                };
+                float best = 0;
+                apply(T);
+                if ((rc = time_job(c, b->d_img[j], b->d_vol[j], &best))) return fail(rc);
+                auto trial = [&](JobTune t) -> int {
+                    apply(t);
+                    float ms = 0;
+                    const int r2 = time_job(c, b->d_img[j], b->d_vol[j], &ms);
+                    if (r2) return r2;
+                    if (ms < best) {
+                        best = ms;
+                        T = t;
+                    }
+                    apply(T);
+                    return UTR_OK;
+                };
+                if (c->plan.all_y_exact)
+                    for (int pl : {1, 2, 4}) {
+                        JobTune t = T;
+                        t.planes = pl;
+                        if ((rc = trial(t))) return fail(rc);
+                    }
+                for (int rw : {4, 8}) {
+                    JobTune t = T;
+                    t.rows_wpc = rw;
+                    if ((rc = trial(t))) return fail(rc);
+                }
+                for (int cw : {2, 4}) {
+                    JobTune t = T;
+                    t.cols_wpc = cw;
+                    if ((rc = trial(t))) return fail(rc);
+                }
+                {
+                    JobTune t = T;
+                    t.form = UTR_FP32_GATHER;
+                    if ((rc = trial(t))) return fail(rc);
+                }
                 std::lock_guard<std::mutex> lk(g_tune_mu);
-                g_tune[key] = chosen;
+                g_tune[key] = T;
             }
-            c->precision = chosen;
+            c->precision = T.form;
+            c->grid_planes = T.planes;
+            c->rows_wpc = T.rows_wpc;
+            c->cols_wpc = T.cols_wpc;
         }
     }
     // capture: fork from s0, one branch per job on its own stream, join on s0
@@ -1265,6 +1322,16 @@ int utr_batch_run(utr_batch* b, double* graph_ms) {
 int utr_batch_job_form(const utr_batch* b, int j) {
     if (!b || j < 0 || j >= (int)b->ctx.size()) return set_err(UTR_E_INVALID, "no such job");
     return b->ctx[j]->precision;
+}
+
+int utr_batch_job_tuning(const utr_batch* b, int j, int* form, int* planes, int* rows_wpc, int* cols_wpc) {
+    if (!b || j < 0 || j >= (int)b->ctx.size()) return set_err(UTR_E_INVALID, "no such job");
+    const utr_ctx* c = b->ctx[j];
+    if (form) *form = c->precision;
+    if (planes) *planes = c->grid_planes;
+    if (rows_wpc) *rows_wpc = c->rows_wpc;
+    if (cols_wpc) *cols_wpc = c->cols_wpc;
+    return UTR_OK;
 }
 
 }  // extern "C"
