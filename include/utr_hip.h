@@ -318,6 +318,13 @@ int utr_batch_create(const utr_job* jobs, int njobs, int autotune, utr_batch** o
  * graph_ms (may be NULL): device time of the graph launch. */
 int utr_batch_run(utr_batch* b, double* graph_ms);
 
+/* Replays the graph `repeats` times back to back with no host work between
+ * the launches (every job's images and volume on the device), then waits.
+ * ms (may be NULL): device time of all repeats.  No counterpart in the
+ * reference: the launch-free form of a job stream for callers that keep
+ * their inputs resident. */
+int utr_batch_launch(utr_batch* b, int repeats, double* ms);
+
 /* Form chosen for job j (UTR_FP32 or UTR_FP32_GATHER). */
 int utr_batch_job_form(const utr_batch* b, int j);
 

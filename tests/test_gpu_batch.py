@@ -63,3 +63,46 @@ def test_batch_autotune():
     assert all(f in (U.UTR_FP32, U.UTR_FP32_GATHER) for f in forms)
     for (p, rows, imgs), o in zip(jobs, out):
         assert rel(o, oracle.reconstruct(p, rows, imgs)) < 1e-5
+
+
+def test_device_batch_launch_replays_equal_the_stream_form():
+    """Device-resident jobs: utr_batch_launch replays the captured graph back
+    to back; every replay (with eager reconstructions in between) gives the
+    per-call form's volume bit for bit."""
+    import torch
+    jobs = [job(64, 30, 3, 1), job(48, 16, 2, 2)]
+    d_imgs = [torch.from_numpy(im).cuda() for _, _, im in jobs]
+    eager = []
+    for (p, rows, _), d in zip(jobs, d_imgs):
+        with U.Reconstructor(p, rows, U.UTR_FP32) as r:
+            v = torch.empty(r.shape, dtype=torch.float32, device="cuda")
+            r.add_images_device(d.data_ptr(), len(rows["angY"]))
+            r.finalize_device(v.data_ptr())
+            torch.cuda.synchronize()
+            eager.append(v.cpu().numpy())
+    B = U.Batch([(p, rows, d) for (p, rows, _), d in zip(jobs, d_imgs)])
+    try:
+        for rep in range(3):
+            for o in B.outputs:
+                o.fill_(float("nan"))
+            ms = B.launch(2 if rep else 1)
+            torch.cuda.synchronize()
+            assert ms > 0
+            for o, e in zip(B.outputs, eager):
+                assert np.array_equal(o.cpu().numpy().view(np.uint32), e.view(np.uint32))
+            # eager work between the replays
+            p, rows, _ = jobs[1]
+            with U.Reconstructor(p, rows, U.UTR_FP32) as r:
+                v = torch.empty(r.shape, dtype=torch.float32, device="cuda")
+                r.add_images_device(d_imgs[1].data_ptr(), len(rows["angY"]))
+                r.finalize_device(v.data_ptr())
+                torch.cuda.synchronize()
+        # host-memory jobs cannot be replayed without their copies
+        Bh = U.Batch([jobs[0]])
+        try:
+            with pytest.raises(U.UTRError):
+                Bh.launch(1)
+        finally:
+            Bh.close()
+    finally:
+        B.close()

@@ -37,7 +37,8 @@ EXPORTS = [
     "utr_set_image_shifts", "utr_set_astigmatism",
     "utr_reset", "utr_dims", "utr_get_timings", "utr_destroy", "utr_last_error",
     "utr_abi_version", "utr_angle_coeffs", "utr_get_spectrum", "utr_get_fourier_volume",
-    "utr_batch_create", "utr_batch_run", "utr_batch_job_form", "utr_batch_job_tuning", "utr_batch_destroy",
+    "utr_batch_create", "utr_batch_run", "utr_batch_launch", "utr_batch_job_form", "utr_batch_job_tuning",
+    "utr_batch_destroy",
     "dht_create", "dht_add_images", "dht_add_images_device", "dht_finalize", "dht_finalize_device",
     "dht_reset", "dht_dims", "dht_get_timings", "dht_destroy", "dht_set_image_correction",
     "dht_create_series", "dht_get_iterations", "dht_get_waves", "utr_host_alloc", "utr_host_free",
@@ -178,6 +179,7 @@ def load():
     L.utr_batch_create.argtypes = [ctypes.POINTER(UtrJob), ctypes.c_int, ctypes.c_int,
                                    ctypes.POINTER(vp)]
     L.utr_batch_run.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
+    L.utr_batch_launch.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     L.utr_batch_job_form.argtypes = [vp, ctypes.c_int]
     ip = ctypes.POINTER(ctypes.c_int)
     L.utr_batch_job_tuning.argtypes = [vp, ctypes.c_int, ip, ip, ip, ip]
@@ -388,7 +390,9 @@ def reconstruct(params, rows, images, precision=UTR_FP32, f64_out=False):
 class Batch:
     """Independent reconstructions of uneven size run as one HIP graph
     (utr_batch_* of include/utr_hip.h).  jobs: list of (params, rows, images)
-    with host images; volumes are returned by run()."""
+    with host images (numpy; volumes returned by run() as numpy arrays) or
+    device images (torch CUDA float32 tensors; volumes allocated as torch
+    CUDA tensors, and launch() replays the graph with no host copies)."""
 
     def __init__(self, jobs, autotune=False, precision=UTR_FP32):
         L = load()
@@ -403,18 +407,29 @@ class Batch:
             cols = [np.ascontiguousarray(rows[c], dtype=np.float64) for c in ("angZ", "angY", "defocus")]
             w = rows.get("weights")
             w = None if w is None else np.ascontiguousarray(w, dtype=np.float32)
-            images = np.ascontiguousarray(images, dtype=np.float32)
             nz = int((max(params["zlo"], params["zhi"]) - min(params["zlo"], params["zhi"])) /
                      params["xst"] + 0.5)
-            out = np.empty((nz, params["ny"], params["nx"]), dtype=np.float32)
+            on_dev = hasattr(images, "is_cuda") and images.is_cuda
+            if on_dev:
+                import torch
+                if images.dtype != torch.float32 or not images.is_contiguous():
+                    raise ValueError("device images must be a contiguous float32 CUDA tensor")
+                if images.numel() != len(cols[0]) * params["ny"] * params["nx"]:
+                    raise ValueError("device images do not match the rows and image size")
+                out = torch.empty((nz, params["ny"], params["nx"]), dtype=torch.float32, device=images.device)
+                J.images = images.data_ptr()
+                J.vol = out.data_ptr()
+            else:
+                images = np.ascontiguousarray(images, dtype=np.float32)
+                out = np.empty((nz, params["ny"], params["nx"]), dtype=np.float32)
+                J.images = images.ctypes.data
+                J.vol = out.ctypes.data
             self._keep += cols + [w, images, out]
             J.nrows = len(cols[0])
             J.angZ_deg, J.angY_deg, J.defocus = (c.ctypes.data for c in cols)
             J.weights = w.ctypes.data if w is not None else None
-            J.images = images.ctypes.data
-            J.vol = out.ctypes.data
-            J.images_on_device = 0
-            J.vol_on_device = 0
+            J.images_on_device = 1 if on_dev else 0
+            J.vol_on_device = 1 if on_dev else 0
             J.precision = precision
             self.outputs.append(out)
         self._jobs = arr
@@ -427,6 +442,13 @@ class Batch:
         _check(self._L.utr_batch_run(self._b, ctypes.byref(ms)))
         self.graph_ms = ms.value
         return self.outputs
+
+    def launch(self, repeats=1):
+        """Replays the graph `repeats` times back to back (device jobs only);
+        returns the device milliseconds of all repeats."""
+        ms = ctypes.c_double()
+        _check(self._L.utr_batch_launch(self._b, int(repeats), ctypes.byref(ms)))
+        return ms.value
 
     def form(self, j):
         return self._L.utr_batch_job_form(self._b, j)

@@ -1319,6 +1319,24 @@ int utr_batch_run(utr_batch* b, double* graph_ms) {
     return UTR_OK;
 }
 
+int utr_batch_launch(utr_batch* b, int repeats, double* ms) {
+    if (!b) return set_err(UTR_E_INVALID, "null batch");
+    if (repeats < 1) return set_err(UTR_E_INVALID, "repeats must be positive");
+    for (const utr_job& J : b->jobs)
+        if (!J.images_on_device || !J.vol_on_device)
+            return set_err(UTR_E_INVALID, "utr_batch_launch: every job must have device images and volumes");
+    UTR_HIP_TRY(hipEventRecord(b->t0, b->s0));
+    for (int r = 0; r < repeats; r++) UTR_HIP_TRY(hipGraphLaunch(b->exec, b->s0));
+    UTR_HIP_TRY(hipEventRecord(b->t1, b->s0));
+    UTR_HIP_TRY(hipEventSynchronize(b->t1));
+    if (ms) {
+        float t = 0;
+        UTR_HIP_TRY(hipEventElapsedTime(&t, b->t0, b->t1));
+        *ms = t;
+    }
+    return UTR_OK;
+}
+
 int utr_batch_job_form(const utr_batch* b, int j) {
     if (!b || j < 0 || j >= (int)b->ctx.size()) return set_err(UTR_E_INVALID, "no such job");
     return b->ctx[j]->precision;
