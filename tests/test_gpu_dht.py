@@ -122,7 +122,7 @@ def dht_forms(p, rows, imgs):
 
 @pytest.mark.parametrize("name", ["m3_escc", "m3_flat", "m4", "shift", "rect", "axis", "deepz"])
 def test_plane_and_brick_splat_agree(name):
-    """Orientations without Z rotation splat in y-plane tiles; the sums are
+    """Orientations without Z rotation splat in columns through y; the sums are
     those of the brick splat bit for bit, and both meet the oracle."""
     if name == "m3_escc":
         p, rows, imgs = case(method=3, escc=1, dzextra=60.0, epsilon=1.0, seed=31, general=False)

@@ -17,7 +17,7 @@
 //                              workgroup per 16x8x8 brick, exact 64-bit
 //                              fixed-point sums in LDS, coordinates evaluated
 //                              in FP64 in the reference's operation order
-//   k_dht_splat_yp             the same sums in y-plane tiles when no
+//   k_dht_splat_col            the same sums in columns through y when no
 //                              orientation rotates about Z
 //   k_dht_finish               scale of DHT.cpp:1325-1340
 // Orientations are processed in batches whose plane stacks fit a memory
@@ -236,7 +236,10 @@ __global__ void __launch_bounds__(256) k_dht_prop_rows(const cplx<float>* __rest
 // next tile's columns are loaded into registers while the current one is
 // transformed.  gridDim.x is a multiple of 8.
 // pmax: max |value| of the batch as float bits.
-template <int N>
+//
+// COL: the planes are stored column-major ([plane][x][y], y fastest) for
+// the column splat; otherwise row-major ([plane][y][x]).
+template <int N, bool COL>
 __global__ void __launch_bounds__(256) k_dht_prop_cols(const cplx<float>* __restrict__ T,
                                                        float* __restrict__ P, DhtGeom g,
                                                        const DhtRow* __restrict__ rows, int a0,
@@ -309,15 +312,17 @@ __global__ void __launch_bounds__(256) k_dht_prop_cols(const cplx<float>* __rest
             for (int q = 0; q < PER; q++) {
                 const int t = tid + 256 * q;
                 if (t < cols * N) {
-                    const int y = t / cols, c = t - y * cols;
+                    // column-major: consecutive threads take consecutive y
+                    const int y = COL ? t % N : t / cols, c = COL ? t / N : t - y * cols;
                     const cplx<float> z = a[lx(c * N + y)];
                     const float re = z.x * inv, im = z.y * inv;
                     const float v = (pack ? re : -2.f * re - (re * re + im * im)) * w;
-                    P[plane + (size_t)y * nx + x0 + c] = v;
+                    const size_t o = COL ? (size_t)(x0 + c) * N + y : (size_t)y * nx + x0 + c;
+                    P[plane + o] = v;
                     vmax = fmaxf(vmax, fabsf(v));
                     if (second) {
                         const float v2 = im * w;
-                        P[plane + (size_t)N * nx + (size_t)y * nx + x0 + c] = v2;
+                        P[plane + (size_t)N * nx + o] = v2;
                         vmax = fmaxf(vmax, fabsf(v2));
                     }
                 }
@@ -488,47 +493,45 @@ __global__ void k_dht_finish(const double* __restrict__ acc3, float* __restrict_
 
 namespace utr {
 
-// Real-space splat in y-plane tiles, for batches whose orientations have no
-// rotation about Z (sinZ = 0, cosZ = 1).  Then a sample's x and z grid
-// coordinates do not depend on its source line j, and its y coordinate
-// depends on j alone: source line j reaches planes jj(j) and jj(j) + 1 with
-// the weights dy0(j), dy1(j).  Each workgroup owns a DP_X x DP_Z (x, z) tile
-// of DP_G consecutive planes J0 .. J0 + DP_G - 1, takes from each orientation
-// the source lines j with jj(j) in J0 - 1 .. J0 + DP_G - 1, clips every
-// source plane n's i-line to the tile and deals the listed samples to its
-// threads in equal runs; a sample's x and z coordinates are formed once for
-// all its lines.  Coordinates, weights and fixed-point terms are those of
-// k_dht_splat expression by expression (the j-dependent terms it adds are
-// zeros here), so both kernels form the same sums.
-constexpr int DP_X = 32, DP_Z = 32, DP_SX = DP_X + 1, DP_PAD = DP_SX * DP_Z, DP_VOX = DP_X * DP_Z;
-constexpr int DP_G = 4;            // planes per workgroup
-constexpr int DP_NC = DP_G + 3;    // candidate source lines J0 - 2 .. J0 + DP_G
-constexpr int DP_NXMAX = 512;      // widest image (x table staged in LDS)
+// Splat of the batch's column-major plane stacks into columns of the volume
+// when no orientation rotates about Z (every angZ = 0, a tilt series about
+// y).  Then sinZ = 0 and cosZ = 1: a sample's x and z grid coordinates
+// depend on its (i, n) only, and its y coordinate on its line j only (line j
+// lands in cell jj(j), weights 1 - dy1(j) and dy1(j)).  Each workgroup owns
+// a DC_X x DC_Z (x, z) column of the volume through all y, one thread per
+// source line j (and j + 256):
+//   per orientation, the threads test the source samples (i, n) around the
+//   column (FP64, the reference's expressions of DHT.cpp:1119-1150) and list
+//   those whose cell (ii, nn) puts a corner in the column, with their x and
+//   z weights; then every thread splats its line j of each listed sample:
+//   the values P[n][i][0..ny) of one sample are contiguous, and the threads
+//   add to consecutive y entries of the column's sums.
+// The fixed-point terms are those of k_dht_splat expression by expression,
+// so both kernels form the same sums.
+constexpr int DC_X = 4, DC_Z = 4;
+constexpr int DC_R = 6;                       // candidate half-width (source samples)
+constexpr int DC_NC = (2 * DC_R + 1) * (2 * DC_R + 1);
+constexpr int DC_NYMAX = 384;                 // tallest image (column sums of 48 KB in LDS)
+static_assert(DC_NC <= 256, "one candidate per thread");
 
-__global__ void __launch_bounds__(256) k_dht_splat_yp(const float* __restrict__ P, DhtGeom g,
-                                                      const DhtRow* __restrict__ rows,
-                                                      const double* __restrict__ xs,
-                                                      const double* __restrict__ ys,
-                                                      const double* __restrict__ zs, int a0, int na,
-                                                      const unsigned* __restrict__ pmax,
-                                                      double* __restrict__ acc3, int ntiles) {
-    __shared__ unsigned long long acc[DP_G * DP_PAD];
-    __shared__ double s_X[2 * DP_NXMAX];  // the orientation's (x_i - xc) sinY, cosY
-    __shared__ double s_Z[2 * 256];       // (z_n - zc) sinY, cosY of the current 256 planes
-    __shared__ unsigned short s_n[256];   // listed source planes n (within the chunk)
-    __shared__ short s_i0[256];           // first i of each listed line
-    __shared__ int s_pre[257];            // sample counts, then their prefix
-    __shared__ int s_cnt[4], s_wsum[4];
-    __shared__ int s_lp[DP_NC];           // candidate line's lower plane jj - J0 (-2: none)
-    __shared__ double s_dy1[DP_NC];       // its dy1
+__global__ void __launch_bounds__(256) k_dht_splat_col(const float* __restrict__ P, DhtGeom g,
+                                                       const DhtRow* __restrict__ rows,
+                                                       const double* __restrict__ xs,
+                                                       const double* __restrict__ ys,
+                                                       const double* __restrict__ zs, int a0, int na,
+                                                       const unsigned* __restrict__ pmax,
+                                                       double* __restrict__ acc3, int ntx, int ntz) {
+    extern __shared__ unsigned long long acc[];  // [DC_Z][DC_X][ny]
+    __shared__ short s_i[DC_NC], s_n[DC_NC];
+    __shared__ signed char s_lx[DC_NC], s_lz[DC_NC];
+    __shared__ double s_dx1[DC_NC], s_dz1[DC_NC];
+    __shared__ int s_cnt[4];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int ngy = (g.ny + DP_G - 1) / DP_G;
-    int tile, Jg;
-    if (!xcd_tile(ntiles, ngy, tile, Jg)) return;
-    const int J0 = Jg * DP_G;
-    const int ntx = (g.nx + DP_X - 1) / DP_X;
-    const int x0 = (tile % ntx) * DP_X, z0 = (tile / ntx) * DP_Z;
-    for (int t = tid; t < DP_G * DP_PAD; t += 256) acc[t] = 0ull;
+    const int tile = (int)blockIdx.x;
+    if (tile >= ntx * ntz) return;
+    const int x0 = (tile % ntx) * DC_X, z0 = (tile / ntx) * DC_Z;
+    const int ny = g.ny;
+    for (int t = tid; t < DC_X * DC_Z * ny; t += 256) acc[t] = 0ull;
     // the scale of k_dht_splat
     int ek = 0, ea = 0;
     const float mk = __uint_as_float(*pmax);
@@ -536,189 +539,110 @@ __global__ void __launch_bounds__(256) k_dht_splat_yp(const float* __restrict__ 
     frexpf((float)na, &ea);
     const int sc = min(53 - ek - ea, 47 - ek);
     const double scale = ldexp(1.0, sc);
-    const size_t n2 = (size_t)g.nx * g.ny;
-    const int nsrc = g.nz;  // source planes n
+    const size_t n2 = (size_t)g.nx * ny;
+    const int nsrc = g.nz;
+    // grid centre offsets: dx1 = cxs + (i - cxs) cosY - (n - czs) sinY and
+    // dz1 = czs + (n - czs) cosY + (i - cxs) sinY (candidate box only)
+    const float cxs = (float)((g.xc - g.xlo) / g.xst), czs = (float)((g.zc - g.zlodz) / g.xst);
+    const float X = (float)x0 + 0.5f * DC_X - 0.5f, Z = (float)z0 + 0.5f * DC_Z - 0.5f;
     for (int ab = 0; ab < na; ab++) {
         const DhtRow& R = rows[a0 + ab];
-        const double* X = xs + (size_t)(a0 + ab) * 2 * g.nx;
-        const double* Y = ys + (size_t)(a0 + ab) * 2 * g.ny;
-        const double* Z = zs + (size_t)(a0 + ab) * 2 * g.nz;
-        __syncthreads();  // previous orientation's lists and tables are consumed
-        for (int t = tid; t < 2 * g.nx; t += 256) s_X[t] = X[t];
-        if (tid < DP_NC) {
-            // y of source line j as DHT.cpp:1143-1148 forms it (xx sinZ = 0).
-            // Rounding puts line j in cell j or j - 1, so the lines reaching
-            // planes J0 .. J0 + DP_G - 1 lie in J0 - 2 .. J0 + DP_G.
-            const int j = J0 - 2 + tid;
-            int lp = -2;
-            double dy1 = 0.0;
-            if (j >= 0 && j < g.ny) {
-                const double yyy = __dsub_rn(__dadd_rn(g.yc, Y[2 * j + 1]), __dmul_rn(0.0, R.sinZ));
-                dy1 = __ddiv_rn(__dsub_rn(yyy, g.ylo), g.xst);
-                const int jj = (int)dy1;
-                if (jj >= 0 && jj <= g.ny2 && jj >= J0 - 1 && jj < J0 + DP_G) {
-                    dy1 = __dsub_rn(dy1, (double)jj);
-                    lp = jj - J0;
-                }
-            }
-            s_lp[tid] = lp;
-            s_dy1[tid] = dy1;
-        }
-        __syncthreads();  // s_lp, s_dy1, s_X written
-        // the listed lines, compacted: h-th line j = J0 - 2 + hj[h]
-        int nvl = 0, hj[DP_NC];
-#pragma unroll
-        for (int h = 0; h < DP_NC; h++) {
-            hj[h] = 0;
-        }
-#pragma unroll
-        for (int h = 0; h < DP_NC; h++)
-            if (s_lp[h] > -2) hj[nvl++] = h;
-        if (nvl == 0) continue;  // uniform: no source line reaches the planes
-        const double ysin = Y[2 * (J0 - 2 + hj[0])];  // y_j sinZ: a zero of either sign, same sum
-        const float* Pab = P + (size_t)ab * g.nplanes * n2;
-        // clip the source planes' i-lines to the tile (one sample of margin),
-        // 256 planes at a time
-        const float bl[2] = {(float)x0 - 1.f, (float)z0 - 1.f};
-        const float bh[2] = {(float)(x0 + DP_X), (float)(z0 + DP_Z)};
-        for (int nb = 0; nb < nsrc; nb += 256) {
+        const double* Xt = xs + (size_t)(a0 + ab) * 2 * g.nx;
+        const double* Yt = ys + (size_t)(a0 + ab) * 2 * ny;
+        const double* Zt = zs + (size_t)(a0 + ab) * 2 * g.nz;
+        __syncthreads();  // the previous orientation's list is consumed (and acc zeroed)
+        // candidates (i, n) in a box about the column's centre
         bool pass = false;
-        int i0 = 0, i1 = -1;
-        if (nb + tid < nsrc) {
-            const int n = nb + tid;
-            s_Z[2 * tid] = Z[2 * n];
-            s_Z[2 * tid + 1] = Z[2 * n + 1];
-            float ilo = 0.f, ihi = (float)(g.nx - 1);
-            const int dd[2] = {0, 2};
-#pragma unroll
-            for (int q = 0; q < 2; q++) {
-                const int d = dd[q];
-                const float base = R.c0[d] + R.un[d] * n;
-                const float u = R.ui[d];
-                const float l = bl[q] - 0.05f - base, h = bh[q] + 0.05f - base;
-                if (fabsf(u) < 1e-6f) {
-                    if (l > 0.f || h < 0.f) ihi = ilo - 1.f;
-                } else {
-                    float t0 = l / u, t1 = h / u;
-                    if (u < 0.f) { const float tt = t0; t0 = t1; t1 = tt; }
-                    ilo = fmaxf(ilo, floorf(t0) - 1.f);
-                    ihi = fminf(ihi, ceilf(t1) + 1.f);
-                }
-            }
-            ilo = fmaxf(ilo, 0.f);
-            ihi = fminf(ihi, (float)(g.nx - 1));
-            if (ilo <= ihi) {
-                i0 = (int)ilo;
-                i1 = (int)ihi;
-            }
-            pass = i0 <= i1;
-        }
-        const unsigned long long mask = __ballot(pass);
-        const int pre = __popcll(mask & ((1ull << lane) - 1ull));
-        if (lane == 0) s_cnt[wave] = __popcll(mask);
-        __syncthreads();
-        int off = 0;
-        for (int w = 0; w < wave; w++) off += s_cnt[w];
-        const int npt = pass ? i1 - i0 + 1 : 0;
-        if (pass) {
-            s_n[off + pre] = (unsigned short)tid;
-            s_i0[off + pre] = (short)i0;
-        }
-        int incl = npt;
-        for (int o = 1; o < 64; o <<= 1) {
-            const int v = __shfl_up(incl, o);
-            if (lane >= o) incl += v;
-        }
-        if (lane == 63) s_wsum[wave] = incl;
-        __syncthreads();
-        int woff = 0;
-        for (int w = 0; w < wave; w++) woff += s_wsum[w];
-        if (pass) s_pre[off + pre] = woff + incl - npt;
-        const int nl = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
-        const int total = s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
-        if (tid == 0) s_pre[nl] = total;
-        __syncthreads();  // lists and s_Z written
-        const int p0 = (int)(((long long)tid * total) >> 8);
-        const int p1 = (int)(((long long)(tid + 1) * total) >> 8);
-        if (p0 < p1) {
-            int lo = 0, hi = nl - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (s_pre[mid] <= p0) lo = mid;
-                else hi = mid - 1;
-            }
-            int q = lo;
-            int nc = s_n[q], ibase = s_i0[q] - s_pre[q], pend = s_pre[q + 1];
-            for (int p = p0; p < p1; p++) {
-                if (p == pend) {
-                    q++;
-                    nc = s_n[q];
-                    ibase = s_i0[q] - s_pre[q];
-                    pend = s_pre[q + 1];
-                }
-                const int i = ibase + p, n = nb + nc;
-                // the source values of all listed lines first (one wait)
-                const size_t po = (size_t)(g.escc ? n : 0) * n2 + i;
-                float kv[DP_NC];
-#pragma unroll
-                for (int h = 0; h < DP_NC; h++)
-                    kv[h] = Pab[po + (size_t)(J0 - 2 + hj[h < nvl ? h : 0]) * g.nx];  // line j >= 0
+        int ci = 0, cn = 0, lxv = 0, lzv = 0;
+        double dx1 = 0.0, dz1 = 0.0;
+        if (tid < DC_NC) {
+            const float sY = (float)R.sinY, cY = (float)R.cosY;
+            const float ua = X - cxs, vb = Z - czs;
+            const float ic = cxs + ua * cY + vb * sY, nc = czs - ua * sY + vb * cY;
+            ci = (int)floorf(ic) - DC_R + tid % (2 * DC_R + 1);
+            cn = (int)floorf(nc) - DC_R + tid / (2 * DC_R + 1);
+            if (ci >= 0 && ci < g.nx && cn >= 0 && cn < nsrc) {
                 // DHT.cpp:1119-1150, operation by operation
-                const double zcos = s_Z[2 * nc + 1], zsin = s_Z[2 * nc];
-                const double zzz = __dadd_rn(__dadd_rn(g.zc, zcos), s_X[2 * i]);
-                double dz1 = __ddiv_rn(__dsub_rn(zzz, g.zlodz), g.xst);
+                const double zcos = Zt[2 * cn + 1], zsin = Zt[2 * cn];
+                const double zzz = __dadd_rn(__dadd_rn(g.zc, zcos), Xt[2 * ci]);
+                dz1 = __ddiv_rn(__dsub_rn(zzz, g.zlodz), g.xst);
                 const int nn = (int)dz1;
-                if (nn < 0 || nn > g.nz2) continue;
-                dz1 = __dsub_rn(dz1, (double)nn);
-                const double dz0 = __dsub_rn(1.0, dz1);
-                const double xx = __dadd_rn(-zsin, s_X[2 * i + 1]);
+                const double ysin = Yt[0];  // y_j sinZ: a zero of either sign, same sum
+                const double xx = __dadd_rn(-zsin, Xt[2 * ci + 1]);
                 const double xx_cosZ = __dmul_rn(xx, R.cosZ);
                 const double xxx = __dadd_rn(__dadd_rn(g.xc, ysin), xx_cosZ);
-                double dx1 = __ddiv_rn(__dsub_rn(xxx, g.xlo), g.xst);
+                dx1 = __ddiv_rn(__dsub_rn(xxx, g.xlo), g.xst);
                 const int ii = (int)dx1;
-                if (ii < 0 || ii > g.nx2) continue;
-                dx1 = __dsub_rn(dx1, (double)ii);
-                const double dx0 = __dsub_rn(1.0, dx1);
-                const int lx = ii - x0, lz = nn - z0;
-                if (lx < -1 || lx >= DP_X || lz < -1 || lz >= DP_Z) continue;
-                const double wx[2] = {dx0, dx1};
+                lxv = ii - x0;
+                lzv = nn - z0;
+                pass = nn >= 0 && nn <= g.nz2 && ii >= 0 && ii <= g.nx2 && lxv >= -1 && lxv < DC_X &&
+                       lzv >= -1 && lzv < DC_Z;
+                if (pass) {
+                    dz1 = __dsub_rn(dz1, (double)nn);
+                    dx1 = __dsub_rn(dx1, (double)ii);
+                }
+            }
+        }
+        const unsigned long long mask = __ballot(pass);
+        if (lane == 0) s_cnt[wave] = __popcll(mask);
+        __syncthreads();
+        int off = __popcll(mask & ((1ull << lane) - 1ull));
+        for (int w = 0; w < wave; w++) off += s_cnt[w];
+        const int nl = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+        if (pass) {
+            s_i[off] = (short)ci;
+            s_n[off] = (short)cn;
+            s_lx[off] = (signed char)lxv;
+            s_lz[off] = (signed char)lzv;
+            s_dx1[off] = dx1;
+            s_dz1[off] = dz1;
+        }
+        __syncthreads();  // the list is written
+        if (nl == 0) continue;  // uniform
+        const float* Pab = P + (size_t)ab * g.nplanes * n2;
+        for (int j = tid; j < ny; j += 256) {
+            // y of source line j as DHT.cpp:1143-1148 forms it (xx sinZ = 0)
+            const double yyy = __dsub_rn(__dadd_rn(g.yc, Yt[2 * j + 1]), __dmul_rn(0.0, R.sinZ));
+            double dy1 = __ddiv_rn(__dsub_rn(yyy, g.ylo), g.xst);
+            const int jj = (int)dy1;
+            if (jj < 0 || jj > g.ny2) continue;
+            dy1 = __dsub_rn(dy1, (double)jj);
+            const double wy[2] = {__dsub_rn(1.0, dy1), dy1};
+            // the first listed sample's value loaded ahead
+            int e = 0;
+            float kv = Pab[((size_t)(g.escc ? s_n[0] : 0) * g.nx + s_i[0]) * ny + j];
+            for (; e < nl; e++) {
+                const float kraw = kv;
+                if (e + 1 < nl) kv = Pab[((size_t)(g.escc ? s_n[e + 1] : 0) * g.nx + s_i[e + 1]) * ny + j];
+                const int lx = s_lx[e], lz = s_lz[e];
+                const double ex1 = s_dx1[e], ez1 = s_dz1[e];
+                const double wx[2] = {__dsub_rn(1.0, ex1), ex1};
+                const double k = (double)kraw;
+                const double wz[2] = {__dsub_rn(1.0, ez1) * k, ez1 * k};
 #pragma unroll
-                for (int h = 0; h < DP_NC; h++) {
-                    if (h >= nvl) break;
-                    const int lp = s_lp[hj[h]];
-                    const double dy1 = s_dy1[hj[h]];
-                    const double k = (double)kv[h];
-                    const double wz[2] = {dz0 * k, dz1 * k};
+                for (int cy = 0; cy < 2; cy++) {
+                    unsigned long long* ap = acc + jj + cy;
 #pragma unroll
-                    for (int cy = 0; cy < 2; cy++) {
-                        const int pl = lp + cy;  // plane J0 + pl takes weight dy0 (cy 0) or dy1
-                        if (pl < 0 || pl >= DP_G) continue;
-                        const double wyv = cy ? dy1 : __dsub_rn(1.0, dy1);
-                        unsigned long long* ap = acc + pl * DP_PAD;
-#pragma unroll
-                        for (int c = 0; c < 4; c++) {
-                            const int cx = c & 1, cz = c >> 1;
-                            const int vx = lx + cx, vz = lz + cz;
-                            if ((unsigned)vx >= (unsigned)DP_X || (unsigned)vz >= (unsigned)DP_Z) continue;
-                            const double w = wx[cx] * wyv * wz[cz];
-                            atomicAdd(&ap[vx + DP_SX * vz], (unsigned long long)dround_i64(w * scale));
-                        }
+                    for (int c = 0; c < 4; c++) {
+                        const int cx = c & 1, cz = c >> 1;
+                        const int vx = lx + cx, vz = lz + cz;
+                        if ((unsigned)vx >= (unsigned)DC_X || (unsigned)vz >= (unsigned)DC_Z) continue;
+                        const double w = wx[cx] * wy[cy] * wz[cz];
+                        atomicAdd(&ap[(size_t)(vz * DC_X + vx) * ny], (unsigned long long)dround_i64(w * scale));
                     }
                 }
             }
         }
-        __syncthreads();  // the lists and s_Z are rewritten for the next planes
-        }
     }
     __syncthreads();
     const double inv = ldexp(1.0, -sc);
-    for (int v = tid; v < DP_G * DP_VOX; v += 256) {
-        const int pl = v / DP_VOX, r = v - pl * DP_VOX;
-        const int lx = r % DP_X, lz = r / DP_X;
-        const int vx = x0 + lx, vz = z0 + lz, J = J0 + pl;
-        if (vx >= g.nx || vz >= g.nz || J >= g.ny) continue;
-        const size_t idx = ((size_t)vz * g.ny + J) * g.nx + vx;
-        acc3[idx] += (double)(long long)acc[pl * DP_PAD + lx + DP_SX * lz] * inv;
+    // add the column's sums to the FP64 volume: lane -> (x, y) with x fastest
+    for (int t = tid; t < DC_X * DC_Z * ny; t += 256) {
+        const int vx = t % DC_X, r = t / DC_X, y = r % ny, vz = r / ny;
+        const int gx = x0 + vx, gz = z0 + vz;
+        if (gx >= g.nx || gz >= g.nz) continue;
+        const size_t idx = ((size_t)gz * ny + y) * g.nx + gx;
+        acc3[idx] += (double)(long long)acc[(size_t)(vz * DC_X + vx) * ny + y] * inv;
     }
 }
 
@@ -1002,11 +926,12 @@ int dht_run(dht_ctx* c, float* d_vol) {
     utr::k_dht_fill<<<(unsigned)((N + 255) / 256), 256, 0, c->stream>>>(c->acc3.p, N);
     const long nbricks = (long)((g.nx + utr::DHT_BX - 1) / utr::DHT_BX) *
                          ((g.ny + utr::DHT_BY - 1) / utr::DHT_BY) * ((g.nz + utr::DHT_BZ - 1) / utr::DHT_BZ);
-    // y-plane tiles when no orientation rotates about Z (UTR_GRID_FORM=brick
-    // selects the bricks; both form the same sums)
+    // columns through y, from column-major plane stacks, when no orientation
+    // rotates about Z (UTR_GRID_FORM=brick selects the bricks; both form the
+    // same sums)
     const char* form = std::getenv("UTR_GRID_FORM");
-    const bool plane = c->all_z0 && !(form && std::strcmp(form, "brick") == 0);
-    const long ntiles = (long)((g.nx + utr::DP_X - 1) / utr::DP_X) * ((g.nz + utr::DP_Z - 1) / utr::DP_Z);
+    const bool colsplat = c->all_z0 && !(form && std::strcmp(form, "brick") == 0) && g.ny <= utr::DC_NYMAX;
+    const int ctx_ = (g.nx + utr::DC_X - 1) / utr::DC_X, ctz = (g.nz + utr::DC_Z - 1) / utr::DC_Z;
     for (int a0 = 0; a0 < c->nang; a0 += c->batch) {
         const int na = std::min(c->batch, c->nang - a0);
         utr::k_dht_zero_u32<<<1, 64, 0, c->stream>>>(c->pmax.p);
@@ -1035,9 +960,14 @@ int dht_run(dht_ctx* c, float* d_vol) {
         const int gx = (g.nx + NB - 1) / NB, total = gx * utr::dht_slots(g);                     \
         const unsigned cap = (unsigned)(c->ncu * std::max(1, std::min(8, 163840 / lds))) & ~7u;  \
         const unsigned grid = std::min(utr::xcd_grid((unsigned)total), cap ? cap : 8u);          \
-        utr::k_dht_prop_cols<N><<<grid, 256, 0, c->stream>>>(                                    \
-            c->T.p, c->P.p + (size_t)ab * g.nplanes * g.nx * g.ny, g, c->rows.p, a0 + ab,        \
-            c->twy.p, c->pmax.p, gx, total);                                                     \
+        if (colsplat)                                                                            \
+            utr::k_dht_prop_cols<N, true><<<grid, 256, 0, c->stream>>>(                          \
+                c->T.p, c->P.p + (size_t)ab * g.nplanes * g.nx * g.ny, g, c->rows.p, a0 + ab,    \
+                c->twy.p, c->pmax.p, gx, total);                                                 \
+        else                                                                                     \
+            utr::k_dht_prop_cols<N, false><<<grid, 256, 0, c->stream>>>(                         \
+                c->T.p, c->P.p + (size_t)ab * g.nplanes * g.nx * g.ny, g, c->rows.p, a0 + ab,    \
+                c->twy.p, c->pmax.p, gx, total);                                                 \
         ok = true;                                                                               \
     }
             UTR_DISPATCH_N(g.ny, DHT_PC)
@@ -1045,10 +975,10 @@ int dht_run(dht_ctx* c, float* d_vol) {
             if (!ok) break;
         }
         if (!ok) return set_err(UTR_E_UNSUPPORTED, "unsupported image height for the FFT");
-        if (plane) {
-            utr::k_dht_splat_yp<<<utr::xcd_grid((unsigned)(ntiles * ((g.ny + utr::DP_G - 1) / utr::DP_G))), 256, 0,
-                                  c->stream>>>(
-                c->P.p, g, c->rows.p, c->xs.p, c->ys.p, c->zs.p, a0, na, c->pmax.p, c->acc3.p, (int)ntiles);
+        if (colsplat) {
+            const size_t lds = sizeof(unsigned long long) * utr::DC_X * utr::DC_Z * g.ny;
+            utr::k_dht_splat_col<<<(unsigned)(ctx_ * ctz), 256, lds, c->stream>>>(
+                c->P.p, g, c->rows.p, c->xs.p, c->ys.p, c->zs.p, a0, na, c->pmax.p, c->acc3.p, ctx_, ctz);
         } else {
             utr::k_dht_splat<<<(unsigned)nbricks, 256, 0, c->stream>>>(c->P.p, g, c->rows.p, c->xs.p, c->ys.p,
                                                                       c->zs.p, a0, na, c->pmax.p, c->acc3.p);
@@ -1117,7 +1047,7 @@ int dht_setup(dht_ctx* c, const dht_params* p, int nang, const double* angZ_deg,
     std::vector<double> xs((size_t)nang * 2 * g.nx), ys((size_t)nang * 2 * g.ny), zs((size_t)nang * 2 * g.nz);
     c->all_z0 = 1;
     for (int r = 0; r < nang; r++) {
-        if (angZ_deg[r] != 0.0 || g.nx > utr::DP_NXMAX) c->all_z0 = 0;  // x table staged in LDS
+        if (angZ_deg[r] != 0.0 || g.ny > utr::DC_NYMAX) c->all_z0 = 0;  // column sums in LDS
         std::memset(&rows[r], 0, sizeof rows[r]);
         dht_tables(g, angZ_deg[r], angY_deg[r], xs.data() + (size_t)r * 2 * g.nx,
                    ys.data() + (size_t)r * 2 * g.ny, zs.data() + (size_t)r * 2 * g.nz, rows[r]);
