@@ -498,8 +498,8 @@ namespace utr {
 // y).  Then sinZ = 0 and cosZ = 1: a sample's x and z grid coordinates
 // depend on its (i, n) only, and its y coordinate on its line j only (line j
 // lands in cell jj(j), weights 1 - dy1(j) and dy1(j)).  Each workgroup owns
-// a DC_X x DC_Z (x, z) column of the volume through all y, one thread per
-// source line j (and j + 256):
+// a DC_X x DC_Z (x, z) column of the volume through all y, two threads per
+// source line j (one takes the even listed samples, the other the odd):
 //   per orientation, the threads test the source samples (i, n) around the
 //   column (FP64, the reference's expressions of DHT.cpp:1119-1150) and list
 //   those whose cell (ii, nn) puts a corner in the column, with their x and
@@ -512,9 +512,10 @@ constexpr int DC_X = 4, DC_Z = 4;
 constexpr int DC_R = 6;                       // candidate half-width (source samples)
 constexpr int DC_NC = (2 * DC_R + 1) * (2 * DC_R + 1);
 constexpr int DC_NYMAX = 384;                 // tallest image (column sums of 48 KB in LDS)
-static_assert(DC_NC <= 256, "one candidate per thread");
+constexpr int DC_T = 512;                     // threads per workgroup
+static_assert(DC_NC <= DC_T, "one candidate per thread");
 
-__global__ void __launch_bounds__(256) k_dht_splat_col(const float* __restrict__ P, DhtGeom g,
+__global__ void __launch_bounds__(DC_T) k_dht_splat_col(const float* __restrict__ P, DhtGeom g,
                                                        const DhtRow* __restrict__ rows,
                                                        const double* __restrict__ xs,
                                                        const double* __restrict__ ys,
@@ -525,13 +526,13 @@ __global__ void __launch_bounds__(256) k_dht_splat_col(const float* __restrict__
     __shared__ short s_i[DC_NC], s_n[DC_NC];
     __shared__ signed char s_lx[DC_NC], s_lz[DC_NC];
     __shared__ double s_dx1[DC_NC], s_dz1[DC_NC];
-    __shared__ int s_cnt[4];
+    __shared__ int s_cnt[DC_T / 64];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int tile = (int)blockIdx.x;
     if (tile >= ntx * ntz) return;
     const int x0 = (tile % ntx) * DC_X, z0 = (tile / ntx) * DC_Z;
     const int ny = g.ny;
-    for (int t = tid; t < DC_X * DC_Z * ny; t += 256) acc[t] = 0ull;
+    for (int t = tid; t < DC_X * DC_Z * ny; t += DC_T) acc[t] = 0ull;
     // the scale of k_dht_splat
     int ek = 0, ea = 0;
     const float mk = __uint_as_float(*pmax);
@@ -586,9 +587,11 @@ __global__ void __launch_bounds__(256) k_dht_splat_col(const float* __restrict__
         const unsigned long long mask = __ballot(pass);
         if (lane == 0) s_cnt[wave] = __popcll(mask);
         __syncthreads();
-        int off = __popcll(mask & ((1ull << lane) - 1ull));
-        for (int w = 0; w < wave; w++) off += s_cnt[w];
-        const int nl = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+        int off = __popcll(mask & ((1ull << lane) - 1ull)), nl = 0;
+        for (int w = 0; w < DC_T / 64; w++) {
+            if (w < wave) off += s_cnt[w];
+            nl += s_cnt[w];
+        }
         if (pass) {
             s_i[off] = (short)ci;
             s_n[off] = (short)cn;
@@ -600,7 +603,11 @@ __global__ void __launch_bounds__(256) k_dht_splat_col(const float* __restrict__
         __syncthreads();  // the list is written
         if (nl == 0) continue;  // uniform
         const float* Pab = P + (size_t)ab * g.nplanes * n2;
-        for (int j = tid; j < ny; j += 256) {
+        // (line j, parity) pairs: thread q takes line q % ny and the listed
+        // samples of parity q / ny
+        for (int q = tid; q < 2 * ny; q += DC_T) {
+            const int j = q % ny, par = q / ny;
+            if (par >= nl) continue;
             // y of source line j as DHT.cpp:1143-1148 forms it (xx sinZ = 0)
             const double yyy = __dsub_rn(__dadd_rn(g.yc, Yt[2 * j + 1]), __dmul_rn(0.0, R.sinZ));
             double dy1 = __ddiv_rn(__dsub_rn(yyy, g.ylo), g.xst);
@@ -608,12 +615,16 @@ __global__ void __launch_bounds__(256) k_dht_splat_col(const float* __restrict__
             if (jj < 0 || jj > g.ny2) continue;
             dy1 = __dsub_rn(dy1, (double)jj);
             const double wy[2] = {__dsub_rn(1.0, dy1), dy1};
-            // the first listed sample's value loaded ahead
-            int e = 0;
-            float kv = Pab[((size_t)(g.escc ? s_n[0] : 0) * g.nx + s_i[0]) * ny + j];
-            for (; e < nl; e++) {
+            auto value = [&](int e) {
+                return Pab[((size_t)(g.escc ? s_n[e] : 0) * g.nx + s_i[e]) * ny + j];
+            };
+            // the values of the next two samples loaded ahead
+            float kv = value(par);
+            float kv2 = par + 2 < nl ? value(par + 2) : 0.f;
+            for (int e = par; e < nl; e += 2) {
                 const float kraw = kv;
-                if (e + 1 < nl) kv = Pab[((size_t)(g.escc ? s_n[e + 1] : 0) * g.nx + s_i[e + 1]) * ny + j];
+                kv = kv2;
+                if (e + 4 < nl) kv2 = value(e + 4);
                 const int lx = s_lx[e], lz = s_lz[e];
                 const double ex1 = s_dx1[e], ez1 = s_dz1[e];
                 const double wx[2] = {__dsub_rn(1.0, ex1), ex1};
@@ -637,7 +648,7 @@ __global__ void __launch_bounds__(256) k_dht_splat_col(const float* __restrict__
     __syncthreads();
     const double inv = ldexp(1.0, -sc);
     // add the column's sums to the FP64 volume: lane -> (x, y) with x fastest
-    for (int t = tid; t < DC_X * DC_Z * ny; t += 256) {
+    for (int t = tid; t < DC_X * DC_Z * ny; t += DC_T) {
         const int vx = t % DC_X, r = t / DC_X, y = r % ny, vz = r / ny;
         const int gx = x0 + vx, gz = z0 + vz;
         if (gx >= g.nx || gz >= g.nz) continue;
@@ -706,7 +717,7 @@ size_t dht_batch_bytes() {
         const long long v = std::atoll(e);
         if (v > 0) return (size_t)v;
     }
-    return size_t(2) << 30;
+    return size_t(6) << 30;
 }
 
 double tem_energy(double wl) {
@@ -977,7 +988,7 @@ int dht_run(dht_ctx* c, float* d_vol) {
         if (!ok) return set_err(UTR_E_UNSUPPORTED, "unsupported image height for the FFT");
         if (colsplat) {
             const size_t lds = sizeof(unsigned long long) * utr::DC_X * utr::DC_Z * g.ny;
-            utr::k_dht_splat_col<<<(unsigned)(ctx_ * ctz), 256, lds, c->stream>>>(
+            utr::k_dht_splat_col<<<(unsigned)(ctx_ * ctz), utr::DC_T, lds, c->stream>>>(
                 c->P.p, g, c->rows.p, c->xs.p, c->ys.p, c->zs.p, a0, na, c->pmax.p, c->acc3.p, ctx_, ctz);
         } else {
             utr::k_dht_splat<<<(unsigned)nbricks, 256, 0, c->stream>>>(c->P.p, g, c->rows.p, c->xs.p, c->ys.p,
