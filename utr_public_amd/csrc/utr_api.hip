@@ -588,7 +588,9 @@ int check_rows(utr_ctx* c, int first_row, int count) {
 // Gridding of the FP32 form on the context's stream: y-plane tiles when no
 // row rotates about Z, bricks otherwise.  UTR_GRID_FORM=brick in the
 // environment selects the bricks for every job (both give the same sums).
-int launch_grid32(utr_ctx* c, float* d_S) {
+// ev_start (may be null): recorded on the stream right before the gridding
+// kernel itself, so that the phase timer covers that kernel alone.
+int launch_grid32(utr_ctx* c, float* d_S, hipEvent_t ev_start) {
     const auto& g = c->plan.g;
     const char* form = std::getenv("UTR_GRID_FORM");
     const bool brick = form && std::strcmp(form, "brick") == 0;
@@ -612,11 +614,21 @@ int launch_grid32(utr_ctx* c, float* d_S) {
         const bool gate = g.nsr2 != 0 && g.ctf_mode != 1 && !g.dht_sctf && !g.do_tie3d &&
                           !g.do_invlap && !g.do_ctf3d;
         const unsigned nb = utr::yp_grid((int)ntiles, (int)ngroups);
+        // plane groups per block of an XCD's launch order (all: heaviest tiles
+        // of all its groups first)
+        int gblk = 0;
+        {
+            const char* e = std::getenv("UTR_GRID_GBLK");
+            gblk = e ? std::atoi(e) : 0;
+        }
+        const int ngx = (int)((ngroups + 7) / 8);
+        if (gblk <= 0 || gblk > ngx) gblk = ngx;
+        if (ev_start) UTR_HIP_TRY(hipEventRecord(ev_start, c->stream));
 #define UTR_GRID_YP(ROT, GATE)                                                                   \
     utr::k_grid_yp<ROT, GATE><<<nb, 256, 0, c->stream>>>(c->spec32.p, c->rows.p, c->rowc.p,      \
                                                          c->plan.nrows, c->scales.p, g, c->V32.p, \
                                                          d_S, (int)ntiles, (int)ngroups, pg,      \
-                                                         c->tile_order.p)
+                                                         c->tile_order.p, gblk)
         if (g.rot_centre_shift) UTR_GRID_YP(true, false);
         else if (gate) UTR_GRID_YP(false, true);
         else UTR_GRID_YP(false, false);
@@ -625,6 +637,7 @@ int launch_grid32(utr_ctx* c, float* d_S) {
         const long nblocks = (long)((g.nx + utr::SBX - 1) / utr::SBX) *
                              ((g.ny + utr::SBY - 1) / utr::SBY) * ((g.nz + utr::SBZ - 1) / utr::SBZ);
         if (nblocks <= 0 || nblocks > 0x7fffffffL) return set_err(UTR_E_INVALID, "volume too large");
+        if (ev_start) UTR_HIP_TRY(hipEventRecord(ev_start, c->stream));
         utr::k_grid_fx<<<(unsigned)nblocks, 256, 0, c->stream>>>(c->spec32.p, c->rows.p, c->plan.nrows,
                                                                  c->rowmax.p, g, c->V32.p, d_S);
     }
@@ -638,12 +651,12 @@ int do_finalize(utr_ctx* c, float* d_vol_out, double* d_vol64_out) {
         return set_err(UTR_E_STATE, "utr_finalize: " + std::to_string(c->plan.nrows - c->n_added) +
                                         " orientation rows have no image");
     const size_t N = vox(c);
-    UTR_HIP_TRY(hipEventRecord(c->ev[0], c->stream));
+    if (c->precision != UTR_FP32) UTR_HIP_TRY(hipEventRecord(c->ev[0], c->stream));
     if (c->precision != UTR_FP64) {
         if (c->keep_sampling) UTR_HIP_TRY(c->S32.alloc(N));
         float* d_S = c->keep_sampling ? c->S32.p : nullptr;
         if (c->precision == UTR_FP32) {
-            const int rc = launch_grid32(c, d_S);
+            const int rc = launch_grid32(c, d_S, c->ev[0]);
             if (rc) return rc;
         } else {
             const long nblocks = (long)((g.nx + utr::GBX - 1) / utr::GBX) *
@@ -1086,7 +1099,7 @@ int enqueue_job(utr_ctx* c, const float* d_img, float* d_vol) {
     });
     if (rc) return rc;
     if (c->precision == UTR_FP32) {
-        rc = launch_grid32(c, nullptr);
+        rc = launch_grid32(c, nullptr, nullptr);
         if (rc) return rc;
     } else {
         const long nblocks = (long)((g.nx + utr::GBX - 1) / utr::GBX) *

@@ -1222,7 +1222,7 @@ __global__ void __launch_bounds__(256, ROT ? 1 : 4) k_grid_yp(const cplx<float>*
                                                  const int* __restrict__ scales, GeomDev g,
                                                  cplx<float>* __restrict__ Vout,
                                                  float* __restrict__ Sout, int ntiles, int ngroups,
-                                                 int pg, const int* __restrict__ tile_order) {
+                                                 int pg, const int* __restrict__ tile_order, int gblk) {
     __shared__ unsigned long long acc[3 * PPAD];
     __shared__ float tiny[3 * PVOX];
     __shared__ unsigned short s_row[P_LIST];  // rows whose line reaches the tile
@@ -1231,10 +1231,15 @@ __global__ void __launch_bounds__(256, ROT ? 1 : 4) k_grid_yp(const cplx<float>*
     __shared__ int s_wsum[4];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // each XCD's groups are taken gblk at a time (fewer planes in flight per
+    // L2), and within such a block by tile rank first
     const int ngx = (ngroups + 7) >> 3;  // groups per XCD
-    const int lidx = (int)(blockIdx.x >> 3), rank = lidx / ngx;
-    const int grp = (int)(blockIdx.x & 7u) * ngx + (lidx - rank * ngx);
-    if (rank >= ntiles || grp >= ngroups) return;
+    const int lidx = (int)(blockIdx.x >> 3);
+    const int blk = lidx / (gblk * ntiles), in = lidx - blk * gblk * ntiles;
+    const int nb = min(gblk, ngx - blk * gblk);  // groups in this block
+    const int rank = in / max(nb, 1);
+    const int grp = (int)(blockIdx.x & 7u) * ngx + blk * gblk + (in - rank * nb);
+    if (nb <= 0 || rank >= ntiles || grp >= ngroups || blk * gblk + (in - rank * nb) >= ngx) return;
     const int tile = tile_order[rank];
     const int ntx = (g.nx + PX - 1) / PX;
     const int x0 = (tile % ntx) * PX, z0 = (tile / ntx) * PZ;
