@@ -265,8 +265,8 @@ def test_plane_and_brick_gridding_agree(name):
     elif name == "one":
         p, rows, imgs = case(n=32, nrows=1, seed=25)
     else:
-        # more rows than one listing chunk (256)
-        p, rows, imgs = case(n=32, nrows=300, seed=26, natoms=12)
+        # more rows than one listing chunk (384)
+        p, rows, imgs = case(n=32, nrows=400, seed=26, natoms=12)
     print("case", name, flush=True)
     (vp, sp), (vb, sb) = grid_forms(p, rows, imgs)
     assert np.array_equal(sp == 0, sb == 0)

@@ -202,7 +202,9 @@ __global__ void __launch_bounds__(256) k_dht_prop_rows(const cplx<float>* __rest
             const int r = t / N, kx = t - r * N;
             const bool in = h[q].x != 0.f || h[q].y != 0.f;  // |ap| = 1 inside the aperture, 0 outside
             if (!pack) {
-                const cplx<float> H = cmul(cmul(s_ex[0][kx], s_ey[0][r]), h[q]);  // cos chi - i sin chi
+                // cos chi - i sin chi (the aperture table only when present)
+                cplx<float> H = cmul(s_ex[0][kx], s_ey[0][r]);
+                if (g.use_ap) H = cmul(H, h[q]);
                 const float fr = H.x, fi = H.y;
                 a[lx(t)] = {v[q].x * fr - v[q].y * fi, v[q].x * fi + v[q].y * fr};
             } else {

@@ -1114,6 +1114,7 @@ constexpr int PSX = PX + 1;          // LDS stride along z (odd: lanes walking a
 constexpr int PPAD = PSX * PZ;       // LDS entries per component
 constexpr int P_LIST = 384;          // rows clipped per chunk (two per thread)
 static_assert(P_LIST <= 512, "two rows per thread");
+constexpr int P_TQ = 256;            // points with tiny corner weights recorded per plane (one per thread)
 
 // Fixed-point scales 2^sv (V3) and 2^sw (Samp3) from the largest spectrum
 // component and the row count.  Per row a voxel receives a total weight
@@ -1201,34 +1202,49 @@ __host__ __device__ constexpr unsigned yp_grid(int ntiles, int ngroups) {
 }
 
 // ROT: rows carry the centre-shift phase (compiled separately: its FP64
-// sincos would otherwise set the register budget of the common case, which
-// fits 4 workgroups per CU).  Workgroup (tile, group) grids planes
-// pg*group .. pg*group + pg - 1; ntiles tiles per plane, ngroups groups.
-// Workgroups are dealt to the XCDs round robin (workgroup L on XCD L % 8)
-// and start in launch order, so each XCD's workgroups are ordered by tile
-// rank first: tile_order lists the tiles by decreasing expected point count
-// (the lines of all rows cross near the plane's centre), and the heaviest
-// tiles start first.
+// sincos would otherwise set the register budget of the common case).
+// Workgroup (tile, group) grids planes pg*group .. pg*group + pg - 1; ntiles
+// tiles per plane, ngroups groups.  Workgroups are dealt to the XCDs round
+// robin (workgroup L on XCD L % 8) and start in launch order, so each XCD's
+// workgroups are ordered by tile rank first: tile_order lists the tiles by
+// decreasing expected point count (the lines of all rows cross near the
+// plane's centre), and the heaviest tiles start first.
 //
 // rowc: the five map coefficients the points use, per row (m0 m1 m2 m6 m8,
 // padded to 6 doubles): a compact table that stays in the L1 cache.
 // GATE: the voxel epilogue is the noise gate alone (no 3D filter, mode
 // other than 1, nonzero NSR, not the sCTF): a straight-line instantiation
 // of voxel_epilogue's noise-gate branch (the same operations).
+//
+// Corner weights below 2^-20 (a point within 2^-20 of a grid plane) keep
+// their relative precision outside the fixed-point sums: the plane's pass
+// adds the fixed-point terms of all other corners and only flags such a
+// corner.  A flagged plane (rare: about one tile plane in a hundred on a
+// general tilt series) then converts its sums to FP64 in place and runs the
+// pass again, adding just those corners' FP32 terms in FP64.  The LDS then
+// holds only the fixed-point sums and the list (28 KB), and the noise-gate
+// instantiation fits 96 VGPRs: 5 workgroups per CU.
+template <bool B>
+struct PassKind {
+    static constexpr bool value = B;
+};
+
 template <bool ROT, bool GATE>
-__global__ void __launch_bounds__(256, ROT ? 1 : 4) k_grid_yp(const cplx<float>* __restrict__ spec,
+__global__ void __launch_bounds__(256, ROT ? 1 : (GATE ? 5 : 4)) k_grid_yp(const cplx<float>* __restrict__ spec,
                                                  const RowParams* __restrict__ rows,
                                                  const double* __restrict__ rowc, int nrows,
                                                  const int* __restrict__ scales, GeomDev g,
                                                  cplx<float>* __restrict__ Vout,
                                                  float* __restrict__ Sout, int ntiles, int ngroups,
                                                  int pg, const int* __restrict__ tile_order, int gblk) {
-    __shared__ unsigned long long acc[3 * PPAD];
-    __shared__ float tiny[3 * PVOX];
-    __shared__ unsigned short s_row[P_LIST];  // rows whose line reaches the tile
+    __shared__ unsigned long long acc[3 * PPAD];  // fixed-point sums (FP64 after a tiny-weight pass)
+    __shared__ unsigned short s_row[P_LIST];      // rows whose line reaches the tile
     __shared__ short s_a0[P_LIST];                // first a of its clipped line
     __shared__ int s_pre[P_LIST + 1];             // exclusive prefix of the point counts
     __shared__ int s_wsum[4];
+    __shared__ int s_tq[P_TQ];                    // points with a corner weight below 2^-20 (row << 16 | a)
+    __shared__ int s_tn;                          // their count (may exceed P_TQ)
+    double* const accd = reinterpret_cast<double*>(acc);
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     // each XCD's groups are taken gblk at a time (fewer planes in flight per
@@ -1268,197 +1284,255 @@ __global__ void __launch_bounds__(256, ROT ? 1 : 4) k_grid_yp(const cplx<float>*
     int n = 0, total = 0, q0 = 0, p0 = 0, p1 = 0;
     for (int j = jfirst; j < jend; j++) {
         for (int t = tid; t < 3 * PPAD; t += 256) acc[t] = 0ull;
-        for (int t = tid; t < 3 * PVOX; t += 256) tiny[t] = 0.f;
+        if (tid == 0) s_tn = 0;
         __syncthreads();  // LDS zeroed (and the previous plane's epilogue done)
         const int b = j - g.nyd2;
         const double bd = (double)b;
         const double qy = __dmul_rn(g.fyst, bd);
         const double qy2 = __dmul_rn(qy, qy);
         const cplx<float>* spec_j = spec + (size_t)j * g.nx + g.nxd2;
-        for (int base = 0; base < nrows && j <= jlast_w; base += P_LIST) {
-            if (nchunks > 1 || j == jfirst) {
-                // list the chunk's rows whose lines reach the tile: two rows
-                // per thread (2 tid, 2 tid + 1), one block scan of the
-                // packed (line count << 20 | point count)
-                const int nchunk = min(P_LIST, nrows - base);
-                int npt[2] = {0, 0}, a0s[2] = {0, 0};
+        // One point of row r: a, its spectrum value, the row's coefficients.
+        // FIX false: the fixed-point terms of every in-tile corner of weight
+        // >= 2^-20, and the point recorded when a smaller nonzero one is in
+        // the tile.  FIX true: only those small corners, their FP32 terms
+        // added in FP64 to the converted sums.
+        auto splat = [&](auto kind, const int r, const int a, const cplx<float> kraw, const double m0,
+                         const double m1, const double m2, const double m6, const double m8,
+                         const double* ph) {
+            constexpr bool FIX = decltype(kind)::value;
+            const double ad = (double)a;
+            const double qx = __dmul_rn(g.fxst, ad);
+            const double zz = g.escc ? __dmul_rn(-g.wl2, __fma_rn(qx, qx, qy2)) : 0.0;
+            const double ox = __fma_rn(ad, m0, __fma_rn(bd, m1, __dmul_rn(zz, m2)));
+            const double oz = __fma_rn(ad, m6, __dmul_rn(zz, m8));
+            const double dx1 = __dadd_rn(g.d_nxd2, ox);
+            const double dz1 = __dadd_rn(g.d_nzd2, oz);
+            const double fnn = floor(dz1), fii = floor(dx1);
+            if (fnn < 0.0 || fnn > zmaxd) return;
+            if (fii < 0.0 || fii > xmaxd) return;
+            const int ii = (int)fii - x0, nn = (int)fnn - z0;
+            if (ii < -1 || ii >= PX || nn < -1 || nn >= PZ) return;
+            const double fx = dx1 - fii, fz = dz1 - fnn;
+            // with both 1D weights of each axis at least 2^-10 every
+            // corner weight is at least 2^-20: the per-corner tests
+            // only run when a fraction lies within 2^-10 of a grid plane
+            const bool near = !(fx >= S_TINY_1D && fx <= 1.0 - S_TINY_1D &&
+                                fz >= S_TINY_1D && fz <= 1.0 - S_TINY_1D);
+            if (FIX && !near) return;
+            const double wxs[2] = {1.0 - fx, fx}, wzs[2] = {1.0 - fz, fz};
+            cplx<float> kv = kraw;
+            if (ROT) {
+                const double phs = -(ph[0] * ox + ph[1] * bd + ph[2] * oz);
+                double sn, c;
+                sincos(phs, &sn, &c);
+                const float cf = (float)c, sf = (float)sn;
+                kv = {kraw.x * cf - kraw.y * sf, kraw.x * sf + kraw.y * cf};
+            }
+            const int li0 = ii + PSX * nn;
+            if (FIX) {
 #pragma unroll
-                for (int h = 0; h < 2; h++) {
-                    const int ri = 2 * tid + h;
-                    if (ri < nchunk)
-                        npt[h] = clip_line(g, rows[base + ri], fbm, fbh, fb2m, fb2h, bmin, bmax, fx2,
-                                           fy2, wl2f, a0s[h]);
+                for (int c = 0; c < 4; c++) {
+                    const int cx = c & 1, cz = c >> 1;
+                    const double wd = wxs[cx] * 1.0 * wzs[cz];
+                    const bool inb = (unsigned)(ii + cx) < (unsigned)PX &&
+                                     (unsigned)(nn + cz) < (unsigned)PZ;
+                    if (!(inb && wd != 0.0 && wd < S_TINY_D)) continue;
+                    const int li = li0 + cx + PSX * cz;
+                    const float w = (float)wd;
+                    atomicAdd(&accd[li], (double)(w * kv.x));
+                    atomicAdd(&accd[PPAD + li], (double)(w * kv.y));
+                    atomicAdd(&accd[2 * PPAD + li], (double)w);
                 }
-                const int v = ((int)(npt[0] > 0) + (int)(npt[1] > 0)) << 20 | (npt[0] + npt[1]);
-                int incl = v;
-                for (int o = 1; o < 64; o <<= 1) {
-                    const int u = __shfl_up(incl, o);
-                    if (lane >= o) incl += u;
-                }
-                if (lane == 63) s_wsum[wave] = incl;
-                __syncthreads();
-                int woff = 0;
-                for (int w = 0; w < wave; w++) woff += s_wsum[w];
-                const int tot = s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
-                const int ex = woff + incl - v;
-                int off = ex >> 20, pre = ex & 0xfffff;
+                return;
+            }
+            // the spectrum value on the V3 fixed-point grid (exact: a power of two)
+            const double kxs = (double)kv.x * vscale, kys = (double)kv.y * vscale;
+            long long tr[4], ti[4], tw[4];
+            bool big[4];
+            double wds[4];
 #pragma unroll
-                for (int h = 0; h < 2; h++) {
-                    if (npt[h] > 0) {
-                        s_row[off] = (unsigned short)(base + 2 * tid + h);
-                        s_a0[off] = (short)a0s[h];
-                        s_pre[off] = pre;
-                        off++;
-                        pre += npt[h];
+            for (int c = 0; c < 4; c++) {
+                const int cx = c & 1, cz = c >> 1;
+                big[c] = (unsigned)(ii + cx) < (unsigned)PX && (unsigned)(nn + cz) < (unsigned)PZ;
+                const double wd = wxs[cx] * 1.0 * wzs[cz];
+                wds[c] = wd;
+                tr[c] = fx_term(wd, kxs);
+                ti[c] = fx_term(wd, kys);
+                tw[c] = fx_term(wd, wscale);
+            }
+            if (near) {
+                bool any_tiny = false;
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    const bool inb = big[c];
+                    big[c] = inb && wds[c] >= S_TINY_D;
+                    any_tiny |= inb && wds[c] != 0.0 && wds[c] < S_TINY_D;
+                }
+                if (any_tiny) {
+                    const int slot = atomicAdd(&s_tn, 1);
+                    if (slot < P_TQ) s_tq[slot] = (int)(((unsigned)r << 16) | ((unsigned)a & 0xffffu));
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                if (!big[c]) continue;
+                const int li = li0 + (c & 1) + PSX * (c >> 1);
+                atomicAdd(&acc[li], (unsigned long long)tr[c]);
+                atomicAdd(&acc[PPAD + li], (unsigned long long)ti[c]);
+                atomicAdd(&acc[2 * PPAD + li], (unsigned long long)tw[c]);
+            }
+        };
+        // One pass of splat over all the plane's listed points.
+        auto pass = [&](auto kind) {
+            constexpr bool FIX = decltype(kind)::value;
+            for (int base = 0; base < nrows && j <= jlast_w; base += P_LIST) {
+                if (nchunks > 1 || (j == jfirst && !FIX)) {
+                    // list the chunk's rows whose lines reach the tile: two rows
+                    // per thread (2 tid, 2 tid + 1), one block scan of the
+                    // packed (line count << 20 | point count)
+                    const int nchunk = min(P_LIST, nrows - base);
+                    int npt[2] = {0, 0}, a0s[2] = {0, 0};
+#pragma unroll
+                    for (int h = 0; h < 2; h++) {
+                        const int ri = 2 * tid + h;
+                        if (ri < nchunk)
+                            npt[h] = clip_line(g, rows[base + ri], fbm, fbh, fb2m, fb2h, bmin, bmax, fx2,
+                                               fy2, wl2f, a0s[h]);
+                    }
+                    const int v = ((int)(npt[0] > 0) + (int)(npt[1] > 0)) << 20 | (npt[0] + npt[1]);
+                    int incl = v;
+                    for (int o = 1; o < 64; o <<= 1) {
+                        const int u = __shfl_up(incl, o);
+                        if (lane >= o) incl += u;
+                    }
+                    if (lane == 63) s_wsum[wave] = incl;
+                    __syncthreads();
+                    int woff = 0;
+                    for (int w = 0; w < wave; w++) woff += s_wsum[w];
+                    const int tot = s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
+                    const int ex = woff + incl - v;
+                    int off = ex >> 20, pre = ex & 0xfffff;
+#pragma unroll
+                    for (int h = 0; h < 2; h++) {
+                        if (npt[h] > 0) {
+                            s_row[off] = (unsigned short)(base + 2 * tid + h);
+                            s_a0[off] = (short)a0s[h];
+                            s_pre[off] = pre;
+                            off++;
+                            pre += npt[h];
+                        }
+                    }
+                    n = tot >> 20;
+                    total = tot & 0xfffff;
+                    if (tid == 0) s_pre[n] = total;
+                    __syncthreads();
+                    // each thread takes an equal run of the listed points, in list
+                    // order; q0: the listed line holding its first point
+                    p0 = (int)(((long long)tid * total) >> 8);
+                    p1 = (int)(((long long)(tid + 1) * total) >> 8);
+                    if (p0 < p1) {
+                        int lo = 0, hi = n - 1;
+                        while (lo < hi) {
+                            const int mid = (lo + hi + 1) >> 1;
+                            if (s_pre[mid] <= p0) lo = mid;
+                            else hi = mid - 1;
+                        }
+                        q0 = lo;
                     }
                 }
-                n = tot >> 20;
-                total = tot & 0xfffff;
-                if (tid == 0) s_pre[n] = total;
-                __syncthreads();
-                // each thread takes an equal run of the listed points, in list
-                // order; q0: the listed line holding its first point
-                p0 = (int)(((long long)tid * total) >> 8);
-                p1 = (int)(((long long)(tid + 1) * total) >> 8);
                 if (p0 < p1) {
-                    int lo = 0, hi = n - 1;
-                    while (lo < hi) {
-                        const int mid = (lo + hi + 1) >> 1;
-                        if (s_pre[mid] <= p0) lo = mid;
-                        else hi = mid - 1;
-                    }
-                    q0 = lo;
-                }
-            }
-            if (p0 < p1) {
-                int q = q0;
-                // the current line: its row's coefficients, first a and end;
-                // the next listed line's spectrum pointer (for the one-ahead
-                // value load)
-                int r = s_row[q];
-                int abase = s_a0[q] - s_pre[q], pend = s_pre[q + 1];
-                double m0, m1, m2, m6, m8;
-                double ph[3] = {0.0, 0.0, 0.0};
-                auto load_row = [&]() {
-                    const double* c6 = rowc + 6 * r;
-                    m0 = c6[0]; m1 = c6[1]; m2 = c6[2]; m6 = c6[3]; m8 = c6[4];
-                    if (ROT) {
-                        ph[0] = rows[r].ph[0];
-                        ph[1] = rows[r].ph[1];
-                        ph[2] = rows[r].ph[2];
-                    }
-                };
-                load_row();
-                const cplx<float>* Kl = spec_j + (size_t)((unsigned)r & 0xffffu) * rstride + abase;
-                const cplx<float>* Kn = Kl;
-                int rn = r;
-                auto next_line = [&]() {
-                    if (q + 1 < n) {
-                        rn = s_row[q + 1];
-                        Kn = spec_j + (size_t)((unsigned)rn & 0xffffu) * rstride + (s_a0[q + 1] - s_pre[q + 1]);
-                    }
-                };
-                next_line();
-                auto splat = [&](const int a, const cplx<float> kraw) {
-                    const double ad = (double)a;
-                    const double qx = __dmul_rn(g.fxst, ad);
-                    const double zz = g.escc ? __dmul_rn(-g.wl2, __fma_rn(qx, qx, qy2)) : 0.0;
-                    const double ox = __fma_rn(ad, m0, __fma_rn(bd, m1, __dmul_rn(zz, m2)));
-                    const double oz = __fma_rn(ad, m6, __dmul_rn(zz, m8));
-                    const double dx1 = __dadd_rn(g.d_nxd2, ox);
-                    const double dz1 = __dadd_rn(g.d_nzd2, oz);
-                    const double fnn = floor(dz1), fii = floor(dx1);
-                    if (fnn < 0.0 || fnn > zmaxd) return;
-                    if (fii < 0.0 || fii > xmaxd) return;
-                    const int ii = (int)fii - x0, nn = (int)fnn - z0;
-                    if (ii < -1 || ii >= PX || nn < -1 || nn >= PZ) return;
-                    const double fx = dx1 - fii, fz = dz1 - fnn;
-                    const double wxs[2] = {1.0 - fx, fx}, wzs[2] = {1.0 - fz, fz};
-                    cplx<float> kv = kraw;
-                    if (ROT) {
-                        const double phs = -(ph[0] * ox + ph[1] * bd + ph[2] * oz);
-                        double sn, c;
-                        sincos(phs, &sn, &c);
-                        const float cf = (float)c, sf = (float)sn;
-                        kv = {kraw.x * cf - kraw.y * sf, kraw.x * sf + kraw.y * cf};
-                    }
-                    // the spectrum value on the V3 fixed-point grid (exact: a power of two)
-                    const double kxs = (double)kv.x * vscale, kys = (double)kv.y * vscale;
-                    long long tr[4], ti[4], tw[4];
-                    float wc[4];
-                    bool big[4], tny[4] = {false, false, false, false};
-                    bool any_tiny = false;
-                    double wds[4];
-#pragma unroll
-                    for (int c = 0; c < 4; c++) {
-                        const int cx = c & 1, cz = c >> 1;
-                        big[c] = (unsigned)(ii + cx) < (unsigned)PX && (unsigned)(nn + cz) < (unsigned)PZ;
-                        const double wd = wxs[cx] * 1.0 * wzs[cz];
-                        wds[c] = wd;
-                        wc[c] = (float)wd;
-                        tr[c] = fx_term(wd, kxs);
-                        ti[c] = fx_term(wd, kys);
-                        tw[c] = fx_term(wd, wscale);
-                    }
-                    // with both 1D weights of each axis at least 2^-10 every corner
-                    // weight is at least 2^-20: the per-corner tests only run when a
-                    // fraction lies within 2^-10 of a grid plane
-                    if (!(fx >= S_TINY_1D && fx <= 1.0 - S_TINY_1D && fz >= S_TINY_1D && fz <= 1.0 - S_TINY_1D)) {
-#pragma unroll
-                        for (int c = 0; c < 4; c++) {
-                            const bool inb = big[c];
-                            big[c] = inb && wds[c] >= S_TINY_D;
-                            tny[c] = inb && wds[c] != 0.0 && wds[c] < S_TINY_D;
-                            any_tiny |= tny[c];
+                    int q = q0;
+                    // the current line: its row's coefficients, first a and end;
+                    // the next listed line's spectrum pointer (for the one-ahead
+                    // value load)
+                    int r = s_row[q];
+                    int abase = s_a0[q] - s_pre[q], pend = s_pre[q + 1];
+                    double m0, m1, m2, m6, m8;
+                    double ph[3] = {0.0, 0.0, 0.0};
+                    auto load_row = [&]() {
+                        const double* c6 = rowc + 6 * r;
+                        m0 = c6[0]; m1 = c6[1]; m2 = c6[2]; m6 = c6[3]; m8 = c6[4];
+                        if (ROT) {
+                            ph[0] = rows[r].ph[0];
+                            ph[1] = rows[r].ph[1];
+                            ph[2] = rows[r].ph[2];
                         }
-                    }
-                    const int li0 = ii + PSX * nn;
-#pragma unroll
-                    for (int c = 0; c < 4; c++) {
-                        if (!big[c]) continue;
-                        const int li = li0 + (c & 1) + PSX * (c >> 1);
-                        atomicAdd(&acc[li], (unsigned long long)tr[c]);
-                        atomicAdd(&acc[PPAD + li], (unsigned long long)ti[c]);
-                        atomicAdd(&acc[2 * PPAD + li], (unsigned long long)tw[c]);
-                    }
-                    if (any_tiny) {
-                        // weights below 2^-20: float accumulators (rare)
-#pragma unroll
-                        for (int c = 0; c < 4; c++) {
-                            if (!tny[c]) continue;
-                            const int v = ii + (c & 1) + PX * (nn + (c >> 1));
-                            const float w = wc[c];
-                            atomicAdd(&tiny[v], w * kv.x);
-                            atomicAdd(&tiny[PVOX + v], w * kv.y);
-                            atomicAdd(&tiny[2 * PVOX + v], w);
+                    };
+                    load_row();
+                    const cplx<float>* Kl = spec_j + (size_t)((unsigned)r & 0xffffu) * rstride + abase;
+                    const cplx<float>* Kn = Kl;
+                    int rn = r;
+                    auto next_line = [&]() {
+                        if (q + 1 < n) {
+                            rn = s_row[q + 1];
+                            Kn = spec_j + (size_t)((unsigned)rn & 0xffffu) * rstride + (s_a0[q + 1] - s_pre[q + 1]);
                         }
+                    };
+                    next_line();
+                    cplx<float> kv = Kl[p0];
+                    for (int p = p0; p < p1; p++) {
+                        if (p == pend) {
+                            // enter the next listed line
+                            q++;
+                            r = rn;
+                            Kl = Kn;
+                            abase = s_a0[q] - s_pre[q];
+                            pend = s_pre[q + 1];
+                            load_row();
+                            next_line();
+                        }
+                        const cplx<float> kraw = kv;
+                        // the next point's value: this row, or the next one
+                        if (p + 1 < p1) kv = (p + 1 < pend) ? Kl[p + 1] : Kn[p + 1];
+                        splat(kind, r, abase + p, kraw, m0, m1, m2, m6, m8, ph);
                     }
-                };
-                cplx<float> kv = Kl[p0];
-                for (int p = p0; p < p1; p++) {
-                    if (p == pend) {
-                        // enter the next listed line
-                        q++;
-                        r = rn;
-                        Kl = Kn;
-                        abase = s_a0[q] - s_pre[q];
-                        pend = s_pre[q + 1];
-                        load_row();
-                        next_line();
-                    }
-                    const cplx<float> kraw = kv;
-                    // the next point's value: this row, or the next one
-                    if (p + 1 < p1) kv = (p + 1 < pend) ? Kl[p + 1] : Kn[p + 1];
-                    splat(abase + p, kraw);
                 }
+                __syncthreads();  // the sums of the chunk are complete; its list may be replaced
             }
-            __syncthreads();  // the sums of the chunk are complete; its list may be replaced
-        }
+        };
+        pass(PassKind<false>{});
         // an opaque copy of the thread index: the voxel terms below are then
         // evaluated per plane instead of being kept live (and spilled)
         // across the plane loop
         int vt = tid;
         asm volatile("" : "+v"(vt));
+        const int tn = __builtin_amdgcn_readfirstlane(s_tn);  // same in all lanes
+        const bool fixd = tn != 0;
+        if (fixd) {
+            // each thread converts its own voxels' sums to FP64 (the values
+            // the epilogue would form), then the small corners are added:
+            // those of the recorded points, or a second pass over all points
+            // when more were met than the record holds
+#pragma unroll
+            for (int k = 0; k < POWN; k++) {
+                const int v = vt + 256 * k;
+                const int li = (v % PX) + PSX * (v / PX);
+#pragma unroll
+                for (int m = 0; m < 3; m++) {
+                    const double d = (double)(long long)acc[m * PPAD + li] * (m < 2 ? vinv : winv);
+                    accd[m * PPAD + li] = d;
+                }
+            }
+            __syncthreads();
+            if (tn <= P_TQ) {
+                if (tid < tn) {
+                    const int e = s_tq[tid];
+                    const int r = (int)((unsigned)e >> 16), a = (int)(short)(e & 0xffff);
+                    const double* c6 = rowc + 6 * r;
+                    double ph[3] = {0.0, 0.0, 0.0};
+                    if (ROT) {
+                        ph[0] = rows[r].ph[0];
+                        ph[1] = rows[r].ph[1];
+                        ph[2] = rows[r].ph[2];
+                    }
+                    const cplx<float> kraw = spec_j[(size_t)r * rstride + a];
+                    splat(PassKind<true>{}, r, a, kraw, c6[0], c6[1], c6[2], c6[3], c6[4], ph);
+                }
+                __syncthreads();
+            } else {
+                pass(PassKind<true>{});
+            }
+        }
 #pragma unroll
         for (int k = 0; k < POWN; k++) {
             const int v = vt + 256 * k;
@@ -1466,9 +1540,16 @@ __global__ void __launch_bounds__(256, ROT ? 1 : 4) k_grid_yp(const cplx<float>*
             const int vx = x0 + lx, vz = z0 + lz;
             if (vx >= g.nx || vz >= g.nz) continue;
             const int li = lx + PSX * lz;
-            const float vr = (float)((double)(long long)acc[li] * vinv + (double)tiny[v]);
-            const float vi = (float)((double)(long long)acc[PPAD + li] * vinv + (double)tiny[PVOX + v]);
-            const float S = (float)((double)(long long)acc[2 * PPAD + li] * winv + (double)tiny[2 * PVOX + v]);
+            float vr, vi, S;
+            if (fixd) {
+                vr = (float)accd[li];
+                vi = (float)accd[PPAD + li];
+                S = (float)accd[2 * PPAD + li];
+            } else {
+                vr = (float)((double)(long long)acc[li] * vinv);
+                vi = (float)((double)(long long)acc[PPAD + li] * vinv);
+                S = (float)((double)(long long)acc[2 * PPAD + li] * winv);
+            }
             const size_t vidx = ((size_t)vz * g.ny + j) * g.nx + vx;
             if (Sout) Sout[vidx] = S;
             if (GATE) {
