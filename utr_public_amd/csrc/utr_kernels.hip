@@ -1110,8 +1110,16 @@ constexpr int PX = 32, PZ = 32;
 constexpr int PVOX = PX * PZ;
 constexpr int POWN = PVOX / 256;
 static_assert(PVOX % 256 == 0, "tile must be a multiple of the workgroup");
-constexpr int PSX = PX + 1;          // LDS stride along z (odd: lanes walking along z spread over banks)
-constexpr int PPAD = PSX * PZ;       // LDS entries per component
+// LDS sums of a tile with a one-cell halo: cell (x, z), x and z in
+// [-1, PX] x [-1, PZ], sits at x + PSX z + P_OFF.  The stride is odd (lanes
+// walking along z spread over the banks) and one short of the halo's width,
+// so the halo cells x = -1 and x = PX of neighbouring rows share a slot; no
+// tile cell shares one.  Every corner of a point whose cell touches the tile
+// then has a slot, and corners outside the tile add to the halo, which is
+// never read: the corners need no bounds tests.
+constexpr int PSX = PX + 1;          // LDS stride along z
+constexpr int P_OFF = PSX + 1;       // slot of cell (0, 0)
+constexpr int PPAD = PX + PSX * PZ + P_OFF + 1;  // LDS entries per component
 constexpr int P_LIST = 384;          // rows clipped per chunk (two per thread)
 static_assert(P_LIST <= 512, "two rows per thread");
 constexpr int P_TQ = 256;            // points with tiny corner weights recorded per plane (one per thread)
@@ -1328,7 +1336,7 @@ __global__ void __launch_bounds__(256, ROT ? 1 : (GATE ? 5 : 4)) k_grid_yp(const
                 const float cf = (float)c, sf = (float)sn;
                 kv = {kraw.x * cf - kraw.y * sf, kraw.x * sf + kraw.y * cf};
             }
-            const int li0 = ii + PSX * nn;
+            const int li0 = ii + PSX * nn + P_OFF;
             if (FIX) {
 #pragma unroll
                 for (int c = 0; c < 4; c++) {
@@ -1345,41 +1353,36 @@ __global__ void __launch_bounds__(256, ROT ? 1 : (GATE ? 5 : 4)) k_grid_yp(const
                 }
                 return;
             }
-            // the spectrum value on the V3 fixed-point grid (exact: a power of two)
-            const double kxs = (double)kv.x * vscale, kys = (double)kv.y * vscale;
-            long long tr[4], ti[4], tw[4];
-            bool big[4];
             double wds[4];
 #pragma unroll
-            for (int c = 0; c < 4; c++) {
-                const int cx = c & 1, cz = c >> 1;
-                big[c] = (unsigned)(ii + cx) < (unsigned)PX && (unsigned)(nn + cz) < (unsigned)PZ;
-                const double wd = wxs[cx] * 1.0 * wzs[cz];
-                wds[c] = wd;
-                tr[c] = fx_term(wd, kxs);
-                ti[c] = fx_term(wd, kys);
-                tw[c] = fx_term(wd, wscale);
-            }
+            for (int c = 0; c < 4; c++) wds[c] = wxs[c & 1] * 1.0 * wzs[c >> 1];
             if (near) {
+                // weights below 2^-20 leave the fixed-point sums (their terms
+                // become 0); the point is recorded when such a corner is in
+                // the tile
                 bool any_tiny = false;
 #pragma unroll
                 for (int c = 0; c < 4; c++) {
-                    const bool inb = big[c];
-                    big[c] = inb && wds[c] >= S_TINY_D;
-                    any_tiny |= inb && wds[c] != 0.0 && wds[c] < S_TINY_D;
+                    const bool tny = wds[c] != 0.0 && wds[c] < S_TINY_D;
+                    any_tiny |= tny && (unsigned)(ii + (c & 1)) < (unsigned)PX &&
+                                (unsigned)(nn + (c >> 1)) < (unsigned)PZ;
+                    if (tny) wds[c] = 0.0;
                 }
                 if (any_tiny) {
                     const int slot = atomicAdd(&s_tn, 1);
                     if (slot < P_TQ) s_tq[slot] = (int)(((unsigned)r << 16) | ((unsigned)a & 0xffffu));
                 }
             }
+            // the spectrum value on the V3 fixed-point grid (exact: a power of
+            // two); all 4 corners add (corners outside the tile to the halo,
+            // zero weights a zero term)
+            const double kxs = (double)kv.x * vscale, kys = (double)kv.y * vscale;
 #pragma unroll
             for (int c = 0; c < 4; c++) {
-                if (!big[c]) continue;
                 const int li = li0 + (c & 1) + PSX * (c >> 1);
-                atomicAdd(&acc[li], (unsigned long long)tr[c]);
-                atomicAdd(&acc[PPAD + li], (unsigned long long)ti[c]);
-                atomicAdd(&acc[2 * PPAD + li], (unsigned long long)tw[c]);
+                atomicAdd(&acc[li], (unsigned long long)fx_term(wds[c], kxs));
+                atomicAdd(&acc[PPAD + li], (unsigned long long)fx_term(wds[c], kys));
+                atomicAdd(&acc[2 * PPAD + li], (unsigned long long)fx_term(wds[c], wscale));
             }
         };
         // One pass of splat over all the plane's listed points.
@@ -1426,10 +1429,10 @@ __global__ void __launch_bounds__(256, ROT ? 1 : (GATE ? 5 : 4)) k_grid_yp(const
                     total = tot & 0xfffff;
                     if (tid == 0) s_pre[n] = total;
                     __syncthreads();
-                    // each thread takes an equal run of the listed points, in list
+                    // each wave takes an equal run of the listed points, in list
                     // order; q0: the listed line holding its first point
-                    p0 = (int)(((long long)tid * total) >> 8);
-                    p1 = (int)(((long long)(tid + 1) * total) >> 8);
+                    p0 = (int)(((long long)wave * total) >> 2);
+                    p1 = (int)(((long long)(wave + 1) * total) >> 2);
                     if (p0 < p1) {
                         int lo = 0, hi = n - 1;
                         while (lo < hi) {
@@ -1440,52 +1443,33 @@ __global__ void __launch_bounds__(256, ROT ? 1 : (GATE ? 5 : 4)) k_grid_yp(const
                         q0 = lo;
                     }
                 }
-                if (p0 < p1) {
-                    int q = q0;
-                    // the current line: its row's coefficients, first a and end;
-                    // the next listed line's spectrum pointer (for the one-ahead
-                    // value load)
-                    int r = s_row[q];
-                    int abase = s_a0[q] - s_pre[q], pend = s_pre[q + 1];
-                    double m0, m1, m2, m6, m8;
-                    double ph[3] = {0.0, 0.0, 0.0};
-                    auto load_row = [&]() {
+                // the wave's run, 64 consecutive points at a time (lane l takes
+                // the chunk's point l): the lanes read consecutive values of
+                // one or a few lines and add to neighbouring cells
+                int qc = __builtin_amdgcn_readfirstlane(q0);
+                const int pw1 = __builtin_amdgcn_readfirstlane(p1);
+                for (int pb = __builtin_amdgcn_readfirstlane(p0); pb < pw1; pb += 64) {
+                    const int p = pb + lane;
+                    int q = qc, pre = s_pre[qc], nxt = s_pre[qc + 1];
+                    if (p < pw1) {
+                        while (nxt <= p) {
+                            q++;
+                            pre = nxt;
+                            nxt = s_pre[q + 1];
+                        }
+                        const int r = s_row[q];
+                        const int a = s_a0[q] + (p - pre);
                         const double* c6 = rowc + 6 * r;
-                        m0 = c6[0]; m1 = c6[1]; m2 = c6[2]; m6 = c6[3]; m8 = c6[4];
+                        double ph[3] = {0.0, 0.0, 0.0};
                         if (ROT) {
                             ph[0] = rows[r].ph[0];
                             ph[1] = rows[r].ph[1];
                             ph[2] = rows[r].ph[2];
                         }
-                    };
-                    load_row();
-                    const cplx<float>* Kl = spec_j + (size_t)((unsigned)r & 0xffffu) * rstride + abase;
-                    const cplx<float>* Kn = Kl;
-                    int rn = r;
-                    auto next_line = [&]() {
-                        if (q + 1 < n) {
-                            rn = s_row[q + 1];
-                            Kn = spec_j + (size_t)((unsigned)rn & 0xffffu) * rstride + (s_a0[q + 1] - s_pre[q + 1]);
-                        }
-                    };
-                    next_line();
-                    cplx<float> kv = Kl[p0];
-                    for (int p = p0; p < p1; p++) {
-                        if (p == pend) {
-                            // enter the next listed line
-                            q++;
-                            r = rn;
-                            Kl = Kn;
-                            abase = s_a0[q] - s_pre[q];
-                            pend = s_pre[q + 1];
-                            load_row();
-                            next_line();
-                        }
-                        const cplx<float> kraw = kv;
-                        // the next point's value: this row, or the next one
-                        if (p + 1 < p1) kv = (p + 1 < pend) ? Kl[p + 1] : Kn[p + 1];
-                        splat(kind, r, abase + p, kraw, m0, m1, m2, m6, m8, ph);
+                        const cplx<float> kraw = spec_j[(size_t)r * rstride + a];
+                        splat(kind, r, a, kraw, c6[0], c6[1], c6[2], c6[3], c6[4], ph);
                     }
+                    qc = __builtin_amdgcn_readlane(q, 63);  // the line of the chunk's last point
                 }
                 __syncthreads();  // the sums of the chunk are complete; its list may be replaced
             }
@@ -1506,7 +1490,7 @@ __global__ void __launch_bounds__(256, ROT ? 1 : (GATE ? 5 : 4)) k_grid_yp(const
 #pragma unroll
             for (int k = 0; k < POWN; k++) {
                 const int v = vt + 256 * k;
-                const int li = (v % PX) + PSX * (v / PX);
+                const int li = (v % PX) + PSX * (v / PX) + P_OFF;
 #pragma unroll
                 for (int m = 0; m < 3; m++) {
                     const double d = (double)(long long)acc[m * PPAD + li] * (m < 2 ? vinv : winv);
@@ -1539,7 +1523,7 @@ __global__ void __launch_bounds__(256, ROT ? 1 : (GATE ? 5 : 4)) k_grid_yp(const
             const int lx = v % PX, lz = v / PX;
             const int vx = x0 + lx, vz = z0 + lz;
             if (vx >= g.nx || vz >= g.nz) continue;
-            const int li = lx + PSX * lz;
+            const int li = lx + PSX * lz + P_OFF;
             float vr, vi, S;
             if (fixd) {
                 vr = (float)accd[li];
