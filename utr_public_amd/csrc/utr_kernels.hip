@@ -444,6 +444,10 @@ __device__ __forceinline__ float recip_nr(float b) {
     return fmaf(fmaf(-b, r, 1.f), r, r);
 }
 __device__ __forceinline__ double recip_nr(double b) { return 1.0 / b; }
+// square root: in FP32 the hardware instruction (within 1 ulp), in FP64 the
+// IEEE one
+__device__ __forceinline__ float vsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double vsqrt(double x) { return sqrt(x); }
 
 // Sampling normalisation + noise filter (UTR.cpp:1050-1128) and the 3D
 // filters (UTR.cpp:1177-1201; UTR.h:15-185) for voxel (i, j, k).  Factors
@@ -470,14 +474,14 @@ __device__ __forceinline__ cplx<T> voxel_epilogue(const GeomDev& g, int i, int j
         if (x == T(0)) return {T(0), T(0)};
         T fnsr;
         if (g.ctf_mode != 1) {
-            fnsr = T(g.nsr2 * (double)sqrt(x));
+            fnsr = T(g.nsr2 * (double)vsqrt(x));
         } else {
             const int k1 = k - g.nzd2, j1 = j - g.nyd2, i1 = i - g.nxd2;
             const double zeta2 = k1 * k1 * g.ep_dzeta2 - g.asigma;
             const double eta2 = j1 * j1 * g.ep_deta2 + zeta2;
-            fnsr = T(g.ep_nsr1 * (double)sqrt(x) / (g.ep_dksi2 * i1 * i1 + eta2 + g.epsilon));
+            fnsr = T(g.ep_nsr1 * (double)vsqrt(x) / (g.ep_dksi2 * i1 * i1 + eta2 + g.epsilon));
         }
-        const T y = sqrt(v.x * v.x + v.y * v.y) * recip_nr(fnsr);
+        const T y = vsqrt(v.x * v.x + v.y * v.y) * recip_nr(fnsr);
         const T f = recip_nr(x) * (T(0.5) + atan(T(100) * (y - target)) * inv_pi);
         v.x *= f;
         v.y *= f;
@@ -1517,6 +1521,9 @@ __global__ void __launch_bounds__(256, ROT ? 1 : (GATE ? 5 : 4)) k_grid_yp(const
                 pass(PassKind<true>{});
             }
         }
+        // voxel (x0 + vt % PX, j, z0 + vt / PX + k 256 / PX) for k < POWN
+        const size_t kstride = (size_t)(256 / PX) * g.ny * g.nx;
+        const size_t vidx0 = ((size_t)(z0 + vt / PX) * g.ny + j) * g.nx + x0 + vt % PX;
 #pragma unroll
         for (int k = 0; k < POWN; k++) {
             const int v = vt + 256 * k;
@@ -1534,14 +1541,14 @@ __global__ void __launch_bounds__(256, ROT ? 1 : (GATE ? 5 : 4)) k_grid_yp(const
                 vi = (float)((double)(long long)acc[PPAD + li] * vinv);
                 S = (float)((double)(long long)acc[2 * PPAD + li] * winv);
             }
-            const size_t vidx = ((size_t)vz * g.ny + j) * g.nx + vx;
+            const size_t vidx = vidx0 + k * kstride;
             if (Sout) Sout[vidx] = S;
             if (GATE) {
                 // voxel_epilogue's noise-gate branch (UTR.cpp:1050-1128)
                 cplx<float> o = {0.f, 0.f};
                 if (S != 0.f) {
-                    const float fnsr = (float)(g.nsr2 * (double)sqrtf(S));
-                    const float y = sqrtf(vr * vr + vi * vi) * recip_nr(fnsr);
+                    const float fnsr = (float)(g.nsr2 * (double)vsqrt(S));
+                    const float y = vsqrt(vr * vr + vi * vi) * recip_nr(fnsr);
                     const float f = recip_nr(S) * (0.5f + atanf(100.f * (y - (float)kTargetSNR)) *
                                                            0.318309886183790671538f);
                     o = {vr * f, vi * f};
