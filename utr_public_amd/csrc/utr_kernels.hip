@@ -1249,9 +1249,8 @@ __global__ void __launch_bounds__(256, ROT ? 1 : (GATE ? 5 : 4)) k_grid_yp(const
                                                  cplx<float>* __restrict__ Vout,
                                                  float* __restrict__ Sout, int ntiles, int ngroups,
                                                  int pg, const int* __restrict__ tile_order, int gblk) {
-    __shared__ unsigned long long acc[3 * PPAD];  // fixed-point sums (FP64 after a tiny-weight pass)
-    __shared__ unsigned short s_row[P_LIST];      // rows whose line reaches the tile
-    __shared__ short s_a0[P_LIST];                // first a of its clipped line
+    __shared__ __align__(16) unsigned long long acc[3 * PPAD];  // fixed-point sums (FP64 after a tiny-weight pass)
+    __shared__ int s_line[P_LIST];                // listed lines: row << 16 | first a of the clipped line
     __shared__ int s_pre[P_LIST + 1];             // exclusive prefix of the point counts
     __shared__ int s_wsum[4];
     __shared__ int s_tq[P_TQ];                    // points with a corner weight below 2^-20 (row << 16 | a)
@@ -1295,8 +1294,12 @@ __global__ void __launch_bounds__(256, ROT ? 1 : (GATE ? 5 : 4)) k_grid_yp(const
     const int nchunks = (nrows + P_LIST - 1) / P_LIST;
     int n = 0, total = 0, q0 = 0, p0 = 0, p1 = 0;
     for (int j = jfirst; j < jend; j++) {
-        for (int t = tid; t < 3 * PPAD; t += 256) acc[t] = 0ull;
-        if (tid == 0) s_tn = 0;
+        for (int t = tid; t < 3 * PPAD / 2; t += 256)
+            reinterpret_cast<ulonglong2*>(acc)[t] = ulonglong2{0ull, 0ull};
+        if (tid == 0) {
+            if (3 * PPAD % 2) acc[3 * PPAD - 1] = 0ull;
+            s_tn = 0;
+        }
         __syncthreads();  // LDS zeroed (and the previous plane's epilogue done)
         const int b = j - g.nyd2;
         const double bd = (double)b;
@@ -1422,8 +1425,8 @@ __global__ void __launch_bounds__(256, ROT ? 1 : (GATE ? 5 : 4)) k_grid_yp(const
 #pragma unroll
                     for (int h = 0; h < 2; h++) {
                         if (npt[h] > 0) {
-                            s_row[off] = (unsigned short)(base + 2 * tid + h);
-                            s_a0[off] = (short)a0s[h];
+                            s_line[off] = (int)(((unsigned)(base + 2 * tid + h) << 16) |
+                                                ((unsigned)a0s[h] & 0xffffu));
                             s_pre[off] = pre;
                             off++;
                             pre += npt[h];
@@ -1461,8 +1464,9 @@ __global__ void __launch_bounds__(256, ROT ? 1 : (GATE ? 5 : 4)) k_grid_yp(const
                             pre = nxt;
                             nxt = s_pre[q + 1];
                         }
-                        const int r = s_row[q];
-                        const int a = s_a0[q] + (p - pre);
+                        const int e = s_line[q];
+                        const int r = (int)((unsigned)e >> 16);
+                        const int a = (int)(short)(e & 0xffff) + (p - pre);
                         const double* c6 = rowc + 6 * r;
                         double ph[3] = {0.0, 0.0, 0.0};
                         if (ROT) {
