@@ -3,7 +3,7 @@ oracle on the same seeded inputs.
 
 Tolerances: the FP32 form must reach relative L2 <= 1e-5 against the FP64
 oracle (the bar the reference's metric names); the plain FP64 form must reach
-1e-10.  Results of the FP32 form are deterministic (no atomics), so two runs
+1e-10.  The FP32 form's sums are exact fixed point (order-free), so two runs
 must agree bit for bit.
 """
 import numpy as np
@@ -246,10 +246,18 @@ def rect_case():
     return p, rows, imgs
 
 
-@pytest.mark.parametrize("name", ["y24", "noescc", "rect", "shift", "axis", "one", "rows300"])
+@pytest.mark.parametrize("name", ["y24", "noescc", "rect", "shift", "axis", "axis_many", "one",
+                                  "rows400"])
 def test_plane_and_brick_gridding_agree(name):
     """The y-plane tiles and the bricks form the same fixed-point sums; only
-    the float channel of weights below 2^-20 may add in another order."""
+    the corner weights below 2^-20 (added outside the fixed-point sums) may
+    add in another order.  'axis' puts up to 54 points with such corners in
+    a tile plane (the plane gridding's recorded-point fix-up), 'axis_many'
+    up to 341 (more than the 256 its record holds: the second pass).
+    In 'axis_many' whole columns of voxels are reached only by such corners,
+    whose terms the bricks sum in FP32 (12 per voxel and plane, relative
+    error up to 12 x 2^-24) and the planes in FP64, so their V agree to 1e-6
+    there; both forms are held to the oracle."""
     if name == "y24":
         p, rows, imgs = case(n=64, nrows=24, seed=21)
     elif name == "noescc":
@@ -262,6 +270,10 @@ def test_plane_and_brick_gridding_agree(name):
         p, rows, imgs = case(n=48, nrows=5, seed=24, nsr=0.0, escc=0)
         rows = dict(angZ=np.zeros(5), angY=np.array([0.0, 90.0, 180.0, 270.0, 89.999999]),
                     defocus=np.full(5, 300.0))
+    elif name == "axis_many":
+        p, rows, imgs = case(n=48, nrows=12, seed=27, nsr=0.0, escc=0)
+        rows = dict(angZ=np.zeros(12), angY=89.999999 + 1e-7 * np.arange(12),
+                    defocus=np.full(12, 300.0))
     elif name == "one":
         p, rows, imgs = case(n=32, nrows=1, seed=25)
     else:
@@ -269,10 +281,12 @@ def test_plane_and_brick_gridding_agree(name):
         p, rows, imgs = case(n=32, nrows=400, seed=26, natoms=12)
     print("case", name, flush=True)
     (vp, sp), (vb, sb) = grid_forms(p, rows, imgs)
-    assert np.array_equal(sp == 0, sb == 0)
-    assert rel(sp, sb) < 1e-7
-    assert rel(vp, vb) < 1e-7
     print("  plane vs brick: V rel %.1e (%d voxels differ), S rel %.1e"
           % (rel(vp, vb), int((vp != vb).sum()), rel(sp, sb)), flush=True)
+    assert np.array_equal(sp == 0, sb == 0)
+    assert rel(sp, sb) < 1e-7
+    assert rel(vp, vb) < (1e-6 if name == "axis_many" else 1e-7)
     ref = oracle.reconstruct(p, rows, imgs, weights=rows.get("weights"))
+    print("  vs oracle: plane %.2e, brick %.2e" % (rel(vp, ref), rel(vb, ref)), flush=True)
     assert rel(vp, ref) < FP32_TOL
+    assert rel(vb, ref) < FP32_TOL
