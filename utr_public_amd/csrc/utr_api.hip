@@ -289,7 +289,6 @@ struct utr_ctx {
     DevBuf<cplx<float>> spec32, V32, H32, twx32, twy32, twz32;
     DevBuf<float> S32, vol32;
     DevBuf<unsigned> rowmax;  // per-row max |K| as float bits (FP32 forms)
-    DevBuf<int> scales;       // fixed-point scales of the plane gridding
     int grid_planes = 0;      // y planes per workgroup of the plane gridding (0: by size)
     int rows_wpc = 0;         // resident workgroups per CU of the persistent row FFT (0: default)
     int cols_wpc = 0;         // same for the column FFT (0: as many as LDS allows)
@@ -609,7 +608,6 @@ int launch_grid32(utr_ctx* c, float* d_S, hipEvent_t ev_start) {
         if (nwg <= 0 || nwg > 0x7ffffff0L) return set_err(UTR_E_INVALID, "volume too large");
         if (c->tile_order.n != (size_t)ntiles || c->rowc.n != 6 * (size_t)c->plan.nrows)
             return set_err(UTR_E_STATE, "plane-gridding tables missing");
-        utr::k_fx_scales<<<1, 256, 0, c->stream>>>(c->rowmax.p, c->plan.nrows, c->scales.p);
         // the noise gate alone in the epilogue (the metric's mode 0 with NSR)
         const bool gate = g.nsr2 != 0 && g.ctf_mode != 1 && !g.dht_sctf && !g.do_tie3d &&
                           !g.do_invlap && !g.do_ctf3d;
@@ -626,7 +624,7 @@ int launch_grid32(utr_ctx* c, float* d_S, hipEvent_t ev_start) {
         if (ev_start) UTR_HIP_TRY(hipEventRecord(ev_start, c->stream));
 #define UTR_GRID_YP(ROT, GATE)                                                                   \
     utr::k_grid_yp<ROT, GATE><<<nb, 256, 0, c->stream>>>(c->spec32.p, c->rows.p, c->rowc.p,      \
-                                                         c->plan.nrows, c->scales.p, g, c->V32.p, \
+                                                         c->plan.nrows, c->rowmax.p, g, c->V32.p, \
                                                          d_S, (int)ntiles, (int)ngroups, pg,      \
                                                          c->tile_order.p, gblk)
         if (g.rot_centre_shift) UTR_GRID_YP(true, false);
@@ -765,8 +763,7 @@ int utr_create(const utr_params* params, int nrows, const double* angZ_deg, cons
     if (precision != UTR_FP64) {
         if ((rc = upload_twiddles<float>(c, c->twx32, c->twy32, c->twz32))) return fail(rc);
         if (c->spec32.alloc((size_t)nrows * n2) != hipSuccess || c->V32.alloc(N) != hipSuccess ||
-            c->H32.alloc(nh) != hipSuccess || c->rowmax.alloc(nrows) != hipSuccess ||
-            c->scales.alloc(2) != hipSuccess)
+            c->H32.alloc(nh) != hipSuccess || c->rowmax.alloc(nrows) != hipSuccess)
             return fail(set_err(UTR_E_NOMEM, "device allocation of the FP32 buffers failed"));
         if (c->plan.all_y_exact) {
             const int rc = upload_plane_tables(c);
@@ -1054,7 +1051,7 @@ int utr_destroy(utr_ctx* c) {
     c->img_prep.release();
     c->prep_aver.release();
     c->spec32.release(); c->V32.release(); c->H32.release(); c->S32.release(); c->vol32.release();
-    c->twx32.release(); c->twy32.release(); c->twz32.release(); c->rowmax.release(); c->scales.release();
+    c->twx32.release(); c->twy32.release(); c->twz32.release(); c->rowmax.release();
     c->tile_order.release(); c->rowc.release();
     c->spec64.release(); c->V64.release(); c->H64.release(); c->S64.release(); c->vol64.release();
     c->twx64.release(); c->twy64.release(); c->twz64.release();

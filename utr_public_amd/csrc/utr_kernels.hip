@@ -1148,18 +1148,6 @@ __device__ __forceinline__ void fx_scales(const unsigned* __restrict__ rowmax, i
     sw = min(61 - ea, 50);
 }
 
-// The fixed-point scales of a job, once: scales[0] = sv, scales[1] = sw.
-__global__ void __launch_bounds__(256) k_fx_scales(const unsigned* __restrict__ rowmax, int nrows,
-                                                   int* __restrict__ scales) {
-    __shared__ float s_red[4];
-    int sv, sw;
-    fx_scales(rowmax, nrows, s_red, sv, sw);
-    if (threadIdx.x == 0) {
-        scales[0] = sv;
-        scales[1] = sw;
-    }
-}
-
 // Clip one row's detector line b in [fb_lo, fb_hi] to the x range
 // [bmin0, bmax0] and z range [bmin1, bmax1] of a tile: the a interval
 // [a0, a1] outside which no point of any of these lines can put a corner
@@ -1245,7 +1233,7 @@ template <bool ROT, bool GATE>
 __global__ void __launch_bounds__(256, ROT ? 1 : (GATE ? 5 : 4)) k_grid_yp(const cplx<float>* __restrict__ spec,
                                                  const RowParams* __restrict__ rows,
                                                  const double* __restrict__ rowc, int nrows,
-                                                 const int* __restrict__ scales, GeomDev g,
+                                                 const unsigned* __restrict__ rowmax, GeomDev g,
                                                  cplx<float>* __restrict__ Vout,
                                                  float* __restrict__ Sout, int ntiles, int ngroups,
                                                  int pg, const int* __restrict__ tile_order, int gblk) {
@@ -1255,6 +1243,7 @@ __global__ void __launch_bounds__(256, ROT ? 1 : (GATE ? 5 : 4)) k_grid_yp(const
     __shared__ int s_wsum[4];
     __shared__ int s_tq[P_TQ];                    // points with a corner weight below 2^-20 (row << 16 | a)
     __shared__ int s_tn;                          // their count (may exceed P_TQ)
+    __shared__ float s_red[4];
     double* const accd = reinterpret_cast<double*>(acc);
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -1274,7 +1263,8 @@ __global__ void __launch_bounds__(256, ROT ? 1 : (GATE ? 5 : 4)) k_grid_yp(const
     // planes that receive weight: j <= ny - 2 (the last plane's cells would
     // need plane ny)
     const int jlast_w = min(jend, g.ny - 1) - 1;
-    const int sv = scales[0], sw = scales[1];
+    int sv, sw;  // fixed-point scales of the job (the same in every workgroup)
+    fx_scales(rowmax, nrows, s_red, sv, sw);
     const double vscale = ldexp(1.0, sv), wscale = ldexp(1.0, sw);
     const double vinv = ldexp(1.0, -sv), winv = ldexp(1.0, -sw);
     const double xmaxd = (double)(g.nx - 2), zmaxd = (double)(g.nz - 2);
