@@ -152,7 +152,7 @@ __global__ void __launch_bounds__(256) k_dht_prop_rows(const cplx<float>* __rest
                                                        const DhtRow* __restrict__ rows, int a0,
                                                        const cplx<float>* __restrict__ tw,
                                                        const cplx<float>* __restrict__ ap) {
-    constexpr int NB = lds_batch(N, sizeof(cplx<float>), LDS_DHT);
+    constexpr int NB = lds_batch(N, sizeof(cplx<float>), LDS_DHT_ROWS);
     constexpr int PER = (NB * N + 255) / 256;  // elements per thread
     __shared__ cplx<float> sm[2 * lds_len(NB * N)];
     __shared__ cplx<float> s_ex[2][N], s_ey[2][NB];
@@ -956,7 +956,7 @@ int dht_run(dht_ctx* c, float* d_vol) {
             ok = false;
 #define DHT_PR(N)                                                                                \
     {                                                                                            \
-        constexpr int NB = utr::lds_batch(N, sizeof(cplx<float>), utr::LDS_DHT);                 \
+        constexpr int NB = utr::lds_batch(N, sizeof(cplx<float>), utr::LDS_DHT_ROWS);                 \
         utr::k_dht_prop_rows<N><<<dim3((g.ny + NB - 1) / NB, utr::dht_slots(g), 1), 256, 0,       \
                                   c->stream>>>(                                                  \
             c->spec.p, c->T.p, g, c->rows.p, a0 + ab, c->twx.p, c->ap.p);                        \

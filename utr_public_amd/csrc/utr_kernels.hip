@@ -40,7 +40,7 @@ __device__ __forceinline__ double dev_log<double>(float x) { return log((double)
 // the persistent column pass prefers 64-byte column segments, as does the
 // Hermitian pack, which reads mirrored column pairs)
 constexpr int LDS_ROWS = 8192, LDS_COLS = 16384, LDS_PACKZ = 16384, LDS_FFTY = 8192, LDS_C2R = 8192,
-              LDS_DHT = 16384;
+              LDS_DHT = 16384, LDS_DHT_ROWS = 8192;
 
 // sequences per LDS buffer of at most `budget` bytes, between 1 and 16
 __host__ __device__ constexpr int lds_batch(int n, int elem_bytes, int budget) {
