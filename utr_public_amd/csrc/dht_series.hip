@@ -76,7 +76,7 @@ __global__ void __launch_bounds__(256) k_ser_rows(cplx<float>* __restrict__ Bp, 
                                                   float* __restrict__ A1, int p0, DhtGeom g,
                                                   const SerImg* __restrict__ im, const int* __restrict__ done,
                                                   const cplx<float>* __restrict__ tw) {
-    constexpr int NB = lds_batch(N, sizeof(cplx<float>), LDS_DHT);
+    constexpr int NB = lds_batch(N, sizeof(cplx<float>), LDS_DHT_ROWS);
     __shared__ cplx<float> sm[2 * lds_len(NB * N)];
     cplx<float>* a = sm;
     cplx<float>* b = sm + lds_len(NB * N);
@@ -313,7 +313,7 @@ __global__ void __launch_bounds__(256) k_ctf_rows(const cplx<float>* __restrict_
                                                   const float* __restrict__ simg, cplx<float>* __restrict__ spec,
                                                   DhtGeom g, const SerAng* __restrict__ an, int a0,
                                                   const cplx<float>* __restrict__ tw) {
-    constexpr int NB = lds_batch(N, sizeof(cplx<float>), LDS_DHT);
+    constexpr int NB = lds_batch(N, sizeof(cplx<float>), LDS_DHT_ROWS);
     __shared__ cplx<float> sm[2 * lds_len(NB * N)];
     cplx<float>* a = sm;
     cplx<float>* b = sm + lds_len(NB * N);
@@ -345,7 +345,7 @@ __global__ void __launch_bounds__(256) k_ctf_rows(const cplx<float>* __restrict_
 template <int N>
 __global__ void __launch_bounds__(256) k_ser_wave(const cplx<float>* __restrict__ T, cplx<float>* __restrict__ out,
                                                   int ny, const cplx<float>* __restrict__ tw) {
-    constexpr int NB = lds_batch(N, sizeof(cplx<float>), LDS_DHT);
+    constexpr int NB = lds_batch(N, sizeof(cplx<float>), LDS_DHT_ROWS);
     __shared__ cplx<float> sm[2 * lds_len(NB * N)];
     cplx<float>* a = sm;
     cplx<float>* b = sm + lds_len(NB * N);
@@ -374,7 +374,7 @@ size_t dht_series_bytes() {
 }
 
 int dht_cols_nb(int ny) { return utr::lds_batch(ny, (int)sizeof(cplx<float>), utr::LDS_DHT); }
-int dht_rows_nb(int nx) { return utr::lds_batch(nx, (int)sizeof(cplx<float>), utr::LDS_DHT); }
+int dht_rows_nb(int nx) { return utr::lds_batch(nx, (int)sizeof(cplx<float>), utr::LDS_DHT_ROWS); }
 
 // Images [first, first+count) of a series context, on the device and
 // already through the input image correction, into their slots.
