@@ -1426,10 +1426,13 @@ __global__ void __launch_bounds__(256, ROT ? 1 : (GATE ? 5 : 4)) k_grid_yp(const
                     total = tot & 0xfffff;
                     if (tid == 0) s_pre[n] = total;
                     __syncthreads();
-                    // each wave takes an equal run of the listed points, in list
-                    // order; q0: the listed line holding its first point
-                    p0 = (int)(((long long)wave * total) >> 2);
-                    p1 = (int)(((long long)(wave + 1) * total) >> 2);
+                    // each wave takes a run of whole 64-point chunks of the listed
+                    // points (the runs differ by at most one chunk, and only the
+                    // last chunk of the list is partial), in list order; q0: the
+                    // listed line holding its first point
+                    const int nch = (total + 63) >> 6;
+                    p0 = min(total, 64 * ((wave * nch) >> 2));
+                    p1 = min(total, 64 * (((wave + 1) * nch) >> 2));
                     if (p0 < p1) {
                         int lo = 0, hi = n - 1;
                         while (lo < hi) {
