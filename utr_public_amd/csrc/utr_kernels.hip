@@ -1101,14 +1101,14 @@ __global__ void __launch_bounds__(256, 3) k_grid_fx(const cplx<float>* __restric
 // each plane.  The lines of neighbouring planes differ only through the
 // Ewald term of b, so one clip of every row's line to the tile, conservative
 // over the group's b range, serves all planes of the group: the clipped
-// lines are listed in row order, each thread takes an equal run of the
-// listed points (crossing from one line to the next, with the next line's
-// coefficients loaded one line ahead), and the planes are then gridded one
-// after another with the same list and the same runs.  Point positions,
-// corner weights and the fixed-point sums are those of k_grid_fx (the same
-// expressions in the same order), so both kernels produce the same V3 and
-// Samp3 sums.  Groups are numbered contiguously per XCD, so a plane's
-// spectrum lines are fetched into one XCD's L2 once.
+// lines are listed in row order, each wave takes a run of the listed points
+// and walks it 64 consecutive points at a time (lane l on the chunk's point
+// l, so the lanes read consecutive values of one or a few lines), and the
+// planes are then gridded one after another with the same list and the same
+// runs.  Point positions, corner weights and the fixed-point sums are those
+// of k_grid_fx (the same expressions in the same order), so both kernels
+// produce the same V3 and Samp3 sums.  Groups are numbered contiguously per
+// XCD, so the planes of a group share one XCD's L2.
 
 constexpr int PX = 32, PZ = 32;
 constexpr int PVOX = PX * PZ;
@@ -1219,11 +1219,12 @@ __host__ __device__ constexpr unsigned yp_grid(int ntiles, int ngroups) {
 // Corner weights below 2^-20 (a point within 2^-20 of a grid plane) keep
 // their relative precision outside the fixed-point sums: the plane's pass
 // adds the fixed-point terms of all other corners and only flags such a
-// corner.  A flagged plane (rare: about one tile plane in a hundred on a
-// general tilt series) then converts its sums to FP64 in place and runs the
-// pass again, adding just those corners' FP32 terms in FP64.  The LDS then
-// holds only the fixed-point sums and the list (28 KB), and the noise-gate
-// instantiation fits 96 VGPRs: 5 workgroups per CU.
+// corner.  A flagged plane then converts its sums to FP64 in place and adds
+// just those corners' FP32 terms in FP64: those of the points it recorded
+// (up to P_TQ), or all of them in a second pass when more were met.  The
+// LDS then holds only the fixed-point sums (with their halo), the list and
+// the record (31 KB), and the noise-gate instantiation fits 96 VGPRs: 5
+// workgroups per CU.
 template <bool B>
 struct PassKind {
     static constexpr bool value = B;
