@@ -247,7 +247,7 @@ def rect_case():
 
 
 @pytest.mark.parametrize("name", ["y24", "noescc", "rect", "shift", "axis", "axis_many", "one",
-                                  "rows400"])
+                                  "rows400", "rows800"])
 def test_plane_and_brick_gridding_agree(name):
     """The y-plane tiles and the bricks form the same fixed-point sums; only
     the corner weights below 2^-20 (added outside the fixed-point sums) may
@@ -276,9 +276,12 @@ def test_plane_and_brick_gridding_agree(name):
                     defocus=np.full(12, 300.0))
     elif name == "one":
         p, rows, imgs = case(n=32, nrows=1, seed=25)
-    else:
-        # more rows than one listing chunk (384)
+    elif name == "rows400":
+        # more rows than the small list holds (384): one list of up to 768
         p, rows, imgs = case(n=32, nrows=400, seed=26, natoms=12)
+    else:
+        # more rows than one list of 768: listing chunks per plane
+        p, rows, imgs = case(n=32, nrows=800, seed=28, natoms=12)
     print("case", name, flush=True)
     (vp, sp), (vb, sb) = grid_forms(p, rows, imgs)
     print("  plane vs brick: V rel %.1e (%d voxels differ), S rel %.1e"

@@ -622,14 +622,24 @@ int launch_grid32(utr_ctx* c, float* d_S, hipEvent_t ev_start) {
         const int ngx = (int)((ngroups + 7) / 8);
         if (gblk <= 0 || gblk > ngx) gblk = ngx;
         if (ev_start) UTR_HIP_TRY(hipEventRecord(ev_start, c->stream));
-#define UTR_GRID_YP(ROT, GATE)                                                                   \
-    utr::k_grid_yp<ROT, GATE><<<nb, 256, 0, c->stream>>>(c->spec32.p, c->rows.p, c->rowc.p,      \
-                                                         c->plan.nrows, c->rowmax.p, g, c->V32.p, \
-                                                         d_S, (int)ntiles, (int)ngroups, pg,      \
-                                                         c->tile_order.p, gblk)
-        if (g.rot_centre_shift) UTR_GRID_YP(true, false);
-        else if (gate) UTR_GRID_YP(false, true);
-        else UTR_GRID_YP(false, false);
+#define UTR_GRID_YP(ROT, GATE, PL)                                                               \
+    utr::k_grid_yp<ROT, GATE, PL><<<nb, 256, 0, c->stream>>>(c->spec32.p, c->rows.p, c->rowc.p,  \
+                                                             c->plan.nrows, c->rowmax.p, g,       \
+                                                             c->V32.p, d_S, (int)ntiles,          \
+                                                             (int)ngroups, pg, c->tile_order.p,   \
+                                                             gblk)
+        // list capacity: one list per plane group up to 768 rows
+        const bool small = c->plan.nrows <= utr::P_LIST_S;
+        if (g.rot_centre_shift) {
+            if (small) UTR_GRID_YP(true, false, utr::P_LIST_S);
+            else UTR_GRID_YP(true, false, utr::P_LIST_L);
+        } else if (gate) {
+            if (small) UTR_GRID_YP(false, true, utr::P_LIST_S);
+            else UTR_GRID_YP(false, true, utr::P_LIST_L);
+        } else {
+            if (small) UTR_GRID_YP(false, false, utr::P_LIST_S);
+            else UTR_GRID_YP(false, false, utr::P_LIST_L);
+        }
 #undef UTR_GRID_YP
     } else {
         const long nblocks = (long)((g.nx + utr::SBX - 1) / utr::SBX) *

@@ -1124,8 +1124,10 @@ static_assert(PVOX % 256 == 0, "tile must be a multiple of the workgroup");
 constexpr int PSX = PX + 1;          // LDS stride along z
 constexpr int P_OFF = PSX + 1;       // slot of cell (0, 0)
 constexpr int PPAD = PX + PSX * PZ + P_OFF + 1;  // LDS entries per component
-constexpr int P_LIST = 384;          // rows clipped per chunk (two per thread)
-static_assert(P_LIST <= 512, "two rows per thread");
+// rows clipped per chunk: the list capacity PL of k_grid_yp (384 for up to
+// 384 rows, two per thread; 768 beyond, three per thread), so jobs of up to
+// 768 rows list each group's lines once
+constexpr int P_LIST_S = 384, P_LIST_L = 768;
 constexpr int P_TQ = 256;            // points with tiny corner weights recorded per plane (one per thread)
 
 // Fixed-point scales 2^sv (V3) and 2^sw (Samp3) from the largest spectrum
@@ -1230,8 +1232,8 @@ struct PassKind {
     static constexpr bool value = B;
 };
 
-template <bool ROT, bool GATE>
-__global__ void __launch_bounds__(256, ROT ? 1 : (GATE ? 5 : 4)) k_grid_yp(const cplx<float>* __restrict__ spec,
+template <bool ROT, bool GATE, int PL>
+__global__ void __launch_bounds__(256, ROT ? 1 : (GATE && PL <= P_LIST_S ? 5 : 4)) k_grid_yp(const cplx<float>* __restrict__ spec,
                                                  const RowParams* __restrict__ rows,
                                                  const double* __restrict__ rowc, int nrows,
                                                  const unsigned* __restrict__ rowmax, GeomDev g,
@@ -1239,8 +1241,9 @@ __global__ void __launch_bounds__(256, ROT ? 1 : (GATE ? 5 : 4)) k_grid_yp(const
                                                  float* __restrict__ Sout, int ntiles, int ngroups,
                                                  int pg, const int* __restrict__ tile_order, int gblk) {
     __shared__ __align__(16) unsigned long long acc[3 * PPAD];  // fixed-point sums (FP64 after a tiny-weight pass)
-    __shared__ int s_line[P_LIST];                // listed lines: row << 16 | first a of the clipped line
-    __shared__ int s_pre[P_LIST + 1];             // exclusive prefix of the point counts
+    constexpr int RPT = (PL + 255) / 256;         // rows clipped per thread
+    __shared__ int s_line[PL];                    // listed lines: row << 16 | first a of the clipped line
+    __shared__ int s_pre[PL + 1];                 // exclusive prefix of the point counts
     __shared__ int s_wsum[4];
     __shared__ int s_tq[P_TQ];                    // points with a corner weight below 2^-20 (row << 16 | a)
     __shared__ int s_tn;                          // their count (may exceed P_TQ)
@@ -1282,7 +1285,7 @@ __global__ void __launch_bounds__(256, ROT ? 1 : (GATE ? 5 : 4)) k_grid_yp(const
     const float fb2m = 0.5f * (b2lo + b2hi), fb2h = 0.5f * (b2hi - b2lo);
     const unsigned rstride = (unsigned)(g.ny * g.nx) & 0xffffffu;  // ny nx <= 2^18
 
-    const int nchunks = (nrows + P_LIST - 1) / P_LIST;
+    const int nchunks = (nrows + PL - 1) / PL;
     int n = 0, total = 0, q0 = 0, p0 = 0, p1 = 0;
     for (int j = jfirst; j < jend; j++) {
         for (int t = tid; t < 3 * PPAD / 2; t += 256)
@@ -1386,21 +1389,27 @@ __global__ void __launch_bounds__(256, ROT ? 1 : (GATE ? 5 : 4)) k_grid_yp(const
         // One pass of splat over all the plane's listed points.
         auto pass = [&](auto kind) {
             constexpr bool FIX = decltype(kind)::value;
-            for (int base = 0; base < nrows && j <= jlast_w; base += P_LIST) {
+            for (int base = 0; base < nrows && j <= jlast_w; base += PL) {
                 if (nchunks > 1 || (j == jfirst && !FIX)) {
-                    // list the chunk's rows whose lines reach the tile: two rows
-                    // per thread (2 tid, 2 tid + 1), one block scan of the
-                    // packed (line count << 20 | point count)
-                    const int nchunk = min(P_LIST, nrows - base);
-                    int npt[2] = {0, 0}, a0s[2] = {0, 0};
+                    // list the chunk's rows whose lines reach the tile: RPT rows
+                    // per thread (RPT tid + h), one block scan of the packed
+                    // (line count << 20 | point count)
+                    const int nchunk = min(PL, nrows - base);
+                    int npt[RPT] = {}, a0s[RPT] = {};
 #pragma unroll
-                    for (int h = 0; h < 2; h++) {
-                        const int ri = 2 * tid + h;
+                    for (int h = 0; h < RPT; h++) {
+                        const int ri = RPT * tid + h;
                         if (ri < nchunk)
                             npt[h] = clip_line(g, rows[base + ri], fbm, fbh, fb2m, fb2h, bmin, bmax, fx2,
                                                fy2, wl2f, a0s[h]);
                     }
-                    const int v = ((int)(npt[0] > 0) + (int)(npt[1] > 0)) << 20 | (npt[0] + npt[1]);
+                    int nl = 0, np = 0;
+#pragma unroll
+                    for (int h = 0; h < RPT; h++) {
+                        nl += (int)(npt[h] > 0);
+                        np += npt[h];
+                    }
+                    const int v = nl << 20 | np;
                     int incl = v;
                     for (int o = 1; o < 64; o <<= 1) {
                         const int u = __shfl_up(incl, o);
@@ -1414,9 +1423,9 @@ __global__ void __launch_bounds__(256, ROT ? 1 : (GATE ? 5 : 4)) k_grid_yp(const
                     const int ex = woff + incl - v;
                     int off = ex >> 20, pre = ex & 0xfffff;
 #pragma unroll
-                    for (int h = 0; h < 2; h++) {
+                    for (int h = 0; h < RPT; h++) {
                         if (npt[h] > 0) {
-                            s_line[off] = (int)(((unsigned)(base + 2 * tid + h) << 16) |
+                            s_line[off] = (int)(((unsigned)(base + RPT * tid + h) << 16) |
                                                 ((unsigned)a0s[h] & 0xffffu));
                             s_pre[off] = pre;
                             off++;
