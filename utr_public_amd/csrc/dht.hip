@@ -140,8 +140,9 @@ __global__ void __launch_bounds__(256) k_dht_cols(cplx<float>* __restrict__ data
 // The phase chi = f2 q^2 + f4 q^4 + f6 q^6 (DHT.cpp:962-1000) is split as
 // exp(-i chi) = ex(fx) ey(fy) ap(fx, fy): ex = exp(-i f2 fx^2) and
 // ey = exp(-i f2 fy^2) depend on the plane's defocus and are formed per
-// workgroup with FP64 phases; ap = exp(-i (f4 q^4 + f6 q^6)), zero outside
-// the objective aperture, is one table per job (read when g.use_ap).
+// workgroup with FP64 phases reduced to [-pi, pi] (FP32 sin and cos of the
+// reduced phase); ap = exp(-i (f4 q^4 + f6 q^6)), zero outside the
+// objective aperture, is one table per job (read when g.use_ap).
 __host__ __device__ inline int dht_slots(const DhtGeom& g) {
     return g.method == 4 ? (g.nplanes + 1) / 2 : g.nplanes;
 }
@@ -186,11 +187,13 @@ __global__ void __launch_bounds__(256) k_dht_prop_rows(const cplx<float>* __rest
         const double d = g.method != 4 ? zn - R.zout : R.zout - zn;  // 1-3: back-propagation
         const double f2 = pi * g.wl * d;
         const double f = kk < N ? dht_freq(kk, N, g.xst) : dht_freq(k0 + kk - N, ny, g.xst);
+        // the phase and its reduction to [-pi, pi] in FP64; sin and cos of
+        // the reduced phase in FP32
         double chi = f2 * (f * f);
         chi -= twopi * rint(chi * (1.0 / twopi));
-        double sn, cs;
-        sincos(chi, &sn, &cs);
-        const cplx<float> e = {(float)cs, (float)-sn};
+        float sn, cs;
+        sincosf((float)chi, &sn, &cs);
+        const cplx<float> e = {cs, -sn};
         if (kk < N) s_ex[m][kk] = e;
         else s_ey[m][kk - N] = e;
     }
