@@ -514,7 +514,12 @@ namespace utr {
 // The fixed-point terms are those of k_dht_splat expression by expression,
 // so both kernels form the same sums.
 constexpr int DC_X = 4, DC_Z = 4;
-constexpr int DC_R = 6;                       // candidate half-width (source samples)
+// candidate half-width (source samples): the cells a column's corners reach
+// form a 5 x 5 square about its centre, whose corners lie 2.5 sqrt(2) = 3.54
+// samples away under any rotation, so a box of floor(centre) +- 4 holds
+// every sample that can reach the column; 5 leaves a sample of margin for
+// the FP32 centre
+constexpr int DC_R = 5;
 constexpr int DC_NC = (2 * DC_R + 1) * (2 * DC_R + 1);
 constexpr int DC_NYMAX = 384;                 // tallest image (column sums of 48 KB in LDS)
 constexpr int DC_T = 512;                     // threads per workgroup
