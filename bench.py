@@ -115,7 +115,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the reconstruction would shard over orientation rows; see DESIGN.md)",
+              "(the reconstruction would shard over orientation rows)",
               file=sys.stderr)
         sys.exit(2)
 
