@@ -114,7 +114,7 @@ def main():
                     help="first run a plain 256 MiB device copy (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
+        print("bench.py measures one GPU only: the library has no multi-GPU path "
               "(the reconstruction would shard over orientation rows)",
               file=sys.stderr)
         sys.exit(2)
